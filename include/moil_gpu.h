@@ -1,0 +1,190 @@
+/*
+ * moil_gpu.h -- C ABI of the MI355X energy/force library for MOIL.
+ *
+ * The library replaces MOIL's energy-path subroutines, which in MOIL are
+ * argument-less Fortran routines communicating through COMMON blocks:
+ *
+ *   ewald_init   symm/ewald_init.f:8-242     -> moil_gpu_init
+ *   squeeze      symm/squeeze.f:1-147        -> moil_gpu_squeeze
+ *   nbondm+nbond pot/nbondm.f:1-247,
+ *                pot/nbond.f:1-262           -> moil_gpu_build_lists
+ *   syminit+nbmsym+nbsym symm/syminit.f:1-178,
+ *                symm/nbmsym.f:1-351,
+ *                symm/nbsym.f:1-192          -> moil_gpu_build_lists
+ *   eforce       pot/eforce.f:1-325          -> moil_gpu_eforce
+ *
+ * Every pointer argument is a plain host (or, for the *_dev variants,
+ * device) pointer; arrays follow MOIL's COMMON layouts and 1-based index
+ * conventions so a Fortran ISO_C_BINDING shim can pass COMMON arrays
+ * straight through (coor/dpot are MOIL's coor(3,npt)/dpot(3,npt)).
+ *
+ * Error convention: every entry point returns 0 on success and a
+ * MOIL_GPU_E* code otherwise; moil_gpu_last_error() returns the message.
+ * MOIL itself prints and STOPs through alert() (intrprtr/alert.f:10-24);
+ * the Fortran shim maps a non-zero status to alert(...,1).
+ *
+ * Threading: one host thread per context; calls on one context are not
+ * re-entrant (MOIL's global state has the same property).
+ */
+#ifndef MOIL_GPU_H
+#define MOIL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOIL_GPU_ABI_VERSION 1
+
+/* status codes */
+#define MOIL_GPU_OK        0
+#define MOIL_GPU_EINVAL    1  /* bad argument / unsupported input      */
+#define MOIL_GPU_ECHARGE   2  /* |sum q| > 1e-7 (symm/ewald_init.f:66) */
+#define MOIL_GPU_ECAPACITY 3  /* a list or grid exceeds its capacity   */
+#define MOIL_GPU_EHIP      4  /* HIP / rocFFT runtime failure          */
+#define MOIL_GPU_ESTATE    5  /* call order (e.g. eforce before lists) */
+
+/* monomer flags, derived by the caller from MOIL's monomer names */
+#define MOIL_MONO_WATER     1 /* 'TIP3'/'SPC*' water group (pot/nbondm.f:81-82) */
+#define MOIL_MONO_SQZ_FIRST 2 /* squeezed by first atom (symm/squeeze.f:75-79)  */
+#define MOIL_MONO_SQZ_CENT  4 /* squeezed by centroid (symm/squeeze.f:70-72)    */
+
+/* precision modes */
+#define MOIL_PREC_FP64  0  /* FP64 pair math: MOIL parity mode              */
+#define MOIL_PREC_MIXED 1  /* FP32 pair math, 64-bit fixed-point forces,
+                              FP64 energies / Ewald self+reciprocal sums    */
+
+/* energy-term switches (MOIL /LENER/ flags, pot/init_ef.f:24-47) */
+#define MOIL_TERM_BOND   (1u << 0)   /* ebyes  */
+#define MOIL_TERM_ANGLE  (1u << 1)   /* ethyes */
+#define MOIL_TERM_TORS   (1u << 2)   /* etoyes */
+#define MOIL_TERM_IMP    (1u << 3)   /* eimyes */
+#define MOIL_TERM_VDW    (1u << 4)   /* evdyes */
+#define MOIL_TERM_EL     (1u << 5)   /* eelyes */
+#define MOIL_TERM_EL14   (1u << 6)   /* e14el_yes */
+#define MOIL_TERM_VDW14  (1u << 7)   /* e14v_yes  */
+#define MOIL_TERM_HVDW0  (1u << 8)   /* hvdw0 hydrogen rule (pot/nbond.f:115) */
+#define MOIL_TERM_DEFAULT 0x1FFu
+
+/* Topology: the energy-path part of MOIL's /CONNINT/ /CONNDBL/ /CONNLOG/
+ * (COMMON/CONNECT.BLOCK).  All indices are 1-based as in MOIL.          */
+typedef struct moil_topology {
+  int32_t npt;
+  int32_t totmon;            /* monomers (monomer division unsupported)   */
+  const int32_t *poipt;      /* [totmon+1], poipt[0]=0, last atom of mono */
+  const int32_t *monoflag;   /* [totmon] MOIL_MONO_* bits                 */
+  const double *ptchg;       /* [npt] charges (e)                         */
+  const double *epsgm6;      /* [npt] 2*sqrt(eps)*sigma^3                 */
+  const double *epsgm12;     /* [npt] 2*sqrt(eps)*sigma^6                 */
+  const int32_t *flagchr;    /* [npt] nonzero = charged (Fortran LOGICAL) */
+  const int32_t *exc1;       /* [npt+1] exclusion CSR ends, exc1[0]=0     */
+  const int32_t *exc2;       /* [exc1[npt]] excluded partners (> atom)   */
+  int32_t nb;                /* bonds  E = k (r-r0)^2                     */
+  const int32_t *ib1, *ib2;
+  const double *kbond, *req;
+  int32_t nangl;             /* angles E = k (theta-theta0)^2, radians    */
+  const int32_t *iangl1, *iangl2, *iangl3;
+  const double *kangl, *angleq;
+  int32_t ntors;             /* proper torsions (cosine polynomial)       */
+  const int32_t *itor1, *itor2, *itor3, *itor4, *period;
+  const double *ktors1, *ktors2, *ktors3, *phase1, *phase2, *phase3;
+  int32_t nimp;              /* improper torsions                          */
+  const int32_t *iimp1, *iimp2, *iimp3, *iimp4;
+  const double *kimp, *impeq;
+  int32_t totspe;            /* scaled 1-4 pairs                           */
+  const int32_t *spec1, *spec2;
+  const double *p14;         /* [3*totspe]: A*v14, B*v14, qq*eps*el14*kofdie */
+  int32_t nwaters;
+  const int32_t *idxtip3;    /* [nwaters] monomer numbers of the waters    */
+} moil_topology;
+
+/* Run parameters (MOIL /NB_D/, /SYMMD/, /EWALD2/, /EWALD3/, eps). */
+typedef struct moil_params {
+  double a, b, c;            /* orthorhombic box edges, box centred at 0  */
+  double cutvdw2;            /* rvmx^2  */
+  double cutele2;            /* relx^2  */
+  double cutvbig2;           /* rvbg^2  */
+  double cutebig2;           /* rebg^2  */
+  double cutmono2;           /* cutm^2  */
+  double eps;                /* dielectric constant                       */
+  double dtol;               /* Ewald direct-sum tolerance -> beta        */
+  int32_t nfft1, nfft2, nfft3;
+  int32_t order;             /* B-spline order (4 supported)              */
+  int32_t precision;         /* MOIL_PREC_*                               */
+  uint32_t terms;            /* MOIL_TERM_* bits                          */
+} moil_params;
+
+/* Energies, same names as MOIL's /ENERGY/ and /EWALD3/ scalars. */
+typedef struct moil_energies {
+  double e_total, e_bond, e_theta, e_tors, e_imp;
+  double e_vdw, e_el, e_vsym, e_lsym, e_el14, e_vdw14;
+  double e_corr, e_self, e_ew_receip, e_dir;
+} moil_energies;
+#define MOIL_N_ENERGIES 15
+
+typedef struct moil_ctx moil_ctx;
+
+/* Library / device info. */
+int32_t moil_gpu_abi_version(void);
+const char *moil_gpu_last_error(void);
+
+/* ewald_init (symm/ewald_init.f:8-242): copies the topology to the GPU,
+ * derives beta, the erfc spline table, e_self and the B-spline moduli,
+ * and plans the FFTs.  Returns MOIL_GPU_ECHARGE for a non-neutral system. */
+int32_t moil_gpu_init(const moil_topology *topo, const moil_params *par,
+                      moil_ctx **out);
+int32_t moil_gpu_finalize(moil_ctx *ctx);
+
+/* beta (ewaldcof), e_self, table rows (mxerftab) and density (erftbdns). */
+int32_t moil_gpu_ewald_info(moil_ctx *ctx, double *ewaldcof, double *e_self,
+                            int32_t *ntab, double *erftbdns);
+/* erfc spline table as MOIL's erf_arr(4,ntab) (symm/ewald_init.f:418-449) */
+int32_t moil_gpu_erf_table(moil_ctx *ctx, double *erf_arr, int32_t cap_rows);
+
+/* squeeze (symm/squeeze.f:1-147): wraps molecules back into the box;
+ * coor(3,npt) is updated in place. */
+int32_t moil_gpu_squeeze(moil_ctx *ctx, double *coor);
+
+/* nbondm + nbond + syminit + nbmsym + nbsym: builds the buffered pair
+ * lists (real and 13 half-shell image translations) from coor(3,npt). */
+int32_t moil_gpu_build_lists(moil_ctx *ctx, const double *coor);
+
+/* eforce (pot/eforce.f:1-325): energies and the gradient dpot(3,npt). */
+int32_t moil_gpu_eforce(moil_ctx *ctx, const double *coor, double *dpot,
+                        moil_energies *e);
+
+/* Device-resident variants: d_coor/d_dpot are device pointers to
+ * (3,npt) float64 arrays, d_energies a device pointer to
+ * MOIL_N_ENERGIES float64 in moil_energies order; `stream` is a
+ * hipStream_t (NULL = the context's own stream).  Asynchronous. */
+int32_t moil_gpu_squeeze_dev(moil_ctx *ctx, double *d_coor, void *stream);
+int32_t moil_gpu_build_lists_dev(moil_ctx *ctx, const double *d_coor, void *stream);
+int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
+                            double *d_energies, void *stream);
+
+/* List inspection for parity tests.  counts[0..7]: water pairs real
+ * listwt1, real listwt2, image listwt1, image listwt2; atom pairs list1,
+ * list2, list3 (real+image), total image atom pairs. */
+int32_t moil_gpu_list_counts(moil_ctx *ctx, int64_t counts[8]);
+/* Canonical water pairs, rows of 4 int32: i monomer (the translated one
+ * for images), j monomer, symmetry operation (0 real, 1..13 as
+ * symm/syminit.f:80-164), list (1 = listwt1, 2 = listwt2). */
+int32_t moil_gpu_dump_water_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
+                                  int64_t *n);
+/* Canonical atom pairs, rows of 4 int32: l (translated for images), m,
+ * symmetry operation, list (1, 2, 3 as pot/nbond.f:136-149). */
+int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
+                                 int64_t *n);
+
+/* Timing of the last eforce on the context's stream (ms per stage):
+ * t[0] real-space water, t[1] real-space atoms, t[2] PME, t[3] bonded,
+ * t[4] total.  Zero when timing is off (moil_gpu_set_timing). */
+int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t on);
+int32_t moil_gpu_stage_times(moil_ctx *ctx, double t[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOIL_GPU_H */

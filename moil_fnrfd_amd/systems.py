@@ -1,0 +1,538 @@
+"""Synthetic MOIL systems and the MOIL `.con` connectivity format.
+
+A :class:`Topology` holds exactly what MOIL keeps in its CONNECT COMMON
+block for the energy path (1-based atom/monomer indices, MOIL's array
+names), so the same arrays can be handed to the GPU library, to the CPU
+oracle and -- written as a `.con` file -- to MOIL's own reader
+(generic/rconn.f:46-497).
+
+MOIL's `.con` text format quantises every parameter (charges f9.5,
+epsgm e12.5, bonds f10.4, angles f10.5 degrees, torsions f10.4/f6.2,
+impropers e15.8, 1-4 terms f30.5/f15.5; generic/rconn.f:107-120).
+Every generator below therefore rounds each parameter through the text
+field it is written with (`_q`) before it is used anywhere, so that MOIL
+and this package always work on identical float64 numbers.
+
+Generators follow the synthetic-system recipes of the energy-path
+benchmark: rigid 3-site water boxes (lattice + jitter, random
+orientations) and random-walk heteropolymers in water.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+KOFDIE = 332.0716                       # COMMON/CONVERT.BLOCK:13
+PI180 = 180.0 / (4.0 * math.atan(1.0))  # pot/init_var.f:8
+
+# TIP3-style rigid water (O-H 0.9572 A, H-O-H 104.52 deg)
+WAT_ROH = 0.9572
+WAT_THETA = 104.52
+WAT_QO = -0.834
+WAT_QH = 0.417
+WAT_EPS_O = 0.1521
+WAT_RMIN2_O = 1.7682
+
+# monomer flags handed to the library (derived from MOIL monomer names)
+MONO_WATER = 1      # 'TIP3' or 'SPC*': water-group lists (pot/nbondm.f:81-82)
+MONO_SQZ_FIRST = 2  # squeezed by its first atom (symm/squeeze.f:75-79)
+MONO_SQZ_CENT = 4   # squeezed by its centroid (symm/squeeze.f:70-72)
+
+_SQZ_FIRST_NAMES = {'TIP3', 'NAAT', 'NAO', 'CL', 'MG', 'CMO', 'CO', 'SUL',
+                    'K', 'AR'}
+_SQZ_CENT_NAMES = {'DMPC', 'DOPC', 'NATA'}
+
+
+def _q(fmt: str, x: float) -> float:
+    """Round `x` through a Fortran-style text field (see module doc)."""
+    return float(format(float(x), fmt))
+
+
+def lj_epsgm(eps: float, rmin_half: float):
+    """MOIL's per-atom LJ factors (connect/fillpt.f:150-179, geometric):
+    epsgm12 = 2*sqrt(eps)*sigma^6, epsgm6 = 2*sqrt(eps)*sigma^3 so that
+    A_ij = epsgm12_i*epsgm12_j and B_ij = epsgm6_i*epsgm6_j."""
+    if eps <= 0.0:
+        return 0.0, 0.0
+    sigma = 2.0 * rmin_half / 2.0 ** (1.0 / 6.0)
+    e12 = 2.0 * math.sqrt(eps) * sigma ** 6
+    e6 = 2.0 * math.sqrt(eps) * sigma ** 3
+    return e12, e6
+
+
+def monomer_flags(monames) -> np.ndarray:
+    out = np.zeros(len(monames), np.int32)
+    for k, nm in enumerate(monames):
+        nm = nm.strip()
+        f = 0
+        if nm == 'TIP3' or nm[:3] == 'SPC':
+            f |= MONO_WATER
+        if nm in _SQZ_FIRST_NAMES or nm[:3] == 'SPC':
+            f |= MONO_SQZ_FIRST
+        if nm in _SQZ_CENT_NAMES:
+            f |= MONO_SQZ_CENT
+        out[k] = f
+    return out
+
+
+@dataclasses.dataclass
+class Topology:
+    """MOIL CONNECT-block content for the energy path (1-based indices)."""
+    npt: int
+    moname: List[str]            # per monomer, 4 chars
+    poipt: np.ndarray            # int32 [totmon+1], poipt[0]=0, last atom
+    ptnm: List[str]
+    ptms: np.ndarray
+    ptchg: np.ndarray
+    epsgm6: np.ndarray
+    epsgm12: np.ndarray
+    exc1: np.ndarray             # int32 [npt+1], exc1[0]=0
+    exc2: np.ndarray             # int32 [totex]
+    ib1: np.ndarray
+    ib2: np.ndarray
+    kbond: np.ndarray
+    req: np.ndarray
+    iangl1: np.ndarray
+    iangl2: np.ndarray
+    iangl3: np.ndarray
+    kangl: np.ndarray
+    angleq: np.ndarray           # radians (as MOIL holds it after rconn)
+    angleq_deg: np.ndarray       # the quantised degrees written to .con
+    itor1: np.ndarray
+    itor2: np.ndarray
+    itor3: np.ndarray
+    itor4: np.ndarray
+    period: np.ndarray
+    ktors1: np.ndarray
+    ktors2: np.ndarray
+    ktors3: np.ndarray
+    phase1: np.ndarray
+    phase2: np.ndarray
+    phase3: np.ndarray
+    iimp1: np.ndarray
+    iimp2: np.ndarray
+    iimp3: np.ndarray
+    iimp4: np.ndarray
+    kimp: np.ndarray
+    impeq: np.ndarray            # radians
+    impeq_deg: np.ndarray
+    spec1: np.ndarray
+    spec2: np.ndarray
+    p14: np.ndarray              # float64 [totspe, 3]
+    idxtip3: np.ndarray          # monomer numbers of waters (1-based)
+
+    @property
+    def totmon(self) -> int:
+        return len(self.moname)
+
+    @property
+    def nwaters(self) -> int:
+        return len(self.idxtip3)
+
+    @property
+    def flagchr(self) -> np.ndarray:
+        # connect/chsort.f:20-24
+        return (np.abs(self.ptchg) > 1e-6).astype(np.uint8)
+
+    @property
+    def monoflag(self) -> np.ndarray:
+        return monomer_flags(self.moname)
+
+    def net_charge(self) -> float:
+        return float(np.sum(self.ptchg))
+
+
+class _TopoBuilder:
+    """Accumulates atoms/monomers/terms; every parameter is quantised."""
+
+    def __init__(self):
+        self.moname, self.poipt = [], [0]
+        self.ptnm, self.ptms, self.ptchg, self.e6, self.e12 = [], [], [], [], []
+        self.bonds, self.angles, self.tors, self.imps, self.spec = [], [], [], [], []
+        self.excl = {}
+        self.idxtip3 = []
+
+    @property
+    def npt(self):
+        return len(self.ptnm)
+
+    def add_monomer(self, name, atoms):
+        """atoms: list of (name, mass, charge, eps, rmin_half)."""
+        for (nm, m, q, eps, rh) in atoms:
+            e12, e6 = lj_epsgm(eps, rh)
+            self.ptnm.append(nm)
+            self.ptms.append(_q('7.3f', m))
+            self.ptchg.append(_q('9.5f', q))
+            self.e6.append(_q('12.5E', e6))
+            self.e12.append(_q('12.5E', e12))
+        self.moname.append(name)
+        self.poipt.append(self.npt)
+        if name == 'TIP3':
+            self.idxtip3.append(len(self.moname))
+
+    def bond(self, i, j, k, r0):
+        self.bonds.append((i, j, _q('10.4f', k), _q('10.4f', r0)))
+
+    def angle(self, i, j, k, kk, th_deg):
+        self.angles.append((i, j, k, _q('10.5f', kk), _q('10.5f', th_deg)))
+
+    def torsion(self, i, j, k, l, period, k1, k2, k3, p1, p2, p3):
+        self.tors.append((i, j, k, l, period, _q('10.4f', k1), _q('10.4f', k2),
+                          _q('10.4f', k3), _q('6.2f', p1), _q('6.2f', p2),
+                          _q('6.2f', p3)))
+
+    def improper(self, i, j, k, l, kk, th_deg):
+        self.imps.append((i, j, k, l, _q('15.8E', kk), _q('15.8E', th_deg)))
+
+    def exclude(self, i, j):
+        lo, hi = (i, j) if i < j else (j, i)
+        self.excl.setdefault(lo, set()).add(hi)
+
+    def special14(self, i, j, v14, el14, eps=1.0, hvdw0=True):
+        """1-4 pair parameters as generic/wconn.f:296-325 (geometric)."""
+        e12i, e12j = self.e12[i - 1], self.e12[j - 1]
+        if (e12i < 1e-3 or e12j < 1e-3) and hvdw0:
+            a = b = 0.0
+        else:
+            a = e12i * e12j * v14
+            b = self.e6[i - 1] * self.e6[j - 1] * v14
+        c = self.ptchg[i - 1] * self.ptchg[j - 1] * eps * el14 * KOFDIE
+        self.spec.append((i, j, _q('30.5f', a), _q('15.5f', b), _q('15.5f', c)))
+
+    def build(self) -> Topology:
+        npt = self.npt
+        exc1 = np.zeros(npt + 1, np.int32)
+        exc2 = []
+        for i in range(1, npt + 1):
+            lst = sorted(self.excl.get(i, ()))
+            exc2.extend(lst)
+            exc1[i] = len(exc2)
+        i32 = lambda v: np.asarray(v, np.int32)
+        f64 = lambda v: np.asarray(v, np.float64)
+        B = self.bonds or []
+        A = self.angles or []
+        T = self.tors or []
+        I = self.imps or []
+        S = self.spec or []
+        col = lambda L, k, n: [t[k] for t in L] if L else []
+        adeg = f64(col(A, 4, 5))
+        ideg = f64(col(I, 5, 6))
+        return Topology(
+            npt=npt, moname=list(self.moname), poipt=i32(self.poipt),
+            ptnm=list(self.ptnm), ptms=f64(self.ptms), ptchg=f64(self.ptchg),
+            epsgm6=f64(self.e6), epsgm12=f64(self.e12),
+            exc1=exc1, exc2=i32(exc2),
+            ib1=i32(col(B, 0, 4)), ib2=i32(col(B, 1, 4)), kbond=f64(col(B, 2, 4)),
+            req=f64(col(B, 3, 4)),
+            iangl1=i32(col(A, 0, 5)), iangl2=i32(col(A, 1, 5)), iangl3=i32(col(A, 2, 5)),
+            kangl=f64(col(A, 3, 5)), angleq=adeg / PI180, angleq_deg=adeg,
+            itor1=i32(col(T, 0, 11)), itor2=i32(col(T, 1, 11)), itor3=i32(col(T, 2, 11)),
+            itor4=i32(col(T, 3, 11)), period=i32(col(T, 4, 11)),
+            ktors1=f64(col(T, 5, 11)), ktors2=f64(col(T, 6, 11)), ktors3=f64(col(T, 7, 11)),
+            phase1=f64(col(T, 8, 11)), phase2=f64(col(T, 9, 11)), phase3=f64(col(T, 10, 11)),
+            iimp1=i32(col(I, 0, 6)), iimp2=i32(col(I, 1, 6)), iimp3=i32(col(I, 2, 6)),
+            iimp4=i32(col(I, 3, 6)), kimp=f64(col(I, 4, 6)), impeq=ideg / PI180,
+            impeq_deg=ideg,
+            spec1=i32(col(S, 0, 5)), spec2=i32(col(S, 1, 5)),
+            p14=f64([s[2:5] for s in S]).reshape(-1, 3),
+            idxtip3=i32(self.idxtip3),
+        )
+
+
+def _water_frame():
+    half = math.radians(WAT_THETA) / 2.0
+    o = np.zeros(3)
+    h1 = np.array([WAT_ROH * math.sin(half), 0.0, WAT_ROH * math.cos(half)])
+    h2 = np.array([-WAT_ROH * math.sin(half), 0.0, WAT_ROH * math.cos(half)])
+    return np.stack([o, h1, h2])
+
+
+def _random_rotations(rng, n):
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def _add_water(tb: _TopoBuilder):
+    base = tb.npt + 1
+    tb.add_monomer('TIP3', [('OH2', 15.999, WAT_QO, WAT_EPS_O, WAT_RMIN2_O),
+                            ('H1', 1.008, WAT_QH, 0.0, 0.0),
+                            ('H2', 1.008, WAT_QH, 0.0, 0.0)])
+    o, h1, h2 = base, base + 1, base + 2
+    tb.bond(o, h1, 450.0, WAT_ROH)
+    tb.bond(o, h2, 450.0, WAT_ROH)
+    tb.angle(h1, o, h2, 55.0, WAT_THETA)
+    tb.exclude(o, h1)
+    tb.exclude(o, h2)
+    tb.exclude(h1, h2)
+
+
+def water_positions(centres, rng):
+    frame = _water_frame()
+    R = _random_rotations(rng, len(centres))
+    pos = centres[:, None, :] + np.einsum('nij,kj->nki', R, frame)
+    return pos.reshape(-1, 3)
+
+
+@dataclasses.dataclass
+class System:
+    topo: Topology
+    coor: np.ndarray          # float64 [npt, 3]
+    box: tuple                # (a, b, c)
+    meta: dict
+
+
+def water_box(n_side: int, L: float, seed: int, jitter: float = 0.3) -> System:
+    """W = n_side^3 rigid waters: O on an n_side^3 lattice of the cubic
+    box [-L/2, L/2]^3 plus U(-jitter, jitter) per coordinate, uniformly
+    random orientations."""
+    ss = np.random.SeedSequence(seed)
+    r_jit, r_rot = [np.random.default_rng(s) for s in ss.spawn(2)]
+    sp = L / n_side
+    g = (np.arange(n_side) + 0.5) * sp - 0.5 * L
+    X, Y, Z = np.meshgrid(g, g, g, indexing='ij')
+    cen = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    cen = cen + r_jit.uniform(-jitter, jitter, cen.shape)
+    coor = water_positions(cen, r_rot)
+    tb = _TopoBuilder()
+    for _ in range(len(cen)):
+        _add_water(tb)
+    topo = tb.build()
+    return System(topo, coor, (L, L, L),
+                  dict(kind='water_box', n_side=n_side, L=L, seed=seed, jitter=jitter))
+
+
+# ---------------------------------------------------------------------------
+# Heteropolymer in water (random-walk chains + 3-site water)
+# ---------------------------------------------------------------------------
+
+def _rand_unit(rng):
+    v = rng.standard_normal(3)
+    return v / np.linalg.norm(v)
+
+
+def _grow_chain(rng, n, start, L):
+    """Random walk with bond U(1.0,1.5) A, angle U(100,120) deg and
+    uniform torsion."""
+    pts = [start]
+    d = _rand_unit(rng)
+    pts.append(start + rng.uniform(1.0, 1.5) * d)
+    for k in range(2, n):
+        b = rng.uniform(1.0, 1.5)
+        th = math.radians(rng.uniform(100.0, 120.0))
+        phi = rng.uniform(-math.pi, math.pi)
+        p1, p2 = pts[-2], pts[-1]
+        u = p2 - p1
+        u /= np.linalg.norm(u)
+        if k == 2:
+            ref = _rand_unit(rng)
+        else:
+            ref = pts[-3] - p1
+        n1 = np.cross(u, ref)
+        if np.linalg.norm(n1) < 1e-8:
+            n1 = np.cross(u, _rand_unit(rng))
+        n1 /= np.linalg.norm(n1)
+        n2 = np.cross(n1, u)
+        dvec = (-math.cos(th) * u + math.sin(th) * (math.cos(phi) * n2 + math.sin(phi) * n1))
+        pts.append(p2 + b * dvec)
+    return np.array(pts)
+
+
+def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = None,
+                  chain_len=(10, 20), n_types: int = 24, mono_size: int = 4,
+                  v14: float = 0.5, el14: float = 0.5,
+                  water_min_dist: float = 2.5) -> System:
+    """Random-walk heteropolymer chains plus 3-site water in a cubic box
+    (centred at the origin).  Bonds/angles/proper torsions from chain
+    topology (k_b U(300,500), k_theta U(50,100), torsion period n in
+    {1,2,3} with barrier U(0,2)), one improper per interior quadruple
+    (i-1,i,i+1,i+2) with phi0 = 0 or 180 deg, n_types LJ types with
+    eps~U(0.02,0.2), Rmin/2~U(1.0,2.1); charges N(0,0.4) shifted to net
+    zero per chain.  Monomers hold <= mono_size consecutive chain atoms.
+    Exclusions 1-2, 1-3 and 1-4 (MOIL excludes 1-4 and re-adds them via
+    the scaled special list, connect/angtor.f:483-577)."""
+    ss = np.random.SeedSequence(seed)
+    rng_walk, rng_par, rng_wat, rng_rot = [np.random.default_rng(s) for s in ss.spawn(4)]
+    types_eps = rng_par.uniform(0.02, 0.2, n_types)
+    types_rmh = rng_par.uniform(1.0, 2.1, n_types)
+    tb = _TopoBuilder()
+    chain_xyz = []
+    chain_atoms = []
+    for c in range(n_chains):
+        n = int(rng_walk.integers(chain_len[0], chain_len[1] + 1))
+        start = rng_walk.uniform(-0.5 * L + 3.0, 0.5 * L - 3.0, 3)
+        xyz = _grow_chain(rng_walk, n, start, L)
+        # keep the chain's atoms inside the primary box
+        lo, hi = xyz.min(0), xyz.max(0)
+        shift = np.where(hi > 0.5 * L - 0.5, 0.5 * L - 0.5 - hi, 0.0)
+        shift = np.where(lo + shift < -0.5 * L + 0.5, -0.5 * L + 0.5 - lo, shift)
+        xyz = xyz + shift
+        q = rng_par.normal(0.0, 0.4, n)
+        q = q - q.mean()
+        q = np.array([_q('9.5f', v) for v in q])
+        # make the quantised charges sum to exactly zero per chain
+        resid = round(float(np.sum(q)) * 1e5)
+        q[-1] = _q('9.5f', q[-1] - resid * 1e-5)
+        typ = rng_par.integers(0, n_types, n)
+        base = tb.npt + 1
+        k = 0
+        while k < n:
+            m = min(mono_size, n - k)
+            atoms = [('C%d' % ((k + t) % 10), 12.011, q[k + t], types_eps[typ[k + t]],
+                      types_rmh[typ[k + t]]) for t in range(m)]
+            tb.add_monomer('PLY%d' % (c % 10) if m == mono_size else 'PLE%d' % (c % 10), atoms)
+            k += m
+        idx = np.arange(base, base + n)
+        for a in range(n - 1):
+            tb.bond(idx[a], idx[a + 1], rng_par.uniform(300, 500),
+                    float(np.linalg.norm(xyz[a + 1] - xyz[a])))
+            tb.exclude(idx[a], idx[a + 1])
+        for a in range(n - 2):
+            v1 = xyz[a] - xyz[a + 1]
+            v2 = xyz[a + 2] - xyz[a + 1]
+            th = math.degrees(math.acos(np.dot(v1, v2) / np.linalg.norm(v1) / np.linalg.norm(v2)))
+            tb.angle(idx[a], idx[a + 1], idx[a + 2], rng_par.uniform(50, 100), th)
+            tb.exclude(idx[a], idx[a + 2])
+        for a in range(n - 3):
+            per = int(rng_par.integers(1, 4))
+            kk = [0.0, 0.0, 0.0]
+            kk[per - 1] = rng_par.uniform(0.0, 2.0)
+            ph = [1.0, 1.0, 1.0]
+            ph[per - 1] = float(rng_par.choice([-1.0, 1.0]))
+            tb.torsion(idx[a], idx[a + 1], idx[a + 2], idx[a + 3], per, kk[0], kk[1], kk[2],
+                       ph[0], ph[1], ph[2])
+            tb.exclude(idx[a], idx[a + 3])
+            tb.improper(idx[a], idx[a + 1], idx[a + 2], idx[a + 3],
+                        rng_par.uniform(5.0, 20.0), float(rng_par.choice([0.0, 180.0])))
+        chain_xyz.append(xyz)
+        chain_atoms.append(idx)
+    # 1-4 specials (scaled) for torsion end atoms
+    seen = set()
+    for idx in chain_atoms:
+        for a in range(len(idx) - 3):
+            key = (int(idx[a]), int(idx[a + 3]))
+            if key not in seen:
+                seen.add(key)
+                tb.special14(key[0], key[1], v14, el14)
+    poly = np.concatenate(chain_xyz) if chain_xyz else np.zeros((0, 3))
+    # water: lattice sites at ~ liquid spacing, dropped near polymer atoms
+    nw_target = n_water
+    if nw_target is None:
+        nw_target = int(round((L ** 3 - 0.0) * 0.0334)) - len(poly) // 3
+    n_side = int(math.ceil(nw_target ** (1.0 / 3.0) * 1.15)) + 1
+    sp = L / n_side
+    g = (np.arange(n_side) + 0.5) * sp - 0.5 * L
+    X, Y, Z = np.meshgrid(g, g, g, indexing='ij')
+    cen = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    cen = cen + rng_wat.uniform(-0.2, 0.2, cen.shape)
+    if len(poly):
+        keep = np.ones(len(cen), bool)
+        cell = max(water_min_dist, 1.0)
+        # brute-force in chunks (sizes here are modest)
+        for s in range(0, len(cen), 4096):
+            d2 = ((cen[s:s + 4096, None, :] - poly[None, :, :]) ** 2).sum(-1)
+            keep[s:s + 4096] = d2.min(1) >= water_min_dist ** 2
+        cen = cen[keep]
+    order = rng_wat.permutation(len(cen))[:nw_target]
+    cen = cen[np.sort(order)]
+    wxyz = water_positions(cen, rng_rot)
+    for _ in range(len(cen)):
+        _add_water(tb)
+    topo = tb.build()
+    coor = np.concatenate([poly, wxyz]) if len(poly) else wxyz
+    return System(topo, coor, (L, L, L),
+                  dict(kind='hetero', n_chains=n_chains, L=L, seed=seed, n_water=len(cen)))
+
+
+# ---------------------------------------------------------------------------
+# MOIL .con writer (layout read by generic/rconn.f:46-497)
+# ---------------------------------------------------------------------------
+
+def _fmt_rows(vals, per, fmt):
+    out = []
+    for s in range(0, len(vals), per):
+        out.append(''.join(fmt % v for v in vals[s:s + per]))
+    return out
+
+
+def write_con(path: str, t: Topology, arith: bool = False):
+    L = []
+    L.append('12.1')
+    L.append('~ CONNECTIVITY FILE FOR MOLECULES: ')
+    L.append('~ totmon npt nb nangl ntors nimp totex totspe lestyp NBULK nmb nchg nbeta nwaters arith prll mdivyes')
+    nchg = int(np.sum(t.flagchr))
+    hdr = [t.totmon, t.npt, len(t.ib1), len(t.iangl1), len(t.itor1), len(t.iimp1),
+           len(t.exc2), len(t.spec1), 0, 1, 0, nchg, 0, t.nwaters]
+    L.append(' ' + ''.join('%8d' % v for v in hdr) + ' %s F F' % ('T' if arith else 'F'))
+    L.append('SYNT')
+    L.append('~ Pointers to last particle of BULK')
+    L.append(' %7d' % t.npt)
+    L.append('~ Monomer names ')
+    L.extend(_fmt_rows([('%-4s' % m)[:4] for m in t.moname], 10, ' %s'))
+    L.append('~ Pointers to last particle of monomer')
+    L.extend(_fmt_rows(list(t.poipt[1:]), 10, ' %7d'))
+    L.append('~ Properties of particles list : ')
+    L.append('~pt mono ptid lesid mutaid ptnm   ptms   ptchg   epsgm6 epsgm12 ptwei ptsaid hbond')
+    mono_of = np.searchsorted(t.poipt[1:], np.arange(1, t.npt + 1))  # 0-based monomer
+    for i in range(t.npt):
+        L.append(' %7d %7d %3d %3d %3d %-4s %7.3f  %9.5f %12.5E%12.5E%10.3E %3d  ' % (
+            i + 1, mono_of[i] + 1, 1, 0, 0, t.ptnm[i][:4], t.ptms[i], t.ptchg[i],
+            t.epsgm6[i], t.epsgm12[i], 1.0, 0))
+    if len(t.ib1):
+        L.append('~ Bonds list: ')
+        L.append('~ ib1 ib2 kbond req ')
+        for a, b, k, r in zip(t.ib1, t.ib2, t.kbond, t.req):
+            L.append(' %7d %7d %10.4f %10.4f' % (a, b, k, r))
+    if len(t.iangl1):
+        L.append('~ Angles list: ')
+        L.append('~ iangl1 iangl2 iangl3 kangl angleq ')
+        for a, b, c, k, th in zip(t.iangl1, t.iangl2, t.iangl3, t.kangl, t.angleq_deg):
+            L.append(' %7d %7d %7d %10.5f %10.5f' % (a, b, c, k, th))
+    if len(t.itor1):
+        L.append('~ Torsions list: ')
+        L.append('~ itor1 itor2 itor3 itor4 period ktors1 ktors2 ktors3 phase1 phase2 phase3 ')
+        for r in zip(t.itor1, t.itor2, t.itor3, t.itor4, t.period, t.ktors1, t.ktors2,
+                     t.ktors3, t.phase1, t.phase2, t.phase3):
+            L.append(' %7d %7d %7d %7d %7d %10.4f %10.4f %10.4f %6.2f%6.2f%6.2f' % r)
+    if len(t.iimp1):
+        L.append('~ Improper torsion properties: ')
+        L.append('~iimp1 iimp2 iimp3 iimp4 kimp impeq ')
+        for r in zip(t.iimp1, t.iimp2, t.iimp3, t.iimp4, t.kimp, t.impeq_deg):
+            L.append(' %7d %7d %7d %7d %15.8E %15.8E' % r)
+    if len(t.exc2):
+        L.append('~ Exclusion list 1-2 1-3 1-4, set as followed:')
+        L.append('~ atom number, number of exclusions and list')
+        for i in range(1, t.npt + 1):
+            n = int(t.exc1[i] - t.exc1[i - 1])
+            if n > 0:
+                L.append(' %7d %7d' % (i, n))
+                L.extend(_fmt_rows(list(t.exc2[t.exc1[i - 1]:t.exc1[i]]), 10, ' %7d'))
+    if len(t.spec1):
+        L.append('~ Special list 1-4  set as followed:')
+        L.append('~ atom number, atom number, esp12*v14 eps6*v14 qiqj*el14 ')
+        for a, b, p in zip(t.spec1, t.spec2, t.p14):
+            L.append(' %7d %7d %30.5f %15.5f %15.5f' % (a, b, p[0], p[1], p[2]))
+    L.append('~ Charge flags. T = charged atom , F = uncharged')
+    fl = ['T' if f else 'F' for f in t.flagchr]
+    L.extend(_fmt_rows(fl, 15, ' %s'))
+    L.append('~ LES copy labels ')
+    L.extend([' ' + r for r in _fmt_rows([0] * t.npt, 15, '%6d')])
+    if t.nwaters:
+        L.append(' ~ The monomers divisions below are TIP3/SPC/SPCE waters')
+        L.extend(_fmt_rows(list(t.idxtip3), 10, '%8d'))
+    with open(path, 'w') as f:
+        f.write('\n'.join(L) + '\n')

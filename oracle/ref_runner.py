@@ -1,0 +1,84 @@
+"""Runs MOIL's own Fortran energy path (built by oracle/ref/Makefile into
+oracle/_ref/moil_dump) on a synthetic system and parses its full-precision
+dump.  Test infrastructure only (golden-fixture generation and the CPU
+baseline); never imported by the product.
+"""
+from __future__ import annotations
+
+import os
+import re
+import resource
+import subprocess
+import tempfile
+
+import numpy as np
+
+from moil_fnrfd_amd.systems import System, write_con
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_EXE = os.path.join(HERE, '_ref', 'moil_dump')
+
+
+def available() -> bool:
+    return os.path.exists(REF_EXE)
+
+
+def parse_dump(path: str) -> dict:
+    out = {}
+    with open(path, 'rb') as f:
+        buf = f.read()
+    p = 0
+    while p < len(buf):
+        name = buf[p:p + 8].decode().strip()
+        typ, n = np.frombuffer(buf, np.int32, 2, p + 8)
+        p += 16
+        dt = np.float64 if typ == 1 else np.int32
+        sz = 8 if typ == 1 else 4
+        out[name] = np.frombuffer(buf, dt, int(n), p).copy()
+        p += sz * int(n)
+    return out
+
+
+def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
+             nlist=1, sigma=0.0, seed=12345, dump_lists=True, workdir=None,
+             timeout=3600) -> dict:
+    """cut: rvmx relx rvbg rebg cutm (Angstrom); ewald: dtol, nfft (3), order."""
+    if not available():
+        raise RuntimeError('oracle/_ref/moil_dump is not built (make -f oracle/ref/Makefile)')
+    tmp = workdir or tempfile.mkdtemp(prefix='moilref_')
+    os.makedirs(tmp, exist_ok=True)
+    con = os.path.join(tmp, 'sys.con')
+    crd = os.path.join(tmp, 'sys.bin')
+    par = os.path.join(tmp, 'sys.par')
+    dmp = os.path.join(tmp, 'sys.dump')
+    write_con(con, sysm.topo)
+    np.ascontiguousarray(sysm.coor, np.float64).tofile(crd)
+    a, b, c = sysm.box
+    n1, n2, n3 = ewald['nfft']
+    with open(par, 'w') as f:
+        f.write(con + '\n' + crd + '\n')
+        f.write('%.17g %.17g %.17g\n' % (a, b, c))
+        f.write('%.17g %.17g %.17g %.17g %.17g\n' % (cut['rvmx'], cut['relx'], cut['rvbg'],
+                                                    cut['rebg'], cut['cutm']))
+        f.write('%.17g %d %d %d %d\n' % (ewald['dtol'], n1, n2, n3, ewald.get('order', 4)))
+        f.write('%.17g\n' % cut.get('eps', 1.0))
+        f.write('%d %d %d %.17g %d %d\n' % (int(squeeze), neval, nlist, sigma, seed,
+                                           int(dump_lists)))
+        f.write(dmp + '\n')
+    def _big_stack():
+        # MOIL keeps large work arrays on the stack (e.g. symm/nbmsym.f:36-46)
+        resource.setrlimit(resource.RLIMIT_STACK,
+                           (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    r = subprocess.run([REF_EXE, par], cwd=tmp, capture_output=True, text=True,
+                       timeout=timeout, preexec_fn=_big_stack)
+    if r.returncode != 0 or not os.path.exists(dmp):
+        raise RuntimeError('moil_dump failed (%d):\n%s\n%s' % (r.returncode, r.stdout[-3000:],
+                                                                r.stderr[-3000:]))
+    d = parse_dump(dmp)
+    d['stdout'] = r.stdout
+    for line in r.stdout.splitlines():
+        if line.startswith('MOILTIME'):
+            kv = dict(re.findall(r'(\w+)=\s*([-+0-9.Ee]+)', line))
+            d['tlist_s'] = float(kv['tlist_s'])
+            d['tforce_s'] = float(kv['tforce_s'])
+    return d
