@@ -177,9 +177,12 @@ int32_t moil_gpu_dump_water_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
 int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
                                  int64_t *n);
 
-/* Timing of the last eforce on the context's stream (ms per stage):
- * t[0] real-space water, t[1] real-space atoms, t[2] PME, t[3] bonded,
- * t[4] total.  Zero when timing is off (moil_gpu_set_timing). */
+/* Stage timing with HIP events on the launch stream.  While timing is on
+ * (moil_gpu_set_timing, which also clears), every eforce records its
+ * stages; moil_gpu_stage_times returns the summed milliseconds since the
+ * last call: t[0] real-space water kernel, t[1] real-space atom kernel,
+ * t[2] PME, t[3] bonded + exclusion + 1-4, t[4] whole eforce, t[5] the
+ * number of eforce calls timed. */
 int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t on);
 int32_t moil_gpu_stage_times(moil_ctx *ctx, double t[8]);
 

@@ -20,6 +20,7 @@ TERM_BOND, TERM_ANGLE, TERM_TORS, TERM_IMP = 1, 2, 4, 8
 TERM_VDW, TERM_EL, TERM_EL14, TERM_VDW14, TERM_HVDW0 = 16, 32, 64, 128, 256
 TERM_DEFAULT = 0x1FF
 
+N_ENERGIES = 15
 ENERGY_NAMES = ('e_total', 'e_bond', 'e_theta', 'e_tors', 'e_imp', 'e_vdw', 'e_el',
                 'e_vsym', 'e_lsym', 'e_el14', 'e_vdw14', 'e_corr', 'e_self',
                 'e_ew_receip', 'e_dir')

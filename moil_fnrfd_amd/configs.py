@@ -1,0 +1,74 @@
+"""Workload definitions: the synthetic systems and MOIL run parameters of
+the energy-path benchmark, plus small parity cases.
+
+Each entry returns (System, cut, ewald) where ``cut`` holds MOIL's
+cutoff keywords in Angstrom (rvmx relx rvbg rebg cutm; squared later as
+pot/energy.f:392-400 does) and ``ewald`` the PME keywords (dtol, grid,
+order).  dtol values reproduce the target beta through MOIL's own
+bisection (symm/ewald_init.f:298-335).
+"""
+from __future__ import annotations
+
+from . import abi
+from .systems import hetero_system, water_box
+
+# dtol giving beta = 0.34 at rc = 9 A and beta = 0.31 at rc = 10 A
+DTOL_B034_RC9 = 1.67573104443267911e-06
+DTOL_B031_RC10 = 1.16486573671995838e-06
+
+
+def water216():
+    """216 rigid waters (648 atoms), 18.6 A box, 9 A cutoffs, 20^3 grid."""
+    s = water_box(6, 18.6, seed=1000)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=9.0, rebg=9.0, cutm=10.0)
+    ew = dict(dtol=DTOL_B034_RC9, nfft=(20, 20, 20), order=4)
+    return s, cut, ew
+
+
+def water32k():
+    """32,768 waters (98,304 atoms) in a 68.9 A box: 10 A cutoffs with a
+    1.5 A list skin, beta 0.31, 72^3 grid."""
+    s = water_box(32, 68.9, seed=1001)
+    cut = dict(rvmx=10.0, relx=10.0, rvbg=11.5, rebg=11.5, cutm=12.5)
+    ew = dict(dtol=DTOL_B031_RC10, nfft=(72, 72, 72), order=4)
+    return s, cut, ew
+
+
+def water_scaled(n_side, seed, L=None):
+    """Water boxes at the 32k-water density (0.100 waters/A^3) with the
+    32k-water cutoffs; grid spacing ~0.96 A."""
+    if L is None:
+        L = 68.9 * n_side / 32.0
+    s = water_box(n_side, L, seed=seed)
+    n = max(8, int(round(L / 0.957)))
+    n += n % 2
+    cut = dict(rvmx=10.0, relx=10.0, rvbg=11.5, rebg=11.5, cutm=12.5)
+    ew = dict(dtol=DTOL_B031_RC10, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def water_buffered(n_side=10, seed=1010):
+    """Liquid-density waters with rvmx < relx < rvbg < rebg so that both
+    water lists and the buffered evaluation checks are exercised."""
+    L = 3.1 * n_side
+    s = water_box(n_side, L, seed=seed)
+    cut = dict(rvmx=7.0, relx=8.0, rvbg=8.5, rebg=9.5, cutm=10.5)
+    n = int(round(L / 1.0))
+    n += n % 2
+    ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def hetero_small(seed=2000, n_chains=12, L=24.0):
+    s = hetero_system(n_chains, L, seed=seed)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
+    n = int(round(L)) + int(round(L)) % 2
+    ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def params_for(sysm, cut, ew, precision=abi.PREC_FP64, terms=abi.TERM_DEFAULT):
+    return abi.make_params(sysm.box, rvmx=cut['rvmx'], relx=cut['relx'], rvbg=cut['rvbg'],
+                           rebg=cut['rebg'], cutm=cut['cutm'], dtol=ew['dtol'], nfft=ew['nfft'],
+                           order=ew.get('order', 4), eps=cut.get('eps', 1.0),
+                           precision=precision, terms=terms)

@@ -1,0 +1,646 @@
+// C ABI of the MI355X MOIL energy library (declared in include/moil_gpu.h):
+// context set-up (the ewald_init equivalent), squeeze, list build and the
+// eforce step that chains the bonded, PME and real-space kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" int32_t moil_gpu_abi_version(void) { return MOIL_GPU_ABI_VERSION; }
+extern "C" const char *moil_gpu_last_error(void) { return g_err.c_str(); }
+
+int pme_conv_blocks(const moil_ctx *c);
+int bonded_max_blocks(const moil_ctx *c);
+
+#define CHECK_RC(x)          \
+  do {                       \
+    int32_t r_ = (x);        \
+    if (r_ != MOIL_GPU_OK) return r_; \
+  } while (0)
+
+static int32_t fail(int32_t code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// ewald_init (symm/ewald_init.f:8-242) on the host, FP64.
+
+// find_ewaldcof, symm/ewald_init.f:298-335: double x until
+// erfc(x rc)/rc < dtol, then bisect for 2^-60 tolerance
+static double ewald_coefficient(double cutoff, double dtol) {
+  double x = 0.5, term;
+  int i = 0;
+  do {
+    x = 2.0 * x;
+    i = i + 1;
+    term = std::erfc(x * cutoff) / cutoff;
+  } while (term >= dtol);
+  int n = i + 60;
+  double xlo = 0.0, xhi = x;
+  for (i = 1; i <= n; i++) {
+    x = (xlo + xhi) / 2;
+    term = std::erfc(x * cutoff) / cutoff;
+    if (term >= dtol) xlo = x;
+    else xhi = x;
+  }
+  return x;
+}
+
+// erfc table: node values erfc(i*del) with clamped end slopes, cubic
+// spline coefficients in MOIL's erf_arr(4,i) meaning (value, first,
+// second, third derivative on interval i); same table as
+// symm/ewald_init.f:418-449.  Slopes from the tridiagonal system
+// s_{i-1} + 4 s_i + s_{i+1} = 3 (f_{i+1} - f_{i-1}) / del.
+static void erfc_spline_table(std::vector<double> &tab, int n, double del) {
+  tab.assign((size_t)4 * n, 0.0);
+  const double fac = 2.0 / std::sqrt(3.14159265358979323846);
+  for (int i = 0; i < n; i++) tab[4 * i] = std::erfc(del * i);
+  std::vector<double> s(n), cp(n), dp(n);
+  s[0] = -fac;
+  double xe = (n - 1) * del;
+  s[n - 1] = -fac * std::exp(-xe * xe);
+  for (int i = 1; i <= n - 2; i++) {
+    double rhs = 3.0 * (tab[4 * (i + 1)] - tab[4 * (i - 1)]) / del;
+    if (i == 1) rhs -= s[0];
+    if (i == n - 2) rhs -= s[n - 1];
+    if (i == 1) {
+      cp[i] = 0.25;
+      dp[i] = rhs / 4.0;
+    } else {
+      double m = 4.0 - cp[i - 1];
+      cp[i] = 1.0 / m;
+      dp[i] = (rhs - dp[i - 1]) / m;
+    }
+  }
+  if (n > 2) {
+    s[n - 2] = dp[n - 2];
+    for (int i = n - 3; i >= 1; i--) s[i] = dp[i] - cp[i] * s[i + 1];
+  }
+  for (int i = 0; i < n - 1; i++) {
+    double dd = (tab[4 * (i + 1)] - tab[4 * i]) / del;
+    double d3 = s[i] + s[i + 1] - 2.0 * dd;
+    tab[4 * i + 1] = s[i];
+    tab[4 * i + 2] = 2.0 * (dd - s[i] - d3) / del;
+    tab[4 * i + 3] = (d3 / del) * (6.0 / del);
+  }
+  tab[4 * (n - 1) + 1] = s[n - 1];
+}
+
+template <typename T>
+static int32_t upload(T **dst, const T *src, size_t n) {
+  size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+  HIP_CHECK(hipMalloc((void **)dst, bytes));
+  if (n > 0 && src) HIP_CHECK(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  return MOIL_GPU_OK;
+}
+
+static void ctx_free(moil_ctx *c) {
+  if (!c) return;
+  lists_free(c);
+  pme_free(c);
+  void *ptrs[] = {c->d_poipt, c->d_monoflag, c->d_flagchr, c->d_ptchg, c->d_eps6, c->d_eps12,
+                  c->d_exc_i, c->d_exc_j, c->d_ib, c->d_bpar, c->d_ia, c->d_apar, c->d_it,
+                  c->d_tpar, c->d_ii, c->d_ipar, c->d_sp, c->d_p14, c->d_wat_o, c->d_wat_mono,
+                  c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int32_t ensure_parts(moil_ctx *c, int need) {
+  if (need <= c->max_parts && c->d_epart) return MOIL_GPU_OK;
+  if (c->d_epart) HIP_CHECK(hipFree(c->d_epart));
+  c->max_parts = std::max(need, 64);
+  HIP_CHECK(hipMalloc(&c->d_epart, sizeof(double) * EN_COUNT * c->max_parts));
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, moil_ctx **out) {
+  if (!t || !p || !out) return fail(MOIL_GPU_EINVAL, "null argument");
+  *out = nullptr;
+  if (t->npt <= 0 || t->totmon <= 0) return fail(MOIL_GPU_EINVAL, "empty system");
+  if (p->order != 4) return fail(MOIL_GPU_EINVAL, "PME interpolation order must be 4");
+  if (!(p->a > 0 && p->b > 0 && p->c > 0)) return fail(MOIL_GPU_EINVAL, "box edges must be > 0");
+  if (p->nfft1 < 4 || p->nfft2 < 4 || p->nfft3 < 4) return fail(MOIL_GPU_EINVAL, "PME grid too small");
+  if (p->precision != MOIL_PREC_FP64 && p->precision != MOIL_PREC_MIXED)
+    return fail(MOIL_GPU_EINVAL, "unknown precision mode");
+  if (p->cutele2 <= 0 || p->eps <= 0 || p->dtol <= 0) return fail(MOIL_GPU_EINVAL, "bad cutoffs/eps/dtol");
+  // half-shell images double count when the list reach exceeds half a box
+  // edge (symm/syminit.f:20-21)
+  double reach = std::sqrt(std::max(p->cutebig2, p->cutvbig2));
+  if (2.0 * reach > std::min(p->a, std::min(p->b, p->c)))
+    return fail(MOIL_GPU_EINVAL, "list cutoff exceeds half the box edge");
+  double qtot = 0;
+  for (int i = 0; i < t->npt; i++) qtot += t->ptchg[i];
+  if (std::fabs(qtot) > 1e-7)  // symm/ewald_init.f:62-72
+    return fail(MOIL_GPU_ECHARGE, "The charge of the system is " + std::to_string(qtot) +
+                                      ": add counterions and try again");
+  if (t->poipt[0] != 0 || t->poipt[t->totmon] != t->npt)
+    return fail(MOIL_GPU_EINVAL, "poipt must start at 0 and end at npt");
+
+  moil_ctx *c = new moil_ctx();
+  c->par = *p;
+  c->npt = t->npt;
+  c->totmon = t->totmon;
+  c->nb = t->nb; c->nangl = t->nangl; c->ntors = t->ntors; c->nimp = t->nimp;
+  c->totspe = t->totspe;
+  c->fix_shift = (p->precision == MOIL_PREC_MIXED) ? FIX_SHIFT_MIXED : FIX_SHIFT_FP64;
+  c->fix_scale = std::ldexp(1.0, c->fix_shift);
+  c->fix_inv = std::ldexp(1.0, -c->fix_shift);
+  int32_t rc;
+#define TRY(x)                 \
+  do {                         \
+    rc = (x);                  \
+    if (rc != MOIL_GPU_OK) {   \
+      ctx_free(c);             \
+      return rc;               \
+    }                          \
+  } while (0)
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EHIP, "hipStreamCreate failed");
+  }
+  // ---- Ewald set-up ----
+  double cutoff = std::sqrt(p->cutele2);
+  c->beta = ewald_coefficient(cutoff, p->dtol);
+  c->erftbdns = 5000.0;  // 0.5*perft, symm/ewald_init.f:103-104
+  c->ntab = (int)(c->beta * c->erftbdns * cutoff * 1.5);
+  erfc_spline_table(c->h_erf, c->ntab, 1.0 / c->erftbdns);
+  {
+    double pi = 4.0 * std::atan(1.0), spi = std::sqrt(1.0 / pi), s = 0;
+    for (int i = 0; i < t->npt; i++) s += t->ptchg[i] * t->ptchg[i];
+    c->e_self = -(c->beta * spi * (KOFDIE / p->eps) * s);  // symm/ewald_init.f:131-150
+  }
+  TRY(upload(&c->d_erf, c->h_erf.data(), c->h_erf.size()));
+  // ---- topology ----
+  TRY(upload(&c->d_poipt, t->poipt, (size_t)t->totmon + 1));
+  TRY(upload(&c->d_monoflag, t->monoflag, (size_t)t->totmon));
+  TRY(upload(&c->d_flagchr, t->flagchr, (size_t)t->npt));
+  TRY(upload(&c->d_ptchg, t->ptchg, (size_t)t->npt));
+  TRY(upload(&c->d_eps6, t->epsgm6, (size_t)t->npt));
+  TRY(upload(&c->d_eps12, t->epsgm12, (size_t)t->npt));
+  {
+    std::vector<int32_t> ei, ej;
+    for (int i = 1; i <= t->npt - 1; i++)  // pot/cdie_ewald.f:68,83 (atom npt ignored)
+      for (int k = t->exc1[i - 1]; k < t->exc1[i]; k++) {
+        ei.push_back(i - 1);
+        ej.push_back(t->exc2[k] - 1);
+      }
+    c->nexcl = (int)ei.size();
+    TRY(upload(&c->d_exc_i, ei.data(), ei.size()));
+    TRY(upload(&c->d_exc_j, ej.data(), ej.size()));
+  }
+  {
+    std::vector<int32_t> ib(2 * (size_t)c->nb);
+    std::vector<double> bp(2 * (size_t)c->nb);
+    for (int m = 0; m < c->nb; m++) {
+      ib[2 * m] = t->ib1[m]; ib[2 * m + 1] = t->ib2[m];
+      bp[2 * m] = t->kbond[m]; bp[2 * m + 1] = t->req[m];
+    }
+    TRY(upload(&c->d_ib, ib.data(), ib.size()));
+    TRY(upload(&c->d_bpar, bp.data(), bp.size()));
+    std::vector<int32_t> ia(3 * (size_t)c->nangl);
+    std::vector<double> ap(2 * (size_t)c->nangl);
+    for (int m = 0; m < c->nangl; m++) {
+      ia[3 * m] = t->iangl1[m]; ia[3 * m + 1] = t->iangl2[m]; ia[3 * m + 2] = t->iangl3[m];
+      ap[2 * m] = t->kangl[m]; ap[2 * m + 1] = t->angleq[m];
+    }
+    TRY(upload(&c->d_ia, ia.data(), ia.size()));
+    TRY(upload(&c->d_apar, ap.data(), ap.size()));
+    std::vector<int32_t> it(4 * (size_t)c->ntors);
+    std::vector<double> tp(6 * (size_t)c->ntors);
+    for (int m = 0; m < c->ntors; m++) {
+      it[4 * m] = t->itor1[m]; it[4 * m + 1] = t->itor2[m];
+      it[4 * m + 2] = t->itor3[m]; it[4 * m + 3] = t->itor4[m];
+      tp[6 * m] = t->ktors1[m]; tp[6 * m + 1] = t->ktors2[m]; tp[6 * m + 2] = t->ktors3[m];
+      tp[6 * m + 3] = t->phase1[m]; tp[6 * m + 4] = t->phase2[m]; tp[6 * m + 5] = t->phase3[m];
+    }
+    TRY(upload(&c->d_it, it.data(), it.size()));
+    TRY(upload(&c->d_tpar, tp.data(), tp.size()));
+    std::vector<int32_t> ii(4 * (size_t)c->nimp);
+    std::vector<double> ip(2 * (size_t)c->nimp);
+    for (int m = 0; m < c->nimp; m++) {
+      ii[4 * m] = t->iimp1[m]; ii[4 * m + 1] = t->iimp2[m];
+      ii[4 * m + 2] = t->iimp3[m]; ii[4 * m + 3] = t->iimp4[m];
+      ip[2 * m] = t->kimp[m]; ip[2 * m + 1] = t->impeq[m];
+    }
+    TRY(upload(&c->d_ii, ii.data(), ii.size()));
+    TRY(upload(&c->d_ipar, ip.data(), ip.size()));
+    std::vector<int32_t> sp(2 * (size_t)c->totspe);
+    for (int m = 0; m < c->totspe; m++) {
+      sp[2 * m] = t->spec1[m]; sp[2 * m + 1] = t->spec2[m];
+    }
+    TRY(upload(&c->d_sp, sp.data(), sp.size()));
+    TRY(upload(&c->d_p14, t->p14, 3 * (size_t)c->totspe));
+  }
+  // ---- waters (monomers named TIP3/SPC*) ----
+  for (int m = 0; m < t->totmon; m++) {
+    if (t->monoflag[m] & MOIL_MONO_WATER) {
+      int nat = t->poipt[m + 1] - t->poipt[m];
+      if (nat < 3) {
+        ctx_free(c);
+        return fail(MOIL_GPU_EINVAL, "water monomer with fewer than 3 atoms");
+      }
+      c->h_wat_mono.push_back(m + 1);
+      c->h_wat_o.push_back(t->poipt[m + 1] - 3);  // pot/watwat_ewald.f:103
+    } else {
+      c->has_nonwater = true;
+    }
+  }
+  c->nwat = (int)c->h_wat_mono.size();
+  if (c->nwat != t->nwaters) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EINVAL, "nwaters does not match the number of TIP3/SPC monomers");
+  }
+  if (c->has_nonwater) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EINVAL, "non-water monomers: atom pair lists not available in this build");
+  }
+  if (c->nwat >= 2) {
+    // parameters of the last water, pot/watwat_ewald.f:40-72
+    int m = t->idxtip3[t->nwaters - 1];
+    int io = t->poipt[m] - 2 - 1, ih = t->poipt[m] - 1;
+    double epst = (p->terms & MOIL_TERM_EL) ? 1.0 : 0.0;
+    double pick = (p->terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
+    c->oo6 = pick * t->epsgm6[io] * t->epsgm6[io];
+    c->oo12 = pick * t->epsgm12[io] * t->epsgm12[io];
+    c->qq_oo = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[io];
+    c->qq_oh = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[ih];
+    c->qq_hh = epst * KOFDIE / p->eps * t->ptchg[ih] * t->ptchg[ih];
+  }
+  TRY(upload(&c->d_wat_mono, c->h_wat_mono.data(), c->h_wat_mono.size()));
+  TRY(upload(&c->d_wat_o, c->h_wat_o.data(), c->h_wat_o.size()));
+  // ---- buffers ----
+  auto mal = [&](void **ptr, size_t bytes) -> int32_t {
+    HIP_CHECK(hipMalloc(ptr, std::max<size_t>(bytes, 16)));
+    return MOIL_GPU_OK;
+  };
+  TRY(mal((void **)&c->d_facc, sizeof(long long) * 3 * c->npt));
+  TRY(mal((void **)&c->d_energy, sizeof(double) * 32));
+  TRY(mal((void **)&c->d_coor, sizeof(double) * 3 * c->npt));
+  TRY(mal((void **)&c->d_dpot, sizeof(double) * 3 * c->npt));
+  TRY(mal(&c->d_wpos, (p->precision == MOIL_PREC_MIXED ? 48 : 72) * (size_t)std::max(c->nwat, 1)));
+  TRY(lists_init(c));
+  TRY(pme_init(c));
+  TRY(ensure_parts(c, std::max(pme_conv_blocks(c), bonded_max_blocks(c))));
+#undef TRY
+  *out = c;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_finalize(moil_ctx *c) {
+  if (c) {
+    hipStreamSynchronize(c->stream);
+    ctx_free(c);
+  }
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_ewald_info(moil_ctx *c, double *ewaldcof, double *e_self,
+                                       int32_t *ntab, double *erftbdns) {
+  if (!c) return fail(MOIL_GPU_EINVAL, "null context");
+  if (ewaldcof) *ewaldcof = c->beta;
+  if (e_self) *e_self = c->e_self;
+  if (ntab) *ntab = c->ntab;
+  if (erftbdns) *erftbdns = c->erftbdns;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_erf_table(moil_ctx *c, double *erf_arr, int32_t cap_rows) {
+  if (!c || !erf_arr) return fail(MOIL_GPU_EINVAL, "null argument");
+  int n = std::min(cap_rows, c->ntab);
+  std::memcpy(erf_arr, c->h_erf.data(), sizeof(double) * 4 * n);
+  return MOIL_GPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// squeeze, symm/squeeze.f:64-145 (one wrap per axis, y first, MOIL order)
+__global__ void k_squeeze(double *__restrict__ coor, const int32_t *__restrict__ poipt,
+                          const int32_t *__restrict__ monoflag, int totmon, double a, double b,
+                          double c) {
+#pragma clang fp contract(off)
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= totmon) return;
+  int f = monoflag[m];
+  bool cm = (f & MOIL_MONO_SQZ_CENT) != 0;
+  if (!(cm || (f & MOIL_MONO_SQZ_FIRST))) return;
+  int is = poipt[m], ie = poipt[m + 1];
+  double xcm, ycm, zcm;
+  if (cm) {
+    xcm = ycm = zcm = 0.0;
+    for (int k = is; k < ie; k++) {
+      xcm = xcm + coor[3 * k];
+      ycm = ycm + coor[3 * k + 1];
+      zcm = zcm + coor[3 * k + 2];
+    }
+    double xn = 1.0 / (double)(ie - is);
+    xcm = xcm * xn;
+    ycm = ycm * xn;
+    zcm = zcm * xn;
+  } else {
+    xcm = coor[3 * is];
+    ycm = coor[3 * is + 1];
+    zcm = coor[3 * is + 2];
+  }
+  const double xlow = -a * 0.5, xhigh = a * 0.5, ylow = -b * 0.5, yhigh = b * 0.5;
+  const double zlow = -c * 0.5, zhigh = c * 0.5;
+  for (int k = is; k < ie; k++) {
+    if (ycm < ylow) coor[3 * k + 1] = coor[3 * k + 1] + b;
+    if (ycm > yhigh) coor[3 * k + 1] = coor[3 * k + 1] - b;
+    if (xcm < xlow) coor[3 * k] = coor[3 * k] + a;
+    if (zcm < zlow) coor[3 * k + 2] = coor[3 * k + 2] + c;
+    if (xcm > xhigh) coor[3 * k] = coor[3 * k] - a;
+    if (zcm > zhigh) coor[3 * k + 2] = coor[3 * k + 2] - c;
+  }
+}
+
+static hipStream_t pick_stream(moil_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+extern "C" int32_t moil_gpu_squeeze_dev(moil_ctx *c, double *d_coor, void *stream) {
+  if (!c || !d_coor) return fail(MOIL_GPU_EINVAL, "null argument");
+  hipStream_t st = pick_stream(c, stream);
+  k_squeeze<<<(c->totmon + 255) / 256, 256, 0, st>>>(d_coor, c->d_poipt, c->d_monoflag, c->totmon,
+                                                      c->par.a, c->par.b, c->par.c);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_squeeze(moil_ctx *c, double *coor) {
+  if (!c || !coor) return fail(MOIL_GPU_EINVAL, "null argument");
+  HIP_CHECK(hipMemcpyAsync(c->d_coor, coor, sizeof(double) * 3 * c->npt, hipMemcpyHostToDevice, c->stream));
+  CHECK_RC(moil_gpu_squeeze_dev(c, c->d_coor, c->stream));
+  HIP_CHECK(hipMemcpyAsync(coor, c->d_coor, sizeof(double) * 3 * c->npt, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, void *stream) {
+  if (!c || !d_coor) return fail(MOIL_GPU_EINVAL, "null argument");
+  hipStream_t st = pick_stream(c, stream);
+  CHECK_RC(lists_build(c, d_coor, st));
+  CHECK_RC(ensure_parts(c, std::max(c->nwchunk, c->nachunk)));
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_build_lists(moil_ctx *c, const double *coor) {
+  if (!c || !coor) return fail(MOIL_GPU_EINVAL, "null argument");
+  HIP_CHECK(hipMemcpyAsync(c->d_coor, coor, sizeof(double) * 3 * c->npt, hipMemcpyHostToDevice, c->stream));
+  CHECK_RC(moil_gpu_build_lists_dev(c, c->d_coor, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MOIL_GPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_zero_i64(long long *__restrict__ p, size_t n) {
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) p[g] = 0;
+}
+__global__ void k_zero_f64(double *__restrict__ p, size_t n) {
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) p[g] = 0.0;
+}
+
+__global__ void k_finalize_forces(const long long *__restrict__ facc, const double *__restrict__ fpme,
+                                  int has_pme, size_t n, double inv, double *__restrict__ dpot) {
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  double v = (double)facc[g] * inv;
+  if (has_pme) v += fpme[g];
+  dpot[g] = v;
+}
+
+struct EnCfg {
+  int max_parts;
+  double e_self;
+  int has_bond, has_angle, has_tors, has_imp, has_nb, has_14;
+};
+
+// deterministic reduction of the per-workgroup partials and assembly of
+// the energy record in eforce's order (pot/eforce.f:100-205)
+__global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep, EnCfg C,
+                                                  double *__restrict__ out) {
+  __shared__ double red[256];
+  __shared__ double tot[EN_COUNT];
+  int t = threadIdx.x;
+  for (int k = 0; k < EN_COUNT; k++) {
+    double s = 0;
+    for (int i = t; i < C.max_parts; i += 256) s += ep[(size_t)k * C.max_parts + i];
+    red[t] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (t < off) red[t] += red[t + off];
+      __syncthreads();
+    }
+    if (t == 0) tot[k] = red[0];
+    __syncthreads();
+  }
+  if (t == 0) {
+    moil_energies e;
+    e.e_bond = tot[EN_BOND];
+    e.e_theta = tot[EN_THETA];
+    e.e_tors = tot[EN_TORS];
+    e.e_imp = tot[EN_IMP];
+    e.e_vdw = tot[EN_VDW];
+    e.e_el = tot[EN_EL];
+    e.e_vsym = tot[EN_VSYM];
+    e.e_lsym = tot[EN_LSYM];
+    e.e_el14 = tot[EN_EL14];
+    e.e_vdw14 = tot[EN_VDW14];
+    e.e_corr = tot[EN_CORR];
+    e.e_self = C.e_self;
+    e.e_ew_receip = tot[EN_RECIP];
+    double et = 0.0;
+    if (C.has_bond) et += e.e_bond;
+    if (C.has_angle) et += e.e_theta;
+    if (C.has_tors) et += e.e_tors;
+    if (C.has_imp) et += e.e_imp;
+    if (C.has_nb) {
+      et += e.e_ew_receip + e.e_corr + e.e_self;
+      et += e.e_vdw + e.e_el;
+      if (C.has_14) et += e.e_el14 + e.e_vdw14;
+      et += e.e_vsym + e.e_lsym;
+    }
+    e.e_total = et;
+    e.e_dir = e.e_el + e.e_lsym;
+    const double *src = &e.e_total;
+    for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = src[k];
+  }
+}
+
+// stage timing: event pairs recorded on the launch stream, summed lazily
+static hipEvent_t ev_get(moil_ctx *c) {
+  if (c->ev_next == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_next++];
+}
+struct StageTimer {
+  moil_ctx *c;
+  int stage;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  StageTimer(moil_ctx *c_, int s_, hipStream_t st_) : c(c_), stage(s_), st(st_) {
+    if (c->timing && (a = ev_get(c))) hipEventRecord(a, st);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = ev_get(c);
+    if (!b) return;
+    hipEventRecord(b, st);
+    c->ev_rec.push_back({stage, {a, b}});
+  }
+};
+
+extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
+                                       double *d_energies, void *stream) {
+  if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "eforce called before build_lists");
+  hipStream_t st = pick_stream(c, stream);
+  const uint32_t T = c->par.terms;
+  const bool nb = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
+  size_t n3 = 3 * (size_t)c->npt;
+  StageTimer whole(c, 4, st);
+  k_zero_i64<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, n3);
+  size_t ne = (size_t)EN_COUNT * c->max_parts;
+  k_zero_f64<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(c->d_epart, ne);
+  {
+    StageTimer tb(c, 3, st);
+    CHECK_RC(bonded_run(c, d_coor, st));
+  }
+  if (nb) {
+    StageTimer tp(c, 2, st);
+    CHECK_RC(pme_run(c, d_coor, st));
+  }
+  if (nb) {
+    CHECK_RC(realspace_water_pack(c, d_coor, st));
+    StageTimer tw(c, 0, st);
+    CHECK_RC(realspace_water(c, d_coor, st));
+  }
+  k_finalize_forces<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, c->d_fpme, nb ? 1 : 0,
+                                                                  n3, c->fix_inv, d_dpot);
+  EnCfg C;
+  C.max_parts = c->max_parts;
+  C.e_self = c->e_self;
+  C.has_bond = (T & MOIL_TERM_BOND) && c->nb > 0;
+  C.has_angle = (T & MOIL_TERM_ANGLE) && c->nangl > 0;
+  C.has_tors = (T & MOIL_TERM_TORS) && c->ntors > 0;
+  C.has_imp = (T & MOIL_TERM_IMP) && c->nimp > 0;
+  C.has_nb = nb;
+  C.has_14 = c->totspe > 0 && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14));
+  k_energies<<<1, 256, 0, st>>>(c->d_epart, C, d_energies ? d_energies : c->d_energy);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_eforce(moil_ctx *c, const double *coor, double *dpot, moil_energies *e) {
+  if (!c || !coor || !dpot) return fail(MOIL_GPU_EINVAL, "null argument");
+  size_t n3 = 3 * (size_t)c->npt;
+  HIP_CHECK(hipMemcpyAsync(c->d_coor, coor, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+  CHECK_RC(moil_gpu_eforce_dev(c, c->d_coor, c->d_dpot, c->d_energy, c->stream));
+  HIP_CHECK(hipMemcpyAsync(dpot, c->d_dpot, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+  double en[MOIL_N_ENERGIES];
+  HIP_CHECK(hipMemcpyAsync(en, c->d_energy, sizeof(en), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (e) std::memcpy(e, en, sizeof(en));
+  return MOIL_GPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int32_t moil_gpu_list_counts(moil_ctx *c, int64_t counts[8]) {
+  if (!c || !counts) return fail(MOIL_GPU_EINVAL, "null argument");
+  for (int k = 0; k < 8; k++) counts[k] = 0;
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "no lists built");
+  if (c->nwitems) {
+    std::vector<WItem> it(c->nwitems);
+    HIP_CHECK(hipMemcpy(it.data(), c->d_witems, sizeof(WItem) * it.size(), hipMemcpyDeviceToHost));
+    for (const WItem &w : it) {
+      int img = w.s > 0 ? 2 : 0;
+      counts[img + 0] += __builtin_popcountll(w.m1);
+      counts[img + 1] += __builtin_popcountll(w.m2);
+    }
+  }
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_dump_water_pairs(moil_ctx *c, int32_t *rows, int64_t cap, int64_t *n) {
+  if (!c || !n) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "no lists built");
+  *n = 0;
+  if (c->nwat == 0 || c->nwitems == 0) return MOIL_GPU_OK;
+  std::vector<WItem> it(c->nwitems);
+  std::vector<int32_t> s2w(c->nwat), boff(c->nwblk * NBUCKET + 1);
+  HIP_CHECK(hipMemcpy(it.data(), c->d_witems, sizeof(WItem) * it.size(), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(s2w.data(), c->d_slot2wat, sizeof(int32_t) * s2w.size(), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(boff.data(), c->d_boff, sizeof(int32_t) * boff.size(), hipMemcpyDeviceToHost));
+  int64_t k = 0;
+  for (int b = 0; b < c->nwblk; b++)
+    for (int q = boff[b * NBUCKET]; q < boff[(b + 1) * NBUCKET]; q++) {
+      const WItem &w = it[q];
+      int jm = c->h_wat_mono[s2w[w.j]];
+      for (int kind = 1; kind <= 2; kind++) {
+        uint64_t m = kind == 1 ? w.m1 : w.m2;
+        while (m) {
+          int i = __builtin_ctzll(m);
+          m &= m - 1;
+          int im = c->h_wat_mono[s2w[b * WBLOCK + i]];
+          if (rows && k < cap) {
+            rows[4 * k] = im;
+            rows[4 * k + 1] = jm;
+            rows[4 * k + 2] = w.s;
+            rows[4 * k + 3] = kind;
+          }
+          k++;
+        }
+      }
+    }
+  *n = k;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t cap, int64_t *n) {
+  if (!c || !n) return fail(MOIL_GPU_EINVAL, "null argument");
+  (void)rows;
+  (void)cap;
+  *n = 0;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_set_timing(moil_ctx *c, int32_t on) {
+  if (!c) return fail(MOIL_GPU_EINVAL, "null context");
+  c->timing = on != 0;
+  c->ev_rec.clear();
+  c->ev_next = 0;
+  return MOIL_GPU_OK;
+}
+
+// t[stage] = summed milliseconds since the last call, t[4+stage]...:
+// t[0] water real space, t[1] atom real space, t[2] PME, t[3] bonded,
+// t[4] whole eforce, t[5] number of eforce calls timed
+extern "C" int32_t moil_gpu_stage_times(moil_ctx *c, double t[8]) {
+  if (!c || !t) return fail(MOIL_GPU_EINVAL, "null argument");
+  for (int k = 0; k < 8; k++) t[k] = 0.0;
+  for (auto &r : c->ev_rec) {
+    HIP_CHECK(hipEventSynchronize(r.second.second));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+    if (r.first >= 0 && r.first < 5) t[r.first] += ms;
+    if (r.first == 4) t[5] += 1.0;
+  }
+  c->ev_rec.clear();
+  c->ev_next = 0;
+  return MOIL_GPU_OK;
+}

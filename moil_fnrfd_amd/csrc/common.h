@@ -1,0 +1,69 @@
+// Shared device/host definitions of the MI355X MOIL energy library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/moil_gpu.h"
+
+#define KOFDIE 332.0716  // COMMON/CONVERT.BLOCK:13
+
+// Water-block geometry: 64 waters (one wavefront's worth of mask bits).
+#define WBLOCK 64
+// Popcount buckets used to order a block's items (descending popcount).
+#define NBUCKET 8
+
+// Fixed-point force accumulators: int64 with FIX_SHIFT fractional bits.
+#define FIX_SHIFT_FP64 40
+#define FIX_SHIFT_MIXED 32
+
+// Energy accumulator slots (per-workgroup partial sums).
+enum {
+  EN_BOND = 0, EN_THETA, EN_TORS, EN_IMP, EN_VDW, EN_EL, EN_VSYM, EN_LSYM,
+  EN_EL14, EN_VDW14, EN_CORR, EN_RECIP, EN_COUNT
+};
+
+// One water-pair work item: water `j` (slot index, the untranslated
+// member), MOIL symmetry operation `s` (0 = real pair, 1..13 = the i-side
+// water is translated by symop s), and masks over the 64 i-waters of the
+// owning block: m1 = listwt1 pairs, m2 = listwt2 pairs.
+struct __attribute__((aligned(32))) WItem {
+  int32_t j;
+  int32_t s;
+  uint64_t m1;
+  uint64_t m2;
+  uint64_t pad;
+};
+
+// Atom-pair work item (same scheme over 64-atom blocks); l1/l2/l3 are
+// MOIL's list1/list2/list3 memberships.
+struct __attribute__((aligned(32))) AItem {
+  int32_t j;
+  int32_t s;
+  uint64_t l1;
+  uint64_t l2;
+  uint64_t l3;
+};
+
+#define HIP_CHECK(x)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                __FILE__ + ":" + std::to_string(__LINE__));                  \
+      return MOIL_GPU_EHIP;                                                  \
+    }                                                                        \
+  } while (0)
+
+void set_error(const std::string &msg);
+
+// MOIL's 13 half-shell symmetry operations in its order
+// (symm/syminit.f:80-164); s = 0 is the untranslated (real) cell.
+__host__ __device__ inline void symop_of(int s, int *ia, int *jb, int *kc) {
+  *ia = (s == 0 || s == 4 || (s >= 8 && s <= 10)) ? 0 : 1;
+  *jb = (s <= 4) ? 0 : (s <= 10 ? 1 : -1);
+  int k0 = (s >= 11) ? 11 : (s >= 8) ? 8 : (s >= 5) ? 5 : 1;
+  *kc = (s == 0) ? 0 : (s == 4) ? 1 : 1 - (s - k0);
+}

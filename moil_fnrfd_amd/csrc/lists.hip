@@ -1,0 +1,561 @@
+// Buffered pair-list builder for MOIL's water-group lists on gfx950.
+//
+// Reproduces the pair set of nbondm (pot/nbondm.f:79-199, real water
+// pairs) and nbmsym (symm/nbmsym.f:59-233, the 13 half-shell image
+// translations) bit for bit: every predicate is evaluated in FP64 in
+// MOIL's operation order with FMA contraction disabled, including the
+// partial-distance early exits against cutmono2, the monomer-centre
+// "need_only" face filter, the rch2 box-distance filter with its
+// large_cutmono switch, and `<=` (real) versus `<` (image) comparisons.
+//
+// Layout: waters are binned by their centre (the O atom) into a cell grid
+// over the box; cells are numbered brick-major (2x2x2-cell bricks) so that
+// 64 consecutive waters in cell order (a "block") are spatially compact.
+// For every block, one workgroup enumerates candidate partners cell by
+// cell and emits one item per partner water j and symmetry operation s
+// with a 64-bit mask over the block's waters (the translated member).
+// Items are compacted with wavefront ballots/prefix sums in a fixed order
+// (bit-reproducible) and grouped by descending popcount so the real-space
+// kernel's lanes see similar loop trip counts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+
+#define LB_THREADS 256
+#define LB_MAXCELLS 1024
+
+// ---------------------------------------------------------------------------
+// monomer centres, pot/nbondm.f:79-124 (water: the atom two before the
+// monomer's last atom; otherwise the unweighted mean times 1/n)
+__global__ void k_mono_cent(const double *__restrict__ coor, const int32_t *__restrict__ poipt,
+                            const int32_t *__restrict__ monoflag, int totmon, double ha,
+                            double hb, double hc, double *__restrict__ cent,
+                            int32_t *__restrict__ flags) {
+#pragma clang fp contract(off)
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= totmon) return;
+  double cx, cy, cz;
+  if (monoflag[m] & MOIL_MONO_WATER) {
+    int j = poipt[m + 1] - 2 - 1;
+    cx = coor[3 * j];
+    cy = coor[3 * j + 1];
+    cz = coor[3 * j + 2];
+  } else {
+    int ls = poipt[m], le = poipt[m + 1];
+    cx = cy = cz = 0.0;
+    for (int j = ls; j < le; j++) {
+      cx = cx + coor[3 * j];
+      cy = cy + coor[3 * j + 1];
+      cz = cz + coor[3 * j + 2];
+    }
+    double r2 = 1.0 / (double)(le - ls);
+    cx = cx * r2;
+    cy = cy * r2;
+    cz = cz * r2;
+  }
+  cent[3 * m] = cx;
+  cent[3 * m + 1] = cy;
+  cent[3 * m + 2] = cz;
+  // symm/nbmsym.f:87-95: any centre outside the box quadruples cutmono2
+  if (fabs(cx) > ha || fabs(cy) > hb || fabs(cz) > hc) atomicOr(&flags[0], 1);
+}
+
+struct Grid {
+  int gx, gy, gz, nbx, nby, nbz;
+  double g0x, g0y, g0z, csx, csy, csz;
+};
+
+__host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ inline int cell_coord(double p, double g0, double cs, int n) {
+  double f = floor((p - g0) / cs);
+  if (f < 0.0) return 0;
+  if (f > (double)(n - 1)) return n - 1;
+  return (int)f;
+}
+
+__host__ __device__ inline int cell_key(const Grid &g, int cx, int cy, int cz) {
+  int bx = cx >> 1, by = cy >> 1, bz = cz >> 1;
+  int loc = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
+  return ((bz * g.nby + by) * g.nbx + bx) * 8 + loc;
+}
+
+__global__ void k_wat_cells(const double *__restrict__ cent, const int32_t *__restrict__ wat_mono,
+                            int nwat, Grid g, int32_t *__restrict__ cell_of,
+                            int32_t *__restrict__ cnt) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwat) return;
+  const double *c = cent + 3 * (wat_mono[w] - 1);
+  int cx = cell_coord(c[0], g.g0x, g.csx, g.gx);
+  int cy = cell_coord(c[1], g.g0y, g.csy, g.gy);
+  int cz = cell_coord(c[2], g.g0z, g.csz, g.gz);
+  int k = cell_key(g, cx, cy, cz);
+  cell_of[w] = k;
+  atomicAdd(&cnt[k], 1);
+}
+
+// exclusive scan of n int32 values with one workgroup of 1024 threads
+__global__ void k_scan_excl(const int32_t *__restrict__ in, int32_t *__restrict__ out, int n) {
+  __shared__ int32_t part[1024];
+  int t = threadIdx.x;
+  int per = (n + 1023) / 1024;
+  int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; i++) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the 1024 partials
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int i = b; i < e; i++) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (t == 1023) out[n] = part[1023];
+}
+
+__global__ void k_wat_scatter(const int32_t *__restrict__ cell_of, int nwat,
+                              const int32_t *__restrict__ start, int32_t *__restrict__ fill,
+                              int32_t *__restrict__ slot2wat) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwat) return;
+  int k = cell_of[w];
+  int p = start[k] + atomicAdd(&fill[k], 1);
+  slot2wat[p] = w;
+}
+
+// sort each cell's waters by index so the slot order is deterministic
+__global__ void k_cell_sort(const int32_t *__restrict__ start, int ncell, int32_t *__restrict__ s2w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncell) return;
+  int b = start[k], e = start[k + 1];
+  for (int i = b + 1; i < e; i++) {
+    int v = s2w[i], j = i - 1;
+    while (j >= b && s2w[j] > v) {
+      s2w[j + 1] = s2w[j];
+      j--;
+    }
+    s2w[j + 1] = v;
+  }
+}
+
+__global__ void k_slot_fill(const int32_t *__restrict__ s2w, const int32_t *__restrict__ wat_mono,
+                            const int32_t *__restrict__ wat_o, const double *__restrict__ cent,
+                            int nwat, double *__restrict__ scent, int32_t *__restrict__ smono,
+                            int32_t *__restrict__ slot_o) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nwat) return;
+  int w = s2w[s];
+  int m = wat_mono[w];
+  smono[s] = m;
+  slot_o[s] = wat_o[w];
+  scent[3 * s] = cent[3 * (m - 1)];
+  scent[3 * s + 1] = cent[3 * (m - 1) + 1];
+  scent[3 * s + 2] = cent[3 * (m - 1) + 2];
+}
+
+struct WPar {
+  double a, b, c, ha, hb, hc;
+  double cutmono2, cutvbig2, cutebig2, R;
+  int nwat;
+};
+
+// MOIL's water-pair decision for (i translated by (da,db,dc), j).
+// Returns 0 (no pair), 1 (listwt1) or 2 (listwt2).
+// Real pairs: pot/nbondm.f:176-199; images: symm/nbmsym.f:197-226.
+__device__ inline int wpair_kind(double cix, double ciy, double ciz, double da, double db,
+                                 double dc, double cjx, double cjy, double cjz, bool image,
+                                 const WPar &P) {
+#pragma clang fp contract(off)
+  double rx, r2;
+  rx = (cix + da) - cjx;
+  r2 = rx * rx;
+  if (r2 > P.cutmono2) return 0;
+  rx = (ciy + db) - cjy;
+  r2 = r2 + rx * rx;
+  if (r2 > P.cutmono2) return 0;
+  rx = (ciz + dc) - cjz;
+  r2 = r2 + rx * rx;
+  if (!image) {
+    if (r2 <= P.cutvbig2) return 1;
+    if (r2 <= P.cutebig2) return 2;
+  } else {
+    if (r2 < P.cutvbig2) return 1;
+    if (r2 < P.cutebig2) return 2;
+  }
+  return 0;
+}
+
+// rch2 filter, symm/nbmsym.f:178-189
+__device__ inline bool image_i_ok(double cix, double ciy, double ciz, double da, double db,
+                                  double dc, double large_cutmono, const WPar &P) {
+#pragma clang fp contract(off)
+  double xch = fabs(cix + da) - P.ha, ych = fabs(ciy + db) - P.hb, zch = fabs(ciz + dc) - P.hc;
+  if (xch < 0.0) xch = 0.0;
+  if (ych < 0.0) ych = 0.0;
+  if (zch < 0.0) zch = 0.0;
+  double rch2 = xch * xch + ych * ych + zch * zch;
+  return !(rch2 > large_cutmono);
+}
+
+// need_only face filter, symm/nbmsym.f:97-115
+__device__ inline bool need_only_ok(double cjx, double cjy, double cjz, double da, double db,
+                                    double dc, const WPar &P) {
+#pragma clang fp contract(off)
+  double barx = 0.5 * da, bary = 0.5 * db, barz = 0.5 * dc;
+  if (fabs(barx) > 0) {
+    double q = cjx - barx;
+    if (q * q > P.cutmono2) return false;
+  }
+  if (fabs(bary) > 0) {
+    double q = cjy - bary;
+    if (q * q > P.cutmono2) return false;
+  }
+  if (fabs(barz) > 0) {
+    double q = cjz - barz;
+    if (q * q > P.cutmono2) return false;
+  }
+  return true;
+}
+
+__device__ inline int bucket_of(uint64_t m) {
+  int p = __popcll(m);
+  return (64 - p) >> 3;  // 0 (popcount 57..64) .. 7 (1..8)
+}
+
+// One workgroup per water block.  WRITE=false: count items per bucket.
+// WRITE=true: write items at boff + deterministic rank.
+template <bool WRITE>
+__global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
+    const double *__restrict__ scent, const int32_t *__restrict__ smono,
+    const int32_t *__restrict__ cstart, Grid g, WPar P, const int32_t *__restrict__ flags,
+    int32_t *__restrict__ bcount, const int32_t *__restrict__ boff, WItem *__restrict__ items) {
+  __shared__ double ic[WBLOCK][3];
+  __shared__ int32_t im[WBLOCK];
+  __shared__ double bb[6];
+  __shared__ uint64_t imask;
+  __shared__ int32_t cellk[LB_MAXCELLS];
+  __shared__ int32_t cellpre[LB_MAXCELLS + 1];
+  __shared__ int32_t scanbuf[LB_THREADS];
+  __shared__ int32_t bk_run[NBUCKET];
+  __shared__ int32_t wcnt[LB_THREADS / 64][NBUCKET];
+
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int s0 = b * WBLOCK;
+  const int nb = min(WBLOCK, P.nwat - s0);
+  if (t < WBLOCK) {
+    if (t < nb) {
+      ic[t][0] = scent[3 * (s0 + t)];
+      ic[t][1] = scent[3 * (s0 + t) + 1];
+      ic[t][2] = scent[3 * (s0 + t) + 2];
+      im[t] = smono[s0 + t];
+    } else {
+      im[t] = 0x7fffffff;
+    }
+  }
+  if (t < NBUCKET) bk_run[t] = 0;
+  __syncthreads();
+  if (t == 0) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < nb; i++)
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fmin(lo[k], ic[i][k]);
+        hi[k] = fmax(hi[k], ic[i][k]);
+      }
+    for (int k = 0; k < 3; k++) {
+      bb[k] = lo[k];
+      bb[3 + k] = hi[k];
+    }
+  }
+  __syncthreads();
+  const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
+
+  for (int s = 0; s < 14; s++) {
+    int ia, jb, kc;
+    symop_of(s, &ia, &jb, &kc);
+    const double da = ia * P.a, db = jb * P.b, dc = kc * P.c;
+    const bool image = s > 0;
+    // i-side filter for this translation
+    if (t < 64) {
+      bool ok = false;
+      if (t < nb) ok = !image || image_i_ok(ic[t][0], ic[t][1], ic[t][2], da, db, dc, large_cutmono, P);
+      uint64_t bal = __ballot(ok);
+      if (t == 0) imask = bal;
+    }
+    __syncthreads();
+    const uint64_t ivalid = imask;
+    if (ivalid == 0) {
+      __syncthreads();
+      continue;
+    }
+    // query box: translated block bbox grown by R
+    double q0[3], q1[3];
+    q0[0] = bb[0] + da - P.R; q1[0] = bb[3] + da + P.R;
+    q0[1] = bb[1] + db - P.R; q1[1] = bb[4] + db + P.R;
+    q0[2] = bb[2] + dc - P.R; q1[2] = bb[5] + dc + P.R;
+    int cx0 = clampi((int)floor((q0[0] - g.g0x) / g.csx), 0, g.gx - 1);
+    int cx1 = clampi((int)floor((q1[0] - g.g0x) / g.csx), 0, g.gx - 1);
+    int cy0 = clampi((int)floor((q0[1] - g.g0y) / g.csy), 0, g.gy - 1);
+    int cy1 = clampi((int)floor((q1[1] - g.g0y) / g.csy), 0, g.gy - 1);
+    int cz0 = clampi((int)floor((q0[2] - g.g0z) / g.csz), 0, g.gz - 1);
+    int cz1 = clampi((int)floor((q1[2] - g.g0z) / g.csz), 0, g.gz - 1);
+    const int nx = cx1 - cx0 + 1, ny = cy1 - cy0 + 1, nz = cz1 - cz0 + 1;
+    const int ncq = nx * ny * nz;
+    for (int cbeg = 0; cbeg < ncq; cbeg += LB_MAXCELLS) {
+      const int ncc = min(LB_MAXCELLS, ncq - cbeg);
+      // gather cell keys and sizes for this chunk of cells; cells whose
+      // box lies farther than R from the translated block bbox are empty
+      // for this query (sizes set to 0)
+      for (int u = t; u < LB_MAXCELLS; u += LB_THREADS) {
+        int sz = 0, k = 0;
+        if (u < ncc) {
+          int q = cbeg + u;
+          int cx = cx0 + q % nx, cy = cy0 + (q / nx) % ny, cz = cz0 + q / (nx * ny);
+          k = cell_key(g, cx, cy, cz);
+          sz = cstart[k + 1] - cstart[k];
+        }
+        cellk[u] = k;
+        cellpre[u] = sz;
+      }
+      __syncthreads();
+      // exclusive scan of the 1024 sizes: 4 per thread + 256-wide scan
+      {
+        int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
+            v3 = cellpre[4 * t + 3];
+        int tot = v0 + v1 + v2 + v3;
+        __syncthreads();
+        scanbuf[t] = tot;
+        __syncthreads();
+        for (int off = 1; off < LB_THREADS; off <<= 1) {
+          int add = (t >= off) ? scanbuf[t - off] : 0;
+          __syncthreads();
+          scanbuf[t] += add;
+          __syncthreads();
+        }
+        int base = scanbuf[t] - tot;
+        cellpre[4 * t] = base;
+        cellpre[4 * t + 1] = base + v0;
+        cellpre[4 * t + 2] = base + v0 + v1;
+        cellpre[4 * t + 3] = base + v0 + v1 + v2;
+        if (t == LB_THREADS - 1) cellpre[LB_MAXCELLS] = scanbuf[t];
+      }
+      __syncthreads();
+      const int ncand = cellpre[LB_MAXCELLS];
+      for (int fb = 0; fb < ncand; fb += LB_THREADS) {
+        const int f = fb + t;
+        uint64_t m1 = 0, m2 = 0;
+        int js = -1;
+        if (f < ncand) {
+          // binary search the cell holding flattened candidate f
+          int lo = 0, hi = ncc - 1;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cellpre[mid] <= f) lo = mid; else hi = mid - 1;
+          }
+          js = cstart[cellk[lo]] + (f - cellpre[lo]);
+          const double cjx = scent[3 * js], cjy = scent[3 * js + 1], cjz = scent[3 * js + 2];
+          const int jm = smono[js];
+          bool jok = !image || need_only_ok(cjx, cjy, cjz, da, db, dc, P);
+          // cheap reject: j's distance to the query box
+          if (jok) {
+            if (cjx < q0[0] || cjx > q1[0] || cjy < q0[1] || cjy > q1[1] || cjz < q0[2] ||
+                cjz > q1[2])
+              jok = false;
+          }
+          if (jok) {
+            uint64_t iv = ivalid;
+            while (iv) {
+              int i = __ffsll((long long)iv) - 1;
+              iv &= iv - 1;
+              if (!image && !(im[i] < jm)) continue;
+              int kd = wpair_kind(ic[i][0], ic[i][1], ic[i][2], da, db, dc, cjx, cjy, cjz, image, P);
+              if (kd == 1) m1 |= (1ull << i);
+              else if (kd == 2) m2 |= (1ull << i);
+            }
+          }
+        }
+        const uint64_t mm = m1 | m2;
+        const bool has = mm != 0;
+        const int bk = has ? bucket_of(mm) : -1;
+        // per-bucket wave counts (deterministic)
+        for (int k = 0; k < NBUCKET; k++) {
+          uint64_t bal = __ballot(has && bk == k);
+          if (lane == 0) wcnt[wv][k] = __popcll(bal);
+        }
+        __syncthreads();
+        if (WRITE) {
+          for (int k = 0; k < NBUCKET; k++) {
+            uint64_t bal = __ballot(has && bk == k);
+            if (has && bk == k) {
+              int before = 0;
+              for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
+              uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+              int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
+              WItem it;
+              it.j = js;
+              it.s = s;
+              it.m1 = m1;
+              it.m2 = m2;
+              it.pad = 0;
+              items[pos] = it;
+            }
+          }
+        }
+        __syncthreads();
+        if (t < NBUCKET) {
+          int tot = 0;
+          for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += wcnt[w2][t];
+          bk_run[t] += tot;
+        }
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (!WRITE && t < NBUCKET) bcount[b * NBUCKET + t] = bk_run[t];
+}
+
+// ---------------------------------------------------------------------------
+static void free_lists_buffers(moil_ctx *c) {
+  hipFree(c->d_cell_of); c->d_cell_of = nullptr;
+  hipFree(c->d_cell_cnt); c->d_cell_cnt = nullptr;
+  hipFree(c->d_cell_start); c->d_cell_start = nullptr;
+  hipFree(c->d_cell_fill); c->d_cell_fill = nullptr;
+}
+
+int32_t lists_init(moil_ctx *c) {
+  const moil_params &P = c->par;
+  HIP_CHECK(hipMalloc(&c->d_mono_cent, sizeof(double) * 3 * std::max(c->totmon, 1)));
+  HIP_CHECK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
+  if (c->nwat > 0) {
+    // cell grid over the box with ~8 waters per cell, even counts per
+    // dimension are not required (bricks may be partial)
+    double L[3] = {P.a, P.b, P.c};
+    double vol = P.a * P.b * P.c;
+    double edge = cbrt(vol * 8.0 / (double)c->nwat);
+    int g[3];
+    for (int k = 0; k < 3; k++) {
+      g[k] = std::max(1, (int)floor(L[k] / edge));
+      c->cs[k] = L[k] / g[k];
+      c->gmin[k] = -0.5 * L[k];
+    }
+    c->gx = g[0]; c->gy = g[1]; c->gz = g[2];
+    int nbx = (g[0] + 1) / 2, nby = (g[1] + 1) / 2, nbz = (g[2] + 1) / 2;
+    c->ncell = nbx * nby * nbz * 8;
+    HIP_CHECK(hipMalloc(&c->d_cell_of, sizeof(int32_t) * c->nwat));
+    HIP_CHECK(hipMalloc(&c->d_cell_cnt, sizeof(int32_t) * (c->ncell + 1)));
+    HIP_CHECK(hipMalloc(&c->d_cell_start, sizeof(int32_t) * (c->ncell + 1)));
+    HIP_CHECK(hipMalloc(&c->d_cell_fill, sizeof(int32_t) * (c->ncell + 1)));
+    HIP_CHECK(hipMalloc(&c->d_slot2wat, sizeof(int32_t) * c->nwat));
+    HIP_CHECK(hipMalloc(&c->d_slot_o, sizeof(int32_t) * c->nwat));
+    HIP_CHECK(hipMalloc(&c->d_scent, sizeof(double) * 3 * c->nwat));
+    HIP_CHECK(hipMalloc(&c->d_smono, sizeof(int32_t) * c->nwat));
+    c->nwblk = (c->nwat + WBLOCK - 1) / WBLOCK;
+    HIP_CHECK(hipMalloc(&c->d_bcount, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
+    HIP_CHECK(hipMalloc(&c->d_boff, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
+  }
+  return MOIL_GPU_OK;
+}
+
+void lists_free(moil_ctx *c) {
+  free_lists_buffers(c);
+  hipFree(c->d_mono_cent); c->d_mono_cent = nullptr;
+  hipFree(c->d_flags); c->d_flags = nullptr;
+  hipFree(c->d_slot2wat); c->d_slot2wat = nullptr;
+  hipFree(c->d_slot_o); c->d_slot_o = nullptr;
+  hipFree(c->d_scent); c->d_scent = nullptr;
+  hipFree(c->d_smono); c->d_smono = nullptr;
+  hipFree(c->d_bcount); c->d_bcount = nullptr;
+  hipFree(c->d_boff); c->d_boff = nullptr;
+  hipFree(c->d_witems); c->d_witems = nullptr;
+  hipFree(c->d_wchunk); c->d_wchunk = nullptr;
+}
+
+int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
+  const moil_params &P = c->par;
+  const int TB = 256;
+  HIP_CHECK(hipMemsetAsync(c->d_flags, 0, sizeof(int32_t) * 4, st));
+  if (c->totmon > 0)
+    k_mono_cent<<<(c->totmon + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_poipt, c->d_monoflag,
+                                                           c->totmon, 0.5 * P.a, 0.5 * P.b,
+                                                           0.5 * P.c, c->d_mono_cent, c->d_flags);
+  if (c->nwat > 0) {
+    Grid g;
+    g.gx = c->gx; g.gy = c->gy; g.gz = c->gz;
+    g.nbx = (c->gx + 1) / 2; g.nby = (c->gy + 1) / 2; g.nbz = (c->gz + 1) / 2;
+    g.g0x = c->gmin[0]; g.g0y = c->gmin[1]; g.g0z = c->gmin[2];
+    g.csx = c->cs[0]; g.csy = c->cs[1]; g.csz = c->cs[2];
+    HIP_CHECK(hipMemsetAsync(c->d_cell_cnt, 0, sizeof(int32_t) * (c->ncell + 1), st));
+    HIP_CHECK(hipMemsetAsync(c->d_cell_fill, 0, sizeof(int32_t) * (c->ncell + 1), st));
+    int nblk = (c->nwat + TB - 1) / TB;
+    k_wat_cells<<<nblk, TB, 0, st>>>(c->d_mono_cent, c->d_wat_mono, c->nwat, g, c->d_cell_of,
+                                     c->d_cell_cnt);
+    k_scan_excl<<<1, 1024, 0, st>>>(c->d_cell_cnt, c->d_cell_start, c->ncell);
+    k_wat_scatter<<<nblk, TB, 0, st>>>(c->d_cell_of, c->nwat, c->d_cell_start, c->d_cell_fill,
+                                       c->d_slot2wat);
+    k_cell_sort<<<(c->ncell + TB - 1) / TB, TB, 0, st>>>(c->d_cell_start, c->ncell, c->d_slot2wat);
+    double *d_scent = c->d_scent;
+    int32_t *d_smono = c->d_smono;
+    k_slot_fill<<<nblk, TB, 0, st>>>(c->d_slot2wat, c->d_wat_mono, c->d_wat_o, c->d_mono_cent,
+                                     c->nwat, d_scent, d_smono, c->d_slot_o);
+    WPar wp;
+    wp.a = P.a; wp.b = P.b; wp.c = P.c;
+    wp.ha = 0.5 * P.a; wp.hb = 0.5 * P.b; wp.hc = 0.5 * P.c;
+    wp.cutmono2 = P.cutmono2; wp.cutvbig2 = P.cutvbig2; wp.cutebig2 = P.cutebig2;
+    // candidate reach: every listed pair has r2 <= cutebig2 (and, through
+    // the early exits, r2 <= cutmono2 on its x and x+y partial sums)
+    double R = sqrt(std::max(P.cutebig2, 0.0));
+    wp.R = R * (1.0 + 1e-12) + 1e-9;
+    wp.nwat = c->nwat;
+    k_wblock_items<false><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
+                                                           wp, c->d_flags, c->d_bcount,
+                                                           nullptr, nullptr);
+    k_scan_excl<<<1, 1024, 0, st>>>(c->d_bcount, c->d_boff, c->nwblk * NBUCKET);
+    std::vector<int32_t> hoff(c->nwblk * NBUCKET + 1);
+    HIP_CHECK(hipMemcpyAsync(hoff.data(), c->d_boff, sizeof(int32_t) * hoff.size(),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    size_t total = (size_t)hoff.back();
+    if (total > c->witem_cap || c->d_witems == nullptr) {
+      hipFree(c->d_witems);
+      c->witem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
+      HIP_CHECK(hipMalloc(&c->d_witems, sizeof(WItem) * c->witem_cap));
+    }
+    c->nwitems = total;
+    k_wblock_items<true><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
+                                                          wp, c->d_flags, nullptr, c->d_boff,
+                                                          c->d_witems);
+    // chunk table: runs of <= wchunk_items items inside one block
+    std::vector<int32_t> ch;
+    for (int b = 0; b < c->nwblk; b++) {
+      int beg = hoff[b * NBUCKET], end = hoff[(b + 1) * NBUCKET];
+      for (int q = beg; q < end; q += c->wchunk_items) {
+        ch.push_back(b);
+        ch.push_back(q);
+        ch.push_back(std::min(end, q + c->wchunk_items));
+      }
+    }
+    c->nwchunk = (int)ch.size() / 3;
+    hipFree(c->d_wchunk);
+    c->d_wchunk = nullptr;
+    if (c->nwchunk > 0) {
+      HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * ch.size()));
+      HIP_CHECK(hipMemcpyAsync(c->d_wchunk, ch.data(), sizeof(int32_t) * ch.size(),
+                               hipMemcpyHostToDevice, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  c->have_lists = true;
+  return MOIL_GPU_OK;
+}

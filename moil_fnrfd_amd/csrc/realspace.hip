@@ -1,0 +1,391 @@
+// Real-space Lennard-Jones + Ewald-erfc Coulomb for MOIL's water groups
+// (pot/watwat_ewald.f:131-612,624-1085 and symm/symwat_ewald.f:96-620,
+// 640-1150): per listed water pair, O-O LJ when r_OO^2 <= rvmx^2 (listwt1
+// only) and all nine site-pair erfc Coulomb terms with no cutoff check
+// (listwt1) or only when r_OO^2 <= relx^2 (listwt2).  Charges and LJ
+// parameters are those of the last water (pot/watwat_ewald.f:40-72).
+//
+// Work decomposition (see lists.hip): a workgroup takes a chunk of the
+// items of one 64-water block.  The block's waters (the translated pair
+// members) sit in LDS; each lane owns one item (partner water j plus a
+// 64-bit mask over the block) and walks the mask.  j-forces accumulate in
+// registers and go to global 64-bit fixed-point accumulators once per
+// item; i-forces accumulate in LDS with 64-bit integer atomics and are
+// flushed once per workgroup.  Integer accumulation makes the forces
+// independent of execution order (bit-reproducible).
+//
+// MOIL_PREC_MIXED: FP32 pair arithmetic.  Positions are carried as the O
+// atom (float hi + float lo) and the H offsets from their O, so every
+// site-site vector is formed from the O-O difference plus small offsets
+// (relative error ~1e-7 of the O-O distance instead of the box size).
+// erfc(x) = exp(-x^2) t P(t), t = 1/(1+p x): the functional form of
+// Abramowitz & Stegun 7.1.26 with a degree-8 polynomial fitted here for
+// relative accuracy 3e-8 on [0, 6].
+// MOIL_PREC_FP64: FP64 arithmetic in MOIL's operation order with MOIL's
+// cubic-spline erfc table (pot/cdie_ewald.f:152-160).
+#include <hip/hip_runtime.h>
+
+#include "ctx.h"
+
+#define RS_THREADS 256
+
+struct WaterRS {
+  double qq_oo, qq_oh, qq_hh, oo12, oo6;
+  double beta, cutvdw2, cutele2, erftbdns;
+  double a, b, c;
+  double fix_scale;
+  int ntab;
+};
+
+// erfc fit coefficients (t = 1/(1 + ERFC_P x)); see file header
+#define ERFC_P 0.418f
+__device__ __forceinline__ float erfc_poly(float t) {
+  float p = 7.663338518e-02f;
+  p = fmaf(p, t, -3.531459250e-01f);
+  p = fmaf(p, t, 5.087678340e-01f);
+  p = fmaf(p, t, -1.952929589e-01f);
+  p = fmaf(p, t, 3.047273108e-01f);
+  p = fmaf(p, t, 1.823138229e-01f);
+  p = fmaf(p, t, 2.404413357e-01f);
+  p = fmaf(p, t, 2.355552142e-01f);
+  return p * t;
+}
+
+__device__ __forceinline__ long long fix64(double v, double scale) {
+  return __double2ll_rn(v * scale);
+}
+
+__device__ __forceinline__ void atomic_add_fix(long long *p, long long v) {
+  if (v != 0) atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// pack water positions per slot
+__global__ void k_pack_water_f32(const double *__restrict__ coor, const int32_t *__restrict__ slot_o,
+                                 int nwat, float4 *__restrict__ wpos) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nwat) return;
+  int o = slot_o[s];
+  double ox = coor[3 * o], oy = coor[3 * o + 1], oz = coor[3 * o + 2];
+  float hx = (float)ox, hy = (float)oy, hz = (float)oz;
+  float lx = (float)(ox - (double)hx), ly = (float)(oy - (double)hy), lz = (float)(oz - (double)hz);
+  float h1x = (float)(coor[3 * o + 3] - ox), h1y = (float)(coor[3 * o + 4] - oy),
+        h1z = (float)(coor[3 * o + 5] - oz);
+  float h2x = (float)(coor[3 * o + 6] - ox), h2y = (float)(coor[3 * o + 7] - oy),
+        h2z = (float)(coor[3 * o + 8] - oz);
+  wpos[3 * s] = make_float4(hx, hy, hz, h1x);
+  wpos[3 * s + 1] = make_float4(lx, ly, lz, h1y);
+  wpos[3 * s + 2] = make_float4(h1z, h2x, h2y, h2z);
+}
+
+__global__ void k_pack_water_f64(const double *__restrict__ coor, const int32_t *__restrict__ slot_o,
+                                 int nwat, double *__restrict__ wpos) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nwat) return;
+  int o = slot_o[s];
+#pragma unroll
+  for (int k = 0; k < 9; k++) wpos[9 * s + k] = coor[3 * o + k];
+}
+
+// ---------------------------------------------------------------------------
+// block-level deterministic reduction of per-thread doubles
+__device__ inline double block_sum(double v, double *red) {
+  int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int off = RS_THREADS / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// FP32 (mixed precision) water kernel
+__global__ __launch_bounds__(RS_THREADS) void k_water_f32(
+    const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
+    const float4 *__restrict__ wpos, const int32_t *__restrict__ slot_o, int nwat, WaterRS P,
+    long long *__restrict__ facc, double *__restrict__ epart, int max_parts) {
+  __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
+  __shared__ long long iacc[WBLOCK * 9];
+  __shared__ double red[RS_THREADS];
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
+  const int s0 = blk * WBLOCK;
+  if (t < WBLOCK) {
+    if (s0 + t < nwat) {
+      iA[t] = wpos[3 * (s0 + t)];
+      iB[t] = wpos[3 * (s0 + t) + 1];
+      iC[t] = wpos[3 * (s0 + t) + 2];
+    }
+  }
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
+  __syncthreads();
+
+  const float beta = (float)P.beta;
+  const float qoo = (float)P.qq_oo, qoh = (float)P.qq_oh, qhh = (float)P.qq_hh;
+  const float a12 = (float)P.oo12, b6 = (float)P.oo6;
+  const float cutvdw2 = (float)P.cutvdw2, cutele2 = (float)P.cutele2;
+  const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
+  const float pe = ERFC_P;
+  const float log2e = 1.4426950408889634f;
+  const double fs = P.fix_scale;
+  double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
+
+  for (int it = beg + t; it < end; it += RS_THREADS) {
+    const WItem item = items[it];
+    const int js = item.j;
+    const float4 jA = wpos[3 * js], jB = wpos[3 * js + 1], jC = wpos[3 * js + 2];
+    int ia, jb, kc;
+    symop_of(item.s, &ia, &jb, &kc);
+    // partner O translated by -T (the i member carries +T), split hi/lo
+    const double ojx = (double)jA.x + (double)jB.x - ia * P.a;
+    const double ojy = (double)jA.y + (double)jB.y - jb * P.b;
+    const double ojz = (double)jA.z + (double)jB.z - kc * P.c;
+    const float jhx = (float)ojx, jhy = (float)ojy, jhz = (float)ojz;
+    const float jlx = (float)(ojx - (double)jhx), jly = (float)(ojy - (double)jhy),
+                jlz = (float)(ojz - (double)jhz);
+    const float j1x = jA.w, j1y = jB.w, j1z = jC.x;
+    const float j2x = jC.y, j2y = jC.z, j2z = jC.w;
+    float fj[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) fj[k] = 0.f;
+    float elj = 0.f, eel = 0.f;
+    uint64_t mask = item.m1 | item.m2;
+    while (mask) {
+      const int i = __ffsll((long long)mask) - 1;
+      const bool k2 = (item.m2 >> i) & 1ull;
+      mask &= mask - 1;
+      const float4 A = iA[i], B = iB[i], C = iC[i];
+      const float dx = (A.x - jhx) + (B.x - jlx);
+      const float dy = (A.y - jhy) + (B.y - jly);
+      const float dz = (A.z - jhz) + (B.z - jlz);
+      const float roo2 = dx * dx + dy * dy + dz * dz;
+      if (k2 && roo2 > cutele2) continue;
+      // site vectors r_ab = x_ia - x_jb
+      float ux[3], uy[3], uz[3];
+      ux[0] = dx; uy[0] = dy; uz[0] = dz;
+      ux[1] = dx + A.w; uy[1] = dy + B.w; uz[1] = dz + C.x;
+      ux[2] = dx + C.y; uy[2] = dy + C.z; uz[2] = dz + C.w;
+      const float vx[3] = {0.f, j1x, j2x}, vy[3] = {0.f, j1y, j2y}, vz[3] = {0.f, j1z, j2z};
+      float fi[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) fi[k] = 0.f;
+#pragma unroll
+      for (int sa = 0; sa < 3; sa++) {
+#pragma unroll
+        for (int sb = 0; sb < 3; sb++) {
+          const float rx = ux[sa] - vx[sb], ry = uy[sa] - vy[sb], rz = uz[sa] - vz[sb];
+          const float r2 = (sa == 0 && sb == 0) ? roo2 : (rx * rx + ry * ry + rz * rz);
+          const float qq = (sa == 0 && sb == 0) ? qoo : ((sa == 0 || sb == 0) ? qoh : qhh);
+          const float s = __builtin_amdgcn_rsqf(r2);
+          const float s2 = s * s;
+          const float x = beta * (r2 * s);
+          const float tt = __builtin_amdgcn_rcpf(fmaf(pe, x, 1.0f));
+          const float ex = __builtin_amdgcn_exp2f(-(x * x) * log2e);
+          const float erfcv = ex * erfc_poly(tt);
+          const float e = qq * erfcv * s;
+          eel += e;
+          float df = -s2 * (e + qq * c2b * ex);
+          if (sa == 0 && sb == 0 && !k2 && roo2 <= cutvdw2) {
+            const float s6 = s2 * s2 * s2;
+            const float A12 = a12 * s6 * s6, B6 = b6 * s6;
+            const float e1 = A12 - B6;
+            elj += e1;
+            df += -6.0f * s2 * (A12 + e1);
+          }
+          const float fx = df * rx, fy = df * ry, fz = df * rz;
+          fi[3 * sa] += fx;
+          fi[3 * sa + 1] += fy;
+          fi[3 * sa + 2] += fz;
+          fj[3 * sb] -= fx;
+          fj[3 * sb + 1] -= fy;
+          fj[3 * sb + 2] -= fz;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        long long v = fix64((double)fi[k], fs);
+        if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
+      }
+    }
+    const int oj = slot_o[js];
+#pragma unroll
+    for (int k = 0; k < 9; k++) atomic_add_fix(&facc[3 * oj + k], fix64((double)fj[k], fs));
+    if (item.s == 0) {
+      e_lj_r += (double)elj;
+      e_el_r += (double)eel;
+    } else {
+      e_lj_i += (double)elj;
+      e_el_i += (double)eel;
+    }
+  }
+  __syncthreads();
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
+    const int i = u / 9, k = u % 9;
+    if (s0 + i < nwat) {
+      const int oi = slot_o[s0 + i];
+      atomic_add_fix(&facc[3 * oi + k], iacc[u]);
+    }
+  }
+  double v;
+  v = block_sum(e_lj_r, red);
+  if (t == 0) epart[EN_VDW * max_parts + ch] = v;
+  v = block_sum(e_el_r, red);
+  if (t == 0) epart[EN_EL * max_parts + ch] = v;
+  v = block_sum(e_lj_i, red);
+  if (t == 0) epart[EN_VSYM * max_parts + ch] = v;
+  v = block_sum(e_el_i, red);
+  if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
+}
+
+// FP64 water kernel (MOIL parity mode)
+__global__ __launch_bounds__(RS_THREADS) void k_water_f64(
+    const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
+    const double *__restrict__ wpos, const int32_t *__restrict__ slot_o, int nwat, WaterRS P,
+    const double *__restrict__ erf_arr, long long *__restrict__ facc, double *__restrict__ epart,
+    int max_parts) {
+  __shared__ double ipos[WBLOCK * 9];
+  __shared__ long long iacc[WBLOCK * 9];
+  __shared__ double red[RS_THREADS];
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
+  const int s0 = blk * WBLOCK;
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
+    ipos[u] = (s0 + u / 9 < nwat) ? wpos[9 * s0 + u] : 0.0;
+    iacc[u] = 0;
+  }
+  __syncthreads();
+  const double del = 1.0 / P.erftbdns, ovt = 1.0 / 3.0;
+  const double fs = P.fix_scale;
+  double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
+  for (int it = beg + t; it < end; it += RS_THREADS) {
+    const WItem item = items[it];
+    const int js = item.j;
+    double xj[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) xj[k] = wpos[9 * js + k];
+    int ia, jb, kc;
+    symop_of(item.s, &ia, &jb, &kc);
+    const double T[3] = {ia * P.a, jb * P.b, kc * P.c};
+    double fj[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) fj[k] = 0.0;
+    double elj = 0, eel = 0;
+    uint64_t mask = item.m1 | item.m2;
+    while (mask) {
+      const int i = __ffsll((long long)mask) - 1;
+      const bool k2 = (item.m2 >> i) & 1ull;
+      mask &= mask - 1;
+      double xi[9];
+#pragma unroll
+      for (int sa = 0; sa < 3; sa++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) xi[3 * sa + k] = ipos[i * 9 + 3 * sa + k] + T[k];
+      const double ox = xi[0] - xj[0], oy = xi[1] - xj[1], oz = xi[2] - xj[2];
+      const double roo2 = ox * ox + oy * oy + oz * oz;
+      if (k2 && roo2 > P.cutele2) continue;
+      double fi[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) fi[k] = 0.0;
+#pragma unroll
+      for (int sa = 0; sa < 3; sa++) {
+#pragma unroll
+        for (int sb = 0; sb < 3; sb++) {
+          const double rx = xi[3 * sa] - xj[3 * sb], ry = xi[3 * sa + 1] - xj[3 * sb + 1],
+                       rz = xi[3 * sa + 2] - xj[3 * sb + 2];
+          const double r2 = rx * rx + ry * ry + rz * rz;
+          const double qq = (sa == 0 && sb == 0) ? P.qq_oo : ((sa == 0 || sb == 0) ? P.qq_oh : P.qq_hh);
+          const double s2 = 1.0 / r2;
+          const double rij = sqrt(r2);
+          const double s = 1.0 / rij;
+          double df = 0.0;
+          if (sa == 0 && sb == 0 && !k2 && r2 <= P.cutvdw2) {
+            const double s6 = s2 * s2 * s2;
+            const double A12 = P.oo12 * s6 * s6, B6 = P.oo6 * s6;
+            const double e1 = A12 - B6;
+            elj += e1;
+            df = -6.0 * s2 * (A12 + e1);
+          }
+          const double xerfc = P.beta * rij;
+          const int idx = (int)(P.erftbdns * xerfc);
+          const double dx = xerfc - idx * del;
+          const double *tb = erf_arr + 4 * idx;
+          const double derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
+          const double erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * ovt)) + tb[0];
+          eel += qq * s * erfcc;
+          df += -qq * s * (s2 * erfcc + s * P.beta * derfc);
+          fi[3 * sa] += df * rx;
+          fi[3 * sa + 1] += df * ry;
+          fi[3 * sa + 2] += df * rz;
+          fj[3 * sb] -= df * rx;
+          fj[3 * sb + 1] -= df * ry;
+          fj[3 * sb + 2] -= df * rz;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        long long v = fix64(fi[k], fs);
+        if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
+      }
+    }
+    const int oj = slot_o[js];
+#pragma unroll
+    for (int k = 0; k < 9; k++) atomic_add_fix(&facc[3 * oj + k], fix64(fj[k], fs));
+    if (item.s == 0) {
+      e_lj_r += elj;
+      e_el_r += eel;
+    } else {
+      e_lj_i += elj;
+      e_el_i += eel;
+    }
+  }
+  __syncthreads();
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
+    const int i = u / 9, k = u % 9;
+    if (s0 + i < nwat) atomic_add_fix(&facc[3 * slot_o[s0 + i] + k], iacc[u]);
+  }
+  double v;
+  v = block_sum(e_lj_r, red);
+  if (t == 0) epart[EN_VDW * max_parts + ch] = v;
+  v = block_sum(e_el_r, red);
+  if (t == 0) epart[EN_EL * max_parts + ch] = v;
+  v = block_sum(e_lj_i, red);
+  if (t == 0) epart[EN_VSYM * max_parts + ch] = v;
+  v = block_sum(e_el_i, red);
+  if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
+}
+
+int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st) {
+  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
+  const int TB = 256, nb = (c->nwat + TB - 1) / TB;
+  if (c->par.precision == MOIL_PREC_MIXED)
+    k_pack_water_f32<<<nb, TB, 0, st>>>(d_coor, c->d_slot_o, c->nwat, (float4 *)c->d_wpos);
+  else
+    k_pack_water_f64<<<nb, TB, 0, st>>>(d_coor, c->d_slot_o, c->nwat, (double *)c->d_wpos);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st) {
+  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
+  (void)d_coor;
+  WaterRS P;
+  P.qq_oo = c->qq_oo; P.qq_oh = c->qq_oh; P.qq_hh = c->qq_hh;
+  P.oo12 = c->oo12; P.oo6 = c->oo6;
+  P.beta = c->beta; P.cutvdw2 = c->par.cutvdw2; P.cutele2 = c->par.cutele2;
+  P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
+  P.fix_scale = c->fix_scale; P.ntab = c->ntab;
+  if (c->par.precision == MOIL_PREC_MIXED) {
+    k_water_f32<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, (const float4 *)c->d_wpos,
+                                                    c->d_slot_o, c->nwat, P, c->d_facc, c->d_epart,
+                                                    c->max_parts);
+  } else {
+    k_water_f64<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, (const double *)c->d_wpos,
+                                                    c->d_slot_o, c->nwat, P, c->d_erf, c->d_facc,
+                                                    c->d_epart, c->max_parts);
+  }
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}

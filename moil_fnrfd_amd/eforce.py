@@ -1,0 +1,178 @@
+"""Python host binding of libmoil_gpu.so, mirroring MOIL's energy-path
+call surface: ``ewald_init`` (context creation), ``squeeze``, ``nbondm``
+(which also performs ``syminit``'s image lists) and ``eforce``.
+
+MOIL's routines take no arguments and communicate through COMMON blocks
+(pot/eforce.f:1, pot/nbondm.f:1, symm/syminit.f:1, symm/squeeze.f:1,
+symm/ewald_init.f:8); here the COMMON content is the
+:class:`~moil_fnrfd_amd.systems.Topology` plus the run parameters, and
+coordinates / gradients are MOIL-layout ``(npt, 3)`` float64 arrays
+(``coor(3,npt)``, ``dpot(3,npt)``; dpot is the gradient dE/dx).
+
+Errors: MOIL prints and STOPs (intrprtr/alert.f:10-24); this binding
+raises :class:`MoilGPUError` carrying the library's message.
+
+The HIP library must be built (``__graft_entry__.build()``); there is no
+CPU fallback -- importing without the library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libmoil_gpu.so')
+_lib = None
+
+
+class MoilGPUError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libmoil_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MoilGPUError('libmoil_gpu.so not found at %s: run __graft_entry__.build()' % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.moil_gpu_abi_version.restype = C.c_int32
+    L.moil_gpu_last_error.restype = C.c_char_p
+    L.moil_gpu_init.argtypes = [C.POINTER(abi.MoilTopology), C.POINTER(abi.MoilParams), C.POINTER(vp)]
+    L.moil_gpu_finalize.argtypes = [vp]
+    L.moil_gpu_ewald_info.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.moil_gpu_erf_table.argtypes = [vp, vp, C.c_int32]
+    L.moil_gpu_squeeze.argtypes = [vp, vp]
+    L.moil_gpu_build_lists.argtypes = [vp, vp]
+    L.moil_gpu_eforce.argtypes = [vp, vp, vp, C.POINTER(abi.MoilEnergies)]
+    L.moil_gpu_squeeze_dev.argtypes = [vp, vp, vp]
+    L.moil_gpu_build_lists_dev.argtypes = [vp, vp, vp]
+    L.moil_gpu_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
+    L.moil_gpu_list_counts.argtypes = [vp, vp]
+    L.moil_gpu_dump_water_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.moil_gpu_dump_atom_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.moil_gpu_set_timing.argtypes = [vp, C.c_int32]
+    L.moil_gpu_stage_times.argtypes = [vp, vp]
+    for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
+                 'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
+                 'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
+                 'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
+                 'moil_gpu_set_timing', 'moil_gpu_stage_times'):
+        getattr(L, name).restype = C.c_int32
+    if L.moil_gpu_abi_version() != abi.ABI_VERSION:
+        raise MoilGPUError('ABI version mismatch')
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != abi.OK:
+        msg = load_library().moil_gpu_last_error().decode(errors='replace')
+        raise MoilGPUError('moil_gpu error %d: %s' % (rc, msg))
+
+
+def _f64(x, npt):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if a.size != 3 * npt:
+        raise ValueError('expected %d coordinates, got %d' % (3 * npt, a.size))
+    return a
+
+
+class MoilGPU:
+    """One MOIL system on the GPU (the state MOIL keeps in COMMON)."""
+
+    def __init__(self, topo, params: abi.MoilParams):
+        L = load_library()
+        self._ts, self._keep = abi.make_topology(topo)
+        self.params = params
+        self.npt = topo.npt
+        h = C.c_void_p()
+        _check(L.moil_gpu_init(C.byref(self._ts), C.byref(params), C.byref(h)))
+        self._h = h
+
+    # -- MOIL call surface -------------------------------------------------
+    def ewald_info(self):
+        b, es, ed = C.c_double(), C.c_double(), C.c_double()
+        nt = C.c_int32()
+        _check(load_library().moil_gpu_ewald_info(self._h, C.byref(b), C.byref(es), C.byref(nt),
+                                                  C.byref(ed)))
+        tab = np.zeros((nt.value, 4))
+        _check(load_library().moil_gpu_erf_table(self._h, tab.ctypes.data, nt.value))
+        return dict(beta=b.value, e_self=es.value, ntab=nt.value, erftbdns=ed.value, erf_arr=tab)
+
+    def squeeze(self, coor):
+        x = _f64(coor, self.npt).copy()
+        _check(load_library().moil_gpu_squeeze(self._h, x.ctypes.data))
+        return x.reshape(self.npt, 3)
+
+    def nbondm(self, coor):
+        """Builds MOIL's real and image pair lists (nbondm + syminit)."""
+        x = _f64(coor, self.npt)
+        _check(load_library().moil_gpu_build_lists(self._h, x.ctypes.data))
+
+    def eforce(self, coor):
+        """Returns (energies dict, dpot (npt,3) float64 gradient)."""
+        x = _f64(coor, self.npt)
+        dpot = np.zeros((self.npt, 3))
+        en = abi.MoilEnergies()
+        _check(load_library().moil_gpu_eforce(self._h, x.ctypes.data, dpot.ctypes.data, C.byref(en)))
+        return en.as_dict(), dpot
+
+    # -- device-resident variants (pointers are device addresses) ----------
+    def squeeze_dev(self, d_coor: int, stream: Optional[int] = None):
+        _check(load_library().moil_gpu_squeeze_dev(self._h, C.c_void_p(d_coor), C.c_void_p(stream or 0)))
+
+    def nbondm_dev(self, d_coor: int, stream: Optional[int] = None):
+        _check(load_library().moil_gpu_build_lists_dev(self._h, C.c_void_p(d_coor),
+                                                        C.c_void_p(stream or 0)))
+
+    def eforce_dev(self, d_coor: int, d_dpot: int, d_energies: int, stream: Optional[int] = None):
+        _check(load_library().moil_gpu_eforce_dev(self._h, C.c_void_p(d_coor), C.c_void_p(d_dpot),
+                                                  C.c_void_p(d_energies), C.c_void_p(stream or 0)))
+
+    # -- inspection -------------------------------------------------------
+    def list_counts(self):
+        c = np.zeros(8, np.int64)
+        _check(load_library().moil_gpu_list_counts(self._h, c.ctypes.data))
+        return c
+
+    def water_pairs(self):
+        n = C.c_int64()
+        _check(load_library().moil_gpu_dump_water_pairs(self._h, None, 0, C.byref(n)))
+        rows = np.zeros((max(n.value, 1), 4), np.int32)
+        _check(load_library().moil_gpu_dump_water_pairs(self._h, rows.ctypes.data, n.value, C.byref(n)))
+        return rows[:n.value]
+
+    def atom_pairs(self):
+        n = C.c_int64()
+        _check(load_library().moil_gpu_dump_atom_pairs(self._h, None, 0, C.byref(n)))
+        rows = np.zeros((max(n.value, 1), 4), np.int32)
+        _check(load_library().moil_gpu_dump_atom_pairs(self._h, rows.ctypes.data, n.value, C.byref(n)))
+        return rows[:n.value]
+
+    def set_timing(self, on=True):
+        _check(load_library().moil_gpu_set_timing(self._h, 1 if on else 0))
+
+    def stage_times(self):
+        t = np.zeros(8)
+        _check(load_library().moil_gpu_stage_times(self._h, t.ctypes.data))
+        return t
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load_library().moil_gpu_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,115 @@
+"""Writes the golden fixtures under tests/golden/ by running MOIL's own
+Fortran energy path (oracle/_ref/moil_dump, built from the reference
+sources by oracle/ref/Makefile) on the synthetic parity systems.
+
+Each fixture holds the inputs (coordinates; the topology is regenerated
+from its seed and checked against a stored checksum) and MOIL's outputs:
+beta, all energy terms, the gradient dpot (full, or a seeded atom subset
+for the 98k-atom box), the canonical water pair set (full rows for the
+small box, counts + SHA-256 of the sorted rows otherwise), the erfc
+spline table and B-spline moduli.
+
+Run from the repository root:  python tests/golden/gen_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from moil_fnrfd_amd import configs  # noqa: E402
+from oracle.oracle import ref_canonical_water_pairs, sort_rows  # noqa: E402
+from oracle.ref_runner import run_moil  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+ENERGY_FIELDS = ['ewaldcof', 'e_self', 'e_corr', 'e_ew_receip', 'e_vdw', 'e_el', 'e_vsym',
+                 'e_lsym', 'e_bond', 'e_theta', 'e_tors', 'e_imp', 'e_el14', 'e_vdw14',
+                 'e_total', 'e_dir', 'erftbdns', 'fpme']
+
+
+def topo_checksum(t) -> str:
+    h = hashlib.sha256()
+    for a in (t.poipt, t.ptchg, t.epsgm6, t.epsgm12, t.exc1, t.exc2, t.ib1, t.ib2, t.kbond,
+              t.req, t.iangl1, t.iangl2, t.iangl3, t.kangl, t.angleq, t.itor1, t.kimp, t.p14,
+              t.idxtip3):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(','.join(t.moname).encode())
+    return h.hexdigest()
+
+
+def rows_sha(rows) -> str:
+    return hashlib.sha256(np.ascontiguousarray(sort_rows(rows), np.int32).tobytes()).hexdigest()
+
+
+def shift_out(s, n_out=12, seed=77):
+    """Translate n_out waters by a full box vector (same periodic system,
+    but the molecules now sit outside the primary box, unwrapped)."""
+    rng = np.random.default_rng(seed)
+    L = s.box[0]
+    x = s.coor.copy()
+    for w in rng.choice(len(s.topo.idxtip3), n_out, replace=False):
+        ax = rng.integers(0, 3)
+        sign = rng.choice([-1.0, 1.0])
+        x[3 * w:3 * w + 3, ax] += sign * L
+    s.coor = x
+    return s
+
+
+def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None):
+    d = run_moil(s, cut, ew, squeeze=squeeze)
+    rows = sort_rows(ref_canonical_water_pairs(d))
+    out = dict(
+        coor_sha=np.frombuffer(hashlib.sha256(np.ascontiguousarray(s.coor, np.float64).tobytes())
+                               .hexdigest().encode(), np.uint8),
+        energies=d['ENERGIES'],
+        counts=d['COUNTS'],
+        pairs_sha=np.frombuffer(rows_sha(rows).encode(), np.uint8),
+        n_pairs=np.array([len(rows), int((rows[:, 3] == 1).sum()), int((rows[:, 3] == 2).sum()),
+                          int((rows[:, 2] > 0).sum())], np.int64),
+        topo_sha=np.frombuffer(topo_checksum(s.topo).encode(), np.uint8),
+        erftab_head=d['ERFTAB'].reshape(-1, 4)[:64],
+        erftab_rows=np.array([len(d['ERFTAB']) // 4], np.int64),
+        bspmod1=d['BSPMOD1'],
+        box=np.array(s.box, np.float64),
+        squeeze=np.array([int(squeeze)]),
+    )
+    dp = d['DPOT'].reshape(-1, 3)
+    if dpot_subset is None:
+        out['dpot'] = dp
+    else:
+        rng = np.random.default_rng(dpot_subset)
+        idx = np.sort(rng.choice(len(dp), 1000, replace=False))
+        out['dpot_idx'] = idx
+        out['dpot'] = dp[idx]
+        out['dpot_sumsq'] = np.array([(dp ** 2).sum()])
+    if full_rows:
+        out['pairs'] = rows
+        out['coor'] = s.coor.astype(np.float64)
+        out['coor_after'] = d['COOR'].reshape(-1, 3)
+    path = os.path.join(OUT, name + '.npz')
+    np.savez_compressed(path, **out)
+    print('%-22s npt=%6d pairs=%8d e_total=%.10f -> %s (%d bytes)'
+          % (name, s.topo.npt, len(rows), d['ENERGIES'][14], os.path.basename(path),
+             os.path.getsize(path)))
+
+
+def main():
+    s, cut, ew = configs.water216()
+    write_case('water216', s, cut, ew)
+    s, cut, ew = configs.water216()
+    write_case('water216_out', shift_out(s), cut, ew)
+    s, cut, ew = configs.water216()
+    write_case('water216_out_sqz', shift_out(s), cut, ew, squeeze=True)
+    s, cut, ew = configs.water_buffered()
+    write_case('water1000_buffered', s, cut, ew, full_rows=False)
+    s, cut, ew = configs.water_scaled(10, seed=1100)
+    write_case('water1000_dense', s, cut, ew, full_rows=False)
+    s, cut, ew = configs.water32k()
+    write_case('water32k', s, cut, ew, full_rows=False, dpot_subset=4242)
+
+
+if __name__ == '__main__':
+    main()

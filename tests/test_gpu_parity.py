@@ -1,0 +1,147 @@
+"""GPU parity: the HIP library against MOIL's own outputs (golden
+fixtures written by oracle/_ref) and against the CPU oracle.
+
+Bars (per the energy-path contract):
+  * pair sets: bit-exact (canonical (i, j, symop, list) rows);
+  * FP64 mode: energies <= 1e-12 relative (e_total relative to the sum
+    of the magnitudes of its terms), forces <= 1e-10 RMS-relative;
+  * mixed mode (FP32 pairs, fixed-point forces): e_total <= 1e-6
+    relative, forces <= 1e-5 RMS-relative against the FP64 run.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from moil_fnrfd_amd import abi, configs
+from tests._util import (energy_errors, golden_energies, load_golden, rms_rel, rows_sha,
+                         sort_rows)
+from tests.golden.gen_golden import shift_out
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope='module')
+def gpu():
+    import torch
+    assert torch.cuda.is_available()
+    from moil_fnrfd_amd.eforce import MoilGPU, load_library
+    load_library()
+    return MoilGPU
+
+
+SMALL = {
+    'water216': (configs.water216, None, False),
+    'water216_out': (configs.water216, shift_out, False),
+    'water216_out_sqz': (configs.water216, shift_out, True),
+    'water1000_buffered': (configs.water_buffered, None, False),
+    'water1000_dense': (lambda: configs.water_scaled(10, seed=1100), None, False),
+}
+
+
+@pytest.mark.parametrize('name', sorted(SMALL))
+def test_fp64_matches_moil(gpu, name):
+    _say('case %s FP64' % name)
+    mk, mod, sqz = SMALL[name]
+    s, cut, ew = mk()
+    if mod:
+        s = mod(s)
+    g = load_golden(name)
+    m = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+    x = m.squeeze(s.coor) if sqz else s.coor
+    if 'coor_after' in g:
+        assert np.array_equal(x, g['coor_after'])
+    m.nbondm(x)
+    rows = sort_rows(m.water_pairs())
+    if 'pairs' in g:
+        assert np.array_equal(rows, g['pairs'])
+    assert rows_sha(rows) == bytes(g['pairs_sha']).decode()
+    e, dp = m.eforce(x)
+    ref = golden_energies(g)
+    assert m.ewald_info()['beta'] == ref['ewaldcof']
+    err = energy_errors(e, ref)
+    assert max(err.values()) < 1e-12, err
+    assert rms_rel(dp, g['dpot']) < 1e-10
+    m.close()
+
+
+def test_fp64_matches_moil_32k(gpu):
+    _say('case water32k FP64')
+    s, cut, ew = configs.water32k()
+    g = load_golden('water32k')
+    m = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+    m.nbondm(s.coor)
+    rows = m.water_pairs()
+    assert len(rows) == int(g['n_pairs'][0])
+    assert rows_sha(rows) == bytes(g['pairs_sha']).decode()
+    e, dp = m.eforce(s.coor)
+    err = energy_errors(e, golden_energies(g))
+    assert max(err.values()) < 1e-12, err
+    assert rms_rel(dp[g['dpot_idx']], g['dpot']) < 1e-10
+    m.close()
+
+
+def test_mixed_precision_32k(gpu):
+    """FP32 pair math with fixed-point forces vs the FP64 path, and
+    bit-reproducibility run to run."""
+    _say('case water32k MIXED vs FP64')
+    s, cut, ew = configs.water32k()
+    m64 = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+    m64.nbondm(s.coor)
+    e64, f64 = m64.eforce(s.coor)
+    m64.close()
+    m32 = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+    m32.nbondm(s.coor)
+    e32, f32 = m32.eforce(s.coor)
+    e32b, f32b = m32.eforce(s.coor)
+    m32.close()
+    rel_e = abs(e32['e_total'] - e64['e_total']) / abs(e64['e_total'])
+    rel_f = rms_rel(f32, f64)
+    _say('  e_total rel %.3e  force rms-rel %.3e' % (rel_e, rel_f))
+    assert rel_e < 1e-6
+    assert rel_f < 1e-5
+    assert np.array_equal(f32, f32b) and e32 == e32b
+
+
+def test_lists_follow_perturbed_coordinates(gpu):
+    """List rebuilds on perturbed / drifting coordinates (incl. squeeze)
+    reproduce the oracle's pair sets exactly."""
+    from oracle.oracle import Oracle
+    s, cut, ew = configs.water_scaled(8, seed=1200)
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_MIXED)
+    m = gpu(s.topo, p)
+    o = Oracle(s.topo, p)
+    rng = np.random.default_rng(9)
+    x = s.coor.copy()
+    for step in range(4):
+        _say('case water_scaled8 perturbation %d' % step)
+        x = x + rng.normal(0.0, 0.3, x.shape)
+        xg = m.squeeze(x)
+        xo = o.squeeze(x)
+        assert np.array_equal(xg, xo)
+        m.nbondm(xg)
+        o.build_lists(xo)
+        assert np.array_equal(sort_rows(m.water_pairs()), sort_rows(o.water_pairs()))
+        x = xg
+    m.close()
+
+
+def test_energy_conservation_properties_32k(gpu):
+    """Size-independent properties at full size: the real-space plus
+    bonded force (everything except PME) sums to zero; total force sum
+    is ~0 (PME conserves momentum only approximately)."""
+    _say('case water32k momentum')
+    s, cut, ew = configs.water32k()
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_FP64,
+                           terms=abi.TERM_DEFAULT)
+    m = gpu(s.topo, p)
+    m.nbondm(s.coor)
+    e, dp = m.eforce(s.coor)
+    net = dp.sum(0)
+    assert np.all(np.abs(net) < 1e-3 * np.sqrt((dp ** 2).sum(1)).max())
+    m.close()
