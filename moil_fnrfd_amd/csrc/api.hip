@@ -115,6 +115,9 @@ static void ctx_free(moil_ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->stream2) hipStreamDestroy(c->stream2);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -137,11 +140,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   if (p->precision != MOIL_PREC_FP64 && p->precision != MOIL_PREC_MIXED)
     return fail(MOIL_GPU_EINVAL, "unknown precision mode");
   if (p->cutele2 <= 0 || p->eps <= 0 || p->dtol <= 0) return fail(MOIL_GPU_EINVAL, "bad cutoffs/eps/dtol");
-  // half-shell images double count when the list reach exceeds half a box
-  // edge (symm/syminit.f:20-21)
-  double reach = std::sqrt(std::max(p->cutebig2, p->cutvbig2));
-  if (2.0 * reach > std::min(p->a, std::min(p->b, p->c)))
-    return fail(MOIL_GPU_EINVAL, "list cutoff exceeds half the box edge");
+  // A list reach above half a box edge makes MOIL's half-shell images
+  // double count pairs (symm/syminit.f:20-21); MOIL does not refuse it, and
+  // neither does this library: the pair set is whatever MOIL's predicates
+  // give.
   double qtot = 0;
   for (int i = 0; i < t->npt; i++) qtot += t->ptchg[i];
   if (std::fabs(qtot) > 1e-7)  // symm/ewald_init.f:62-72
@@ -168,9 +170,12 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       return rc;               \
     }                          \
   } while (0)
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     ctx_free(c);
-    return fail(MOIL_GPU_EHIP, "hipStreamCreate failed");
+    return fail(MOIL_GPU_EHIP, "hipStreamCreate/hipEventCreate failed");
   }
   // ---- Ewald set-up ----
   double cutoff = std::sqrt(p->cutele2);
@@ -288,6 +293,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     return MOIL_GPU_OK;
   };
   TRY(mal((void **)&c->d_facc, sizeof(long long) * 3 * c->npt));
+  if (hipMemset(c->d_facc, 0, sizeof(long long) * 3 * c->npt) != hipSuccess) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EHIP, "hipMemset failed");
+  }
   TRY(mal((void **)&c->d_energy, sizeof(double) * 32));
   TRY(mal((void **)&c->d_coor, sizeof(double) * 3 * c->npt));
   TRY(mal((void **)&c->d_dpot, sizeof(double) * 3 * c->npt));
@@ -403,50 +412,42 @@ extern "C" int32_t moil_gpu_build_lists(moil_ctx *c, const double *coor) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_zero_i64(long long *__restrict__ p, size_t n) {
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) p[g] = 0;
-}
-__global__ void k_zero_f64(double *__restrict__ p, size_t n) {
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) p[g] = 0.0;
-}
 
-__global__ void k_finalize_forces(const long long *__restrict__ facc, const double *__restrict__ fpme,
+// dpot = fixed-point accumulator + reciprocal gradient; the accumulator
+// is cleared for the next evaluation on the way
+__global__ void k_finalize_forces(long long *__restrict__ facc, const double *__restrict__ fpme,
                                   int has_pme, size_t n, double inv, double *__restrict__ dpot) {
   size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   double v = (double)facc[g] * inv;
   if (has_pme) v += fpme[g];
   dpot[g] = v;
+  facc[g] = 0;
 }
 
 struct EnCfg {
   int max_parts;
+  int nparts[EN_COUNT];
   double e_self;
   int has_bond, has_angle, has_tors, has_imp, has_nb, has_14;
 };
 
-// deterministic reduction of the per-workgroup partials and assembly of
+// deterministic reduction of the per-workgroup partials (one wavefront per
+// energy term, fixed strided order + fixed shuffle tree) and assembly of
 // the energy record in eforce's order (pot/eforce.f:100-205)
-__global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep, EnCfg C,
-                                                  double *__restrict__ out) {
-  __shared__ double red[256];
+__global__ __launch_bounds__(1024) void k_energies(const double *__restrict__ ep, EnCfg C,
+                                                   double *__restrict__ out) {
   __shared__ double tot[EN_COUNT];
-  int t = threadIdx.x;
-  for (int k = 0; k < EN_COUNT; k++) {
-    double s = 0;
-    for (int i = t; i < C.max_parts; i += 256) s += ep[(size_t)k * C.max_parts + i];
-    red[t] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (t < off) red[t] += red[t + off];
-      __syncthreads();
-    }
-    if (t == 0) tot[k] = red[0];
-    __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (w < EN_COUNT) {
+    double s = 0.0;
+    const double *row = ep + (size_t)w * C.max_parts;
+    for (int i = lane; i < C.nparts[w]; i += 64) s += row[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) tot[w] = s;
   }
-  if (t == 0) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
     moil_energies e;
     e.e_bond = tot[EN_BOND];
     e.e_theta = tot[EN_THETA];
@@ -505,6 +506,8 @@ struct StageTimer {
   }
 };
 
+int bonded_blocks(const moil_ctx *c);
+
 extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
                                        double *d_energies, void *stream) {
   if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
@@ -513,27 +516,44 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   const uint32_t T = c->par.terms;
   const bool nb = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
   size_t n3 = 3 * (size_t)c->npt;
+  if (c->facc_dirty) {
+    HIP_CHECK(hipMemsetAsync(c->d_facc, 0, sizeof(long long) * n3, st));
+    c->facc_dirty = false;
+  }
+  c->facc_dirty = true;  // until the finalize kernel is enqueued
   StageTimer whole(c, 4, st);
-  k_zero_i64<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, n3);
-  size_t ne = (size_t)EN_COUNT * c->max_parts;
-  k_zero_f64<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(c->d_epart, ne);
+  // reciprocal space on a second stream, overlapped with bonded + real space
+  if (nb) {
+    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    {
+      StageTimer tp(c, 2, c->stream2);
+      CHECK_RC(pme_run(c, d_coor, c->stream2));
+    }
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+  }
   {
     StageTimer tb(c, 3, st);
     CHECK_RC(bonded_run(c, d_coor, st));
-  }
-  if (nb) {
-    StageTimer tp(c, 2, st);
-    CHECK_RC(pme_run(c, d_coor, st));
   }
   if (nb) {
     CHECK_RC(realspace_water_pack(c, d_coor, st));
     StageTimer tw(c, 0, st);
     CHECK_RC(realspace_water(c, d_coor, st));
   }
+  if (nb) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
   k_finalize_forces<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, c->d_fpme, nb ? 1 : 0,
                                                                   n3, c->fix_inv, d_dpot);
+  c->facc_dirty = false;
   EnCfg C;
   C.max_parts = c->max_parts;
+  const int nbond = bonded_blocks(c);
+  for (int k = 0; k < EN_COUNT; k++) C.nparts[k] = 0;
+  C.nparts[EN_BOND] = C.nparts[EN_THETA] = C.nparts[EN_TORS] = C.nparts[EN_IMP] = nbond;
+  C.nparts[EN_CORR] = C.nparts[EN_VDW14] = C.nparts[EN_EL14] = nbond;
+  const int nw = (nb && c->nwat >= 2) ? c->nwchunk : 0;
+  C.nparts[EN_VDW] = C.nparts[EN_EL] = C.nparts[EN_VSYM] = C.nparts[EN_LSYM] = nw;
+  C.nparts[EN_RECIP] = nb ? pme_conv_blocks(c) : 0;
   C.e_self = c->e_self;
   C.has_bond = (T & MOIL_TERM_BOND) && c->nb > 0;
   C.has_angle = (T & MOIL_TERM_ANGLE) && c->nangl > 0;
@@ -541,7 +561,7 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   C.has_imp = (T & MOIL_TERM_IMP) && c->nimp > 0;
   C.has_nb = nb;
   C.has_14 = c->totspe > 0 && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14));
-  k_energies<<<1, 256, 0, st>>>(c->d_epart, C, d_energies ? d_energies : c->d_energy);
+  k_energies<<<1, 1024, 0, st>>>(c->d_epart, C, d_energies ? d_energies : c->d_energy);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

@@ -38,81 +38,7 @@ __device__ __forceinline__ void add3(long long *facc, int atom, double fx, doubl
   if (v) atomicAdd((unsigned long long *)&facc[3 * atom + 2], (unsigned long long)v);
 }
 
-__device__ inline void block_partial(double e, double *out) {
-  __shared__ double red[BT];
-  int t = threadIdx.x;
-  red[t] = e;
-  __syncthreads();
-  for (int off = BT / 2; off > 0; off >>= 1) {
-    if (t < off) red[t] += red[t + off];
-    __syncthreads();
-  }
-  if (t == 0) out[blockIdx.x] = red[0];
-}
-
 #define XYZ(a) coor[3 * (a)], coor[3 * (a) + 1], coor[3 * (a) + 2]
-
-__global__ __launch_bounds__(BT) void k_bonds(const double *__restrict__ coor, const int32_t *__restrict__ ib,
-                                              const double *__restrict__ par, int nb, BPar P,
-                                              long long *__restrict__ facc, double *__restrict__ ep) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e = 0;
-  if (m < nb) {
-    int i = ib[2 * m] - 1, j = ib[2 * m + 1] - 1;
-    double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
-           rz = coor[3 * i + 2] - coor[3 * j + 2];
-    double s = sqrt(rx * rx + ry * ry + rz * rz);
-    double db = s - par[2 * m + 1];
-    double df = par[2 * m] * db;
-    e = df * db;
-    df = df * (2.0 / s);
-    add3(facc, i, rx * df, ry * df, rz * df, P.fix_scale);
-    add3(facc, j, -rx * df, -ry * df, -rz * df, P.fix_scale);
-  }
-  block_partial(e, ep);
-}
-
-__global__ __launch_bounds__(BT) void k_angles(const double *__restrict__ coor, const int32_t *__restrict__ ia,
-                                               const double *__restrict__ par, int n, BPar P,
-                                               long long *__restrict__ facc, double *__restrict__ ep) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e = 0;
-  if (m < n) {
-    int i = ia[3 * m] - 1, j = ia[3 * m + 1] - 1, k = ia[3 * m + 2] - 1;
-    double di[3], dj[3];
-    for (int q = 0; q < 3; q++) {
-      di[q] = coor[3 * i + q] - coor[3 * j + q];
-      dj[q] = coor[3 * k + q] - coor[3 * j + q];
-    }
-    double ri = sqrt(di[0] * di[0] + di[1] * di[1] + di[2] * di[2]);
-    double rj = sqrt(dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2]);
-    double rir = 1.0 / ri, rjr = 1.0 / rj;
-    for (int q = 0; q < 3; q++) {
-      di[q] *= rir;
-      dj[q] *= rjr;
-    }
-    double cst = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
-    cst = fmin(1.0, fmax(-1.0, cst));
-    double at = acos(cst);
-    double da = at - par[2 * m + 1];
-    double df = par[2 * m] * da;
-    e = df * da;
-    df = df + df;
-    double s = sin(at);
-    if (s < 1e-6 && s > 0.0) s = 1e-6;
-    else if (s < 0.0 && s > -1e-6) s = -1e-6;
-    df = -df / s;
-    double t1[3], t2[3];
-    for (int q = 0; q < 3; q++) {
-      t1[q] = rir * (dj[q] - cst * di[q]);
-      t2[q] = rjr * (di[q] - cst * dj[q]);
-    }
-    add3(facc, i, df * t1[0], df * t1[1], df * t1[2], P.fix_scale);
-    add3(facc, j, -df * (t1[0] + t2[0]), -df * (t1[1] + t2[1]), -df * (t1[2] + t2[2]), P.fix_scale);
-    add3(facc, k, df * t2[0], df * t2[1], df * t2[2], P.fix_scale);
-  }
-  block_partial(e, ep);
-}
 
 // cos(phi) of the dihedral and its gradient (chain rule on
 // u=(a.a)(b.b)-(a.b)^2, v=(b.b)(c.c)-(b.c)^2, w=(a.b)(b.c)-(a.c)(b.b),
@@ -159,115 +85,198 @@ __device__ inline double dihedral(const double *xi, const double *xj, const doub
   return co;
 }
 
-__global__ __launch_bounds__(BT) void k_tors(const double *__restrict__ coor, const int32_t *__restrict__ it,
-                                             const double *__restrict__ par, int n, BPar P,
-                                             long long *__restrict__ facc, double *__restrict__ ep) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e = 0;
-  if (m < n) {
-    int id[4] = {it[4 * m] - 1, it[4 * m + 1] - 1, it[4 * m + 2] - 1, it[4 * m + 3] - 1};
-    double g[4][3];
-    double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, nullptr);
-    const double *p = par + 6 * m;  // k1 k2 k3 ph1 ph2 ph3
-    e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
-        p[5] * (p[2] * co * (4.0 * co * co - 3.0));
-    double df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
-    for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
-  }
-  block_partial(e, ep);
+struct BSeg {
+  int nb, nangl, ntors, nimp, nexcl, n14;
+  const int32_t *ib, *ia, *it, *ii, *ei, *ej, *sp;
+  const double *bp, *ap, *tp, *ip, *p14, *q, *erf;
+};
+
+__device__ inline void term_bond(const double *coor, const BSeg &S, int m, const BPar &P,
+                                 long long *facc, double &e) {
+  int i = S.ib[2 * m] - 1, j = S.ib[2 * m + 1] - 1;
+  double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
+         rz = coor[3 * i + 2] - coor[3 * j + 2];
+  double s = sqrt(rx * rx + ry * ry + rz * rz);
+  double db = s - S.bp[2 * m + 1];
+  double df = S.bp[2 * m] * db;
+  e = df * db;
+  df = df * (2.0 / s);
+  add3(facc, i, rx * df, ry * df, rz * df, P.fix_scale);
+  add3(facc, j, -rx * df, -ry * df, -rz * df, P.fix_scale);
 }
 
-__global__ __launch_bounds__(BT) void k_imps(const double *__restrict__ coor, const int32_t *__restrict__ ii,
-                                             const double *__restrict__ par, int n, BPar P,
-                                             long long *__restrict__ facc, double *__restrict__ ep) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e = 0;
-  if (m < n) {
-    const double pi = 3.14159265358979323846;
-    int id[4] = {ii[4 * m] - 1, ii[4 * m + 1] - 1, ii[4 * m + 2] - 1, ii[4 * m + 3] - 1};
-    double g[4][3], sgn;
-    double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, &sgn);
-    co = fmin(1.0, fmax(-1.0, co));
-    double k = par[2 * m], eq = par[2 * m + 1], df;
-    if (fabs(eq) < 1e-6) {
-      df = -2.0 * k;
-      e = -df * (1.0 - co);
-    } else if (fabs(fabs(eq) - pi) < 1e-6) {
-      df = 2.0 * k;
-      e = df * (1.0 + co);
-    } else {
-      double phi = acos(co);
-      if (sgn < 0) phi = -phi;
-      double delta = phi - eq;
-      if (delta > pi) delta -= 2.0 * pi;
-      if (delta < -pi) delta += 2.0 * pi;
-      df = k * delta;
-      e = df * delta;
-      double yy = sin(phi);
-      if (fabs(yy) > 1e-6) df = -2.0 * df / yy;
-      else if ((phi > 0.0 && phi < pi / 2.0) || (phi < 0.0 && phi > -pi / 2.0)) df = df * 1e6;
-      else df = -df * 1e6;
-    }
-    for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
+__device__ inline void term_angle(const double *coor, const BSeg &S, int m, const BPar &P,
+                                  long long *facc, double &e) {
+  int i = S.ia[3 * m] - 1, j = S.ia[3 * m + 1] - 1, k = S.ia[3 * m + 2] - 1;
+  double di[3], dj[3];
+  for (int q = 0; q < 3; q++) {
+    di[q] = coor[3 * i + q] - coor[3 * j + q];
+    dj[q] = coor[3 * k + q] - coor[3 * j + q];
   }
-  block_partial(e, ep);
+  double ri = sqrt(di[0] * di[0] + di[1] * di[1] + di[2] * di[2]);
+  double rj = sqrt(dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2]);
+  double rir = 1.0 / ri, rjr = 1.0 / rj;
+  for (int q = 0; q < 3; q++) {
+    di[q] *= rir;
+    dj[q] *= rjr;
+  }
+  double cst = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
+  cst = fmin(1.0, fmax(-1.0, cst));
+  double at = acos(cst);
+  double da = at - S.ap[2 * m + 1];
+  double df = S.ap[2 * m] * da;
+  e = df * da;
+  df = df + df;
+  double sn = sin(at);
+  if (sn < 1e-6 && sn > 0.0) sn = 1e-6;
+  else if (sn < 0.0 && sn > -1e-6) sn = -1e-6;
+  df = -df / sn;
+  double t1[3], t2[3];
+  for (int q = 0; q < 3; q++) {
+    t1[q] = rir * (dj[q] - cst * di[q]);
+    t2[q] = rjr * (di[q] - cst * dj[q]);
+  }
+  add3(facc, i, df * t1[0], df * t1[1], df * t1[2], P.fix_scale);
+  add3(facc, j, -df * (t1[0] + t2[0]), -df * (t1[1] + t2[1]), -df * (t1[2] + t2[2]), P.fix_scale);
+  add3(facc, k, df * t2[0], df * t2[1], df * t2[2], P.fix_scale);
 }
 
-__global__ __launch_bounds__(BT) void k_e14(const double *__restrict__ coor, const int32_t *__restrict__ sp,
-                                            const double *__restrict__ p14, int n, BPar P,
-                                            long long *__restrict__ facc, double *__restrict__ epv,
-                                            double *__restrict__ epe) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e1 = 0, e2 = 0;
-  if (m < n) {
-    int j = sp[2 * m] - 1, k = sp[2 * m + 1] - 1;
-    double rx = coor[3 * j] - coor[3 * k], ry = coor[3 * j + 1] - coor[3 * k + 1],
-           rz = coor[3 * j + 2] - coor[3 * k + 2];
-    double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, df1 = 0, df2 = 0;
-    const double *p = p14 + 3 * m;
-    if (P.do_v14 && p[0] > 1e-12) {
-      double s6 = s2 * s2 * s2, A = p[0] * s6 * s6 * P.pick, B = p[1] * s6 * P.pick;
-      e1 = A - B;
-      df1 = -6.0 * s2 * (A + e1);
-    }
-    if (P.do_el14 && fabs(p[2]) > 1e-12) {
-      double s = sqrt(s2);
-      e2 = p[2] * s * P.epstmp14;
-      df2 = -e2 * s2;
-    }
-    double df = df1 + df2;
-    add3(facc, j, df * rx, df * ry, df * rz, P.fix_scale);
-    add3(facc, k, -df * rx, -df * ry, -df * rz, P.fix_scale);
-  }
-  block_partial(e1, epv);
-  block_partial(e2, epe);
+__device__ inline void term_tors(const double *coor, const BSeg &S, int m, const BPar &P,
+                                 long long *facc, double &e) {
+  int id[4] = {S.it[4 * m] - 1, S.it[4 * m + 1] - 1, S.it[4 * m + 2] - 1, S.it[4 * m + 3] - 1};
+  double g[4][3];
+  double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, nullptr);
+  const double *p = S.tp + 6 * m;  // k1 k2 k3 ph1 ph2 ph3
+  e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
+      p[5] * (p[2] * co * (4.0 * co * co - 3.0));
+  double df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
+  for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
 }
 
-__global__ __launch_bounds__(BT) void k_excl(const double *__restrict__ coor, const int32_t *__restrict__ ei,
-                                             const int32_t *__restrict__ ej, const double *__restrict__ q,
-                                             int n, BPar P, const double *__restrict__ erf_arr,
-                                             long long *__restrict__ facc, double *__restrict__ ep) {
-  int m = blockIdx.x * BT + threadIdx.x;
-  double e = 0;
-  if (m < n) {
-    int i = ei[m], j = ej[m];
-    double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
-           rz = coor[3 * i + 2] - coor[3 * j + 2];
-    double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, rij = sqrt(r2), s = 1.0 / rij;
-    double qq = q[i] * P.qfac * q[j];
-    double xerfc = P.beta * rij;
-    int idx = (int)(P.erftbdns * xerfc);
-    double dx = xerfc - idx * (1.0 / P.erftbdns);
-    const double *t = erf_arr + 4 * idx;
-    double derfc = -t[1] - dx * (t[2] + 0.5 * dx * t[3]);
-    double erfcc = dx * (t[1] + 0.5 * dx * (t[2] + dx * t[3] * (1.0 / 3.0))) + t[0];
-    double erf = 1.0 - erfcc;
-    e = -qq * s * erf;
-    double df = qq * s * (s2 * erf - s * P.beta * derfc);
-    add3(facc, i, df * rx, df * ry, df * rz, P.fix_scale);
-    add3(facc, j, -df * rx, -df * ry, -df * rz, P.fix_scale);
+__device__ inline void term_imp(const double *coor, const BSeg &S, int m, const BPar &P,
+                                long long *facc, double &e) {
+  const double pi = 3.14159265358979323846;
+  int id[4] = {S.ii[4 * m] - 1, S.ii[4 * m + 1] - 1, S.ii[4 * m + 2] - 1, S.ii[4 * m + 3] - 1};
+  double g[4][3], sgn;
+  double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, &sgn);
+  co = fmin(1.0, fmax(-1.0, co));
+  double k = S.ip[2 * m], eq = S.ip[2 * m + 1], df;
+  if (fabs(eq) < 1e-6) {
+    df = -2.0 * k;
+    e = -df * (1.0 - co);
+  } else if (fabs(fabs(eq) - pi) < 1e-6) {
+    df = 2.0 * k;
+    e = df * (1.0 + co);
+  } else {
+    double phi = acos(co);
+    if (sgn < 0) phi = -phi;
+    double delta = phi - eq;
+    if (delta > pi) delta -= 2.0 * pi;
+    if (delta < -pi) delta += 2.0 * pi;
+    df = k * delta;
+    e = df * delta;
+    double yy = sin(phi);
+    if (fabs(yy) > 1e-6) df = -2.0 * df / yy;
+    else if ((phi > 0.0 && phi < pi / 2.0) || (phi < 0.0 && phi > -pi / 2.0)) df = df * 1e6;
+    else df = -df * 1e6;
   }
-  block_partial(e, ep);
+  for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
+}
+
+__device__ inline void term_excl(const double *coor, const BSeg &S, int m, const BPar &P,
+                                 long long *facc, double &e) {
+  int i = S.ei[m], j = S.ej[m];
+  double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
+         rz = coor[3 * i + 2] - coor[3 * j + 2];
+  double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, rij = sqrt(r2), s = 1.0 / rij;
+  double qq = S.q[i] * P.qfac * S.q[j];
+  double xerfc = P.beta * rij;
+  int idx = (int)(P.erftbdns * xerfc);
+  double dx = xerfc - idx * (1.0 / P.erftbdns);
+  const double *t = S.erf + 4 * idx;
+  double derfc = -t[1] - dx * (t[2] + 0.5 * dx * t[3]);
+  double erfcc = dx * (t[1] + 0.5 * dx * (t[2] + dx * t[3] * (1.0 / 3.0))) + t[0];
+  double erf = 1.0 - erfcc;
+  e = -qq * s * erf;
+  double df = qq * s * (s2 * erf - s * P.beta * derfc);
+  add3(facc, i, df * rx, df * ry, df * rz, P.fix_scale);
+  add3(facc, j, -df * rx, -df * ry, -df * rz, P.fix_scale);
+}
+
+__device__ inline void term_14(const double *coor, const BSeg &S, int m, const BPar &P,
+                               long long *facc, double &e1, double &e2) {
+  int j = S.sp[2 * m] - 1, k = S.sp[2 * m + 1] - 1;
+  double rx = coor[3 * j] - coor[3 * k], ry = coor[3 * j + 1] - coor[3 * k + 1],
+         rz = coor[3 * j + 2] - coor[3 * k + 2];
+  double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, df1 = 0, df2 = 0;
+  const double *p = S.p14 + 3 * m;
+  if (P.do_v14 && p[0] > 1e-12) {
+    double s6 = s2 * s2 * s2, A = p[0] * s6 * s6 * P.pick, B = p[1] * s6 * P.pick;
+    e1 = A - B;
+    df1 = -6.0 * s2 * (A + e1);
+  }
+  if (P.do_el14 && fabs(p[2]) > 1e-12) {
+    double s = sqrt(s2);
+    e2 = p[2] * s * P.epstmp14;
+    df2 = -e2 * s2;
+  }
+  double df = df1 + df2;
+  add3(facc, j, df * rx, df * ry, df * rz, P.fix_scale);
+  add3(facc, k, -df * rx, -df * ry, -df * rz, P.fix_scale);
+}
+
+// all O(N) terms in one launch over a flat index space
+// [bonds | angles | torsions | impropers | exclusions | 1-4 pairs]
+__global__ __launch_bounds__(BT) void k_bonded(const double *__restrict__ coor, BSeg S, BPar P,
+                                               long long *__restrict__ facc,
+                                               double *__restrict__ ep, int mp) {
+  __shared__ double red[BT];
+  int m = blockIdx.x * BT + threadIdx.x;
+  double e[7] = {0, 0, 0, 0, 0, 0, 0};  // bond theta tors imp corr vdw14 el14
+  int o1 = S.nb, o2 = o1 + S.nangl, o3 = o2 + S.ntors, o4 = o3 + S.nimp, o5 = o4 + S.nexcl,
+      o6 = o5 + S.n14;
+  if (m < o1) term_bond(coor, S, m, P, facc, e[0]);
+  else if (m < o2) term_angle(coor, S, m - o1, P, facc, e[1]);
+  else if (m < o3) term_tors(coor, S, m - o2, P, facc, e[2]);
+  else if (m < o4) term_imp(coor, S, m - o3, P, facc, e[3]);
+  else if (m < o5) term_excl(coor, S, m - o4, P, facc, e[4]);
+  else if (m < o6) term_14(coor, S, m - o5, P, facc, e[5], e[6]);
+  const int slot[7] = {EN_BOND, EN_THETA, EN_TORS, EN_IMP, EN_CORR, EN_VDW14, EN_EL14};
+  int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    red[t] = e[q];
+    __syncthreads();
+    for (int off = BT / 2; off > 0; off >>= 1) {
+      if (t < off) red[t] += red[t + off];
+      __syncthreads();
+    }
+    if (t == 0) ep[(size_t)slot[q] * mp + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+static BSeg make_seg(const moil_ctx *c) {
+  const uint32_t T = c->par.terms;
+  BSeg S;
+  const bool nbon = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
+  S.nb = (T & MOIL_TERM_BOND) ? c->nb : 0;
+  S.nangl = (T & MOIL_TERM_ANGLE) ? c->nangl : 0;
+  S.ntors = (T & MOIL_TERM_TORS) ? c->ntors : 0;
+  S.nimp = (T & MOIL_TERM_IMP) ? c->nimp : 0;
+  S.nexcl = nbon ? c->nexcl : 0;
+  S.n14 = (nbon && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14))) ? c->totspe : 0;
+  S.ib = c->d_ib; S.ia = c->d_ia; S.it = c->d_it; S.ii = c->d_ii;
+  S.ei = c->d_exc_i; S.ej = c->d_exc_j; S.sp = c->d_sp;
+  S.bp = c->d_bpar; S.ap = c->d_apar; S.tp = c->d_tpar; S.ip = c->d_ipar;
+  S.p14 = c->d_p14; S.q = c->d_ptchg; S.erf = c->d_erf;
+  return S;
+}
+
+int bonded_blocks(const moil_ctx *c) {
+  BSeg S = make_seg(c);
+  int n = S.nb + S.nangl + S.ntors + S.nimp + S.nexcl + S.n14;
+  return (n + BT - 1) / BT;
 }
 
 int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
@@ -282,37 +291,12 @@ int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.do_el14 = (T & MOIL_TERM_EL14) != 0;
   P.do_v14 = (T & MOIL_TERM_VDW14) != 0;
   P.max_parts = c->max_parts;
-  double *ep = c->d_epart;
-  const size_t MP = c->max_parts;
-  if ((T & MOIL_TERM_BOND) && c->nb > 0)
-    k_bonds<<<(c->nb + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_ib, c->d_bpar, c->nb, P, c->d_facc,
-                                                  ep + EN_BOND * MP);
-  if ((T & MOIL_TERM_ANGLE) && c->nangl > 0)
-    k_angles<<<(c->nangl + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_ia, c->d_apar, c->nangl, P,
-                                                      c->d_facc, ep + EN_THETA * MP);
-  if ((T & MOIL_TERM_TORS) && c->ntors > 0)
-    k_tors<<<(c->ntors + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_it, c->d_tpar, c->ntors, P,
-                                                    c->d_facc, ep + EN_TORS * MP);
-  if ((T & MOIL_TERM_IMP) && c->nimp > 0)
-    k_imps<<<(c->nimp + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_ii, c->d_ipar, c->nimp, P,
-                                                   c->d_facc, ep + EN_IMP * MP);
-  if ((T & (MOIL_TERM_VDW | MOIL_TERM_EL))) {
-    if (c->nexcl > 0)
-      k_excl<<<(c->nexcl + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_exc_i, c->d_exc_j, c->d_ptchg,
-                                                      c->nexcl, P, c->d_erf, c->d_facc,
-                                                      ep + EN_CORR * MP);
-    if (c->totspe > 0 && (P.do_el14 || P.do_v14))
-      k_e14<<<(c->totspe + BT - 1) / BT, BT, 0, st>>>(d_coor, c->d_sp, c->d_p14, c->totspe, P,
-                                                      c->d_facc, ep + EN_VDW14 * MP,
-                                                      ep + EN_EL14 * MP);
-  }
+  BSeg S = make_seg(c);
+  int nblk = bonded_blocks(c);
+  if (nblk > 0)
+    k_bonded<<<nblk, BT, 0, st>>>(d_coor, S, P, c->d_facc, c->d_epart, c->max_parts);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
 
-int bonded_max_blocks(const moil_ctx *c) {
-  int m = 1;
-  int ns[6] = {c->nb, c->nangl, c->ntors, c->nimp, c->nexcl, c->totspe};
-  for (int k = 0; k < 6; k++) m = max(m, (ns[k] + BT - 1) / BT);
-  return m;
-}
+int bonded_max_blocks(const moil_ctx *c) { return max(1, bonded_blocks(c)); }

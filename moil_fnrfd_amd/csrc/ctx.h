@@ -18,7 +18,10 @@ struct moil_ctx {
   moil_params par;
   int npt = 0, totmon = 0, nwat = 0, nb = 0, nangl = 0, ntors = 0, nimp = 0, totspe = 0;
   int nexcl = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // default stream of the context
+  hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool facc_dirty = false;
   int fix_shift = FIX_SHIFT_FP64;
   double fix_scale = 0, fix_inv = 0;
 
@@ -50,7 +53,12 @@ struct moil_ctx {
   double *d_eterm = nullptr;   // influence function on the half spectrum
   std::vector<double> bsp[3];
   int nf1 = 0, nf2 = 0, nf3 = 0, nc1 = 0;  // grid dims; nc1 = nf1/2+1
-  long long *d_qacc = nullptr;  // fixed-point charge grid [nf3][nf2][nf1]
+  int32_t *d_pcnt = nullptr;    // [G+1] atoms per stencil-base grid point
+  int32_t *d_pstart = nullptr;  // [G+1] exclusive scan of d_pcnt
+  int32_t *d_pbsum = nullptr;   // scan block sums
+  int32_t *d_abin = nullptr, *d_arank = nullptr;  // [npt] base point, rank in bin
+  void *d_aw = nullptr, *d_sw = nullptr;  // [npt][4] (w1 w2 w3 q*fc), atom / bin order
+  int32_t *d_sx = nullptr;      // [npt] base x index, bin order
   void *d_qreal = nullptr;      // real grid (float or double)
   void *d_qcplx = nullptr;      // half spectrum
   rocfft_plan plan_fwd = nullptr, plan_bwd = nullptr;

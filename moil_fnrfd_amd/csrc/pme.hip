@@ -12,10 +12,14 @@
 // eterm = exp(-pi^2 m^2/beta^2) / (pi V |b1 b2 b3|^2 m^2)
 // (pot/ewald_receip.f:723-812).
 //
-// GPU design: the charge grid is accumulated in 64-bit fixed point with
-// integer atomics (order independent, bit-reproducible), converted to
-// float (mixed) or double (FP64) for rocFFT's real-to-complex transform
-// -- the only library call.  The convolution runs on the half spectrum
+// GPU design: spreading is done without atomics on the grid.  Atoms are
+// binned by their stencil base point int(fr) with a counting sort (one
+// integer atomic per atom on the bin counts, a device-wide scan, a
+// scatter); then one thread per grid point gathers the 4x4x4 bins whose
+// stencils cover it and sums the B-spline weighted charges in fixed point
+// (order independent, so bit-reproducible), writing the float (mixed) or
+// double (FP64) grid for rocFFT's real-to-complex transform -- the only
+// library call.  The convolution runs on the half spectrum
 // with the influence function tabulated once in FP64 at init; the
 // reciprocal energy is summed in FP64 with self-conjugate planes weighted
 // once and all others twice.  The gather recomputes the B-splines from
@@ -41,6 +45,11 @@ struct PmePar {
 // order-4 cardinal B-spline weights and derivatives at fractional offset
 // w (pot/ewald_receip.f:1013-1080: linear init, one recursion to order 3,
 // derivative from order 3, final recursion to order 4)
+template <typename R>
+__device__ __forceinline__ R pick4(const R t[4], int a) {
+  return a == 0 ? t[0] : (a == 1 ? t[1] : (a == 2 ? t[2] : t[3]));
+}
+
 template <typename R>
 __device__ __forceinline__ void bspline4(R w, R th[4], R dth[4]) {
   R c1 = (R)1 - w, c2 = w;
@@ -71,54 +80,202 @@ __device__ __forceinline__ void frac_coord(double x, double r, int n, int *i0, d
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// stencil base bin of every atom, counting-sort histogram
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_spread(const double *__restrict__ coor,
-                                                        const double *__restrict__ q, int npt,
-                                                        PmePar P, long long *__restrict__ qacc) {
+__global__ __launch_bounds__(PME_THREADS) void k_pme_bin(const double *__restrict__ coor,
+                                                         const double *__restrict__ q, int npt,
+                                                         PmePar P, int32_t *__restrict__ cnt,
+                                                         int32_t *__restrict__ abin,
+                                                         int32_t *__restrict__ arank,
+                                                         R *__restrict__ aw) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= npt) return;
-  double qn = q[n] * P.fc;
-  if (qn == 0.0) return;
   int i0, j0, k0;
   double w1, w2, w3;
   frac_coord(coor[3 * n], P.ra, P.n1, &i0, &w1);
   frac_coord(coor[3 * n + 1], P.rb, P.n2, &j0, &w2);
   frac_coord(coor[3 * n + 2], P.rc, P.n3, &k0, &w3);
-  R t1[4], t2[4], t3[4], d[4];
-  bspline4<R>((R)w1, t1, d);
-  bspline4<R>((R)w2, t2, d);
-  bspline4<R>((R)w3, t3, d);
-  const R qr = (R)qn;
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int k = wrapi(k0 + c, P.n3);
-    const R p3 = t3[c] * qr;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int j = wrapi(j0 + b, P.n2);
-      const R p2 = t2[b] * p3;
-      long long *row = qacc + ((size_t)k * P.n2 + j) * P.n1;
-#pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const int i = wrapi(i0 + a, P.n1);
-        long long v = __double2ll_rn((double)(t1[a] * p2) * P.qscale);
-        if (v != 0) atomicAdd((unsigned long long *)&row[i], (unsigned long long)v);
-      }
-    }
-  }
+  // base point int(fr) = i0 + 3, in [0, n] (n only when fr == n exactly)
+  int c1 = i0 + 3, c2 = j0 + 3, c3 = k0 + 3;
+  if (c1 >= P.n1) c1 -= P.n1;
+  if (c2 >= P.n2) c2 -= P.n2;
+  if (c3 >= P.n3) c3 -= P.n3;
+  int bin = (c3 * P.n2 + c2) * P.n1 + c1;
+  abin[n] = bin;
+  arank[n] = atomicAdd(&cnt[bin], 1);
+  aw[4 * n] = (R)w1;
+  aw[4 * n + 1] = (R)w2;
+  aw[4 * n + 2] = (R)w3;
+  aw[4 * n + 3] = (R)(q[n] * P.fc);
 }
 
 template <typename R>
-__global__ void k_qconvert(const long long *__restrict__ qacc, size_t G, double qinv,
-                           R *__restrict__ qreal) {
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  qreal[g] = (R)((double)qacc[g] * qinv);
+__global__ __launch_bounds__(PME_THREADS) void k_pme_scatter(int npt, const int32_t *__restrict__ abin,
+                                                             const int32_t *__restrict__ arank,
+                                                             const int32_t *__restrict__ start,
+                                                             const R *__restrict__ aw,
+                                                             R *__restrict__ sw,
+                                                             int32_t *__restrict__ sx, int n1) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= npt) return;
+  int b = abin[n];
+  int p = start[b] + arank[n];
+  sw[4 * p] = aw[4 * n];
+  sw[4 * p + 1] = aw[4 * n + 1];
+  sw[4 * p + 2] = aw[4 * n + 2];
+  sw[4 * p + 3] = aw[4 * n + 3];
+  sx[p] = b % n1;
 }
 
-__global__ void k_zero_ll(long long *__restrict__ p, size_t n) {
+__device__ __forceinline__ void fix_split_f(float f, int &h, int &l) {
+  const float x = f * 256.0f;
+  const float hr = rintf(x);
+  const float r = x - hr;
+  h = (int)hr;
+  l = (int)rintf(r * 16777216.0f);
+}
+
+// one thread per grid point: Q(g) = sum over atoms with base point b,
+// b - 3 <= g <= b (periodic), of q fc theta1 theta2 theta3
+template <typename R>
+__global__ __launch_bounds__(PME_THREADS) void k_pme_spread(const int32_t *__restrict__ start,
+                                                            const R *__restrict__ sw,
+                                                            const int32_t *__restrict__ sx,
+                                                            PmePar P, R *__restrict__ grid) {
+  const size_t G = (size_t)P.n1 * P.n2 * P.n3;
   size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) p[g] = 0;
+  if (g >= G) return;
+  const int g1 = (int)(g % P.n1);
+  const int g2 = (int)((g / P.n1) % P.n2);
+  const int g3 = (int)(g / ((size_t)P.n1 * P.n2));
+  int acc_h = 0, acc_l = 0;  // FP32 path: fixed point 2^-32 split h*2^24+l
+  long long acc = 0;         // FP64 path: fixed point 2^-40
+#pragma unroll
+  for (int d3 = 0; d3 < 4; d3++) {
+    int c3 = g3 + d3;
+    if (c3 >= P.n3) c3 -= P.n3;
+#pragma unroll
+    for (int d2 = 0; d2 < 4; d2++) {
+      int c2 = g2 + d2;
+      if (c2 >= P.n2) c2 -= P.n2;
+      const int row = (c3 * P.n2 + c2) * P.n1;
+      // bins g1..g1+3 of this row: one or two contiguous runs
+      int e1 = g1 + 3;
+      int lo[2], hi[2], nr = 1;
+      if (e1 < P.n1) {
+        lo[0] = start[row + g1];
+        hi[0] = start[row + e1 + 1];
+      } else {
+        lo[0] = start[row + g1];
+        hi[0] = start[row + P.n1];
+        lo[1] = start[row];
+        hi[1] = start[row + (e1 - P.n1) + 1];
+        nr = 2;
+      }
+      for (int rr = 0; rr < nr; rr++) {
+        for (int a = lo[rr]; a < hi[rr]; a++) {
+          int x1 = sx[a] - g1;
+          if (x1 < 0) x1 += P.n1;  // 0..3
+          R t1[4], t2[4], t3[4], dd[4];
+          bspline4<R>(sw[4 * a], t1, dd);
+          bspline4<R>(sw[4 * a + 1], t2, dd);
+          bspline4<R>(sw[4 * a + 2], t3, dd);
+          const R v = sw[4 * a + 3] * t3[3 - d3] * t2[3 - d2] * pick4<R>(t1, 3 - x1);
+          if (sizeof(R) == 4) {
+            int h, l;
+            fix_split_f((float)v, h, l);
+            acc_h += h;
+            acc_l += l;
+          } else {
+            acc += __double2ll_rn((double)v * P.qscale);
+          }
+        }
+      }
+    }
+  }
+  if (sizeof(R) == 4)
+    grid[g] = (R)((double)(((long long)acc_h << 24) + (long long)acc_l) * 0x1p-32);
+  else
+    grid[g] = (R)((double)acc * P.qinv);
+}
+
+// device-wide exclusive scan in three passes (per-block sums, scan of the
+// block sums, per-block rescan); the last pass also clears the counts for
+// the next evaluation
+#define SCAN_PER 2048
+__global__ __launch_bounds__(256) void k_scan_sums(const int32_t *__restrict__ in, size_t n,
+                                                   int32_t *__restrict__ bsum) {
+  __shared__ int red[256];
+  size_t b0 = (size_t)blockIdx.x * SCAN_PER;
+  int s = 0;
+  for (int k = threadIdx.x; k < SCAN_PER; k += 256) {
+    size_t i = b0 + k;
+    if (i < n) s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb) {
+  __shared__ int part[1024];
+  int t = threadIdx.x;
+  int per = (nb + 1023) / 1024;
+  int b = t * per, e = min(nb, b + per);
+  int s = 0;
+  for (int i = b; i < e; i++) s += bsum[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int i = b; i < e; i++) {
+    int v = bsum[i];
+    bsum[i] = run;
+    run += v;
+  }
+  if (t == 1023) bsum[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(int32_t *__restrict__ cnt, size_t n,
+                                                    const int32_t *__restrict__ bsum,
+                                                    int32_t *__restrict__ out) {
+  __shared__ int part[256];
+  size_t b0 = (size_t)blockIdx.x * SCAN_PER;
+  const int per = SCAN_PER / 256;  // 8 consecutive elements per thread
+  int v[per];
+  int s = 0;
+  for (int k = 0; k < per; k++) {
+    size_t i = b0 + (size_t)threadIdx.x * per + k;
+    v[k] = (i < n) ? cnt[i] : 0;
+    s += v[k];
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int a = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int run = bsum[blockIdx.x] + part[threadIdx.x] - s;
+  for (int k = 0; k < per; k++) {
+    size_t i = b0 + (size_t)threadIdx.x * per + k;
+    if (i < n) {
+      out[i] = run;
+      cnt[i] = 0;
+    }
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = bsum[gridDim.x];
 }
 
 // half-spectrum convolution + energy (weights 1 for the k1 = 0 and
@@ -282,9 +439,19 @@ int32_t pme_init(moil_ctx *c) {
   }
   HIP_CHECK(hipMalloc(&c->d_eterm, sizeof(double) * H));
   HIP_CHECK(hipMemcpy(c->d_eterm, et.data(), sizeof(double) * H, hipMemcpyHostToDevice));
-  HIP_CHECK(hipMalloc(&c->d_qacc, sizeof(long long) * G));
   const bool f32 = c->par.precision == MOIL_PREC_MIXED;
   const size_t rs = f32 ? sizeof(float) : sizeof(double);
+  const int na = std::max(c->npt, 1);
+  const size_t nscan = (G + SCAN_PER - 1) / SCAN_PER;
+  HIP_CHECK(hipMalloc(&c->d_pcnt, sizeof(int32_t) * (G + 1)));
+  HIP_CHECK(hipMemset(c->d_pcnt, 0, sizeof(int32_t) * (G + 1)));
+  HIP_CHECK(hipMalloc(&c->d_pstart, sizeof(int32_t) * (G + 1)));
+  HIP_CHECK(hipMalloc(&c->d_pbsum, sizeof(int32_t) * (nscan + 1)));
+  HIP_CHECK(hipMalloc(&c->d_abin, sizeof(int32_t) * na));
+  HIP_CHECK(hipMalloc(&c->d_arank, sizeof(int32_t) * na));
+  HIP_CHECK(hipMalloc(&c->d_aw, 4 * rs * na));
+  HIP_CHECK(hipMalloc(&c->d_sw, 4 * rs * na));
+  HIP_CHECK(hipMalloc(&c->d_sx, sizeof(int32_t) * na));
   HIP_CHECK(hipMalloc(&c->d_qreal, rs * G));
   HIP_CHECK(hipMalloc(&c->d_qcplx, 2 * rs * H));
   HIP_CHECK(hipMalloc(&c->d_fpme, sizeof(double) * 3 * std::max(c->npt, 1)));
@@ -319,7 +486,14 @@ void pme_free(moil_ctx *c) {
   c->exec_fwd = c->exec_bwd = nullptr;
   hipFree(c->d_fftwork); c->d_fftwork = nullptr;
   hipFree(c->d_eterm); c->d_eterm = nullptr;
-  hipFree(c->d_qacc); c->d_qacc = nullptr;
+  hipFree(c->d_pcnt); c->d_pcnt = nullptr;
+  hipFree(c->d_pstart); c->d_pstart = nullptr;
+  hipFree(c->d_pbsum); c->d_pbsum = nullptr;
+  hipFree(c->d_abin); c->d_abin = nullptr;
+  hipFree(c->d_arank); c->d_arank = nullptr;
+  hipFree(c->d_aw); c->d_aw = nullptr;
+  hipFree(c->d_sw); c->d_sw = nullptr;
+  hipFree(c->d_sx); c->d_sx = nullptr;
   hipFree(c->d_qreal); c->d_qreal = nullptr;
   hipFree(c->d_qcplx); c->d_qcplx = nullptr;
   hipFree(c->d_fpme); c->d_fpme = nullptr;
@@ -341,10 +515,17 @@ static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.qscale = ldexp(1.0, QFIX_SHIFT);
   P.qinv = ldexp(1.0, -QFIX_SHIFT);
   const int gb = (int)((G + PME_THREADS - 1) / PME_THREADS);
-  k_zero_ll<<<gb, PME_THREADS, 0, st>>>(c->d_qacc, G);
   const int ab = (c->npt + PME_THREADS - 1) / PME_THREADS;
-  k_spread<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_qacc);
-  k_qconvert<R><<<gb, PME_THREADS, 0, st>>>(c->d_qacc, G, P.qinv, (R *)c->d_qreal);
+  const int nscan = (int)((G + SCAN_PER - 1) / SCAN_PER);
+  k_pme_bin<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_pcnt, c->d_abin,
+                                           c->d_arank, (R *)c->d_aw);
+  k_scan_sums<<<nscan, 256, 0, st>>>(c->d_pcnt, G, c->d_pbsum);
+  k_scan_top<<<1, 1024, 0, st>>>(c->d_pbsum, nscan);
+  k_scan_apply<<<nscan, 256, 0, st>>>(c->d_pcnt, G, c->d_pbsum, c->d_pstart);
+  k_pme_scatter<R><<<ab, PME_THREADS, 0, st>>>(c->npt, c->d_abin, c->d_arank, c->d_pstart,
+                                               (const R *)c->d_aw, (R *)c->d_sw, c->d_sx, c->nf1);
+  k_pme_spread<R><<<gb, PME_THREADS, 0, st>>>(c->d_pstart, (const R *)c->d_sw, c->d_sx, P,
+                                              (R *)c->d_qreal);
   HIP_CHECK(hipGetLastError());
   void *in[1] = {c->d_qreal}, *out[1] = {c->d_qcplx};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_fwd, st));

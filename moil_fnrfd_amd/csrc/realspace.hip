@@ -55,6 +55,20 @@ __device__ __forceinline__ long long fix64(double v, double scale) {
   return __double2ll_rn(v * scale);
 }
 
+// round(f * 2^32) as h * 2^24 + l with int32 h, l (|l| <= 2^23), from FP32
+// operations only: x = f 2^8 (exact), h = rint(x), r = x - h is exact
+// (|r| <= 1/2), l = rint(r 2^24).  Valid for |f| < 2^23.
+__device__ __forceinline__ void fix_split(float f, int &h, int &l) {
+  const float x = f * 256.0f;
+  const float hr = rintf(x);
+  const float r = x - hr;
+  h = (int)hr;
+  l = (int)rintf(r * 16777216.0f);
+}
+__device__ __forceinline__ long long fix_join(int h, int l) {
+  return ((long long)h << 24) + (long long)l;
+}
+
 __device__ __forceinline__ void atomic_add_fix(long long *p, long long v) {
   if (v != 0) atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
@@ -107,8 +121,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
     const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
     const float4 *__restrict__ wpos, const int32_t *__restrict__ slot_o, int nwat, WaterRS P,
     long long *__restrict__ facc, double *__restrict__ epart, int max_parts) {
+  // i-force partials in fixed point split as h * 2^24 + l (int32 each):
+  // a chunk holds <= 256 items, so an accumulator receives <= 256 terms
+  // with |l| <= 2^23 and cannot overflow
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
-  __shared__ long long iacc[WBLOCK * 9];
+  __shared__ int iacc_h[WBLOCK * 9], iacc_l[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
   const int ch = blockIdx.x, t = threadIdx.x;
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
@@ -120,7 +137,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
       iC[t] = wpos[3 * (s0 + t) + 2];
     }
   }
-  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
+    iacc_h[u] = 0;
+    iacc_l[u] = 0;
+  }
   __syncthreads();
 
   const float beta = (float)P.beta;
@@ -130,7 +150,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
   const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
   const float pe = ERFC_P;
   const float log2e = 1.4426950408889634f;
-  const double fs = P.fix_scale;
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
   for (int it = beg + t; it < end; it += RS_THREADS) {
@@ -206,13 +225,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
       }
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        long long v = fix64((double)fi[k], fs);
-        if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
+        int h, l;
+        fix_split(fi[k], h, l);
+        atomicAdd(&iacc_h[i * 9 + k], h);
+        atomicAdd(&iacc_l[i * 9 + k], l);
       }
     }
     const int oj = slot_o[js];
 #pragma unroll
-    for (int k = 0; k < 9; k++) atomic_add_fix(&facc[3 * oj + k], fix64((double)fj[k], fs));
+    for (int k = 0; k < 9; k++) {
+      int h, l;
+      fix_split(fj[k], h, l);
+      atomic_add_fix(&facc[3 * oj + k], fix_join(h, l));
+    }
     if (item.s == 0) {
       e_lj_r += (double)elj;
       e_el_r += (double)eel;
@@ -226,7 +251,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
     const int i = u / 9, k = u % 9;
     if (s0 + i < nwat) {
       const int oi = slot_o[s0 + i];
-      atomic_add_fix(&facc[3 * oi + k], iacc[u]);
+      atomic_add_fix(&facc[3 * oi + k], fix_join(iacc_h[u], iacc_l[u]));
     }
   }
   double v;
