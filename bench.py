@@ -69,13 +69,31 @@ def cpu_baseline(sysm, cut, ew):
                        % nev)
 
 
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device='cuda:0')
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_probe_bytes_read': 8 * n, 'copy_probe_bytes_written': 8 * n,
+                      'launches': 10}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=1000)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only run a plain device copy of MIB MiB ten times (the reference '
+                         'kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)')
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only (--gpus %d requested); the multi-GPU '
               'decomposition is a design note, not an implemented path.' % args.gpus,
@@ -98,16 +116,16 @@ def main():
     coor = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
     dpot = torch.empty_like(coor)
     energ = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
-    # the per-step N(0, 0.01 A) displacements are generated before timing
-    # and stay resident in HBM (float32, sigma applied)
+    # per-step N(0, 0.01 A) displacements: drawn on the device each step
+    # (float32 normals, added to the float64 coordinates)
     gen = torch.Generator(device=dev)
     gen.manual_seed(1001)
-    noise = torch.randn((args.warmup + args.steps, sysm.topo.npt, 3), generator=gen, device=dev,
-                        dtype=torch.float32).mul_(SIGMA)
+    noise = torch.empty(coor.shape, dtype=torch.float32, device=dev)
 
     def step(k, base=0):
         if k > 0:
-            coor.add_(noise[base + k])
+            torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float32, out=noise)
+            coor.add_(noise, alpha=SIGMA)
         if k % NLIST == 0:
             eng.squeeze_dev(coor.data_ptr(), sh)
             eng.nbondm_dev(coor.data_ptr(), sh)

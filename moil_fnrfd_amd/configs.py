@@ -60,7 +60,10 @@ def water_buffered(n_side=10, seed=1010):
 
 
 def hetero_small(seed=2000, n_chains=12, L=24.0):
-    s = hetero_system(n_chains, L, seed=seed)
+    """~1.4k-atom heteropolymer-in-water box for the atom-list parity
+    cases: 15 % uncharged and 15 % zero-LJ chain atoms so that every MOIL
+    atom list (list1/list2/list3) and the hydrogen rule are exercised."""
+    s = hetero_system(n_chains, L, seed=seed, frac_neutral=0.15, frac_hlike=0.15)
     cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
     n = int(round(L)) + int(round(L)) % 2
     ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)

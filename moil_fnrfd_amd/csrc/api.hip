@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -107,11 +109,14 @@ static int32_t upload(T **dst, const T *src, size_t n) {
 static void ctx_free(moil_ctx *c) {
   if (!c) return;
   lists_free(c);
+  atoms_free(c);
   pme_free(c);
   void *ptrs[] = {c->d_poipt, c->d_monoflag, c->d_flagchr, c->d_ptchg, c->d_eps6, c->d_eps12,
                   c->d_exc_i, c->d_exc_j, c->d_ib, c->d_bpar, c->d_ia, c->d_apar, c->d_it,
                   c->d_tpar, c->d_ii, c->d_ipar, c->d_sp, c->d_p14, c->d_wat_o, c->d_wat_mono,
-                  c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos};
+                  c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos,
+                  c->d_wi_boff, c->d_wi_aoff, c->d_wi_eoff, c->d_wi_zero, c->d_wi_bidx,
+                  c->d_wi_aidx, c->d_wi_eidx, c->d_wi_bpar, c->d_wi_apar, c->d_etot, c->d_ticket};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -177,6 +182,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     ctx_free(c);
     return fail(MOIL_GPU_EHIP, "hipStreamCreate/hipEventCreate failed");
   }
+  {
+    const char *e = std::getenv("MOIL_GPU_PME_CONCURRENT");
+    c->pme_concurrent = e && e[0] == '1';
+  }
   // ---- Ewald set-up ----
   double cutoff = std::sqrt(p->cutele2);
   c->beta = ewald_coefficient(cutoff, p->dtol);
@@ -196,34 +205,133 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   TRY(upload(&c->d_ptchg, t->ptchg, (size_t)t->npt));
   TRY(upload(&c->d_eps6, t->epsgm6, (size_t)t->npt));
   TRY(upload(&c->d_eps12, t->epsgm12, (size_t)t->npt));
+  // ---- waters (monomers named TIP3/SPC*) ----
+  for (int m = 0; m < t->totmon; m++) {
+    if (t->monoflag[m] & MOIL_MONO_WATER) {
+      int nat = t->poipt[m + 1] - t->poipt[m];
+      if (nat != 3) {
+        ctx_free(c);
+        return fail(MOIL_GPU_EINVAL, "water monomer without exactly 3 atoms (O, H, H)");
+      }
+      c->h_wat_mono.push_back(m + 1);
+      c->h_wat_o.push_back(t->poipt[m + 1] - 3);  // pot/watwat_ewald.f:103
+    }
+  }
+  c->nwat = (int)c->h_wat_mono.size();
+  if (c->nwat != t->nwaters) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EINVAL, "nwaters does not match the number of TIP3/SPC monomers");
+  }
+  // MOIL's water routines assume the three intra-water pairs are excluded
+  // (the atom lists then never hold an intra-water pair)
+  for (int w = 0; w < c->nwat; w++) {
+    const int o = c->h_wat_o[w];
+    int found = 0;
+    for (int k = t->exc1[o]; k < t->exc1[o + 1]; k++)
+      found += (t->exc2[k] - 1 == o + 1) + (t->exc2[k] - 1 == o + 2);
+    for (int k = t->exc1[o + 1]; k < t->exc1[o + 2]; k++) found += (t->exc2[k] - 1 == o + 2);
+    if (found != 3) {
+      ctx_free(c);
+      return fail(MOIL_GPU_EINVAL, "water monomer whose O-H, O-H and H-H pairs are not all excluded");
+    }
+  }
+  // water index of every atom (-1 outside waters) for the intra-water split
+  std::vector<int32_t> wat_of(t->npt, -1);
+  for (int w = 0; w < c->nwat; w++)
+    for (int k = 0; k < 3; k++) wat_of[c->h_wat_o[w] + k] = w;
+  auto same_water = [&](std::initializer_list<int> atoms0) -> int {
+    int w = -2;
+    for (int a : atoms0) {
+      int x = wat_of[a];
+      if (x < 0 || (w != -2 && x != w)) return -1;
+      w = x;
+    }
+    return w;
+  };
+  // per-water CSR of intra-water terms (original order kept within a water)
+  std::vector<std::vector<int>> wb(c->nwat), wa(c->nwat), we(c->nwat);
+  std::vector<int32_t> wi_eidx;
   {
     std::vector<int32_t> ei, ej;
+    std::vector<std::pair<int, int>> all;
     for (int i = 1; i <= t->npt - 1; i++)  // pot/cdie_ewald.f:68,83 (atom npt ignored)
-      for (int k = t->exc1[i - 1]; k < t->exc1[i]; k++) {
-        ei.push_back(i - 1);
-        ej.push_back(t->exc2[k] - 1);
+      for (int k = t->exc1[i - 1]; k < t->exc1[i]; k++) all.push_back({i - 1, t->exc2[k] - 1});
+    c->nexcl = (int)all.size();
+    for (size_t m = 0; m < all.size(); m++) {
+      int w = same_water({all[m].first, all[m].second});
+      if (w >= 0) {
+        we[w].push_back((int)m);
+      } else {
+        ei.push_back(all[m].first);
+        ej.push_back(all[m].second);
       }
-    c->nexcl = (int)ei.size();
+    }
+    c->gnexcl = (int)ei.size();
     TRY(upload(&c->d_exc_i, ei.data(), ei.size()));
     TRY(upload(&c->d_exc_j, ej.data(), ej.size()));
+    for (int w = 0; w < c->nwat; w++)
+      for (int m : we[w]) {
+        wi_eidx.push_back(all[m].first - c->h_wat_o[w]);
+        wi_eidx.push_back(all[m].second - c->h_wat_o[w]);
+      }
   }
   {
-    std::vector<int32_t> ib(2 * (size_t)c->nb);
-    std::vector<double> bp(2 * (size_t)c->nb);
+    std::vector<int32_t> ib, wi_bidx;
+    std::vector<double> bp, wi_bpar;
     for (int m = 0; m < c->nb; m++) {
-      ib[2 * m] = t->ib1[m]; ib[2 * m + 1] = t->ib2[m];
-      bp[2 * m] = t->kbond[m]; bp[2 * m + 1] = t->req[m];
+      int w = same_water({t->ib1[m] - 1, t->ib2[m] - 1});
+      if (w >= 0) {
+        wb[w].push_back(m);
+      } else {
+        ib.push_back(t->ib1[m]); ib.push_back(t->ib2[m]);
+        bp.push_back(t->kbond[m]); bp.push_back(t->req[m]);
+      }
     }
+    c->gnb = (int)ib.size() / 2;
     TRY(upload(&c->d_ib, ib.data(), ib.size()));
     TRY(upload(&c->d_bpar, bp.data(), bp.size()));
-    std::vector<int32_t> ia(3 * (size_t)c->nangl);
-    std::vector<double> ap(2 * (size_t)c->nangl);
+    std::vector<int32_t> ia, wi_aidx;
+    std::vector<double> ap, wi_apar;
     for (int m = 0; m < c->nangl; m++) {
-      ia[3 * m] = t->iangl1[m]; ia[3 * m + 1] = t->iangl2[m]; ia[3 * m + 2] = t->iangl3[m];
-      ap[2 * m] = t->kangl[m]; ap[2 * m + 1] = t->angleq[m];
+      int w = same_water({t->iangl1[m] - 1, t->iangl2[m] - 1, t->iangl3[m] - 1});
+      if (w >= 0) {
+        wa[w].push_back(m);
+      } else {
+        ia.push_back(t->iangl1[m]); ia.push_back(t->iangl2[m]); ia.push_back(t->iangl3[m]);
+        ap.push_back(t->kangl[m]); ap.push_back(t->angleq[m]);
+      }
     }
+    c->gnangl = (int)ia.size() / 3;
     TRY(upload(&c->d_ia, ia.data(), ia.size()));
     TRY(upload(&c->d_apar, ap.data(), ap.size()));
+    // intra-water CSR
+    std::vector<int32_t> boff(c->nwat + 1, 0), aoff(c->nwat + 1, 0), eoff(c->nwat + 1, 0);
+    for (int w = 0; w < c->nwat; w++) {
+      const int a0 = c->h_wat_o[w];
+      for (int m : wb[w]) {
+        wi_bidx.push_back(t->ib1[m] - 1 - a0); wi_bidx.push_back(t->ib2[m] - 1 - a0);
+        wi_bpar.push_back(t->kbond[m]); wi_bpar.push_back(t->req[m]);
+      }
+      for (int m : wa[w]) {
+        wi_aidx.push_back(t->iangl1[m] - 1 - a0); wi_aidx.push_back(t->iangl2[m] - 1 - a0);
+        wi_aidx.push_back(t->iangl3[m] - 1 - a0);
+        wi_apar.push_back(t->kangl[m]); wi_apar.push_back(t->angleq[m]);
+      }
+      boff[w + 1] = boff[w] + (int)wb[w].size();
+      aoff[w + 1] = aoff[w] + (int)wa[w].size();
+      eoff[w + 1] = eoff[w] + (int)we[w].size();
+    }
+    c->wi_nterms = boff[c->nwat] + aoff[c->nwat] + eoff[c->nwat];
+    std::vector<int32_t> zero(c->nwat + 1, 0);
+    TRY(upload(&c->d_wi_boff, boff.data(), boff.size()));
+    TRY(upload(&c->d_wi_aoff, aoff.data(), aoff.size()));
+    TRY(upload(&c->d_wi_eoff, eoff.data(), eoff.size()));
+    TRY(upload(&c->d_wi_zero, zero.data(), zero.size()));
+    TRY(upload(&c->d_wi_bidx, wi_bidx.data(), wi_bidx.size()));
+    TRY(upload(&c->d_wi_aidx, wi_aidx.data(), wi_aidx.size()));
+    TRY(upload(&c->d_wi_eidx, wi_eidx.data(), wi_eidx.size()));
+    TRY(upload(&c->d_wi_bpar, wi_bpar.data(), wi_bpar.size()));
+    TRY(upload(&c->d_wi_apar, wi_apar.data(), wi_apar.size()));
     std::vector<int32_t> it(4 * (size_t)c->ntors);
     std::vector<double> tp(6 * (size_t)c->ntors);
     for (int m = 0; m < c->ntors; m++) {
@@ -250,29 +358,6 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     TRY(upload(&c->d_sp, sp.data(), sp.size()));
     TRY(upload(&c->d_p14, t->p14, 3 * (size_t)c->totspe));
   }
-  // ---- waters (monomers named TIP3/SPC*) ----
-  for (int m = 0; m < t->totmon; m++) {
-    if (t->monoflag[m] & MOIL_MONO_WATER) {
-      int nat = t->poipt[m + 1] - t->poipt[m];
-      if (nat < 3) {
-        ctx_free(c);
-        return fail(MOIL_GPU_EINVAL, "water monomer with fewer than 3 atoms");
-      }
-      c->h_wat_mono.push_back(m + 1);
-      c->h_wat_o.push_back(t->poipt[m + 1] - 3);  // pot/watwat_ewald.f:103
-    } else {
-      c->has_nonwater = true;
-    }
-  }
-  c->nwat = (int)c->h_wat_mono.size();
-  if (c->nwat != t->nwaters) {
-    ctx_free(c);
-    return fail(MOIL_GPU_EINVAL, "nwaters does not match the number of TIP3/SPC monomers");
-  }
-  if (c->has_nonwater) {
-    ctx_free(c);
-    return fail(MOIL_GPU_EINVAL, "non-water monomers: atom pair lists not available in this build");
-  }
   if (c->nwat >= 2) {
     // parameters of the last water, pot/watwat_ewald.f:40-72
     int m = t->idxtip3[t->nwaters - 1];
@@ -284,6 +369,16 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     c->qq_oo = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[io];
     c->qq_oh = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[ih];
     c->qq_hh = epst * KOFDIE / p->eps * t->ptchg[ih] * t->ptchg[ih];
+  }
+  if (p->precision == MOIL_PREC_FP64) {
+    // the FP64 pair kernels read MOIL's erfc table (1.5 relx long,
+    // symm/ewald_init.f:107-126) at every listed distance: pairs reach
+    // max(rvbg, rebg), water site pairs up to ~2 A more
+    const double reach = std::sqrt(std::max(p->cutvbig2, p->cutebig2)) + (c->nwat > 0 ? 2.0 : 0.0);
+    if (c->beta * c->erftbdns * reach >= (double)(c->ntab - 1)) {
+      ctx_free(c);
+      return fail(MOIL_GPU_EINVAL, "list cutoffs reach beyond MOIL's erfc table (1.5 relx)");
+    }
   }
   TRY(upload(&c->d_wat_mono, c->h_wat_mono.data(), c->h_wat_mono.size()));
   TRY(upload(&c->d_wat_o, c->h_wat_o.data(), c->h_wat_o.size()));
@@ -298,10 +393,17 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     return fail(MOIL_GPU_EHIP, "hipMemset failed");
   }
   TRY(mal((void **)&c->d_energy, sizeof(double) * 32));
+  TRY(mal((void **)&c->d_etot, sizeof(double) * EN_COUNT));
+  TRY(mal((void **)&c->d_ticket, sizeof(unsigned int) * 4));
+  if (hipMemset(c->d_ticket, 0, sizeof(unsigned int) * 4) != hipSuccess) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EHIP, "hipMemset failed");
+  }
   TRY(mal((void **)&c->d_coor, sizeof(double) * 3 * c->npt));
   TRY(mal((void **)&c->d_dpot, sizeof(double) * 3 * c->npt));
   TRY(mal(&c->d_wpos, (p->precision == MOIL_PREC_MIXED ? 48 : 72) * (size_t)std::max(c->nwat, 1)));
   TRY(lists_init(c));
+  TRY(atoms_init(c, t));
   TRY(pme_init(c));
   TRY(ensure_parts(c, std::max(pme_conv_blocks(c), bonded_max_blocks(c))));
 #undef TRY
@@ -399,7 +501,8 @@ extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, v
   if (!c || !d_coor) return fail(MOIL_GPU_EINVAL, "null argument");
   hipStream_t st = pick_stream(c, stream);
   CHECK_RC(lists_build(c, d_coor, st));
-  CHECK_RC(ensure_parts(c, std::max(c->nwchunk, c->nachunk)));
+  CHECK_RC(atoms_build(c, d_coor, st));
+  CHECK_RC(ensure_parts(c, c->nwchunk + c->nachunk));
   return MOIL_GPU_OK;
 }
 
@@ -432,52 +535,62 @@ struct EnCfg {
   int has_bond, has_angle, has_tors, has_imp, has_nb, has_14;
 };
 
-// deterministic reduction of the per-workgroup partials (one wavefront per
-// energy term, fixed strided order + fixed shuffle tree) and assembly of
-// the energy record in eforce's order (pot/eforce.f:100-205)
-__global__ __launch_bounds__(1024) void k_energies(const double *__restrict__ ep, EnCfg C,
-                                                   double *__restrict__ out) {
-  __shared__ double tot[EN_COUNT];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (w < EN_COUNT) {
-    double s = 0.0;
-    const double *row = ep + (size_t)w * C.max_parts;
-    for (int i = lane; i < C.nparts[w]; i += 64) s += row[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) tot[w] = s;
+// deterministic reduction of the per-workgroup partials: one workgroup
+// per energy term (fixed strided order + fixed shuffle tree), then the
+// last workgroup to finish assembles the energy record in eforce's order
+// (pot/eforce.f:100-205) and resets the ticket for the next call
+__global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep, EnCfg C,
+                                                  double *__restrict__ tot,
+                                                  unsigned int *__restrict__ ticket,
+                                                  double *__restrict__ out) {
+  __shared__ double red[4];
+  __shared__ bool last;
+  const int term = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double s = 0.0;
+  const double *row = ep + (size_t)term * C.max_parts;
+  for (int i = t; i < C.nparts[term]; i += 256) s += row[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (t == 0) {
+    tot[term] = (red[0] + red[1]) + (red[2] + red[3]);
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == (unsigned)(gridDim.x - 1);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    moil_energies e;
-    e.e_bond = tot[EN_BOND];
-    e.e_theta = tot[EN_THETA];
-    e.e_tors = tot[EN_TORS];
-    e.e_imp = tot[EN_IMP];
-    e.e_vdw = tot[EN_VDW];
-    e.e_el = tot[EN_EL];
-    e.e_vsym = tot[EN_VSYM];
-    e.e_lsym = tot[EN_LSYM];
-    e.e_el14 = tot[EN_EL14];
-    e.e_vdw14 = tot[EN_VDW14];
-    e.e_corr = tot[EN_CORR];
-    e.e_self = C.e_self;
-    e.e_ew_receip = tot[EN_RECIP];
-    double et = 0.0;
-    if (C.has_bond) et += e.e_bond;
-    if (C.has_angle) et += e.e_theta;
-    if (C.has_tors) et += e.e_tors;
-    if (C.has_imp) et += e.e_imp;
-    if (C.has_nb) {
-      et += e.e_ew_receip + e.e_corr + e.e_self;
-      et += e.e_vdw + e.e_el;
-      if (C.has_14) et += e.e_el14 + e.e_vdw14;
-      et += e.e_vsym + e.e_lsym;
-    }
-    e.e_total = et;
-    e.e_dir = e.e_el + e.e_lsym;
-    const double *src = &e.e_total;
-    for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = src[k];
+  if (!last || t != 0) return;
+  __threadfence();
+  volatile const double *T = tot;
+  moil_energies e;
+  e.e_bond = T[EN_BOND];
+  e.e_theta = T[EN_THETA];
+  e.e_tors = T[EN_TORS];
+  e.e_imp = T[EN_IMP];
+  e.e_vdw = T[EN_VDW];
+  e.e_el = T[EN_EL];
+  e.e_vsym = T[EN_VSYM];
+  e.e_lsym = T[EN_LSYM];
+  e.e_el14 = T[EN_EL14];
+  e.e_vdw14 = T[EN_VDW14];
+  e.e_corr = T[EN_CORR];
+  e.e_self = C.e_self;
+  e.e_ew_receip = T[EN_RECIP];
+  double et = 0.0;
+  if (C.has_bond) et += e.e_bond;
+  if (C.has_angle) et += e.e_theta;
+  if (C.has_tors) et += e.e_tors;
+  if (C.has_imp) et += e.e_imp;
+  if (C.has_nb) {
+    et += e.e_ew_receip + e.e_corr + e.e_self;
+    et += e.e_vdw + e.e_el;
+    if (C.has_14) et += e.e_el14 + e.e_vdw14;
+    et += e.e_vsym + e.e_lsym;
   }
+  e.e_total = et;
+  e.e_dir = e.e_el + e.e_lsym;
+  const double *src = &e.e_total;
+  for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = src[k];
+  *ticket = 0u;
 }
 
 // stage timing: event pairs recorded on the launch stream, summed lazily
@@ -507,6 +620,7 @@ struct StageTimer {
 };
 
 int bonded_blocks(const moil_ctx *c);
+int water_intra_blocks(const moil_ctx *c);
 
 extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
                                        double *d_energies, void *stream) {
@@ -522,8 +636,11 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   }
   c->facc_dirty = true;  // until the finalize kernel is enqueued
   StageTimer whole(c, 4, st);
-  // reciprocal space on a second stream, overlapped with bonded + real space
-  if (nb) {
+  // reciprocal space: in order on the launch stream by default; with
+  // pme_concurrent on a second stream overlapped with bonded + real space
+  // (the real-space kernel fills every CU, so the overlap mostly slows the
+  // latency-bound PME kernels down)
+  if (nb && c->pme_concurrent) {
     HIP_CHECK(hipEventRecord(c->ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     {
@@ -531,28 +648,40 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
       CHECK_RC(pme_run(c, d_coor, c->stream2));
     }
     HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+  } else if (nb) {
+    StageTimer tp(c, 2, st);
+    CHECK_RC(pme_run(c, d_coor, st));
   }
   {
     StageTimer tb(c, 3, st);
     CHECK_RC(bonded_run(c, d_coor, st));
   }
+  const int nwpart = (nb && c->nwat >= 2) ? c->nwchunk : 0;
   if (nb) {
     CHECK_RC(realspace_water_pack(c, d_coor, st));
-    StageTimer tw(c, 0, st);
-    CHECK_RC(realspace_water(c, d_coor, st));
+    {
+      StageTimer tw(c, 0, st);
+      CHECK_RC(realspace_water(c, d_coor, st));
+    }
+    CHECK_RC(realspace_water_reduce(c, st));
+    {
+      StageTimer ta(c, 1, st);
+      CHECK_RC(realspace_atoms(c, d_coor, st, nwpart));
+    }
+    CHECK_RC(realspace_atoms_reduce(c, st));
   }
-  if (nb) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  if (nb && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
   k_finalize_forces<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, c->d_fpme, nb ? 1 : 0,
                                                                   n3, c->fix_inv, d_dpot);
   c->facc_dirty = false;
   EnCfg C;
   C.max_parts = c->max_parts;
-  const int nbond = bonded_blocks(c);
+  const int nbond = bonded_blocks(c), nwi = water_intra_blocks(c);
   for (int k = 0; k < EN_COUNT; k++) C.nparts[k] = 0;
-  C.nparts[EN_BOND] = C.nparts[EN_THETA] = C.nparts[EN_TORS] = C.nparts[EN_IMP] = nbond;
-  C.nparts[EN_CORR] = C.nparts[EN_VDW14] = C.nparts[EN_EL14] = nbond;
-  const int nw = (nb && c->nwat >= 2) ? c->nwchunk : 0;
-  C.nparts[EN_VDW] = C.nparts[EN_EL] = C.nparts[EN_VSYM] = C.nparts[EN_LSYM] = nw;
+  C.nparts[EN_TORS] = C.nparts[EN_IMP] = C.nparts[EN_VDW14] = C.nparts[EN_EL14] = nbond;
+  C.nparts[EN_BOND] = C.nparts[EN_THETA] = C.nparts[EN_CORR] = nbond + nwi;
+  const int nrs = nwpart + (nb ? c->nachunk : 0);
+  C.nparts[EN_VDW] = C.nparts[EN_EL] = C.nparts[EN_VSYM] = C.nparts[EN_LSYM] = nrs;
   C.nparts[EN_RECIP] = nb ? pme_conv_blocks(c) : 0;
   C.e_self = c->e_self;
   C.has_bond = (T & MOIL_TERM_BOND) && c->nb > 0;
@@ -561,7 +690,8 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   C.has_imp = (T & MOIL_TERM_IMP) && c->nimp > 0;
   C.has_nb = nb;
   C.has_14 = c->totspe > 0 && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14));
-  k_energies<<<1, 1024, 0, st>>>(c->d_epart, C, d_energies ? d_energies : c->d_energy);
+  k_energies<<<EN_COUNT, 256, 0, st>>>(c->d_epart, C, c->d_etot, c->d_ticket,
+                                      d_energies ? d_energies : c->d_energy);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
@@ -591,6 +721,18 @@ extern "C" int32_t moil_gpu_list_counts(moil_ctx *c, int64_t counts[8]) {
       int img = w.s > 0 ? 2 : 0;
       counts[img + 0] += __builtin_popcountll(w.m1);
       counts[img + 1] += __builtin_popcountll(w.m2);
+    }
+  }
+  if (c->naitems) {
+    std::vector<AItem> it(c->naitems);
+    HIP_CHECK(hipMemcpy(it.data(), c->d_aitems, sizeof(AItem) * it.size(), hipMemcpyDeviceToHost));
+    for (const AItem &a : it) {
+      const int64_t n1 = __builtin_popcountll(a.m1), n2 = __builtin_popcountll(a.m2),
+                    n3 = __builtin_popcountll(a.m3);
+      counts[4] += n1;
+      counts[5] += n2;
+      counts[6] += n3;
+      if (a.s > 0) counts[7] += n1 + n2 + n3;
     }
   }
   return MOIL_GPU_OK;
@@ -633,9 +775,47 @@ extern "C" int32_t moil_gpu_dump_water_pairs(moil_ctx *c, int32_t *rows, int64_t
 
 extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t cap, int64_t *n) {
   if (!c || !n) return fail(MOIL_GPU_EINVAL, "null argument");
-  (void)rows;
-  (void)cap;
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "no lists built");
   *n = 0;
+  if (c->naitems == 0) return MOIL_GPU_OK;
+  std::vector<AItem> it(c->naitems);
+  std::vector<int32_t> s2a(c->nsol);
+  HIP_CHECK(hipMemcpy(it.data(), c->d_aitems, sizeof(AItem) * it.size(), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(s2a.data(), c->d_sslot2atom, sizeof(int32_t) * s2a.size(), hipMemcpyDeviceToHost));
+  int64_t k = 0;
+  for (int b = 0; b < c->nablk; b++)
+    for (int q = c->h_aboff[b * NBUCKET]; q < c->h_aboff[(b + 1) * NBUCKET]; q++) {
+      const AItem &a = it[q];
+      const bool rev = a.s > 13;
+      const int sym = rev ? a.s - 13 : a.s;
+      for (int kind = 1; kind <= 3; kind++) {
+        uint64_t m = kind == 1 ? a.m1 : (kind == 2 ? a.m2 : a.m3);
+        while (m) {
+          const int i = __builtin_ctzll(m);
+          m &= m - 1;
+          const int ai = s2a[b * WBLOCK + i] + 1, aj = a.j + 1;
+          int l, mm;
+          if (sym == 0) {  // real: list owner is the lower atom
+            l = std::min(ai, aj);
+            mm = std::max(ai, aj);
+          } else if (!rev) {
+            l = ai;
+            mm = aj;
+          } else {
+            l = aj;
+            mm = ai;
+          }
+          if (rows && k < cap) {
+            rows[4 * k] = l;
+            rows[4 * k + 1] = mm;
+            rows[4 * k + 2] = sym;
+            rows[4 * k + 3] = kind;
+          }
+          k++;
+        }
+      }
+    }
+  *n = k;
   return MOIL_GPU_OK;
 }
 

@@ -1,0 +1,848 @@
+// MOIL's atom-pair lists and their real-space energy on gfx950: every pair
+// with at least one atom outside a water monomer.
+//
+// Lists (pot/nbond.f:34-149 behind pot/nbondm.f:172-211 for real pairs;
+// symm/nbsym.f:58-142 behind symm/nbmsym.f:178-233 for the 13 image
+// translations), reproduced bit for bit with FP64 predicates in MOIL's
+// operation order and FMA contraction off:
+//  * real pairs: the two monomers are one non-water monomer (atoms l < m)
+//    or two monomers, not both water, whose centres satisfy
+//    r2 <= cutmono2; excluded pairs are dropped (pot/nbond.f:95-103);
+//  * image pairs (l + T, m): the image monomer passes the rch2 box filter,
+//    the real monomer the need_only face filter, centre r2 < cutmono2,
+//    not both water, no exclusion test (symm/nbsym.f:79-142);
+//  * classification on the atom distance: r2 < cutvbig2 -> list1 (both
+//    charged) or list2, else r2 < cutebig2 and both charged -> list3; the
+//    hydrogen rule (epsgm12 < 1e-3 within cutebig2: list3 if both
+//    charged, else dropped) applies to real pairs when hvdw0 is set
+//    (pot/nbond.f:115-126) and always to image pairs (symm/nbsym.f:105-115).
+//
+// Layout: the atoms of non-water monomers ("solute") are sorted by cell
+// into 64-atom blocks; partners j come from a cell grid over all atoms.
+// A block is searched in 27 directions: real, forward images (the block
+// atom is the translated member l, partner any atom) and reverse images
+// (a water atom is the translated member l, the block atom is m).  Items
+// are (partner atom, direction, list1/list2/list3 masks over the block),
+// split, bucketed and chunked exactly like the water items.
+//
+// Energy (pot/cdie_ewald.f:309-673, symm/symcdie_ewald.f:100-549): list1
+// Coulomb always plus LJ when r2 <= rvmx^2; list2 LJ when r2 <= rvmx^2;
+// list3 Coulomb when r2 <= relx^2; LJ A/r^12 - B/r^6 with geometric
+// combination A = epsgm12_l epsgm12_m, B = epsgm6_l epsgm6_m; Coulomb
+// q_l q_m (kofdie/eps) erfc(beta r)/r.  Gradients go to per-item and
+// per-chunk partial rows (no global atomics) that one reduction pass adds
+// to the fixed-point accumulators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ctx.h"
+#include "grid.h"
+
+#define ABT 256
+#define AB_MAXCELLS 1024
+#define ADIRS 27
+
+// per-atom flag bits
+#define AF_CHG 1    // flagchr
+#define AF_SMALL 2  // epsgm12 < 1e-3 (hydrogen rule)
+#define AF_WAT 4    // atom of a water monomer
+
+struct APar : BoxPar {
+  double R;
+  int hvdw0;
+  int nsol;
+};
+
+__device__ inline void dir_of(int d, int &s, bool &rev) {
+  rev = d > 13;
+  s = rev ? d - 13 : d;
+}
+
+// exclusion test for atoms lo < hi (0-based): partners up to lo+64 from
+// the bit mask, farther ones from MOIL's exclusion list (exc1/exc2)
+__device__ inline bool excluded(int lo, int hi, const uint64_t *__restrict__ xmask,
+                                const int32_t *__restrict__ xfar, const int32_t *__restrict__ exc1,
+                                const int32_t *__restrict__ exc2) {
+  const int d = hi - lo;
+  if (d <= 64) return (xmask[lo] >> (d - 1)) & 1ull;
+  if (!xfar[lo]) return false;
+  for (int k = exc1[lo]; k < exc1[lo + 1]; k++)
+    if (exc2[k] - 1 == hi) return true;
+  return false;
+}
+
+// MOIL's list choice for an admitted pair (0 = none, 1/2/3 = list1/2/3)
+__device__ inline int apair_class(double r2, int fl, int fm, bool hrule, const APar &P) {
+  const bool chg = (fl & AF_CHG) && (fm & AF_CHG);
+  if (hrule && ((fl | fm) & AF_SMALL) && r2 < P.cutebig2) return chg ? 3 : 0;
+  if (r2 < P.cutvbig2) return chg ? 1 : 2;
+  if (r2 < P.cutebig2 && chg) return 3;
+  return 0;
+}
+
+__device__ inline double dist2(double ax, double ay, double az, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  double rx = ax - bx, ry = ay - by, rz = az - bz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// ---------------------------------------------------------------------------
+// binning of atom sets into the cell grid
+__global__ void k_atom_cells(const double *__restrict__ coor, const int32_t *__restrict__ ids, int n,
+                             Grid g, int32_t *__restrict__ cell_of, int32_t *__restrict__ cnt) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int a = ids ? ids[q] : q;
+  int cx = cell_coord(coor[3 * a], g.g0x, g.csx, g.gx);
+  int cy = cell_coord(coor[3 * a + 1], g.g0y, g.csy, g.gy);
+  int cz = cell_coord(coor[3 * a + 2], g.g0z, g.csz, g.gz);
+  int k = cell_key(g, cx, cy, cz);
+  cell_of[q] = k;
+  atomicAdd(&cnt[k], 1);
+}
+
+__global__ void k_atom_scatter(const int32_t *__restrict__ cell_of, const int32_t *__restrict__ ids,
+                               int n, const int32_t *__restrict__ start, int32_t *__restrict__ fill,
+                               int32_t *__restrict__ out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int k = cell_of[q];
+  out[start[k] + atomicAdd(&fill[k], 1)] = ids ? ids[q] : q;
+}
+
+__global__ void k_slot_of(const int32_t *__restrict__ sslot2atom, int nsol,
+                          int32_t *__restrict__ slot_of) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nsol) slot_of[sslot2atom[q]] = q;
+}
+
+// ---------------------------------------------------------------------------
+// one workgroup per solute block; WRITE=false counts items per bucket,
+// WRITE=true writes them at boff + deterministic rank
+template <bool WRITE>
+__global__ __launch_bounds__(ABT) void k_ablock_items(
+    const double *__restrict__ coor, const int32_t *__restrict__ sslot2atom,
+    const int32_t *__restrict__ aslot2atom, const int32_t *__restrict__ acstart, Grid g,
+    const int32_t *__restrict__ mono_of, const int32_t *__restrict__ aflag,
+    const double *__restrict__ mcent, const int32_t *__restrict__ flags,
+    const uint64_t *__restrict__ xmask, const int32_t *__restrict__ xfar,
+    const int32_t *__restrict__ exc1, const int32_t *__restrict__ exc2, APar P,
+    int32_t *__restrict__ bcount, const int32_t *__restrict__ boff, AItem *__restrict__ items) {
+#pragma clang fp contract(off)
+  __shared__ double ix[WBLOCK][3], ic[WBLOCK][3];
+  __shared__ int32_t iatom[WBLOCK], imono[WBLOCK], iflag[WBLOCK];
+  __shared__ double bb[6];
+  __shared__ uint64_t imask;
+  __shared__ int32_t cellk[AB_MAXCELLS];
+  __shared__ int32_t cellpre[AB_MAXCELLS + 1];
+  __shared__ int32_t scanbuf[ABT];
+  __shared__ int32_t bk_run[NBUCKET];
+  __shared__ int32_t wcnt[ABT / 64][NBUCKET];
+
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int s0 = b * WBLOCK;
+  const int nb = min(WBLOCK, P.nsol - s0);
+  if (t < WBLOCK) {
+    if (t < nb) {
+      const int a = sslot2atom[s0 + t], m = mono_of[a];
+      iatom[t] = a;
+      imono[t] = m;
+      iflag[t] = aflag[a];
+      for (int k = 0; k < 3; k++) {
+        ix[t][k] = coor[3 * a + k];
+        ic[t][k] = mcent[3 * m + k];
+      }
+    } else {
+      iatom[t] = -1;
+    }
+  }
+  if (t < NBUCKET) bk_run[t] = 0;
+  __syncthreads();
+  if (t == 0) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < nb; i++)
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fmin(lo[k], ix[i][k]);
+        hi[k] = fmax(hi[k], ix[i][k]);
+      }
+    for (int k = 0; k < 3; k++) {
+      bb[k] = lo[k];
+      bb[3 + k] = hi[k];
+    }
+  }
+  __syncthreads();
+  const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
+
+  for (int d = 0; d < ADIRS; d++) {
+    int s;
+    bool rev;
+    dir_of(d, s, rev);
+    int ia, jb, kc;
+    symop_of(s, &ia, &jb, &kc);
+    const double da = ia * P.a, db = jb * P.b, dc = kc * P.c;
+    const bool image = s > 0;
+    // block-atom filter for this direction
+    if (t < 64) {
+      bool ok = false;
+      if (t < nb) {
+        if (!image) ok = true;
+        else if (!rev) ok = image_i_ok(ic[t][0], ic[t][1], ic[t][2], da, db, dc, large_cutmono, P);
+        else ok = need_only_ok(ic[t][0], ic[t][1], ic[t][2], da, db, dc, P);
+      }
+      uint64_t bal = __ballot(ok);
+      if (t == 0) imask = bal;
+    }
+    __syncthreads();
+    const uint64_t ivalid = imask;
+    if (ivalid == 0) {
+      __syncthreads();
+      continue;
+    }
+    // query box: block bbox moved to the partner's frame, grown by R
+    const double sx = rev ? -da : da, sy = rev ? -db : db, sz = rev ? -dc : dc;
+    double q0[3], q1[3];
+    q0[0] = bb[0] + sx - P.R; q1[0] = bb[3] + sx + P.R;
+    q0[1] = bb[1] + sy - P.R; q1[1] = bb[4] + sy + P.R;
+    q0[2] = bb[2] + sz - P.R; q1[2] = bb[5] + sz + P.R;
+    const int cx0 = clampi((int)floor((q0[0] - g.g0x) / g.csx), 0, g.gx - 1);
+    const int cx1 = clampi((int)floor((q1[0] - g.g0x) / g.csx), 0, g.gx - 1);
+    const int cy0 = clampi((int)floor((q0[1] - g.g0y) / g.csy), 0, g.gy - 1);
+    const int cy1 = clampi((int)floor((q1[1] - g.g0y) / g.csy), 0, g.gy - 1);
+    const int cz0 = clampi((int)floor((q0[2] - g.g0z) / g.csz), 0, g.gz - 1);
+    const int cz1 = clampi((int)floor((q1[2] - g.g0z) / g.csz), 0, g.gz - 1);
+    const int nx = cx1 - cx0 + 1, ny = cy1 - cy0 + 1, nz = cz1 - cz0 + 1;
+    const int ncq = nx * ny * nz;
+    for (int cbeg = 0; cbeg < ncq; cbeg += AB_MAXCELLS) {
+      const int ncc = min(AB_MAXCELLS, ncq - cbeg);
+      for (int u = t; u < AB_MAXCELLS; u += ABT) {
+        int sz2 = 0, k = 0;
+        if (u < ncc) {
+          int q = cbeg + u;
+          int cx = cx0 + q % nx, cy = cy0 + (q / nx) % ny, cz = cz0 + q / (nx * ny);
+          k = cell_key(g, cx, cy, cz);
+          sz2 = acstart[k + 1] - acstart[k];
+        }
+        cellk[u] = k;
+        cellpre[u] = sz2;
+      }
+      __syncthreads();
+      {
+        int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
+            v3 = cellpre[4 * t + 3];
+        int tot = v0 + v1 + v2 + v3;
+        __syncthreads();
+        scanbuf[t] = tot;
+        __syncthreads();
+        for (int off = 1; off < ABT; off <<= 1) {
+          int add = (t >= off) ? scanbuf[t - off] : 0;
+          __syncthreads();
+          scanbuf[t] += add;
+          __syncthreads();
+        }
+        int base = scanbuf[t] - tot;
+        cellpre[4 * t] = base;
+        cellpre[4 * t + 1] = base + v0;
+        cellpre[4 * t + 2] = base + v0 + v1;
+        cellpre[4 * t + 3] = base + v0 + v1 + v2;
+        if (t == ABT - 1) cellpre[AB_MAXCELLS] = scanbuf[t];
+      }
+      __syncthreads();
+      const int ncand = cellpre[AB_MAXCELLS];
+      for (int fb = 0; fb < ncand; fb += ABT) {
+        const int f = fb + t;
+        uint64_t m1 = 0, m2 = 0, m3 = 0;
+        int ja = -1;
+        if (f < ncand) {
+          int lo = 0, hi = ncc - 1;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cellpre[mid] <= f) lo = mid; else hi = mid - 1;
+          }
+          ja = aslot2atom[acstart[cellk[lo]] + (f - cellpre[lo])];
+          const double xj = coor[3 * ja], yj = coor[3 * ja + 1], zj = coor[3 * ja + 2];
+          const int jm = mono_of[ja], jf = aflag[ja];
+          const double cjx = mcent[3 * jm], cjy = mcent[3 * jm + 1], cjz = mcent[3 * jm + 2];
+          bool jok = true;
+          if (image && !rev) jok = need_only_ok(cjx, cjy, cjz, da, db, dc, P);
+          if (rev) jok = (jf & AF_WAT) && image_i_ok(cjx, cjy, cjz, da, db, dc, large_cutmono, P);
+          if (jok && (xj < q0[0] || xj > q1[0] || yj < q0[1] || yj > q1[1] || zj < q0[2] ||
+                      zj > q1[2]))
+            jok = false;
+          if (jok) {
+            uint64_t iv = ivalid;
+            while (iv) {
+              const int i = __ffsll((long long)iv) - 1;
+              iv &= iv - 1;
+              const int ai = iatom[i];
+              int cls = 0;
+              if (!image) {
+                // real pair (pot/nbondm.f:176-199, pot/nbond.f:67-149)
+                if (!(jf & AF_WAT) && !(ai < ja)) continue;
+                if (imono[i] != jm) {
+                  if (!(dist2(ic[i][0], ic[i][1], ic[i][2], cjx, cjy, cjz) <= P.cutmono2)) continue;
+                }
+                const int lo2 = min(ai, ja), hi2 = max(ai, ja);
+                if (excluded(lo2, hi2, xmask, xfar, exc1, exc2)) continue;
+                const double r2 = dist2(ix[i][0], ix[i][1], ix[i][2], xj, yj, zj);
+                cls = apair_class(r2, iflag[i], jf, P.hvdw0 != 0, P);
+              } else if (!rev) {
+                // block atom translated (symm/nbmsym.f:197-233, symm/nbsym.f:79-142)
+                if (!(dist2(ic[i][0] + da, ic[i][1] + db, ic[i][2] + dc, cjx, cjy, cjz) < P.cutmono2))
+                  continue;
+                const double r2 = dist2(ix[i][0] + da, ix[i][1] + db, ix[i][2] + dc, xj, yj, zj);
+                cls = apair_class(r2, iflag[i], jf, true, P);
+              } else {
+                // water atom translated, block atom real
+                if (!(dist2(cjx + da, cjy + db, cjz + dc, ic[i][0], ic[i][1], ic[i][2]) < P.cutmono2))
+                  continue;
+                const double r2 = dist2(xj + da, yj + db, zj + dc, ix[i][0], ix[i][1], ix[i][2]);
+                cls = apair_class(r2, jf, iflag[i], true, P);
+              }
+              if (cls == 1) m1 |= 1ull << i;
+              else if (cls == 2) m2 |= 1ull << i;
+              else if (cls == 3) m3 |= 1ull << i;
+            }
+          }
+        }
+        // split into items of <= WSPLIT pairs, bucketed by popcount
+        const uint64_t mm = m1 | m2 | m3;
+        const int pc = __popcll(mm);
+        const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
+        const int bk = rem ? WSPLIT - rem : -1;
+        int incl = nfull;
+        for (int off = 1; off < 64; off <<= 1) {
+          int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
+        }
+        const int wtot0 = __shfl(incl, 63, 64);
+        if (lane == 0) wcnt[wv][0] = wtot0;
+        for (int k = 1; k < NBUCKET; k++) {
+          uint64_t bal = __ballot(bk == k);
+          if (lane == 0) wcnt[wv][k] = __popcll(bal);
+        }
+        __syncthreads();
+        uint64_t rest = mm;
+        if (WRITE && nfull > 0) {
+          int before0 = 0;
+          for (int w2 = 0; w2 < wv; w2++) before0 += wcnt[w2][0];
+          const int pos0 = boff[b * NBUCKET] + bk_run[0] + before0 + (incl - nfull);
+          for (int q = 0; q < nfull; q++) {
+            uint64_t piece = 0;
+            for (int k = 0; k < WSPLIT; k++) {
+              const uint64_t low = rest & (~rest + 1ull);
+              piece |= low;
+              rest ^= low;
+            }
+            AItem it;
+            it.j = ja;
+            it.s = d;
+            it.m1 = m1 & piece;
+            it.m2 = m2 & piece;
+            it.m3 = m3 & piece;
+            items[pos0 + q] = it;
+          }
+        }
+        if (WRITE) {
+          for (int k = 1; k < NBUCKET; k++) {
+            uint64_t bal = __ballot(bk == k);
+            if (bk == k) {
+              int before = 0;
+              for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
+              uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+              const int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
+              AItem it;
+              it.j = ja;
+              it.s = d;
+              it.m1 = m1 & rest;
+              it.m2 = m2 & rest;
+              it.m3 = m3 & rest;
+              items[pos] = it;
+            }
+          }
+        }
+        __syncthreads();
+        if (t < NBUCKET) {
+          int tot = 0;
+          for (int w2 = 0; w2 < ABT / 64; w2++) tot += wcnt[w2][t];
+          bk_run[t] += tot;
+        }
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (!WRITE && t < NBUCKET) bcount[b * NBUCKET + t] = bk_run[t];
+}
+
+__global__ void k_ajcount(const AItem *__restrict__ items, size_t n, int32_t *__restrict__ cnt) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) atomicAdd(&cnt[items[q].j], 1);
+}
+__global__ void k_ajfill(const AItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
+                         int32_t *__restrict__ fill, int32_t *__restrict__ jlist) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int j = items[q].j;
+  jlist[start[j] + atomicAdd(&fill[j], 1)] = (int32_t)q;
+}
+__global__ void k_ajpos(const int32_t *__restrict__ jlist, size_t n, int32_t *__restrict__ jpos) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) jpos[jlist[q]] = (int32_t)q;
+}
+
+// ---------------------------------------------------------------------------
+// real-space energy over the atom items
+struct ARS {
+  double qfac, pick;  // kofdie/eps (or 0), 1 (or 0) for LJ
+  double beta, cutvdw2, cutele2, erftbdns;
+  double a, b, c;
+  double fix_scale;
+};
+
+#define ERFC_P 0.418f
+__device__ __forceinline__ float a_erfc_poly(float t) {
+  float p = 7.663338518e-02f;
+  p = fmaf(p, t, -3.531459250e-01f);
+  p = fmaf(p, t, 5.087678340e-01f);
+  p = fmaf(p, t, -1.952929589e-01f);
+  p = fmaf(p, t, 3.047273108e-01f);
+  p = fmaf(p, t, 1.823138229e-01f);
+  p = fmaf(p, t, 2.404413357e-01f);
+  p = fmaf(p, t, 2.355552142e-01f);
+  return p * t;
+}
+
+__device__ __forceinline__ long long a_fix32(float f) {
+  // round(f 2^32) from FP32 operations (valid for |f| < 2^23)
+  const float x = f * 256.0f;
+  const float hr = rintf(x);
+  const float r = x - hr;
+  return ((long long)(int)hr << 24) + (long long)(int)rintf(r * 16777216.0f);
+}
+
+__device__ inline double a_block_sum(double v, double *red) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__device__ __forceinline__ uint64_t a_rotr64(uint64_t x, int r) {
+  return r ? ((x >> r) | (x << (64 - r))) : x;
+}
+
+template <bool F64>
+__global__ __launch_bounds__(ABT) void k_atoms(
+    const int32_t *__restrict__ chunks, const AItem *__restrict__ items,
+    const double *__restrict__ coor, const int32_t *__restrict__ sslot2atom, int nsol,
+    const double *__restrict__ q, const double *__restrict__ e12, const double *__restrict__ e6,
+    const double *__restrict__ erf_arr, ARS P, const int32_t *__restrict__ jpos,
+    long long *__restrict__ jpart,
+    long long *__restrict__ ipart, double *__restrict__ epart, int max_parts, int part0) {
+  __shared__ double ix[WBLOCK][3];
+  __shared__ double iq[WBLOCK], ia12[WBLOCK], ib6[WBLOCK];
+  __shared__ unsigned long long iacc[WBLOCK * 3];
+  __shared__ double red[4];
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
+  const int s0 = blk * WBLOCK;
+  if (t < WBLOCK) {
+    if (s0 + t < nsol) {
+      const int a = sslot2atom[s0 + t];
+      ix[t][0] = coor[3 * a];
+      ix[t][1] = coor[3 * a + 1];
+      ix[t][2] = coor[3 * a + 2];
+      iq[t] = q[a] * P.qfac;
+      ia12[t] = e12[a] * P.pick;
+      ib6[t] = e6[a] * P.pick;
+    } else {
+      ix[t][0] = ix[t][1] = ix[t][2] = 0.0;
+      iq[t] = ia12[t] = ib6[t] = 0.0;
+    }
+  }
+  for (int u = t; u < WBLOCK * 3; u += ABT) iacc[u] = 0;
+  __syncthreads();
+  const int rot = t & 63;
+  const float betaf = (float)P.beta, c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
+  const float cutvdw2f = (float)P.cutvdw2, cutele2f = (float)P.cutele2;
+  const float log2e = 1.4426950408889634f;
+  const double del = 1.0 / P.erftbdns, ovt = 1.0 / 3.0;
+  double ev_r = 0, ee_r = 0, ev_i = 0, ee_i = 0;
+  for (int it = beg + t; it < end; it += ABT) {
+    const AItem item = items[it];
+    const int ja = item.j;
+    int s;
+    bool rev;
+    dir_of(item.s, s, rev);
+    int sa, sb, sc;
+    symop_of(s, &sa, &sb, &sc);
+    const double T[3] = {sa * P.a, sb * P.b, sc * P.c};
+    double xj[3] = {coor[3 * ja], coor[3 * ja + 1], coor[3 * ja + 2]};
+    // reverse images: the partner is the translated member, x_j + T
+    if (rev)
+      for (int k = 0; k < 3; k++) xj[k] = xj[k] + T[k];
+    const double qj = q[ja], aj = e12[ja], bj = e6[ja];
+    double fj[3] = {0, 0, 0};
+    double ev = 0, ee = 0;
+    const uint64_t m2 = item.m2, m3 = item.m3;
+    uint64_t mask = a_rotr64(item.m1 | m2 | m3, rot);
+    while (mask) {
+      const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
+      mask &= mask - 1;
+      const int kind = ((m2 >> i) & 1ull) ? 2 : (((m3 >> i) & 1ull) ? 3 : 1);
+      // d = x_l - x_m with the translation on the image member
+      double d[3];
+      if (s > 0 && !rev) {
+#pragma clang fp contract(off)
+        for (int k = 0; k < 3; k++) d[k] = (ix[i][k] + T[k]) - xj[k];
+      } else {
+        for (int k = 0; k < 3; k++) d[k] = ix[i][k] - xj[k];
+      }
+      const double qq = iq[i] * qj, A = ia12[i] * aj, B = ib6[i] * bj;
+      double df = 0.0, e_v = 0.0, e_e = 0.0;
+      if (F64) {
+#pragma clang fp contract(off)
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double s2 = 1.0 / r2;
+        const bool lj = (kind != 3) && r2 <= P.cutvdw2;
+        const bool el = (kind == 1) || (kind == 3 && r2 <= P.cutele2);
+        if (kind == 2 && lj) {
+          const double s6 = s2 * s2 * s2, A12 = A * s6 * s6, B6 = B * s6;
+          e_v = A12 - B6;
+          df = -6.0 * s2 * (A12 + e_v);
+        } else if (el) {
+          const double rij = sqrt(r2), sv = 1.0 / rij;
+          const double xerfc = P.beta * rij;
+          const int idx = (int)(P.erftbdns * xerfc);
+          const double dx = xerfc - idx * del;
+          const double *tb = erf_arr + 4 * idx;
+          const double derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
+          const double erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * ovt)) + tb[0];
+          e_e = qq * sv * erfcc;
+          df = -qq * sv * (s2 * erfcc + sv * P.beta * derfc);
+          if (kind == 1 && lj) {
+            const double s6 = s2 * s2 * s2, A12 = A * s6 * s6, B6 = B * s6;
+            e_v = A12 - B6;
+            df += -6.0 * s2 * (A12 + e_v);
+          }
+        }
+      } else {
+        const float dx = (float)d[0], dy = (float)d[1], dz = (float)d[2];
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float sv = __builtin_amdgcn_rsqf(r2);
+        const float s2 = sv * sv;
+        const bool lj = (kind != 3) && r2 <= cutvdw2f;
+        const bool el = (kind == 1) || (kind == 3 && r2 <= cutele2f);
+        float dff = 0.f;
+        if (el) {
+          const float qqf = (float)qq;
+          const float x = betaf * (r2 * sv);
+          const float tt = __builtin_amdgcn_rcpf(fmaf(ERFC_P, x, 1.0f));
+          const float ex = __builtin_amdgcn_exp2f(-(x * x) * log2e);
+          const float e = qqf * (ex * a_erfc_poly(tt)) * sv;
+          e_e = (double)e;
+          dff = -s2 * (e + qqf * c2b * ex);
+        }
+        if (lj) {
+          const float s6 = s2 * s2 * s2;
+          const float A12 = (float)A * s6 * s6, B6 = (float)B * s6;
+          const float e1 = A12 - B6;
+          e_v = (double)e1;
+          dff += -6.0f * s2 * (A12 + e1);
+        }
+        df = (double)dff;
+      }
+      ev += e_v;
+      ee += e_e;
+      // gradient: +df d on the block atom, -df d on the partner
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double g = df * d[k];
+        fj[k] -= g;
+        const long long v = F64 ? __double2ll_rn(g * P.fix_scale) : a_fix32((float)g);
+        if (v) atomicAdd(&iacc[i * 3 + k], (unsigned long long)v);
+      }
+    }
+    long long *jrow = jpart + (size_t)jpos[it] * 3;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      jrow[k] = F64 ? __double2ll_rn(fj[k] * P.fix_scale) : a_fix32((float)fj[k]);
+    if (s == 0) {
+      ev_r += ev;
+      ee_r += ee;
+    } else {
+      ev_i += ev;
+      ee_i += ee;
+    }
+  }
+  __syncthreads();
+  for (int u = t; u < WBLOCK * 3; u += ABT) ipart[(size_t)ch * (WBLOCK * 3) + u] = (long long)iacc[u];
+  double v;
+  v = a_block_sum(ev_r, red);
+  if (t == 0) epart[EN_VDW * max_parts + part0 + ch] = v;
+  v = a_block_sum(ee_r, red);
+  if (t == 0) epart[EN_EL * max_parts + part0 + ch] = v;
+  v = a_block_sum(ev_i, red);
+  if (t == 0) epart[EN_VSYM * max_parts + part0 + ch] = v;
+  v = a_block_sum(ee_i, red);
+  if (t == 0) epart[EN_LSYM * max_parts + part0 + ch] = v;
+}
+
+// one thread per (atom, component): the atom's partner rows (contiguous,
+// in item order) and, for a block atom, its entries in its block's chunk
+// rows, added to its fixed-point accumulator
+__global__ __launch_bounds__(256) void k_atoms_reduce(
+    const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
+    const int32_t *__restrict__ slot_of, const int32_t *__restrict__ cfirst,
+    const long long *__restrict__ ipart, int npt, long long *__restrict__ facc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 3 * npt) return;
+  const int a = g / 3, k = g - 3 * a;
+  long long acc = 0;
+  const int q1 = jstart[a + 1];
+  for (int q = jstart[a]; q < q1; q++) acc += jpart[(size_t)q * 3 + k];
+  const int s = slot_of[a];
+  if (s >= 0) {
+    const int b = s / WBLOCK, i = s % WBLOCK;
+    for (int c = cfirst[b]; c < cfirst[b + 1]; c++) acc += ipart[(size_t)c * (WBLOCK * 3) + i * 3 + k];
+  }
+  if (acc) facc[3 * (size_t)a + k] += acc;
+}
+
+// ---------------------------------------------------------------------------
+int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
+  const moil_params &P = c->par;
+  std::vector<int32_t> mono_of(c->npt), aflag(c->npt, 0);
+  std::vector<uint64_t> xmask(c->npt, 0);
+  std::vector<int32_t> xfar(c->npt, 0);
+  for (int m = 0; m < t->totmon; m++)
+    for (int a = t->poipt[m]; a < t->poipt[m + 1]; a++) {
+      mono_of[a] = m;
+      if (t->monoflag[m] & MOIL_MONO_WATER) aflag[a] |= AF_WAT;
+      else c->h_sol.push_back(a);
+    }
+  for (int a = 0; a < c->npt; a++) {
+    if (t->flagchr[a]) aflag[a] |= AF_CHG;
+    if (t->epsgm12[a] < 1e-3) aflag[a] |= AF_SMALL;
+    for (int k = t->exc1[a]; k < t->exc1[a + 1]; k++) {
+      const int m = t->exc2[k] - 1;
+      if (m > a && m - a <= 64) xmask[a] |= 1ull << (m - a - 1);
+      else xfar[a] = 1;
+    }
+  }
+  c->nsol = (int)c->h_sol.size();
+  HIP_CHECK(hipMalloc(&c->d_mono_of, sizeof(int32_t) * std::max(c->npt, 1)));
+  HIP_CHECK(hipMemcpy(c->d_mono_of, mono_of.data(), sizeof(int32_t) * c->npt, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMalloc(&c->d_aflag, sizeof(int32_t) * std::max(c->npt, 1)));
+  HIP_CHECK(hipMemcpy(c->d_aflag, aflag.data(), sizeof(int32_t) * c->npt, hipMemcpyHostToDevice));
+  if (c->nsol == 0) return MOIL_GPU_OK;
+  HIP_CHECK(hipMalloc(&c->d_xmask, sizeof(uint64_t) * c->npt));
+  HIP_CHECK(hipMemcpy(c->d_xmask, xmask.data(), sizeof(uint64_t) * c->npt, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMalloc(&c->d_xfar, sizeof(int32_t) * c->npt));
+  HIP_CHECK(hipMemcpy(c->d_xfar, xfar.data(), sizeof(int32_t) * c->npt, hipMemcpyHostToDevice));
+  const int nexc = t->exc1[c->npt];
+  HIP_CHECK(hipMalloc(&c->d_exc1, sizeof(int32_t) * (c->npt + 1)));
+  HIP_CHECK(hipMemcpy(c->d_exc1, t->exc1, sizeof(int32_t) * (c->npt + 1), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMalloc(&c->d_exc2, sizeof(int32_t) * std::max(nexc, 1)));
+  if (nexc) HIP_CHECK(hipMemcpy(c->d_exc2, t->exc2, sizeof(int32_t) * nexc, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMalloc(&c->d_sol, sizeof(int32_t) * c->nsol));
+  HIP_CHECK(hipMemcpy(c->d_sol, c->h_sol.data(), sizeof(int32_t) * c->nsol, hipMemcpyHostToDevice));
+  // all-atom cell grid, ~8 atoms per cell
+  const double L[3] = {P.a, P.b, P.c};
+  const double edge = cbrt(P.a * P.b * P.c * 8.0 / (double)c->npt);
+  for (int k = 0; k < 3; k++) {
+    c->ag[k] = std::max(1, (int)floor(L[k] / edge));
+    c->acs[k] = L[k] / c->ag[k];
+  }
+  c->ancell = ((c->ag[0] + 1) / 2) * ((c->ag[1] + 1) / 2) * ((c->ag[2] + 1) / 2) * 8;
+  HIP_CHECK(hipMalloc(&c->d_acell_of, sizeof(int32_t) * c->npt));
+  HIP_CHECK(hipMalloc(&c->d_acnt, sizeof(int32_t) * (c->ancell + 1)));
+  HIP_CHECK(hipMalloc(&c->d_astart, sizeof(int32_t) * (c->ancell + 1)));
+  HIP_CHECK(hipMalloc(&c->d_aslot2atom, sizeof(int32_t) * c->npt));
+  HIP_CHECK(hipMalloc(&c->d_sstart, sizeof(int32_t) * (c->ancell + 1)));
+  HIP_CHECK(hipMalloc(&c->d_sslot2atom, sizeof(int32_t) * c->nsol));
+  HIP_CHECK(hipMalloc(&c->d_sslot_of, sizeof(int32_t) * c->npt));
+  c->nablk = (c->nsol + WBLOCK - 1) / WBLOCK;
+  HIP_CHECK(hipMalloc(&c->d_abcount, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
+  HIP_CHECK(hipMalloc(&c->d_aboff, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
+  HIP_CHECK(hipMalloc(&c->d_ajcnt, sizeof(int32_t) * (c->npt + 1)));
+  HIP_CHECK(hipMalloc(&c->d_ajstart, sizeof(int32_t) * (c->npt + 1)));
+  HIP_CHECK(hipMalloc(&c->d_acfirst, sizeof(int32_t) * (c->nablk + 1)));
+  return MOIL_GPU_OK;
+}
+
+void atoms_free(moil_ctx *c) {
+  void *ptrs[] = {c->d_mono_of, c->d_aflag, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2,
+                  c->d_sol, c->d_acell_of, c->d_acnt, c->d_astart, c->d_aslot2atom,
+                  c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
+                  c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajstart,
+                  c->d_ajlist, c->d_ajpos, c->d_acfirst};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+}
+
+int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
+  c->naitems = 0;
+  c->nachunk = 0;
+  if (c->nsol == 0) return MOIL_GPU_OK;
+  const moil_params &P = c->par;
+  const int TB = 256;
+  Grid g;
+  g.gx = c->ag[0]; g.gy = c->ag[1]; g.gz = c->ag[2];
+  g.nbx = (g.gx + 1) / 2; g.nby = (g.gy + 1) / 2; g.nbz = (g.gz + 1) / 2;
+  g.g0x = -0.5 * P.a; g.g0y = -0.5 * P.b; g.g0z = -0.5 * P.c;
+  g.csx = c->acs[0]; g.csy = c->acs[1]; g.csz = c->acs[2];
+  // all atoms by cell (partner side)
+  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+  k_atom_cells<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(d_coor, nullptr, c->npt, g, c->d_acell_of,
+                                                      c->d_acnt);
+  k_scan_excl<<<1, 1024, 0, st>>>(c->d_acnt, c->d_astart, c->ancell);
+  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+  k_atom_scatter<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, nullptr, c->npt,
+                                                        c->d_astart, c->d_acnt, c->d_aslot2atom);
+  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_astart, c->ancell, c->d_aslot2atom);
+  // solute atoms by cell (block side)
+  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+  k_atom_cells<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_sol, c->nsol, g, c->d_acell_of,
+                                                       c->d_acnt);
+  k_scan_excl<<<1, 1024, 0, st>>>(c->d_acnt, c->d_sstart, c->ancell);
+  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+  k_atom_scatter<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, c->d_sol, c->nsol,
+                                                         c->d_sstart, c->d_acnt, c->d_sslot2atom);
+  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->ancell, c->d_sslot2atom);
+  HIP_CHECK(hipMemsetAsync(c->d_sslot_of, 0xff, sizeof(int32_t) * c->npt, st));
+  k_slot_of<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, c->nsol, c->d_sslot_of);
+
+  APar ap;
+  ap.a = P.a; ap.b = P.b; ap.c = P.c;
+  ap.ha = 0.5 * P.a; ap.hb = 0.5 * P.b; ap.hc = 0.5 * P.c;
+  ap.cutmono2 = P.cutmono2; ap.cutvbig2 = P.cutvbig2; ap.cutebig2 = P.cutebig2;
+  ap.R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0)) * (1.0 + 1e-12) + 1e-9;
+  ap.hvdw0 = (P.terms & MOIL_TERM_HVDW0) ? 1 : 0;
+  ap.nsol = c->nsol;
+  k_ablock_items<false><<<c->nablk, ABT, 0, st>>>(
+      d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
+      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
+      nullptr, nullptr);
+  k_scan_excl<<<1, 1024, 0, st>>>(c->d_abcount, c->d_aboff, c->nablk * NBUCKET);
+  c->h_aboff.assign(c->nablk * NBUCKET + 1, 0);
+  HIP_CHECK(hipMemcpyAsync(c->h_aboff.data(), c->d_aboff, sizeof(int32_t) * c->h_aboff.size(),
+                           hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  const std::vector<int32_t> &hoff = c->h_aboff;
+  const size_t total = (size_t)hoff.back();
+  if (total > c->aitem_cap || c->d_aitems == nullptr) {
+    hipFree(c->d_aitems);
+    hipFree(c->d_ajlist);
+    hipFree(c->d_ajpos);
+    hipFree(c->d_ajpart);
+    c->aitem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
+    HIP_CHECK(hipMalloc(&c->d_aitems, sizeof(AItem) * c->aitem_cap));
+    HIP_CHECK(hipMalloc(&c->d_ajlist, sizeof(int32_t) * c->aitem_cap));
+    HIP_CHECK(hipMalloc(&c->d_ajpos, sizeof(int32_t) * c->aitem_cap));
+    HIP_CHECK(hipMalloc(&c->d_ajpart, sizeof(long long) * 3 * c->aitem_cap));
+  }
+  c->naitems = total;
+  k_ablock_items<true><<<c->nablk, ABT, 0, st>>>(
+      d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
+      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, nullptr,
+      c->d_aboff, c->d_aitems);
+  // chunk table by pair budget (items are popcount-descending per block)
+  std::vector<int32_t> ch;
+  for (int b = 0; b < c->nablk; b++) {
+    int q = hoff[b * NBUCKET];
+    const int end = hoff[(b + 1) * NBUCKET];
+    while (q < end) {
+      int e = q;
+      long pairs = 0;
+      while (e < end && pairs < c->wchunk_pairs && e - q < c->wchunk_max_items) {
+        int k = (int)(std::upper_bound(hoff.begin() + b * NBUCKET, hoff.begin() + (b + 1) * NBUCKET, e) -
+                      (hoff.begin() + b * NBUCKET)) - 1;
+        const int pc = WSPLIT - k;
+        const int nxt = std::min(hoff[b * NBUCKET + k + 1], end);
+        const long want = std::max<long>(1, (c->wchunk_pairs - pairs + pc - 1) / pc);
+        int take = (int)std::min<long>(nxt - e, want);
+        take = std::min(take, c->wchunk_max_items - (e - q));
+        pairs += (long)take * pc;
+        e += take;
+      }
+      ch.push_back(b);
+      ch.push_back(q);
+      ch.push_back(e);
+      q = e;
+    }
+  }
+  c->nachunk = (int)ch.size() / 3;
+  hipFree(c->d_achunk);
+  c->d_achunk = nullptr;
+  std::vector<int32_t> cfirst(c->nablk + 1, c->nachunk);
+  for (int q = c->nachunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
+  for (int b = c->nablk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
+  if (c->nachunk > 0) {
+    HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * ch.size()));
+    HIP_CHECK(hipMemcpyAsync(c->d_achunk, ch.data(), sizeof(int32_t) * ch.size(),
+                             hipMemcpyHostToDevice, st));
+  }
+  if (c->nachunk > c->aipart_cap || c->d_aipart == nullptr) {
+    hipFree(c->d_aipart);
+    c->aipart_cap = std::max(c->nachunk + c->nachunk / 4 + 64, 256);
+    HIP_CHECK(hipMalloc(&c->d_aipart, sizeof(long long) * WBLOCK * 3 * (size_t)c->aipart_cap));
+  }
+  HIP_CHECK(hipMemcpyAsync(c->d_acfirst, cfirst.data(), sizeof(int32_t) * cfirst.size(),
+                           hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
+  if (total > 0) {
+    const int ib = (int)((total + TB - 1) / TB);
+    k_ajcount<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajcnt);
+    k_scan_excl<<<1, 1024, 0, st>>>(c->d_ajcnt, c->d_ajstart, c->npt);
+    HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
+    k_ajfill<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajstart, c->d_ajcnt, c->d_ajlist);
+    k_cell_sort<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_ajstart, c->npt, c->d_ajlist);
+    k_ajpos<<<ib, TB, 0, st>>>(c->d_ajlist, total, c->d_ajpos);
+  } else {
+    HIP_CHECK(hipMemsetAsync(c->d_ajstart, 0, sizeof(int32_t) * (c->npt + 1), st));
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+  return MOIL_GPU_OK;
+}
+
+int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0) {
+  if (c->nachunk == 0) return MOIL_GPU_OK;
+  ARS P;
+  const uint32_t T = c->par.terms;
+  P.qfac = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;
+  P.pick = (T & MOIL_TERM_VDW) ? 1.0 : 0.0;
+  P.beta = c->beta;
+  P.cutvdw2 = c->par.cutvdw2;
+  P.cutele2 = c->par.cutele2;
+  P.erftbdns = c->erftbdns;
+  P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
+  P.fix_scale = c->fix_scale;
+  if (c->par.precision == MOIL_PREC_MIXED)
+    k_atoms<false><<<c->nachunk, ABT, 0, st>>>(c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom,
+                                               c->nsol, c->d_ptchg, c->d_eps12, c->d_eps6, c->d_erf,
+                                               P, c->d_ajpos, c->d_ajpart, c->d_aipart,
+                                               c->d_epart, c->max_parts, part0);
+  else
+    k_atoms<true><<<c->nachunk, ABT, 0, st>>>(c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom,
+                                              c->nsol, c->d_ptchg, c->d_eps12, c->d_eps6, c->d_erf,
+                                              P, c->d_ajpos, c->d_ajpart, c->d_aipart,
+                                              c->d_epart, c->max_parts, part0);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+int32_t realspace_atoms_reduce(moil_ctx *c, hipStream_t st) {
+  if (c->nachunk == 0) return MOIL_GPU_OK;
+  k_atoms_reduce<<<(3 * c->npt + 255) / 256, 256, 0, st>>>(c->d_ajstart, c->d_ajpart, c->d_sslot_of,
+                                                            c->d_acfirst, c->d_aipart, c->npt,
+                                                            c->d_facc);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}

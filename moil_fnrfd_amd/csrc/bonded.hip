@@ -10,8 +10,14 @@
 //   ener14  pot/ener14.f:38-95    p14 LJ + plain Coulomb
 //   e_corr  pot/cdie_ewald.f:83-199  -q_i q_j erf(beta r)/r over the
 //                                  exclusion list (atoms 1..npt-1)
-// One thread per term; gradients go to the 64-bit fixed-point
-// accumulators with integer atomics, energies to per-workgroup partials.
+// Intra-water terms (bonds, angle, exclusion corrections inside a 3-site
+// water monomer) run one thread per water: the thread owns the water's
+// three atoms, sums their per-term fixed-point gradients in registers and
+// stores them (this kernel runs first in an evaluation, on the cleared
+// accumulators).  All other terms run one thread per term with integer
+// atomics.  Each term's gradient is rounded to fixed point on its own in
+// both kernels, so the integer sums do not depend on the grouping.
+// Energies go to per-workgroup partials.
 #include <hip/hip_runtime.h>
 
 #include "ctx.h"
@@ -37,6 +43,29 @@ __device__ __forceinline__ void add3(long long *facc, int atom, double fx, doubl
   v = __double2ll_rn(fz * s);
   if (v) atomicAdd((unsigned long long *)&facc[3 * atom + 2], (unsigned long long)v);
 }
+
+// gradient sinks of the term functions: global atomics, or the registers
+// of a thread that owns atoms a0 .. a0+2
+struct AtomicSink {
+  long long *facc;
+  double s;
+  __device__ void add(int atom, double fx, double fy, double fz) { add3(facc, atom, fx, fy, fz, s); }
+};
+struct WaterSink {
+  long long acc[9];
+  int a0;
+  double s;
+  __device__ void add(int atom, double fx, double fy, double fz) {
+    const int l = atom - a0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (k == l) {
+        acc[3 * k] += __double2ll_rn(fx * s);
+        acc[3 * k + 1] += __double2ll_rn(fy * s);
+        acc[3 * k + 2] += __double2ll_rn(fz * s);
+      }
+  }
+};
 
 #define XYZ(a) coor[3 * (a)], coor[3 * (a) + 1], coor[3 * (a) + 2]
 
@@ -91,23 +120,29 @@ struct BSeg {
   const double *bp, *ap, *tp, *ip, *p14, *q, *erf;
 };
 
-__device__ inline void term_bond(const double *coor, const BSeg &S, int m, const BPar &P,
-                                 long long *facc, double &e) {
-  int i = S.ib[2 * m] - 1, j = S.ib[2 * m + 1] - 1;
+template <class Sink>
+__device__ inline void bond_calc(const double *coor, int i, int j, double kb, double r0, Sink &sk,
+                                 double &e) {
   double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
          rz = coor[3 * i + 2] - coor[3 * j + 2];
   double s = sqrt(rx * rx + ry * ry + rz * rz);
-  double db = s - S.bp[2 * m + 1];
-  double df = S.bp[2 * m] * db;
+  double db = s - r0;
+  double df = kb * db;
   e = df * db;
   df = df * (2.0 / s);
-  add3(facc, i, rx * df, ry * df, rz * df, P.fix_scale);
-  add3(facc, j, -rx * df, -ry * df, -rz * df, P.fix_scale);
+  sk.add(i, rx * df, ry * df, rz * df);
+  sk.add(j, -rx * df, -ry * df, -rz * df);
 }
 
-__device__ inline void term_angle(const double *coor, const BSeg &S, int m, const BPar &P,
-                                  long long *facc, double &e) {
-  int i = S.ia[3 * m] - 1, j = S.ia[3 * m + 1] - 1, k = S.ia[3 * m + 2] - 1;
+__device__ inline void term_bond(const double *coor, const BSeg &S, int m, const BPar &P,
+                                 long long *facc, double &e) {
+  AtomicSink sk{facc, P.fix_scale};
+  bond_calc(coor, S.ib[2 * m] - 1, S.ib[2 * m + 1] - 1, S.bp[2 * m], S.bp[2 * m + 1], sk, e);
+}
+
+template <class Sink>
+__device__ inline void angle_calc(const double *coor, int i, int j, int k, double ka, double th0,
+                                  Sink &sk, double &e) {
   double di[3], dj[3];
   for (int q = 0; q < 3; q++) {
     di[q] = coor[3 * i + q] - coor[3 * j + q];
@@ -123,8 +158,8 @@ __device__ inline void term_angle(const double *coor, const BSeg &S, int m, cons
   double cst = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
   cst = fmin(1.0, fmax(-1.0, cst));
   double at = acos(cst);
-  double da = at - S.ap[2 * m + 1];
-  double df = S.ap[2 * m] * da;
+  double da = at - th0;
+  double df = ka * da;
   e = df * da;
   df = df + df;
   double sn = sin(at);
@@ -136,9 +171,16 @@ __device__ inline void term_angle(const double *coor, const BSeg &S, int m, cons
     t1[q] = rir * (dj[q] - cst * di[q]);
     t2[q] = rjr * (di[q] - cst * dj[q]);
   }
-  add3(facc, i, df * t1[0], df * t1[1], df * t1[2], P.fix_scale);
-  add3(facc, j, -df * (t1[0] + t2[0]), -df * (t1[1] + t2[1]), -df * (t1[2] + t2[2]), P.fix_scale);
-  add3(facc, k, df * t2[0], df * t2[1], df * t2[2], P.fix_scale);
+  sk.add(i, df * t1[0], df * t1[1], df * t1[2]);
+  sk.add(j, -df * (t1[0] + t2[0]), -df * (t1[1] + t2[1]), -df * (t1[2] + t2[2]));
+  sk.add(k, df * t2[0], df * t2[1], df * t2[2]);
+}
+
+__device__ inline void term_angle(const double *coor, const BSeg &S, int m, const BPar &P,
+                                  long long *facc, double &e) {
+  AtomicSink sk{facc, P.fix_scale};
+  angle_calc(coor, S.ia[3 * m] - 1, S.ia[3 * m + 1] - 1, S.ia[3 * m + 2] - 1, S.ap[2 * m],
+             S.ap[2 * m + 1], sk, e);
 }
 
 __device__ inline void term_tors(const double *coor, const BSeg &S, int m, const BPar &P,
@@ -183,24 +225,30 @@ __device__ inline void term_imp(const double *coor, const BSeg &S, int m, const 
   for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
 }
 
-__device__ inline void term_excl(const double *coor, const BSeg &S, int m, const BPar &P,
-                                 long long *facc, double &e) {
-  int i = S.ei[m], j = S.ej[m];
+template <class Sink>
+__device__ inline void excl_calc(const double *coor, int i, int j, const double *q,
+                                 const double *erf_arr, const BPar &P, Sink &sk, double &e) {
   double rx = coor[3 * i] - coor[3 * j], ry = coor[3 * i + 1] - coor[3 * j + 1],
          rz = coor[3 * i + 2] - coor[3 * j + 2];
   double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, rij = sqrt(r2), s = 1.0 / rij;
-  double qq = S.q[i] * P.qfac * S.q[j];
+  double qq = q[i] * P.qfac * q[j];
   double xerfc = P.beta * rij;
   int idx = (int)(P.erftbdns * xerfc);
   double dx = xerfc - idx * (1.0 / P.erftbdns);
-  const double *t = S.erf + 4 * idx;
+  const double *t = erf_arr + 4 * idx;
   double derfc = -t[1] - dx * (t[2] + 0.5 * dx * t[3]);
   double erfcc = dx * (t[1] + 0.5 * dx * (t[2] + dx * t[3] * (1.0 / 3.0))) + t[0];
   double erf = 1.0 - erfcc;
   e = -qq * s * erf;
   double df = qq * s * (s2 * erf - s * P.beta * derfc);
-  add3(facc, i, df * rx, df * ry, df * rz, P.fix_scale);
-  add3(facc, j, -df * rx, -df * ry, -df * rz, P.fix_scale);
+  sk.add(i, df * rx, df * ry, df * rz);
+  sk.add(j, -df * rx, -df * ry, -df * rz);
+}
+
+__device__ inline void term_excl(const double *coor, const BSeg &S, int m, const BPar &P,
+                                 long long *facc, double &e) {
+  AtomicSink sk{facc, P.fix_scale};
+  excl_calc(coor, S.ei[m], S.ej[m], S.q, S.erf, P, sk, e);
 }
 
 __device__ inline void term_14(const double *coor, const BSeg &S, int m, const BPar &P,
@@ -225,6 +273,37 @@ __device__ inline void term_14(const double *coor, const BSeg &S, int m, const B
   add3(facc, k, -df * rx, -df * ry, -df * rz, P.fix_scale);
 }
 
+// deterministic workgroup sums of N per-thread values (wavefront shuffle
+// tree, then the BT/64 wavefront totals in order); result in red[0..N-1]
+template <int N, int NT = BT>
+__device__ inline void block_sums(const double *v, double *red) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[q * (NT / 64) + wv] = x;
+  }
+  __syncthreads();
+  double tot = 0.0;
+  if (t < N)
+    for (int k = 0; k < NT / 64; k++) tot += red[t * (NT / 64) + k];
+  __syncthreads();
+  if (t < N) red[t] = tot;
+  __syncthreads();
+}
+
+#define WIT 64  // threads per workgroup of the per-water kernel
+
+struct WIntra {
+  int nwat;
+  const int32_t *first;                 // [nwat] first atom (0-based)
+  const int32_t *boff, *aoff, *eoff;    // [nwat+1] CSR offsets
+  const int32_t *bidx, *aidx, *eidx;    // local atom numbers 0..2
+  const double *bpar, *apar;            // k, r0 / k, theta0
+  const double *q, *erf;
+};
+
 // all O(N) terms in one launch over a flat index space
 // [bonds | angles | torsions | impropers | exclusions | 1-4 pairs]
 __global__ __launch_bounds__(BT) void k_bonded(const double *__restrict__ coor, BSeg S, BPar P,
@@ -242,29 +321,58 @@ __global__ __launch_bounds__(BT) void k_bonded(const double *__restrict__ coor, 
   else if (m < o5) term_excl(coor, S, m - o4, P, facc, e[4]);
   else if (m < o6) term_14(coor, S, m - o5, P, facc, e[5], e[6]);
   const int slot[7] = {EN_BOND, EN_THETA, EN_TORS, EN_IMP, EN_CORR, EN_VDW14, EN_EL14};
-  int t = threadIdx.x;
+  block_sums<7>(e, red);
+  if (threadIdx.x < 7) ep[(size_t)slot[threadIdx.x] * mp + blockIdx.x] = red[threadIdx.x];
+}
+
+// one thread per 3-site water: its intra-water bonds, angles and
+// exclusion corrections (terms listed per water in CSR form, local atom
+// numbers 0..2); the water's gradients are stored, not added
+__global__ __launch_bounds__(WIT) void k_water_intra(const double *__restrict__ coor, WIntra W,
+                                                     BPar P, long long *__restrict__ facc,
+                                                     double *__restrict__ ep, int mp, int part0) {
+  __shared__ double red[WIT];
+  const int w = blockIdx.x * WIT + threadIdx.x;
+  double e[3] = {0, 0, 0};  // bond theta corr
+  if (w < W.nwat) {
+    const int a0 = W.first[w];
+    WaterSink sk;
+    sk.a0 = a0;
+    sk.s = P.fix_scale;
 #pragma unroll
-  for (int q = 0; q < 7; q++) {
-    red[t] = e[q];
-    __syncthreads();
-    for (int off = BT / 2; off > 0; off >>= 1) {
-      if (t < off) red[t] += red[t + off];
-      __syncthreads();
+    for (int k = 0; k < 9; k++) sk.acc[k] = 0;
+    double et;
+    for (int m = W.boff[w]; m < W.boff[w + 1]; m++) {
+      bond_calc(coor, a0 + W.bidx[2 * m], a0 + W.bidx[2 * m + 1], W.bpar[2 * m],
+                W.bpar[2 * m + 1], sk, et);
+      e[0] += et;
     }
-    if (t == 0) ep[(size_t)slot[q] * mp + blockIdx.x] = red[0];
-    __syncthreads();
+    for (int m = W.aoff[w]; m < W.aoff[w + 1]; m++) {
+      angle_calc(coor, a0 + W.aidx[3 * m], a0 + W.aidx[3 * m + 1], a0 + W.aidx[3 * m + 2],
+                 W.apar[2 * m], W.apar[2 * m + 1], sk, et);
+      e[1] += et;
+    }
+    for (int m = W.eoff[w]; m < W.eoff[w + 1]; m++) {
+      excl_calc(coor, a0 + W.eidx[2 * m], a0 + W.eidx[2 * m + 1], W.q, W.erf, P, sk, et);
+      e[2] += et;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) facc[3 * a0 + k] = sk.acc[k];
   }
+  block_sums<3, WIT>(e, red);
+  const int slot[3] = {EN_BOND, EN_THETA, EN_CORR};
+  if (threadIdx.x < 3) ep[(size_t)slot[threadIdx.x] * mp + part0 + blockIdx.x] = red[threadIdx.x];
 }
 
 static BSeg make_seg(const moil_ctx *c) {
   const uint32_t T = c->par.terms;
   BSeg S;
   const bool nbon = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
-  S.nb = (T & MOIL_TERM_BOND) ? c->nb : 0;
-  S.nangl = (T & MOIL_TERM_ANGLE) ? c->nangl : 0;
+  S.nb = (T & MOIL_TERM_BOND) ? c->gnb : 0;
+  S.nangl = (T & MOIL_TERM_ANGLE) ? c->gnangl : 0;
   S.ntors = (T & MOIL_TERM_TORS) ? c->ntors : 0;
   S.nimp = (T & MOIL_TERM_IMP) ? c->nimp : 0;
-  S.nexcl = nbon ? c->nexcl : 0;
+  S.nexcl = nbon ? c->gnexcl : 0;
   S.n14 = (nbon && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14))) ? c->totspe : 0;
   S.ib = c->d_ib; S.ia = c->d_ia; S.it = c->d_it; S.ii = c->d_ii;
   S.ei = c->d_exc_i; S.ej = c->d_exc_j; S.sp = c->d_sp;
@@ -277,6 +385,14 @@ int bonded_blocks(const moil_ctx *c) {
   BSeg S = make_seg(c);
   int n = S.nb + S.nangl + S.ntors + S.nimp + S.nexcl + S.n14;
   return (n + BT - 1) / BT;
+}
+
+static bool water_intra_on(const moil_ctx *c) {
+  return c->nwat > 0 && c->wi_nterms > 0;
+}
+
+int water_intra_blocks(const moil_ctx *c) {
+  return water_intra_on(c) ? (c->nwat + WIT - 1) / WIT : 0;
 }
 
 int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
@@ -293,10 +409,26 @@ int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.max_parts = c->max_parts;
   BSeg S = make_seg(c);
   int nblk = bonded_blocks(c);
+  // intra-water terms first: they store into the cleared accumulators
+  if (water_intra_on(c)) {
+    const uint32_t T2 = c->par.terms;
+    const bool nbon = (T2 & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
+    WIntra W;
+    W.nwat = c->nwat;
+    W.first = c->d_wat_o;
+    W.boff = (T2 & MOIL_TERM_BOND) ? c->d_wi_boff : c->d_wi_zero;
+    W.aoff = (T2 & MOIL_TERM_ANGLE) ? c->d_wi_aoff : c->d_wi_zero;
+    W.eoff = nbon ? c->d_wi_eoff : c->d_wi_zero;
+    W.bidx = c->d_wi_bidx; W.aidx = c->d_wi_aidx; W.eidx = c->d_wi_eidx;
+    W.bpar = c->d_wi_bpar; W.apar = c->d_wi_apar;
+    W.q = c->d_ptchg; W.erf = c->d_erf;
+    k_water_intra<<<water_intra_blocks(c), WIT, 0, st>>>(d_coor, W, P, c->d_facc, c->d_epart,
+                                                         c->max_parts, nblk);
+  }
   if (nblk > 0)
     k_bonded<<<nblk, BT, 0, st>>>(d_coor, S, P, c->d_facc, c->d_epart, c->max_parts);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
 
-int bonded_max_blocks(const moil_ctx *c) { return max(1, bonded_blocks(c)); }
+int bonded_max_blocks(const moil_ctx *c) { return max(1, bonded_blocks(c) + water_intra_blocks(c)); }

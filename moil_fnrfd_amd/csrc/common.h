@@ -12,8 +12,12 @@
 
 // Water-block geometry: 64 waters (one wavefront's worth of mask bits).
 #define WBLOCK 64
-// Popcount buckets used to order a block's items (descending popcount).
-#define NBUCKET 8
+// A (j, s) entry of a block is split into items of at most WSPLIT pairs
+// (consecutive mask bits), so one lane's loop is at most WSPLIT pairs
+// long; a block's items are grouped by exact popcount, descending
+// (bucket k holds the items with popcount WSPLIT - k).
+#define WSPLIT 16
+#define NBUCKET WSPLIT
 
 // Fixed-point force accumulators: int64 with FIX_SHIFT fractional bits.
 #define FIX_SHIFT_FP64 40
@@ -37,14 +41,16 @@ struct __attribute__((aligned(32))) WItem {
   uint64_t pad;
 };
 
-// Atom-pair work item (same scheme over 64-atom blocks); l1/l2/l3 are
-// MOIL's list1/list2/list3 memberships.
+// Atom-pair work item over a 64-atom solute block: partner atom `j`
+// (0-based), direction `s` (0 real, 1..13 the block atom is the image
+// under symop s, 14..26 the partner is the image under symop s-13), and
+// the block atoms' memberships in MOIL's list1/list2/list3.
 struct __attribute__((aligned(32))) AItem {
   int32_t j;
   int32_t s;
-  uint64_t l1;
-  uint64_t l2;
-  uint64_t l3;
+  uint64_t m1;
+  uint64_t m2;
+  uint64_t m3;
 };
 
 #define HIP_CHECK(x)                                                         \

@@ -18,9 +18,13 @@ struct moil_ctx {
   moil_params par;
   int npt = 0, totmon = 0, nwat = 0, nb = 0, nangl = 0, ntors = 0, nimp = 0, totspe = 0;
   int nexcl = 0;
+  // generic (not intra-water) bonds, angles and exclusions; the
+  // intra-water ones are listed per water in the d_wi_* arrays
+  int gnb = 0, gnangl = 0, gnexcl = 0, wi_nterms = 0;
   hipStream_t stream = nullptr;   // default stream of the context
   hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool pme_concurrent = false;    // MOIL_GPU_PME_CONCURRENT=1 in the environment
   bool facc_dirty = false;
   int fix_shift = FIX_SHIFT_FP64;
   double fix_scale = 0, fix_inv = 0;
@@ -39,11 +43,14 @@ struct moil_ctx {
   double *d_ipar = nullptr;    // [2*nimp]
   int32_t *d_sp = nullptr;     // [2*totspe]
   double *d_p14 = nullptr;     // [3*totspe]
+  int32_t *d_wi_boff = nullptr, *d_wi_aoff = nullptr, *d_wi_eoff = nullptr;  // [nwat+1]
+  int32_t *d_wi_zero = nullptr;                                             // [nwat+1] zeros
+  int32_t *d_wi_bidx = nullptr, *d_wi_aidx = nullptr, *d_wi_eidx = nullptr;  // local 0..2
+  double *d_wi_bpar = nullptr, *d_wi_apar = nullptr;
   int32_t *d_wat_o = nullptr;     // [nwat] O atom (0-based)
   int32_t *d_wat_mono = nullptr;  // [nwat] monomer number (1-based)
   std::vector<int32_t> h_wat_mono, h_wat_o;
   double qq_oo = 0, qq_oh = 0, qq_hh = 0, oo12 = 0, oo6 = 0;  // last-water parameters
-  bool has_nonwater = false;
 
   // ---- Ewald ----
   double beta = 0, e_self = 0, erftbdns = 5000.0;
@@ -53,12 +60,13 @@ struct moil_ctx {
   double *d_eterm = nullptr;   // influence function on the half spectrum
   std::vector<double> bsp[3];
   int nf1 = 0, nf2 = 0, nf3 = 0, nc1 = 0;  // grid dims; nc1 = nf1/2+1
-  int32_t *d_pcnt = nullptr;    // [G+1] atoms per stencil-base grid point
-  int32_t *d_pstart = nullptr;  // [G+1] exclusive scan of d_pcnt
-  int32_t *d_pbsum = nullptr;   // scan block sums
-  int32_t *d_abin = nullptr, *d_arank = nullptr;  // [npt] base point, rank in bin
-  void *d_aw = nullptr, *d_sw = nullptr;  // [npt][4] (w1 w2 w3 q*fc), atom / bin order
-  int32_t *d_sx = nullptr;      // [npt] base x index, bin order
+  int pt_nt[3] = {0, 0, 0}, pt_r[3] = {0, 0, 0}, pt_ntile = 0;  // spreading tiles
+  int32_t *d_tcnt = nullptr, *d_tstart = nullptr;  // [ntile+1] atoms per tile, scan
+  long long *d_tbuf = nullptr;                      // [ntile][r1 r2 r3] tile regions
+  int32_t *d_atile = nullptr, *d_arank = nullptr;   // [npt] tile, rank in tile
+  void *d_sw = nullptr;          // [npt][12] theta1, theta2, q*fc*theta3, tile order
+  int32_t *d_sbase = nullptr;   // [npt] packed base point, tile order
+  int32_t *d_sidx = nullptr;    // [npt] atom index, tile order
   void *d_qreal = nullptr;      // real grid (float or double)
   void *d_qcplx = nullptr;      // half spectrum
   rocfft_plan plan_fwd = nullptr, plan_bwd = nullptr;
@@ -86,26 +94,52 @@ struct moil_ctx {
   size_t witem_cap = 0, nwitems = 0;
   int32_t *d_wchunk = nullptr;     // [nchunk][3]: block, begin, end
   int nwchunk = 0;
-  int wchunk_items = 256;
+  // gradient partials of the real-space water kernel (no global atomics):
+  // jpart[item][9] (the item's j-water), ipart[chunk][64][9] (the chunk's
+  // block waters), both 64-bit fixed point; jstart/jlist: items per slot j
+  long long *d_jpart = nullptr, *d_ipart = nullptr;
+  int ipart_cap = 0;
+  int32_t *d_jcnt = nullptr, *d_jstart = nullptr, *d_jlist = nullptr;
+  int32_t *d_jpos = nullptr;       // [item] row of the item in jpart (CSR position)
+  int32_t *d_cfirst = nullptr;     // [nwblk+1] first chunk of each block
+  int wchunk_pairs = 4096;         // pair budget of one real-space workgroup
+  int wchunk_max_items = 2048;
   // packed per-slot water coordinates for the real-space kernel
   void *d_wpos = nullptr;          // FP32: 3 float4 per slot; FP64: 9 double per slot
   int32_t *d_slot_o = nullptr;     // [nwat] O atom of the slot
 
-  // ---- atom lists (non-water monomers) ----
-  int natblk = 0;
-  int32_t *d_aslot2atom = nullptr;
+  // ---- atom lists (pairs with an atom outside water monomers) ----
+  int nsol = 0;                          // atoms of non-water monomers ("solute")
+  std::vector<int32_t> h_sol;            // their indices (0-based)
+  int32_t *d_sol = nullptr;
+  int32_t *d_mono_of = nullptr;          // [npt] monomer (0-based)
+  int32_t *d_aflag = nullptr;            // [npt] charged / small LJ / water bits
+  uint64_t *d_xmask = nullptr;           // [npt] exclusions l+1 .. l+64
+  int32_t *d_xfar = nullptr;             // [npt] has exclusions beyond l+64
+  int32_t *d_exc1 = nullptr, *d_exc2 = nullptr;  // MOIL exclusion CSR
+  int ag[3] = {0, 0, 0}, ancell = 0;     // all-atom cell grid
+  double acs[3] = {0, 0, 0};
+  int32_t *d_acell_of = nullptr, *d_acnt = nullptr, *d_astart = nullptr, *d_aslot2atom = nullptr;
+  int32_t *d_sstart = nullptr, *d_sslot2atom = nullptr, *d_sslot_of = nullptr;  // solute slots
+  int nablk = 0;                         // solute blocks of WBLOCK slots
+  int32_t *d_abcount = nullptr, *d_aboff = nullptr;
+  std::vector<int32_t> h_aboff;
   AItem *d_aitems = nullptr;
   size_t aitem_cap = 0, naitems = 0;
   int32_t *d_achunk = nullptr;
   int nachunk = 0;
-  int32_t *d_abcount = nullptr, *d_aboff = nullptr;
-  std::vector<int32_t> h_apartner_mono;  // (unused placeholder)
+  long long *d_ajpart = nullptr, *d_aipart = nullptr;  // [item][3], [chunk][64][3]
+  int aipart_cap = 0;
+  int32_t *d_ajcnt = nullptr, *d_ajstart = nullptr, *d_ajlist = nullptr, *d_acfirst = nullptr;
+  int32_t *d_ajpos = nullptr;
 
   // ---- accumulators ----
   long long *d_facc = nullptr;     // [3*npt] fixed-point gradient
   double *d_epart = nullptr;       // [EN_COUNT][max_parts] partial energies
   int max_parts = 0;
   double *d_energy = nullptr;      // [MOIL_N_ENERGIES]
+  double *d_etot = nullptr;          // [EN_COUNT] per-term totals
+  unsigned int *d_ticket = nullptr;  // last-workgroup ticket of k_energies
   double *d_coor = nullptr;        // [3*npt] staging for host-pointer API
   double *d_dpot = nullptr;        // [3*npt]
   bool have_lists = false;
@@ -124,7 +158,13 @@ void lists_free(moil_ctx *c);
 // real space (realspace.hip)
 int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st);
 int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st);
-int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st);
+int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st);
+// atom lists and atom real space (atoms.hip)
+int32_t atoms_init(moil_ctx *c, const moil_topology *t);
+int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st);
+void atoms_free(moil_ctx *c);
+int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0);
+int32_t realspace_atoms_reduce(moil_ctx *c, hipStream_t st);
 // PME (pme.hip)
 int32_t pme_init(moil_ctx *c);
 int32_t pme_run(moil_ctx *c, const double *d_coor, hipStream_t st);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "grid.h"
 
 #define LB_THREADS 256
 #define LB_MAXCELLS 1024
@@ -63,26 +64,6 @@ __global__ void k_mono_cent(const double *__restrict__ coor, const int32_t *__re
   cent[3 * m + 2] = cz;
   // symm/nbmsym.f:87-95: any centre outside the box quadruples cutmono2
   if (fabs(cx) > ha || fabs(cy) > hb || fabs(cz) > hc) atomicOr(&flags[0], 1);
-}
-
-struct Grid {
-  int gx, gy, gz, nbx, nby, nbz;
-  double g0x, g0y, g0z, csx, csy, csz;
-};
-
-__host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-__device__ inline int cell_coord(double p, double g0, double cs, int n) {
-  double f = floor((p - g0) / cs);
-  if (f < 0.0) return 0;
-  if (f > (double)(n - 1)) return n - 1;
-  return (int)f;
-}
-
-__host__ __device__ inline int cell_key(const Grid &g, int cx, int cy, int cz) {
-  int bx = cx >> 1, by = cy >> 1, bz = cz >> 1;
-  int loc = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
-  return ((bz * g.nby + by) * g.nbx + bx) * 8 + loc;
 }
 
 __global__ void k_wat_cells(const double *__restrict__ cent, const int32_t *__restrict__ wat_mono,
@@ -164,9 +145,8 @@ __global__ void k_slot_fill(const int32_t *__restrict__ s2w, const int32_t *__re
   scent[3 * s + 2] = cent[3 * (m - 1) + 2];
 }
 
-struct WPar {
-  double a, b, c, ha, hb, hc;
-  double cutmono2, cutvbig2, cutebig2, R;
+struct WPar : BoxPar {
+  double R;
   int nwat;
 };
 
@@ -196,41 +176,15 @@ __device__ inline int wpair_kind(double cix, double ciy, double ciz, double da, 
   return 0;
 }
 
-// rch2 filter, symm/nbmsym.f:178-189
-__device__ inline bool image_i_ok(double cix, double ciy, double ciz, double da, double db,
-                                  double dc, double large_cutmono, const WPar &P) {
-#pragma clang fp contract(off)
-  double xch = fabs(cix + da) - P.ha, ych = fabs(ciy + db) - P.hb, zch = fabs(ciz + dc) - P.hc;
-  if (xch < 0.0) xch = 0.0;
-  if (ych < 0.0) ych = 0.0;
-  if (zch < 0.0) zch = 0.0;
-  double rch2 = xch * xch + ych * ych + zch * zch;
-  return !(rch2 > large_cutmono);
-}
-
-// need_only face filter, symm/nbmsym.f:97-115
-__device__ inline bool need_only_ok(double cjx, double cjy, double cjz, double da, double db,
-                                    double dc, const WPar &P) {
-#pragma clang fp contract(off)
-  double barx = 0.5 * da, bary = 0.5 * db, barz = 0.5 * dc;
-  if (fabs(barx) > 0) {
-    double q = cjx - barx;
-    if (q * q > P.cutmono2) return false;
+// removes the n lowest set bits of m and returns them
+__device__ inline uint64_t take_low_bits(uint64_t &m, int n) {
+  uint64_t out = 0;
+  for (int k = 0; k < n; k++) {
+    const uint64_t low = m & (~m + 1ull);
+    out |= low;
+    m ^= low;
   }
-  if (fabs(bary) > 0) {
-    double q = cjy - bary;
-    if (q * q > P.cutmono2) return false;
-  }
-  if (fabs(barz) > 0) {
-    double q = cjz - barz;
-    if (q * q > P.cutmono2) return false;
-  }
-  return true;
-}
-
-__device__ inline int bucket_of(uint64_t m) {
-  int p = __popcll(m);
-  return (64 - p) >> 3;  // 0 (popcount 57..64) .. 7 (1..8)
+  return out;
 }
 
 // One workgroup per water block.  WRITE=false: count items per bucket.
@@ -384,28 +338,56 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
             }
           }
         }
+        // split into nfull items of WSPLIT pairs (bucket 0) and at most one
+        // remainder item of rem pairs (bucket WSPLIT - rem)
         const uint64_t mm = m1 | m2;
-        const bool has = mm != 0;
-        const int bk = has ? bucket_of(mm) : -1;
-        // per-bucket wave counts (deterministic)
-        for (int k = 0; k < NBUCKET; k++) {
-          uint64_t bal = __ballot(has && bk == k);
+        const int pc = __popcll(mm);
+        const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
+        const int bk = rem ? WSPLIT - rem : -1;
+        // per-bucket wave counts (deterministic): bucket 0 by a wave scan
+        // of nfull, the others by ballots (one remainder item per lane)
+        int incl = nfull;
+        for (int off = 1; off < 64; off <<= 1) {
+          int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
+        }
+        const int wtot0 = __shfl(incl, 63, 64);
+        if (lane == 0) wcnt[wv][0] = wtot0;
+        for (int k = 1; k < NBUCKET; k++) {
+          uint64_t bal = __ballot(bk == k);
           if (lane == 0) wcnt[wv][k] = __popcll(bal);
         }
         __syncthreads();
+        uint64_t rest = mm;  // bits left for the remainder item
+        if (WRITE && nfull > 0) {
+          int before0 = 0;
+          for (int w2 = 0; w2 < wv; w2++) before0 += wcnt[w2][0];
+          int pos0 = boff[b * NBUCKET] + bk_run[0] + before0 + (incl - nfull);
+          for (int q = 0; q < nfull; q++) {
+            const uint64_t piece = take_low_bits(rest, WSPLIT);
+            WItem it;
+            it.j = js;
+            it.s = s;
+            it.m1 = m1 & piece;
+            it.m2 = m2 & piece;
+            it.pad = 0;
+            items[pos0 + q] = it;
+          }
+        }
         if (WRITE) {
-          for (int k = 0; k < NBUCKET; k++) {
-            uint64_t bal = __ballot(has && bk == k);
-            if (has && bk == k) {
+          for (int k = 1; k < NBUCKET; k++) {
+            uint64_t bal = __ballot(bk == k);
+            if (bk == k) {
               int before = 0;
               for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
               uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
               int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
+              const uint64_t piece = rest;
               WItem it;
               it.j = js;
               it.s = s;
-              it.m1 = m1;
-              it.m2 = m2;
+              it.m1 = m1 & piece;
+              it.m2 = m2 & piece;
               it.pad = 0;
               items[pos] = it;
             }
@@ -424,6 +406,26 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     __syncthreads();
   }
   if (!WRITE && t < NBUCKET) bcount[b * NBUCKET + t] = bk_run[t];
+}
+
+// items per partner slot j: the real-space kernel leaves each item's
+// j-gradient in its own row of jpart, and the per-water reduction reads
+// the rows of water j in item order through this CSR
+__global__ void k_jcount(const WItem *__restrict__ items, size_t n, int32_t *__restrict__ cnt) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) atomicAdd(&cnt[items[q].j], 1);
+}
+__global__ void k_jfill(const WItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
+                        int32_t *__restrict__ fill, int32_t *__restrict__ jlist) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int j = items[q].j;
+  jlist[start[j] + atomicAdd(&fill[j], 1)] = (int32_t)q;
+}
+// jpos[item] = the item's position in the CSR (its partial row)
+__global__ void k_jpos(const int32_t *__restrict__ jlist, size_t n, int32_t *__restrict__ jpos) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) jpos[jlist[q]] = (int32_t)q;
 }
 
 // ---------------------------------------------------------------------------
@@ -480,6 +482,13 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_boff); c->d_boff = nullptr;
   hipFree(c->d_witems); c->d_witems = nullptr;
   hipFree(c->d_wchunk); c->d_wchunk = nullptr;
+  hipFree(c->d_jcnt); c->d_jcnt = nullptr;
+  hipFree(c->d_jstart); c->d_jstart = nullptr;
+  hipFree(c->d_jlist); c->d_jlist = nullptr;
+  hipFree(c->d_jpos); c->d_jpos = nullptr;
+  hipFree(c->d_jpart); c->d_jpart = nullptr;
+  hipFree(c->d_ipart); c->d_ipart = nullptr;
+  hipFree(c->d_cfirst); c->d_cfirst = nullptr;
 }
 
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
@@ -513,9 +522,8 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     wp.a = P.a; wp.b = P.b; wp.c = P.c;
     wp.ha = 0.5 * P.a; wp.hb = 0.5 * P.b; wp.hc = 0.5 * P.c;
     wp.cutmono2 = P.cutmono2; wp.cutvbig2 = P.cutvbig2; wp.cutebig2 = P.cutebig2;
-    // candidate reach: every listed pair has r2 <= cutebig2 (and, through
-    // the early exits, r2 <= cutmono2 on its x and x+y partial sums)
-    double R = sqrt(std::max(P.cutebig2, 0.0));
+    // candidate reach: every listed pair has r2 <= max(cutvbig2, cutebig2)
+    double R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0));
     wp.R = R * (1.0 + 1e-12) + 1e-9;
     wp.nwat = c->nwat;
     k_wblock_items<false><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
@@ -529,30 +537,82 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     size_t total = (size_t)hoff.back();
     if (total > c->witem_cap || c->d_witems == nullptr) {
       hipFree(c->d_witems);
+      hipFree(c->d_jlist);
+      hipFree(c->d_jpos);
+      hipFree(c->d_jpart);
       c->witem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
       HIP_CHECK(hipMalloc(&c->d_witems, sizeof(WItem) * c->witem_cap));
+      HIP_CHECK(hipMalloc(&c->d_jlist, sizeof(int32_t) * c->witem_cap));
+      HIP_CHECK(hipMalloc(&c->d_jpos, sizeof(int32_t) * c->witem_cap));
+      HIP_CHECK(hipMalloc(&c->d_jpart, sizeof(long long) * 9 * c->witem_cap));
     }
     c->nwitems = total;
     k_wblock_items<true><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
                                                           wp, c->d_flags, nullptr, c->d_boff,
                                                           c->d_witems);
-    // chunk table: runs of <= wchunk_items items inside one block
+    // chunk table: runs of a block's items (popcount-descending) holding
+    // about wchunk_pairs pairs, so workgroups carry similar work
     std::vector<int32_t> ch;
     for (int b = 0; b < c->nwblk; b++) {
-      int beg = hoff[b * NBUCKET], end = hoff[(b + 1) * NBUCKET];
-      for (int q = beg; q < end; q += c->wchunk_items) {
+      int q = hoff[b * NBUCKET];
+      const int end = hoff[(b + 1) * NBUCKET];
+      while (q < end) {
+        int e = q;
+        long pairs = 0;
+        while (e < end && pairs < c->wchunk_pairs && e - q < c->wchunk_max_items) {
+          // bucket of item e: its popcount is WSPLIT - k
+          int k = (int)(std::upper_bound(hoff.begin() + b * NBUCKET, hoff.begin() + (b + 1) * NBUCKET,
+                                         e) - (hoff.begin() + b * NBUCKET)) - 1;
+          const int pc = WSPLIT - k;
+          const int nxt = std::min(hoff[b * NBUCKET + k + 1], end);
+          const long want = std::max<long>(1, (c->wchunk_pairs - pairs + pc - 1) / pc);
+          int take = (int)std::min<long>(nxt - e, want);
+          take = std::min(take, c->wchunk_max_items - (e - q));
+          pairs += (long)take * pc;
+          e += take;
+        }
         ch.push_back(b);
         ch.push_back(q);
-        ch.push_back(std::min(end, q + c->wchunk_items));
+        ch.push_back(e);
+        q = e;
       }
     }
     c->nwchunk = (int)ch.size() / 3;
     hipFree(c->d_wchunk);
     c->d_wchunk = nullptr;
+    // first chunk of every block (chunks are emitted block by block)
+    std::vector<int32_t> cfirst(c->nwblk + 1, c->nwchunk);
+    for (int q = c->nwchunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
+    for (int b = c->nwblk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
     if (c->nwchunk > 0) {
       HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * ch.size()));
       HIP_CHECK(hipMemcpyAsync(c->d_wchunk, ch.data(), sizeof(int32_t) * ch.size(),
                                hipMemcpyHostToDevice, st));
+    }
+    if (c->nwchunk > c->ipart_cap || c->d_ipart == nullptr) {
+      hipFree(c->d_ipart);
+      c->ipart_cap = std::max(c->nwchunk + c->nwchunk / 4 + 64, 256);
+      HIP_CHECK(hipMalloc(&c->d_ipart, sizeof(long long) * WBLOCK * 9 * (size_t)c->ipart_cap));
+    }
+    if (!c->d_cfirst) HIP_CHECK(hipMalloc(&c->d_cfirst, sizeof(int32_t) * (c->nwblk + 1)));
+    HIP_CHECK(hipMemcpyAsync(c->d_cfirst, cfirst.data(), sizeof(int32_t) * cfirst.size(),
+                             hipMemcpyHostToDevice, st));
+    // CSR of the items by partner slot j, in item order within a slot
+    if (!c->d_jcnt) {
+      HIP_CHECK(hipMalloc(&c->d_jcnt, sizeof(int32_t) * (c->nwat + 1)));
+      HIP_CHECK(hipMalloc(&c->d_jstart, sizeof(int32_t) * (c->nwat + 1)));
+    }
+    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
+    if (total > 0) {
+      const int ib = (int)((total + TB - 1) / TB);
+      k_jcount<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jcnt);
+      k_scan_excl<<<1, 1024, 0, st>>>(c->d_jcnt, c->d_jstart, c->nwat);
+      HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
+      k_jfill<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jstart, c->d_jcnt, c->d_jlist);
+      k_cell_sort<<<(c->nwat + TB - 1) / TB, TB, 0, st>>>(c->d_jstart, c->nwat, c->d_jlist);
+      k_jpos<<<ib, TB, 0, st>>>(c->d_jlist, total, c->d_jpos);
+    } else {
+      HIP_CHECK(hipMemsetAsync(c->d_jstart, 0, sizeof(int32_t) * (c->nwat + 1), st));
     }
     HIP_CHECK(hipStreamSynchronize(st));
   }

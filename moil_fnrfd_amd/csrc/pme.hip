@@ -12,18 +12,19 @@
 // eterm = exp(-pi^2 m^2/beta^2) / (pi V |b1 b2 b3|^2 m^2)
 // (pot/ewald_receip.f:723-812).
 //
-// GPU design: spreading is done without atomics on the grid.  Atoms are
-// binned by their stencil base point int(fr) with a counting sort (one
-// integer atomic per atom on the bin counts, a device-wide scan, a
-// scatter); then one thread per grid point gathers the 4x4x4 bins whose
-// stencils cover it and sums the B-spline weighted charges in fixed point
-// (order independent, so bit-reproducible), writing the float (mixed) or
-// double (FP64) grid for rocFFT's real-to-complex transform -- the only
-// library call.  The convolution runs on the half spectrum
-// with the influence function tabulated once in FP64 at init; the
-// reciprocal energy is summed in FP64 with self-conjugate planes weighted
-// once and all others twice.  The gather recomputes the B-splines from
-// the coordinates instead of storing them, and writes each atom's
+// GPU design: atoms are counting-sorted by grid tile (8x8x8 points; the
+// tile holding the atom's stencil base point int(fr)).  One workgroup per
+// tile scatters its atoms' 64 B-spline weighted charges (one lane per
+// stencil point) into a tile + 3-point halo region in LDS with 64-bit
+// fixed-point atomics, which are order independent (bit-reproducible);
+// a per-point pass sums the <= 8 tile regions covering each point and
+// writes the float (mixed) or double (FP64) grid for rocFFT's
+// real-to-complex transform -- the only library call.  The convolution
+// runs on the half spectrum with the influence function tabulated once in
+// FP64 at init; the reciprocal energy is summed in FP64 with
+// self-conjugate planes weighted once and all others twice.  The gather
+// walks the atoms in tile order (cache-friendly stencils), recomputes the
+// B-splines and derivatives from the coordinates, and writes each atom's
 // reciprocal gradient directly (no atomics).
 #include <hip/hip_runtime.h>
 
@@ -33,7 +34,7 @@
 #include "ctx.h"
 
 #define PME_THREADS 256
-#define QFIX_SHIFT 40
+#define QFIX_SHIFT 40  // grid charge fixed point: 2^-40
 
 struct PmePar {
   int n1, n2, n3, nc1;
@@ -80,155 +81,83 @@ __device__ __forceinline__ void frac_coord(double x, double r, int n, int *i0, d
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// stencil base bin of every atom, counting-sort histogram
-template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_bin(const double *__restrict__ coor,
-                                                         const double *__restrict__ q, int npt,
-                                                         PmePar P, int32_t *__restrict__ cnt,
-                                                         int32_t *__restrict__ abin,
-                                                         int32_t *__restrict__ arank,
-                                                         R *__restrict__ aw) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= npt) return;
+// Grid tiles: dimension k is cut into nt_k = ceil(n_k/8) tiles with
+// boundaries lo = floor(t n_k / nt_k) (sizes 4..8 for n_k >= 4, so a
+// tile's 3-point low-side halo reaches one neighbour only).  An atom
+// belongs to the tile holding its stencil base point int(fr); its 64
+// stencil points base-3 .. base lie in that tile or its halo.
+struct TilePar {
+  int n1, n2, n3;
+  int nt1, nt2, nt3;
+  int r1, r2, r3;  // LDS region dims: max tile size + 3
+  int rsz;         // r1 r2 r3
+};
+
+__device__ __forceinline__ int tile_of(int g, int n, int nt) { return ((g + 1) * nt - 1) / n; }
+__device__ __forceinline__ int tile_lo(int t, int n, int nt) { return (t * n) / nt; }
+
+__device__ __forceinline__ void base_point(const double *x, const PmePar &P, int &c1, int &c2,
+                                           int &c3, double &w1, double &w2, double &w3) {
   int i0, j0, k0;
-  double w1, w2, w3;
-  frac_coord(coor[3 * n], P.ra, P.n1, &i0, &w1);
-  frac_coord(coor[3 * n + 1], P.rb, P.n2, &j0, &w2);
-  frac_coord(coor[3 * n + 2], P.rc, P.n3, &k0, &w3);
+  frac_coord(x[0], P.ra, P.n1, &i0, &w1);
+  frac_coord(x[1], P.rb, P.n2, &j0, &w2);
+  frac_coord(x[2], P.rc, P.n3, &k0, &w3);
   // base point int(fr) = i0 + 3, in [0, n] (n only when fr == n exactly)
-  int c1 = i0 + 3, c2 = j0 + 3, c3 = k0 + 3;
+  c1 = i0 + 3; c2 = j0 + 3; c3 = k0 + 3;
   if (c1 >= P.n1) c1 -= P.n1;
   if (c2 >= P.n2) c2 -= P.n2;
   if (c3 >= P.n3) c3 -= P.n3;
-  int bin = (c3 * P.n2 + c2) * P.n1 + c1;
-  abin[n] = bin;
-  arank[n] = atomicAdd(&cnt[bin], 1);
-  aw[4 * n] = (R)w1;
-  aw[4 * n + 1] = (R)w2;
-  aw[4 * n + 2] = (R)w3;
-  aw[4 * n + 3] = (R)(q[n] * P.fc);
 }
 
-template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_scatter(int npt, const int32_t *__restrict__ abin,
-                                                             const int32_t *__restrict__ arank,
-                                                             const int32_t *__restrict__ start,
-                                                             const R *__restrict__ aw,
-                                                             R *__restrict__ sw,
-                                                             int32_t *__restrict__ sx, int n1) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= npt) return;
-  int b = abin[n];
-  int p = start[b] + arank[n];
-  sw[4 * p] = aw[4 * n];
-  sw[4 * p + 1] = aw[4 * n + 1];
-  sw[4 * p + 2] = aw[4 * n + 2];
-  sw[4 * p + 3] = aw[4 * n + 3];
-  sx[p] = b % n1;
-}
-
-__device__ __forceinline__ void fix_split_f(float f, int &h, int &l) {
-  const float x = f * 256.0f;
-  const float hr = rintf(x);
-  const float r = x - hr;
-  h = (int)hr;
-  l = (int)rintf(r * 16777216.0f);
-}
-
-// one thread per grid point: Q(g) = sum over atoms with base point b,
-// b - 3 <= g <= b (periodic), of q fc theta1 theta2 theta3
-template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_spread(const int32_t *__restrict__ start,
-                                                            const R *__restrict__ sw,
-                                                            const int32_t *__restrict__ sx,
-                                                            PmePar P, R *__restrict__ grid) {
-  const size_t G = (size_t)P.n1 * P.n2 * P.n3;
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  const int g1 = (int)(g % P.n1);
-  const int g2 = (int)((g / P.n1) % P.n2);
-  const int g3 = (int)(g / ((size_t)P.n1 * P.n2));
-  int acc_h = 0, acc_l = 0;  // FP32 path: fixed point 2^-32 split h*2^24+l
-  long long acc = 0;         // FP64 path: fixed point 2^-40
-#pragma unroll
-  for (int d3 = 0; d3 < 4; d3++) {
-    int c3 = g3 + d3;
-    if (c3 >= P.n3) c3 -= P.n3;
-#pragma unroll
-    for (int d2 = 0; d2 < 4; d2++) {
-      int c2 = g2 + d2;
-      if (c2 >= P.n2) c2 -= P.n2;
-      const int row = (c3 * P.n2 + c2) * P.n1;
-      // bins g1..g1+3 of this row: one or two contiguous runs
-      int e1 = g1 + 3;
-      int lo[2], hi[2], nr = 1;
-      if (e1 < P.n1) {
-        lo[0] = start[row + g1];
-        hi[0] = start[row + e1 + 1];
-      } else {
-        lo[0] = start[row + g1];
-        hi[0] = start[row + P.n1];
-        lo[1] = start[row];
-        hi[1] = start[row + (e1 - P.n1) + 1];
-        nr = 2;
-      }
-      for (int rr = 0; rr < nr; rr++) {
-        for (int a = lo[rr]; a < hi[rr]; a++) {
-          int x1 = sx[a] - g1;
-          if (x1 < 0) x1 += P.n1;  // 0..3
-          R t1[4], t2[4], t3[4], dd[4];
-          bspline4<R>(sw[4 * a], t1, dd);
-          bspline4<R>(sw[4 * a + 1], t2, dd);
-          bspline4<R>(sw[4 * a + 2], t3, dd);
-          const R v = sw[4 * a + 3] * t3[3 - d3] * t2[3 - d2] * pick4<R>(t1, 3 - x1);
-          if (sizeof(R) == 4) {
-            int h, l;
-            fix_split_f((float)v, h, l);
-            acc_h += h;
-            acc_l += l;
-          } else {
-            acc += __double2ll_rn((double)v * P.qscale);
-          }
-        }
-      }
-    }
+// tile of every atom and its rank inside the tile (counting sort).  Atoms
+// are visited in the previous evaluation's tile order, so the lanes of a
+// wavefront mostly share a tile, and each wavefront issues one counter
+// atomic per distinct tile (ranks inside a tile may come out in any
+// order: the spreading sums are integer and the gather is per atom)
+__global__ __launch_bounds__(PME_THREADS) void k_pme_tbin(const double *__restrict__ coor, int npt,
+                                                          const int32_t *__restrict__ order,
+                                                          PmePar P, TilePar T,
+                                                          int32_t *__restrict__ tcnt,
+                                                          int32_t *__restrict__ atile,
+                                                          int32_t *__restrict__ arank) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < npt;
+  const int n = valid ? order[p] : 0;
+  int tile = -1;
+  if (valid) {
+    int c1, c2, c3;
+    double w1, w2, w3;
+    base_point(coor + 3 * n, P, c1, c2, c3, w1, w2, w3);
+    tile = (tile_of(c3, T.n3, T.nt3) * T.nt2 + tile_of(c2, T.n2, T.nt2)) * T.nt1 +
+           tile_of(c1, T.n1, T.nt1);
+    atile[n] = tile;
   }
-  if (sizeof(R) == 4)
-    grid[g] = (R)((double)(((long long)acc_h << 24) + (long long)acc_l) * 0x1p-32);
-  else
-    grid[g] = (R)((double)acc * P.qinv);
+  uint64_t todo = __ballot(valid);
+  const uint64_t below = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lt = __shfl(tile, leader, 64);
+    const uint64_t same = __ballot(valid && tile == lt);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&tcnt[lt], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (valid && tile == lt) arank[n] = base + __popcll(same & below);
+    todo &= ~same;
+  }
 }
 
-// device-wide exclusive scan in three passes (per-block sums, scan of the
-// block sums, per-block rescan); the last pass also clears the counts for
-// the next evaluation
-#define SCAN_PER 2048
-__global__ __launch_bounds__(256) void k_scan_sums(const int32_t *__restrict__ in, size_t n,
-                                                   int32_t *__restrict__ bsum) {
-  __shared__ int red[256];
-  size_t b0 = (size_t)blockIdx.x * SCAN_PER;
-  int s = 0;
-  for (int k = threadIdx.x; k < SCAN_PER; k += 256) {
-    size_t i = b0 + k;
-    if (i < n) s += in[i];
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb) {
+// exclusive scan of the tile counts (one workgroup); clears the counts
+// for the next evaluation
+__global__ __launch_bounds__(1024) void k_pme_tscan(int32_t *__restrict__ cnt, int n,
+                                                    int32_t *__restrict__ start) {
   __shared__ int part[1024];
-  int t = threadIdx.x;
-  int per = (nb + 1023) / 1024;
-  int b = t * per, e = min(nb, b + per);
-  int s = 0;
-  for (int i = b; i < e; i++) s += bsum[i];
-  part[t] = s;
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(n, b + per);
+  int sum = 0;
+  for (int i = b; i < e; i++) sum += cnt[i];
+  part[t] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
     int v = (t >= off) ? part[t - off] : 0;
@@ -236,46 +165,117 @@ __global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, i
     part[t] += v;
     __syncthreads();
   }
-  int run = (t == 0) ? 0 : part[t - 1];
+  int run = part[t] - sum;
   for (int i = b; i < e; i++) {
-    int v = bsum[i];
-    bsum[i] = run;
+    int v = cnt[i];
+    start[i] = run;
     run += v;
+    cnt[i] = 0;
   }
-  if (t == 1023) bsum[nb] = part[1023];
+  if (t == 1023) start[n] = part[1023];
 }
 
-__global__ __launch_bounds__(256) void k_scan_apply(int32_t *__restrict__ cnt, size_t n,
-                                                    const int32_t *__restrict__ bsum,
-                                                    int32_t *__restrict__ out) {
-  __shared__ int part[256];
-  size_t b0 = (size_t)blockIdx.x * SCAN_PER;
-  const int per = SCAN_PER / 256;  // 8 consecutive elements per thread
-  int v[per];
-  int s = 0;
-  for (int k = 0; k < per; k++) {
-    size_t i = b0 + (size_t)threadIdx.x * per + k;
-    v[k] = (i < n) ? cnt[i] : 0;
-    s += v[k];
+// counting-sort scatter; the sorted record of an atom holds its B-spline
+// weights theta1(4), theta2(4) and q sqrt(kofdie/eps) theta3(4), its base
+// point (packed 10 bits per dimension) and its atom index
+template <typename R>
+__global__ __launch_bounds__(PME_THREADS) void k_pme_tscatter(const double *__restrict__ coor,
+                                                              const double *__restrict__ q, int npt,
+                                                              PmePar P,
+                                                              const int32_t *__restrict__ atile,
+                                                              const int32_t *__restrict__ arank,
+                                                              const int32_t *__restrict__ tstart,
+                                                              R *__restrict__ sw,
+                                                              int32_t *__restrict__ sbase,
+                                                              int32_t *__restrict__ sidx) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= npt) return;
+  int c1, c2, c3;
+  double w1, w2, w3;
+  base_point(coor + 3 * n, P, c1, c2, c3, w1, w2, w3);
+  R t1[4], t2[4], t3[4], dd[4];
+  bspline4<R>((R)w1, t1, dd);
+  bspline4<R>((R)w2, t2, dd);
+  bspline4<R>((R)w3, t3, dd);
+  const R qf = (R)(q[n] * P.fc);
+  const int p = tstart[atile[n]] + arank[n];
+  R *o = sw + 12 * (size_t)p;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    o[k] = t1[k];
+    o[4 + k] = t2[k];
+    o[8 + k] = qf * t3[k];
   }
-  part[threadIdx.x] = s;
+  sbase[p] = c1 | (c2 << 10) | (c3 << 20);
+  sidx[p] = n;
+}
+
+// one workgroup per tile: every atom of the tile adds its 64 stencil
+// weights (one lane per stencil point) into the tile + halo region in
+// LDS, in 64-bit fixed point (order independent); the region is then
+// written to the tile's slot of tbuf
+template <typename R>
+__global__ __launch_bounds__(PME_THREADS) void k_pme_tspread(const int32_t *__restrict__ tstart,
+                                                             const R *__restrict__ sw,
+                                                             const int32_t *__restrict__ sbase,
+                                                             TilePar T, double qscale,
+                                                             long long *__restrict__ tbuf) {
+  __shared__ unsigned long long acc[11 * 11 * 11];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
+  const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
+            lo3 = tile_lo(tt3, T.n3, T.nt3);
+  for (int u = t; u < T.rsz; u += PME_THREADS) acc[u] = 0;
   __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    int a = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += a;
-    __syncthreads();
+  const int lane = t & 63, wv = t >> 6;
+  const int la = lane & 3, lb = (lane >> 2) & 3, lc = lane >> 4;
+  const int beg = tstart[tile], end = tstart[tile + 1];
+  for (int a = beg + wv; a < end; a += PME_THREADS / 64) {
+    const int bp = sbase[a];
+    const int b1 = bp & 1023, b2 = (bp >> 10) & 1023, b3 = bp >> 20;
+    // stencil point base-3+l has local coordinate base - lo + l
+    const int l1 = b1 - lo1 + la, l2 = b2 - lo2 + lb, l3 = b3 - lo3 + lc;
+    const R *o = sw + 12 * (size_t)a;
+    const R v = o[8 + lc] * o[4 + lb] * o[la];
+    const long long f = __double2ll_rn((double)v * qscale);
+    if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
   }
-  int run = bsum[blockIdx.x] + part[threadIdx.x] - s;
-  for (int k = 0; k < per; k++) {
-    size_t i = b0 + (size_t)threadIdx.x * per + k;
-    if (i < n) {
-      out[i] = run;
-      cnt[i] = 0;
-    }
-    run += v[k];
+  __syncthreads();
+  long long *dst = tbuf + (size_t)tile * T.rsz;
+  for (int u = t; u < T.rsz; u += PME_THREADS) dst[u] = (long long)acc[u];
+}
+
+// per grid point: its own tile's region entry plus the halo entries of
+// the next tile in each dimension where it lies in that tile's halo
+// (at most 8 terms), converted to the transform's real type
+template <typename R>
+__global__ __launch_bounds__(PME_THREADS) void k_pme_tsum(const long long *__restrict__ tbuf,
+                                                          TilePar T, double qinv,
+                                                          R *__restrict__ grid) {
+  const size_t G = (size_t)T.n1 * T.n2 * T.n3;
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int gg[3] = {(int)(g % T.n1), (int)((g / T.n1) % T.n2), (int)(g / ((size_t)T.n1 * T.n2))};
+  const int nn[3] = {T.n1, T.n2, T.n3}, nt[3] = {T.nt1, T.nt2, T.nt3};
+  int tl[3][2], lc[3][2], nc[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int tk = tile_of(gg[d], nn[d], nt[d]);
+    const int lo = tile_lo(tk, nn[d], nt[d]), hi = tile_lo(tk + 1, nn[d], nt[d]);
+    tl[d][0] = tk;
+    lc[d][0] = gg[d] - lo + 3;
+    tl[d][1] = (tk + 1 == nt[d]) ? 0 : tk + 1;
+    lc[d][1] = gg[d] - hi + 3;
+    nc[d] = (gg[d] >= hi - 3) ? 2 : 1;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = bsum[gridDim.x];
+  long long sum = 0;
+  for (int c = 0; c < nc[2]; c++)
+    for (int b = 0; b < nc[1]; b++)
+      for (int a = 0; a < nc[0]; a++) {
+        const int tile = (tl[2][c] * T.nt2 + tl[1][b]) * T.nt1 + tl[0][a];
+        sum += tbuf[(size_t)tile * T.rsz + (lc[2][c] * T.r2 + lc[1][b]) * T.r1 + lc[0][a]];
+      }
+  grid[g] = (R)((double)sum * qinv);
 }
 
 // half-spectrum convolution + energy (weights 1 for the k1 = 0 and
@@ -309,10 +309,14 @@ __global__ __launch_bounds__(PME_THREADS) void k_conv(R *__restrict__ S, const d
 template <typename R>
 __global__ __launch_bounds__(PME_THREADS) void k_gather(const double *__restrict__ coor,
                                                         const double *__restrict__ q, int npt,
-                                                        PmePar P, const R *__restrict__ grid,
+                                                        PmePar P, const int32_t *__restrict__ sidx,
+                                                        const R *__restrict__ grid,
                                                         double *__restrict__ fpme) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= npt) return;
+  // atoms in counting-sort order: neighbouring threads read overlapping
+  // grid stencils
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  const int n = sidx[p];
   double qn = q[n] * P.fc;
   if (qn == 0.0) {
     fpme[3 * n] = fpme[3 * n + 1] = fpme[3 * n + 2] = 0.0;
@@ -404,6 +408,10 @@ static const char *rf_err(rocfft_status s) {
 int32_t pme_init(moil_ctx *c) {
   const moil_params &P = c->par;
   c->nf1 = P.nfft1; c->nf2 = P.nfft2; c->nf3 = P.nfft3;
+  if (c->nf1 < 4 || c->nf2 < 4 || c->nf3 < 4 || c->nf1 > 1023 || c->nf2 > 1023 || c->nf3 > 1023) {
+    set_error("PME grid dimensions must lie in 4..1023");
+    return MOIL_GPU_EINVAL;
+  }
   c->nc1 = c->nf1 / 2 + 1;
   bsp_moduli(c->bsp[0], c->nf1);
   bsp_moduli(c->bsp[1], c->nf2);
@@ -442,16 +450,31 @@ int32_t pme_init(moil_ctx *c) {
   const bool f32 = c->par.precision == MOIL_PREC_MIXED;
   const size_t rs = f32 ? sizeof(float) : sizeof(double);
   const int na = std::max(c->npt, 1);
-  const size_t nscan = (G + SCAN_PER - 1) / SCAN_PER;
-  HIP_CHECK(hipMalloc(&c->d_pcnt, sizeof(int32_t) * (G + 1)));
-  HIP_CHECK(hipMemset(c->d_pcnt, 0, sizeof(int32_t) * (G + 1)));
-  HIP_CHECK(hipMalloc(&c->d_pstart, sizeof(int32_t) * (G + 1)));
-  HIP_CHECK(hipMalloc(&c->d_pbsum, sizeof(int32_t) * (nscan + 1)));
-  HIP_CHECK(hipMalloc(&c->d_abin, sizeof(int32_t) * na));
+  // grid tiles for the spreading
+  {
+    const int n[3] = {c->nf1, c->nf2, c->nf3};
+    for (int d = 0; d < 3; d++) {
+      c->pt_nt[d] = (n[d] + 7) / 8;
+      c->pt_r[d] = (n[d] + c->pt_nt[d] - 1) / c->pt_nt[d] + 3;
+    }
+    c->pt_ntile = c->pt_nt[0] * c->pt_nt[1] * c->pt_nt[2];
+  }
+  const size_t rsz = (size_t)c->pt_r[0] * c->pt_r[1] * c->pt_r[2];
+  HIP_CHECK(hipMalloc(&c->d_tcnt, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMemset(c->d_tcnt, 0, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMalloc(&c->d_tstart, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMalloc(&c->d_tbuf, sizeof(long long) * rsz * c->pt_ntile));
+  HIP_CHECK(hipMalloc(&c->d_atile, sizeof(int32_t) * na));
   HIP_CHECK(hipMalloc(&c->d_arank, sizeof(int32_t) * na));
-  HIP_CHECK(hipMalloc(&c->d_aw, 4 * rs * na));
-  HIP_CHECK(hipMalloc(&c->d_sw, 4 * rs * na));
-  HIP_CHECK(hipMalloc(&c->d_sx, sizeof(int32_t) * na));
+  HIP_CHECK(hipMalloc(&c->d_sw, 12 * rs * na));
+  HIP_CHECK(hipMalloc(&c->d_sbase, sizeof(int32_t) * na));
+  HIP_CHECK(hipMalloc(&c->d_sidx, sizeof(int32_t) * na));
+  {
+    // first evaluation visits the atoms in index order
+    std::vector<int32_t> iota(na);
+    for (int k = 0; k < na; k++) iota[k] = k;
+    HIP_CHECK(hipMemcpy(c->d_sidx, iota.data(), sizeof(int32_t) * na, hipMemcpyHostToDevice));
+  }
   HIP_CHECK(hipMalloc(&c->d_qreal, rs * G));
   HIP_CHECK(hipMalloc(&c->d_qcplx, 2 * rs * H));
   HIP_CHECK(hipMalloc(&c->d_fpme, sizeof(double) * 3 * std::max(c->npt, 1)));
@@ -486,14 +509,14 @@ void pme_free(moil_ctx *c) {
   c->exec_fwd = c->exec_bwd = nullptr;
   hipFree(c->d_fftwork); c->d_fftwork = nullptr;
   hipFree(c->d_eterm); c->d_eterm = nullptr;
-  hipFree(c->d_pcnt); c->d_pcnt = nullptr;
-  hipFree(c->d_pstart); c->d_pstart = nullptr;
-  hipFree(c->d_pbsum); c->d_pbsum = nullptr;
-  hipFree(c->d_abin); c->d_abin = nullptr;
+  hipFree(c->d_tcnt); c->d_tcnt = nullptr;
+  hipFree(c->d_tstart); c->d_tstart = nullptr;
+  hipFree(c->d_tbuf); c->d_tbuf = nullptr;
+  hipFree(c->d_atile); c->d_atile = nullptr;
   hipFree(c->d_arank); c->d_arank = nullptr;
-  hipFree(c->d_aw); c->d_aw = nullptr;
   hipFree(c->d_sw); c->d_sw = nullptr;
-  hipFree(c->d_sx); c->d_sx = nullptr;
+  hipFree(c->d_sbase); c->d_sbase = nullptr;
+  hipFree(c->d_sidx); c->d_sidx = nullptr;
   hipFree(c->d_qreal); c->d_qreal = nullptr;
   hipFree(c->d_qcplx); c->d_qcplx = nullptr;
   hipFree(c->d_fpme); c->d_fpme = nullptr;
@@ -516,16 +539,20 @@ static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.qinv = ldexp(1.0, -QFIX_SHIFT);
   const int gb = (int)((G + PME_THREADS - 1) / PME_THREADS);
   const int ab = (c->npt + PME_THREADS - 1) / PME_THREADS;
-  const int nscan = (int)((G + SCAN_PER - 1) / SCAN_PER);
-  k_pme_bin<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_pcnt, c->d_abin,
-                                           c->d_arank, (R *)c->d_aw);
-  k_scan_sums<<<nscan, 256, 0, st>>>(c->d_pcnt, G, c->d_pbsum);
-  k_scan_top<<<1, 1024, 0, st>>>(c->d_pbsum, nscan);
-  k_scan_apply<<<nscan, 256, 0, st>>>(c->d_pcnt, G, c->d_pbsum, c->d_pstart);
-  k_pme_scatter<R><<<ab, PME_THREADS, 0, st>>>(c->npt, c->d_abin, c->d_arank, c->d_pstart,
-                                               (const R *)c->d_aw, (R *)c->d_sw, c->d_sx, c->nf1);
-  k_pme_spread<R><<<gb, PME_THREADS, 0, st>>>(c->d_pstart, (const R *)c->d_sw, c->d_sx, P,
-                                              (R *)c->d_qreal);
+  TilePar T;
+  T.n1 = c->nf1; T.n2 = c->nf2; T.n3 = c->nf3;
+  T.nt1 = c->pt_nt[0]; T.nt2 = c->pt_nt[1]; T.nt3 = c->pt_nt[2];
+  T.r1 = c->pt_r[0]; T.r2 = c->pt_r[1]; T.r3 = c->pt_r[2];
+  T.rsz = T.r1 * T.r2 * T.r3;
+  k_pme_tbin<<<ab, PME_THREADS, 0, st>>>(d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile,
+                                         c->d_arank);
+  k_pme_tscan<<<1, 1024, 0, st>>>(c->d_tcnt, c->pt_ntile, c->d_tstart);
+  k_pme_tscatter<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_atile,
+                                                c->d_arank, c->d_tstart, (R *)c->d_sw,
+                                                c->d_sbase, c->d_sidx);
+  k_pme_tspread<R><<<c->pt_ntile, PME_THREADS, 0, st>>>(c->d_tstart, (const R *)c->d_sw,
+                                                        c->d_sbase, T, P.qscale, c->d_tbuf);
+  k_pme_tsum<R><<<gb, PME_THREADS, 0, st>>>(c->d_tbuf, T, P.qinv, (R *)c->d_qreal);
   HIP_CHECK(hipGetLastError());
   void *in[1] = {c->d_qreal}, *out[1] = {c->d_qcplx};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_fwd, st));
@@ -536,8 +563,8 @@ static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
   void *in2[1] = {c->d_qcplx}, *out2[1] = {c->d_qreal};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_bwd, st));
   RF_CHECK(rocfft_execute(c->plan_bwd, in2, out2, c->exec_bwd));
-  k_gather<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, (const R *)c->d_qreal,
-                                          c->d_fpme);
+  k_gather<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_sidx,
+                                          (const R *)c->d_qreal, c->d_fpme);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

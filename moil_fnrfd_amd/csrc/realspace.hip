@@ -6,13 +6,18 @@
 // parameters are those of the last water (pot/watwat_ewald.f:40-72).
 //
 // Work decomposition (see lists.hip): a workgroup takes a chunk of the
-// items of one 64-water block.  The block's waters (the translated pair
-// members) sit in LDS; each lane owns one item (partner water j plus a
-// 64-bit mask over the block) and walks the mask.  j-forces accumulate in
-// registers and go to global 64-bit fixed-point accumulators once per
-// item; i-forces accumulate in LDS with 64-bit integer atomics and are
-// flushed once per workgroup.  Integer accumulation makes the forces
-// independent of execution order (bit-reproducible).
+// items of one 64-water block, sized by its pair count.  The block's
+// waters (the translated pair members) sit in LDS; each lane owns an item
+// (partner water j plus a mask of at most 16 bits over the block) and
+// walks the mask.  Items come sorted by popcount, so the lanes of a
+// wavefront run loops of equal length.  j-forces accumulate in registers
+// and are stored once per item into the item's row of a partial buffer
+// (rows ordered by partner water, so each water's rows are contiguous); i-forces accumulate in LDS with 64-bit integer atomics and are
+// stored once per workgroup into the chunk's row.  No global atomics: on
+// this GPU they are performed at the memory side and cost far more than
+// the stores plus one gather pass (k_water_reduce) that sums each water's
+// rows in a fixed order.  All partials are 64-bit fixed point, so the
+// forces are independent of execution order (bit-reproducible).
 //
 // MOIL_PREC_MIXED: FP32 pair arithmetic.  Positions are carried as the O
 // atom (float hi + float lo) and the H offsets from their O, so every
@@ -69,8 +74,11 @@ __device__ __forceinline__ long long fix_join(int h, int l) {
   return ((long long)h << 24) + (long long)l;
 }
 
-__device__ __forceinline__ void atomic_add_fix(long long *p, long long v) {
-  if (v != 0) atomicAdd((unsigned long long *)p, (unsigned long long)v);
+// Each lane walks its block mask starting at its own lane number (bit b of
+// the rotated mask is block water (b + rot) mod 64), so lanes with similar
+// masks do not all add into the same i-water's LDS accumulators at once.
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
+  return r ? ((x >> r) | (x << (64 - r))) : x;
 }
 
 // ---------------------------------------------------------------------------
@@ -119,13 +127,12 @@ __device__ inline double block_sum(double v, double *red) {
 // FP32 (mixed precision) water kernel
 __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
     const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
-    const float4 *__restrict__ wpos, const int32_t *__restrict__ slot_o, int nwat, WaterRS P,
-    long long *__restrict__ facc, double *__restrict__ epart, int max_parts) {
-  // i-force partials in fixed point split as h * 2^24 + l (int32 each):
-  // a chunk holds <= 256 items, so an accumulator receives <= 256 terms
-  // with |l| <= 2^23 and cannot overflow
+    const int32_t *__restrict__ jpos, const float4 *__restrict__ wpos, int nwat, WaterRS P,
+    long long *__restrict__ jpart, long long *__restrict__ ipart, double *__restrict__ epart,
+    int max_parts) {
+  // i-force partials: 64-bit fixed point (2^-32) in LDS
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
-  __shared__ int iacc_h[WBLOCK * 9], iacc_l[WBLOCK * 9];
+  __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
   const int ch = blockIdx.x, t = threadIdx.x;
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
@@ -137,10 +144,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
       iC[t] = wpos[3 * (s0 + t) + 2];
     }
   }
-  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
-    iacc_h[u] = 0;
-    iacc_l[u] = 0;
-  }
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
   __syncthreads();
 
   const float beta = (float)P.beta;
@@ -150,6 +154,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
   const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
   const float pe = ERFC_P;
   const float log2e = 1.4426950408889634f;
+  const int rot = t & 63;
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
   for (int it = beg + t; it < end; it += RS_THREADS) {
@@ -171,9 +176,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
 #pragma unroll
     for (int k = 0; k < 9; k++) fj[k] = 0.f;
     float elj = 0.f, eel = 0.f;
-    uint64_t mask = item.m1 | item.m2;
+    uint64_t mask = rotr64(item.m1 | item.m2, rot);
     while (mask) {
-      const int i = __ffsll((long long)mask) - 1;
+      const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
       const bool k2 = (item.m2 >> i) & 1ull;
       mask &= mask - 1;
       const float4 A = iA[i], B = iB[i], C = iC[i];
@@ -227,16 +232,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
       for (int k = 0; k < 9; k++) {
         int h, l;
         fix_split(fi[k], h, l);
-        atomicAdd(&iacc_h[i * 9 + k], h);
-        atomicAdd(&iacc_l[i * 9 + k], l);
+        atomicAdd(&iacc[i * 9 + k], (unsigned long long)fix_join(h, l));
       }
     }
-    const int oj = slot_o[js];
+    long long *jrow = jpart + (size_t)jpos[it] * 9;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       int h, l;
       fix_split(fj[k], h, l);
-      atomic_add_fix(&facc[3 * oj + k], fix_join(h, l));
+      jrow[k] = fix_join(h, l);
     }
     if (item.s == 0) {
       e_lj_r += (double)elj;
@@ -247,13 +251,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
     }
   }
   __syncthreads();
-  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
-    const int i = u / 9, k = u % 9;
-    if (s0 + i < nwat) {
-      const int oi = slot_o[s0 + i];
-      atomic_add_fix(&facc[3 * oi + k], fix_join(iacc_h[u], iacc_l[u]));
-    }
-  }
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS)
+    ipart[(size_t)ch * (WBLOCK * 9) + u] = (long long)iacc[u];
   double v;
   v = block_sum(e_lj_r, red);
   if (t == 0) epart[EN_VDW * max_parts + ch] = v;
@@ -268,9 +267,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
 // FP64 water kernel (MOIL parity mode)
 __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
     const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
-    const double *__restrict__ wpos, const int32_t *__restrict__ slot_o, int nwat, WaterRS P,
-    const double *__restrict__ erf_arr, long long *__restrict__ facc, double *__restrict__ epart,
-    int max_parts) {
+    const int32_t *__restrict__ jpos, const double *__restrict__ wpos, int nwat, WaterRS P,
+    const double *__restrict__ erf_arr, long long *__restrict__ jpart, long long *__restrict__ ipart,
+    double *__restrict__ epart, int max_parts) {
   __shared__ double ipos[WBLOCK * 9];
   __shared__ long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
@@ -284,6 +283,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
   __syncthreads();
   const double del = 1.0 / P.erftbdns, ovt = 1.0 / 3.0;
   const double fs = P.fix_scale;
+  const int rot = t & 63;
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
   for (int it = beg + t; it < end; it += RS_THREADS) {
     const WItem item = items[it];
@@ -298,9 +298,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
 #pragma unroll
     for (int k = 0; k < 9; k++) fj[k] = 0.0;
     double elj = 0, eel = 0;
-    uint64_t mask = item.m1 | item.m2;
+    uint64_t mask = rotr64(item.m1 | item.m2, rot);
     while (mask) {
-      const int i = __ffsll((long long)mask) - 1;
+      const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
       const bool k2 = (item.m2 >> i) & 1ull;
       mask &= mask - 1;
       double xi[9];
@@ -355,9 +355,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
         if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
       }
     }
-    const int oj = slot_o[js];
+    long long *jrow = jpart + (size_t)jpos[it] * 9;
 #pragma unroll
-    for (int k = 0; k < 9; k++) atomic_add_fix(&facc[3 * oj + k], fix64(fj[k], fs));
+    for (int k = 0; k < 9; k++) jrow[k] = fix64(fj[k], fs);
     if (item.s == 0) {
       e_lj_r += elj;
       e_el_r += eel;
@@ -367,10 +367,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
     }
   }
   __syncthreads();
-  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
-    const int i = u / 9, k = u % 9;
-    if (s0 + i < nwat) atomic_add_fix(&facc[3 * slot_o[s0 + i] + k], iacc[u]);
-  }
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) ipart[(size_t)ch * (WBLOCK * 9) + u] = iacc[u];
   double v;
   v = block_sum(e_lj_r, red);
   if (t == 0) epart[EN_VDW * max_parts + ch] = v;
@@ -403,14 +400,43 @@ int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale; P.ntab = c->ntab;
   if (c->par.precision == MOIL_PREC_MIXED) {
-    k_water_f32<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, (const float4 *)c->d_wpos,
-                                                    c->d_slot_o, c->nwat, P, c->d_facc, c->d_epart,
-                                                    c->max_parts);
+    k_water_f32<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, c->d_jpos,
+                                                    (const float4 *)c->d_wpos, c->nwat, P,
+                                                    c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
   } else {
-    k_water_f64<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, (const double *)c->d_wpos,
-                                                    c->d_slot_o, c->nwat, P, c->d_erf, c->d_facc,
-                                                    c->d_epart, c->max_parts);
+    k_water_f64<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, c->d_jpos,
+                                                    (const double *)c->d_wpos, c->nwat, P, c->d_erf,
+                                                    c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
   }
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+// one thread per (water slot, gradient component): the slot's partials --
+// its jpart rows (stored contiguously, in item order, at the slot's CSR
+// range) and its entries in the ipart rows of its block's chunks -- summed
+// as integers and added to the fixed-point accumulator.  The slot's atoms
+// are touched by no other thread, so the update needs no atomics.
+__global__ __launch_bounds__(256) void k_water_reduce(
+    const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
+    const int32_t *__restrict__ cfirst, const long long *__restrict__ ipart,
+    const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 9 * nwat) return;
+  const int s = g / 9, k = g - 9 * s;
+  long long acc = 0;
+  const int q1 = jstart[s + 1];
+  for (int q = jstart[s]; q < q1; q++) acc += jpart[(size_t)q * 9 + k];
+  const int b = s / WBLOCK, i = s % WBLOCK;
+  for (int c = cfirst[b]; c < cfirst[b + 1]; c++) acc += ipart[(size_t)c * (WBLOCK * 9) + i * 9 + k];
+  facc[3 * (size_t)slot_o[s] + k] += acc;
+}
+
+int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st) {
+  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
+  k_water_reduce<<<(9 * c->nwat + 255) / 256, 256, 0, st>>>(c->d_jstart, c->d_jpart, c->d_cfirst,
+                                                             c->d_ipart, c->d_slot_o, c->nwat,
+                                                             c->d_facc);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

@@ -325,37 +325,96 @@ def _rand_unit(rng):
     return v / np.linalg.norm(v)
 
 
-def _grow_chain(rng, n, start, L):
+def _grow_chain(rng, n, start, L, dmin=0.0):
     """Random walk with bond U(1.0,1.5) A, angle U(100,120) deg and
-    uniform torsion."""
+    uniform torsion; with dmin > 0 each step is redrawn (up to 100 times)
+    until the new atom is at least dmin from every atom four or more bonds
+    back along the chain."""
     pts = [start]
     d = _rand_unit(rng)
     pts.append(start + rng.uniform(1.0, 1.5) * d)
     for k in range(2, n):
-        b = rng.uniform(1.0, 1.5)
-        th = math.radians(rng.uniform(100.0, 120.0))
-        phi = rng.uniform(-math.pi, math.pi)
-        p1, p2 = pts[-2], pts[-1]
-        u = p2 - p1
-        u /= np.linalg.norm(u)
-        if k == 2:
-            ref = _rand_unit(rng)
-        else:
-            ref = pts[-3] - p1
-        n1 = np.cross(u, ref)
-        if np.linalg.norm(n1) < 1e-8:
-            n1 = np.cross(u, _rand_unit(rng))
-        n1 /= np.linalg.norm(n1)
-        n2 = np.cross(n1, u)
-        dvec = (-math.cos(th) * u + math.sin(th) * (math.cos(phi) * n2 + math.sin(phi) * n1))
-        pts.append(p2 + b * dvec)
+        for _try in range(100):
+            x = _chain_step(rng, pts, k)
+            if dmin <= 0 or k < 4:
+                break
+            d2 = ((np.array(pts[:k - 3]) - x) ** 2).sum(1)
+            if d2.min() >= dmin * dmin:
+                break
+        pts.append(x)
     return np.array(pts)
+
+
+def _chain_step(rng, pts, k):
+    """Next atom of the walk: bond U(1.0,1.5) A, angle U(100,120) deg,
+    uniform torsion."""
+    b = rng.uniform(1.0, 1.5)
+    th = math.radians(rng.uniform(100.0, 120.0))
+    phi = rng.uniform(-math.pi, math.pi)
+    p1, p2 = pts[-2], pts[-1]
+    u = p2 - p1
+    u /= np.linalg.norm(u)
+    ref = _rand_unit(rng) if k == 2 else pts[-3] - p1
+    n1 = np.cross(u, ref)
+    if np.linalg.norm(n1) < 1e-8:
+        n1 = np.cross(u, _rand_unit(rng))
+    n1 /= np.linalg.norm(n1)
+    n2 = np.cross(n1, u)
+    dvec = (-math.cos(th) * u + math.sin(th) * (math.cos(phi) * n2 + math.sin(phi) * n1))
+    return p2 + b * dvec
+
+
+class _CellHash:
+    """Periodic cell hash of placed atoms for minimum-image contact checks."""
+
+    def __init__(self, L, h):
+        self.L = L
+        self.n = max(1, int(L // h))
+        self.h = L / self.n
+        self.cells = {}
+
+    def _key(self, x):
+        return tuple(int(np.floor((x[k] + 0.5 * self.L) / self.h)) % self.n for k in range(3))
+
+    def add(self, x):
+        self.cells.setdefault(self._key(x), []).append(np.array(x, float))
+
+    def too_close(self, x, dmin):
+        k = self._key(x)
+        d2min = dmin * dmin
+        r = range(-1, 2) if self.n >= 3 else range(0, self.n)
+        for a in r:
+            for b in r:
+                for c in r:
+                    for y in self.cells.get(((k[0] + a) % self.n, (k[1] + b) % self.n,
+                                             (k[2] + c) % self.n), ()):
+                        d = x - y
+                        d -= self.L * np.round(d / self.L)
+                        if float(d @ d) < d2min:
+                            return True
+        return False
+
+
+def _chain_contacts_ok(xyz, hashed, L, dmin):
+    """True if no atom of the chain is closer than dmin (minimum image) to
+    an atom of a chain already placed, or to an atom of its own chain four
+    or more bonds away."""
+    for a in range(len(xyz)):
+        if hashed.too_close(xyz[a], dmin):
+            return False
+        for b in range(a + 4, len(xyz)):
+            d = xyz[a] - xyz[b]
+            d -= L * np.round(d / L)
+            if float(d @ d) < dmin * dmin:
+                return False
+    return True
 
 
 def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = None,
                   chain_len=(10, 20), n_types: int = 24, mono_size: int = 4,
                   v14: float = 0.5, el14: float = 0.5,
-                  water_min_dist: float = 2.5) -> System:
+                  water_min_dist: float = 2.5, chain_min_dist: float = 3.0,
+                  frac_neutral: float = 0.0, frac_hlike: float = 0.0) -> System:
     """Random-walk heteropolymer chains plus 3-site water in a cubic box
     (centred at the origin).  Bonds/angles/proper torsions from chain
     topology (k_b U(300,500), k_theta U(50,100), torsion period n in
@@ -364,7 +423,13 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
     eps~U(0.02,0.2), Rmin/2~U(1.0,2.1); charges N(0,0.4) shifted to net
     zero per chain.  Monomers hold <= mono_size consecutive chain atoms.
     Exclusions 1-2, 1-3 and 1-4 (MOIL excludes 1-4 and re-adds them via
-    the scaled special list, connect/angtor.f:483-577)."""
+    the scaled special list, connect/angtor.f:483-577).
+
+    Chains are redrawn until no chain atom is closer than chain_min_dist
+    (minimum image) to another chain atom four or more bonds away.
+    frac_neutral of the chain atoms get charge 0 (uncharged: MOIL's list2)
+    and frac_hlike get zero LJ (MOIL's hydrogen rule, pot/nbond.f:115-126);
+    both default to 0, the plain heteropolymer recipe."""
     ss = np.random.SeedSequence(seed)
     rng_walk, rng_par, rng_wat, rng_rot = [np.random.default_rng(s) for s in ss.spawn(4)]
     types_eps = rng_par.uniform(0.02, 0.2, n_types)
@@ -372,27 +437,40 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
     tb = _TopoBuilder()
     chain_xyz = []
     chain_atoms = []
+    hashed = _CellHash(L, max(chain_min_dist, 1.0))
     for c in range(n_chains):
         n = int(rng_walk.integers(chain_len[0], chain_len[1] + 1))
-        start = rng_walk.uniform(-0.5 * L + 3.0, 0.5 * L - 3.0, 3)
-        xyz = _grow_chain(rng_walk, n, start, L)
-        # keep the chain's atoms inside the primary box
-        lo, hi = xyz.min(0), xyz.max(0)
-        shift = np.where(hi > 0.5 * L - 0.5, 0.5 * L - 0.5 - hi, 0.0)
-        shift = np.where(lo + shift < -0.5 * L + 0.5, -0.5 * L + 0.5 - lo, shift)
-        xyz = xyz + shift
+        for _attempt in range(200):
+            start = rng_walk.uniform(-0.5 * L + 3.0, 0.5 * L - 3.0, 3)
+            xyz = _grow_chain(rng_walk, n, start, L, dmin=chain_min_dist)
+            # keep the chain's atoms inside the primary box
+            lo, hi = xyz.min(0), xyz.max(0)
+            shift = np.where(hi > 0.5 * L - 0.5, 0.5 * L - 0.5 - hi, 0.0)
+            shift = np.where(lo + shift < -0.5 * L + 0.5, -0.5 * L + 0.5 - lo, shift)
+            xyz = xyz + shift
+            if chain_min_dist <= 0 or _chain_contacts_ok(xyz, hashed, L, chain_min_dist):
+                break
+        else:
+            raise RuntimeError('hetero_system: no contact-free placement for chain %d' % c)
+        for x in xyz:
+            hashed.add(x)
         q = rng_par.normal(0.0, 0.4, n)
-        q = q - q.mean()
+        neutral = rng_par.random(n) < frac_neutral
+        neutral[-1] = False  # the last atom absorbs the quantisation residual
+        q[neutral] = 0.0
+        q[~neutral] -= q[~neutral].mean()
         q = np.array([_q('9.5f', v) for v in q])
         # make the quantised charges sum to exactly zero per chain
         resid = round(float(np.sum(q)) * 1e5)
         q[-1] = _q('9.5f', q[-1] - resid * 1e-5)
         typ = rng_par.integers(0, n_types, n)
+        hlike = rng_par.random(n) < frac_hlike
         base = tb.npt + 1
         k = 0
         while k < n:
             m = min(mono_size, n - k)
-            atoms = [('C%d' % ((k + t) % 10), 12.011, q[k + t], types_eps[typ[k + t]],
+            atoms = [('C%d' % ((k + t) % 10), 12.011, q[k + t],
+                      0.0 if hlike[k + t] else types_eps[typ[k + t]],
                       types_rmh[typ[k + t]]) for t in range(m)]
             tb.add_monomer('PLY%d' % (c % 10) if m == mono_size else 'PLE%d' % (c % 10), atoms)
             k += m

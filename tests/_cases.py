@@ -1,0 +1,23 @@
+"""The parity systems of the golden fixtures (tests/golden/gen_golden.py):
+name -> (system maker, coordinate modifier, squeeze before the lists)."""
+from moil_fnrfd_amd import configs
+from tests.golden.gen_golden import perturb, shift_out
+
+CASES = {
+    'water216': (configs.water216, None, False),
+    'water216_out': (configs.water216, shift_out, False),
+    'water216_out_sqz': (configs.water216, shift_out, True),
+    'water1000_buffered': (configs.water_buffered, None, False),
+    'water1000_dense': (lambda: configs.water_scaled(10, seed=1100), None, False),
+    'hetero_small': (configs.hetero_small, lambda s: perturb(s, 0.05, 2001), True),
+    'hetero_small_out': (lambda: configs.hetero_small(seed=2100, n_chains=6, L=20.0),
+                         lambda s: perturb(shift_out(s, n_out=20, seed=78), 0.05, 2101), False),
+}
+
+
+def make_case(name):
+    mk, mod, sqz = CASES[name]
+    s, cut, ew = mk()
+    if mod:
+        s = mod(s)
+    return s, cut, ew, sqz

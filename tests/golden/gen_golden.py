@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 from moil_fnrfd_amd import configs  # noqa: E402
-from oracle.oracle import ref_canonical_water_pairs, sort_rows  # noqa: E402
+from oracle.oracle import ref_canonical_atom_pairs, ref_canonical_water_pairs, sort_rows  # noqa: E402
 from oracle.ref_runner import run_moil  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -53,14 +53,24 @@ def shift_out(s, n_out=12, seed=77):
     for w in rng.choice(len(s.topo.idxtip3), n_out, replace=False):
         ax = rng.integers(0, 3)
         sign = rng.choice([-1.0, 1.0])
-        x[3 * w:3 * w + 3, ax] += sign * L
+        a0 = int(s.topo.poipt[s.topo.idxtip3[w] - 1])
+        x[a0:a0 + 3, ax] += sign * L
     s.coor = x
+    return s
+
+
+def perturb(s, sigma, seed):
+    """Gaussian displacement of every atom (moves bonded terms off their
+    equilibrium values)."""
+    rng = np.random.default_rng(seed)
+    s.coor = s.coor + rng.normal(0.0, sigma, s.coor.shape)
     return s
 
 
 def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None):
     d = run_moil(s, cut, ew, squeeze=squeeze)
     rows = sort_rows(ref_canonical_water_pairs(d))
+    arows = sort_rows(ref_canonical_atom_pairs(d))
     out = dict(
         coor_sha=np.frombuffer(hashlib.sha256(np.ascontiguousarray(s.coor, np.float64).tobytes())
                                .hexdigest().encode(), np.uint8),
@@ -69,6 +79,9 @@ def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None
         pairs_sha=np.frombuffer(rows_sha(rows).encode(), np.uint8),
         n_pairs=np.array([len(rows), int((rows[:, 3] == 1).sum()), int((rows[:, 3] == 2).sum()),
                           int((rows[:, 2] > 0).sum())], np.int64),
+        apairs_sha=np.frombuffer(rows_sha(arows).encode(), np.uint8),
+        n_apairs=np.array([len(arows)] + [int((arows[:, 3] == k).sum()) for k in (1, 2, 3)] +
+                          [int((arows[:, 2] > 0).sum())], np.int64),
         topo_sha=np.frombuffer(topo_checksum(s.topo).encode(), np.uint8),
         erftab_head=d['ERFTAB'].reshape(-1, 4)[:64],
         erftab_rows=np.array([len(d['ERFTAB']) // 4], np.int64),
@@ -87,12 +100,13 @@ def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None
         out['dpot_sumsq'] = np.array([(dp ** 2).sum()])
     if full_rows:
         out['pairs'] = rows
+        out['apairs'] = arows
         out['coor'] = s.coor.astype(np.float64)
         out['coor_after'] = d['COOR'].reshape(-1, 3)
     path = os.path.join(OUT, name + '.npz')
     np.savez_compressed(path, **out)
-    print('%-22s npt=%6d pairs=%8d e_total=%.10f -> %s (%d bytes)'
-          % (name, s.topo.npt, len(rows), d['ENERGIES'][14], os.path.basename(path),
+    print('%-22s npt=%6d pairs=%8d atom pairs=%8d e_total=%.10f -> %s (%d bytes)'
+          % (name, s.topo.npt, len(rows), len(arows), d['ENERGIES'][14], os.path.basename(path),
              os.path.getsize(path)))
 
 
@@ -109,6 +123,10 @@ def main():
     write_case('water1000_dense', s, cut, ew, full_rows=False)
     s, cut, ew = configs.water32k()
     write_case('water32k', s, cut, ew, full_rows=False, dpot_subset=4242)
+    s, cut, ew = configs.hetero_small()
+    write_case('hetero_small', perturb(s, 0.05, 2001), cut, ew, squeeze=True)
+    s, cut, ew = configs.hetero_small(seed=2100, n_chains=6, L=20.0)
+    write_case('hetero_small_out', perturb(shift_out(s, n_out=20, seed=78), 0.05, 2101), cut, ew)
 
 
 if __name__ == '__main__':

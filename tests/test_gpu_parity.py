@@ -16,7 +16,7 @@ import pytest
 from moil_fnrfd_amd import abi, configs
 from tests._util import (energy_errors, golden_energies, load_golden, rms_rel, rows_sha,
                          sort_rows)
-from tests.golden.gen_golden import shift_out
+from tests._cases import CASES, make_case
 
 pytestmark = pytest.mark.gpu
 
@@ -35,22 +35,13 @@ def gpu():
     return MoilGPU
 
 
-SMALL = {
-    'water216': (configs.water216, None, False),
-    'water216_out': (configs.water216, shift_out, False),
-    'water216_out_sqz': (configs.water216, shift_out, True),
-    'water1000_buffered': (configs.water_buffered, None, False),
-    'water1000_dense': (lambda: configs.water_scaled(10, seed=1100), None, False),
-}
+SMALL = CASES
 
 
 @pytest.mark.parametrize('name', sorted(SMALL))
 def test_fp64_matches_moil(gpu, name):
     _say('case %s FP64' % name)
-    mk, mod, sqz = SMALL[name]
-    s, cut, ew = mk()
-    if mod:
-        s = mod(s)
+    s, cut, ew, sqz = make_case(name)
     g = load_golden(name)
     m = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
     x = m.squeeze(s.coor) if sqz else s.coor
@@ -61,6 +52,13 @@ def test_fp64_matches_moil(gpu, name):
     if 'pairs' in g:
         assert np.array_equal(rows, g['pairs'])
     assert rows_sha(rows) == bytes(g['pairs_sha']).decode()
+    arows = sort_rows(m.atom_pairs())
+    if 'apairs' in g:
+        assert np.array_equal(arows, g['apairs'])
+    assert rows_sha(arows) == bytes(g['apairs_sha']).decode()
+    cnt = m.list_counts()
+    assert int(cnt[4] + cnt[5] + cnt[6]) == int(g['n_apairs'][0])
+    assert int(cnt[7]) == int(g['n_apairs'][4])
     e, dp = m.eforce(x)
     ref = golden_energies(g)
     assert m.ewald_info()['beta'] == ref['ewaldcof']
@@ -144,4 +142,55 @@ def test_energy_conservation_properties_32k(gpu):
     e, dp = m.eforce(s.coor)
     net = dp.sum(0)
     assert np.all(np.abs(net) < 1e-3 * np.sqrt((dp ** 2).sum(1)).max())
+    m.close()
+
+
+@pytest.mark.parametrize('name', ['hetero_small', 'hetero_small_out'])
+def test_mixed_precision_hetero(gpu, name):
+    """Mixed mode on the atom lists (list1/list2/list3, images both ways)
+    against the FP64 path, and run-to-run bit reproducibility."""
+    _say('case %s MIXED vs FP64' % name)
+    s, cut, ew, sqz = make_case(name)
+    m64 = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+    x = m64.squeeze(s.coor) if sqz else s.coor
+    m64.nbondm(x)
+    e64, f64 = m64.eforce(x)
+    m64.close()
+    m32 = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+    m32.nbondm(x)
+    e32, f32 = m32.eforce(x)
+    e32b, f32b = m32.eforce(x)
+    m32.close()
+    scale = sum(abs(e64[k]) for k in ('e_vdw', 'e_el', 'e_vsym', 'e_lsym', 'e_corr', 'e_self',
+                                      'e_ew_receip', 'e_bond', 'e_theta', 'e_tors', 'e_imp',
+                                      'e_el14', 'e_vdw14'))
+    rel_e = abs(e32['e_total'] - e64['e_total']) / scale
+    rel_f = rms_rel(f32, f64)
+    _say('  e_total rel %.3e  force rms-rel %.3e' % (rel_e, rel_f))
+    assert rel_e < 1e-6
+    assert rel_f < 1e-5
+    assert np.array_equal(f32, f32b) and e32 == e32b
+
+
+def test_hetero_lists_follow_perturbed_coordinates(gpu):
+    """Atom and water lists rebuilt on drifting heteropolymer coordinates
+    match the oracle's exactly."""
+    from oracle.oracle import Oracle
+    s, cut, ew, _ = make_case('hetero_small')
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_MIXED)
+    m = gpu(s.topo, p)
+    o = Oracle(s.topo, p)
+    rng = np.random.default_rng(19)
+    x = s.coor.copy()
+    for step in range(3):
+        _say('case hetero_small perturbation %d' % step)
+        x = x + rng.normal(0.0, 0.2, x.shape)
+        xg = m.squeeze(x)
+        xo = o.squeeze(x)
+        assert np.array_equal(xg, xo)
+        m.nbondm(xg)
+        o.build_lists(xo)
+        assert np.array_equal(sort_rows(m.water_pairs()), sort_rows(o.water_pairs()))
+        assert np.array_equal(sort_rows(m.atom_pairs()), sort_rows(o.atom_pairs()))
+        x = xg
     m.close()

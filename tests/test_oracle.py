@@ -15,23 +15,11 @@ from moil_fnrfd_amd import abi, configs
 from oracle.oracle import Oracle
 from tests._util import (energy_errors, golden_energies, load_golden, rms_rel, rows_sha,
                          sort_rows)
-from tests.golden.gen_golden import shift_out, topo_checksum
-
-CASES = {
-    'water216': (configs.water216, None, False),
-    'water216_out': (configs.water216, shift_out, False),
-    'water216_out_sqz': (configs.water216, shift_out, True),
-    'water1000_buffered': (configs.water_buffered, None, False),
-    'water1000_dense': (lambda: configs.water_scaled(10, seed=1100), None, False),
-}
-
+from tests._cases import CASES, make_case
+from tests.golden.gen_golden import topo_checksum
 
 def _case(name):
-    mk, mod, sqz = CASES[name]
-    s, cut, ew = mk()
-    if mod:
-        s = mod(s)
-    return s, cut, ew, sqz
+    return make_case(name)
 
 
 @pytest.mark.parametrize('name', sorted(CASES))
@@ -39,6 +27,8 @@ def test_oracle_matches_moil(name):
     g = load_golden(name)
     s, cut, ew, sqz = _case(name)
     assert topo_checksum(s.topo) == bytes(g['topo_sha']).decode()
+    if 'coor' in g:
+        assert np.array_equal(s.coor, g['coor'])
     o = Oracle(s.topo, configs.params_for(s, cut, ew))
     x = o.squeeze(s.coor) if sqz else s.coor
     if 'coor_after' in g:
@@ -48,6 +38,10 @@ def test_oracle_matches_moil(name):
     if 'pairs' in g:
         assert np.array_equal(rows, g['pairs'])
     assert rows_sha(rows) == bytes(g['pairs_sha']).decode()
+    arows = sort_rows(o.atom_pairs())
+    if 'apairs' in g:
+        assert np.array_equal(arows, g['apairs'])
+    assert rows_sha(arows) == bytes(g['apairs_sha']).decode()
     e, dp = o.eforce(x)
     ref = golden_energies(g)
     info = o.ewald_info()
