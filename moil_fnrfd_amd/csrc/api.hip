@@ -175,6 +175,11 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       return rc;               \
     }                          \
   } while (0)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    ctx_free(c);
+    return fail(MOIL_GPU_EHIP, "no HIP device available");
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||

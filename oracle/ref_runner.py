@@ -17,10 +17,13 @@ from moil_fnrfd_amd.systems import System, write_con
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_EXE = os.path.join(HERE, '_ref', 'moil_dump')
+# the same MOIL driver and objects with MOIL's energy path replaced by the
+# product's Fortran shim over libmoil_gpu.so (oracle/ref/Makefile, target gpu)
+SHIM_EXE = os.path.join(HERE, '_ref', 'moil_dump_gpu')
 
 
-def available() -> bool:
-    return os.path.exists(REF_EXE)
+def available(exe=REF_EXE) -> bool:
+    return os.path.exists(exe)
 
 
 def parse_dump(path: str) -> dict:
@@ -41,10 +44,12 @@ def parse_dump(path: str) -> dict:
 
 def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
              nlist=1, sigma=0.0, seed=12345, dump_lists=True, workdir=None,
-             timeout=3600) -> dict:
-    """cut: rvmx relx rvbg rebg cutm (Angstrom); ewald: dtol, nfft (3), order."""
-    if not available():
-        raise RuntimeError('oracle/_ref/moil_dump is not built (make -f oracle/ref/Makefile)')
+             timeout=3600, exe=REF_EXE, env=None) -> dict:
+    """cut: rvmx relx rvbg rebg cutm (Angstrom); ewald: dtol, nfft (3), order.
+    exe: the MOIL driver to run (REF_EXE, or SHIM_EXE for MOIL calling the
+    GPU library); env: extra environment variables."""
+    if not available(exe):
+        raise RuntimeError('%s is not built (make -f oracle/ref/Makefile)' % exe)
     tmp = workdir or tempfile.mkdtemp(prefix='moilref_')
     os.makedirs(tmp, exist_ok=True)
     con = os.path.join(tmp, 'sys.con')
@@ -69,8 +74,10 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
         # MOIL keeps large work arrays on the stack (e.g. symm/nbmsym.f:36-46)
         resource.setrlimit(resource.RLIMIT_STACK,
                            (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
-    r = subprocess.run([REF_EXE, par], cwd=tmp, capture_output=True, text=True,
-                       timeout=timeout, preexec_fn=_big_stack)
+    full_env = dict(os.environ)
+    full_env.update(env or {})
+    r = subprocess.run([exe, par], cwd=tmp, capture_output=True, text=True,
+                       timeout=timeout, preexec_fn=_big_stack, env=full_env)
     if r.returncode != 0 or not os.path.exists(dmp):
         raise RuntimeError('moil_dump failed (%d):\n%s\n%s' % (r.returncode, r.stdout[-3000:],
                                                                 r.stderr[-3000:]))

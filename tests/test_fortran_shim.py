@@ -1,0 +1,42 @@
+"""MOIL's own Fortran calling the GPU library: the dump driver and MOIL's
+objects (rconn, init, timing loop) linked with the ISO_C_BINDING shim
+(moil_fnrfd_amd/fortran/moil_gpu_shim.f) in place of MOIL's ewald_init,
+squeeze, nbondm, syminit and eforce (oracle/ref/Makefile, target gpu).
+Its energies and gradient must match MOIL's CPU run (the golden fixtures)
+to the FP64 bars, and to the mixed-precision bars in the default mode."""
+import numpy as np
+import pytest
+
+from oracle import ref_runner
+from tests._cases import make_case
+from tests._util import energy_errors, golden_energies, load_golden, rms_rel
+from tests.golden.gen_golden import ENERGY_FIELDS
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not ref_runner.available(ref_runner.SHIM_EXE),
+                                 reason='oracle/_ref/moil_dump_gpu not built')]
+
+
+@pytest.mark.parametrize('name', ['water216_out_sqz', 'hetero_small'])
+@pytest.mark.parametrize('fp64', [True, False])
+def test_moil_driver_through_shim(name, fp64):
+    print('case %s shim fp64=%s' % (name, fp64), flush=True)
+    s, cut, ew, sqz = make_case(name)
+    g = load_golden(name)
+    d = ref_runner.run_moil(s, cut, ew, squeeze=sqz, dump_lists=False, timeout=300,
+                            exe=ref_runner.SHIM_EXE, env={'MOIL_GPU_FP64': '1' if fp64 else '0'})
+    e = dict(zip(ENERGY_FIELDS, d['ENERGIES']))
+    ref = golden_energies(g)
+    assert np.array_equal(d['COOR'].reshape(-1, 3), g['coor_after'])  # squeeze on the GPU
+    assert e['ewaldcof'] == ref['ewaldcof']
+    dp = d['DPOT'].reshape(-1, 3)
+    if fp64:
+        err = energy_errors(e, ref)
+        assert max(err.values()) < 1e-12, err
+        assert rms_rel(dp, g['dpot']) < 1e-10
+    else:
+        scale = sum(abs(ref[k]) for k in ('e_bond', 'e_theta', 'e_tors', 'e_imp', 'e_vdw', 'e_el',
+                                          'e_vsym', 'e_lsym', 'e_el14', 'e_vdw14', 'e_corr',
+                                          'e_self', 'e_ew_receip'))
+        assert abs(e['e_total'] - ref['e_total']) / scale < 1e-6
+        assert rms_rel(dp, g['dpot']) < 1e-5
