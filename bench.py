@@ -33,6 +33,19 @@ def algorithmic_bytes_realspace(n_water_pairs, npt):
     return 4 * E + 16 * Sj + 16 * npt + 24 * npt
 
 
+def measured_traffic():
+    """HBM bytes per launch of k_water_f32 from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy of known
+    size); None if the summary is absent."""
+    path = os.path.join(ROOT, 'profiles', 'k_water_f32_water32k_counters.md')
+    try:
+        import re
+        m = re.search(r'traffic_bytes_per_launch: (\d+)', open(path).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(sysm, cut, ew):
     """MOIL's own Fortran (oracle/_ref) on the host cores: one list build
     plus a bounded number of eforce calls; rate with the list build
@@ -116,26 +129,36 @@ def main():
     coor = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
     dpot = torch.empty_like(coor)
     energ = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
-    # per-step N(0, 0.01 A) displacements: drawn on the device each step
-    # (float32 normals, added to the float64 coordinates)
+    # per-step N(0, 0.01 A) displacements: float64 normals drawn on the
+    # device NLIST steps at a time into a resident buffer
     gen = torch.Generator(device=dev)
     gen.manual_seed(1001)
-    noise = torch.empty(coor.shape, dtype=torch.float32, device=dev)
+    noise = torch.empty((NLIST,) + tuple(coor.shape), dtype=torch.float64, device=dev)
+    ev_list = []
 
     def step(k, base=0):
-        if k > 0:
-            torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float32, out=noise)
-            coor.add_(noise, alpha=SIGMA)
         if k % NLIST == 0:
+            torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float64, out=noise)
+        if k > 0:
+            coor.add_(noise[k % NLIST], alpha=SIGMA)
+        if k % NLIST == 0:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
             eng.squeeze_dev(coor.data_ptr(), sh)
             eng.nbondm_dev(coor.data_ptr(), sh)
+            e1.record(stream)
+            ev_list.append((e0, e1))
         eng.eforce_dev(coor.data_ptr(), dpot.data_ptr(), energ.data_ptr(), sh)
 
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
     n_pairs = int(eng.list_counts()[:4].sum())
-    eng.set_timing(True)
+    ev_list.clear()
+    # timed region: level-1 timing (the real-space kernel's own dispatch
+    # events, nothing inserted between kernels)
+    eng.set_timing(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -143,7 +166,15 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = eng.stage_times()
-    eng.set_timing(False)
+    t_list_ms = [a.elapsed_time(b) for a, b in ev_list]
+    # stage breakdown from a short untimed pass with events between stages
+    eng.set_timing(2)
+    for k in range(NLIST):
+        step(k + 1, args.warmup + args.steps)
+    torch.cuda.synchronize()
+    st2 = eng.stage_times()
+    eng.set_timing(0)
+    n2 = max(st2[5], 1.0)
     e_last = energ.cpu().numpy()
     elapsed = t1 - t0
     evals_per_s = args.steps / elapsed
@@ -172,8 +203,9 @@ def main():
             'atoms': sysm.topo.npt,
             'water_pairs': n_pairs,
             'list_rebuild_every': NLIST,
-            'stage_ms': {'realspace_water': t_rs_ms, 'pme': st[2] / n_ev,
-                         'bonded_excl': st[3] / n_ev, 'eforce_total': st[4] / n_ev},
+            'stage_ms': {'realspace_water': t_rs_ms, 'pme': st2[2] / n2,
+                         'bonded_excl': st2[3] / n2, 'eforce_total': st2[4] / n2,
+                         'list_rebuild': (sum(t_list_ms) / len(t_list_ms)) if t_list_ms else None},
             'e_total_last': float(e_last[0]),
         },
         'roofline': {
@@ -182,7 +214,7 @@ def main():
             'peak': peak,
             'unit': 'GB/s',
             'frac': achieved / peak,
-            'traffic': None,
+            'traffic': measured_traffic(),
             'kernel': 'k_water_f32',
             'algorithmic_bytes_per_launch': B_rs,
         },

@@ -156,8 +156,10 @@ int32_t moil_gpu_eforce(moil_ctx *ctx, const double *coor, double *dpot,
 
 /* Device-resident variants: d_coor/d_dpot are device pointers to
  * (3,npt) float64 arrays, d_energies a device pointer to
- * MOIL_N_ENERGIES float64 in moil_energies order; `stream` is a
- * hipStream_t (NULL = the context's own stream).  Asynchronous. */
+ * MOIL_N_ENERGIES float64 in moil_energies order; `stream` is the
+ * caller's hipStream_t (NULL = the default null stream); the work is
+ * ordered after the caller's earlier work on that stream.  Asynchronous
+ * (build_lists_dev synchronizes the stream internally). */
 int32_t moil_gpu_squeeze_dev(moil_ctx *ctx, double *d_coor, void *stream);
 int32_t moil_gpu_build_lists_dev(moil_ctx *ctx, const double *d_coor, void *stream);
 int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
@@ -177,13 +179,15 @@ int32_t moil_gpu_dump_water_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
 int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
                                  int64_t *n);
 
-/* Stage timing with HIP events on the launch stream.  While timing is on
- * (moil_gpu_set_timing, which also clears), every eforce records its
- * stages; moil_gpu_stage_times returns the summed milliseconds since the
- * last call: t[0] real-space water kernel, t[1] real-space atom kernel,
- * t[2] PME, t[3] bonded + exclusion + 1-4, t[4] whole eforce, t[5] the
- * number of eforce calls timed. */
-int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t on);
+/* Stage timing with HIP events on the launch stream.  moil_gpu_set_timing
+ * (which also clears) selects level 0 (off), 1 (the real-space kernels,
+ * timed by events their own dispatch records: no extra work between
+ * kernels) or 2 (also PME, bonded and the whole eforce, by event records
+ * between stages); moil_gpu_stage_times returns the summed milliseconds
+ * since the last call: t[0] real-space water kernel, t[1] real-space atom
+ * kernel, t[2] PME, t[3] bonded + exclusion + 1-4, t[4] whole eforce,
+ * t[5] the number of eforce calls timed. */
+int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t level);
 int32_t moil_gpu_stage_times(moil_ctx *ctx, double t[8]);
 
 #ifdef __cplusplus

@@ -482,7 +482,12 @@ __global__ void k_squeeze(double *__restrict__ coor, const int32_t *__restrict__
   }
 }
 
-static hipStream_t pick_stream(moil_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+// the caller's stream; NULL is the default (null) stream, so device-pointer
+// calls stay ordered with the caller's own work on that stream
+static hipStream_t pick_stream(moil_ctx *c, void *s) {
+  (void)c;
+  return (hipStream_t)s;
+}
 
 extern "C" int32_t moil_gpu_squeeze_dev(moil_ctx *c, double *d_coor, void *stream) {
   if (!c || !d_coor) return fail(MOIL_GPU_EINVAL, "null argument");
@@ -612,8 +617,10 @@ struct StageTimer {
   int stage;
   hipStream_t st;
   hipEvent_t a = nullptr;
+  // stage events only at timing level 2 (an event record between kernels
+  // costs a few microseconds of dispatch gap)
   StageTimer(moil_ctx *c_, int s_, hipStream_t st_) : c(c_), stage(s_), st(st_) {
-    if (c->timing && (a = ev_get(c))) hipEventRecord(a, st);
+    if (c->timing >= 2 && (a = ev_get(c))) hipEventRecord(a, st);
   }
   ~StageTimer() {
     if (!a) return;
@@ -640,6 +647,7 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
     c->facc_dirty = false;
   }
   c->facc_dirty = true;  // until the finalize kernel is enqueued
+  if (c->timing >= 1) c->n_timed++;
   StageTimer whole(c, 4, st);
   // reciprocal space: in order on the launch stream by default; with
   // pme_concurrent on a second stream overlapped with bonded + real space
@@ -664,15 +672,23 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   const int nwpart = (nb && c->nwat >= 2) ? c->nwchunk : 0;
   if (nb) {
     CHECK_RC(realspace_water_pack(c, d_coor, st));
-    {
-      StageTimer tw(c, 0, st);
-      CHECK_RC(realspace_water(c, d_coor, st));
+    // the real-space kernels are timed at any timing level by events that
+    // their own dispatch records (no extra packets between kernels)
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (c->timing >= 1 && c->nwat >= 2 && c->nwchunk > 0) {
+      ea = ev_get(c);
+      eb = ev_get(c);
     }
+    CHECK_RC(realspace_water(c, d_coor, st, ea, eb));
+    if (ea && eb) c->ev_rec.push_back({0, {ea, eb}});
     CHECK_RC(realspace_water_reduce(c, st));
-    {
-      StageTimer ta(c, 1, st);
-      CHECK_RC(realspace_atoms(c, d_coor, st, nwpart));
+    ea = eb = nullptr;
+    if (c->timing >= 1 && c->nachunk > 0) {
+      ea = ev_get(c);
+      eb = ev_get(c);
     }
+    CHECK_RC(realspace_atoms(c, d_coor, st, nwpart, ea, eb));
+    if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
     CHECK_RC(realspace_atoms_reduce(c, st));
   }
   if (nb && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
@@ -824,17 +840,18 @@ extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t 
   return MOIL_GPU_OK;
 }
 
-extern "C" int32_t moil_gpu_set_timing(moil_ctx *c, int32_t on) {
+extern "C" int32_t moil_gpu_set_timing(moil_ctx *c, int32_t level) {
   if (!c) return fail(MOIL_GPU_EINVAL, "null context");
-  c->timing = on != 0;
+  c->timing = level < 0 ? 0 : (level > 2 ? 2 : level);
   c->ev_rec.clear();
   c->ev_next = 0;
+  c->n_timed = 0;
   return MOIL_GPU_OK;
 }
 
-// t[stage] = summed milliseconds since the last call, t[4+stage]...:
-// t[0] water real space, t[1] atom real space, t[2] PME, t[3] bonded,
-// t[4] whole eforce, t[5] number of eforce calls timed
+// t[stage] = summed milliseconds since the last call: t[0] water real
+// space, t[1] atom real space (level >= 1); t[2] PME, t[3] bonded, t[4]
+// whole eforce (level 2); t[5] number of eforce calls timed
 extern "C" int32_t moil_gpu_stage_times(moil_ctx *c, double t[8]) {
   if (!c || !t) return fail(MOIL_GPU_EINVAL, "null argument");
   for (int k = 0; k < 8; k++) t[k] = 0.0;
@@ -843,9 +860,10 @@ extern "C" int32_t moil_gpu_stage_times(moil_ctx *c, double t[8]) {
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, r.second.first, r.second.second));
     if (r.first >= 0 && r.first < 5) t[r.first] += ms;
-    if (r.first == 4) t[5] += 1.0;
   }
+  t[5] = (double)c->n_timed;
   c->ev_rec.clear();
   c->ev_next = 0;
+  c->n_timed = 0;
   return MOIL_GPU_OK;
 }

@@ -32,6 +32,7 @@
 // q_l q_m (kofdie/eps) erfc(beta r)/r.  Gradients go to per-item and
 // per-chunk partial rows (no global atomics) that one reduction pass adds
 // to the fixed-point accumulators.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -382,16 +383,14 @@ __global__ void k_ajcount(const AItem *__restrict__ items, size_t n, int32_t *__
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) atomicAdd(&cnt[items[q].j], 1);
 }
+// jpos[item] = the item's row in the partial buffer, inside its partner
+// atom's range (rows are summed as integers, so their order is free)
 __global__ void k_ajfill(const AItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
-                         int32_t *__restrict__ fill, int32_t *__restrict__ jlist) {
+                         int32_t *__restrict__ fill, int32_t *__restrict__ jpos) {
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int j = items[q].j;
-  jlist[start[j] + atomicAdd(&fill[j], 1)] = (int32_t)q;
-}
-__global__ void k_ajpos(const int32_t *__restrict__ jlist, size_t n, int32_t *__restrict__ jpos) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) jpos[jlist[q]] = (int32_t)q;
+  jpos[q] = start[j] + atomicAdd(&fill[j], 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,11 +416,12 @@ __device__ __forceinline__ float a_erfc_poly(float t) {
 }
 
 __device__ __forceinline__ long long a_fix32(float f) {
-  // round(f 2^32) from FP32 operations (valid for |f| < 2^23)
-  const float x = f * 256.0f;
-  const float hr = rintf(x);
-  const float r = x - hr;
-  return ((long long)(int)hr << 24) + (long long)(int)rintf(r * 16777216.0f);
+  // floor(f 2^32) from FP32 operations: high word floor(f), low word
+  // fract(f) 2^32 (valid for |f| < 2^31)
+  const float fl = floorf(f);
+  const unsigned hi = (unsigned)(int)fl;
+  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
+  return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 __device__ inline double a_block_sum(double v, double *red) {
@@ -594,9 +594,9 @@ __global__ __launch_bounds__(ABT) void k_atoms(
   if (t == 0) epart[EN_LSYM * max_parts + part0 + ch] = v;
 }
 
-// one thread per (atom, component): the atom's partner rows (contiguous,
-// in item order) and, for a block atom, its entries in its block's chunk
-// rows, added to its fixed-point accumulator
+// one thread per (atom, component): the atom's partner rows (contiguous)
+// and, for a block atom, its entries in its block's chunk rows, all
+// 64-bit fixed point, summed as integers and added to its accumulator
 __global__ __launch_bounds__(256) void k_atoms_reduce(
     const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
     const int32_t *__restrict__ slot_of, const int32_t *__restrict__ cfirst,
@@ -682,7 +682,7 @@ void atoms_free(moil_ctx *c) {
                   c->d_sol, c->d_acell_of, c->d_acnt, c->d_astart, c->d_aslot2atom,
                   c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
                   c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajstart,
-                  c->d_ajlist, c->d_ajpos, c->d_acfirst};
+                  c->d_ajpos, c->d_acfirst};
   for (void *p : ptrs)
     if (p) hipFree(p);
 }
@@ -739,12 +739,10 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   const size_t total = (size_t)hoff.back();
   if (total > c->aitem_cap || c->d_aitems == nullptr) {
     hipFree(c->d_aitems);
-    hipFree(c->d_ajlist);
     hipFree(c->d_ajpos);
     hipFree(c->d_ajpart);
     c->aitem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
     HIP_CHECK(hipMalloc(&c->d_aitems, sizeof(AItem) * c->aitem_cap));
-    HIP_CHECK(hipMalloc(&c->d_ajlist, sizeof(int32_t) * c->aitem_cap));
     HIP_CHECK(hipMalloc(&c->d_ajpos, sizeof(int32_t) * c->aitem_cap));
     HIP_CHECK(hipMalloc(&c->d_ajpart, sizeof(long long) * 3 * c->aitem_cap));
   }
@@ -779,13 +777,15 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     }
   }
   c->nachunk = (int)ch.size() / 3;
-  hipFree(c->d_achunk);
-  c->d_achunk = nullptr;
   std::vector<int32_t> cfirst(c->nablk + 1, c->nachunk);
   for (int q = c->nachunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
   for (int b = c->nablk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
   if (c->nachunk > 0) {
-    HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * ch.size()));
+    if (c->nachunk > c->achunk_cap || !c->d_achunk) {
+      hipFree(c->d_achunk);
+      c->achunk_cap = c->nachunk + c->nachunk / 4 + 64;
+      HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * 3 * (size_t)c->achunk_cap));
+    }
     HIP_CHECK(hipMemcpyAsync(c->d_achunk, ch.data(), sizeof(int32_t) * ch.size(),
                              hipMemcpyHostToDevice, st));
   }
@@ -802,9 +802,7 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     k_ajcount<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajcnt);
     k_scan_excl<<<1, 1024, 0, st>>>(c->d_ajcnt, c->d_ajstart, c->npt);
     HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
-    k_ajfill<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajstart, c->d_ajcnt, c->d_ajlist);
-    k_cell_sort<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_ajstart, c->npt, c->d_ajlist);
-    k_ajpos<<<ib, TB, 0, st>>>(c->d_ajlist, total, c->d_ajpos);
+    k_ajfill<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajstart, c->d_ajcnt, c->d_ajpos);
   } else {
     HIP_CHECK(hipMemsetAsync(c->d_ajstart, 0, sizeof(int32_t) * (c->npt + 1), st));
   }
@@ -812,7 +810,8 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   return MOIL_GPU_OK;
 }
 
-int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0) {
+int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0,
+                        hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (c->nachunk == 0) return MOIL_GPU_OK;
   ARS P;
   const uint32_t T = c->par.terms;
@@ -825,15 +824,19 @@ int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int p
   P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale;
   if (c->par.precision == MOIL_PREC_MIXED)
-    k_atoms<false><<<c->nachunk, ABT, 0, st>>>(c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom,
-                                               c->nsol, c->d_ptchg, c->d_eps12, c->d_eps6, c->d_erf,
-                                               P, c->d_ajpos, c->d_ajpart, c->d_aipart,
-                                               c->d_epart, c->max_parts, part0);
+    hipExtLaunchKernelGGL(k_atoms<false>, dim3(c->nachunk), dim3(ABT), 0, st, ev_start, ev_stop, 0,
+                          (const int32_t *)c->d_achunk, (const AItem *)c->d_aitems, d_coor,
+                          (const int32_t *)c->d_sslot2atom, c->nsol, (const double *)c->d_ptchg,
+                          (const double *)c->d_eps12, (const double *)c->d_eps6,
+                          (const double *)c->d_erf, P, (const int32_t *)c->d_ajpos, c->d_ajpart,
+                          c->d_aipart, c->d_epart, c->max_parts, part0);
   else
-    k_atoms<true><<<c->nachunk, ABT, 0, st>>>(c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom,
-                                              c->nsol, c->d_ptchg, c->d_eps12, c->d_eps6, c->d_erf,
-                                              P, c->d_ajpos, c->d_ajpart, c->d_aipart,
-                                              c->d_epart, c->max_parts, part0);
+    hipExtLaunchKernelGGL(k_atoms<true>, dim3(c->nachunk), dim3(ABT), 0, st, ev_start, ev_stop, 0,
+                          (const int32_t *)c->d_achunk, (const AItem *)c->d_aitems, d_coor,
+                          (const int32_t *)c->d_sslot2atom, c->nsol, (const double *)c->d_ptchg,
+                          (const double *)c->d_eps12, (const double *)c->d_eps6,
+                          (const double *)c->d_erf, P, (const int32_t *)c->d_ajpos, c->d_ajpart,
+                          c->d_aipart, c->d_epart, c->max_parts, part0);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

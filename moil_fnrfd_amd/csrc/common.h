@@ -41,6 +41,15 @@ struct __attribute__((aligned(32))) WItem {
   uint64_t pad;
 };
 
+// Candidate record of the water-list builder: partner slot j, symmetry
+// operation s and the unsplit listwt1/listwt2 masks over the block.
+struct __attribute__((aligned(8))) WRaw {
+  int32_t j;
+  int32_t s;
+  uint64_t m1;
+  uint64_t m2;
+};
+
 // Atom-pair work item over a 64-atom solute block: partner atom `j`
 // (0-based), direction `s` (0 real, 1..13 the block atom is the image
 // under symop s, 14..26 the partner is the image under symop s-13), and

@@ -90,17 +90,21 @@ struct moil_ctx {
   int nwblk = 0;                   // water blocks of WBLOCK slots
   int32_t *d_bcount = nullptr;     // [nwblk*NBUCKET+1]
   int32_t *d_boff = nullptr;       // [nwblk*NBUCKET+1]
+  WRaw *d_wraw = nullptr;          // [nwblk][wraw_cap] candidate records
+  int wraw_cap = 0;
+  int32_t *d_rcount = nullptr;     // [nwblk] records per block
   WItem *d_witems = nullptr;
   size_t witem_cap = 0, nwitems = 0;
   int32_t *d_wchunk = nullptr;     // [nchunk][3]: block, begin, end
-  int nwchunk = 0;
+  int nwchunk = 0, wchunk_cap = 0;
   // gradient partials of the real-space water kernel (no global atomics):
-  // jpart[item][9] (the item's j-water), ipart[chunk][64][9] (the chunk's
-  // block waters), both 64-bit fixed point; jstart/jlist: items per slot j
+  // jpart[row][9] (an item's j-water; the rows of one water contiguous),
+  // ipart[chunk][64][9] (the chunk's block waters), both 64-bit fixed
+  // point; jstart: row range per slot j, jpos: row of each item
   long long *d_jpart = nullptr, *d_ipart = nullptr;
   int ipart_cap = 0;
-  int32_t *d_jcnt = nullptr, *d_jstart = nullptr, *d_jlist = nullptr;
-  int32_t *d_jpos = nullptr;       // [item] row of the item in jpart (CSR position)
+  int32_t *d_jcnt = nullptr, *d_jstart = nullptr;
+  int32_t *d_jpos = nullptr;       // [item] row of the item in jpart
   int32_t *d_cfirst = nullptr;     // [nwblk+1] first chunk of each block
   int wchunk_pairs = 4096;         // pair budget of one real-space workgroup
   int wchunk_max_items = 2048;
@@ -127,10 +131,11 @@ struct moil_ctx {
   AItem *d_aitems = nullptr;
   size_t aitem_cap = 0, naitems = 0;
   int32_t *d_achunk = nullptr;
-  int nachunk = 0;
-  long long *d_ajpart = nullptr, *d_aipart = nullptr;  // [item][3], [chunk][64][3]
+  int nachunk = 0, achunk_cap = 0;
+  long long *d_ajpart = nullptr;         // [row][3] partner rows, fixed point
+  long long *d_aipart = nullptr;         // [chunk][64][3] fixed point
   int aipart_cap = 0;
-  int32_t *d_ajcnt = nullptr, *d_ajstart = nullptr, *d_ajlist = nullptr, *d_acfirst = nullptr;
+  int32_t *d_ajcnt = nullptr, *d_ajstart = nullptr, *d_acfirst = nullptr;
   int32_t *d_ajpos = nullptr;
 
   // ---- accumulators ----
@@ -145,7 +150,8 @@ struct moil_ctx {
   bool have_lists = false;
 
   // ---- timing (event pairs per stage, summed on request) ----
-  bool timing = false;
+  int timing = 0;                  // 0 off, 1 real-space kernels, 2 all stages
+  long n_timed = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;  // stage, (start, stop)
   size_t ev_next = 0;
@@ -157,13 +163,15 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st);
 void lists_free(moil_ctx *c);
 // real space (realspace.hip)
 int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st);
-int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st);
+int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st, hipEvent_t ev_start,
+                        hipEvent_t ev_stop);
 int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st);
 // atom lists and atom real space (atoms.hip)
 int32_t atoms_init(moil_ctx *c, const moil_topology *t);
 int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st);
 void atoms_free(moil_ctx *c);
-int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0);
+int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0,
+                        hipEvent_t ev_start, hipEvent_t ev_stop);
 int32_t realspace_atoms_reduce(moil_ctx *c, hipStream_t st);
 // PME (pme.hip)
 int32_t pme_init(moil_ctx *c);

@@ -187,13 +187,88 @@ __device__ inline uint64_t take_low_bits(uint64_t &m, int n) {
   return out;
 }
 
-// One workgroup per water block.  WRITE=false: count items per bucket.
-// WRITE=true: write items at boff + deterministic rank.
+// Deterministic emission of one batch of candidates (one per thread of
+// the workgroup, mask 0 = none): each mask is split into nfull items of
+// WSPLIT pairs (bucket 0) and at most one remainder item (bucket WSPLIT -
+// popcount remainder); per-bucket positions come from wavefront scans and
+// ballots in thread order, so the item order is fixed.  WRITE=false only
+// advances the per-bucket counts.  Every thread of the workgroup calls it.
+struct EmitLDS {
+  int32_t bk_run[NBUCKET];
+  int32_t wcnt[LB_THREADS / 64][NBUCKET];
+};
+
 template <bool WRITE>
+__device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
+                                  const int32_t *__restrict__ boff_b, WItem *__restrict__ items) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint64_t mm = m1 | m2;
+  const int pc = __popcll(mm);
+  const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
+  const int bk = rem ? WSPLIT - rem : -1;
+  int incl = nfull;
+  for (int off = 1; off < 64; off <<= 1) {
+    int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const int wtot0 = __shfl(incl, 63, 64);
+  if (lane == 0) L.wcnt[wv][0] = wtot0;
+  for (int k = 1; k < NBUCKET; k++) {
+    uint64_t bal = __ballot(bk == k);
+    if (lane == 0) L.wcnt[wv][k] = __popcll(bal);
+  }
+  __syncthreads();
+  if (WRITE) {
+    uint64_t rest = mm;  // bits left for the remainder item
+    if (nfull > 0) {
+      int before0 = 0;
+      for (int w2 = 0; w2 < wv; w2++) before0 += L.wcnt[w2][0];
+      const int pos0 = boff_b[0] + L.bk_run[0] + before0 + (incl - nfull);
+      for (int q = 0; q < nfull; q++) {
+        const uint64_t piece = take_low_bits(rest, WSPLIT);
+        WItem it;
+        it.j = js;
+        it.s = s;
+        it.m1 = m1 & piece;
+        it.m2 = m2 & piece;
+        it.pad = 0;
+        items[pos0 + q] = it;
+      }
+    }
+    for (int k = 1; k < NBUCKET; k++) {
+      uint64_t bal = __ballot(bk == k);
+      if (bk == k) {
+        int before = 0;
+        for (int w2 = 0; w2 < wv; w2++) before += L.wcnt[w2][k];
+        uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+        const int pos = boff_b[k] + L.bk_run[k] + before + __popcll(lower);
+        WItem it;
+        it.j = js;
+        it.s = s;
+        it.m1 = m1 & rest;
+        it.m2 = m2 & rest;
+        it.pad = 0;
+        items[pos] = it;
+      }
+    }
+  }
+  __syncthreads();
+  if (t < NBUCKET) {
+    int tot = 0;
+    for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += L.wcnt[w2][t];
+    L.bk_run[t] += tot;
+  }
+  __syncthreads();
+}
+
+// One workgroup per water block: evaluates MOIL's predicates for every
+// candidate partner once, keeps the non-empty (j, s, masks) records in
+// candidate order in the block's slice of `raw` (capacity rcap; the count
+// is exact even when the slice overflows) and counts the items per bucket.
 __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     const double *__restrict__ scent, const int32_t *__restrict__ smono,
     const int32_t *__restrict__ cstart, Grid g, WPar P, const int32_t *__restrict__ flags,
-    int32_t *__restrict__ bcount, const int32_t *__restrict__ boff, WItem *__restrict__ items) {
+    int32_t *__restrict__ bcount, WRaw *__restrict__ raw, int rcap, int32_t *__restrict__ rcount) {
   __shared__ double ic[WBLOCK][3];
   __shared__ int32_t im[WBLOCK];
   __shared__ double bb[6];
@@ -201,8 +276,9 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   __shared__ int32_t cellk[LB_MAXCELLS];
   __shared__ int32_t cellpre[LB_MAXCELLS + 1];
   __shared__ int32_t scanbuf[LB_THREADS];
-  __shared__ int32_t bk_run[NBUCKET];
-  __shared__ int32_t wcnt[LB_THREADS / 64][NBUCKET];
+  __shared__ EmitLDS L;
+  __shared__ int32_t rw[LB_THREADS / 64];
+  __shared__ int32_t r_run;
 
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int s0 = b * WBLOCK;
@@ -217,7 +293,8 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
       im[t] = 0x7fffffff;
     }
   }
-  if (t < NBUCKET) bk_run[t] = 0;
+  if (t < NBUCKET) L.bk_run[t] = 0;
+  if (t == 0) r_run = 0;
   __syncthreads();
   if (t == 0) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -233,6 +310,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   }
   __syncthreads();
   const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
+  WRaw *rblk = raw + (size_t)b * rcap;
 
   for (int s = 0; s < 14; s++) {
     int ia, jb, kc;
@@ -267,9 +345,6 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     const int ncq = nx * ny * nz;
     for (int cbeg = 0; cbeg < ncq; cbeg += LB_MAXCELLS) {
       const int ncc = min(LB_MAXCELLS, ncq - cbeg);
-      // gather cell keys and sizes for this chunk of cells; cells whose
-      // box lies farther than R from the translated block bbox are empty
-      // for this query (sizes set to 0)
       for (int u = t; u < LB_MAXCELLS; u += LB_THREADS) {
         int sz = 0, k = 0;
         if (u < ncc) {
@@ -338,74 +413,66 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
             }
           }
         }
-        // split into nfull items of WSPLIT pairs (bucket 0) and at most one
-        // remainder item of rem pairs (bucket WSPLIT - rem)
-        const uint64_t mm = m1 | m2;
-        const int pc = __popcll(mm);
-        const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
-        const int bk = rem ? WSPLIT - rem : -1;
-        // per-bucket wave counts (deterministic): bucket 0 by a wave scan
-        // of nfull, the others by ballots (one remainder item per lane)
-        int incl = nfull;
-        for (int off = 1; off < 64; off <<= 1) {
-          int v = __shfl_up(incl, off, 64);
-          if (lane >= off) incl += v;
-        }
-        const int wtot0 = __shfl(incl, 63, 64);
-        if (lane == 0) wcnt[wv][0] = wtot0;
-        for (int k = 1; k < NBUCKET; k++) {
-          uint64_t bal = __ballot(bk == k);
-          if (lane == 0) wcnt[wv][k] = __popcll(bal);
-        }
+        // keep the record in candidate order
+        const bool has = (m1 | m2) != 0;
+        const uint64_t bal = __ballot(has);
+        if (lane == 0) rw[wv] = __popcll(bal);
         __syncthreads();
-        uint64_t rest = mm;  // bits left for the remainder item
-        if (WRITE && nfull > 0) {
-          int before0 = 0;
-          for (int w2 = 0; w2 < wv; w2++) before0 += wcnt[w2][0];
-          int pos0 = boff[b * NBUCKET] + bk_run[0] + before0 + (incl - nfull);
-          for (int q = 0; q < nfull; q++) {
-            const uint64_t piece = take_low_bits(rest, WSPLIT);
-            WItem it;
-            it.j = js;
-            it.s = s;
-            it.m1 = m1 & piece;
-            it.m2 = m2 & piece;
-            it.pad = 0;
-            items[pos0 + q] = it;
-          }
-        }
-        if (WRITE) {
-          for (int k = 1; k < NBUCKET; k++) {
-            uint64_t bal = __ballot(bk == k);
-            if (bk == k) {
-              int before = 0;
-              for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
-              uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
-              int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
-              const uint64_t piece = rest;
-              WItem it;
-              it.j = js;
-              it.s = s;
-              it.m1 = m1 & piece;
-              it.m2 = m2 & piece;
-              it.pad = 0;
-              items[pos] = it;
-            }
+        if (has) {
+          int before = 0;
+          for (int w2 = 0; w2 < wv; w2++) before += rw[w2];
+          const uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+          const int pos = r_run + before + __popcll(lower);
+          if (pos < rcap) {
+            WRaw r;
+            r.j = js;
+            r.s = s;
+            r.m1 = m1;
+            r.m2 = m2;
+            rblk[pos] = r;
           }
         }
         __syncthreads();
-        if (t < NBUCKET) {
+        if (t == 0) {
           int tot = 0;
-          for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += wcnt[w2][t];
-          bk_run[t] += tot;
+          for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += rw[w2];
+          r_run += tot;
         }
-        __syncthreads();
+        emit_batch<false>(L, js, s, m1, m2, nullptr, nullptr);
       }
       __syncthreads();
     }
     __syncthreads();
   }
-  if (!WRITE && t < NBUCKET) bcount[b * NBUCKET + t] = bk_run[t];
+  if (t < NBUCKET) bcount[b * NBUCKET + t] = L.bk_run[t];
+  if (t == 0) rcount[b] = r_run;
+}
+
+// One workgroup per water block: reads the block's records back in order
+// and writes its items at boff + the same deterministic ranks.
+__global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restrict__ raw, int rcap,
+                                                             const int32_t *__restrict__ rcount,
+                                                             const int32_t *__restrict__ boff,
+                                                             WItem *__restrict__ items) {
+  __shared__ EmitLDS L;
+  const int b = blockIdx.x, t = threadIdx.x;
+  if (t < NBUCKET) L.bk_run[t] = 0;
+  __syncthreads();
+  const int n = rcount[b];
+  const WRaw *rblk = raw + (size_t)b * rcap;
+  for (int fb = 0; fb < n; fb += LB_THREADS) {
+    const int f = fb + t;
+    int js = -1, s = 0;
+    uint64_t m1 = 0, m2 = 0;
+    if (f < n) {
+      const WRaw r = rblk[f];
+      js = r.j;
+      s = r.s;
+      m1 = r.m1;
+      m2 = r.m2;
+    }
+    emit_batch<true>(L, js, s, m1, m2, boff + b * NBUCKET, items);
+  }
 }
 
 // items per partner slot j: the real-space kernel leaves each item's
@@ -415,17 +482,15 @@ __global__ void k_jcount(const WItem *__restrict__ items, size_t n, int32_t *__r
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) atomicAdd(&cnt[items[q].j], 1);
 }
+// jpos[item] = the item's row in the partial buffer, inside its partner's
+// CSR range (the order within a range does not matter: the rows are
+// summed as integers)
 __global__ void k_jfill(const WItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
-                        int32_t *__restrict__ fill, int32_t *__restrict__ jlist) {
+                        int32_t *__restrict__ fill, int32_t *__restrict__ jpos) {
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int j = items[q].j;
-  jlist[start[j] + atomicAdd(&fill[j], 1)] = (int32_t)q;
-}
-// jpos[item] = the item's position in the CSR (its partial row)
-__global__ void k_jpos(const int32_t *__restrict__ jlist, size_t n, int32_t *__restrict__ jpos) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) jpos[jlist[q]] = (int32_t)q;
+  jpos[q] = start[j] + atomicAdd(&fill[j], 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -482,9 +547,10 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_boff); c->d_boff = nullptr;
   hipFree(c->d_witems); c->d_witems = nullptr;
   hipFree(c->d_wchunk); c->d_wchunk = nullptr;
+  hipFree(c->d_wraw); c->d_wraw = nullptr;
+  hipFree(c->d_rcount); c->d_rcount = nullptr;
   hipFree(c->d_jcnt); c->d_jcnt = nullptr;
   hipFree(c->d_jstart); c->d_jstart = nullptr;
-  hipFree(c->d_jlist); c->d_jlist = nullptr;
   hipFree(c->d_jpos); c->d_jpos = nullptr;
   hipFree(c->d_jpart); c->d_jpart = nullptr;
   hipFree(c->d_ipart); c->d_ipart = nullptr;
@@ -526,30 +592,43 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     double R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0));
     wp.R = R * (1.0 + 1e-12) + 1e-9;
     wp.nwat = c->nwat;
-    k_wblock_items<false><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
-                                                           wp, c->d_flags, c->d_bcount,
-                                                           nullptr, nullptr);
-    k_scan_excl<<<1, 1024, 0, st>>>(c->d_bcount, c->d_boff, c->nwblk * NBUCKET);
-    std::vector<int32_t> hoff(c->nwblk * NBUCKET + 1);
-    HIP_CHECK(hipMemcpyAsync(hoff.data(), c->d_boff, sizeof(int32_t) * hoff.size(),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    // predicates once per candidate; records kept per block (capacity from
+    // the previous build, redone with the exact need if it overflowed)
+    if (!c->d_rcount) HIP_CHECK(hipMalloc(&c->d_rcount, sizeof(int32_t) * (c->nwblk + 1)));
+    std::vector<int32_t> hoff(c->nwblk * NBUCKET + 1), hrc(c->nwblk);
+    for (int attempt = 0; attempt < 2; attempt++) {
+      if (!c->d_wraw) {
+        c->wraw_cap = std::max(c->wraw_cap, 1024);
+        HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk));
+      }
+      k_wblock_items<<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
+                                                      c->d_flags, c->d_bcount, c->d_wraw,
+                                                      c->wraw_cap, c->d_rcount);
+      k_scan_excl<<<1, 1024, 0, st>>>(c->d_bcount, c->d_boff, c->nwblk * NBUCKET);
+      HIP_CHECK(hipMemcpyAsync(hoff.data(), c->d_boff, sizeof(int32_t) * hoff.size(),
+                               hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(hrc.data(), c->d_rcount, sizeof(int32_t) * hrc.size(),
+                               hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      const int need = *std::max_element(hrc.begin(), hrc.end());
+      if (need <= c->wraw_cap) break;
+      hipFree(c->d_wraw);
+      c->d_wraw = nullptr;
+      c->wraw_cap = need + need / 4 + 256;
+    }
     size_t total = (size_t)hoff.back();
     if (total > c->witem_cap || c->d_witems == nullptr) {
       hipFree(c->d_witems);
-      hipFree(c->d_jlist);
       hipFree(c->d_jpos);
       hipFree(c->d_jpart);
       c->witem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
       HIP_CHECK(hipMalloc(&c->d_witems, sizeof(WItem) * c->witem_cap));
-      HIP_CHECK(hipMalloc(&c->d_jlist, sizeof(int32_t) * c->witem_cap));
       HIP_CHECK(hipMalloc(&c->d_jpos, sizeof(int32_t) * c->witem_cap));
       HIP_CHECK(hipMalloc(&c->d_jpart, sizeof(long long) * 9 * c->witem_cap));
     }
     c->nwitems = total;
-    k_wblock_items<true><<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g,
-                                                          wp, c->d_flags, nullptr, c->d_boff,
-                                                          c->d_witems);
+    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, c->d_boff,
+                                                    c->d_witems);
     // chunk table: runs of a block's items (popcount-descending) holding
     // about wchunk_pairs pairs, so workgroups carry similar work
     std::vector<int32_t> ch;
@@ -578,14 +657,18 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
       }
     }
     c->nwchunk = (int)ch.size() / 3;
-    hipFree(c->d_wchunk);
-    c->d_wchunk = nullptr;
     // first chunk of every block (chunks are emitted block by block)
     std::vector<int32_t> cfirst(c->nwblk + 1, c->nwchunk);
     for (int q = c->nwchunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
     for (int b = c->nwblk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
     if (c->nwchunk > 0) {
-      HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * ch.size()));
+      // grow-only buffers: no device free/allocate (which synchronize) in
+      // the steady state
+      if (c->nwchunk > c->wchunk_cap || !c->d_wchunk) {
+        hipFree(c->d_wchunk);
+        c->wchunk_cap = c->nwchunk + c->nwchunk / 4 + 64;
+        HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * 3 * (size_t)c->wchunk_cap));
+      }
       HIP_CHECK(hipMemcpyAsync(c->d_wchunk, ch.data(), sizeof(int32_t) * ch.size(),
                                hipMemcpyHostToDevice, st));
     }
@@ -597,7 +680,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     if (!c->d_cfirst) HIP_CHECK(hipMalloc(&c->d_cfirst, sizeof(int32_t) * (c->nwblk + 1)));
     HIP_CHECK(hipMemcpyAsync(c->d_cfirst, cfirst.data(), sizeof(int32_t) * cfirst.size(),
                              hipMemcpyHostToDevice, st));
-    // CSR of the items by partner slot j, in item order within a slot
+    // row ranges of the partial buffer by partner slot j
     if (!c->d_jcnt) {
       HIP_CHECK(hipMalloc(&c->d_jcnt, sizeof(int32_t) * (c->nwat + 1)));
       HIP_CHECK(hipMalloc(&c->d_jstart, sizeof(int32_t) * (c->nwat + 1)));
@@ -608,9 +691,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
       k_jcount<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jcnt);
       k_scan_excl<<<1, 1024, 0, st>>>(c->d_jcnt, c->d_jstart, c->nwat);
       HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
-      k_jfill<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jstart, c->d_jcnt, c->d_jlist);
-      k_cell_sort<<<(c->nwat + TB - 1) / TB, TB, 0, st>>>(c->d_jstart, c->nwat, c->d_jlist);
-      k_jpos<<<ib, TB, 0, st>>>(c->d_jlist, total, c->d_jpos);
+      k_jfill<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jstart, c->d_jcnt, c->d_jpos);
     } else {
       HIP_CHECK(hipMemsetAsync(c->d_jstart, 0, sizeof(int32_t) * (c->nwat + 1), st));
     }

@@ -23,9 +23,9 @@
 // runs on the half spectrum with the influence function tabulated once in
 // FP64 at init; the reciprocal energy is summed in FP64 with
 // self-conjugate planes weighted once and all others twice.  The gather
-// walks the atoms in tile order (cache-friendly stencils), recomputes the
-// B-splines and derivatives from the coordinates, and writes each atom's
-// reciprocal gradient directly (no atomics).
+// runs one workgroup per tile with the tile + halo grid region in LDS,
+// recomputes the B-splines and derivatives from the coordinates, and
+// writes each atom's reciprocal gradient directly (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,8 +78,6 @@ __device__ __forceinline__ void frac_coord(double x, double r, int n, int *i0, d
   *w = f - (double)fi;
   *i0 = fi - 3;  // first stencil point int(fr)-3
 }
-
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Grid tiles: dimension k is cut into nt_k = ceil(n_k/8) tiles with
 // boundaries lo = floor(t n_k / nt_k) (sizes 4..8 for n_k >= 4, so a
@@ -210,10 +208,12 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tscatter(const double *__re
   sidx[p] = n;
 }
 
-// one workgroup per tile: every atom of the tile adds its 64 stencil
-// weights (one lane per stencil point) into the tile + halo region in
-// LDS, in 64-bit fixed point (order independent); the region is then
-// written to the tile's slot of tbuf
+// one workgroup per tile: the tile's atom records are staged in LDS
+// (coalesced loads, 256 atoms per pass), then every atom adds its 64
+// stencil weights (one lane per stencil point) into the tile + halo
+// region in LDS, in 64-bit fixed point (order independent); the region
+// is then written to the tile's slot of tbuf
+#define TS_STAGE 256
 template <typename R>
 __global__ __launch_bounds__(PME_THREADS) void k_pme_tspread(const int32_t *__restrict__ tstart,
                                                              const R *__restrict__ sw,
@@ -221,24 +221,32 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tspread(const int32_t *__re
                                                              TilePar T, double qscale,
                                                              long long *__restrict__ tbuf) {
   __shared__ unsigned long long acc[11 * 11 * 11];
+  __shared__ R rec[TS_STAGE * 12];
+  __shared__ int32_t rb[TS_STAGE];
   const int tile = blockIdx.x, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
   for (int u = t; u < T.rsz; u += PME_THREADS) acc[u] = 0;
-  __syncthreads();
   const int lane = t & 63, wv = t >> 6;
   const int la = lane & 3, lb = (lane >> 2) & 3, lc = lane >> 4;
   const int beg = tstart[tile], end = tstart[tile + 1];
-  for (int a = beg + wv; a < end; a += PME_THREADS / 64) {
-    const int bp = sbase[a];
-    const int b1 = bp & 1023, b2 = (bp >> 10) & 1023, b3 = bp >> 20;
-    // stencil point base-3+l has local coordinate base - lo + l
-    const int l1 = b1 - lo1 + la, l2 = b2 - lo2 + lb, l3 = b3 - lo3 + lc;
-    const R *o = sw + 12 * (size_t)a;
-    const R v = o[8 + lc] * o[4 + lb] * o[la];
-    const long long f = __double2ll_rn((double)v * qscale);
-    if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
+  for (int b0 = beg; b0 < end; b0 += TS_STAGE) {
+    const int n = min(TS_STAGE, end - b0);
+    __syncthreads();
+    for (int u = t; u < 12 * n; u += PME_THREADS) rec[u] = sw[12 * (size_t)b0 + u];
+    for (int u = t; u < n; u += PME_THREADS) rb[u] = sbase[b0 + u];
+    __syncthreads();
+    for (int a = wv; a < n; a += PME_THREADS / 64) {
+      const int bp = rb[a];
+      const int b1 = bp & 1023, b2 = (bp >> 10) & 1023, b3 = bp >> 20;
+      // stencil point base-3+l has local coordinate base - lo + l
+      const int l1 = b1 - lo1 + la, l2 = b2 - lo2 + lb, l3 = b3 - lo3 + lc;
+      const R *o = rec + 12 * a;
+      const R v = o[8 + lc] * o[4 + lb] * o[la];
+      const long long f = __double2ll_rn((double)v * qscale);
+      if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
+    }
   }
   __syncthreads();
   long long *dst = tbuf + (size_t)tile * T.rsz;
@@ -306,57 +314,75 @@ __global__ __launch_bounds__(PME_THREADS) void k_conv(R *__restrict__ S, const d
   if (t == 0) epart[blockIdx.x] = 0.5 * red[0];
 }
 
+// one workgroup per tile: the tile + halo region of the convolved grid
+// goes to LDS, then one thread per atom of the tile (tile order)
+// recomputes its B-splines and derivatives from the coordinates and sums
+// its 64 stencil points from LDS; each atom's reciprocal gradient is
+// written directly (no atomics)
 template <typename R>
 __global__ __launch_bounds__(PME_THREADS) void k_gather(const double *__restrict__ coor,
-                                                        const double *__restrict__ q, int npt,
-                                                        PmePar P, const int32_t *__restrict__ sidx,
+                                                        const double *__restrict__ q,
+                                                        const int32_t *__restrict__ tstart,
+                                                        const int32_t *__restrict__ sidx,
+                                                        PmePar P, TilePar T,
                                                         const R *__restrict__ grid,
                                                         double *__restrict__ fpme) {
-  // atoms in counting-sort order: neighbouring threads read overlapping
-  // grid stencils
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npt) return;
-  const int n = sidx[p];
-  double qn = q[n] * P.fc;
-  if (qn == 0.0) {
-    fpme[3 * n] = fpme[3 * n + 1] = fpme[3 * n + 2] = 0.0;
-    return;
+  __shared__ R reg[11 * 11 * 11];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
+  const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
+            lo3 = tile_lo(tt3, T.n3, T.nt3);
+  for (int u = t; u < T.rsz; u += PME_THREADS) {
+    const int l1 = u % T.r1, l2 = (u / T.r1) % T.r2, l3 = u / (T.r1 * T.r2);
+    // local l <-> grid point lo - 3 + l (periodic); points past the tile's
+    // own extent are never read
+    int g1 = lo1 - 3 + l1, g2 = lo2 - 3 + l2, g3 = lo3 - 3 + l3;
+    g1 = g1 < 0 ? g1 + T.n1 : (g1 >= T.n1 ? g1 - T.n1 : g1);
+    g2 = g2 < 0 ? g2 + T.n2 : (g2 >= T.n2 ? g2 - T.n2 : g2);
+    g3 = g3 < 0 ? g3 + T.n3 : (g3 >= T.n3 ? g3 - T.n3 : g3);
+    reg[u] = grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
   }
-  int i0, j0, k0;
-  double w1, w2, w3;
-  frac_coord(coor[3 * n], P.ra, P.n1, &i0, &w1);
-  frac_coord(coor[3 * n + 1], P.rb, P.n2, &j0, &w2);
-  frac_coord(coor[3 * n + 2], P.rc, P.n3, &k0, &w3);
-  R t1[4], t2[4], t3[4], d1[4], d2[4], d3[4];
-  bspline4<R>((R)w1, t1, d1);
-  bspline4<R>((R)w2, t2, d2);
-  bspline4<R>((R)w3, t3, d3);
-  double f1 = 0, f2 = 0, f3 = 0;
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int k = wrapi(k0 + c, P.n3);
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int j = wrapi(j0 + b, P.n2);
-      const R *row = grid + ((size_t)k * P.n2 + j) * P.n1;
-      R g1 = 0, g2 = 0, g3 = 0;
-#pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const int i = wrapi(i0 + a, P.n1);
-        const R Q = row[i];
-        g1 += Q * d1[a];
-        g2 += Q * t1[a];
-      }
-      g3 = g2;
-      f1 += (double)(g1 * t2[b] * t3[c]);
-      f2 += (double)(g2 * d2[b] * t3[c]);
-      f3 += (double)(g3 * t2[b] * d3[c]);
+  __syncthreads();
+  const int beg = tstart[tile], end = tstart[tile + 1];
+  for (int p = beg + t; p < end; p += PME_THREADS) {
+    const int n = sidx[p];
+    const double qn = q[n] * P.fc;
+    if (qn == 0.0) {
+      fpme[3 * n] = fpme[3 * n + 1] = fpme[3 * n + 2] = 0.0;
+      continue;
     }
+    int c1, c2, c3;
+    double w1, w2, w3;
+    base_point(coor + 3 * n, P, c1, c2, c3, w1, w2, w3);
+    R t1[4], t2[4], t3[4], d1[4], d2[4], d3[4];
+    bspline4<R>((R)w1, t1, d1);
+    bspline4<R>((R)w2, t2, d2);
+    bspline4<R>((R)w3, t3, d3);
+    // stencil point base-3+a has local coordinate base - lo + a
+    const int b1 = c1 - lo1, b2 = c2 - lo2, b3 = c3 - lo3;
+    double f1 = 0, f2 = 0, f3 = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const R *row = reg + ((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1;
+        R g1 = 0, g2 = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          const R Q = row[a];
+          g1 += Q * d1[a];
+          g2 += Q * t1[a];
+        }
+        f1 += (double)(g1 * t2[b] * t3[c]);
+        f2 += (double)(g2 * d2[b] * t3[c]);
+        f3 += (double)(g2 * t2[b] * d3[c]);
+      }
+    }
+    // dE/dfr_k = qn * sum(...) * n_k ; dpot -= recip * f with f = -dE/dfr
+    fpme[3 * n] = qn * f1 * (double)P.n1 * P.ra;
+    fpme[3 * n + 1] = qn * f2 * (double)P.n2 * P.rb;
+    fpme[3 * n + 2] = qn * f3 * (double)P.n3 * P.rc;
   }
-  // dE/dfr_k = qn * sum(...) * n_k ; dpot -= recip * f with f = -dE/dfr
-  fpme[3 * n] = qn * f1 * (double)P.n1 * P.ra;
-  fpme[3 * n + 1] = qn * f2 * (double)P.n2 * P.rb;
-  fpme[3 * n + 2] = qn * f3 * (double)P.n3 * P.rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -563,8 +589,8 @@ static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
   void *in2[1] = {c->d_qcplx}, *out2[1] = {c->d_qreal};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_bwd, st));
   RF_CHECK(rocfft_execute(c->plan_bwd, in2, out2, c->exec_bwd));
-  k_gather<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_sidx,
-                                          (const R *)c->d_qreal, c->d_fpme);
+  k_gather<R><<<c->pt_ntile, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->d_tstart, c->d_sidx, P, T,
+                                                   (const R *)c->d_qreal, c->d_fpme);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

@@ -11,13 +11,14 @@
 // (partner water j plus a mask of at most 16 bits over the block) and
 // walks the mask.  Items come sorted by popcount, so the lanes of a
 // wavefront run loops of equal length.  j-forces accumulate in registers
-// and are stored once per item into the item's row of a partial buffer
-// (rows ordered by partner water, so each water's rows are contiguous); i-forces accumulate in LDS with 64-bit integer atomics and are
+// and are stored once per item, in fixed point, into the item's row of a
+// partial buffer (rows grouped by partner water, so each water's rows are
+// contiguous); i-forces accumulate in LDS with 64-bit integer atomics and are
 // stored once per workgroup into the chunk's row.  No global atomics: on
 // this GPU they are performed at the memory side and cost far more than
 // the stores plus one gather pass (k_water_reduce) that sums each water's
-// rows in a fixed order.  All partials are 64-bit fixed point, so the
-// forces are independent of execution order (bit-reproducible).
+// rows.  All partials are 64-bit fixed point, so the forces are
+// independent of execution order (bit-reproducible).
 //
 // MOIL_PREC_MIXED: FP32 pair arithmetic.  Positions are carried as the O
 // atom (float hi + float lo) and the H offsets from their O, so every
@@ -25,9 +26,11 @@
 // (relative error ~1e-7 of the O-O distance instead of the box size).
 // erfc(x) = exp(-x^2) t P(t), t = 1/(1+p x): the functional form of
 // Abramowitz & Stegun 7.1.26 with a degree-8 polynomial fitted here for
-// relative accuracy 3e-8 on [0, 6].
+// relative accuracy 3e-8 on [0, 6].  Forces are rounded per term to
+// 64-bit fixed point with 32 fractional bits (floor) before any sum.
 // MOIL_PREC_FP64: FP64 arithmetic in MOIL's operation order with MOIL's
 // cubic-spline erfc table (pot/cdie_ewald.f:152-160).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "ctx.h"
@@ -60,18 +63,15 @@ __device__ __forceinline__ long long fix64(double v, double scale) {
   return __double2ll_rn(v * scale);
 }
 
-// round(f * 2^32) as h * 2^24 + l with int32 h, l (|l| <= 2^23), from FP32
-// operations only: x = f 2^8 (exact), h = rint(x), r = x - h is exact
-// (|r| <= 1/2), l = rint(r 2^24).  Valid for |f| < 2^23.
-__device__ __forceinline__ void fix_split(float f, int &h, int &l) {
-  const float x = f * 256.0f;
-  const float hr = rintf(x);
-  const float r = x - hr;
-  h = (int)hr;
-  l = (int)rintf(r * 16777216.0f);
-}
-__device__ __forceinline__ long long fix_join(int h, int l) {
-  return ((long long)h << 24) + (long long)l;
+// floor(f * 2^32) as a 64-bit integer from FP32 operations only: the high
+// word is floor(f), the low word fract(f) 2^32 (f - floor(f) is exact and
+// below 1, its product with 2^32 truncates to the floor).  The two 32-bit
+// halves are the int64 directly, no shifts.  Valid for |f| < 2^31.
+__device__ __forceinline__ long long fix32(float f) {
+  const float fl = floorf(f);
+  const unsigned hi = (unsigned)(int)fl;
+  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
+  return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 // Each lane walks its block mask starting at its own lane number (bit b of
@@ -147,13 +147,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
   __syncthreads();
 
-  const float beta = (float)P.beta;
+  // x' = beta sqrt(log2 e) r: exp(-beta^2 r^2) = exp2(-x'^2), and
+  // 1 + p beta r = 1 + p' x' with p' = p / sqrt(log2 e)
+  const double sl2e = sqrt(1.4426950408889634);
+  const float betap = (float)(P.beta * sl2e);
+  const float pe = (float)(ERFC_P / sl2e);
   const float qoo = (float)P.qq_oo, qoh = (float)P.qq_oh, qhh = (float)P.qq_hh;
+  const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
+  const float qcoo = qoo * c2b, qcoh = qoh * c2b, qchh = qhh * c2b;
   const float a12 = (float)P.oo12, b6 = (float)P.oo6;
   const float cutvdw2 = (float)P.cutvdw2, cutele2 = (float)P.cutele2;
-  const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
-  const float pe = ERFC_P;
-  const float log2e = 1.4426950408889634f;
   const int rot = t & 63;
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
@@ -203,15 +206,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
           const float rx = ux[sa] - vx[sb], ry = uy[sa] - vy[sb], rz = uz[sa] - vz[sb];
           const float r2 = (sa == 0 && sb == 0) ? roo2 : (rx * rx + ry * ry + rz * rz);
           const float qq = (sa == 0 && sb == 0) ? qoo : ((sa == 0 || sb == 0) ? qoh : qhh);
+          const float qc = (sa == 0 && sb == 0) ? qcoo : ((sa == 0 || sb == 0) ? qcoh : qchh);
           const float s = __builtin_amdgcn_rsqf(r2);
           const float s2 = s * s;
-          const float x = beta * (r2 * s);
-          const float tt = __builtin_amdgcn_rcpf(fmaf(pe, x, 1.0f));
-          const float ex = __builtin_amdgcn_exp2f(-(x * x) * log2e);
-          const float erfcv = ex * erfc_poly(tt);
-          const float e = qq * erfcv * s;
+          const float xp = betap * (r2 * s);
+          const float tt = __builtin_amdgcn_rcpf(fmaf(pe, xp, 1.0f));
+          const float ex = __builtin_amdgcn_exp2f(-(xp * xp));
+          const float e = qq * (ex * erfc_poly(tt) * s);
           eel += e;
-          float df = -s2 * (e + qq * c2b * ex);
+          float df = -s2 * fmaf(qc, ex, e);
           if (sa == 0 && sb == 0 && !k2 && roo2 <= cutvdw2) {
             const float s6 = s2 * s2 * s2;
             const float A12 = a12 * s6 * s6, B6 = b6 * s6;
@@ -219,29 +222,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
             elj += e1;
             df += -6.0f * s2 * (A12 + e1);
           }
-          const float fx = df * rx, fy = df * ry, fz = df * rz;
-          fi[3 * sa] += fx;
-          fi[3 * sa + 1] += fy;
-          fi[3 * sa + 2] += fz;
-          fj[3 * sb] -= fx;
-          fj[3 * sb + 1] -= fy;
-          fj[3 * sb + 2] -= fz;
+          fi[3 * sa] = fmaf(df, rx, fi[3 * sa]);
+          fi[3 * sa + 1] = fmaf(df, ry, fi[3 * sa + 1]);
+          fi[3 * sa + 2] = fmaf(df, rz, fi[3 * sa + 2]);
+          fj[3 * sb] = fmaf(-df, rx, fj[3 * sb]);
+          fj[3 * sb + 1] = fmaf(-df, ry, fj[3 * sb + 1]);
+          fj[3 * sb + 2] = fmaf(-df, rz, fj[3 * sb + 2]);
         }
       }
 #pragma unroll
-      for (int k = 0; k < 9; k++) {
-        int h, l;
-        fix_split(fi[k], h, l);
-        atomicAdd(&iacc[i * 9 + k], (unsigned long long)fix_join(h, l));
-      }
+      for (int k = 0; k < 9; k++) atomicAdd(&iacc[i * 9 + k], (unsigned long long)fix32(fi[k]));
     }
     long long *jrow = jpart + (size_t)jpos[it] * 9;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      int h, l;
-      fix_split(fj[k], h, l);
-      jrow[k] = fix_join(h, l);
-    }
+    for (int k = 0; k < 9; k++) jrow[k] = fix32(fj[k]);
     if (item.s == 0) {
       e_lj_r += (double)elj;
       e_el_r += (double)eel;
@@ -390,7 +384,8 @@ int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st) 
   return MOIL_GPU_OK;
 }
 
-int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st) {
+int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st, hipEvent_t ev_start,
+                        hipEvent_t ev_stop) {
   if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
   (void)d_coor;
   WaterRS P;
@@ -399,24 +394,29 @@ int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.beta = c->beta; P.cutvdw2 = c->par.cutvdw2; P.cutele2 = c->par.cutele2;
   P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale; P.ntab = c->ntab;
+  // launched through hipExtLaunchKernelGGL so optional start/stop events
+  // are recorded by the dispatch itself
   if (c->par.precision == MOIL_PREC_MIXED) {
-    k_water_f32<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, c->d_jpos,
-                                                    (const float4 *)c->d_wpos, c->nwat, P,
-                                                    c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
+    hipExtLaunchKernelGGL(k_water_f32, dim3(c->nwchunk), dim3(RS_THREADS), 0, st, ev_start, ev_stop,
+                          0, (const int32_t *)c->d_wchunk, (const WItem *)c->d_witems,
+                          (const int32_t *)c->d_jpos, (const float4 *)c->d_wpos, c->nwat, P,
+                          c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
   } else {
-    k_water_f64<<<c->nwchunk, RS_THREADS, 0, st>>>(c->d_wchunk, c->d_witems, c->d_jpos,
-                                                    (const double *)c->d_wpos, c->nwat, P, c->d_erf,
-                                                    c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
+    hipExtLaunchKernelGGL(k_water_f64, dim3(c->nwchunk), dim3(RS_THREADS), 0, st, ev_start, ev_stop,
+                          0, (const int32_t *)c->d_wchunk, (const WItem *)c->d_witems,
+                          (const int32_t *)c->d_jpos, (const double *)c->d_wpos, c->nwat, P,
+                          (const double *)c->d_erf, c->d_jpart, c->d_ipart, c->d_epart,
+                          c->max_parts);
   }
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
 
 // one thread per (water slot, gradient component): the slot's partials --
-// its jpart rows (stored contiguously, in item order, at the slot's CSR
-// range) and its entries in the ipart rows of its block's chunks -- summed
-// as integers and added to the fixed-point accumulator.  The slot's atoms
-// are touched by no other thread, so the update needs no atomics.
+// its jpart rows (contiguous at the slot's CSR range) and its entries in
+// the ipart rows of its block's chunks, all 64-bit fixed point -- summed
+// as integers (order independent) and added to the accumulator.  The
+// slot's atoms are touched by no other thread, so no atomics.
 __global__ __launch_bounds__(256) void k_water_reduce(
     const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
     const int32_t *__restrict__ cfirst, const long long *__restrict__ ipart,
@@ -424,12 +424,17 @@ __global__ __launch_bounds__(256) void k_water_reduce(
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 9 * nwat) return;
   const int s = g / 9, k = g - 9 * s;
-  long long acc = 0;
-  const int q1 = jstart[s + 1];
-  for (int q = jstart[s]; q < q1; q++) acc += jpart[(size_t)q * 9 + k];
+  long long a0 = 0, a1 = 0;
+  const int q0 = jstart[s], q1 = jstart[s + 1];
+  int q = q0;
+  for (; q + 1 < q1; q += 2) {
+    a0 += jpart[(size_t)q * 9 + k];
+    a1 += jpart[(size_t)(q + 1) * 9 + k];
+  }
+  if (q < q1) a0 += jpart[(size_t)q * 9 + k];
   const int b = s / WBLOCK, i = s % WBLOCK;
-  for (int c = cfirst[b]; c < cfirst[b + 1]; c++) acc += ipart[(size_t)c * (WBLOCK * 9) + i * 9 + k];
-  facc[3 * (size_t)slot_o[s] + k] += acc;
+  for (int c = cfirst[b]; c < cfirst[b + 1]; c++) a1 += ipart[(size_t)c * (WBLOCK * 9) + i * 9 + k];
+  facc[3 * (size_t)slot_o[s] + k] += a0 + a1;
 }
 
 int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st) {
