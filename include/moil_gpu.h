@@ -165,6 +165,24 @@ int32_t moil_gpu_build_lists_dev(moil_ctx *ctx, const double *d_coor, void *stre
 int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
                             double *d_energies, void *stream);
 
+/* Batched evaluation (no counterpart in MOIL, which evaluates one system
+ * per process; the chain/path drivers loop over slices, chain/sds.f:71-140).
+ * A batch groups existing contexts (independent systems or path slices of
+ * any sizes); moil_gpu_batch_eforce_dev evaluates all of them, d_coor[i],
+ * d_dpot[i], d_energies[i] being system i's device arrays as for
+ * moil_gpu_eforce_dev.  The step is captured once into a hipGraph over
+ * `nstreams` internal streams and replayed; it is recaptured when a
+ * system's lists were rebuilt or the pointers change.  Ordered after the
+ * caller's earlier work on `stream` (NULL = null stream) and before its
+ * later work.  Results are identical to one moil_gpu_eforce_dev per
+ * system. */
+typedef struct moil_batch moil_batch;
+int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nstreams, moil_batch **out);
+int32_t moil_gpu_batch_eforce_dev(moil_batch *batch, const double *const *d_coor,
+                                  double *const *d_dpot, double *const *d_energies,
+                                  void *stream);
+int32_t moil_gpu_batch_destroy(moil_batch *batch);
+
 /* List inspection for parity tests.  counts[0..7]: water pairs real
  * listwt1, real listwt2, image listwt1, image listwt2; atom pairs list1,
  * list2, list3 (real+image), total image atom pairs. */

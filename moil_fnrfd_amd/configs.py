@@ -75,3 +75,53 @@ def params_for(sysm, cut, ew, precision=abi.PREC_FP64, terms=abi.TERM_DEFAULT):
                            rebg=cut['rebg'], cutm=cut['cutm'], dtol=ew['dtol'], nfft=ew['nfft'],
                            order=ew.get('order', 4), eps=cut.get('eps', 1.0),
                            precision=precision, terms=terms)
+
+
+def fft_size(n_min):
+    """Smallest 2^a 3^b 5^c >= n_min (and >= 8)."""
+    n = max(int(n_min), 8)
+    while True:
+        m = n
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return n
+        n += 1
+
+
+def batch_system(n_atoms, seed, n_chains):
+    """One system of the batched workload: a cubic box at liquid water
+    density (0.1002 atoms/A^3) holding n_chains heteropolymer chains and
+    3-site water, 9 A cutoffs with a 1.5 A list skin, beta 0.34, PME grid
+    at ~1 A spacing."""
+    L = (n_atoms / (3 * 0.0334)) ** (1.0 / 3.0)
+    s = hetero_system(n_chains, L, seed=seed)
+    n = fft_size(L)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=10.5, rebg=10.5, cutm=18.0)
+    ew = dict(dtol=DTOL_B034_RC9, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def batch_workload(seed=3000, n_systems=64, n_slices=32, slice_atoms=12000):
+    """The batched workload: n_systems independent periodic systems with
+    atom counts log-uniform in [3,000, 50,000], each with 1-8 chains, plus
+    n_slices slices of a slice_atoms-atom system linearly interpolated
+    between two conformations (the second = the first + N(0, 0.5 A)).
+    Returns a list of (System, cut, ewald)."""
+    import copy
+
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_systems):
+        n = int(round(np.exp(rng.uniform(np.log(3000.0), np.log(50000.0)))))
+        out.append(batch_system(n, seed + 1 + k, int(rng.integers(1, 9))))
+    base, cut, ew = batch_system(slice_atoms, seed + 1000, 8)
+    far = base.coor + rng.normal(0.0, 0.5, base.coor.shape)
+    for k in range(n_slices):
+        s = copy.copy(base)
+        lam = k / max(n_slices - 1, 1)
+        s.coor = (1.0 - lam) * base.coor + lam * far
+        out.append((s, cut, ew))
+    return out

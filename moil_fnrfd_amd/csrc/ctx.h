@@ -95,7 +95,7 @@ struct moil_ctx {
   int32_t *d_rcount = nullptr;     // [nwblk] records per block
   WItem *d_witems = nullptr;
   size_t witem_cap = 0, nwitems = 0;
-  int32_t *d_wchunk = nullptr;     // [nchunk][3]: block, begin, end
+  int32_t *d_wchunk = nullptr;     // [nchunk][3] block, begin, end; then [nchunk] dispatch order
   int nwchunk = 0, wchunk_cap = 0;
   // gradient partials of the real-space water kernel (no global atomics):
   // jpart[row][9] (an item's j-water; the rows of one water contiguous),
@@ -148,6 +148,7 @@ struct moil_ctx {
   double *d_coor = nullptr;        // [3*npt] staging for host-pointer API
   double *d_dpot = nullptr;        // [3*npt]
   bool have_lists = false;
+  long list_gen = 0;               // incremented by every list build
 
   // ---- timing (event pairs per stage, summed on request) ----
   int timing = 0;                  // 0 off, 1 real-space kernels, 2 all stages

@@ -630,21 +630,26 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, c->d_boff,
                                                     c->d_witems);
     // chunk table: runs of a block's items (popcount-descending) holding
-    // about wchunk_pairs pairs, so workgroups carry similar work
+    // about wchunk_pairs pairs and at least one item per lane of the
+    // workgroup, so workgroups carry similar work; [nchunk][3] followed by
+    // the dispatch order [nchunk]
     std::vector<int32_t> ch;
+    std::vector<long> cpairs;
     for (int b = 0; b < c->nwblk; b++) {
       int q = hoff[b * NBUCKET];
       const int end = hoff[(b + 1) * NBUCKET];
       while (q < end) {
         int e = q;
         long pairs = 0;
-        while (e < end && pairs < c->wchunk_pairs && e - q < c->wchunk_max_items) {
+        while (e < end && (pairs < c->wchunk_pairs || e - q < LB_THREADS) &&
+               e - q < c->wchunk_max_items) {
           // bucket of item e: its popcount is WSPLIT - k
           int k = (int)(std::upper_bound(hoff.begin() + b * NBUCKET, hoff.begin() + (b + 1) * NBUCKET,
                                          e) - (hoff.begin() + b * NBUCKET)) - 1;
           const int pc = WSPLIT - k;
           const int nxt = std::min(hoff[b * NBUCKET + k + 1], end);
-          const long want = std::max<long>(1, (c->wchunk_pairs - pairs + pc - 1) / pc);
+          const long want = std::max<long>(std::max<long>(1, LB_THREADS - (e - q)),
+                                           (c->wchunk_pairs - pairs + pc - 1) / pc);
           int take = (int)std::min<long>(nxt - e, want);
           take = std::min(take, c->wchunk_max_items - (e - q));
           pairs += (long)take * pc;
@@ -653,11 +658,19 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
         ch.push_back(b);
         ch.push_back(q);
         ch.push_back(e);
+        cpairs.push_back(pairs);
         q = e;
       }
     }
     c->nwchunk = (int)ch.size() / 3;
-    // first chunk of every block (chunks are emitted block by block)
+    // dispatch order: largest chunks first (the tail is then made of small
+    // workgroups); chunk indices, and so the partial rows, are unchanged
+    std::vector<int32_t> order(c->nwchunk);
+    for (int q = 0; q < c->nwchunk; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return cpairs[x] > cpairs[y]; });
+    for (int q = 0; q < c->nwchunk; q++) ch.push_back(order[q]);
+    // first chunk of every block (chunks are numbered block by block)
     std::vector<int32_t> cfirst(c->nwblk + 1, c->nwchunk);
     for (int q = c->nwchunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
     for (int b = c->nwblk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
@@ -667,7 +680,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
       if (c->nwchunk > c->wchunk_cap || !c->d_wchunk) {
         hipFree(c->d_wchunk);
         c->wchunk_cap = c->nwchunk + c->nwchunk / 4 + 64;
-        HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * 3 * (size_t)c->wchunk_cap));
+        HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * 4 * (size_t)c->wchunk_cap));
       }
       HIP_CHECK(hipMemcpyAsync(c->d_wchunk, ch.data(), sizeof(int32_t) * ch.size(),
                                hipMemcpyHostToDevice, st));
@@ -698,5 +711,6 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     HIP_CHECK(hipStreamSynchronize(st));
   }
   c->have_lists = true;
+  c->list_gen++;
   return MOIL_GPU_OK;
 }

@@ -8,7 +8,7 @@
 // Work decomposition (see lists.hip): a workgroup takes a chunk of the
 // items of one 64-water block, sized by its pair count.  The block's
 // waters (the translated pair members) sit in LDS; each lane owns an item
-// (partner water j plus a mask of at most 16 bits over the block) and
+// (partner water j plus a mask of at most WSPLIT = 32 bits over the block) and
 // walks the mask.  Items come sorted by popcount, so the lanes of a
 // wavefront run loops of equal length.  j-forces accumulate in registers
 // and are stored once per item, in fixed point, into the item's row of a
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = blockIdx.x, t = threadIdx.x;
+  const int ch = chunks[3 * gridDim.x + blockIdx.x], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
   __shared__ double ipos[WBLOCK * 9];
   __shared__ long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = blockIdx.x, t = threadIdx.x;
+  const int ch = chunks[3 * gridDim.x + blockIdx.x], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {

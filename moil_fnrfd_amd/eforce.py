@@ -39,9 +39,10 @@ def load_library():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise MoilGPUError('libmoil_gpu.so not found at %s: run __graft_entry__.build()' % LIB_PATH)
-    L = C.CDLL(LIB_PATH)
+    path = os.environ.get('MOIL_GPU_LIBRARY', LIB_PATH)
+    if not os.path.exists(path):
+        raise MoilGPUError('libmoil_gpu.so not found at %s: run __graft_entry__.build()' % path)
+    L = C.CDLL(path)
     vp = C.c_void_p
     L.moil_gpu_abi_version.restype = C.c_int32
     L.moil_gpu_last_error.restype = C.c_char_p
@@ -60,12 +61,16 @@ def load_library():
     L.moil_gpu_dump_water_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_dump_atom_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_set_timing.argtypes = [vp, C.c_int32]
+    L.moil_gpu_batch_create.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.moil_gpu_batch_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
+    L.moil_gpu_batch_destroy.argtypes = [vp]
     L.moil_gpu_stage_times.argtypes = [vp, vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
-                 'moil_gpu_set_timing', 'moil_gpu_stage_times'):
+                 'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_batch_create',
+                 'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -170,6 +175,36 @@ class MoilGPU:
     def close(self):
         if getattr(self, '_h', None):
             load_library().moil_gpu_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MoilBatch:
+    """Batched evaluation of several MoilGPU systems (moil_gpu_batch_*):
+    one call evaluates all of them, replaying a captured hipGraph."""
+
+    def __init__(self, systems, nstreams=8):
+        arr = (C.c_void_p * len(systems))(*[m._h for m in systems])
+        h = C.c_void_p()
+        _check(load_library().moil_gpu_batch_create(arr, len(systems), int(nstreams), C.byref(h)))
+        self._h = h
+        self.n = len(systems)
+
+    def eforce_dev(self, d_coor, d_dpot, d_energies, stream=0):
+        """Lists of device pointers (one per system), as MoilGPU.eforce_dev."""
+        a = (C.c_void_p * self.n)(*d_coor)
+        b = (C.c_void_p * self.n)(*d_dpot)
+        e = (C.c_void_p * self.n)(*d_energies)
+        _check(load_library().moil_gpu_batch_eforce_dev(self._h, a, b, e, C.c_void_p(stream or 0)))
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load_library().moil_gpu_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,0 +1,82 @@
+"""Batched evaluation (moil_gpu_batch_*): several systems of different
+sizes and kinds evaluated by one call that replays a captured hipGraph.
+Every replay must reproduce one moil_gpu_eforce_dev per system bit for bit
+(the kernels are the same and their sums are order independent), also
+after eager work between replays and after a list rebuild forces a new
+capture."""
+import numpy as np
+import pytest
+
+from moil_fnrfd_amd import abi, configs
+from tests._cases import make_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    out = []
+    for name in ('water216', 'water1000_buffered', 'hetero_small', 'hetero_small_out'):
+        s, cut, ew, sqz = make_case(name)
+        out.append((name, s, cut, ew))
+    s, cut, ew = configs.water_scaled(8, seed=1300)
+    out.append(('water512', s, cut, ew))
+    return out
+
+
+def test_batch_matches_plain_calls():
+    import torch
+    from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
+    torch.cuda.set_device(0)
+    dev = torch.device('cuda:0')
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    systems, coor, dpot_b, en_b, dpot_p, en_p = [], [], [], [], [], []
+    for name, s, cut, ew in _systems():
+        print('case batch system %s npt=%d' % (name, s.topo.npt), flush=True)
+        m = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+        x = torch.tensor(s.coor, dtype=torch.float64, device=dev)
+        m.squeeze_dev(x.data_ptr(), sh)
+        m.nbondm_dev(x.data_ptr(), sh)
+        systems.append(m)
+        coor.append(x)
+        for lst in (dpot_b, dpot_p):
+            lst.append(torch.empty_like(x))
+        for lst in (en_b, en_p):
+            lst.append(torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev))
+    batch = MoilBatch(systems, nstreams=3)
+
+    def plain():
+        for i, m in enumerate(systems):
+            m.eforce_dev(coor[i].data_ptr(), dpot_p[i].data_ptr(), en_p[i].data_ptr(), sh)
+
+    def batched():
+        batch.eforce_dev([c.data_ptr() for c in coor], [d.data_ptr() for d in dpot_b],
+                         [e.data_ptr() for e in en_b], sh)
+
+    def same():
+        torch.cuda.synchronize()
+        for i in range(len(systems)):
+            assert torch.equal(dpot_b[i], dpot_p[i]), i
+            assert torch.equal(en_b[i], en_p[i]), i
+
+    for rep in range(3):
+        print('case batch replay %d' % rep, flush=True)
+        for d in dpot_b:
+            d.fill_(0.0)
+        plain()
+        batched()
+        same()
+    # coordinates move and one system rebuilds its lists: new capture
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    for rep in range(2):
+        print('case batch after rebuild %d' % rep, flush=True)
+        for x in coor:
+            x.add_(torch.randn(x.shape, generator=gen, device=dev, dtype=torch.float64), alpha=0.02)
+        systems[2].nbondm_dev(coor[2].data_ptr(), sh)
+        plain()
+        batched()
+        same()
+    batch.close()
+    for m in systems:
+        m.close()
