@@ -146,6 +146,14 @@ def batched_leg(steps=10, nstreams=8):
             fn()
         torch.cuda.synchronize()
         res[name] = (time.perf_counter() - t0) / steps
+    # per-stage GPU time of each system run alone (events between stages)
+    stages = np.zeros(5)
+    for i, m in enumerate(systems):
+        m.set_timing(2)
+        m.eforce_dev(cp[i], dp[i], ep[i], sh)
+        torch.cuda.synchronize()
+        stages += np.asarray(m.stage_times()[:5])
+        m.set_timing(0)
     # accuracy of the batched (mixed) forces against FP64 on the first slice
     k = len(work) - 32
     f_mixed = dpot[k].cpu().numpy()
@@ -167,6 +175,9 @@ def batched_leg(steps=10, nstreams=8):
         'plain_ms_per_step': 1e3 * res['plain'],
         'batched_ms_per_step': 1e3 * res['batched'],
         'speedup': res['plain'] / res['batched'],
+        'alone_stage_ms': {'realspace_water': stages[0], 'realspace_atoms': stages[1],
+                           'pme': stages[2], 'bonded_excl': stages[3],
+                           'eforce_total': stages[4]},
         'force_rms_rel_vs_fp64': err,
         'workload': '64 systems, atom counts log-uniform in [3k, 50k] (water + 1-8 '
                     'heteropolymer chains, 9 A cutoffs, 1.5 A skin, ~1 A PME grids) + 32 '
@@ -182,6 +193,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the batched/plain leg (96 systems)')
+    ap.add_argument('--batched-only', action='store_true',
+                    help='run only the batched/plain leg and print its JSON')
+    ap.add_argument('--batched-steps', type=int, default=10)
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB ten times (the reference '
                          'kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)')
@@ -202,6 +216,9 @@ def main():
     from moil_fnrfd_amd.eforce import MoilGPU
 
     torch.cuda.set_device(0)
+    if args.batched_only:
+        print(json.dumps(batched_leg(steps=args.batched_steps)))
+        return
     sysm, cut, ew = configs.water32k()
     params = configs.params_for(sysm, cut, ew, precision=abi.PREC_MIXED)
     eng = MoilGPU(sysm.topo, params)
@@ -303,7 +320,7 @@ def main():
     }
     eng.close()
     if not args.no_batched:
-        line['batched'] = batched_leg()
+        line['batched'] = batched_leg(steps=args.batched_steps)
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(sysm, cut, ew)
     print(json.dumps(line))

@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "ctx.h"
+#include "multi.h"
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
@@ -20,8 +20,6 @@ void set_error(const std::string &msg) { g_err = msg; }
 extern "C" int32_t moil_gpu_abi_version(void) { return MOIL_GPU_ABI_VERSION; }
 extern "C" const char *moil_gpu_last_error(void) { return g_err.c_str(); }
 
-int pme_conv_blocks(const moil_ctx *c);
-int bonded_max_blocks(const moil_ctx *c);
 
 #define CHECK_RC(x)          \
   do {                       \
@@ -528,34 +526,33 @@ extern "C" int32_t moil_gpu_build_lists(moil_ctx *c, const double *coor) {
 
 // dpot = fixed-point accumulator + reciprocal gradient; the accumulator
 // is cleared for the next evaluation on the way
-__global__ void k_finalize_forces(long long *__restrict__ facc, const double *__restrict__ fpme,
-                                  int has_pme, size_t n, double inv, double *__restrict__ dpot) {
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void finalize_impl(long long *__restrict__ facc,
+                                              const double *__restrict__ fpme, int has_pme,
+                                              size_t n, double inv, double *__restrict__ dpot,
+                                              int bid) {
+  size_t g = (size_t)bid * 256 + threadIdx.x;
   if (g >= n) return;
   double v = (double)facc[g] * inv;
   if (has_pme) v += fpme[g];
   dpot[g] = v;
   facc[g] = 0;
 }
-
-struct EnCfg {
-  int max_parts;
-  int nparts[EN_COUNT];
-  double e_self;
-  int has_bond, has_angle, has_tors, has_imp, has_nb, has_14;
-};
+__device__ __forceinline__ void finalize_body(const FinArgs &A, int bid, int nblk) {
+  finalize_impl(A.facc, A.fpme, A.has_pme, A.n, A.inv, A.dpot, bid);
+}
+DEFINE_KERNEL(k_finalize_forces, FinArgs, 256, finalize_body)
 
 // deterministic reduction of the per-workgroup partials: one workgroup
 // per energy term (fixed strided order + fixed shuffle tree), then the
 // last workgroup to finish assembles the energy record in eforce's order
 // (pot/eforce.f:100-205) and resets the ticket for the next call
-__global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep, EnCfg C,
-                                                  double *__restrict__ tot,
-                                                  unsigned int *__restrict__ ticket,
-                                                  double *__restrict__ out) {
+__device__ __forceinline__ void energies_impl(const double *__restrict__ ep, const EnCfg &C,
+                                              double *__restrict__ tot,
+                                              unsigned int *__restrict__ ticket,
+                                              double *__restrict__ out, int bid, int nblk) {
   __shared__ double red[4];
   __shared__ bool last;
-  const int term = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int term = bid, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double s = 0.0;
   const double *row = ep + (size_t)term * C.max_parts;
   for (int i = t; i < C.nparts[term]; i += 256) s += row[i];
@@ -565,7 +562,7 @@ __global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep,
   if (t == 0) {
     tot[term] = (red[0] + red[1]) + (red[2] + red[3]);
     __threadfence();
-    last = atomicAdd(ticket, 1u) == (unsigned)(gridDim.x - 1);
+    last = atomicAdd(ticket, 1u) == (unsigned)(nblk - 1);
   }
   __syncthreads();
   if (!last || t != 0) return;
@@ -602,6 +599,10 @@ __global__ __launch_bounds__(256) void k_energies(const double *__restrict__ ep,
   for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = src[k];
   *ticket = 0u;
 }
+__device__ __forceinline__ void energies_body(const EnArgs &A, int bid, int nblk) {
+  energies_impl(A.ep, A.C, A.tot, A.ticket, A.out, bid, nblk);
+}
+DEFINE_KERNEL(k_energies, EnArgs, 256, energies_body)
 
 // stage timing: event pairs recorded on the launch stream, summed lazily
 static hipEvent_t ev_get(moil_ctx *c) {
@@ -631,77 +632,34 @@ struct StageTimer {
   }
 };
 
-int bonded_blocks(const moil_ctx *c);
-int water_intra_blocks(const moil_ctx *c);
-
-extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
-                                       double *d_energies, void *stream) {
-  if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
-  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "eforce called before build_lists");
-  hipStream_t st = pick_stream(c, stream);
+// Every launch of one evaluation with its arguments: reciprocal space (if
+// the nonbonded terms are on), bonded terms, water and atom real space,
+// force assembly (fixed-point accumulator + reciprocal gradient -> dpot)
+// and the energy record.
+void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_energies,
+               EvalPlan *p) {
   const uint32_t T = c->par.terms;
   const bool nb = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
-  size_t n3 = 3 * (size_t)c->npt;
-  if (c->facc_dirty) {
-    HIP_CHECK(hipMemsetAsync(c->d_facc, 0, sizeof(long long) * n3, st));
-    c->facc_dirty = false;
-  }
-  c->facc_dirty = true;  // until the finalize kernel is enqueued
-  if (c->timing >= 1) c->n_timed++;
-  StageTimer whole(c, 4, st);
-  // reciprocal space: in order on the launch stream by default; with
-  // pme_concurrent on a second stream overlapped with bonded + real space
-  // (the real-space kernel fills every CU, so the overlap mostly slows the
-  // latency-bound PME kernels down)
-  if (nb && c->pme_concurrent) {
-    HIP_CHECK(hipEventRecord(c->ev_fork, st));
-    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    {
-      StageTimer tp(c, 2, c->stream2);
-      CHECK_RC(pme_run(c, d_coor, c->stream2));
-    }
-    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
-  } else if (nb) {
-    StageTimer tp(c, 2, st);
-    CHECK_RC(pme_run(c, d_coor, st));
-  }
-  {
-    StageTimer tb(c, 3, st);
-    CHECK_RC(bonded_run(c, d_coor, st));
-  }
-  const int nwpart = (nb && c->nwat >= 2) ? c->nwchunk : 0;
+  *p = EvalPlan();
+  p->f64 = c->par.precision != MOIL_PREC_MIXED;
+  if (nb) pme_plan(c, d_coor, p);
+  bonded_plan(c, d_coor, p);
+  int nwpart = 0;
   if (nb) {
-    CHECK_RC(realspace_water_pack(c, d_coor, st));
-    // the real-space kernels are timed at any timing level by events that
-    // their own dispatch records (no extra packets between kernels)
-    hipEvent_t ea = nullptr, eb = nullptr;
-    if (c->timing >= 1 && c->nwat >= 2 && c->nwchunk > 0) {
-      ea = ev_get(c);
-      eb = ev_get(c);
-    }
-    CHECK_RC(realspace_water(c, d_coor, st, ea, eb));
-    if (ea && eb) c->ev_rec.push_back({0, {ea, eb}});
-    CHECK_RC(realspace_water_reduce(c, st));
-    ea = eb = nullptr;
-    if (c->timing >= 1 && c->nachunk > 0) {
-      ea = ev_get(c);
-      eb = ev_get(c);
-    }
-    CHECK_RC(realspace_atoms(c, d_coor, st, nwpart, ea, eb));
-    if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
-    CHECK_RC(realspace_atoms_reduce(c, st));
+    water_plan(c, d_coor, p);
+    nwpart = p->n_water;
+    atoms_plan(c, d_coor, nwpart, p);
   }
-  if (nb && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
-  k_finalize_forces<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_facc, c->d_fpme, nb ? 1 : 0,
-                                                                  n3, c->fix_inv, d_dpot);
-  c->facc_dirty = false;
+  const size_t n3 = 3 * (size_t)c->npt;
+  p->fin = {c->d_facc, c->d_fpme, nb ? 1 : 0, n3, c->fix_inv, d_dpot};
+  p->n_fin = (int)((n3 + 255) / 256);
   EnCfg C;
   C.max_parts = c->max_parts;
   const int nbond = bonded_blocks(c), nwi = water_intra_blocks(c);
   for (int k = 0; k < EN_COUNT; k++) C.nparts[k] = 0;
   C.nparts[EN_TORS] = C.nparts[EN_IMP] = C.nparts[EN_VDW14] = C.nparts[EN_EL14] = nbond;
   C.nparts[EN_BOND] = C.nparts[EN_THETA] = C.nparts[EN_CORR] = nbond + nwi;
-  const int nrs = nwpart + (nb ? c->nachunk : 0);
+  const int nrs = nwpart + (nb ? p->n_atoms : 0);
   C.nparts[EN_VDW] = C.nparts[EN_EL] = C.nparts[EN_VSYM] = C.nparts[EN_LSYM] = nrs;
   C.nparts[EN_RECIP] = nb ? pme_conv_blocks(c) : 0;
   C.e_self = c->e_self;
@@ -711,8 +669,69 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   C.has_imp = (T & MOIL_TERM_IMP) && c->nimp > 0;
   C.has_nb = nb;
   C.has_14 = c->totspe > 0 && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14));
-  k_energies<<<EN_COUNT, 256, 0, st>>>(c->d_epart, C, c->d_etot, c->d_ticket,
-                                      d_energies ? d_energies : c->d_energy);
+  p->en = {c->d_epart, C, c->d_etot, c->d_ticket, d_energies ? d_energies : c->d_energy};
+}
+
+extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
+                                       double *d_energies, void *stream) {
+  if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "eforce called before build_lists");
+  hipStream_t st = pick_stream(c, stream);
+  EvalPlan p;
+  eval_plan(c, d_coor, d_dpot, d_energies, &p);
+  if (c->facc_dirty) {
+    HIP_CHECK(hipMemsetAsync(c->d_facc, 0, sizeof(long long) * 3 * (size_t)c->npt, st));
+    c->facc_dirty = false;
+  }
+  c->facc_dirty = true;  // until the finalize kernel is enqueued
+  if (c->timing >= 1) c->n_timed++;
+  StageTimer whole(c, 4, st);
+  // reciprocal space: in order on the launch stream by default; with
+  // pme_concurrent on a second stream overlapped with bonded + real space
+  // (the real-space kernel fills every CU, so the overlap mostly slows the
+  // latency-bound PME kernels down)
+  if (p.pme && c->pme_concurrent) {
+    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    {
+      StageTimer tp(c, 2, c->stream2);
+      CHECK_RC(pme_run(c, p, c->stream2));
+    }
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+  } else if (p.pme) {
+    StageTimer tp(c, 2, st);
+    CHECK_RC(pme_run(c, p, st));
+  }
+  {
+    StageTimer tb(c, 3, st);
+    launch_k_water_intra(p.intra, p.n_intra, st);
+    launch_k_bonded(p.bonded, p.n_bonded, st);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
+  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
+  // the real-space kernels are timed at any timing level by events that
+  // their own dispatch records (no extra packets between kernels)
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (c->timing >= 1 && p.n_water > 0) {
+    ea = ev_get(c);
+    eb = ev_get(c);
+  }
+  launch_k_water_ev(p.water, p.f64, p.n_water, st, ea, eb);
+  if (ea && eb) c->ev_rec.push_back({0, {ea, eb}});
+  launch_k_water_reduce(p.wred, p.n_wred, st);
+  ea = eb = nullptr;
+  if (c->timing >= 1 && p.n_atoms > 0) {
+    ea = ev_get(c);
+    eb = ev_get(c);
+  }
+  launch_k_atoms_ev(p.atoms, p.f64, p.n_atoms, st, ea, eb);
+  if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
+  launch_k_atoms_reduce(p.ared, p.n_ared, st);
+  if (p.pme && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  launch_k_finalize_forces(p.fin, p.n_fin, st);
+  c->facc_dirty = false;
+  launch_k_energies(p.en, EN_COUNT, st);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

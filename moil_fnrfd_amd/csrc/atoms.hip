@@ -39,7 +39,7 @@
 #include <cmath>
 #include <vector>
 
-#include "ctx.h"
+#include "multi.h"
 #include "grid.h"
 
 #define ABT 256
@@ -395,12 +395,6 @@ __global__ void k_ajfill(const AItem *__restrict__ items, size_t n, const int32_
 
 // ---------------------------------------------------------------------------
 // real-space energy over the atom items
-struct ARS {
-  double qfac, pick;  // kofdie/eps (or 0), 1 (or 0) for LJ
-  double beta, cutvdw2, cutele2, erftbdns;
-  double a, b, c;
-  double fix_scale;
-};
 
 #define ERFC_P 0.418f
 __device__ __forceinline__ float a_erfc_poly(float t) {
@@ -438,18 +432,18 @@ __device__ __forceinline__ uint64_t a_rotr64(uint64_t x, int r) {
 }
 
 template <bool F64>
-__global__ __launch_bounds__(ABT) void k_atoms(
+__device__ __forceinline__ void atoms_impl(
     const int32_t *__restrict__ chunks, const AItem *__restrict__ items,
     const double *__restrict__ coor, const int32_t *__restrict__ sslot2atom, int nsol,
     const double *__restrict__ q, const double *__restrict__ e12, const double *__restrict__ e6,
-    const double *__restrict__ erf_arr, ARS P, const int32_t *__restrict__ jpos,
+    const double *__restrict__ erf_arr, const ARS &P, const int32_t *__restrict__ jpos,
     long long *__restrict__ jpart,
-    long long *__restrict__ ipart, double *__restrict__ epart, int max_parts, int part0) {
+    long long *__restrict__ ipart, double *__restrict__ epart, int max_parts, int part0, int bid) {
   __shared__ double ix[WBLOCK][3];
   __shared__ double iq[WBLOCK], ia12[WBLOCK], ib6[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 3];
   __shared__ double red[4];
-  const int ch = blockIdx.x, t = threadIdx.x;
+  const int ch = bid, t = threadIdx.x;
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
@@ -594,14 +588,22 @@ __global__ __launch_bounds__(ABT) void k_atoms(
   if (t == 0) epart[EN_LSYM * max_parts + part0 + ch] = v;
 }
 
+template <bool F64>
+__device__ __forceinline__ void atoms_body(const AtomArgs &A, int bid, int nblk) {
+  atoms_impl<F64>(A.chunks, A.items, A.coor, A.sslot2atom, A.nsol, A.q, A.e12, A.e6, A.erf, A.P,
+                  A.jpos, A.jpart, A.ipart, A.epart, A.max_parts, A.part0, bid);
+}
+DEFINE_KERNEL(k_atoms_f32, AtomArgs, ABT, atoms_body<false>)
+DEFINE_KERNEL(k_atoms_f64, AtomArgs, ABT, atoms_body<true>)
+
 // one thread per (atom, component): the atom's partner rows (contiguous)
 // and, for a block atom, its entries in its block's chunk rows, all
 // 64-bit fixed point, summed as integers and added to its accumulator
-__global__ __launch_bounds__(256) void k_atoms_reduce(
+__device__ __forceinline__ void atoms_reduce_impl(
     const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
     const int32_t *__restrict__ slot_of, const int32_t *__restrict__ cfirst,
-    const long long *__restrict__ ipart, int npt, long long *__restrict__ facc) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long *__restrict__ ipart, int npt, long long *__restrict__ facc, int bid) {
+  const int g = bid * 256 + threadIdx.x;
   if (g >= 3 * npt) return;
   const int a = g / 3, k = g - 3 * a;
   long long acc = 0;
@@ -614,6 +616,10 @@ __global__ __launch_bounds__(256) void k_atoms_reduce(
   }
   if (acc) facc[3 * (size_t)a + k] += acc;
 }
+__device__ __forceinline__ void atoms_reduce_body(const AReduceArgs &A, int bid, int nblk) {
+  atoms_reduce_impl(A.jstart, A.jpart, A.slot_of, A.cfirst, A.ipart, A.npt, A.facc, bid);
+}
+DEFINE_KERNEL(k_atoms_reduce, AReduceArgs, 256, atoms_reduce_body)
 
 // ---------------------------------------------------------------------------
 int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
@@ -810,9 +816,11 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   return MOIL_GPU_OK;
 }
 
-int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0,
-                        hipEvent_t ev_start, hipEvent_t ev_stop) {
-  if (c->nachunk == 0) return MOIL_GPU_OK;
+// the atom real-space launches of one evaluation (none with an empty
+// list); the energy partials follow the water kernel's (part0)
+void atoms_plan(const moil_ctx *c, const double *d_coor, int part0, EvalPlan *p) {
+  p->n_atoms = p->n_ared = 0;
+  if (c->nachunk == 0) return;
   ARS P;
   const uint32_t T = c->par.terms;
   P.qfac = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;
@@ -823,29 +831,40 @@ int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int p
   P.erftbdns = c->erftbdns;
   P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale;
-  if (c->par.precision == MOIL_PREC_MIXED)
-    hipExtLaunchKernelGGL(k_atoms<false>, dim3(c->nachunk), dim3(ABT), 0, st, ev_start, ev_stop, 0,
-                          (const int32_t *)c->d_achunk, (const AItem *)c->d_aitems, d_coor,
-                          (const int32_t *)c->d_sslot2atom, c->nsol, (const double *)c->d_ptchg,
-                          (const double *)c->d_eps12, (const double *)c->d_eps6,
-                          (const double *)c->d_erf, P, (const int32_t *)c->d_ajpos, c->d_ajpart,
-                          c->d_aipart, c->d_epart, c->max_parts, part0);
-  else
-    hipExtLaunchKernelGGL(k_atoms<true>, dim3(c->nachunk), dim3(ABT), 0, st, ev_start, ev_stop, 0,
-                          (const int32_t *)c->d_achunk, (const AItem *)c->d_aitems, d_coor,
-                          (const int32_t *)c->d_sslot2atom, c->nsol, (const double *)c->d_ptchg,
-                          (const double *)c->d_eps12, (const double *)c->d_eps6,
-                          (const double *)c->d_erf, P, (const int32_t *)c->d_ajpos, c->d_ajpart,
-                          c->d_aipart, c->d_epart, c->max_parts, part0);
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
+  AtomArgs &A = p->atoms;
+  A.chunks = c->d_achunk;
+  A.items = c->d_aitems;
+  A.coor = d_coor;
+  A.sslot2atom = c->d_sslot2atom;
+  A.nsol = c->nsol;
+  A.q = c->d_ptchg;
+  A.e12 = c->d_eps12;
+  A.e6 = c->d_eps6;
+  A.erf = c->d_erf;
+  A.P = P;
+  A.jpos = c->d_ajpos;
+  A.jpart = c->d_ajpart;
+  A.ipart = c->d_aipart;
+  A.epart = c->d_epart;
+  A.max_parts = c->max_parts;
+  A.part0 = part0;
+  p->n_atoms = c->nachunk;
+  AReduceArgs &R = p->ared;
+  R.jstart = c->d_ajstart;
+  R.jpart = c->d_ajpart;
+  R.slot_of = c->d_sslot_of;
+  R.cfirst = c->d_acfirst;
+  R.ipart = c->d_aipart;
+  R.npt = c->npt;
+  R.facc = c->d_facc;
+  p->n_ared = (3 * c->npt + 255) / 256;
 }
 
-int32_t realspace_atoms_reduce(moil_ctx *c, hipStream_t st) {
-  if (c->nachunk == 0) return MOIL_GPU_OK;
-  k_atoms_reduce<<<(3 * c->npt + 255) / 256, 256, 0, st>>>(c->d_ajstart, c->d_ajpart, c->d_sslot_of,
-                                                            c->d_acfirst, c->d_aipart, c->npt,
-                                                            c->d_facc);
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
+void launch_k_atoms_ev(const AtomArgs &a, bool f64, int nblk, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1) {
+  if (nblk <= 0) return;
+  if (f64)
+    hipExtLaunchKernelGGL(k_atoms_f64, dim3(nblk), dim3(ABT), 0, st, e0, e1, 0, a);
+  else
+    hipExtLaunchKernelGGL(k_atoms_f32, dim3(nblk), dim3(ABT), 0, st, e0, e1, 0, a);
 }

@@ -1,34 +1,117 @@
 // Batched evaluation of many independent systems (MOIL contexts of uneven
-// size, e.g. the slices of a chain/path minimisation or a set of separate
-// boxes): one call evaluates all of them.
+// size, e.g. the slices of a chain/path minimisation, chain/sds.f:71-140,
+// or a set of separate boxes): one call evaluates all of them.
 //
-// The whole batch step is captured once into a hipGraph: the systems'
-// eforce sequences are spread round-robin over `nstreams` internal streams
-// (default priority) forked from and joined back into a capture stream, so
-// independent small systems run concurrently and the step is replayed
-// with one graph launch instead of ~25 launches per system.  The graph is
-// recaptured when a system's lists were rebuilt (grid sizes and buffers
-// change) or the caller passes different device pointers.  Nothing inside
-// the captured region is a memset: the per-evaluation clears of the
-// kernels are done by kernels, and a context's one-time accumulator clear
-// is run eagerly before capture.  Work is ordered after the caller's
-// stream and the caller's stream after the batch through events.
+// Every kernel of the evaluation runs once for the whole batch: a
+// multi-system launch covers the workgroups of all systems, each
+// workgroup finding its system through a table of per-system argument
+// blocks and workgroup offsets (kargs.h, multi.h).  The 3-D transforms
+// run as one batched rocFFT plan per distinct grid size, on transform
+// buffers owned by the batch (the systems' spreading writes into them and
+// the gather reads from them).  The whole step -- about twenty kernel
+// launches plus two transforms per grid size, independent of the number
+// of systems -- is captured once into a hipGraph and replayed; it is
+// recaptured when a system's lists were rebuilt (chunk counts and list
+// buffers change) or the caller passes different device pointers.
+// Nothing captured is a memset: the per-evaluation clears are done by
+// the kernels themselves, and a context's one-time accumulator clear runs
+// eagerly before capture.  The caller's stream is ordered before and
+// after the batch through events.  Systems are laid out in the grids in
+// decreasing atom count, so the largest systems' workgroups start first.
 #include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
 
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <tuple>
 #include <vector>
 
-#include "ctx.h"
+#include "kargs.h"
+
+#define RF_TRY(x)                                            \
+  do {                                                       \
+    if ((x) != rocfft_status_success) {                      \
+      set_error(std::string("batch: rocFFT failure at ") +  \
+                __FILE__ + ":" + std::to_string(__LINE__));  \
+      return MOIL_GPU_EHIP;                                  \
+    }                                                        \
+  } while (0)
+
+#define CHECK_RC_B(x)                  \
+  do {                                 \
+    const int32_t r_ = (x);            \
+    if (r_ != MOIL_GPU_OK) return r_;  \
+  } while (0)
+
+namespace {
+
+// one batched transform pair over the systems sharing a grid size
+struct FftGroup {
+  int n1 = 0, n2 = 0, n3 = 0;
+  std::vector<int> members;  // system indices
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info_f = nullptr, info_b = nullptr;
+  void *work = nullptr;
+  void *qreal = nullptr, *qcplx = nullptr;
+  size_t G = 0, H = 0;
+};
+
+// per-kernel table of argument blocks + workgroup offsets
+template <class A>
+struct Tab {
+  std::vector<A> args;
+  std::vector<int32_t> bs{0};
+  size_t off_a = 0, off_b = 0;  // byte offsets in the device blob
+  void add(const A &a, int n) {
+    args.push_back(a);
+    bs.push_back(bs.back() + std::max(n, 0));
+  }
+  int total() const { return bs.back(); }
+  int nsys() const { return (int)args.size(); }
+  void clear() {
+    args.clear();
+    bs.assign(1, 0);
+  }
+};
+
+struct Tables {
+  Tab<TbinArgs> tbin;
+  Tab<TscanArgs> tscan;
+  Tab<TscatterArgs> tscat;
+  Tab<TspreadArgs> tspread;
+  Tab<TsumArgs> tsum;
+  Tab<ConvArgs> conv;
+  Tab<GatherArgs> gather;
+  Tab<IntraArgs> intra;
+  Tab<BondedArgs> bonded;
+  Tab<PackArgs> pack;
+  Tab<WaterArgs> water;
+  Tab<WReduceArgs> wred;
+  Tab<AtomArgs> atoms;
+  Tab<AReduceArgs> ared;
+  Tab<FinArgs> fin;
+  Tab<EnArgs> en;
+};
+
+}  // namespace
 
 struct moil_batch {
   std::vector<moil_ctx *> ctx;
-  std::vector<hipStream_t> streams;
-  hipStream_t cap = nullptr;                  // capture / launch stream
-  std::vector<hipEvent_t> ev_fork, ev_join;   // per stream
+  std::vector<int> order;  // systems by decreasing atom count
+  bool f64 = false;
+  hipStream_t cap = nullptr;  // capture / launch stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<long> gen;                      // list generation captured
-  std::vector<const void *> key;              // pointers captured
+  std::vector<long> gen;           // list generation captured
+  std::vector<const void *> key;   // pointers captured
+  std::vector<FftGroup> groups;
+  std::vector<int> group_of, slot_of;  // per system: its group, index in it
+  Tables tab;
+  char *d_blob = nullptr;
+  size_t blob_cap = 0;
 };
 
 static void batch_drop_graph(moil_batch *b) {
@@ -41,17 +124,81 @@ static void batch_drop_graph(moil_batch *b) {
 static void batch_free(moil_batch *b) {
   if (!b) return;
   batch_drop_graph(b);
-  for (hipEvent_t e : b->ev_fork) hipEventDestroy(e);
-  for (hipEvent_t e : b->ev_join) hipEventDestroy(e);
+  for (FftGroup &g : b->groups) {
+    if (g.fwd) rocfft_plan_destroy(g.fwd);
+    if (g.bwd) rocfft_plan_destroy(g.bwd);
+    if (g.info_f) rocfft_execution_info_destroy(g.info_f);
+    if (g.info_b) rocfft_execution_info_destroy(g.info_b);
+    hipFree(g.work);
+    hipFree(g.qreal);
+    hipFree(g.qcplx);
+  }
+  hipFree(b->d_blob);
   if (b->ev_in) hipEventDestroy(b->ev_in);
   if (b->ev_out) hipEventDestroy(b->ev_out);
-  for (hipStream_t s : b->streams) hipStreamDestroy(s);
   if (b->cap) hipStreamDestroy(b->cap);
   delete b;
 }
 
+// batched transforms, one group per grid size
+static int32_t batch_fft_groups(moil_batch *b) {
+  std::map<std::tuple<int, int, int>, int> idx;
+  const int n = (int)b->ctx.size();
+  b->group_of.assign(n, -1);
+  b->slot_of.assign(n, -1);
+  for (int i = 0; i < n; i++) {
+    const moil_ctx *c = b->ctx[i];
+    auto k = std::make_tuple(c->nf1, c->nf2, c->nf3);
+    auto it = idx.find(k);
+    int g;
+    if (it == idx.end()) {
+      g = (int)b->groups.size();
+      idx[k] = g;
+      b->groups.emplace_back();
+      b->groups[g].n1 = c->nf1;
+      b->groups[g].n2 = c->nf2;
+      b->groups[g].n3 = c->nf3;
+    } else {
+      g = it->second;
+    }
+    b->group_of[i] = g;
+    b->slot_of[i] = (int)b->groups[g].members.size();
+    b->groups[g].members.push_back(i);
+  }
+  const size_t rs = b->f64 ? sizeof(double) : sizeof(float);
+  const rocfft_precision prec = b->f64 ? rocfft_precision_double : rocfft_precision_single;
+  RF_TRY(rocfft_setup());
+  for (FftGroup &g : b->groups) {
+    const size_t m = g.members.size();
+    g.G = (size_t)g.n1 * g.n2 * g.n3;
+    g.H = (size_t)(g.n1 / 2 + 1) * g.n2 * g.n3;
+    HIP_CHECK(hipMalloc(&g.qreal, rs * g.G * m));
+    HIP_CHECK(hipMalloc(&g.qcplx, 2 * rs * g.H * m));
+    size_t len[3] = {(size_t)g.n1, (size_t)g.n2, (size_t)g.n3};
+    // default layouts: contiguous transforms, distance G (real) and H
+    // (complex) between the members of the batch
+    RF_TRY(rocfft_plan_create(&g.fwd, rocfft_placement_notinplace,
+                              rocfft_transform_type_real_forward, prec, 3, len, m, nullptr));
+    RF_TRY(rocfft_plan_create(&g.bwd, rocfft_placement_notinplace,
+                              rocfft_transform_type_real_inverse, prec, 3, len, m, nullptr));
+    size_t w1 = 0, w2 = 0;
+    RF_TRY(rocfft_plan_get_work_buffer_size(g.fwd, &w1));
+    RF_TRY(rocfft_plan_get_work_buffer_size(g.bwd, &w2));
+    const size_t w = std::max(w1, w2);
+    if (w) HIP_CHECK(hipMalloc(&g.work, w));
+    RF_TRY(rocfft_execution_info_create(&g.info_f));
+    RF_TRY(rocfft_execution_info_create(&g.info_b));
+    if (w) {
+      RF_TRY(rocfft_execution_info_set_work_buffer(g.info_f, g.work, w));
+      RF_TRY(rocfft_execution_info_set_work_buffer(g.info_b, g.work, w));
+    }
+  }
+  return MOIL_GPU_OK;
+}
+
 extern "C" int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nstreams,
                                          moil_batch **out) {
+  (void)nstreams;  // one launch per kernel covers the batch
   if (!ctxs || n < 1 || !out) {
     set_error("batch_create: null argument or empty batch");
     return MOIL_GPU_EINVAL;
@@ -61,27 +208,30 @@ extern "C" int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nst
       set_error("batch_create: null context");
       return MOIL_GPU_EINVAL;
     }
-  if (nstreams < 1) nstreams = 1;
-  if (nstreams > n) nstreams = n;
+  for (int i = 1; i < n; i++)
+    if (ctxs[i]->par.precision != ctxs[0]->par.precision) {
+      set_error("batch_create: all systems of a batch must use the same precision");
+      return MOIL_GPU_EINVAL;
+    }
   moil_batch *b = new moil_batch();
   b->ctx.assign(ctxs, ctxs + n);
+  b->f64 = ctxs[0]->par.precision != MOIL_PREC_MIXED;
+  b->order.resize(n);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return ctxs[x]->npt > ctxs[y]->npt; });
   bool ok = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess;
-  for (int k = 0; ok && k < nstreams; k++) {
-    hipStream_t s;
-    hipEvent_t f, j;
-    ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
-    if (ok) b->streams.push_back(s);
-    ok = ok && hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess;
-    if (ok) b->ev_fork.push_back(f);
-    ok = ok && hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
-    if (ok) b->ev_join.push_back(j);
-  }
   if (!ok) {
     batch_free(b);
     set_error("batch_create: stream/event creation failed");
     return MOIL_GPU_EHIP;
+  }
+  const int32_t rc = batch_fft_groups(b);
+  if (rc != MOIL_GPU_OK) {
+    batch_free(b);
+    return rc;
   }
   *out = b;
   return MOIL_GPU_OK;
@@ -95,8 +245,138 @@ extern "C" int32_t moil_gpu_batch_destroy(moil_batch *b) {
   return MOIL_GPU_OK;
 }
 
-extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
-                                       double *d_energies, void *stream);
+template <class A>
+static void blob_layout(Tab<A> &t, size_t &off) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  t.off_a = off;
+  off = up(off + sizeof(A) * std::max<size_t>(t.args.size(), 1));
+  t.off_b = off;
+  off = up(off + sizeof(int32_t) * t.bs.size());
+}
+template <class A>
+static void blob_fill(const Tab<A> &t, std::vector<char> &h) {
+  if (!t.args.empty()) memcpy(h.data() + t.off_a, t.args.data(), sizeof(A) * t.args.size());
+  memcpy(h.data() + t.off_b, t.bs.data(), sizeof(int32_t) * t.bs.size());
+}
+#define TAB_PTRS(b, t) (decltype(b->tab.t.args.data()))(b->d_blob + b->tab.t.off_a), \
+                       (const int32_t *)(b->d_blob + b->tab.t.off_b), b->tab.t.nsys(), b->tab.t.total()
+
+// argument tables of every kernel for the current lists and pointers
+static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *const *d_dpot,
+                            double *const *d_energies) {
+  Tables &T = b->tab;
+  T = Tables();
+  const size_t rs = b->f64 ? sizeof(double) : sizeof(float);
+  for (int i : b->order) {
+    moil_ctx *c = b->ctx[i];
+    EvalPlan p;
+    eval_plan(c, d_coor[i], d_dpot[i], d_energies[i], &p);
+    if (p.pme) {
+      const FftGroup &g = b->groups[b->group_of[i]];
+      const size_t k = (size_t)b->slot_of[i];
+      void *qr = (char *)g.qreal + rs * g.G * k;
+      void *qc = (char *)g.qcplx + 2 * rs * g.H * k;
+      p.tsum.grid = qr;
+      p.conv.S = qc;
+      p.gather.grid = qr;
+      T.tbin.add(p.tbin, p.n_tbin);
+      T.tscan.add(p.tscan, 1);
+      T.tscat.add(p.tscat, p.n_tbin);
+      T.tspread.add(p.tspread, p.n_tile);
+      T.tsum.add(p.tsum, p.n_tsum);
+      T.conv.add(p.conv, p.n_conv);
+      T.gather.add(p.gather, p.n_tile);
+    }
+    T.intra.add(p.intra, p.n_intra);
+    T.bonded.add(p.bonded, p.n_bonded);
+    T.pack.add(p.pack, p.n_pack);
+    T.water.add(p.water, p.n_water);
+    T.wred.add(p.wred, p.n_wred);
+    T.atoms.add(p.atoms, p.n_atoms);
+    T.ared.add(p.ared, p.n_ared);
+    T.fin.add(p.fin, p.n_fin);
+    T.en.add(p.en, EN_COUNT);
+  }
+  size_t off = 0;
+  blob_layout(T.tbin, off); blob_layout(T.tscan, off); blob_layout(T.tscat, off);
+  blob_layout(T.tspread, off); blob_layout(T.tsum, off); blob_layout(T.conv, off);
+  blob_layout(T.gather, off); blob_layout(T.intra, off); blob_layout(T.bonded, off);
+  blob_layout(T.pack, off); blob_layout(T.water, off); blob_layout(T.wred, off);
+  blob_layout(T.atoms, off); blob_layout(T.ared, off); blob_layout(T.fin, off);
+  blob_layout(T.en, off);
+  std::vector<char> h(off, 0);
+  blob_fill(T.tbin, h); blob_fill(T.tscan, h); blob_fill(T.tscat, h);
+  blob_fill(T.tspread, h); blob_fill(T.tsum, h); blob_fill(T.conv, h);
+  blob_fill(T.gather, h); blob_fill(T.intra, h); blob_fill(T.bonded, h);
+  blob_fill(T.pack, h); blob_fill(T.water, h); blob_fill(T.wred, h);
+  blob_fill(T.atoms, h); blob_fill(T.ared, h); blob_fill(T.fin, h);
+  blob_fill(T.en, h);
+  if (off > b->blob_cap) {
+    HIP_CHECK(hipStreamSynchronize(b->cap));
+    hipFree(b->d_blob);
+    b->d_blob = nullptr;
+    b->blob_cap = 0;
+    HIP_CHECK(hipMalloc(&b->d_blob, off));
+    b->blob_cap = off;
+  }
+  // the previous graph (which reads the blob) has finished: the capture
+  // stream was synchronised or every replay precedes this copy on it
+  HIP_CHECK(hipMemcpyAsync(b->d_blob, h.data(), off, hipMemcpyHostToDevice, b->cap));
+  HIP_CHECK(hipStreamSynchronize(b->cap));
+  return MOIL_GPU_OK;
+}
+
+// the batch step: every kernel once over all systems, transforms per group
+static int32_t batch_enqueue(moil_batch *b, hipStream_t st) {
+  const bool f = b->f64;
+  launch_k_pme_tbin_multi(TAB_PTRS(b, tbin), st);
+  launch_k_pme_tscan_multi(TAB_PTRS(b, tscan), st);
+  if (f) {
+    launch_k_pme_tscatter_f64_multi(TAB_PTRS(b, tscat), st);
+    launch_k_pme_tspread_f64_multi(TAB_PTRS(b, tspread), st);
+    launch_k_pme_tsum_f64_multi(TAB_PTRS(b, tsum), st);
+  } else {
+    launch_k_pme_tscatter_f32_multi(TAB_PTRS(b, tscat), st);
+    launch_k_pme_tspread_f32_multi(TAB_PTRS(b, tspread), st);
+    launch_k_pme_tsum_f32_multi(TAB_PTRS(b, tsum), st);
+  }
+  HIP_CHECK(hipGetLastError());
+  if (b->tab.tbin.nsys() > 0) {
+    for (FftGroup &g : b->groups) {
+      void *in[1] = {g.qreal}, *out[1] = {g.qcplx};
+      RF_TRY(rocfft_execution_info_set_stream(g.info_f, st));
+      RF_TRY(rocfft_execute(g.fwd, in, out, g.info_f));
+    }
+  }
+  if (f) launch_k_conv_f64_multi(TAB_PTRS(b, conv), st);
+  else launch_k_conv_f32_multi(TAB_PTRS(b, conv), st);
+  if (b->tab.tbin.nsys() > 0) {
+    for (FftGroup &g : b->groups) {
+      void *in[1] = {g.qcplx}, *out[1] = {g.qreal};
+      RF_TRY(rocfft_execution_info_set_stream(g.info_b, st));
+      RF_TRY(rocfft_execute(g.bwd, in, out, g.info_b));
+    }
+  }
+  if (f) launch_k_gather_f64_multi(TAB_PTRS(b, gather), st);
+  else launch_k_gather_f32_multi(TAB_PTRS(b, gather), st);
+  launch_k_water_intra_multi(TAB_PTRS(b, intra), st);
+  launch_k_bonded_multi(TAB_PTRS(b, bonded), st);
+  if (f) {
+    launch_k_pack_water_f64_multi(TAB_PTRS(b, pack), st);
+    launch_k_water_f64_multi(TAB_PTRS(b, water), st);
+  } else {
+    launch_k_pack_water_f32_multi(TAB_PTRS(b, pack), st);
+    launch_k_water_f32_multi(TAB_PTRS(b, water), st);
+  }
+  launch_k_water_reduce_multi(TAB_PTRS(b, wred), st);
+  if (f) launch_k_atoms_f64_multi(TAB_PTRS(b, atoms), st);
+  else launch_k_atoms_f32_multi(TAB_PTRS(b, atoms), st);
+  launch_k_atoms_reduce_multi(TAB_PTRS(b, ared), st);
+  launch_k_finalize_forces_multi(TAB_PTRS(b, fin), st);
+  launch_k_energies_multi(TAB_PTRS(b, en), st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
 
 extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const *d_coor,
                                              double *const *d_dpot, double *const *d_energies,
@@ -133,25 +413,11 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
                                  b->cap));
         b->ctx[i]->facc_dirty = false;
       }
-    for (int i = 0; i < n; i++) b->ctx[i]->timing = 0;
+    CHECK_RC_B(batch_tables(b, d_coor, d_dpot, d_energies));
     HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int ns = (int)b->streams.size();
-    int32_t rc = MOIL_GPU_OK;
-    for (int k = 0; k < ns; k++) {
-      if (hipEventRecord(b->ev_fork[k], b->cap) != hipSuccess ||
-          hipStreamWaitEvent(b->streams[k], b->ev_fork[k], 0) != hipSuccess)
-        rc = MOIL_GPU_EHIP;
-    }
-    for (int i = 0; rc == MOIL_GPU_OK && i < n; i++)
-      rc = moil_gpu_eforce_dev(b->ctx[i], d_coor[i], d_dpot[i], d_energies[i],
-                               b->streams[i % ns]);
-    for (int k = 0; k < ns; k++) {
-      if (hipEventRecord(b->ev_join[k], b->streams[k]) != hipSuccess ||
-          hipStreamWaitEvent(b->cap, b->ev_join[k], 0) != hipSuccess)
-        rc = MOIL_GPU_EHIP;
-    }
+    const int32_t rc = batch_enqueue(b, b->cap);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    const hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (rc != MOIL_GPU_OK) {
       if (g) hipGraphDestroy(g);
       return rc;

@@ -20,18 +20,10 @@
 // Energies go to per-workgroup partials.
 #include <hip/hip_runtime.h>
 
-#include "ctx.h"
+#include "multi.h"
 
 #define BT 256
 
-struct BPar {
-  double fix_scale;
-  double epstmp14, pick;  // ener14 switches
-  double qfac;            // kofdie/eps (exclusion correction)
-  double beta, erftbdns;
-  int do_el14, do_v14;
-  int max_parts;
-};
 
 __device__ __forceinline__ void add3(long long *facc, int atom, double fx, double fy, double fz,
                                      double s) {
@@ -114,11 +106,6 @@ __device__ inline double dihedral(const double *xi, const double *xj, const doub
   return co;
 }
 
-struct BSeg {
-  int nb, nangl, ntors, nimp, nexcl, n14;
-  const int32_t *ib, *ia, *it, *ii, *ei, *ej, *sp;
-  const double *bp, *ap, *tp, *ip, *p14, *q, *erf;
-};
 
 template <class Sink>
 __device__ inline void bond_calc(const double *coor, int i, int j, double kb, double r0, Sink &sk,
@@ -295,22 +282,14 @@ __device__ inline void block_sums(const double *v, double *red) {
 
 #define WIT 64  // threads per workgroup of the per-water kernel
 
-struct WIntra {
-  int nwat;
-  const int32_t *first;                 // [nwat] first atom (0-based)
-  const int32_t *boff, *aoff, *eoff;    // [nwat+1] CSR offsets
-  const int32_t *bidx, *aidx, *eidx;    // local atom numbers 0..2
-  const double *bpar, *apar;            // k, r0 / k, theta0
-  const double *q, *erf;
-};
 
 // all O(N) terms in one launch over a flat index space
 // [bonds | angles | torsions | impropers | exclusions | 1-4 pairs]
-__global__ __launch_bounds__(BT) void k_bonded(const double *__restrict__ coor, BSeg S, BPar P,
-                                               long long *__restrict__ facc,
-                                               double *__restrict__ ep, int mp) {
+__device__ __forceinline__ void bonded_impl(const double *__restrict__ coor, const BSeg &S,
+                                            const BPar &P, long long *__restrict__ facc,
+                                            double *__restrict__ ep, int mp, int bid) {
   __shared__ double red[BT];
-  int m = blockIdx.x * BT + threadIdx.x;
+  int m = bid * BT + threadIdx.x;
   double e[7] = {0, 0, 0, 0, 0, 0, 0};  // bond theta tors imp corr vdw14 el14
   int o1 = S.nb, o2 = o1 + S.nangl, o3 = o2 + S.ntors, o4 = o3 + S.nimp, o5 = o4 + S.nexcl,
       o6 = o5 + S.n14;
@@ -322,17 +301,22 @@ __global__ __launch_bounds__(BT) void k_bonded(const double *__restrict__ coor, 
   else if (m < o6) term_14(coor, S, m - o5, P, facc, e[5], e[6]);
   const int slot[7] = {EN_BOND, EN_THETA, EN_TORS, EN_IMP, EN_CORR, EN_VDW14, EN_EL14};
   block_sums<7>(e, red);
-  if (threadIdx.x < 7) ep[(size_t)slot[threadIdx.x] * mp + blockIdx.x] = red[threadIdx.x];
+  if (threadIdx.x < 7) ep[(size_t)slot[threadIdx.x] * mp + bid] = red[threadIdx.x];
 }
+__device__ __forceinline__ void bonded_body(const BondedArgs &A, int bid, int nblk) {
+  bonded_impl(A.coor, A.S, A.P, A.facc, A.ep, A.mp, bid);
+}
+DEFINE_KERNEL(k_bonded, BondedArgs, BT, bonded_body)
 
 // one thread per 3-site water: its intra-water bonds, angles and
 // exclusion corrections (terms listed per water in CSR form, local atom
 // numbers 0..2); the water's gradients are stored, not added
-__global__ __launch_bounds__(WIT) void k_water_intra(const double *__restrict__ coor, WIntra W,
-                                                     BPar P, long long *__restrict__ facc,
-                                                     double *__restrict__ ep, int mp, int part0) {
+__device__ __forceinline__ void water_intra_impl(const double *__restrict__ coor, const WIntra &W,
+                                                 const BPar &P, long long *__restrict__ facc,
+                                                 double *__restrict__ ep, int mp, int part0,
+                                                 int bid) {
   __shared__ double red[WIT];
-  const int w = blockIdx.x * WIT + threadIdx.x;
+  const int w = bid * WIT + threadIdx.x;
   double e[3] = {0, 0, 0};  // bond theta corr
   if (w < W.nwat) {
     const int a0 = W.first[w];
@@ -361,8 +345,12 @@ __global__ __launch_bounds__(WIT) void k_water_intra(const double *__restrict__ 
   }
   block_sums<3, WIT>(e, red);
   const int slot[3] = {EN_BOND, EN_THETA, EN_CORR};
-  if (threadIdx.x < 3) ep[(size_t)slot[threadIdx.x] * mp + part0 + blockIdx.x] = red[threadIdx.x];
+  if (threadIdx.x < 3) ep[(size_t)slot[threadIdx.x] * mp + part0 + bid] = red[threadIdx.x];
 }
+__device__ __forceinline__ void water_intra_body(const IntraArgs &A, int bid, int nblk) {
+  water_intra_impl(A.coor, A.W, A.P, A.facc, A.ep, A.mp, A.part0, bid);
+}
+DEFINE_KERNEL(k_water_intra, IntraArgs, WIT, water_intra_body)
 
 static BSeg make_seg(const moil_ctx *c) {
   const uint32_t T = c->par.terms;
@@ -395,7 +383,9 @@ int water_intra_blocks(const moil_ctx *c) {
   return water_intra_on(c) ? (c->nwat + WIT - 1) / WIT : 0;
 }
 
-int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
+// the bonded launches of one evaluation: the intra-water terms first
+// (they store into the cleared accumulators), then the generic terms
+void bonded_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   BPar P;
   const uint32_t T = c->par.terms;
   P.fix_scale = c->fix_scale;
@@ -407,28 +397,36 @@ int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.do_el14 = (T & MOIL_TERM_EL14) != 0;
   P.do_v14 = (T & MOIL_TERM_VDW14) != 0;
   P.max_parts = c->max_parts;
-  BSeg S = make_seg(c);
-  int nblk = bonded_blocks(c);
-  // intra-water terms first: they store into the cleared accumulators
+  const int nblk = bonded_blocks(c);
+  p->n_intra = 0;
   if (water_intra_on(c)) {
-    const uint32_t T2 = c->par.terms;
-    const bool nbon = (T2 & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
-    WIntra W;
+    const bool nbon = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
+    IntraArgs &A = p->intra;
+    WIntra &W = A.W;
     W.nwat = c->nwat;
     W.first = c->d_wat_o;
-    W.boff = (T2 & MOIL_TERM_BOND) ? c->d_wi_boff : c->d_wi_zero;
-    W.aoff = (T2 & MOIL_TERM_ANGLE) ? c->d_wi_aoff : c->d_wi_zero;
+    W.boff = (T & MOIL_TERM_BOND) ? c->d_wi_boff : c->d_wi_zero;
+    W.aoff = (T & MOIL_TERM_ANGLE) ? c->d_wi_aoff : c->d_wi_zero;
     W.eoff = nbon ? c->d_wi_eoff : c->d_wi_zero;
     W.bidx = c->d_wi_bidx; W.aidx = c->d_wi_aidx; W.eidx = c->d_wi_eidx;
     W.bpar = c->d_wi_bpar; W.apar = c->d_wi_apar;
     W.q = c->d_ptchg; W.erf = c->d_erf;
-    k_water_intra<<<water_intra_blocks(c), WIT, 0, st>>>(d_coor, W, P, c->d_facc, c->d_epart,
-                                                         c->max_parts, nblk);
+    A.coor = d_coor;
+    A.P = P;
+    A.facc = c->d_facc;
+    A.ep = c->d_epart;
+    A.mp = c->max_parts;
+    A.part0 = nblk;
+    p->n_intra = water_intra_blocks(c);
   }
-  if (nblk > 0)
-    k_bonded<<<nblk, BT, 0, st>>>(d_coor, S, P, c->d_facc, c->d_epart, c->max_parts);
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
+  BondedArgs &B = p->bonded;
+  B.coor = d_coor;
+  B.S = make_seg(c);
+  B.P = P;
+  B.facc = c->d_facc;
+  B.ep = c->d_epart;
+  B.mp = c->max_parts;
+  p->n_bonded = nblk;
 }
 
 int bonded_max_blocks(const moil_ctx *c) { return max(1, bonded_blocks(c) + water_intra_blocks(c)); }

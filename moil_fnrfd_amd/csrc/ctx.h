@@ -162,21 +162,11 @@ struct moil_ctx {
 int32_t lists_init(moil_ctx *c);
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st);
 void lists_free(moil_ctx *c);
-// real space (realspace.hip)
-int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st);
-int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st, hipEvent_t ev_start,
-                        hipEvent_t ev_stop);
-int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st);
-// atom lists and atom real space (atoms.hip)
+// atom lists (atoms.hip)
 int32_t atoms_init(moil_ctx *c, const moil_topology *t);
 int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st);
 void atoms_free(moil_ctx *c);
-int32_t realspace_atoms(moil_ctx *c, const double *d_coor, hipStream_t st, int part0,
-                        hipEvent_t ev_start, hipEvent_t ev_stop);
-int32_t realspace_atoms_reduce(moil_ctx *c, hipStream_t st);
-// PME (pme.hip)
+// PME set-up (pme.hip)
 int32_t pme_init(moil_ctx *c);
-int32_t pme_run(moil_ctx *c, const double *d_coor, hipStream_t st);
 void pme_free(moil_ctx *c);
-// bonded, exclusion correction, 1-4 (bonded.hip)
-int32_t bonded_run(moil_ctx *c, const double *d_coor, hipStream_t st);
+// The launches of an evaluation are described by plans (kargs.h).

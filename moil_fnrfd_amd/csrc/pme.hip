@@ -31,17 +31,11 @@
 #include <cmath>
 #include <vector>
 
-#include "ctx.h"
+#include "multi.h"
 
 #define PME_THREADS 256
 #define QFIX_SHIFT 40  // grid charge fixed point: 2^-40
 
-struct PmePar {
-  int n1, n2, n3, nc1;
-  double ra, rb, rc;  // 1/a, 1/b, 1/c
-  double fc;          // sqrt(kofdie/eps)
-  double qscale, qinv;
-};
 
 // order-4 cardinal B-spline weights and derivatives at fractional offset
 // w (pot/ewald_receip.f:1013-1080: linear init, one recursion to order 3,
@@ -84,12 +78,7 @@ __device__ __forceinline__ void frac_coord(double x, double r, int n, int *i0, d
 // tile's 3-point low-side halo reaches one neighbour only).  An atom
 // belongs to the tile holding its stencil base point int(fr); its 64
 // stencil points base-3 .. base lie in that tile or its halo.
-struct TilePar {
-  int n1, n2, n3;
-  int nt1, nt2, nt3;
-  int r1, r2, r3;  // LDS region dims: max tile size + 3
-  int rsz;         // r1 r2 r3
-};
+// (TilePar: r1 r2 r3 = LDS region dims, max tile size + 3; rsz = r1 r2 r3)
 
 __device__ __forceinline__ int tile_of(int g, int n, int nt) { return ((g + 1) * nt - 1) / n; }
 __device__ __forceinline__ int tile_lo(int t, int n, int nt) { return (t * n) / nt; }
@@ -112,13 +101,12 @@ __device__ __forceinline__ void base_point(const double *x, const PmePar &P, int
 // wavefront mostly share a tile, and each wavefront issues one counter
 // atomic per distinct tile (ranks inside a tile may come out in any
 // order: the spreading sums are integer and the gather is per atom)
-__global__ __launch_bounds__(PME_THREADS) void k_pme_tbin(const double *__restrict__ coor, int npt,
-                                                          const int32_t *__restrict__ order,
-                                                          PmePar P, TilePar T,
-                                                          int32_t *__restrict__ tcnt,
-                                                          int32_t *__restrict__ atile,
-                                                          int32_t *__restrict__ arank) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tbin_impl(const double *__restrict__ coor, int npt,
+                                          const int32_t *__restrict__ order, const PmePar &P,
+                                          const TilePar &T, int32_t *__restrict__ tcnt,
+                                          int32_t *__restrict__ atile,
+                                          int32_t *__restrict__ arank, int bid) {
+  const int p = bid * PME_THREADS + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = p < npt;
   const int n = valid ? order[p] : 0;
@@ -144,11 +132,15 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tbin(const double *__restri
     todo &= ~same;
   }
 }
+__device__ __forceinline__ void tbin_body(const TbinArgs &A, int bid, int nblk) {
+  tbin_impl(A.coor, A.npt, A.order, A.P, A.T, A.tcnt, A.atile, A.arank, bid);
+}
+DEFINE_KERNEL(k_pme_tbin, TbinArgs, PME_THREADS, tbin_body)
 
 // exclusive scan of the tile counts (one workgroup); clears the counts
 // for the next evaluation
-__global__ __launch_bounds__(1024) void k_pme_tscan(int32_t *__restrict__ cnt, int n,
-                                                    int32_t *__restrict__ start) {
+__device__ __forceinline__ void tscan_impl(int32_t *__restrict__ cnt, int n,
+                                           int32_t *__restrict__ start) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
   const int per = (n + 1023) / 1024;
@@ -172,21 +164,23 @@ __global__ __launch_bounds__(1024) void k_pme_tscan(int32_t *__restrict__ cnt, i
   }
   if (t == 1023) start[n] = part[1023];
 }
+__device__ __forceinline__ void tscan_body(const TscanArgs &A, int bid, int nblk) {
+  tscan_impl(A.cnt, A.n, A.start);
+}
+DEFINE_KERNEL(k_pme_tscan, TscanArgs, 1024, tscan_body)
 
 // counting-sort scatter; the sorted record of an atom holds its B-spline
 // weights theta1(4), theta2(4) and q sqrt(kofdie/eps) theta3(4), its base
 // point (packed 10 bits per dimension) and its atom index
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_tscatter(const double *__restrict__ coor,
-                                                              const double *__restrict__ q, int npt,
-                                                              PmePar P,
-                                                              const int32_t *__restrict__ atile,
-                                                              const int32_t *__restrict__ arank,
-                                                              const int32_t *__restrict__ tstart,
-                                                              R *__restrict__ sw,
-                                                              int32_t *__restrict__ sbase,
-                                                              int32_t *__restrict__ sidx) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tscatter_impl(const double *__restrict__ coor,
+                                              const double *__restrict__ q, int npt,
+                                              const PmePar &P, const int32_t *__restrict__ atile,
+                                              const int32_t *__restrict__ arank,
+                                              const int32_t *__restrict__ tstart,
+                                              R *__restrict__ sw, int32_t *__restrict__ sbase,
+                                              int32_t *__restrict__ sidx, int bid) {
+  int n = bid * PME_THREADS + threadIdx.x;
   if (n >= npt) return;
   int c1, c2, c3;
   double w1, w2, w3;
@@ -207,6 +201,13 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tscatter(const double *__re
   sbase[p] = c1 | (c2 << 10) | (c3 << 20);
   sidx[p] = n;
 }
+template <typename R>
+__device__ __forceinline__ void tscatter_body(const TscatterArgs &A, int bid, int nblk) {
+  tscatter_impl<R>(A.coor, A.q, A.npt, A.P, A.atile, A.arank, A.tstart, (R *)A.sw, A.sbase,
+                   A.sidx, bid);
+}
+DEFINE_KERNEL(k_pme_tscatter_f32, TscatterArgs, PME_THREADS, tscatter_body<float>)
+DEFINE_KERNEL(k_pme_tscatter_f64, TscatterArgs, PME_THREADS, tscatter_body<double>)
 
 // one workgroup per tile: the tile's atom records are staged in LDS
 // (coalesced loads, 256 atoms per pass), then every atom adds its 64
@@ -215,15 +216,14 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tscatter(const double *__re
 // is then written to the tile's slot of tbuf
 #define TS_STAGE 256
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_tspread(const int32_t *__restrict__ tstart,
-                                                             const R *__restrict__ sw,
-                                                             const int32_t *__restrict__ sbase,
-                                                             TilePar T, double qscale,
-                                                             long long *__restrict__ tbuf) {
+__device__ __forceinline__ void tspread_impl(const int32_t *__restrict__ tstart,
+                                             const R *__restrict__ sw,
+                                             const int32_t *__restrict__ sbase, const TilePar &T,
+                                             double qscale, long long *__restrict__ tbuf, int bid) {
   __shared__ unsigned long long acc[11 * 11 * 11];
   __shared__ R rec[TS_STAGE * 12];
   __shared__ int32_t rb[TS_STAGE];
-  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tile = bid, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
@@ -252,16 +252,21 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tspread(const int32_t *__re
   long long *dst = tbuf + (size_t)tile * T.rsz;
   for (int u = t; u < T.rsz; u += PME_THREADS) dst[u] = (long long)acc[u];
 }
+template <typename R>
+__device__ __forceinline__ void tspread_body(const TspreadArgs &A, int bid, int nblk) {
+  tspread_impl<R>(A.tstart, (const R *)A.sw, A.sbase, A.T, A.qscale, A.tbuf, bid);
+}
+DEFINE_KERNEL(k_pme_tspread_f32, TspreadArgs, PME_THREADS, tspread_body<float>)
+DEFINE_KERNEL(k_pme_tspread_f64, TspreadArgs, PME_THREADS, tspread_body<double>)
 
 // per grid point: its own tile's region entry plus the halo entries of
 // the next tile in each dimension where it lies in that tile's halo
 // (at most 8 terms), converted to the transform's real type
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_pme_tsum(const long long *__restrict__ tbuf,
-                                                          TilePar T, double qinv,
-                                                          R *__restrict__ grid) {
+__device__ __forceinline__ void tsum_impl(const long long *__restrict__ tbuf, const TilePar &T,
+                                          double qinv, R *__restrict__ grid, int bid) {
   const size_t G = (size_t)T.n1 * T.n2 * T.n3;
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t g = (size_t)bid * PME_THREADS + threadIdx.x;
   if (g >= G) return;
   const int gg[3] = {(int)(g % T.n1), (int)((g / T.n1) % T.n2), (int)(g / ((size_t)T.n1 * T.n2))};
   const int nn[3] = {T.n1, T.n2, T.n3}, nt[3] = {T.nt1, T.nt2, T.nt3};
@@ -285,15 +290,21 @@ __global__ __launch_bounds__(PME_THREADS) void k_pme_tsum(const long long *__res
       }
   grid[g] = (R)((double)sum * qinv);
 }
+template <typename R>
+__device__ __forceinline__ void tsum_body(const TsumArgs &A, int bid, int nblk) {
+  tsum_impl<R>(A.tbuf, A.T, A.qinv, (R *)A.grid, bid);
+}
+DEFINE_KERNEL(k_pme_tsum_f32, TsumArgs, PME_THREADS, tsum_body<float>)
+DEFINE_KERNEL(k_pme_tsum_f64, TsumArgs, PME_THREADS, tsum_body<double>)
 
 // half-spectrum convolution + energy (weights 1 for the k1 = 0 and
 // k1 = n1/2 planes, 2 otherwise)
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_conv(R *__restrict__ S, const double *__restrict__ eterm,
-                                                      int nc1, int n1, size_t H,
-                                                      double *__restrict__ epart) {
+__device__ __forceinline__ void conv_impl(R *__restrict__ S, const double *__restrict__ eterm,
+                                          int nc1, int n1, size_t H, double *__restrict__ epart,
+                                          int bid) {
   __shared__ double red[PME_THREADS];
-  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t g = (size_t)bid * PME_THREADS + threadIdx.x;
   double e = 0.0;
   if (g < H) {
     int k1 = (int)(g % nc1);
@@ -311,8 +322,14 @@ __global__ __launch_bounds__(PME_THREADS) void k_conv(R *__restrict__ S, const d
     if (t < off) red[t] += red[t + off];
     __syncthreads();
   }
-  if (t == 0) epart[blockIdx.x] = 0.5 * red[0];
+  if (t == 0) epart[bid] = 0.5 * red[0];
 }
+template <typename R>
+__device__ __forceinline__ void conv_body(const ConvArgs &A, int bid, int nblk) {
+  conv_impl<R>((R *)A.S, A.eterm, A.nc1, A.n1, A.H, A.epart, bid);
+}
+DEFINE_KERNEL(k_conv_f32, ConvArgs, PME_THREADS, conv_body<float>)
+DEFINE_KERNEL(k_conv_f64, ConvArgs, PME_THREADS, conv_body<double>)
 
 // one workgroup per tile: the tile + halo region of the convolved grid
 // goes to LDS, then one thread per atom of the tile (tile order)
@@ -320,15 +337,14 @@ __global__ __launch_bounds__(PME_THREADS) void k_conv(R *__restrict__ S, const d
 // its 64 stencil points from LDS; each atom's reciprocal gradient is
 // written directly (no atomics)
 template <typename R>
-__global__ __launch_bounds__(PME_THREADS) void k_gather(const double *__restrict__ coor,
-                                                        const double *__restrict__ q,
-                                                        const int32_t *__restrict__ tstart,
-                                                        const int32_t *__restrict__ sidx,
-                                                        PmePar P, TilePar T,
-                                                        const R *__restrict__ grid,
-                                                        double *__restrict__ fpme) {
+__device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
+                                            const double *__restrict__ q,
+                                            const int32_t *__restrict__ tstart,
+                                            const int32_t *__restrict__ sidx, const PmePar &P,
+                                            const TilePar &T, const R *__restrict__ grid,
+                                            double *__restrict__ fpme, int bid) {
   __shared__ R reg[11 * 11 * 11];
-  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tile = bid, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
@@ -384,6 +400,12 @@ __global__ __launch_bounds__(PME_THREADS) void k_gather(const double *__restrict
     fpme[3 * n + 2] = qn * f3 * (double)P.n3 * P.rc;
   }
 }
+template <typename R>
+__device__ __forceinline__ void gather_body(const GatherArgs &A, int bid, int nblk) {
+  gather_impl<R>(A.coor, A.q, A.tstart, A.sidx, A.P, A.T, (const R *)A.grid, A.fpme, bid);
+}
+DEFINE_KERNEL(k_gather_f32, GatherArgs, PME_THREADS, gather_body<float>)
+DEFINE_KERNEL(k_gather_f64, GatherArgs, PME_THREADS, gather_body<double>)
 
 // ---------------------------------------------------------------------------
 static void bsp_moduli(std::vector<double> &mod, int nfft) {
@@ -553,8 +575,9 @@ int pme_conv_blocks(const moil_ctx *c) {
   return (int)((H + PME_THREADS - 1) / PME_THREADS);
 }
 
-template <typename R>
-static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
+// the reciprocal-space launches of one evaluation (grid buffers of the
+// context; a batch substitutes its own transform buffers)
+void pme_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   const size_t G = (size_t)c->nf1 * c->nf2 * c->nf3;
   const size_t H = (size_t)c->nc1 * c->nf2 * c->nf3;
   PmePar P;
@@ -563,39 +586,55 @@ static int32_t pme_run_t(moil_ctx *c, const double *d_coor, hipStream_t st) {
   P.fc = (c->par.terms & MOIL_TERM_EL) ? sqrt(KOFDIE / c->par.eps) : 0.0;
   P.qscale = ldexp(1.0, QFIX_SHIFT);
   P.qinv = ldexp(1.0, -QFIX_SHIFT);
-  const int gb = (int)((G + PME_THREADS - 1) / PME_THREADS);
-  const int ab = (c->npt + PME_THREADS - 1) / PME_THREADS;
   TilePar T;
   T.n1 = c->nf1; T.n2 = c->nf2; T.n3 = c->nf3;
   T.nt1 = c->pt_nt[0]; T.nt2 = c->pt_nt[1]; T.nt3 = c->pt_nt[2];
   T.r1 = c->pt_r[0]; T.r2 = c->pt_r[1]; T.r3 = c->pt_r[2];
   T.rsz = T.r1 * T.r2 * T.r3;
-  k_pme_tbin<<<ab, PME_THREADS, 0, st>>>(d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile,
-                                         c->d_arank);
-  k_pme_tscan<<<1, 1024, 0, st>>>(c->d_tcnt, c->pt_ntile, c->d_tstart);
-  k_pme_tscatter<R><<<ab, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->npt, P, c->d_atile,
-                                                c->d_arank, c->d_tstart, (R *)c->d_sw,
-                                                c->d_sbase, c->d_sidx);
-  k_pme_tspread<R><<<c->pt_ntile, PME_THREADS, 0, st>>>(c->d_tstart, (const R *)c->d_sw,
-                                                        c->d_sbase, T, P.qscale, c->d_tbuf);
-  k_pme_tsum<R><<<gb, PME_THREADS, 0, st>>>(c->d_tbuf, T, P.qinv, (R *)c->d_qreal);
+  p->pme = true;
+  p->n_tbin = (c->npt + PME_THREADS - 1) / PME_THREADS;
+  p->tbin = {d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile, c->d_arank};
+  p->tscan = {c->d_tcnt, c->pt_ntile, c->d_tstart};
+  p->tscat = {d_coor, c->d_ptchg, c->npt, P, c->d_atile, c->d_arank, c->d_tstart,
+              c->d_sw, c->d_sbase, c->d_sidx};
+  p->n_tile = c->pt_ntile;
+  p->tspread = {c->d_tstart, c->d_sw, c->d_sbase, T, P.qscale, c->d_tbuf};
+  p->n_tsum = (int)((G + PME_THREADS - 1) / PME_THREADS);
+  p->tsum = {c->d_tbuf, T, P.qinv, c->d_qreal};
+  p->n_conv = (int)((H + PME_THREADS - 1) / PME_THREADS);
+  p->conv = {c->d_qcplx, c->d_eterm, c->nc1, c->nf1, H,
+             c->d_epart + (size_t)EN_RECIP * c->max_parts};
+  p->gather = {d_coor, c->d_ptchg, c->d_tstart, c->d_sidx, P, T, c->d_qreal, c->d_fpme};
+}
+
+// spreading (binning, scan, sorted records, tile regions, grid sum)
+void pme_launch_spread(const EvalPlan &p, hipStream_t st) {
+  launch_k_pme_tbin(p.tbin, p.n_tbin, st);
+  launch_k_pme_tscan(p.tscan, 1, st);
+  if (p.f64) {
+    launch_k_pme_tscatter_f64(p.tscat, p.n_tbin, st);
+    launch_k_pme_tspread_f64(p.tspread, p.n_tile, st);
+    launch_k_pme_tsum_f64(p.tsum, p.n_tsum, st);
+  } else {
+    launch_k_pme_tscatter_f32(p.tscat, p.n_tbin, st);
+    launch_k_pme_tspread_f32(p.tspread, p.n_tile, st);
+    launch_k_pme_tsum_f32(p.tsum, p.n_tsum, st);
+  }
+}
+
+int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
+  pme_launch_spread(p, st);
   HIP_CHECK(hipGetLastError());
   void *in[1] = {c->d_qreal}, *out[1] = {c->d_qcplx};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_fwd, st));
   RF_CHECK(rocfft_execute(c->plan_fwd, in, out, c->exec_fwd));
-  const int hb = (int)((H + PME_THREADS - 1) / PME_THREADS);
-  k_conv<R><<<hb, PME_THREADS, 0, st>>>((R *)c->d_qcplx, c->d_eterm, c->nc1, c->nf1, H,
-                                          c->d_epart + (size_t)EN_RECIP * c->max_parts);
+  if (p.f64) launch_k_conv_f64(p.conv, p.n_conv, st);
+  else launch_k_conv_f32(p.conv, p.n_conv, st);
   void *in2[1] = {c->d_qcplx}, *out2[1] = {c->d_qreal};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_bwd, st));
   RF_CHECK(rocfft_execute(c->plan_bwd, in2, out2, c->exec_bwd));
-  k_gather<R><<<c->pt_ntile, PME_THREADS, 0, st>>>(d_coor, c->d_ptchg, c->d_tstart, c->d_sidx, P, T,
-                                                   (const R *)c->d_qreal, c->d_fpme);
+  if (p.f64) launch_k_gather_f64(p.gather, p.n_tile, st);
+  else launch_k_gather_f32(p.gather, p.n_tile, st);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
-}
-
-int32_t pme_run(moil_ctx *c, const double *d_coor, hipStream_t st) {
-  if (c->par.precision == MOIL_PREC_MIXED) return pme_run_t<float>(c, d_coor, st);
-  return pme_run_t<double>(c, d_coor, st);
 }

@@ -33,17 +33,9 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
-#include "ctx.h"
+#include "multi.h"
 
 #define RS_THREADS 256
-
-struct WaterRS {
-  double qq_oo, qq_oh, qq_hh, oo12, oo6;
-  double beta, cutvdw2, cutele2, erftbdns;
-  double a, b, c;
-  double fix_scale;
-  int ntab;
-};
 
 // erfc fit coefficients (t = 1/(1 + ERFC_P x)); see file header
 #define ERFC_P 0.418f
@@ -83,11 +75,12 @@ __device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
 
 // ---------------------------------------------------------------------------
 // pack water positions per slot
-__global__ void k_pack_water_f32(const double *__restrict__ coor, const int32_t *__restrict__ slot_o,
-                                 int nwat, float4 *__restrict__ wpos) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nwat) return;
-  int o = slot_o[s];
+__device__ __forceinline__ void pack_f32_body(const PackArgs &A, int bid, int nblk) {
+  int s = bid * RS_THREADS + threadIdx.x;
+  if (s >= A.nwat) return;
+  const double *coor = A.coor;
+  float4 *wpos = (float4 *)A.wpos;
+  int o = A.slot_o[s];
   double ox = coor[3 * o], oy = coor[3 * o + 1], oz = coor[3 * o + 2];
   float hx = (float)ox, hy = (float)oy, hz = (float)oz;
   float lx = (float)(ox - (double)hx), ly = (float)(oy - (double)hy), lz = (float)(oz - (double)hz);
@@ -99,15 +92,17 @@ __global__ void k_pack_water_f32(const double *__restrict__ coor, const int32_t 
   wpos[3 * s + 1] = make_float4(lx, ly, lz, h1y);
   wpos[3 * s + 2] = make_float4(h1z, h2x, h2y, h2z);
 }
+DEFINE_KERNEL(k_pack_water_f32, PackArgs, RS_THREADS, pack_f32_body)
 
-__global__ void k_pack_water_f64(const double *__restrict__ coor, const int32_t *__restrict__ slot_o,
-                                 int nwat, double *__restrict__ wpos) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nwat) return;
-  int o = slot_o[s];
+__device__ __forceinline__ void pack_f64_body(const PackArgs &A, int bid, int nblk) {
+  int s = bid * RS_THREADS + threadIdx.x;
+  if (s >= A.nwat) return;
+  int o = A.slot_o[s];
+  double *wpos = (double *)A.wpos;
 #pragma unroll
-  for (int k = 0; k < 9; k++) wpos[9 * s + k] = coor[3 * o + k];
+  for (int k = 0; k < 9; k++) wpos[9 * s + k] = A.coor[3 * o + k];
 }
+DEFINE_KERNEL(k_pack_water_f64, PackArgs, RS_THREADS, pack_f64_body)
 
 // ---------------------------------------------------------------------------
 // block-level deterministic reduction of per-thread doubles
@@ -125,16 +120,16 @@ __device__ inline double block_sum(double v, double *red) {
 }
 
 // FP32 (mixed precision) water kernel
-__global__ __launch_bounds__(RS_THREADS) void k_water_f32(
+__device__ __forceinline__ void water_f32_impl(
     const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
-    const int32_t *__restrict__ jpos, const float4 *__restrict__ wpos, int nwat, WaterRS P,
+    const int32_t *__restrict__ jpos, const float4 *__restrict__ wpos, int nwat, const WaterRS &P,
     long long *__restrict__ jpart, long long *__restrict__ ipart, double *__restrict__ epart,
-    int max_parts) {
+    int max_parts, int bid, int nblk) {
   // i-force partials: 64-bit fixed point (2^-32) in LDS
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = chunks[3 * gridDim.x + blockIdx.x], t = threadIdx.x;  // largest first
+  const int ch = chunks[3 * nblk + bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
@@ -258,16 +253,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f32(
   if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
 }
 
+__device__ __forceinline__ void water_f32_body(const WaterArgs &A, int bid, int nblk) {
+  water_f32_impl(A.chunks, A.items, A.jpos, (const float4 *)A.wpos, A.nwat, A.P, A.jpart, A.ipart,
+                 A.epart, A.max_parts, bid, nblk);
+}
+DEFINE_KERNEL(k_water_f32, WaterArgs, RS_THREADS, water_f32_body)
+
 // FP64 water kernel (MOIL parity mode)
-__global__ __launch_bounds__(RS_THREADS) void k_water_f64(
+__device__ __forceinline__ void water_f64_impl(
     const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
-    const int32_t *__restrict__ jpos, const double *__restrict__ wpos, int nwat, WaterRS P,
+    const int32_t *__restrict__ jpos, const double *__restrict__ wpos, int nwat, const WaterRS &P,
     const double *__restrict__ erf_arr, long long *__restrict__ jpart, long long *__restrict__ ipart,
-    double *__restrict__ epart, int max_parts) {
+    double *__restrict__ epart, int max_parts, int bid, int nblk) {
   __shared__ double ipos[WBLOCK * 9];
   __shared__ long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = chunks[3 * gridDim.x + blockIdx.x], t = threadIdx.x;  // largest first
+  const int ch = chunks[3 * nblk + bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
@@ -373,55 +374,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_water_f64(
   if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
 }
 
-int32_t realspace_water_pack(moil_ctx *c, const double *d_coor, hipStream_t st) {
-  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
-  const int TB = 256, nb = (c->nwat + TB - 1) / TB;
-  if (c->par.precision == MOIL_PREC_MIXED)
-    k_pack_water_f32<<<nb, TB, 0, st>>>(d_coor, c->d_slot_o, c->nwat, (float4 *)c->d_wpos);
-  else
-    k_pack_water_f64<<<nb, TB, 0, st>>>(d_coor, c->d_slot_o, c->nwat, (double *)c->d_wpos);
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
+__device__ __forceinline__ void water_f64_body(const WaterArgs &A, int bid, int nblk) {
+  water_f64_impl(A.chunks, A.items, A.jpos, (const double *)A.wpos, A.nwat, A.P, A.erf, A.jpart,
+                 A.ipart, A.epart, A.max_parts, bid, nblk);
 }
-
-int32_t realspace_water(moil_ctx *c, const double *d_coor, hipStream_t st, hipEvent_t ev_start,
-                        hipEvent_t ev_stop) {
-  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
-  (void)d_coor;
-  WaterRS P;
-  P.qq_oo = c->qq_oo; P.qq_oh = c->qq_oh; P.qq_hh = c->qq_hh;
-  P.oo12 = c->oo12; P.oo6 = c->oo6;
-  P.beta = c->beta; P.cutvdw2 = c->par.cutvdw2; P.cutele2 = c->par.cutele2;
-  P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
-  P.fix_scale = c->fix_scale; P.ntab = c->ntab;
-  // launched through hipExtLaunchKernelGGL so optional start/stop events
-  // are recorded by the dispatch itself
-  if (c->par.precision == MOIL_PREC_MIXED) {
-    hipExtLaunchKernelGGL(k_water_f32, dim3(c->nwchunk), dim3(RS_THREADS), 0, st, ev_start, ev_stop,
-                          0, (const int32_t *)c->d_wchunk, (const WItem *)c->d_witems,
-                          (const int32_t *)c->d_jpos, (const float4 *)c->d_wpos, c->nwat, P,
-                          c->d_jpart, c->d_ipart, c->d_epart, c->max_parts);
-  } else {
-    hipExtLaunchKernelGGL(k_water_f64, dim3(c->nwchunk), dim3(RS_THREADS), 0, st, ev_start, ev_stop,
-                          0, (const int32_t *)c->d_wchunk, (const WItem *)c->d_witems,
-                          (const int32_t *)c->d_jpos, (const double *)c->d_wpos, c->nwat, P,
-                          (const double *)c->d_erf, c->d_jpart, c->d_ipart, c->d_epart,
-                          c->max_parts);
-  }
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
-}
+DEFINE_KERNEL(k_water_f64, WaterArgs, RS_THREADS, water_f64_body)
 
 // one thread per (water slot, gradient component): the slot's partials --
 // its jpart rows (contiguous at the slot's CSR range) and its entries in
 // the ipart rows of its block's chunks, all 64-bit fixed point -- summed
 // as integers (order independent) and added to the accumulator.  The
 // slot's atoms are touched by no other thread, so no atomics.
-__global__ __launch_bounds__(256) void k_water_reduce(
+__device__ __forceinline__ void water_reduce_impl(
     const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
     const int32_t *__restrict__ cfirst, const long long *__restrict__ ipart,
-    const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc, int bid) {
+  const int g = bid * 256 + threadIdx.x;
   if (g >= 9 * nwat) return;
   const int s = g / 9, k = g - 9 * s;
   long long a0 = 0, a1 = 0;
@@ -436,12 +404,59 @@ __global__ __launch_bounds__(256) void k_water_reduce(
   for (int c = cfirst[b]; c < cfirst[b + 1]; c++) a1 += ipart[(size_t)c * (WBLOCK * 9) + i * 9 + k];
   facc[3 * (size_t)slot_o[s] + k] += a0 + a1;
 }
+__device__ __forceinline__ void water_reduce_body(const WReduceArgs &A, int bid, int nblk) {
+  water_reduce_impl(A.jstart, A.jpart, A.cfirst, A.ipart, A.slot_o, A.nwat, A.facc, bid);
+}
+DEFINE_KERNEL(k_water_reduce, WReduceArgs, 256, water_reduce_body)
 
-int32_t realspace_water_reduce(moil_ctx *c, hipStream_t st) {
-  if (c->nwat < 2 || c->nwchunk == 0) return MOIL_GPU_OK;
-  k_water_reduce<<<(9 * c->nwat + 255) / 256, 256, 0, st>>>(c->d_jstart, c->d_jpart, c->d_cfirst,
-                                                             c->d_ipart, c->d_slot_o, c->nwat,
-                                                             c->d_facc);
-  HIP_CHECK(hipGetLastError());
-  return MOIL_GPU_OK;
+// ---------------------------------------------------------------------------
+// the water real-space launches of one evaluation: pack, pair kernel,
+// reduction (none with fewer than two waters or an empty list)
+void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
+  p->n_pack = p->n_water = p->n_wred = 0;
+  if (c->nwat < 2 || c->nwchunk == 0) return;
+  p->pack.coor = d_coor;
+  p->pack.slot_o = c->d_slot_o;
+  p->pack.nwat = c->nwat;
+  p->pack.wpos = c->d_wpos;
+  p->n_pack = (c->nwat + RS_THREADS - 1) / RS_THREADS;
+  WaterRS P;
+  P.qq_oo = c->qq_oo; P.qq_oh = c->qq_oh; P.qq_hh = c->qq_hh;
+  P.oo12 = c->oo12; P.oo6 = c->oo6;
+  P.beta = c->beta; P.cutvdw2 = c->par.cutvdw2; P.cutele2 = c->par.cutele2;
+  P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
+  P.fix_scale = c->fix_scale; P.ntab = c->ntab;
+  WaterArgs &W = p->water;
+  W.chunks = c->d_wchunk;
+  W.items = c->d_witems;
+  W.jpos = c->d_jpos;
+  W.wpos = c->d_wpos;
+  W.nwat = c->nwat;
+  W.P = P;
+  W.erf = c->d_erf;
+  W.jpart = c->d_jpart;
+  W.ipart = c->d_ipart;
+  W.epart = c->d_epart;
+  W.max_parts = c->max_parts;
+  p->n_water = c->nwchunk;
+  WReduceArgs &R = p->wred;
+  R.jstart = c->d_jstart;
+  R.jpart = c->d_jpart;
+  R.cfirst = c->d_cfirst;
+  R.ipart = c->d_ipart;
+  R.slot_o = c->d_slot_o;
+  R.nwat = c->nwat;
+  R.facc = c->d_facc;
+  p->n_wred = (9 * c->nwat + 255) / 256;
+}
+
+// launched through hipExtLaunchKernelGGL so that optional start/stop
+// events are recorded by the dispatch itself
+void launch_k_water_ev(const WaterArgs &a, bool f64, int nblk, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1) {
+  if (nblk <= 0) return;
+  if (f64)
+    hipExtLaunchKernelGGL(k_water_f64, dim3(nblk), dim3(RS_THREADS), 0, st, e0, e1, 0, a);
+  else
+    hipExtLaunchKernelGGL(k_water_f32, dim3(nblk), dim3(RS_THREADS), 0, st, e0, e1, 0, a);
 }

@@ -1,9 +1,12 @@
 """Batched evaluation (moil_gpu_batch_*): several systems of different
-sizes and kinds evaluated by one call that replays a captured hipGraph.
-Every replay must reproduce one moil_gpu_eforce_dev per system bit for bit
-(the kernels are the same and their sums are order independent), also
-after eager work between replays and after a list rebuild forces a new
-capture."""
+sizes and kinds evaluated by one call that replays a captured hipGraph of
+multi-system launches (one grid per kernel over all systems, one batched
+transform per grid size).  Every replay must reproduce one
+moil_gpu_eforce_dev per system bit for bit (the kernel bodies are the same
+and their sums are order independent), in both precisions, also after
+eager work between replays and after a list rebuild forces a new capture.
+Two pairs of systems share a grid size, so batched transforms of more than
+one member are covered."""
 import numpy as np
 import pytest
 
@@ -20,10 +23,13 @@ def _systems():
         out.append((name, s, cut, ew))
     s, cut, ew = configs.water_scaled(8, seed=1300)
     out.append(('water512', s, cut, ew))
+    s, cut, ew = configs.water_scaled(8, seed=1301)
+    out.append(('water512b', s, cut, ew))
     return out
 
 
-def test_batch_matches_plain_calls():
+@pytest.mark.parametrize('prec', [abi.PREC_MIXED, abi.PREC_FP64], ids=['mixed', 'fp64'])
+def test_batch_matches_plain_calls(prec):
     import torch
     from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
     torch.cuda.set_device(0)
@@ -32,8 +38,8 @@ def test_batch_matches_plain_calls():
     sh = stream.cuda_stream
     systems, coor, dpot_b, en_b, dpot_p, en_p = [], [], [], [], [], []
     for name, s, cut, ew in _systems():
-        print('case batch system %s npt=%d' % (name, s.topo.npt), flush=True)
-        m = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+        print('case batch system %s npt=%d prec=%d' % (name, s.topo.npt, prec), flush=True)
+        m = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=prec))
         x = torch.tensor(s.coor, dtype=torch.float64, device=dev)
         m.squeeze_dev(x.data_ptr(), sh)
         m.nbondm_dev(x.data_ptr(), sh)
@@ -79,4 +85,18 @@ def test_batch_matches_plain_calls():
         same()
     batch.close()
     for m in systems:
+        m.close()
+
+
+def test_batch_refuses_mixed_precisions():
+    import torch
+    from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
+    torch.cuda.set_device(0)
+    ms = []
+    for prec in (abi.PREC_MIXED, abi.PREC_FP64):
+        s, cut, ew = configs.water216()
+        ms.append(MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=prec)))
+    with pytest.raises(RuntimeError, match='same precision'):
+        MoilBatch(ms)
+    for m in ms:
         m.close()
