@@ -185,6 +185,131 @@ def batched_leg(steps=10, nstreams=8):
     }
 
 
+def adjoint_leg(n_conf=128, seed=4001):
+    """Differentiation workload: the 10,048-atom heteropolymer-in-water
+    system, n_conf conformations (base + N(0, 0.1 A) per coordinate).  Per
+    conformation, timed: list build, forward energy + dE/dx, and the
+    parameter adjoint (dE/dq per atom, dE/deps and dE/dRmin per LJ type)
+    in mixed precision (FP32 pairs, FP64 reductions).  Untimed: the same in
+    FP64 for the error report (max and RMS relative over all
+    conformations), and central finite differences (h = 1e-4, FP64) of a
+    few parameters on the first conformation."""
+    import numpy as np
+    import torch
+
+    from moil_fnrfd_amd import abi, configs
+    from moil_fnrfd_amd.adjoint import lj_factors, type_gradients
+    from moil_fnrfd_amd.eforce import MoilGPU
+
+    dev = torch.device('cuda:0')
+    sh = torch.cuda.current_stream().cuda_stream
+    s, cut, ew = configs.hetero_10k()
+    m = s.meta
+    e12, e6 = (a.numpy() for a in lj_factors(torch.tensor(m['lj_eps']),
+                                             torch.tensor(m['lj_rmin_half']), m['lj_type']))
+    eng = {}
+    for prec in (abi.PREC_MIXED, abi.PREC_FP64):
+        g = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=prec))
+        g.set_params(s.topo.ptchg, e6, e12, 0.5, 0.5)
+        eng[prec] = g
+    npt = s.topo.npt
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    base = torch.tensor(s.coor, dtype=torch.float64, device=dev)
+    X = base + 0.1 * torch.randn((n_conf, npt, 3), generator=gen, device=dev,
+                                 dtype=torch.float64)
+    out = {p: (torch.empty_like(X), torch.zeros((n_conf, 3, npt), dtype=torch.float64, device=dev),
+               torch.zeros((n_conf, abi.N_ENERGIES), dtype=torch.float64, device=dev))
+           for p in eng}
+
+    def run(prec):
+        g = eng[prec]
+        dp, pg, en = out[prec]
+        for b in range(n_conf):
+            x = X[b]
+            g.squeeze_dev(x.data_ptr(), sh)
+            g.nbondm_dev(x.data_ptr(), sh)
+            g.eforce_dev(x.data_ptr(), dp[b].data_ptr(), en[b].data_ptr(), sh)
+            g.param_grad_dev(x.data_ptr(), pg[b, 0].data_ptr(), pg[b, 1].data_ptr(),
+                             pg[b, 2].data_ptr(), sh)
+
+    run(abi.PREC_MIXED)  # warm-up (and the squeezed coordinates)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(abi.PREC_MIXED)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    run(abi.PREC_FP64)
+    torch.cuda.synchronize()
+
+    def errs(a, b):
+        a, b = a.reshape(len(a), -1), b.reshape(len(b), -1)
+        rms = float(torch.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+        mx = float(((a - b).abs().amax(dim=1) / b.abs().amax(dim=1)).max())
+        return {'rms_rel': rms, 'max_rel': mx}
+
+    d32, p32, _ = out[abi.PREC_MIXED]
+    d64, p64, _ = out[abi.PREC_FP64]
+    t32 = np.stack([np.concatenate(type_gradients(p[1], p[2], e12, e6, m['lj_eps'],
+                                                  m['lj_rmin_half'], m['lj_type']))
+                    for p in p32.cpu().numpy()])
+    t64 = np.stack([np.concatenate(type_gradients(p[1], p[2], e12, e6, m['lj_eps'],
+                                                  m['lj_rmin_half'], m['lj_type']))
+                    for p in p64.cpu().numpy()])
+    rep = {
+        'dE_dx': errs(d32, d64),
+        'dE_dq': errs(p32[:, 0], p64[:, 0]),
+        'dE_deps_dRmin_per_type': errs(torch.tensor(t32), torch.tensor(t64)),
+    }
+    # finite differences of the FP64 energy on the first conformation
+    g = eng[abi.PREC_FP64]
+    x0 = X[0].cpu().numpy()
+    g.nbondm(x0)
+    dq0, de12_0, de6_0 = g.param_grad(x0)
+    geps, grmh = type_gradients(de12_0, de6_0, e12, e6, m['lj_eps'], m['lj_rmin_half'],
+                                m['lj_type'])
+    h = 1e-4
+    q0 = s.topo.ptchg
+    worst = 0.0
+    for i in (0, 100, 1000, 2000):
+        dqv = np.zeros(npt)
+        dqv[i] = h
+        ep, em = [], []
+        for sgn, acc in ((1, ep), (-1, em)):
+            g.set_params(q0 + sgn * dqv, e6, e12, 0.5, 0.5)
+            acc.append(g.eforce(x0)[0]['e_total'])
+        fd = (ep[0] - em[0]) / (2 * h)
+        worst = max(worst, abs(fd - dq0[i]) / max(abs(fd), 1.0))
+    eps0 = torch.tensor(m['lj_eps'])
+    rmh0 = torch.tensor(m['lj_rmin_half'])
+    for t in (0, 7, 15, len(eps0) - 1):
+        for which, ad in ((0, geps[t]), (1, grmh[t])):
+            vals = []
+            for sgn in (1, -1):
+                ee, rr = eps0.clone(), rmh0.clone()
+                (ee if which == 0 else rr)[t] *= 1 + sgn * h
+                a12, a6 = lj_factors(ee, rr, m['lj_type'])
+                g.set_params(q0, a6.numpy(), a12.numpy(), 0.5, 0.5)
+                vals.append(g.eforce(x0)[0]['e_total'])
+            step = 2 * h * float((eps0 if which == 0 else rmh0)[t])
+            fd = (vals[0] - vals[1]) / step
+            worst = max(worst, abs(fd - ad) / max(abs(fd), 1.0))
+    rep['fd_max_rel_fp64'] = worst
+    for g in eng.values():
+        g.close()
+    return {
+        'conformations': n_conf,
+        'atoms': npt,
+        'lj_types': len(m['lj_eps']),
+        'conf_per_s': n_conf / dt,
+        'ms_per_conf': 1e3 * dt / n_conf,
+        'errors_mixed_vs_fp64': rep,
+        'workload': '10,048-atom heteropolymer in water (200 chains, 46.4 A box, 48^3 grid), '
+                    'conformations = base + N(0, 0.1 A); per conformation list build + energy '
+                    '+ dE/dx + dE/dq per atom + dE/deps, dE/dRmin per LJ type (mixed)',
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -196,6 +321,10 @@ def main():
     ap.add_argument('--batched-only', action='store_true',
                     help='run only the batched/plain leg and print its JSON')
     ap.add_argument('--batched-steps', type=int, default=10)
+    ap.add_argument('--no-adjoint', action='store_true',
+                    help='skip the differentiation leg (128 conformations)')
+    ap.add_argument('--adjoint-only', action='store_true',
+                    help='run only the differentiation leg and print its JSON')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB ten times (the reference '
                          'kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)')
@@ -218,6 +347,9 @@ def main():
     torch.cuda.set_device(0)
     if args.batched_only:
         print(json.dumps(batched_leg(steps=args.batched_steps)))
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg()))
         return
     sysm, cut, ew = configs.water32k()
     params = configs.params_for(sysm, cut, ew, precision=abi.PREC_MIXED)
@@ -321,6 +453,8 @@ def main():
     eng.close()
     if not args.no_batched:
         line['batched'] = batched_leg(steps=args.batched_steps)
+    if not args.no_adjoint:
+        line['adjoint'] = adjoint_leg()
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(sysm, cut, ew)
     print(json.dumps(line))

@@ -167,21 +167,45 @@ int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
 
 /* Batched evaluation (no counterpart in MOIL, which evaluates one system
  * per process; the chain/path drivers loop over slices, chain/sds.f:71-140).
- * A batch groups existing contexts (independent systems or path slices of
- * any sizes); moil_gpu_batch_eforce_dev evaluates all of them, d_coor[i],
- * d_dpot[i], d_energies[i] being system i's device arrays as for
- * moil_gpu_eforce_dev.  The step is captured once into a hipGraph over
- * `nstreams` internal streams and replayed; it is recaptured when a
- * system's lists were rebuilt or the pointers change.  Ordered after the
- * caller's earlier work on `stream` (NULL = null stream) and before its
- * later work.  Results are identical to one moil_gpu_eforce_dev per
- * system. */
+ * A batch groups existing contexts of one precision (independent systems
+ * or path slices of any sizes); moil_gpu_batch_eforce_dev evaluates all of
+ * them, d_coor[i], d_dpot[i], d_energies[i] being system i's device arrays
+ * as for moil_gpu_eforce_dev.  Every kernel runs once over all systems
+ * (per-system argument tables and workgroup offsets), the transforms as
+ * one batched plan per grid size; the step is captured once into a
+ * hipGraph and replayed, and recaptured when a system's lists were
+ * rebuilt or the pointers change.  `nstreams` is accepted for
+ * compatibility and unused.  Ordered after the caller's earlier work on
+ * `stream` (NULL = null stream) and before its later work.  Results are
+ * identical to one moil_gpu_eforce_dev per system. */
 typedef struct moil_batch moil_batch;
 int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nstreams, moil_batch **out);
 int32_t moil_gpu_batch_eforce_dev(moil_batch *batch, const double *const *d_coor,
                                   double *const *d_dpot, double *const *d_energies,
                                   void *stream);
 int32_t moil_gpu_batch_destroy(moil_batch *batch);
+
+/* Force-field parameters and the energy's gradient with respect to them
+ * (no counterpart in MOIL; for force-field fitting and the differentiable
+ * binding).  moil_gpu_set_params replaces the per-atom charges and LJ
+ * factors (MOIL's ptchg, epsgm6, epsgm12 of CONNECT.BLOCK) and rebuilds
+ * what MOIL derives from them: e_self (symm/ewald_init.f:131-150), the
+ * last-water parameters (pot/watwat_ewald.f:40-72) and the 1-4 pair
+ * parameters p14 from the scalings v14 and el14 (generic/wconn.f:296-325).
+ * The list classification (flagchr, the hydrogen rule) keeps the values of
+ * moil_gpu_init; no neutrality check.
+ * moil_gpu_param_grad[_dev] returns dE/dq, dE/d(epsgm12) and dE/d(epsgm6)
+ * per atom, (npt) float64 each, of the energy moil_gpu_eforce evaluates on
+ * the current lists (dE/dx is eforce's dpot).  Every pair distance is
+ * recomputed; the reciprocal grid is rebuilt.  Water pairs follow MOIL's
+ * use of the last water's parameters, each water atom receiving its own
+ * share.  Requires moil_gpu_set_params when the system has 1-4 pairs. */
+int32_t moil_gpu_set_params(moil_ctx *ctx, const double *ptchg, const double *epsgm6,
+                            const double *epsgm12, double v14, double el14);
+int32_t moil_gpu_param_grad(moil_ctx *ctx, const double *coor, double *dq, double *de12,
+                            double *de6);
+int32_t moil_gpu_param_grad_dev(moil_ctx *ctx, const double *d_coor, double *d_dq,
+                                double *d_de12, double *d_de6, void *stream);
 
 /* List inspection for parity tests.  counts[0..7]: water pairs real
  * listwt1, real listwt2, image listwt1, image listwt2; atom pairs list1,

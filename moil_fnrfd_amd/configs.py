@@ -10,7 +10,7 @@ bisection (symm/ewald_init.f:298-335).
 from __future__ import annotations
 
 from . import abi
-from .systems import hetero_system, water_box
+from .systems import hetero_packed, hetero_system, water_box
 
 # dtol giving beta = 0.34 at rc = 9 A and beta = 0.31 at rc = 10 A
 DTOL_B034_RC9 = 1.67573104443267911e-06
@@ -68,6 +68,34 @@ def hetero_small(seed=2000, n_chains=12, L=24.0):
     n = int(round(L)) + int(round(L)) % 2
     ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
     return s, cut, ew
+
+
+def hetero_box(n_chains, L, seed):
+    """Heteropolymer in water by the large-system recipe (hetero_packed):
+    n_chains chains of 10-20 atoms in an L-A box at 0.1006 atoms/A^3, 10 A
+    cutoffs with a 1.5 A list skin, explicit monomer cutoff 18 A (>= rebg
+    + twice the largest monomer radius + 1 A), beta 0.31, PME grid at ~1 A
+    spacing."""
+    s = hetero_packed(n_chains, L, seed=seed)
+    cut = dict(rvmx=10.0, relx=10.0, rvbg=11.5, rebg=11.5, cutm=18.0)
+    n = fft_size(L)
+    ew = dict(dtol=DTOL_B031_RC10, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def hetero_1m(seed=2100):
+    """The ~1.0M-atom solvated heteropolymer: 20,000 chains of 10-20 atoms
+    (~300k chain atoms, 24 LJ types, 1-4 pairs scaled 0.5) and ~233k
+    3-site waters in a 215 A box, 10 A cutoffs + 1.5 A skin, 216^3 grid."""
+    s, cut, ew = hetero_box(20000, 215.0, seed)
+    ew['nfft'] = (216, 216, 216)
+    return s, cut, ew
+
+
+def hetero_10k(seed=4000):
+    """The ~10,000-atom system of the same recipe (200 chains, 46.4 A box,
+    48^3 grid)."""
+    return hetero_box(200, 46.4, seed)
 
 
 def params_for(sysm, cut, ew, precision=abi.PREC_FP64, terms=abi.TERM_DEFAULT):

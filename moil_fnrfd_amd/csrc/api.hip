@@ -114,7 +114,8 @@ static void ctx_free(moil_ctx *c) {
                   c->d_tpar, c->d_ii, c->d_ipar, c->d_sp, c->d_p14, c->d_wat_o, c->d_wat_mono,
                   c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos,
                   c->d_wi_boff, c->d_wi_aoff, c->d_wi_eoff, c->d_wi_zero, c->d_wi_bidx,
-                  c->d_wi_aidx, c->d_wi_eidx, c->d_wi_bpar, c->d_wi_apar, c->d_etot, c->d_ticket};
+                  c->d_wi_aidx, c->d_wi_eidx, c->d_wi_bpar, c->d_wi_apar, c->d_etot, c->d_ticket,
+                  c->d_xall, c->d_adj};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -206,6 +207,9 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   TRY(upload(&c->d_monoflag, t->monoflag, (size_t)t->totmon));
   TRY(upload(&c->d_flagchr, t->flagchr, (size_t)t->npt));
   TRY(upload(&c->d_ptchg, t->ptchg, (size_t)t->npt));
+  c->h_q.assign(t->ptchg, t->ptchg + t->npt);
+  c->h_e12.assign(t->epsgm12, t->epsgm12 + t->npt);
+  c->h_e6.assign(t->epsgm6, t->epsgm6 + t->npt);
   TRY(upload(&c->d_eps6, t->epsgm6, (size_t)t->npt));
   TRY(upload(&c->d_eps12, t->epsgm12, (size_t)t->npt));
   // ---- waters (monomers named TIP3/SPC*) ----
@@ -260,6 +264,15 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     for (int i = 1; i <= t->npt - 1; i++)  // pot/cdie_ewald.f:68,83 (atom npt ignored)
       for (int k = t->exc1[i - 1]; k < t->exc1[i]; k++) all.push_back({i - 1, t->exc2[k] - 1});
     c->nexcl = (int)all.size();
+    {
+      std::vector<int32_t> xp(2 * all.size());
+      for (size_t m = 0; m < all.size(); m++) {
+        xp[2 * m] = all[m].first;
+        xp[2 * m + 1] = all[m].second;
+      }
+      c->nxall = (int)all.size();
+      TRY(upload(&c->d_xall, xp.data(), xp.size()));
+    }
     for (size_t m = 0; m < all.size(); m++) {
       int w = same_water({all[m].first, all[m].second});
       if (w >= 0) {
@@ -359,6 +372,7 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       sp[2 * m] = t->spec1[m]; sp[2 * m + 1] = t->spec2[m];
     }
     TRY(upload(&c->d_sp, sp.data(), sp.size()));
+    c->h_sp = sp;
     TRY(upload(&c->d_p14, t->p14, 3 * (size_t)c->totspe));
   }
   if (c->nwat >= 2) {
@@ -884,5 +898,86 @@ extern "C" int32_t moil_gpu_stage_times(moil_ctx *c, double t[8]) {
   c->ev_rec.clear();
   c->ev_next = 0;
   c->n_timed = 0;
+  return MOIL_GPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Force-field parameters and their gradients (no counterpart in MOIL)
+
+extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const double *epsgm6,
+                                       const double *epsgm12, double v14, double el14) {
+  if (!c || !ptchg || !epsgm6 || !epsgm12) return fail(MOIL_GPU_EINVAL, "null argument");
+  const size_t n = (size_t)c->npt;
+  c->h_q.assign(ptchg, ptchg + n);
+  c->h_e6.assign(epsgm6, epsgm6 + n);
+  c->h_e12.assign(epsgm12, epsgm12 + n);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipDeviceSynchronize());
+  HIP_CHECK(hipMemcpy(c->d_ptchg, ptchg, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_eps6, epsgm6, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_eps12, epsgm12, sizeof(double) * n, hipMemcpyHostToDevice));
+  // self energy (symm/ewald_init.f:131-150)
+  {
+    double s = 0;
+    for (size_t i = 0; i < n; i++) s += ptchg[i] * ptchg[i];
+    c->e_self = -(c->beta * std::sqrt(1.0 / (4.0 * std::atan(1.0))) * (KOFDIE / c->par.eps) * s);
+  }
+  // last-water parameters (pot/watwat_ewald.f:40-72)
+  if (c->nwat >= 2) {
+    const int io = c->h_wat_o[c->nwat - 1], ih = io + 2;
+    const double epst = (c->par.terms & MOIL_TERM_EL) ? 1.0 : 0.0;
+    const double pick = (c->par.terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
+    c->oo6 = pick * epsgm6[io] * epsgm6[io];
+    c->oo12 = pick * epsgm12[io] * epsgm12[io];
+    c->qq_oo = epst * KOFDIE / c->par.eps * ptchg[io] * ptchg[io];
+    c->qq_oh = epst * KOFDIE / c->par.eps * ptchg[io] * ptchg[ih];
+    c->qq_hh = epst * KOFDIE / c->par.eps * ptchg[ih] * ptchg[ih];
+  }
+  // 1-4 pair parameters (generic/wconn.f:296-325, geometric rule, hvdw0)
+  if (c->totspe > 0) {
+    std::vector<double> p14(3 * (size_t)c->totspe);
+    for (int m = 0; m < c->totspe; m++) {
+      const int j = c->h_sp[2 * m] - 1, k = c->h_sp[2 * m + 1] - 1;
+      if (epsgm12[j] < 1e-3 || epsgm12[k] < 1e-3) {
+        p14[3 * m] = p14[3 * m + 1] = 0.0;
+      } else {
+        p14[3 * m] = epsgm12[j] * epsgm12[k] * v14;
+        p14[3 * m + 1] = epsgm6[j] * epsgm6[k] * v14;
+      }
+      p14[3 * m + 2] = ptchg[j] * ptchg[k] * c->par.eps * el14 * KOFDIE;
+    }
+    HIP_CHECK(hipMemcpy(c->d_p14, p14.data(), sizeof(double) * p14.size(), hipMemcpyHostToDevice));
+  }
+  c->v14 = v14;
+  c->el14 = el14;
+  c->have_scale14 = true;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_param_grad_dev(moil_ctx *c, const double *d_coor, double *d_dq,
+                                           double *d_de12, double *d_de6, void *stream) {
+  if (!c || !d_coor || !d_dq || !d_de12 || !d_de6) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "param_grad called before build_lists");
+  if (c->totspe > 0 && !c->have_scale14)
+    return fail(MOIL_GPU_ESTATE,
+                "param_grad needs the 1-4 scalings: call moil_gpu_set_params first");
+  return adjoint_run(c, d_coor, d_dq, d_de12, d_de6, pick_stream(c, stream));
+}
+
+extern "C" int32_t moil_gpu_param_grad(moil_ctx *c, const double *coor, double *dq, double *de12,
+                                       double *de6) {
+  if (!c || !coor || !dq || !de12 || !de6) return fail(MOIL_GPU_EINVAL, "null argument");
+  const size_t n = (size_t)c->npt;
+  if (!c->d_adj) HIP_CHECK(hipMalloc(&c->d_adj, sizeof(double) * 3 * n));
+  HIP_CHECK(hipMemcpyAsync(c->d_coor, coor, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  CHECK_RC(moil_gpu_param_grad_dev(c, c->d_coor, c->d_adj, c->d_adj + n, c->d_adj + 2 * n,
+                                   c->stream));
+  HIP_CHECK(hipMemcpyAsync(dq, c->d_adj, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(de12, c->d_adj + n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_CHECK(hipMemcpyAsync(de6, c->d_adj + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return MOIL_GPU_OK;
 }

@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "multi.h"
+#include "pairmath.h"
 #include "grid.h"
 
 #define ABT 256
@@ -396,28 +397,6 @@ __global__ void k_ajfill(const AItem *__restrict__ items, size_t n, const int32_
 // ---------------------------------------------------------------------------
 // real-space energy over the atom items
 
-#define ERFC_P 0.418f
-__device__ __forceinline__ float a_erfc_poly(float t) {
-  float p = 7.663338518e-02f;
-  p = fmaf(p, t, -3.531459250e-01f);
-  p = fmaf(p, t, 5.087678340e-01f);
-  p = fmaf(p, t, -1.952929589e-01f);
-  p = fmaf(p, t, 3.047273108e-01f);
-  p = fmaf(p, t, 1.823138229e-01f);
-  p = fmaf(p, t, 2.404413357e-01f);
-  p = fmaf(p, t, 2.355552142e-01f);
-  return p * t;
-}
-
-__device__ __forceinline__ long long a_fix32(float f) {
-  // floor(f 2^32) from FP32 operations: high word floor(f), low word
-  // fract(f) 2^32 (valid for |f| < 2^31)
-  const float fl = floorf(f);
-  const unsigned hi = (unsigned)(int)fl;
-  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
-
 __device__ inline double a_block_sum(double v, double *red) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -425,10 +404,6 @@ __device__ inline double a_block_sum(double v, double *red) {
   if (lane == 0) red[wv] = v;
   __syncthreads();
   return (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-__device__ __forceinline__ uint64_t a_rotr64(uint64_t x, int r) {
-  return r ? ((x >> r) | (x << (64 - r))) : x;
 }
 
 template <bool F64>
@@ -485,7 +460,7 @@ __device__ __forceinline__ void atoms_impl(
     double fj[3] = {0, 0, 0};
     double ev = 0, ee = 0;
     const uint64_t m2 = item.m2, m3 = item.m3;
-    uint64_t mask = a_rotr64(item.m1 | m2 | m3, rot);
+    uint64_t mask = rotr64(item.m1 | m2 | m3, rot);
     while (mask) {
       const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
       mask &= mask - 1;
@@ -539,7 +514,7 @@ __device__ __forceinline__ void atoms_impl(
           const float x = betaf * (r2 * sv);
           const float tt = __builtin_amdgcn_rcpf(fmaf(ERFC_P, x, 1.0f));
           const float ex = __builtin_amdgcn_exp2f(-(x * x) * log2e);
-          const float e = qqf * (ex * a_erfc_poly(tt)) * sv;
+          const float e = qqf * (ex * erfc_poly(tt)) * sv;
           e_e = (double)e;
           dff = -s2 * (e + qqf * c2b * ex);
         }
@@ -559,14 +534,14 @@ __device__ __forceinline__ void atoms_impl(
       for (int k = 0; k < 3; k++) {
         const double g = df * d[k];
         fj[k] -= g;
-        const long long v = F64 ? __double2ll_rn(g * P.fix_scale) : a_fix32((float)g);
+        const long long v = F64 ? __double2ll_rn(g * P.fix_scale) : fix32((float)g);
         if (v) atomicAdd(&iacc[i * 3 + k], (unsigned long long)v);
       }
     }
     long long *jrow = jpart + (size_t)jpos[it] * 3;
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      jrow[k] = F64 ? __double2ll_rn(fj[k] * P.fix_scale) : a_fix32((float)fj[k]);
+      jrow[k] = F64 ? __double2ll_rn(fj[k] * P.fix_scale) : fix32((float)fj[k]);
     if (s == 0) {
       ev_r += ev;
       ee_r += ee;

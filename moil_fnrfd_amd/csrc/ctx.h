@@ -150,6 +150,15 @@ struct moil_ctx {
   bool have_lists = false;
   long list_gen = 0;               // incremented by every list build
 
+  // ---- parameter gradients (adjoint.hip) ----
+  std::vector<double> h_q, h_e12, h_e6;  // current per-atom charges / LJ factors
+  std::vector<int32_t> h_sp;              // 1-4 pairs (1-based), [2 totspe]
+  int32_t *d_xall = nullptr;              // every exclusion pair (0-based), [2 nxall]
+  int nxall = 0;
+  double v14 = 0, el14 = 0;               // 1-4 scalings, known after set_params
+  bool have_scale14 = false;
+  double *d_adj = nullptr;                // [3 npt] host-call outputs dq, de12, de6
+
   // ---- timing (event pairs per stage, summed on request) ----
   int timing = 0;                  // 0 off, 1 real-space kernels, 2 all stages
   long n_timed = 0;
@@ -169,4 +178,7 @@ void atoms_free(moil_ctx *c);
 // PME set-up (pme.hip)
 int32_t pme_init(moil_ctx *c);
 void pme_free(moil_ctx *c);
+// parameter gradients (adjoint.hip): dq, de12, de6 per atom
+int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_de12,
+                    double *d_de6, hipStream_t st);
 // The launches of an evaluation are described by plans (kargs.h).

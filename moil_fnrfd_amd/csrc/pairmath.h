@@ -1,0 +1,51 @@
+// FP32 pair arithmetic shared by the real-space kernels and their adjoint.
+//
+// erfc(x) = exp(-x^2) t P(t), t = 1/(1+p x): the functional form of
+// Abramowitz & Stegun 7.1.26 with a degree-8 polynomial fitted for this
+// library (relative error 3e-8 on [0, 6]).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ERFC_P 0.418f
+__device__ __forceinline__ float erfc_poly(float t) {
+  float p = 7.663338518e-02f;
+  p = fmaf(p, t, -3.531459250e-01f);
+  p = fmaf(p, t, 5.087678340e-01f);
+  p = fmaf(p, t, -1.952929589e-01f);
+  p = fmaf(p, t, 3.047273108e-01f);
+  p = fmaf(p, t, 1.823138229e-01f);
+  p = fmaf(p, t, 2.404413357e-01f);
+  p = fmaf(p, t, 2.355552142e-01f);
+  return p * t;
+}
+
+// floor(f * 2^32) as a 64-bit integer from FP32 operations only: the high
+// word is floor(f), the low word fract(f) 2^32 (f - floor(f) is exact and
+// below 1, its product with 2^32 truncates to the floor).  The two 32-bit
+// halves are the int64 directly, no shifts.  Valid for |f| < 2^31.
+__device__ __forceinline__ long long fix32(float f) {
+  const float fl = floorf(f);
+  const unsigned hi = (unsigned)(int)fl;
+  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Each lane walks its block mask starting at its own lane number (bit b of
+// the rotated mask is block member (b + rot) mod 64), so lanes with similar
+// masks do not all add into the same LDS accumulators at once.
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
+  return r ? ((x >> r) | (x << (64 - r))) : x;
+}
+
+// MOIL's cubic-spline erfc table (pot/cdie_ewald.f:152-160): erfc and its
+// derivative at x = beta r
+__device__ __forceinline__ void erfc_table(const double *__restrict__ erf_arr, double erftbdns,
+                                           double x, double &erfcc, double &derfc) {
+  const int idx = (int)(erftbdns * x);
+  const double dx = x - idx * (1.0 / erftbdns);
+  const double *tb = erf_arr + 4 * idx;
+  derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
+  erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * (1.0 / 3.0))) + tb[0];
+}

@@ -638,3 +638,74 @@ int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
+
+// reciprocal potential for the charge gradient: one workgroup per tile
+// (as the gather), dE/dq_n = fc sum_p theta1 theta2 theta3 psi(p) with psi
+// the convolved grid after the inverse transform (the same grid whose
+// derivative-weighted sum the gather turns into forces); added to dq
+template <typename R>
+__global__ __launch_bounds__(PME_THREADS) void k_recip_pot(
+    const double *__restrict__ coor, const int32_t *__restrict__ tstart,
+    const int32_t *__restrict__ sidx, PmePar P, TilePar T, const R *__restrict__ grid,
+    double *__restrict__ dq) {
+  __shared__ R reg[11 * 11 * 11];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
+  const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
+            lo3 = tile_lo(tt3, T.n3, T.nt3);
+  for (int u = t; u < T.rsz; u += PME_THREADS) {
+    const int l1 = u % T.r1, l2 = (u / T.r1) % T.r2, l3 = u / (T.r1 * T.r2);
+    int g1 = lo1 - 3 + l1, g2 = lo2 - 3 + l2, g3 = lo3 - 3 + l3;
+    g1 = g1 < 0 ? g1 + T.n1 : (g1 >= T.n1 ? g1 - T.n1 : g1);
+    g2 = g2 < 0 ? g2 + T.n2 : (g2 >= T.n2 ? g2 - T.n2 : g2);
+    g3 = g3 < 0 ? g3 + T.n3 : (g3 >= T.n3 ? g3 - T.n3 : g3);
+    reg[u] = grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
+  }
+  __syncthreads();
+  const int beg = tstart[tile], end = tstart[tile + 1];
+  for (int p = beg + t; p < end; p += PME_THREADS) {
+    const int n = sidx[p];
+    int c1, c2, c3;
+    double w1, w2, w3;
+    base_point(coor + 3 * n, P, c1, c2, c3, w1, w2, w3);
+    R t1[4], t2[4], t3[4], dd[4];
+    bspline4<R>((R)w1, t1, dd);
+    bspline4<R>((R)w2, t2, dd);
+    bspline4<R>((R)w3, t3, dd);
+    const int b1 = c1 - lo1, b2 = c2 - lo2, b3 = c3 - lo3;
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const R *row = reg + ((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1;
+        R g = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) g += row[a] * t1[a];
+        s += (double)(g * t2[b] * t3[c]);
+      }
+    }
+    dq[n] += P.fc * s;
+  }
+}
+
+int32_t pme_potential(moil_ctx *c, const EvalPlan &p, double *d_dq, hipStream_t st) {
+  pme_launch_spread(p, st);
+  void *in[1] = {c->d_qreal}, *out[1] = {c->d_qcplx};
+  RF_CHECK(rocfft_execution_info_set_stream(c->exec_fwd, st));
+  RF_CHECK(rocfft_execute(c->plan_fwd, in, out, c->exec_fwd));
+  if (p.f64) launch_k_conv_f64(p.conv, p.n_conv, st);
+  else launch_k_conv_f32(p.conv, p.n_conv, st);
+  void *in2[1] = {c->d_qcplx}, *out2[1] = {c->d_qreal};
+  RF_CHECK(rocfft_execution_info_set_stream(c->exec_bwd, st));
+  RF_CHECK(rocfft_execute(c->plan_bwd, in2, out2, c->exec_bwd));
+  const GatherArgs &g = p.gather;
+  if (p.f64)
+    k_recip_pot<double><<<p.n_tile, PME_THREADS, 0, st>>>(g.coor, g.tstart, g.sidx, g.P, g.T,
+                                                           (const double *)g.grid, d_dq);
+  else
+    k_recip_pot<float><<<p.n_tile, PME_THREADS, 0, st>>>(g.coor, g.tstart, g.sidx, g.P, g.T,
+                                                          (const float *)g.grid, d_dq);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}

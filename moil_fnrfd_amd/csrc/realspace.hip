@@ -34,43 +34,12 @@
 #include <hip/hip_runtime.h>
 
 #include "multi.h"
+#include "pairmath.h"
 
 #define RS_THREADS 256
 
-// erfc fit coefficients (t = 1/(1 + ERFC_P x)); see file header
-#define ERFC_P 0.418f
-__device__ __forceinline__ float erfc_poly(float t) {
-  float p = 7.663338518e-02f;
-  p = fmaf(p, t, -3.531459250e-01f);
-  p = fmaf(p, t, 5.087678340e-01f);
-  p = fmaf(p, t, -1.952929589e-01f);
-  p = fmaf(p, t, 3.047273108e-01f);
-  p = fmaf(p, t, 1.823138229e-01f);
-  p = fmaf(p, t, 2.404413357e-01f);
-  p = fmaf(p, t, 2.355552142e-01f);
-  return p * t;
-}
-
 __device__ __forceinline__ long long fix64(double v, double scale) {
   return __double2ll_rn(v * scale);
-}
-
-// floor(f * 2^32) as a 64-bit integer from FP32 operations only: the high
-// word is floor(f), the low word fract(f) 2^32 (f - floor(f) is exact and
-// below 1, its product with 2^32 truncates to the floor).  The two 32-bit
-// halves are the int64 directly, no shifts.  Valid for |f| < 2^31.
-__device__ __forceinline__ long long fix32(float f) {
-  const float fl = floorf(f);
-  const unsigned hi = (unsigned)(int)fl;
-  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
-
-// Each lane walks its block mask starting at its own lane number (bit b of
-// the rotated mask is block water (b + rot) mod 64), so lanes with similar
-// masks do not all add into the same i-water's LDS accumulators at once.
-__device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
-  return r ? ((x >> r) | (x << (64 - r))) : x;
 }
 
 // ---------------------------------------------------------------------------

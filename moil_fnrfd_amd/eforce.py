@@ -65,12 +65,16 @@ def load_library():
     L.moil_gpu_batch_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_batch_destroy.argtypes = [vp]
     L.moil_gpu_stage_times.argtypes = [vp, vp]
+    L.moil_gpu_set_params.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
+    L.moil_gpu_param_grad.argtypes = [vp, vp, vp, vp, vp]
+    L.moil_gpu_param_grad_dev.argtypes = [vp, vp, vp, vp, vp, vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
                  'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_batch_create',
-                 'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy'):
+                 'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
+                 'moil_gpu_param_grad', 'moil_gpu_param_grad_dev'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -142,6 +146,34 @@ class MoilGPU:
     def eforce_dev(self, d_coor: int, d_dpot: int, d_energies: int, stream: Optional[int] = None):
         _check(load_library().moil_gpu_eforce_dev(self._h, C.c_void_p(d_coor), C.c_void_p(d_dpot),
                                                   C.c_void_p(d_energies), C.c_void_p(stream or 0)))
+
+    # -- force-field parameters and their gradients ----------------------
+    def set_params(self, ptchg, epsgm6, epsgm12, v14=0.5, el14=0.5):
+        """Per-atom charges and LJ factors (MOIL's ptchg, epsgm6, epsgm12)
+        plus the 1-4 scalings; rebuilds e_self, the last-water and the 1-4
+        pair parameters."""
+        q = np.ascontiguousarray(ptchg, np.float64)
+        a6 = np.ascontiguousarray(epsgm6, np.float64)
+        a12 = np.ascontiguousarray(epsgm12, np.float64)
+        if not (q.size == a6.size == a12.size == self.npt):
+            raise ValueError('expected %d values per parameter' % self.npt)
+        _check(load_library().moil_gpu_set_params(self._h, q.ctypes.data, a6.ctypes.data,
+                                                  a12.ctypes.data, float(v14), float(el14)))
+
+    def param_grad(self, coor):
+        """(dE/dq, dE/d epsgm12, dE/d epsgm6), each (npt,) float64, on the
+        current lists."""
+        x = _f64(coor, self.npt)
+        out = np.zeros((3, self.npt))
+        _check(load_library().moil_gpu_param_grad(self._h, x.ctypes.data, out[0].ctypes.data,
+                                                  out[1].ctypes.data, out[2].ctypes.data))
+        return out[0], out[1], out[2]
+
+    def param_grad_dev(self, d_coor: int, d_dq: int, d_de12: int, d_de6: int,
+                       stream: Optional[int] = None):
+        _check(load_library().moil_gpu_param_grad_dev(
+            self._h, C.c_void_p(d_coor), C.c_void_p(d_dq), C.c_void_p(d_de12), C.c_void_p(d_de6),
+            C.c_void_p(stream or 0)))
 
     # -- inspection -------------------------------------------------------
     def list_counts(self):

@@ -154,13 +154,16 @@ class _TopoBuilder:
         self.bonds, self.angles, self.tors, self.imps, self.spec = [], [], [], [], []
         self.excl = {}
         self.idxtip3 = []
+        self.atype = []  # LJ type per atom (-1: no LJ; WATER_O_TYPE: water O)
 
     @property
     def npt(self):
         return len(self.ptnm)
 
-    def add_monomer(self, name, atoms):
-        """atoms: list of (name, mass, charge, eps, rmin_half)."""
+    def add_monomer(self, name, atoms, types=None):
+        """atoms: list of (name, mass, charge, eps, rmin_half); types: the
+        atoms' LJ type numbers (default -1)."""
+        self.atype.extend(types if types is not None else [-1] * len(atoms))
         for (nm, m, q, eps, rh) in atoms:
             e12, e6 = lj_epsgm(eps, rh)
             self.ptnm.append(nm)
@@ -267,11 +270,14 @@ def _random_rotations(rng, n):
     return R
 
 
+WATER_O_TYPE = -2  # LJ type marker of the water O (one shared type)
+
+
 def _add_water(tb: _TopoBuilder):
     base = tb.npt + 1
     tb.add_monomer('TIP3', [('OH2', 15.999, WAT_QO, WAT_EPS_O, WAT_RMIN2_O),
                             ('H1', 1.008, WAT_QH, 0.0, 0.0),
-                            ('H2', 1.008, WAT_QH, 0.0, 0.0)])
+                            ('H2', 1.008, WAT_QH, 0.0, 0.0)], [WATER_O_TYPE, -1, -1])
     o, h1, h2 = base, base + 1, base + 2
     tb.bond(o, h1, 450.0, WAT_ROH)
     tb.bond(o, h2, 450.0, WAT_ROH)
@@ -410,6 +416,60 @@ def _chain_contacts_ok(xyz, hashed, L, dmin):
     return True
 
 
+def _chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types, mono_size,
+                    frac_neutral, frac_hlike):
+    """Atoms, monomers and bonded terms of one chain at coordinates xyz
+    (chain number c): charges N(0,0.4) shifted to net zero (quantised
+    exactly to zero per chain), LJ types, bonds/angles at their current
+    geometry, torsions, impropers and the 1-2/1-3/1-4 exclusions.
+    Returns the chain's 1-based atom numbers."""
+    n = len(xyz)
+    q = rng_par.normal(0.0, 0.4, n)
+    neutral = rng_par.random(n) < frac_neutral
+    neutral[-1] = False  # the last atom absorbs the quantisation residual
+    q[neutral] = 0.0
+    q[~neutral] -= q[~neutral].mean()
+    q = np.array([_q('9.5f', v) for v in q])
+    # make the quantised charges sum to exactly zero per chain
+    resid = round(float(np.sum(q)) * 1e5)
+    q[-1] = _q('9.5f', q[-1] - resid * 1e-5)
+    typ = rng_par.integers(0, n_types, n)
+    hlike = rng_par.random(n) < frac_hlike
+    base = tb.npt + 1
+    k = 0
+    while k < n:
+        m = min(mono_size, n - k)
+        atoms = [('C%d' % ((k + t) % 10), 12.011, q[k + t],
+                  0.0 if hlike[k + t] else types_eps[typ[k + t]],
+                  types_rmh[typ[k + t]]) for t in range(m)]
+        tb.add_monomer('PLY%d' % (c % 10) if m == mono_size else 'PLE%d' % (c % 10), atoms,
+                       [-1 if hlike[k + t] else int(typ[k + t]) for t in range(m)])
+        k += m
+    idx = np.arange(base, base + n)
+    for a in range(n - 1):
+        tb.bond(idx[a], idx[a + 1], rng_par.uniform(300, 500),
+                float(np.linalg.norm(xyz[a + 1] - xyz[a])))
+        tb.exclude(idx[a], idx[a + 1])
+    for a in range(n - 2):
+        v1 = xyz[a] - xyz[a + 1]
+        v2 = xyz[a + 2] - xyz[a + 1]
+        th = math.degrees(math.acos(np.dot(v1, v2) / np.linalg.norm(v1) / np.linalg.norm(v2)))
+        tb.angle(idx[a], idx[a + 1], idx[a + 2], rng_par.uniform(50, 100), th)
+        tb.exclude(idx[a], idx[a + 2])
+    for a in range(n - 3):
+        per = int(rng_par.integers(1, 4))
+        kk = [0.0, 0.0, 0.0]
+        kk[per - 1] = rng_par.uniform(0.0, 2.0)
+        ph = [1.0, 1.0, 1.0]
+        ph[per - 1] = float(rng_par.choice([-1.0, 1.0]))
+        tb.torsion(idx[a], idx[a + 1], idx[a + 2], idx[a + 3], per, kk[0], kk[1], kk[2],
+                   ph[0], ph[1], ph[2])
+        tb.exclude(idx[a], idx[a + 3])
+        tb.improper(idx[a], idx[a + 1], idx[a + 2], idx[a + 3],
+                    rng_par.uniform(5.0, 20.0), float(rng_par.choice([0.0, 180.0])))
+    return idx
+
+
 def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = None,
                   chain_len=(10, 20), n_types: int = 24, mono_size: int = 4,
                   v14: float = 0.5, el14: float = 0.5,
@@ -454,48 +514,8 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
             raise RuntimeError('hetero_system: no contact-free placement for chain %d' % c)
         for x in xyz:
             hashed.add(x)
-        q = rng_par.normal(0.0, 0.4, n)
-        neutral = rng_par.random(n) < frac_neutral
-        neutral[-1] = False  # the last atom absorbs the quantisation residual
-        q[neutral] = 0.0
-        q[~neutral] -= q[~neutral].mean()
-        q = np.array([_q('9.5f', v) for v in q])
-        # make the quantised charges sum to exactly zero per chain
-        resid = round(float(np.sum(q)) * 1e5)
-        q[-1] = _q('9.5f', q[-1] - resid * 1e-5)
-        typ = rng_par.integers(0, n_types, n)
-        hlike = rng_par.random(n) < frac_hlike
-        base = tb.npt + 1
-        k = 0
-        while k < n:
-            m = min(mono_size, n - k)
-            atoms = [('C%d' % ((k + t) % 10), 12.011, q[k + t],
-                      0.0 if hlike[k + t] else types_eps[typ[k + t]],
-                      types_rmh[typ[k + t]]) for t in range(m)]
-            tb.add_monomer('PLY%d' % (c % 10) if m == mono_size else 'PLE%d' % (c % 10), atoms)
-            k += m
-        idx = np.arange(base, base + n)
-        for a in range(n - 1):
-            tb.bond(idx[a], idx[a + 1], rng_par.uniform(300, 500),
-                    float(np.linalg.norm(xyz[a + 1] - xyz[a])))
-            tb.exclude(idx[a], idx[a + 1])
-        for a in range(n - 2):
-            v1 = xyz[a] - xyz[a + 1]
-            v2 = xyz[a + 2] - xyz[a + 1]
-            th = math.degrees(math.acos(np.dot(v1, v2) / np.linalg.norm(v1) / np.linalg.norm(v2)))
-            tb.angle(idx[a], idx[a + 1], idx[a + 2], rng_par.uniform(50, 100), th)
-            tb.exclude(idx[a], idx[a + 2])
-        for a in range(n - 3):
-            per = int(rng_par.integers(1, 4))
-            kk = [0.0, 0.0, 0.0]
-            kk[per - 1] = rng_par.uniform(0.0, 2.0)
-            ph = [1.0, 1.0, 1.0]
-            ph[per - 1] = float(rng_par.choice([-1.0, 1.0]))
-            tb.torsion(idx[a], idx[a + 1], idx[a + 2], idx[a + 3], per, kk[0], kk[1], kk[2],
-                       ph[0], ph[1], ph[2])
-            tb.exclude(idx[a], idx[a + 3])
-            tb.improper(idx[a], idx[a + 1], idx[a + 2], idx[a + 3],
-                        rng_par.uniform(5.0, 20.0), float(rng_par.choice([0.0, 180.0])))
+        idx = _chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types, mono_size,
+                              frac_neutral, frac_hlike)
         chain_xyz.append(xyz)
         chain_atoms.append(idx)
     # 1-4 specials (scaled) for torsion end atoms
@@ -534,6 +554,206 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
     coor = np.concatenate([poly, wxyz]) if len(poly) else wxyz
     return System(topo, coor, (L, L, L),
                   dict(kind='hetero', n_chains=n_chains, L=L, seed=seed, n_water=len(cen)))
+
+
+def _wrap01(x, L):
+    """Coordinates of the centred box mapped into [0, L) (periodic)."""
+    w = np.mod(x + 0.5 * L, L)
+    return np.where(w >= L, w - L, w)
+
+
+def _unit_rows(rng, shape):
+    v = rng.standard_normal(shape + (3,))
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _grow_chains_packed(rng, lengths, L, dmin, ncand=16, retries=12, groups=8):
+    """All chains grown together, one atom per chain per step, by the
+    random-walk rule of _chain_step (bond U(1.0,1.5) A, angle U(100,120)
+    deg, uniform torsion).  Each step draws ncand candidates per chain and
+    keeps the first one at least dmin (minimum image) from every atom
+    already placed except the chain's own last three; chains with no such
+    candidate draw new sets (up to `retries` times) and finally keep the
+    candidate farthest from them.  The atoms of one step are placed in
+    `groups` random groups, each checked against the atoms of the earlier
+    groups too; inside a group, the later chain of a pair closer than dmin
+    moves to another qualifying candidate.  Returns a list of (n_i, 3)
+    arrays (chains continuous, not wrapped)."""
+    from scipy.spatial import cKDTree
+    lengths = np.asarray(lengths)
+    nc, nmax = len(lengths), int(lengths.max())
+    X = np.zeros((nc, nmax, 3))
+    P = np.zeros((0, 3))                 # placed atoms, wrapped
+    owner = np.zeros(0, np.int64)        # chain of each placed atom
+    pos = np.zeros(0, np.int64)          # its position along the chain
+
+    def candidates(ch, k):
+        n = len(ch)
+        if k == 0:
+            return rng.uniform(-0.5 * L, 0.5 * L, (n, ncand, 3))
+        b = rng.uniform(1.0, 1.5, (n, ncand))
+        if k == 1:
+            d = _unit_rows(rng, (n, ncand))
+        else:
+            th = np.radians(rng.uniform(100.0, 120.0, (n, ncand)))
+            phi = rng.uniform(-math.pi, math.pi, (n, ncand))
+            p1, p2 = X[ch, k - 2], X[ch, k - 1]
+            u = p2 - p1
+            u /= np.linalg.norm(u, axis=1, keepdims=True)
+            ref = X[ch, k - 3] - p1 if k >= 3 else _unit_rows(rng, (n,))
+            n1 = np.cross(u, ref)
+            nn = np.linalg.norm(n1, axis=1)
+            bad = nn < 1e-8
+            if bad.any():
+                n1[bad] = np.cross(u[bad], _unit_rows(rng, (int(bad.sum()),)))
+                nn[bad] = np.linalg.norm(n1[bad], axis=1)
+            n1 /= nn[:, None]
+            n2 = np.cross(n1, u)
+            d = (-np.cos(th)[..., None] * u[:, None, :] + np.sin(th)[..., None] *
+                 (np.cos(phi)[..., None] * n2[:, None, :] + np.sin(phi)[..., None] *
+                  n1[:, None, :]))
+        return X[ch, k - 1][:, None, :] + b[..., None] * d
+
+    def nearest(tree, ch, cand, k, own_check):
+        n = len(ch)
+        if tree is None:
+            return np.full((n, ncand), np.inf)
+        npts = tree.n
+        dist, idx = tree.query(_wrap01(cand.reshape(-1, 3), L), k=4, distance_upper_bound=dmin)
+        dist = dist.reshape(n, ncand, 4)
+        idx = idx.reshape(n, ncand, 4)
+        hit = idx < npts
+        if own_check:
+            safe_idx = np.where(hit, idx, 0)
+            own = hit & (owner[safe_idx] == ch[:, None, None]) & (pos[safe_idx] >= k - 3)
+            hit = hit & ~own
+        return np.where(hit, dist, np.inf).min(axis=2)
+
+    for k in range(nmax):
+        act = np.nonzero(lengths > k)[0]
+        tree = cKDTree(P, boxsize=L) if len(P) else None
+        # the step's atoms are placed in groups; each group is checked
+        # against the atoms placed before the step and by earlier groups
+        perm = rng.permutation(len(act))
+        step_pts = np.zeros((0, 3))
+        for grp in np.array_split(perm, min(groups, len(act))):
+            ch = act[np.sort(grp)]
+            na = len(ch)
+
+            def near_of(chs, cc):
+                nr = nearest(tree, chs, cc, k, True)
+                if len(step_pts):
+                    nr = np.minimum(nr, nearest(cKDTree(step_pts, boxsize=L), chs, cc, k, False))
+                return nr
+            cand = candidates(ch, k)
+            near = near_of(ch, cand)
+            for _ in range(retries):
+                stuck = np.nonzero(~(near >= dmin).any(axis=1))[0]
+                if len(stuck) == 0:
+                    break
+                c2 = candidates(ch[stuck], k)
+                n2 = near_of(ch[stuck], c2)
+                better = n2.max(axis=1) > near[stuck].max(axis=1)
+                cand[stuck[better]] = c2[better]
+                near[stuck[better]] = n2[better]
+            okc = near >= dmin
+            choice = np.where(okc.any(axis=1), okc.argmax(axis=1), near.argmax(axis=1))
+            new = cand[np.arange(na), choice]
+            # conflicts inside the group: move the later chain of a pair to
+            # another qualifying candidate
+            tried = np.zeros((na, ncand), bool)
+            tried[np.arange(na), choice] = True
+            for _ in range(8):
+                pairs = cKDTree(_wrap01(new, L), boxsize=L).query_pairs(dmin, output_type='ndarray')
+                if len(pairs) == 0:
+                    break
+                for m in np.unique(pairs.max(axis=1)):
+                    alt = np.nonzero(okc[m] & ~tried[m])[0]
+                    if len(alt) == 0:
+                        continue
+                    tried[m, alt[0]] = True
+                    choice[m] = alt[0]
+                    new[m] = cand[m, alt[0]]
+            X[ch, k] = new
+            step_pts = np.concatenate([step_pts, _wrap01(new, L)])
+            P = np.concatenate([P, _wrap01(new, L)])
+            owner = np.concatenate([owner, ch])
+            pos = np.concatenate([pos, np.full(na, k)])
+    out = []
+    for c in range(nc):
+        xyz = X[c, :lengths[c]].copy()
+        # whole-box shifts only (distances unchanged): centroid in the box
+        cen = xyz.mean(axis=0)
+        xyz -= L * np.round(cen / L)
+        out.append(xyz)
+    return out
+
+
+def _lj_type_table(tb, types_eps, types_rmh):
+    """LJ types of a heteropolymer system: the chain types, then the water
+    O as one more type; per atom its type number (-1: no LJ)."""
+    nt = len(types_eps)
+    at = np.asarray(tb.atype, np.int64)
+    at = np.where(at == WATER_O_TYPE, nt, at)
+    eps = np.concatenate([np.asarray(types_eps, float), [WAT_EPS_O]])
+    rmh = np.concatenate([np.asarray(types_rmh, float), [WAT_RMIN2_O]])
+    return dict(lj_type=at, lj_eps=eps, lj_rmin_half=rmh)
+
+
+def hetero_packed(n_chains: int, L: float, seed: int, n_atoms: Optional[int] = None,
+                  chain_len=(10, 20), n_types: int = 24, mono_size: int = 4,
+                  v14: float = 0.5, el14: float = 0.5, chain_min_dist: float = 2.0,
+                  water_min_dist: float = 2.2) -> System:
+    """The heteropolymer-in-water recipe at scale (hundreds of thousands of
+    chain atoms): n_chains random-walk chains of U{10..20} atoms with the
+    same topology rules as hetero_system, grown together by
+    _grow_chains_packed with chain_min_dist between non-neighbouring chain
+    atoms; the rest of the box filled with 3-site waters on a jittered
+    lattice whose spacing is chosen so that enough sites lie at least
+    water_min_dist from every chain atom (minimum image) to reach n_atoms
+    atoms in total (default: 0.1006 atoms/A^3)."""
+    from scipy.spatial import cKDTree
+    ss = np.random.SeedSequence(seed)
+    rng_walk, rng_par, rng_wat, rng_rot = [np.random.default_rng(s) for s in ss.spawn(4)]
+    types_eps = rng_par.uniform(0.02, 0.2, n_types)
+    types_rmh = rng_par.uniform(1.0, 2.1, n_types)
+    lengths = rng_walk.integers(chain_len[0], chain_len[1] + 1, n_chains)
+    chains = _grow_chains_packed(rng_walk, lengths, L, chain_min_dist)
+    tb = _TopoBuilder()
+    chain_atoms = []
+    for c, xyz in enumerate(chains):
+        chain_atoms.append(_chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types,
+                                           mono_size, 0.0, 0.0))
+    for idx in chain_atoms:
+        for a in range(len(idx) - 3):
+            tb.special14(int(idx[a]), int(idx[a + 3]), v14, el14)
+    poly = np.concatenate(chains)
+    if n_atoms is None:
+        n_atoms = int(round(0.1006 * L ** 3))
+    nw = max(0, (n_atoms - len(poly)) // 3)
+    tree = cKDTree(_wrap01(poly, L), boxsize=L)
+    n_side = max(2, int(math.ceil((nw * 1.2) ** (1.0 / 3.0))))
+    while True:
+        sp = L / n_side
+        g = (np.arange(n_side) + 0.5) * sp - 0.5 * L
+        X, Y, Z = np.meshgrid(g, g, g, indexing='ij')
+        cen = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        cen = cen + rng_wat.uniform(-0.2, 0.2, cen.shape)
+        dist, _ = tree.query(_wrap01(cen, L), k=1, distance_upper_bound=water_min_dist)
+        free = cen[~np.isfinite(dist)]
+        if len(free) >= nw:
+            break
+        n_side += max(1, n_side // 20)
+    order = rng_wat.permutation(len(free))[:nw]
+    cen = free[np.sort(order)]
+    wxyz = water_positions(cen, rng_rot)
+    for _ in range(len(cen)):
+        _add_water(tb)
+    topo = tb.build()
+    coor = np.concatenate([poly, wxyz])
+    return System(topo, coor, (L, L, L),
+                  dict(kind='hetero_packed', n_chains=n_chains, L=L, seed=seed, n_water=len(cen),
+                       water_spacing=sp, **_lj_type_table(tb, types_eps, types_rmh)))
 
 
 # ---------------------------------------------------------------------------
