@@ -185,6 +185,109 @@ def batched_leg(steps=10, nstreams=8):
     }
 
 
+def timed_loop(sysm, cut, ew, steps, warmup, seed):
+    """The metric's loop on one system (mixed precision, coordinates and
+    outputs resident in HBM): per step N(0, 0.01 A) noise added to the
+    coordinates (drawn on the device NLIST steps at a time), every NLIST-th
+    step a list rebuild (squeeze + build_lists), then eforce.  Times `steps`
+    steps after `warmup` untimed ones with the real-space kernel's own
+    dispatch events (no packets between kernels), then a short untimed pass
+    with events between the stages for the breakdown."""
+    import numpy as np
+    import torch
+
+    from moil_fnrfd_amd import abi, configs
+    from moil_fnrfd_amd.eforce import MoilGPU
+
+    params = configs.params_for(sysm, cut, ew, precision=abi.PREC_MIXED)
+    eng = MoilGPU(sysm.topo, params)
+    dev = torch.device('cuda:0')
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    coor = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
+    dpot = torch.empty_like(coor)
+    energ = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    noise = torch.empty((NLIST,) + tuple(coor.shape), dtype=torch.float64, device=dev)
+    ev_list = []
+
+    def step(k):
+        if k % NLIST == 0:
+            torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float64, out=noise)
+        if k > 0:
+            coor.add_(noise[k % NLIST], alpha=SIGMA)
+        if k % NLIST == 0:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            eng.squeeze_dev(coor.data_ptr(), sh)
+            eng.nbondm_dev(coor.data_ptr(), sh)
+            e1.record(stream)
+            ev_list.append((e0, e1))
+        eng.eforce_dev(coor.data_ptr(), dpot.data_ptr(), energ.data_ptr(), sh)
+
+    for k in range(warmup):
+        step(k)
+    torch.cuda.synchronize()
+    n_pairs = int(eng.list_counts()[:4].sum())
+    ev_list.clear()
+    eng.set_timing(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = eng.stage_times()
+    t_list_ms = [a.elapsed_time(b) for a, b in ev_list]
+    eng.set_timing(2)
+    for k in range(NLIST):
+        step(k + 1)
+    torch.cuda.synchronize()
+    st2 = eng.stage_times()
+    eng.set_timing(0)
+    n2 = max(st2[5], 1.0)
+    n_ev = max(st[5], 1.0)
+    counts = eng.list_counts()
+    e_last = float(energ.cpu().numpy()[0])
+    eng.close()
+    return {
+        'elapsed': t1 - t0,
+        'n_pairs': n_pairs,
+        'list_counts': [int(c) for c in counts],
+        't_rs_ms': st[0] / n_ev,
+        'stage_ms': {'realspace_water': st[0] / n_ev, 'realspace_atoms': st[1] / n_ev,
+                     'pme': st2[2] / n2, 'bonded_excl': st2[3] / n2,
+                     'eforce_total': st2[4] / n2,
+                     'list_rebuild': (sum(t_list_ms) / len(t_list_ms)) if t_list_ms else None},
+        'e_total_last': e_last,
+    }
+
+
+def large_leg(steps=200, warmup=20):
+    """The ~1.0M-atom solvated heteropolymer (20,000 chains, 215 A box,
+    216^3 grid) through the metric's loop: evals/s with a list rebuild
+    every NLIST evaluations, mixed precision."""
+    from moil_fnrfd_amd import configs
+    t0 = time.perf_counter()
+    sysm, cut, ew = configs.hetero_1m()
+    t_gen = time.perf_counter() - t0
+    r = timed_loop(sysm, cut, ew, steps, warmup, seed=2102)
+    return {
+        'workload': 'hetero1m: %d atoms (20,000 chains of 10-20 atoms + %d waters), 215 A box, '
+                    'rvmx=relx=10 A, rvbg=rebg=11.5 A, PME 216^3, list rebuild every %d evals'
+                    % (sysm.topo.npt, sysm.meta['n_water'], NLIST),
+        'atoms': sysm.topo.npt,
+        'steps': steps,
+        'evals_per_s': steps / r['elapsed'],
+        'ms_per_step': 1e3 * r['elapsed'] / steps,
+        'list_counts': r['list_counts'],
+        'stage_ms': r['stage_ms'],
+        'generate_s': t_gen,
+    }
+
+
 def adjoint_leg(n_conf=128, seed=4001):
     """Differentiation workload: the 10,048-atom heteropolymer-in-water
     system, n_conf conformations (base + N(0, 0.1 A) per coordinate).  Per
@@ -323,6 +426,9 @@ def main():
     ap.add_argument('--batched-steps', type=int, default=10)
     ap.add_argument('--no-adjoint', action='store_true',
                     help='skip the differentiation leg (128 conformations)')
+    ap.add_argument('--large-only', action='store_true',
+                    help='run only the ~1.0M-atom heteropolymer loop and print its JSON')
+    ap.add_argument('--large-steps', type=int, default=200)
     ap.add_argument('--adjoint-only', action='store_true',
                     help='run only the differentiation leg and print its JSON')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
@@ -338,11 +444,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from moil_fnrfd_amd import abi, configs
-    from moil_fnrfd_amd.eforce import MoilGPU
+    from moil_fnrfd_amd import configs
 
     torch.cuda.set_device(0)
     if args.batched_only:
@@ -351,66 +455,14 @@ def main():
     if args.adjoint_only:
         print(json.dumps(adjoint_leg()))
         return
+    if args.large_only:
+        print(json.dumps(large_leg(steps=args.large_steps)))
+        return
     sysm, cut, ew = configs.water32k()
-    params = configs.params_for(sysm, cut, ew, precision=abi.PREC_MIXED)
-    eng = MoilGPU(sysm.topo, params)
-    dev = torch.device('cuda:0')
-    stream = torch.cuda.current_stream()
-    sh = stream.cuda_stream
-    coor = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
-    dpot = torch.empty_like(coor)
-    energ = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
-    # per-step N(0, 0.01 A) displacements: float64 normals drawn on the
-    # device NLIST steps at a time into a resident buffer
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(1001)
-    noise = torch.empty((NLIST,) + tuple(coor.shape), dtype=torch.float64, device=dev)
-    ev_list = []
-
-    def step(k, base=0):
-        if k % NLIST == 0:
-            torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float64, out=noise)
-        if k > 0:
-            coor.add_(noise[k % NLIST], alpha=SIGMA)
-        if k % NLIST == 0:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            eng.squeeze_dev(coor.data_ptr(), sh)
-            eng.nbondm_dev(coor.data_ptr(), sh)
-            e1.record(stream)
-            ev_list.append((e0, e1))
-        eng.eforce_dev(coor.data_ptr(), dpot.data_ptr(), energ.data_ptr(), sh)
-
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
-    n_pairs = int(eng.list_counts()[:4].sum())
-    ev_list.clear()
-    # timed region: level-1 timing (the real-space kernel's own dispatch
-    # events, nothing inserted between kernels)
-    eng.set_timing(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(k, args.warmup)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    st = eng.stage_times()
-    t_list_ms = [a.elapsed_time(b) for a, b in ev_list]
-    # stage breakdown from a short untimed pass with events between stages
-    eng.set_timing(2)
-    for k in range(NLIST):
-        step(k + 1, args.warmup + args.steps)
-    torch.cuda.synchronize()
-    st2 = eng.stage_times()
-    eng.set_timing(0)
-    n2 = max(st2[5], 1.0)
-    e_last = energ.cpu().numpy()
-    elapsed = t1 - t0
-    evals_per_s = args.steps / elapsed
-    n_ev = max(st[5], 1.0)
-    t_rs_ms = st[0] / n_ev
+    r = timed_loop(sysm, cut, ew, args.steps, args.warmup, seed=1001)
+    evals_per_s = args.steps / r['elapsed']
+    n_pairs = r['n_pairs']
+    t_rs_ms = r['t_rs_ms']
     B_rs = algorithmic_bytes_realspace(n_pairs, sysm.topo.npt)
     achieved = B_rs / (t_rs_ms * 1e-3) / 1e9
     peak = 8000.0
@@ -421,7 +473,7 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': 1e3 * elapsed / args.steps,
+        'ms_per_step': 1e3 * r['elapsed'] / args.steps,
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
@@ -434,10 +486,8 @@ def main():
             'atoms': sysm.topo.npt,
             'water_pairs': n_pairs,
             'list_rebuild_every': NLIST,
-            'stage_ms': {'realspace_water': t_rs_ms, 'pme': st2[2] / n2,
-                         'bonded_excl': st2[3] / n2, 'eforce_total': st2[4] / n2,
-                         'list_rebuild': (sum(t_list_ms) / len(t_list_ms)) if t_list_ms else None},
-            'e_total_last': float(e_last[0]),
+            'stage_ms': r['stage_ms'],
+            'e_total_last': r['e_total_last'],
         },
         'roofline': {
             'bound': 'hbm',
@@ -450,7 +500,6 @@ def main():
             'algorithmic_bytes_per_launch': B_rs,
         },
     }
-    eng.close()
     if not args.no_batched:
         line['batched'] = batched_leg(steps=args.batched_steps)
     if not args.no_adjoint:

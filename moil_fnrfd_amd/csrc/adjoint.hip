@@ -235,8 +235,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_atoms(
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
-    if (s0 + t < nsol) {
-      const int a = sslot2atom[s0 + t];
+    const int a = (s0 + t < nsol) ? sslot2atom[s0 + t] : -1;
+    if (a >= 0) {
       ix[t][0] = coor[3 * a];
       ix[t][1] = coor[3 * a + 1];
       ix[t][2] = coor[3 * a + 2];
@@ -338,6 +338,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_atoms(
     const int w = u / 3, k = u - 3 * w;
     if (s0 + w >= nsol || acc[u] == 0.0) continue;
     const int a = sslot2atom[s0 + w];
+    if (a < 0) continue;
     double *out = k == 0 ? dq : (k == 1 ? de12 : de6);
     atomicAdd(&out[a], acc[u]);
   }
@@ -467,11 +468,11 @@ int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_d
       A.a = c->par.a; A.b = c->par.b; A.c = c->par.c;
       if (f64)
         k_adj_atoms<true><<<c->nachunk, ADJ_THREADS, 0, st>>>(
-            c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom, c->nsol, c->d_ptchg, c->d_eps12,
+            c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom, c->nsslot, c->d_ptchg, c->d_eps12,
             c->d_eps6, c->d_erf, A, d_dq, d_de12, d_de6);
       else
         k_adj_atoms<false><<<c->nachunk, ADJ_THREADS, 0, st>>>(
-            c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom, c->nsol, c->d_ptchg, c->d_eps12,
+            c->d_achunk, c->d_aitems, d_coor, c->d_sslot2atom, c->nsslot, c->d_ptchg, c->d_eps12,
             c->d_eps6, c->d_erf, A, d_dq, d_de12, d_de6);
     }
   }

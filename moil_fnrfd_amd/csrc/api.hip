@@ -115,7 +115,7 @@ static void ctx_free(moil_ctx *c) {
                   c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos,
                   c->d_wi_boff, c->d_wi_aoff, c->d_wi_eoff, c->d_wi_zero, c->d_wi_bidx,
                   c->d_wi_aidx, c->d_wi_eidx, c->d_wi_bpar, c->d_wi_apar, c->d_etot, c->d_ticket,
-                  c->d_xall, c->d_adj};
+                  c->d_xall, c->d_adj, c->d_scan_ws};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -833,7 +833,7 @@ extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t 
   *n = 0;
   if (c->naitems == 0) return MOIL_GPU_OK;
   std::vector<AItem> it(c->naitems);
-  std::vector<int32_t> s2a(c->nsol);
+  std::vector<int32_t> s2a(c->nsslot);
   HIP_CHECK(hipMemcpy(it.data(), c->d_aitems, sizeof(AItem) * it.size(), hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(s2a.data(), c->d_sslot2atom, sizeof(int32_t) * s2a.size(), hipMemcpyDeviceToHost));
   int64_t k = 0;

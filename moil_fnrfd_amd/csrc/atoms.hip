@@ -115,24 +115,62 @@ __global__ void k_atom_scatter(const int32_t *__restrict__ cell_of, const int32_
   out[start[k] + atomicAdd(&fill[k], 1)] = ids ? ids[q] : q;
 }
 
-__global__ void k_slot_of(const int32_t *__restrict__ sslot2atom, int nsol,
+__global__ void k_slot_of(const int32_t *__restrict__ sslot2atom, int nslot,
                           int32_t *__restrict__ slot_of) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nsol) slot_of[sslot2atom[q]] = q;
+  if (q < nslot && sslot2atom[q] >= 0) slot_of[sslot2atom[q]] = q;
+}
+
+// Solute slots padded per brick row: the cell keys of one row of bricks
+// (fixed bz, by) are contiguous, and each row's atoms start a new block,
+// so a block never spans the end of a row (whose next atoms lie a whole
+// box length away in x).  Padding slots hold -1.
+__global__ void k_row_pad(const int32_t *__restrict__ sstart, int nrow, int rowcells,
+                          int32_t *__restrict__ rowcnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  const int n = sstart[(r + 1) * rowcells] - sstart[r * rowcells];
+  rowcnt[r] = (n + WBLOCK - 1) / WBLOCK * WBLOCK;
+}
+
+__global__ void k_fill_i32(int32_t *__restrict__ a, int32_t v, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = v;
+}
+
+__global__ void k_pad_slots(const int32_t *__restrict__ sslot_u, int nsol,
+                            const int32_t *__restrict__ sstart, const int32_t *__restrict__ rowoff,
+                            int nrow, int rowcells, int32_t *__restrict__ sslot2atom) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nsol) return;
+  int lo = 0, hi = nrow - 1;  // last row starting at or before u
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sstart[mid * rowcells] <= u) lo = mid; else hi = mid - 1;
+  }
+  sslot2atom[rowoff[lo] + (u - sstart[lo * rowcells])] = sslot_u[u];
 }
 
 // ---------------------------------------------------------------------------
-// one workgroup per solute block; WRITE=false counts items per bucket,
-// WRITE=true writes them at boff + deterministic rank
-template <bool WRITE>
-__global__ __launch_bounds__(ABT) void k_ablock_items(
+// One workgroup per solute block: evaluates MOIL's predicates once for
+// every candidate partner and direction, keeps the non-empty records
+// (partner, direction, list1/2/3 masks over the block) in candidate order
+// in the block's slice of `raw` (capacity rcap; the count is exact even
+// when the slice overflows), and counts the items per popcount bucket.
+// For each (block atom, partner) the distance test comes first: a pair
+// beyond max(rvbg, rebg) is in no list whatever the monomer prefilter and
+// the exclusions say, so those (with their memory lookups) are only
+// evaluated for pairs inside the reach.  All tests are conjunctions with
+// MOIL's operations and order, so the pair set is unchanged.
+__global__ __launch_bounds__(ABT) void k_ablock_raw(
     const double *__restrict__ coor, const int32_t *__restrict__ sslot2atom,
     const int32_t *__restrict__ aslot2atom, const int32_t *__restrict__ acstart, Grid g,
     const int32_t *__restrict__ mono_of, const int32_t *__restrict__ aflag,
     const double *__restrict__ mcent, const int32_t *__restrict__ flags,
     const uint64_t *__restrict__ xmask, const int32_t *__restrict__ xfar,
     const int32_t *__restrict__ exc1, const int32_t *__restrict__ exc2, APar P,
-    int32_t *__restrict__ bcount, const int32_t *__restrict__ boff, AItem *__restrict__ items) {
+    int32_t *__restrict__ bcount, AItem *__restrict__ raw, int rcap,
+    int32_t *__restrict__ rcount) {
 #pragma clang fp contract(off)
   __shared__ double ix[WBLOCK][3], ic[WBLOCK][3];
   __shared__ int32_t iatom[WBLOCK], imono[WBLOCK], iflag[WBLOCK];
@@ -141,35 +179,37 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
   __shared__ int32_t cellk[AB_MAXCELLS];
   __shared__ int32_t cellpre[AB_MAXCELLS + 1];
   __shared__ int32_t scanbuf[ABT];
-  __shared__ int32_t bk_run[NBUCKET];
-  __shared__ int32_t wcnt[ABT / 64][NBUCKET];
+  __shared__ int32_t bk_cnt[NBUCKET];
+  __shared__ int32_t rw[ABT / 64];
+  __shared__ int32_t r_run;
 
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int s0 = b * WBLOCK;
-  const int nb = min(WBLOCK, P.nsol - s0);
   if (t < WBLOCK) {
-    if (t < nb) {
-      const int a = sslot2atom[s0 + t], m = mono_of[a];
-      iatom[t] = a;
+    const int a = (s0 + t < P.nsol) ? sslot2atom[s0 + t] : -1;
+    iatom[t] = a;
+    if (a >= 0) {
+      const int m = mono_of[a];
       imono[t] = m;
       iflag[t] = aflag[a];
       for (int k = 0; k < 3; k++) {
         ix[t][k] = coor[3 * a + k];
         ic[t][k] = mcent[3 * m + k];
       }
-    } else {
-      iatom[t] = -1;
     }
   }
-  if (t < NBUCKET) bk_run[t] = 0;
+  if (t < NBUCKET) bk_cnt[t] = 0;
+  if (t == 0) r_run = 0;
   __syncthreads();
   if (t == 0) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < nb; i++)
+    for (int i = 0; i < WBLOCK; i++) {
+      if (iatom[i] < 0) continue;
       for (int k = 0; k < 3; k++) {
         lo[k] = fmin(lo[k], ix[i][k]);
         hi[k] = fmax(hi[k], ix[i][k]);
       }
+    }
     for (int k = 0; k < 3; k++) {
       bb[k] = lo[k];
       bb[3 + k] = hi[k];
@@ -177,6 +217,8 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
   }
   __syncthreads();
   const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
+  const double reach2 = fmax(P.cutvbig2, P.cutebig2);
+  AItem *rblk = raw + (size_t)b * rcap;
 
   for (int d = 0; d < ADIRS; d++) {
     int s;
@@ -189,7 +231,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
     // block-atom filter for this direction
     if (t < 64) {
       bool ok = false;
-      if (t < nb) {
+      if (iatom[t] >= 0) {
         if (!image) ok = true;
         else if (!rev) ok = image_i_ok(ic[t][0], ic[t][1], ic[t][2], da, db, dc, large_cutmono, P);
         else ok = need_only_ok(ic[t][0], ic[t][1], ic[t][2], da, db, dc, P);
@@ -265,16 +307,14 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
           }
           ja = aslot2atom[acstart[cellk[lo]] + (f - cellpre[lo])];
           const double xj = coor[3 * ja], yj = coor[3 * ja + 1], zj = coor[3 * ja + 2];
-          const int jm = mono_of[ja], jf = aflag[ja];
-          const double cjx = mcent[3 * jm], cjy = mcent[3 * jm + 1], cjz = mcent[3 * jm + 2];
-          bool jok = true;
-          if (image && !rev) jok = need_only_ok(cjx, cjy, cjz, da, db, dc, P);
-          if (rev) jok = (jf & AF_WAT) && image_i_ok(cjx, cjy, cjz, da, db, dc, large_cutmono, P);
-          if (jok && (xj < q0[0] || xj > q1[0] || yj < q0[1] || yj > q1[1] || zj < q0[2] ||
-                      zj > q1[2]))
-            jok = false;
+          bool jok = !(xj < q0[0] || xj > q1[0] || yj < q0[1] || yj > q1[1] || zj < q0[2] ||
+                       zj > q1[2]);
           if (jok) {
-            uint64_t iv = ivalid;
+            const int jm = mono_of[ja], jf = aflag[ja];
+            const double cjx = mcent[3 * jm], cjy = mcent[3 * jm + 1], cjz = mcent[3 * jm + 2];
+            if (image && !rev) jok = need_only_ok(cjx, cjy, cjz, da, db, dc, P);
+            if (rev) jok = (jf & AF_WAT) && image_i_ok(cjx, cjy, cjz, da, db, dc, large_cutmono, P);
+            uint64_t iv = jok ? ivalid : 0ull;
             while (iv) {
               const int i = __ffsll((long long)iv) - 1;
               iv &= iv - 1;
@@ -283,24 +323,27 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
               if (!image) {
                 // real pair (pot/nbondm.f:176-199, pot/nbond.f:67-149)
                 if (!(jf & AF_WAT) && !(ai < ja)) continue;
+                const double r2 = dist2(ix[i][0], ix[i][1], ix[i][2], xj, yj, zj);
+                if (!(r2 < reach2)) continue;
                 if (imono[i] != jm) {
                   if (!(dist2(ic[i][0], ic[i][1], ic[i][2], cjx, cjy, cjz) <= P.cutmono2)) continue;
                 }
                 const int lo2 = min(ai, ja), hi2 = max(ai, ja);
                 if (excluded(lo2, hi2, xmask, xfar, exc1, exc2)) continue;
-                const double r2 = dist2(ix[i][0], ix[i][1], ix[i][2], xj, yj, zj);
                 cls = apair_class(r2, iflag[i], jf, P.hvdw0 != 0, P);
               } else if (!rev) {
                 // block atom translated (symm/nbmsym.f:197-233, symm/nbsym.f:79-142)
+                const double r2 = dist2(ix[i][0] + da, ix[i][1] + db, ix[i][2] + dc, xj, yj, zj);
+                if (!(r2 < reach2)) continue;
                 if (!(dist2(ic[i][0] + da, ic[i][1] + db, ic[i][2] + dc, cjx, cjy, cjz) < P.cutmono2))
                   continue;
-                const double r2 = dist2(ix[i][0] + da, ix[i][1] + db, ix[i][2] + dc, xj, yj, zj);
                 cls = apair_class(r2, iflag[i], jf, true, P);
               } else {
                 // water atom translated, block atom real
+                const double r2 = dist2(xj + da, yj + db, zj + dc, ix[i][0], ix[i][1], ix[i][2]);
+                if (!(r2 < reach2)) continue;
                 if (!(dist2(cjx + da, cjy + db, cjz + dc, ic[i][0], ic[i][1], ic[i][2]) < P.cutmono2))
                   continue;
-                const double r2 = dist2(xj + da, yj + db, zj + dc, ix[i][0], ix[i][1], ix[i][2]);
                 cls = apair_class(r2, jf, iflag[i], true, P);
               }
               if (cls == 1) m1 |= 1ull << i;
@@ -309,75 +352,134 @@ __global__ __launch_bounds__(ABT) void k_ablock_items(
             }
           }
         }
-        // split into items of <= WSPLIT pairs, bucketed by popcount
+        // bucket counts (order independent) and the record in candidate order
         const uint64_t mm = m1 | m2 | m3;
         const int pc = __popcll(mm);
-        const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
-        const int bk = rem ? WSPLIT - rem : -1;
-        int incl = nfull;
-        for (int off = 1; off < 64; off <<= 1) {
-          int v = __shfl_up(incl, off, 64);
-          if (lane >= off) incl += v;
-        }
-        const int wtot0 = __shfl(incl, 63, 64);
-        if (lane == 0) wcnt[wv][0] = wtot0;
-        for (int k = 1; k < NBUCKET; k++) {
-          uint64_t bal = __ballot(bk == k);
-          if (lane == 0) wcnt[wv][k] = __popcll(bal);
-        }
+        if (pc >= WSPLIT) atomicAdd(&bk_cnt[0], pc / WSPLIT);
+        if (pc % WSPLIT) atomicAdd(&bk_cnt[WSPLIT - pc % WSPLIT], 1);
+        const bool has = mm != 0;
+        const uint64_t bal = __ballot(has);
+        if (lane == 0) rw[wv] = __popcll(bal);
         __syncthreads();
-        uint64_t rest = mm;
-        if (WRITE && nfull > 0) {
-          int before0 = 0;
-          for (int w2 = 0; w2 < wv; w2++) before0 += wcnt[w2][0];
-          const int pos0 = boff[b * NBUCKET] + bk_run[0] + before0 + (incl - nfull);
-          for (int q = 0; q < nfull; q++) {
-            uint64_t piece = 0;
-            for (int k = 0; k < WSPLIT; k++) {
-              const uint64_t low = rest & (~rest + 1ull);
-              piece |= low;
-              rest ^= low;
-            }
-            AItem it;
-            it.j = ja;
-            it.s = d;
-            it.m1 = m1 & piece;
-            it.m2 = m2 & piece;
-            it.m3 = m3 & piece;
-            items[pos0 + q] = it;
-          }
-        }
-        if (WRITE) {
-          for (int k = 1; k < NBUCKET; k++) {
-            uint64_t bal = __ballot(bk == k);
-            if (bk == k) {
-              int before = 0;
-              for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
-              uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
-              const int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
-              AItem it;
-              it.j = ja;
-              it.s = d;
-              it.m1 = m1 & rest;
-              it.m2 = m2 & rest;
-              it.m3 = m3 & rest;
-              items[pos] = it;
-            }
+        if (has) {
+          int before = 0;
+          for (int w2 = 0; w2 < wv; w2++) before += rw[w2];
+          const uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+          const int pos = r_run + before + __popcll(lower);
+          if (pos < rcap) {
+            AItem r;
+            r.j = ja;
+            r.s = d;
+            r.m1 = m1;
+            r.m2 = m2;
+            r.m3 = m3;
+            rblk[pos] = r;
           }
         }
         __syncthreads();
-        if (t < NBUCKET) {
+        if (t == 0) {
           int tot = 0;
-          for (int w2 = 0; w2 < ABT / 64; w2++) tot += wcnt[w2][t];
-          bk_run[t] += tot;
+          for (int w2 = 0; w2 < ABT / 64; w2++) tot += rw[w2];
+          r_run += tot;
         }
-        __syncthreads();
       }
       __syncthreads();
     }
     __syncthreads();
   }
-  if (!WRITE && t < NBUCKET) bcount[b * NBUCKET + t] = bk_run[t];
+  __syncthreads();
+  if (t < NBUCKET) bcount[b * NBUCKET + t] = bk_cnt[t];
+  if (t == 0) rcount[b] = r_run;
+}
+
+// One workgroup per solute block: reads the block's records back in order
+// and writes its items -- each record split into items of <= WSPLIT pairs
+// (consecutive mask bits), bucketed by popcount -- at boff + ranks taken in
+// thread order (wavefront scans and ballots), so the item order is fixed.
+__global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ raw, int rcap,
+                                                      const int32_t *__restrict__ rcount,
+                                                      const int32_t *__restrict__ boff,
+                                                      AItem *__restrict__ items) {
+  __shared__ int32_t bk_run[NBUCKET];
+  __shared__ int32_t wcnt[ABT / 64][NBUCKET];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < NBUCKET) bk_run[t] = 0;
+  __syncthreads();
+  const int n = rcount[b];
+  const AItem *rblk = raw + (size_t)b * rcap;
+  for (int fb = 0; fb < n; fb += ABT) {
+    const int f = fb + t;
+    int ja = -1, d = 0;
+    uint64_t m1 = 0, m2 = 0, m3 = 0;
+    if (f < n) {
+      const AItem r = rblk[f];
+      ja = r.j;
+      d = r.s;
+      m1 = r.m1;
+      m2 = r.m2;
+      m3 = r.m3;
+    }
+    const uint64_t mm = m1 | m2 | m3;
+    const int pc = __popcll(mm);
+    const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
+    const int bk = rem ? WSPLIT - rem : -1;
+    int incl = nfull;
+    for (int off = 1; off < 64; off <<= 1) {
+      int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int wtot0 = __shfl(incl, 63, 64);
+    if (lane == 0) wcnt[wv][0] = wtot0;
+    for (int k = 1; k < NBUCKET; k++) {
+      uint64_t bal = __ballot(bk == k);
+      if (lane == 0) wcnt[wv][k] = __popcll(bal);
+    }
+    __syncthreads();
+    uint64_t rest = mm;
+    if (nfull > 0) {
+      int before0 = 0;
+      for (int w2 = 0; w2 < wv; w2++) before0 += wcnt[w2][0];
+      const int pos0 = boff[b * NBUCKET] + bk_run[0] + before0 + (incl - nfull);
+      for (int q = 0; q < nfull; q++) {
+        uint64_t piece = 0;
+        for (int k = 0; k < WSPLIT; k++) {
+          const uint64_t low = rest & (~rest + 1ull);
+          piece |= low;
+          rest ^= low;
+        }
+        AItem it;
+        it.j = ja;
+        it.s = d;
+        it.m1 = m1 & piece;
+        it.m2 = m2 & piece;
+        it.m3 = m3 & piece;
+        items[pos0 + q] = it;
+      }
+    }
+    for (int k = 1; k < NBUCKET; k++) {
+      uint64_t bal = __ballot(bk == k);
+      if (bk == k) {
+        int before = 0;
+        for (int w2 = 0; w2 < wv; w2++) before += wcnt[w2][k];
+        uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
+        const int pos = boff[b * NBUCKET + k] + bk_run[k] + before + __popcll(lower);
+        AItem it;
+        it.j = ja;
+        it.s = d;
+        it.m1 = m1 & rest;
+        it.m2 = m2 & rest;
+        it.m3 = m3 & rest;
+        items[pos] = it;
+      }
+    }
+    __syncthreads();
+    if (t < NBUCKET) {
+      int tot = 0;
+      for (int w2 = 0; w2 < ABT / 64; w2++) tot += wcnt[w2][t];
+      bk_run[t] += tot;
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void k_ajcount(const AItem *__restrict__ items, size_t n, int32_t *__restrict__ cnt) {
@@ -422,8 +524,8 @@ __device__ __forceinline__ void atoms_impl(
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
-    if (s0 + t < nsol) {
-      const int a = sslot2atom[s0 + t];
+    const int a = (s0 + t < nsol) ? sslot2atom[s0 + t] : -1;
+    if (a >= 0) {
       ix[t][0] = coor[3 * a];
       ix[t][1] = coor[3 * a + 1];
       ix[t][2] = coor[3 * a + 2];
@@ -647,14 +749,22 @@ int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
   HIP_CHECK(hipMalloc(&c->d_astart, sizeof(int32_t) * (c->ancell + 1)));
   HIP_CHECK(hipMalloc(&c->d_aslot2atom, sizeof(int32_t) * c->npt));
   HIP_CHECK(hipMalloc(&c->d_sstart, sizeof(int32_t) * (c->ancell + 1)));
-  HIP_CHECK(hipMalloc(&c->d_sslot2atom, sizeof(int32_t) * c->nsol));
+  // solute slots: each brick row padded to whole blocks
+  c->nrow = ((c->ag[1] + 1) / 2) * ((c->ag[2] + 1) / 2);
+  c->rowcells = ((c->ag[0] + 1) / 2) * 8;
+  c->nablk = (c->nsol + (WBLOCK - 1) * c->nrow + WBLOCK - 1) / WBLOCK;
+  c->nsslot = c->nablk * WBLOCK;
+  HIP_CHECK(hipMalloc(&c->d_sslot_u, sizeof(int32_t) * c->nsol));
+  HIP_CHECK(hipMalloc(&c->d_sslot2atom, sizeof(int32_t) * c->nsslot));
+  HIP_CHECK(hipMalloc(&c->d_rowcnt, sizeof(int32_t) * (c->nrow + 1)));
+  HIP_CHECK(hipMalloc(&c->d_rowoff, sizeof(int32_t) * (c->nrow + 1)));
   HIP_CHECK(hipMalloc(&c->d_sslot_of, sizeof(int32_t) * c->npt));
-  c->nablk = (c->nsol + WBLOCK - 1) / WBLOCK;
   HIP_CHECK(hipMalloc(&c->d_abcount, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_aboff, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajcnt, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajstart, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_acfirst, sizeof(int32_t) * (c->nablk + 1)));
+  HIP_CHECK(hipMalloc(&c->d_arcount, sizeof(int32_t) * (c->nablk + 1)));
   return MOIL_GPU_OK;
 }
 
@@ -663,7 +773,8 @@ void atoms_free(moil_ctx *c) {
                   c->d_sol, c->d_acell_of, c->d_acnt, c->d_astart, c->d_aslot2atom,
                   c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
                   c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajstart,
-                  c->d_ajpos, c->d_acfirst};
+                  c->d_ajpos, c->d_acfirst, c->d_araw, c->d_arcount, c->d_sslot_u,
+                  c->d_rowcnt, c->d_rowoff};
   for (void *p : ptrs)
     if (p) hipFree(p);
 }
@@ -683,7 +794,7 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_cells<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(d_coor, nullptr, c->npt, g, c->d_acell_of,
                                                       c->d_acnt);
-  k_scan_excl<<<1, 1024, 0, st>>>(c->d_acnt, c->d_astart, c->ancell);
+  CHECK_SCAN(scan_excl(c, c->d_acnt, c->d_astart, c->ancell, st));
   HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_scatter<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, nullptr, c->npt,
                                                         c->d_astart, c->d_acnt, c->d_aslot2atom);
@@ -692,13 +803,20 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_cells<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_sol, c->nsol, g, c->d_acell_of,
                                                        c->d_acnt);
-  k_scan_excl<<<1, 1024, 0, st>>>(c->d_acnt, c->d_sstart, c->ancell);
+  CHECK_SCAN(scan_excl(c, c->d_acnt, c->d_sstart, c->ancell, st));
   HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_scatter<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, c->d_sol, c->nsol,
-                                                         c->d_sstart, c->d_acnt, c->d_sslot2atom);
-  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->ancell, c->d_sslot2atom);
+                                                         c->d_sstart, c->d_acnt, c->d_sslot_u);
+  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->ancell, c->d_sslot_u);
+  // blocks end with their brick row (padding slots -1)
+  k_row_pad<<<(c->nrow + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->nrow, c->rowcells, c->d_rowcnt);
+  CHECK_SCAN(scan_excl(c, c->d_rowcnt, c->d_rowoff, c->nrow, st));
+  k_fill_i32<<<(c->nsslot + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, -1, c->nsslot);
+  k_pad_slots<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_sslot_u, c->nsol, c->d_sstart,
+                                                      c->d_rowoff, c->nrow, c->rowcells,
+                                                      c->d_sslot2atom);
   HIP_CHECK(hipMemsetAsync(c->d_sslot_of, 0xff, sizeof(int32_t) * c->npt, st));
-  k_slot_of<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, c->nsol, c->d_sslot_of);
+  k_slot_of<<<(c->nsslot + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, c->nsslot, c->d_sslot_of);
 
   APar ap;
   ap.a = P.a; ap.b = P.b; ap.c = P.c;
@@ -706,16 +824,32 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   ap.cutmono2 = P.cutmono2; ap.cutvbig2 = P.cutvbig2; ap.cutebig2 = P.cutebig2;
   ap.R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0)) * (1.0 + 1e-12) + 1e-9;
   ap.hvdw0 = (P.terms & MOIL_TERM_HVDW0) ? 1 : 0;
-  ap.nsol = c->nsol;
-  k_ablock_items<false><<<c->nablk, ABT, 0, st>>>(
-      d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
-      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
-      nullptr, nullptr);
-  k_scan_excl<<<1, 1024, 0, st>>>(c->d_abcount, c->d_aboff, c->nablk * NBUCKET);
+  ap.nsol = c->nsslot;
+  // predicates once per candidate; records kept per block (capacity from
+  // the previous build, redone with the exact need if it overflowed)
   c->h_aboff.assign(c->nablk * NBUCKET + 1, 0);
-  HIP_CHECK(hipMemcpyAsync(c->h_aboff.data(), c->d_aboff, sizeof(int32_t) * c->h_aboff.size(),
-                           hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<int32_t> hrc(c->nablk);
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!c->d_araw) {
+      c->araw_cap = std::max(c->araw_cap, 1024);
+      HIP_CHECK(hipMalloc(&c->d_araw, sizeof(AItem) * (size_t)c->araw_cap * c->nablk));
+    }
+    k_ablock_raw<<<c->nablk, ABT, 0, st>>>(
+        d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
+        c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
+        c->d_araw, c->araw_cap, c->d_arcount);
+    CHECK_SCAN(scan_excl(c, c->d_abcount, c->d_aboff, c->nablk * NBUCKET, st));
+    HIP_CHECK(hipMemcpyAsync(c->h_aboff.data(), c->d_aboff, sizeof(int32_t) * c->h_aboff.size(),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(hrc.data(), c->d_arcount, sizeof(int32_t) * hrc.size(),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const int need = *std::max_element(hrc.begin(), hrc.end());
+    if (need <= c->araw_cap) break;
+    hipFree(c->d_araw);
+    c->d_araw = nullptr;
+    c->araw_cap = need + need / 4 + 256;
+  }
   const std::vector<int32_t> &hoff = c->h_aboff;
   const size_t total = (size_t)hoff.back();
   if (total > c->aitem_cap || c->d_aitems == nullptr) {
@@ -728,10 +862,8 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     HIP_CHECK(hipMalloc(&c->d_ajpart, sizeof(long long) * 3 * c->aitem_cap));
   }
   c->naitems = total;
-  k_ablock_items<true><<<c->nablk, ABT, 0, st>>>(
-      d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
-      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, nullptr,
-      c->d_aboff, c->d_aitems);
+  k_ablock_write<<<c->nablk, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, c->d_aboff,
+                                           c->d_aitems);
   // chunk table by pair budget (items are popcount-descending per block)
   std::vector<int32_t> ch;
   for (int b = 0; b < c->nablk; b++) {
@@ -781,7 +913,7 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   if (total > 0) {
     const int ib = (int)((total + TB - 1) / TB);
     k_ajcount<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajcnt);
-    k_scan_excl<<<1, 1024, 0, st>>>(c->d_ajcnt, c->d_ajstart, c->npt);
+    CHECK_SCAN(scan_excl(c, c->d_ajcnt, c->d_ajstart, c->npt, st));
     HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
     k_ajfill<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajstart, c->d_ajcnt, c->d_ajpos);
   } else {
@@ -811,7 +943,7 @@ void atoms_plan(const moil_ctx *c, const double *d_coor, int part0, EvalPlan *p)
   A.items = c->d_aitems;
   A.coor = d_coor;
   A.sslot2atom = c->d_sslot2atom;
-  A.nsol = c->nsol;
+  A.nsol = c->nsslot;
   A.q = c->d_ptchg;
   A.e12 = c->d_eps12;
   A.e6 = c->d_eps6;

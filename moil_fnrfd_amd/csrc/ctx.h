@@ -114,6 +114,10 @@ struct moil_ctx {
 
   // ---- atom lists (pairs with an atom outside water monomers) ----
   int nsol = 0;                          // atoms of non-water monomers ("solute")
+  int nsslot = 0;                        // solute slots: blocks of WBLOCK, padded per brick row
+  int nrow = 0, rowcells = 0;            // brick rows of the cell grid, cell keys per row
+  int32_t *d_sslot_u = nullptr;          // [nsol] solute atoms in cell order, unpadded
+  int32_t *d_rowcnt = nullptr, *d_rowoff = nullptr;  // padded slots per row, their scan
   std::vector<int32_t> h_sol;            // their indices (0-based)
   int32_t *d_sol = nullptr;
   int32_t *d_mono_of = nullptr;          // [npt] monomer (0-based)
@@ -127,6 +131,9 @@ struct moil_ctx {
   int32_t *d_sstart = nullptr, *d_sslot2atom = nullptr, *d_sslot_of = nullptr;  // solute slots
   int nablk = 0;                         // solute blocks of WBLOCK slots
   int32_t *d_abcount = nullptr, *d_aboff = nullptr;
+  AItem *d_araw = nullptr;               // [nablk][araw_cap] raw records of the predicate pass
+  int araw_cap = 0;
+  int32_t *d_arcount = nullptr;          // [nablk] records per block
   std::vector<int32_t> h_aboff;
   AItem *d_aitems = nullptr;
   size_t aitem_cap = 0, naitems = 0;
@@ -149,6 +156,8 @@ struct moil_ctx {
   double *d_dpot = nullptr;        // [3*npt]
   bool have_lists = false;
   long list_gen = 0;               // incremented by every list build
+  int32_t *d_scan_ws = nullptr;    // scan workspace (grow-only)
+  size_t scan_ws_cap = 0;
 
   // ---- parameter gradients (adjoint.hip) ----
   std::vector<double> h_q, h_e12, h_e6;  // current per-atom charges / LJ factors
@@ -167,6 +176,14 @@ struct moil_ctx {
   size_t ev_next = 0;
 };
 
+// exclusive scan of n int32 into out[0..n] (out[n] = total), on stream st
+// (lists.hip); workspace grows in the context
+int32_t scan_excl(moil_ctx *c, const int32_t *in, int32_t *out, int n, hipStream_t st);
+#define CHECK_SCAN(x)                        \
+  do {                                       \
+    const int32_t rc_ = (x);                 \
+    if (rc_ != MOIL_GPU_OK) return rc_;      \
+  } while (0)
 // list builder (lists.hip)
 int32_t lists_init(moil_ctx *c);
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st);

@@ -63,7 +63,5 @@ __device__ inline bool need_only_ok(double cjx, double cjy, double cjz, double d
 }
 
 
-// exclusive scan of n int32 values with one workgroup of 1024 threads
-__global__ void k_scan_excl(const int32_t *__restrict__ in, int32_t *__restrict__ out, int n);
 // insertion sort of each range [start[k], start[k+1]) of arr
 __global__ void k_cell_sort(const int32_t *__restrict__ start, int ncell, int32_t *__restrict__ arr);

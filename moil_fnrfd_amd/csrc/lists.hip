@@ -80,29 +80,93 @@ __global__ void k_wat_cells(const double *__restrict__ cent, const int32_t *__re
   atomicAdd(&cnt[k], 1);
 }
 
-// exclusive scan of n int32 values with one workgroup of 1024 threads
-__global__ void k_scan_excl(const int32_t *__restrict__ in, int32_t *__restrict__ out, int n) {
-  __shared__ int32_t part[1024];
-  int t = threadIdx.x;
-  int per = (n + 1023) / 1024;
-  int b = t * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; i++) s += in[i];
-  part[t] = s;
+// Exclusive scan of n int32 values into out[0..n] (out[n] = total), any n:
+// 2048-element tiles per workgroup (coalesced loads through LDS, eight
+// values per thread, wavefront shuffles for the 256 thread sums), the tile
+// totals scanned the same way, then the tile offsets added.  Integer
+// arithmetic, so the result does not depend on the decomposition.
+#define SCAN_TB 256
+#define SCAN_PER 8
+#define SCAN_TILE (SCAN_TB * SCAN_PER)
+
+__global__ __launch_bounds__(SCAN_TB) void k_scan_tile(const int32_t *__restrict__ in,
+                                                       int32_t *__restrict__ out, int n,
+                                                       int32_t *__restrict__ tot) {
+  __shared__ int32_t v[SCAN_TILE];
+  __shared__ int32_t ws[SCAN_TB / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; k++) {
+    const size_t i = base + k * SCAN_TB + t;
+    v[k * SCAN_TB + t] = i < (size_t)n ? in[i] : 0;
+  }
   __syncthreads();
-  // Hillis-Steele inclusive scan over the 1024 partials
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  int pre[SCAN_PER], sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; k++) {
+    pre[k] = sum;
+    sum += v[t * SCAN_PER + k];
   }
-  int run = (t == 0) ? 0 : part[t - 1];
-  for (int i = b; i < e; i++) {
-    out[i] = run;
-    run += in[i];
+  int incl = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
   }
-  if (t == 1023) out[n] = part[1023];
+  if (lane == 63) ws[wv] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wv; w++) wbase += ws[w];
+  const int ex = wbase + incl - sum;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; k++) v[t * SCAN_PER + k] = ex + pre[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; k++) {
+    const size_t i = base + k * SCAN_TB + t;
+    if (i < (size_t)n) out[i] = v[k * SCAN_TB + t];
+  }
+  if (t == SCAN_TB - 1) tot[blockIdx.x] = wbase + incl;
+}
+
+__global__ __launch_bounds__(SCAN_TB) void k_scan_add(int32_t *__restrict__ out, int n,
+                                                      const int32_t *__restrict__ toff, int ntile) {
+  const size_t i = (size_t)blockIdx.x * SCAN_TB + threadIdx.x;
+  if (i < (size_t)n) out[i] += toff[i / SCAN_TILE];
+  if (i == 0) out[n] = toff[ntile];
+}
+
+static size_t scan_ws_need(int n) {
+  const size_t nt = ((size_t)n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nt <= 1) return 0;
+  return nt + (nt + 1) + scan_ws_need((int)nt);
+}
+
+static void scan_rec(const int32_t *in, int32_t *out, int n, int32_t *ws, hipStream_t st) {
+  const int nt = (int)(((size_t)n + SCAN_TILE - 1) / SCAN_TILE);
+  if (nt <= 1) {
+    k_scan_tile<<<1, SCAN_TB, 0, st>>>(in, out, n, out + n);
+    return;
+  }
+  int32_t *tot = ws, *toff = ws + nt;
+  k_scan_tile<<<nt, SCAN_TB, 0, st>>>(in, out, n, tot);
+  scan_rec(tot, toff, nt, ws + nt + (nt + 1), st);
+  k_scan_add<<<(unsigned)(((size_t)n + SCAN_TB - 1) / SCAN_TB), SCAN_TB, 0, st>>>(out, n, toff, nt);
+}
+
+int32_t scan_excl(moil_ctx *c, const int32_t *in, int32_t *out, int n, hipStream_t st) {
+  const size_t need = scan_ws_need(n);
+  if (need > c->scan_ws_cap) {
+    HIP_CHECK(hipStreamSynchronize(st));
+    hipFree(c->d_scan_ws);
+    c->d_scan_ws = nullptr;
+    c->scan_ws_cap = 0;
+    HIP_CHECK(hipMalloc(&c->d_scan_ws, sizeof(int32_t) * (need + need / 2 + 64)));
+    c->scan_ws_cap = need + need / 2 + 64;
+  }
+  scan_rec(in, out, n, c->d_scan_ws, st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
 }
 
 __global__ void k_wat_scatter(const int32_t *__restrict__ cell_of, int nwat,
@@ -191,14 +255,13 @@ __device__ inline uint64_t take_low_bits(uint64_t &m, int n) {
 // the workgroup, mask 0 = none): each mask is split into nfull items of
 // WSPLIT pairs (bucket 0) and at most one remainder item (bucket WSPLIT -
 // popcount remainder); per-bucket positions come from wavefront scans and
-// ballots in thread order, so the item order is fixed.  WRITE=false only
-// advances the per-bucket counts.  Every thread of the workgroup calls it.
+// ballots in thread order, so the item order is fixed.  Every thread of
+// the workgroup calls it.
 struct EmitLDS {
   int32_t bk_run[NBUCKET];
   int32_t wcnt[LB_THREADS / 64][NBUCKET];
 };
 
-template <bool WRITE>
 __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
                                   const int32_t *__restrict__ boff_b, WItem *__restrict__ items) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -218,7 +281,7 @@ __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64
     if (lane == 0) L.wcnt[wv][k] = __popcll(bal);
   }
   __syncthreads();
-  if (WRITE) {
+  {
     uint64_t rest = mm;  // bits left for the remainder item
     if (nfull > 0) {
       int before0 = 0;
@@ -438,12 +501,16 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
           for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += rw[w2];
           r_run += tot;
         }
-        emit_batch<false>(L, js, s, m1, m2, nullptr, nullptr);
+        // items per popcount bucket (counts only: order independent)
+        const int pc = __popcll(m1 | m2);
+        if (pc >= WSPLIT) atomicAdd(&L.bk_run[0], pc / WSPLIT);
+        if (pc % WSPLIT) atomicAdd(&L.bk_run[WSPLIT - pc % WSPLIT], 1);
       }
       __syncthreads();
     }
     __syncthreads();
   }
+  __syncthreads();
   if (t < NBUCKET) bcount[b * NBUCKET + t] = L.bk_run[t];
   if (t == 0) rcount[b] = r_run;
 }
@@ -471,7 +538,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restr
       m1 = r.m1;
       m2 = r.m2;
     }
-    emit_batch<true>(L, js, s, m1, m2, boff + b * NBUCKET, items);
+    emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items);
   }
 }
 
@@ -576,7 +643,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     int nblk = (c->nwat + TB - 1) / TB;
     k_wat_cells<<<nblk, TB, 0, st>>>(c->d_mono_cent, c->d_wat_mono, c->nwat, g, c->d_cell_of,
                                      c->d_cell_cnt);
-    k_scan_excl<<<1, 1024, 0, st>>>(c->d_cell_cnt, c->d_cell_start, c->ncell);
+    CHECK_SCAN(scan_excl(c, c->d_cell_cnt, c->d_cell_start, c->ncell, st));
     k_wat_scatter<<<nblk, TB, 0, st>>>(c->d_cell_of, c->nwat, c->d_cell_start, c->d_cell_fill,
                                        c->d_slot2wat);
     k_cell_sort<<<(c->ncell + TB - 1) / TB, TB, 0, st>>>(c->d_cell_start, c->ncell, c->d_slot2wat);
@@ -604,7 +671,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
       k_wblock_items<<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
                                                       c->d_flags, c->d_bcount, c->d_wraw,
                                                       c->wraw_cap, c->d_rcount);
-      k_scan_excl<<<1, 1024, 0, st>>>(c->d_bcount, c->d_boff, c->nwblk * NBUCKET);
+      CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * NBUCKET, st));
       HIP_CHECK(hipMemcpyAsync(hoff.data(), c->d_boff, sizeof(int32_t) * hoff.size(),
                                hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipMemcpyAsync(hrc.data(), c->d_rcount, sizeof(int32_t) * hrc.size(),
@@ -702,7 +769,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     if (total > 0) {
       const int ib = (int)((total + TB - 1) / TB);
       k_jcount<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jcnt);
-      k_scan_excl<<<1, 1024, 0, st>>>(c->d_jcnt, c->d_jstart, c->nwat);
+      CHECK_SCAN(scan_excl(c, c->d_jcnt, c->d_jstart, c->nwat, st));
       HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
       k_jfill<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jstart, c->d_jcnt, c->d_jpos);
     } else {
