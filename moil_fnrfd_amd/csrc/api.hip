@@ -519,12 +519,29 @@ extern "C" int32_t moil_gpu_squeeze(moil_ctx *c, double *coor) {
   return MOIL_GPU_OK;
 }
 
+// The water and atom list builds are enqueued without waiting; their sizes
+// and any capacity overflow come back in one small copy at the end.  An
+// overflow (the first build, or lists that grew past earlier sizes) grows
+// the buffers and redoes the build.
 extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, void *stream) {
   if (!c || !d_coor) return fail(MOIL_GPU_EINVAL, "null argument");
   hipStream_t st = pick_stream(c, stream);
-  CHECK_RC(lists_build(c, d_coor, st));
-  CHECK_RC(atoms_build(c, d_coor, st));
+  int32_t status[ST_COUNT];
+  for (int attempt = 0;; attempt++) {
+    HIP_CHECK(hipMemsetAsync(c->d_status, 0, sizeof(int32_t) * ST_COUNT, st));
+    CHECK_RC(lists_build(c, d_coor, st));
+    CHECK_RC(atoms_build(c, d_coor, st));
+    HIP_CHECK(hipMemcpyAsync(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const bool w = lists_grow(c, status), a = atoms_grow(c, status);
+    if (!w && !a) break;
+    if (attempt >= 2) return fail(MOIL_GPU_EHIP, "list build did not fit after growing its buffers");
+  }
+  CHECK_RC(lists_finish(c, status));
+  CHECK_RC(atoms_finish(c, status));
   CHECK_RC(ensure_parts(c, c->nwchunk + c->nachunk));
+  c->have_lists = true;
+  c->list_gen++;
   return MOIL_GPU_OK;
 }
 
@@ -833,12 +850,13 @@ extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t 
   *n = 0;
   if (c->naitems == 0) return MOIL_GPU_OK;
   std::vector<AItem> it(c->naitems);
-  std::vector<int32_t> s2a(c->nsslot);
+  std::vector<int32_t> s2a(c->nsslot), aboff((size_t)c->nablk * NBUCKET + 1);
+  HIP_CHECK(hipMemcpy(aboff.data(), c->d_aboff, sizeof(int32_t) * aboff.size(), hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(it.data(), c->d_aitems, sizeof(AItem) * it.size(), hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(s2a.data(), c->d_sslot2atom, sizeof(int32_t) * s2a.size(), hipMemcpyDeviceToHost));
   int64_t k = 0;
   for (int b = 0; b < c->nablk; b++)
-    for (int q = c->h_aboff[b * NBUCKET]; q < c->h_aboff[(b + 1) * NBUCKET]; q++) {
+    for (int q = aboff[b * NBUCKET]; q < aboff[(b + 1) * NBUCKET]; q++) {
       const AItem &a = it[q];
       const bool rev = a.s > 13;
       const int sym = rev ? a.s - 13 : a.s;

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
     const uint64_t *__restrict__ xmask, const int32_t *__restrict__ xfar,
     const int32_t *__restrict__ exc1, const int32_t *__restrict__ exc2, APar P,
     int32_t *__restrict__ bcount, AItem *__restrict__ raw, int rcap,
-    int32_t *__restrict__ rcount) {
+    int32_t *__restrict__ rcount, int32_t *__restrict__ rneed) {
 #pragma clang fp contract(off)
   __shared__ double ix[WBLOCK][3], ic[WBLOCK][3];
   __shared__ int32_t iatom[WBLOCK], imono[WBLOCK], iflag[WBLOCK];
@@ -389,7 +389,10 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
   }
   __syncthreads();
   if (t < NBUCKET) bcount[b * NBUCKET + t] = bk_cnt[t];
-  if (t == 0) rcount[b] = r_run;
+  if (t == 0) {
+    rcount[b] = r_run;
+    if (r_run > rcap) atomicMax(rneed, r_run);
+  }
 }
 
 // One workgroup per solute block: reads the block's records back in order
@@ -399,13 +402,13 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
 __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ raw, int rcap,
                                                       const int32_t *__restrict__ rcount,
                                                       const int32_t *__restrict__ boff,
-                                                      AItem *__restrict__ items) {
+                                                      AItem *__restrict__ items, size_t cap) {
   __shared__ int32_t bk_run[NBUCKET];
   __shared__ int32_t wcnt[ABT / 64][NBUCKET];
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (t < NBUCKET) bk_run[t] = 0;
   __syncthreads();
-  const int n = rcount[b];
+  const int n = min(rcount[b], rcap);
   const AItem *rblk = raw + (size_t)b * rcap;
   for (int fb = 0; fb < n; fb += ABT) {
     const int f = fb + t;
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ 
         it.m1 = m1 & piece;
         it.m2 = m2 & piece;
         it.m3 = m3 & piece;
-        items[pos0 + q] = it;
+        if ((size_t)(pos0 + q) < cap) items[pos0 + q] = it;
       }
     }
     for (int k = 1; k < NBUCKET; k++) {
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ 
         it.m1 = m1 & rest;
         it.m2 = m2 & rest;
         it.m3 = m3 & rest;
-        items[pos] = it;
+        if ((size_t)pos < cap) items[pos] = it;
       }
     }
     __syncthreads();
@@ -482,18 +485,32 @@ __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ 
   }
 }
 
-__global__ void k_ajcount(const AItem *__restrict__ items, size_t n, int32_t *__restrict__ cnt) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) atomicAdd(&cnt[items[q].j], 1);
+// (nothing when the predicate pass dropped records: the items are then
+// incomplete and the build is redone)
+__global__ void k_ajcount(const AItem *__restrict__ items, const int32_t *__restrict__ nitem,
+                          size_t cap, const int32_t *__restrict__ dropped,
+                          int32_t *__restrict__ cnt) {
+  const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[items[q].j], 1);
 }
-// jpos[item] = the item's row in the partial buffer, inside its partner
-// atom's range (rows are summed as integers, so their order is free)
-__global__ void k_ajfill(const AItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
-                         int32_t *__restrict__ fill, int32_t *__restrict__ jpos) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int j = items[q].j;
-  jpos[q] = start[j] + atomicAdd(&fill[j], 1);
+__global__ void k_ajfill(const AItem *__restrict__ items, const int32_t *__restrict__ nitem,
+                         size_t cap, const int32_t *__restrict__ dropped,
+                         const int32_t *__restrict__ start, int32_t *__restrict__ fill,
+                         int32_t *__restrict__ jpos) {
+  const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int j = items[q].j;
+    jpos[q] = start[j] + atomicAdd(&fill[j], 1);
+  }
+}
+
+__global__ void k_acopy_status(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+                               int32_t *__restrict__ sa, int32_t *__restrict__ sb) {
+  *sa = *a;
+  *sb = *b;
 }
 
 // ---------------------------------------------------------------------------
@@ -774,7 +791,7 @@ void atoms_free(moil_ctx *c) {
                   c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
                   c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajstart,
                   c->d_ajpos, c->d_acfirst, c->d_araw, c->d_arcount, c->d_sslot_u,
-                  c->d_rowcnt, c->d_rowoff};
+                  c->d_rowcnt, c->d_rowoff, c->d_account};
   for (void *p : ptrs)
     if (p) hipFree(p);
 }
@@ -825,101 +842,93 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   ap.R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0)) * (1.0 + 1e-12) + 1e-9;
   ap.hvdw0 = (P.terms & MOIL_TERM_HVDW0) ? 1 : 0;
   ap.nsol = c->nsslot;
-  // predicates once per candidate; records kept per block (capacity from
-  // the previous build, redone with the exact need if it overflowed)
-  c->h_aboff.assign(c->nablk * NBUCKET + 1, 0);
-  std::vector<int32_t> hrc(c->nablk);
-  for (int attempt = 0; attempt < 2; attempt++) {
-    if (!c->d_araw) {
-      c->araw_cap = std::max(c->araw_cap, 1024);
-      HIP_CHECK(hipMalloc(&c->d_araw, sizeof(AItem) * (size_t)c->araw_cap * c->nablk));
-    }
-    k_ablock_raw<<<c->nablk, ABT, 0, st>>>(
-        d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
-        c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
-        c->d_araw, c->araw_cap, c->d_arcount);
-    CHECK_SCAN(scan_excl(c, c->d_abcount, c->d_aboff, c->nablk * NBUCKET, st));
-    HIP_CHECK(hipMemcpyAsync(c->h_aboff.data(), c->d_aboff, sizeof(int32_t) * c->h_aboff.size(),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(hrc.data(), c->d_arcount, sizeof(int32_t) * hrc.size(),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    const int need = *std::max_element(hrc.begin(), hrc.end());
-    if (need <= c->araw_cap) break;
-    hipFree(c->d_araw);
-    c->d_araw = nullptr;
-    c->araw_cap = need + need / 4 + 256;
+  // enqueued without waiting, like the water lists: capacities from earlier
+  // builds, overflows reported in the status words (ST_A*)
+  if (!c->d_araw) {
+    c->araw_cap = std::max(c->araw_cap, 1024);
+    HIP_CHECK(hipMalloc(&c->d_araw, sizeof(AItem) * (size_t)c->araw_cap * c->nablk));
   }
-  const std::vector<int32_t> &hoff = c->h_aboff;
-  const size_t total = (size_t)hoff.back();
-  if (total > c->aitem_cap || c->d_aitems == nullptr) {
-    hipFree(c->d_aitems);
-    hipFree(c->d_ajpos);
-    hipFree(c->d_ajpart);
-    c->aitem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
+  if (!c->d_aitems) {
+    c->aitem_cap = std::max<size_t>(c->aitem_cap, 4096);
     HIP_CHECK(hipMalloc(&c->d_aitems, sizeof(AItem) * c->aitem_cap));
     HIP_CHECK(hipMalloc(&c->d_ajpos, sizeof(int32_t) * c->aitem_cap));
     HIP_CHECK(hipMalloc(&c->d_ajpart, sizeof(long long) * 3 * c->aitem_cap));
   }
-  c->naitems = total;
+  if (!c->d_achunk) {
+    c->achunk_cap = std::max(c->achunk_cap, 1024);
+    HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * 3 * (size_t)c->achunk_cap));
+  }
+  if (!c->d_account) HIP_CHECK(hipMalloc(&c->d_account, sizeof(int32_t) * (c->nablk + 1)));
+  k_ablock_raw<<<c->nablk, ABT, 0, st>>>(
+      d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
+      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
+      c->d_araw, c->araw_cap, c->d_arcount, c->d_status + ST_ARAW);
+  CHECK_SCAN(scan_excl(c, c->d_abcount, c->d_aboff, c->nablk * NBUCKET, st));
   k_ablock_write<<<c->nablk, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, c->d_aboff,
-                                           c->d_aitems);
-  // chunk table by pair budget (items are popcount-descending per block)
-  std::vector<int32_t> ch;
-  for (int b = 0; b < c->nablk; b++) {
-    int q = hoff[b * NBUCKET];
-    const int end = hoff[(b + 1) * NBUCKET];
-    while (q < end) {
-      int e = q;
-      long pairs = 0;
-      while (e < end && pairs < c->wchunk_pairs && e - q < c->wchunk_max_items) {
-        int k = (int)(std::upper_bound(hoff.begin() + b * NBUCKET, hoff.begin() + (b + 1) * NBUCKET, e) -
-                      (hoff.begin() + b * NBUCKET)) - 1;
-        const int pc = WSPLIT - k;
-        const int nxt = std::min(hoff[b * NBUCKET + k + 1], end);
-        const long want = std::max<long>(1, (c->wchunk_pairs - pairs + pc - 1) / pc);
-        int take = (int)std::min<long>(nxt - e, want);
-        take = std::min(take, c->wchunk_max_items - (e - q));
-        pairs += (long)take * pc;
-        e += take;
-      }
-      ch.push_back(b);
-      ch.push_back(q);
-      ch.push_back(e);
-      q = e;
-    }
+                                           c->d_aitems, c->aitem_cap);
+  ChunkPar cp;
+  cp.nblk = c->nablk;
+  cp.pairs = c->wchunk_pairs;
+  cp.max_items = c->wchunk_max_items;
+  cp.min_items = 1;
+  cp.cap = c->achunk_cap;
+  CHECK_SCAN(chunk_table(c, c->d_aboff, cp, c->d_account, c->d_acfirst, c->d_achunk, nullptr,
+                         nullptr, st));
+  const int32_t *d_nitem = c->d_aboff + (size_t)c->nablk * NBUCKET;
+  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
+  k_ajcount<<<1024, TB, 0, st>>>(c->d_aitems, d_nitem, c->aitem_cap, c->d_status + ST_ARAW,
+                                 c->d_ajcnt);
+  CHECK_SCAN(scan_excl(c, c->d_ajcnt, c->d_ajstart, c->npt, st));
+  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
+  k_ajfill<<<1024, TB, 0, st>>>(c->d_aitems, d_nitem, c->aitem_cap, c->d_status + ST_ARAW,
+                                c->d_ajstart, c->d_ajcnt, c->d_ajpos);
+  k_acopy_status<<<1, 1, 0, st>>>(d_nitem, c->d_acfirst + c->nablk, c->d_status + ST_AITEMS,
+                                  c->d_status + ST_ACHUNK);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+bool atoms_grow(moil_ctx *c, const int32_t *status) {
+  bool again = false;
+  if (c->nsol == 0) return false;
+  if (status[ST_ARAW] > c->araw_cap) {
+    hipFree(c->d_araw);
+    c->d_araw = nullptr;
+    c->araw_cap = status[ST_ARAW] + status[ST_ARAW] / 4 + 256;
+    again = true;
   }
-  c->nachunk = (int)ch.size() / 3;
-  std::vector<int32_t> cfirst(c->nablk + 1, c->nachunk);
-  for (int q = c->nachunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
-  for (int b = c->nablk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
-  if (c->nachunk > 0) {
-    if (c->nachunk > c->achunk_cap || !c->d_achunk) {
-      hipFree(c->d_achunk);
-      c->achunk_cap = c->nachunk + c->nachunk / 4 + 64;
-      HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * 3 * (size_t)c->achunk_cap));
-    }
-    HIP_CHECK(hipMemcpyAsync(c->d_achunk, ch.data(), sizeof(int32_t) * ch.size(),
-                             hipMemcpyHostToDevice, st));
+  if ((size_t)status[ST_AITEMS] > c->aitem_cap) {
+    hipFree(c->d_aitems);
+    hipFree(c->d_ajpos);
+    hipFree(c->d_ajpart);
+    c->d_aitems = nullptr;
+    c->d_ajpos = nullptr;
+    c->d_ajpart = nullptr;
+    c->aitem_cap = (size_t)status[ST_AITEMS] + status[ST_AITEMS] / 4 + 1024;
+    again = true;
   }
+  if (status[ST_ACHUNK] > c->achunk_cap) {
+    hipFree(c->d_achunk);
+    c->d_achunk = nullptr;
+    c->achunk_cap = status[ST_ACHUNK] + status[ST_ACHUNK] / 4 + 64;
+    again = true;
+  }
+  return again;
+}
+
+int32_t atoms_finish(moil_ctx *c, const int32_t *status) {
+  if (c->nsol == 0) {
+    c->naitems = 0;
+    c->nachunk = 0;
+    return MOIL_GPU_OK;
+  }
+  c->naitems = (size_t)status[ST_AITEMS];
+  c->nachunk = status[ST_ACHUNK];
   if (c->nachunk > c->aipart_cap || c->d_aipart == nullptr) {
     hipFree(c->d_aipart);
     c->aipart_cap = std::max(c->nachunk + c->nachunk / 4 + 64, 256);
     HIP_CHECK(hipMalloc(&c->d_aipart, sizeof(long long) * WBLOCK * 3 * (size_t)c->aipart_cap));
   }
-  HIP_CHECK(hipMemcpyAsync(c->d_acfirst, cfirst.data(), sizeof(int32_t) * cfirst.size(),
-                           hipMemcpyHostToDevice, st));
-  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
-  if (total > 0) {
-    const int ib = (int)((total + TB - 1) / TB);
-    k_ajcount<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajcnt);
-    CHECK_SCAN(scan_excl(c, c->d_ajcnt, c->d_ajstart, c->npt, st));
-    HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
-    k_ajfill<<<ib, TB, 0, st>>>(c->d_aitems, total, c->d_ajstart, c->d_ajcnt, c->d_ajpos);
-  } else {
-    HIP_CHECK(hipMemsetAsync(c->d_ajstart, 0, sizeof(int32_t) * (c->npt + 1), st));
-  }
-  HIP_CHECK(hipStreamSynchronize(st));
   return MOIL_GPU_OK;
 }
 

@@ -95,7 +95,10 @@ struct moil_ctx {
   int32_t *d_rcount = nullptr;     // [nwblk] records per block
   WItem *d_witems = nullptr;
   size_t witem_cap = 0, nwitems = 0;
-  int32_t *d_wchunk = nullptr;     // [nchunk][3] block, begin, end; then [nchunk] dispatch order
+  int32_t *d_wchunk = nullptr;     // [cap][3] block, begin, end
+  int32_t *d_wcpairs = nullptr;    // [cap] pairs per chunk
+  int32_t *d_worder = nullptr;     // [cap] dispatch order (largest first)
+  int32_t *d_wccount = nullptr;    // [nwblk+1] chunks per block
   int nwchunk = 0, wchunk_cap = 0;
   // gradient partials of the real-space water kernel (no global atomics):
   // jpart[row][9] (an item's j-water; the rows of one water contiguous),
@@ -134,7 +137,7 @@ struct moil_ctx {
   AItem *d_araw = nullptr;               // [nablk][araw_cap] raw records of the predicate pass
   int araw_cap = 0;
   int32_t *d_arcount = nullptr;          // [nablk] records per block
-  std::vector<int32_t> h_aboff;
+  int32_t *d_account = nullptr;          // [nablk+1] chunks per block
   AItem *d_aitems = nullptr;
   size_t aitem_cap = 0, naitems = 0;
   int32_t *d_achunk = nullptr;
@@ -157,6 +160,7 @@ struct moil_ctx {
   bool have_lists = false;
   long list_gen = 0;               // incremented by every list build
   int32_t *d_scan_ws = nullptr;    // scan workspace (grow-only)
+  int32_t *d_status = nullptr;     // [8] sizes / overflow needs of a list build (ST_*)
   size_t scan_ws_cap = 0;
 
   // ---- parameter gradients (adjoint.hip) ----
@@ -184,6 +188,21 @@ int32_t scan_excl(moil_ctx *c, const int32_t *in, int32_t *out, int n, hipStream
     const int32_t rc_ = (x);                 \
     if (rc_ != MOIL_GPU_OK) return rc_;      \
   } while (0)
+// status words of a list build: raw records needed (if over capacity),
+// items and chunks of the water and the atom lists
+enum { ST_WRAW = 0, ST_WITEMS, ST_WCHUNK, ST_ARAW, ST_AITEMS, ST_ACHUNK, ST_COUNT };
+// chunk table of a list (lists.hip): item runs of one block of about
+// `pairs` pairs, min_items..max_items items; cap = table capacity
+struct ChunkPar {
+  int nblk, pairs, max_items, min_items, cap;
+};
+int32_t chunk_table(moil_ctx *c, const int32_t *d_boff, ChunkPar P, int32_t *d_ccount,
+                    int32_t *d_cfirst, int32_t *d_chunks, int32_t *d_cpairs, int32_t *d_order,
+                    hipStream_t st);
+bool lists_grow(moil_ctx *c, const int32_t *status);
+int32_t lists_finish(moil_ctx *c, const int32_t *status);
+bool atoms_grow(moil_ctx *c, const int32_t *status);
+int32_t atoms_finish(moil_ctx *c, const int32_t *status);
 // list builder (lists.hip)
 int32_t lists_init(moil_ctx *c);
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st);

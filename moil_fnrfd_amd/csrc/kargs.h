@@ -27,7 +27,8 @@ struct PackArgs {
   void *wpos;  // float4[3] (mixed) or double[9] (FP64) per water slot
 };
 struct WaterArgs {
-  const int32_t *chunks;  // [nchunk][3] + [nchunk] dispatch order
+  const int32_t *chunks;  // [nchunk][3] block, begin, end
+  const int32_t *order;   // [nchunk] dispatch order
   const WItem *items;
   const int32_t *jpos;
   const void *wpos;

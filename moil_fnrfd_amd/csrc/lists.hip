@@ -263,7 +263,8 @@ struct EmitLDS {
 };
 
 __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
-                                  const int32_t *__restrict__ boff_b, WItem *__restrict__ items) {
+                                  const int32_t *__restrict__ boff_b, WItem *__restrict__ items,
+                                  size_t cap) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const uint64_t mm = m1 | m2;
   const int pc = __popcll(mm);
@@ -295,7 +296,7 @@ __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64
         it.m1 = m1 & piece;
         it.m2 = m2 & piece;
         it.pad = 0;
-        items[pos0 + q] = it;
+        if ((size_t)(pos0 + q) < cap) items[pos0 + q] = it;
       }
     }
     for (int k = 1; k < NBUCKET; k++) {
@@ -311,7 +312,7 @@ __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64
         it.m1 = m1 & rest;
         it.m2 = m2 & rest;
         it.pad = 0;
-        items[pos] = it;
+        if ((size_t)pos < cap) items[pos] = it;
       }
     }
   }
@@ -331,7 +332,8 @@ __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64
 __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     const double *__restrict__ scent, const int32_t *__restrict__ smono,
     const int32_t *__restrict__ cstart, Grid g, WPar P, const int32_t *__restrict__ flags,
-    int32_t *__restrict__ bcount, WRaw *__restrict__ raw, int rcap, int32_t *__restrict__ rcount) {
+    int32_t *__restrict__ bcount, WRaw *__restrict__ raw, int rcap, int32_t *__restrict__ rcount,
+    int32_t *__restrict__ rneed) {
   __shared__ double ic[WBLOCK][3];
   __shared__ int32_t im[WBLOCK];
   __shared__ double bb[6];
@@ -512,7 +514,10 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   }
   __syncthreads();
   if (t < NBUCKET) bcount[b * NBUCKET + t] = L.bk_run[t];
-  if (t == 0) rcount[b] = r_run;
+  if (t == 0) {
+    rcount[b] = r_run;
+    if (r_run > rcap) atomicMax(rneed, r_run);
+  }
 }
 
 // One workgroup per water block: reads the block's records back in order
@@ -520,12 +525,12 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
 __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restrict__ raw, int rcap,
                                                              const int32_t *__restrict__ rcount,
                                                              const int32_t *__restrict__ boff,
-                                                             WItem *__restrict__ items) {
+                                                             WItem *__restrict__ items, size_t cap) {
   __shared__ EmitLDS L;
   const int b = blockIdx.x, t = threadIdx.x;
   if (t < NBUCKET) L.bk_run[t] = 0;
   __syncthreads();
-  const int n = rcount[b];
+  const int n = min(rcount[b], rcap);
   const WRaw *rblk = raw + (size_t)b * rcap;
   for (int fb = 0; fb < n; fb += LB_THREADS) {
     const int f = fb + t;
@@ -538,27 +543,140 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restr
       m1 = r.m1;
       m2 = r.m2;
     }
-    emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items);
+    emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items, cap);
   }
 }
 
 // items per partner slot j: the real-space kernel leaves each item's
 // j-gradient in its own row of jpart, and the per-water reduction reads
 // the rows of water j in item order through this CSR
-__global__ void k_jcount(const WItem *__restrict__ items, size_t n, int32_t *__restrict__ cnt) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) atomicAdd(&cnt[items[q].j], 1);
+// (nothing when the predicate pass dropped records: the items are then
+// incomplete and the build is redone)
+__global__ void k_jcount(const WItem *__restrict__ items, const int32_t *__restrict__ nitem,
+                         size_t cap, const int32_t *__restrict__ dropped, int32_t *__restrict__ cnt) {
+  const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[items[q].j], 1);
 }
 // jpos[item] = the item's row in the partial buffer, inside its partner's
 // CSR range (the order within a range does not matter: the rows are
 // summed as integers)
-__global__ void k_jfill(const WItem *__restrict__ items, size_t n, const int32_t *__restrict__ start,
-                        int32_t *__restrict__ fill, int32_t *__restrict__ jpos) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int j = items[q].j;
-  jpos[q] = start[j] + atomicAdd(&fill[j], 1);
+__global__ void k_jfill(const WItem *__restrict__ items, const int32_t *__restrict__ nitem,
+                        size_t cap, const int32_t *__restrict__ dropped,
+                        const int32_t *__restrict__ start, int32_t *__restrict__ fill,
+                        int32_t *__restrict__ jpos) {
+  const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int j = items[q].j;
+    jpos[q] = start[j] + atomicAdd(&fill[j], 1);
+  }
 }
+
+// ---------------------------------------------------------------------------
+// Chunk tables on the device.  A chunk (one workgroup of the pair kernel)
+// is a run of one block's items (popcount-descending buckets, boff) holding
+// about `pairs` pairs, at least `min_items` items (one per lane) and at
+// most `max_items`.  One thread per block walks its items twice: to count
+// its chunks, then (at the scanned offset) to write them.
+__device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkPar P, int base,
+                                 int32_t *__restrict__ chunks, int32_t *__restrict__ cpairs) {
+  const int32_t *o = boff + (size_t)b * NBUCKET;
+  const int end = o[NBUCKET];
+  int q = o[0], n = 0, k = 0;
+  while (q < end) {
+    int e = q;
+    long pairs = 0;
+    while (e < end && (pairs < P.pairs || e - q < P.min_items) && e - q < P.max_items) {
+      while (o[k + 1] <= e) k++;  // bucket of item e: popcount WSPLIT - k
+      const int pc = WSPLIT - k;
+      const int nxt = min(o[k + 1], end);
+      const long want = max(max(1L, (long)(P.min_items - (e - q))),
+                            (P.pairs - pairs + pc - 1) / pc);
+      int take = (int)min((long)(nxt - e), want);
+      take = min(take, P.max_items - (e - q));
+      pairs += (long)take * pc;
+      e += take;
+    }
+    if (chunks && base + n < P.cap) {
+      chunks[3 * (size_t)(base + n)] = b;
+      chunks[3 * (size_t)(base + n) + 1] = q;
+      chunks[3 * (size_t)(base + n) + 2] = e;
+      if (cpairs) cpairs[base + n] = (int32_t)min(pairs, (long)INT32_MAX);
+    }
+    n++;
+    q = e;
+  }
+  return n;
+}
+
+__global__ void k_chunk_count(const int32_t *__restrict__ boff, ChunkPar P,
+                              int32_t *__restrict__ ccount) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < P.nblk) ccount[b] = chunk_walk(boff, b, P, 0, nullptr, nullptr);
+}
+
+__global__ void k_chunk_write(const int32_t *__restrict__ boff, const int32_t *__restrict__ cfirst,
+                              ChunkPar P, int32_t *__restrict__ chunks,
+                              int32_t *__restrict__ cpairs) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < P.nblk) chunk_walk(boff, b, P, cfirst[b], chunks, cpairs);
+}
+
+// dispatch order, largest chunks first: counting sort on the pair count in
+// steps of 32 (one workgroup; ties in any order -- the order changes only
+// the timing, each chunk writes its own partial rows)
+#define ORDER_KEYS 1024
+__global__ __launch_bounds__(1024) void k_chunk_order(const int32_t *__restrict__ nchunk_p,
+                                                      const int32_t *__restrict__ cpairs, int cap,
+                                                      int32_t *__restrict__ order) {
+  __shared__ int32_t cnt[ORDER_KEYS];
+  const int t = threadIdx.x;
+  const int n = min(*nchunk_p, cap);
+  cnt[t] = 0;
+  __syncthreads();
+  for (int q = t; q < n; q += 1024)
+    atomicAdd(&cnt[ORDER_KEYS - 1 - min(cpairs[q] >> 5, ORDER_KEYS - 1)], 1);
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int k = 0; k < ORDER_KEYS; k++) {
+      const int v = cnt[k];
+      cnt[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < n; q += 1024)
+    order[atomicAdd(&cnt[ORDER_KEYS - 1 - min(cpairs[q] >> 5, ORDER_KEYS - 1)], 1)] = q;
+}
+
+int32_t chunk_table(moil_ctx *c, const int32_t *d_boff, ChunkPar P, int32_t *d_ccount,
+                    int32_t *d_cfirst, int32_t *d_chunks, int32_t *d_cpairs, int32_t *d_order,
+                    hipStream_t st) {
+  const int TB = 256;
+  if (P.nblk <= 0) return MOIL_GPU_OK;
+  k_chunk_count<<<(P.nblk + TB - 1) / TB, TB, 0, st>>>(d_boff, P, d_ccount);
+  CHECK_SCAN(scan_excl(c, d_ccount, d_cfirst, P.nblk, st));
+  k_chunk_write<<<(P.nblk + TB - 1) / TB, TB, 0, st>>>(d_boff, d_cfirst, P, d_chunks, d_cpairs);
+  if (d_order) k_chunk_order<<<1, 1024, 0, st>>>(d_cfirst + P.nblk, d_cpairs, P.cap, d_order);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+// sizes written by the build into the status words (read once by the host)
+__global__ void k_copy_status(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+                              int32_t *__restrict__ sa, int32_t *__restrict__ sb) {
+  *sa = *a;
+  *sb = *b;
+}
+
+#define CHECK_RC_L(x)                    \
+  do {                                   \
+    const int32_t rc_ = (x);             \
+    if (rc_ != MOIL_GPU_OK) return rc_;  \
+  } while (0)
 
 // ---------------------------------------------------------------------------
 static void free_lists_buffers(moil_ctx *c) {
@@ -572,6 +690,7 @@ int32_t lists_init(moil_ctx *c) {
   const moil_params &P = c->par;
   HIP_CHECK(hipMalloc(&c->d_mono_cent, sizeof(double) * 3 * std::max(c->totmon, 1)));
   HIP_CHECK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
+  HIP_CHECK(hipMalloc(&c->d_status, sizeof(int32_t) * ST_COUNT));
   if (c->nwat > 0) {
     // cell grid over the box with ~8 waters per cell, even counts per
     // dimension are not required (bricks may be partial)
@@ -622,6 +741,10 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_jpart); c->d_jpart = nullptr;
   hipFree(c->d_ipart); c->d_ipart = nullptr;
   hipFree(c->d_cfirst); c->d_cfirst = nullptr;
+  hipFree(c->d_wccount); c->d_wccount = nullptr;
+  hipFree(c->d_wcpairs); c->d_wcpairs = nullptr;
+  hipFree(c->d_worder); c->d_worder = nullptr;
+  hipFree(c->d_status); c->d_status = nullptr;
 }
 
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
@@ -659,125 +782,111 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     double R = sqrt(std::max(std::max(P.cutvbig2, P.cutebig2), 0.0));
     wp.R = R * (1.0 + 1e-12) + 1e-9;
     wp.nwat = c->nwat;
-    // predicates once per candidate; records kept per block (capacity from
-    // the previous build, redone with the exact need if it overflowed)
+    // Everything below is enqueued without waiting: the predicate pass
+    // keeps its records per block (capacity from earlier builds), the items
+    // and the chunk table are written on the device, and every capacity
+    // overflow is reported in the status words that moil_gpu_build_lists_dev
+    // reads once at the end (the build is then redone with enough room).
     if (!c->d_rcount) HIP_CHECK(hipMalloc(&c->d_rcount, sizeof(int32_t) * (c->nwblk + 1)));
-    std::vector<int32_t> hoff(c->nwblk * NBUCKET + 1), hrc(c->nwblk);
-    for (int attempt = 0; attempt < 2; attempt++) {
-      if (!c->d_wraw) {
-        c->wraw_cap = std::max(c->wraw_cap, 1024);
-        HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk));
-      }
-      k_wblock_items<<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
-                                                      c->d_flags, c->d_bcount, c->d_wraw,
-                                                      c->wraw_cap, c->d_rcount);
-      CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * NBUCKET, st));
-      HIP_CHECK(hipMemcpyAsync(hoff.data(), c->d_boff, sizeof(int32_t) * hoff.size(),
-                               hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipMemcpyAsync(hrc.data(), c->d_rcount, sizeof(int32_t) * hrc.size(),
-                               hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipStreamSynchronize(st));
-      const int need = *std::max_element(hrc.begin(), hrc.end());
-      if (need <= c->wraw_cap) break;
-      hipFree(c->d_wraw);
-      c->d_wraw = nullptr;
-      c->wraw_cap = need + need / 4 + 256;
-    }
-    size_t total = (size_t)hoff.back();
-    if (total > c->witem_cap || c->d_witems == nullptr) {
-      hipFree(c->d_witems);
-      hipFree(c->d_jpos);
-      hipFree(c->d_jpart);
-      c->witem_cap = std::max<size_t>(total + total / 4 + 1024, 4096);
-      HIP_CHECK(hipMalloc(&c->d_witems, sizeof(WItem) * c->witem_cap));
-      HIP_CHECK(hipMalloc(&c->d_jpos, sizeof(int32_t) * c->witem_cap));
-      HIP_CHECK(hipMalloc(&c->d_jpart, sizeof(long long) * 9 * c->witem_cap));
-    }
-    c->nwitems = total;
-    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, c->d_boff,
-                                                    c->d_witems);
-    // chunk table: runs of a block's items (popcount-descending) holding
-    // about wchunk_pairs pairs and at least one item per lane of the
-    // workgroup, so workgroups carry similar work; [nchunk][3] followed by
-    // the dispatch order [nchunk]
-    std::vector<int32_t> ch;
-    std::vector<long> cpairs;
-    for (int b = 0; b < c->nwblk; b++) {
-      int q = hoff[b * NBUCKET];
-      const int end = hoff[(b + 1) * NBUCKET];
-      while (q < end) {
-        int e = q;
-        long pairs = 0;
-        while (e < end && (pairs < c->wchunk_pairs || e - q < LB_THREADS) &&
-               e - q < c->wchunk_max_items) {
-          // bucket of item e: its popcount is WSPLIT - k
-          int k = (int)(std::upper_bound(hoff.begin() + b * NBUCKET, hoff.begin() + (b + 1) * NBUCKET,
-                                         e) - (hoff.begin() + b * NBUCKET)) - 1;
-          const int pc = WSPLIT - k;
-          const int nxt = std::min(hoff[b * NBUCKET + k + 1], end);
-          const long want = std::max<long>(std::max<long>(1, LB_THREADS - (e - q)),
-                                           (c->wchunk_pairs - pairs + pc - 1) / pc);
-          int take = (int)std::min<long>(nxt - e, want);
-          take = std::min(take, c->wchunk_max_items - (e - q));
-          pairs += (long)take * pc;
-          e += take;
-        }
-        ch.push_back(b);
-        ch.push_back(q);
-        ch.push_back(e);
-        cpairs.push_back(pairs);
-        q = e;
-      }
-    }
-    c->nwchunk = (int)ch.size() / 3;
-    // dispatch order: largest chunks first (the tail is then made of small
-    // workgroups); chunk indices, and so the partial rows, are unchanged
-    std::vector<int32_t> order(c->nwchunk);
-    for (int q = 0; q < c->nwchunk; q++) order[q] = q;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return cpairs[x] > cpairs[y]; });
-    for (int q = 0; q < c->nwchunk; q++) ch.push_back(order[q]);
-    // first chunk of every block (chunks are numbered block by block)
-    std::vector<int32_t> cfirst(c->nwblk + 1, c->nwchunk);
-    for (int q = c->nwchunk - 1; q >= 0; q--) cfirst[ch[3 * q]] = q;
-    for (int b = c->nwblk - 1; b >= 0; b--) cfirst[b] = std::min(cfirst[b], cfirst[b + 1]);
-    if (c->nwchunk > 0) {
-      // grow-only buffers: no device free/allocate (which synchronize) in
-      // the steady state
-      if (c->nwchunk > c->wchunk_cap || !c->d_wchunk) {
-        hipFree(c->d_wchunk);
-        c->wchunk_cap = c->nwchunk + c->nwchunk / 4 + 64;
-        HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * 4 * (size_t)c->wchunk_cap));
-      }
-      HIP_CHECK(hipMemcpyAsync(c->d_wchunk, ch.data(), sizeof(int32_t) * ch.size(),
-                               hipMemcpyHostToDevice, st));
-    }
-    if (c->nwchunk > c->ipart_cap || c->d_ipart == nullptr) {
-      hipFree(c->d_ipart);
-      c->ipart_cap = std::max(c->nwchunk + c->nwchunk / 4 + 64, 256);
-      HIP_CHECK(hipMalloc(&c->d_ipart, sizeof(long long) * WBLOCK * 9 * (size_t)c->ipart_cap));
-    }
+    if (!c->d_wccount) HIP_CHECK(hipMalloc(&c->d_wccount, sizeof(int32_t) * (c->nwblk + 1)));
     if (!c->d_cfirst) HIP_CHECK(hipMalloc(&c->d_cfirst, sizeof(int32_t) * (c->nwblk + 1)));
-    HIP_CHECK(hipMemcpyAsync(c->d_cfirst, cfirst.data(), sizeof(int32_t) * cfirst.size(),
-                             hipMemcpyHostToDevice, st));
-    // row ranges of the partial buffer by partner slot j
     if (!c->d_jcnt) {
       HIP_CHECK(hipMalloc(&c->d_jcnt, sizeof(int32_t) * (c->nwat + 1)));
       HIP_CHECK(hipMalloc(&c->d_jstart, sizeof(int32_t) * (c->nwat + 1)));
     }
-    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
-    if (total > 0) {
-      const int ib = (int)((total + TB - 1) / TB);
-      k_jcount<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jcnt);
-      CHECK_SCAN(scan_excl(c, c->d_jcnt, c->d_jstart, c->nwat, st));
-      HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
-      k_jfill<<<ib, TB, 0, st>>>(c->d_witems, total, c->d_jstart, c->d_jcnt, c->d_jpos);
-    } else {
-      HIP_CHECK(hipMemsetAsync(c->d_jstart, 0, sizeof(int32_t) * (c->nwat + 1), st));
+    if (!c->d_wraw) {
+      c->wraw_cap = std::max(c->wraw_cap, 1024);
+      HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk));
     }
-    HIP_CHECK(hipStreamSynchronize(st));
+    if (!c->d_witems) {
+      c->witem_cap = std::max<size_t>(c->witem_cap, 4096);
+      HIP_CHECK(hipMalloc(&c->d_witems, sizeof(WItem) * c->witem_cap));
+      HIP_CHECK(hipMalloc(&c->d_jpos, sizeof(int32_t) * c->witem_cap));
+      HIP_CHECK(hipMalloc(&c->d_jpart, sizeof(long long) * 9 * c->witem_cap));
+    }
+    if (!c->d_wchunk) {
+      c->wchunk_cap = std::max(c->wchunk_cap, 1024);
+      HIP_CHECK(hipMalloc(&c->d_wchunk, sizeof(int32_t) * 3 * (size_t)c->wchunk_cap));
+      HIP_CHECK(hipMalloc(&c->d_wcpairs, sizeof(int32_t) * (size_t)c->wchunk_cap));
+      HIP_CHECK(hipMalloc(&c->d_worder, sizeof(int32_t) * (size_t)c->wchunk_cap));
+    }
+    k_wblock_items<<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
+                                                    c->d_flags, c->d_bcount, c->d_wraw,
+                                                    c->wraw_cap, c->d_rcount,
+                                                    c->d_status + ST_WRAW);
+    CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * NBUCKET, st));
+    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, c->d_boff,
+                                                    c->d_witems, c->witem_cap);
+    ChunkPar cp;
+    cp.nblk = c->nwblk;
+    cp.pairs = c->wchunk_pairs;
+    cp.max_items = c->wchunk_max_items;
+    cp.min_items = LB_THREADS;
+    cp.cap = c->wchunk_cap;
+    CHECK_RC_L(chunk_table(c, c->d_boff, cp, c->d_wccount, c->d_cfirst, c->d_wchunk, c->d_wcpairs,
+                           c->d_worder, st));
+    // row ranges of the partial buffer by partner slot j
+    const int32_t *d_nitem = c->d_boff + (size_t)c->nwblk * NBUCKET;
+    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
+    k_jcount<<<1024, TB, 0, st>>>(c->d_witems, d_nitem, c->witem_cap, c->d_status + ST_WRAW,
+                                  c->d_jcnt);
+    CHECK_SCAN(scan_excl(c, c->d_jcnt, c->d_jstart, c->nwat, st));
+    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
+    k_jfill<<<1024, TB, 0, st>>>(c->d_witems, d_nitem, c->witem_cap, c->d_status + ST_WRAW,
+                                 c->d_jstart, c->d_jcnt, c->d_jpos);
+    k_copy_status<<<1, 1, 0, st>>>(d_nitem, c->d_cfirst + c->nwblk, c->d_status + ST_WITEMS,
+                                   c->d_status + ST_WCHUNK);
+    HIP_CHECK(hipGetLastError());
   }
-  c->have_lists = true;
-  c->list_gen++;
   return MOIL_GPU_OK;
 }
+
+// capacities after a build whose status reported an overflow (false: the
+// build fitted); grown buffers are empty, the build must be redone
+bool lists_grow(moil_ctx *c, const int32_t *status) {
+  bool again = false;
+  if (c->nwat <= 0) return false;
+  if (status[ST_WRAW] > c->wraw_cap) {
+    hipFree(c->d_wraw);
+    c->d_wraw = nullptr;
+    c->wraw_cap = status[ST_WRAW] + status[ST_WRAW] / 4 + 256;
+    again = true;
+  }
+  if ((size_t)status[ST_WITEMS] > c->witem_cap) {
+    hipFree(c->d_witems);
+    hipFree(c->d_jpos);
+    hipFree(c->d_jpart);
+    c->d_witems = nullptr;
+    c->d_jpos = nullptr;
+    c->d_jpart = nullptr;
+    c->witem_cap = (size_t)status[ST_WITEMS] + status[ST_WITEMS] / 4 + 1024;
+    again = true;
+  }
+  if (status[ST_WCHUNK] > c->wchunk_cap) {
+    hipFree(c->d_wchunk);
+    hipFree(c->d_wcpairs);
+    hipFree(c->d_worder);
+    c->d_wchunk = c->d_wcpairs = c->d_worder = nullptr;
+    c->wchunk_cap = status[ST_WCHUNK] + status[ST_WCHUNK] / 4 + 64;
+    again = true;
+  }
+  return again;
+}
+
+// sizes of a build that fitted; per-chunk partial rows sized to match
+int32_t lists_finish(moil_ctx *c, const int32_t *status) {
+  if (c->nwat <= 0) {
+    c->nwitems = 0;
+    c->nwchunk = 0;
+    return MOIL_GPU_OK;
+  }
+  c->nwitems = (size_t)status[ST_WITEMS];
+  c->nwchunk = status[ST_WCHUNK];
+  if (c->nwchunk > c->ipart_cap || c->d_ipart == nullptr) {
+    hipFree(c->d_ipart);
+    c->ipart_cap = std::max(c->nwchunk + c->nwchunk / 4 + 64, 256);
+    HIP_CHECK(hipMalloc(&c->d_ipart, sizeof(long long) * WBLOCK * 9 * (size_t)c->ipart_cap));
+  }
+  return MOIL_GPU_OK;
+}
+

@@ -90,7 +90,8 @@ __device__ inline double block_sum(double v, double *red) {
 
 // FP32 (mixed precision) water kernel
 __device__ __forceinline__ void water_f32_impl(
-    const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
+    const int32_t *__restrict__ chunks, const int32_t *__restrict__ order,
+    const WItem *__restrict__ items,
     const int32_t *__restrict__ jpos, const float4 *__restrict__ wpos, int nwat, const WaterRS &P,
     long long *__restrict__ jpart, long long *__restrict__ ipart, double *__restrict__ epart,
     int max_parts, int bid, int nblk) {
@@ -98,7 +99,7 @@ __device__ __forceinline__ void water_f32_impl(
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = chunks[3 * nblk + bid], t = threadIdx.x;  // largest first
+  const int ch = order[bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   if (t < WBLOCK) {
@@ -223,21 +224,22 @@ __device__ __forceinline__ void water_f32_impl(
 }
 
 __device__ __forceinline__ void water_f32_body(const WaterArgs &A, int bid, int nblk) {
-  water_f32_impl(A.chunks, A.items, A.jpos, (const float4 *)A.wpos, A.nwat, A.P, A.jpart, A.ipart,
+  water_f32_impl(A.chunks, A.order, A.items, A.jpos, (const float4 *)A.wpos, A.nwat, A.P, A.jpart, A.ipart,
                  A.epart, A.max_parts, bid, nblk);
 }
 DEFINE_KERNEL(k_water_f32, WaterArgs, RS_THREADS, water_f32_body)
 
 // FP64 water kernel (MOIL parity mode)
 __device__ __forceinline__ void water_f64_impl(
-    const int32_t *__restrict__ chunks, const WItem *__restrict__ items,
+    const int32_t *__restrict__ chunks, const int32_t *__restrict__ order,
+    const WItem *__restrict__ items,
     const int32_t *__restrict__ jpos, const double *__restrict__ wpos, int nwat, const WaterRS &P,
     const double *__restrict__ erf_arr, long long *__restrict__ jpart, long long *__restrict__ ipart,
     double *__restrict__ epart, int max_parts, int bid, int nblk) {
   __shared__ double ipos[WBLOCK * 9];
   __shared__ long long iacc[WBLOCK * 9];
   __shared__ double red[RS_THREADS];
-  const int ch = chunks[3 * nblk + bid], t = threadIdx.x;  // largest first
+  const int ch = order[bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
@@ -344,7 +346,7 @@ __device__ __forceinline__ void water_f64_impl(
 }
 
 __device__ __forceinline__ void water_f64_body(const WaterArgs &A, int bid, int nblk) {
-  water_f64_impl(A.chunks, A.items, A.jpos, (const double *)A.wpos, A.nwat, A.P, A.erf, A.jpart,
+  water_f64_impl(A.chunks, A.order, A.items, A.jpos, (const double *)A.wpos, A.nwat, A.P, A.erf, A.jpart,
                  A.ipart, A.epart, A.max_parts, bid, nblk);
 }
 DEFINE_KERNEL(k_water_f64, WaterArgs, RS_THREADS, water_f64_body)
@@ -397,6 +399,7 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   P.fix_scale = c->fix_scale; P.ntab = c->ntab;
   WaterArgs &W = p->water;
   W.chunks = c->d_wchunk;
+  W.order = c->d_worder;
   W.items = c->d_witems;
   W.jpos = c->d_jpos;
   W.wpos = c->d_wpos;
