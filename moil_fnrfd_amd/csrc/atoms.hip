@@ -307,8 +307,12 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
           }
           ja = aslot2atom[acstart[cellk[lo]] + (f - cellpre[lo])];
           const double xj = coor[3 * ja], yj = coor[3 * ja + 1], zj = coor[3 * ja + 2];
-          bool jok = !(xj < q0[0] || xj > q1[0] || yj < q0[1] || yj > q1[1] || zj < q0[2] ||
-                       zj > q1[2]);
+          // j farther than the reach R from the block's bounding box (in
+          // j's frame) is farther than R from every block atom
+          const double ex = fmax(fmax(q0[0] + P.R - xj, xj - (q1[0] - P.R)), 0.0);
+          const double ey = fmax(fmax(q0[1] + P.R - yj, yj - (q1[1] - P.R)), 0.0);
+          const double ez = fmax(fmax(q0[2] + P.R - zj, zj - (q1[2] - P.R)), 0.0);
+          bool jok = ex * ex + ey * ey + ez * ez <= P.R * P.R;
           if (jok) {
             const int jm = mono_of[ja], jf = aflag[ja];
             const double cjx = mcent[3 * jm], cjy = mcent[3 * jm + 1], cjz = mcent[3 * jm + 2];

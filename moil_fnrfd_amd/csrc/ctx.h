@@ -24,7 +24,7 @@ struct moil_ctx {
   hipStream_t stream = nullptr;   // default stream of the context
   hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool pme_concurrent = false;    // MOIL_GPU_PME_CONCURRENT=1 in the environment
+  bool pme_concurrent = true;     // off with MOIL_GPU_PME_CONCURRENT=0 in the environment
   bool facc_dirty = false;
   int fix_shift = FIX_SHIFT_FP64;
   double fix_scale = 0, fix_inv = 0;
@@ -99,6 +99,8 @@ struct moil_ctx {
   int32_t *d_wcpairs = nullptr;    // [cap] pairs per chunk
   int32_t *d_worder = nullptr;     // [cap] dispatch order (largest first)
   int32_t *d_wccount = nullptr;    // [nwblk+1] chunks per block
+  int wnpart = 1;                  // workgroups per block in the predicate pass
+  int32_t *d_bpart = nullptr;      // [nwblk][wnpart][NBUCKET] item counts per part
   int nwchunk = 0, wchunk_cap = 0;
   // gradient partials of the real-space water kernel (no global atomics):
   // jpart[row][9] (an item's j-water; the rows of one water contiguous),

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     const double *__restrict__ scent, const int32_t *__restrict__ smono,
     const int32_t *__restrict__ cstart, Grid g, WPar P, const int32_t *__restrict__ flags,
     int32_t *__restrict__ bcount, WRaw *__restrict__ raw, int rcap, int32_t *__restrict__ rcount,
-    int32_t *__restrict__ rneed) {
+    int32_t *__restrict__ rneed, int npart) {
   __shared__ double ic[WBLOCK][3];
   __shared__ int32_t im[WBLOCK];
   __shared__ double bb[6];
@@ -345,9 +345,13 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   __shared__ int32_t rw[LB_THREADS / 64];
   __shared__ int32_t r_run;
 
-  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  // workgroup = (block, part): the part takes every npart-th batch of
+  // candidates of its block (batches numbered over translations and cells)
+  const int b = blockIdx.x / npart, part = blockIdx.x - b * npart;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int s0 = b * WBLOCK;
   const int nb = min(WBLOCK, P.nwat - s0);
+  int batch = 0;
   if (t < WBLOCK) {
     if (t < nb) {
       ic[t][0] = scent[3 * (s0 + t)];
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   }
   __syncthreads();
   const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
-  WRaw *rblk = raw + (size_t)b * rcap;
+  WRaw *rblk = raw + (size_t)blockIdx.x * rcap;
 
   for (int s = 0; s < 14; s++) {
     int ia, jb, kc;
@@ -446,6 +450,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
       __syncthreads();
       const int ncand = cellpre[LB_MAXCELLS];
       for (int fb = 0; fb < ncand; fb += LB_THREADS) {
+        if (batch++ % npart != part) continue;  // uniform over the workgroup
         const int f = fb + t;
         uint64_t m1 = 0, m2 = 0;
         int js = -1;
@@ -460,11 +465,13 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
           const double cjx = scent[3 * js], cjy = scent[3 * js + 1], cjz = scent[3 * js + 2];
           const int jm = smono[js];
           bool jok = !image || need_only_ok(cjx, cjy, cjz, da, db, dc, P);
-          // cheap reject: j's distance to the query box
+          // cheap reject: j farther than the reach R from the (translated)
+          // block bounding box is farther than R from every block water
           if (jok) {
-            if (cjx < q0[0] || cjx > q1[0] || cjy < q0[1] || cjy > q1[1] || cjz < q0[2] ||
-                cjz > q1[2])
-              jok = false;
+            const double ex = fmax(fmax(q0[0] + P.R - cjx, cjx - (q1[0] - P.R)), 0.0);
+            const double ey = fmax(fmax(q0[1] + P.R - cjy, cjy - (q1[1] - P.R)), 0.0);
+            const double ez = fmax(fmax(q0[2] + P.R - cjz, cjz - (q1[2] - P.R)), 0.0);
+            if (ex * ex + ey * ey + ez * ez > P.R * P.R) jok = false;
           }
           if (jok) {
             uint64_t iv = ivalid;
@@ -513,9 +520,9 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
     __syncthreads();
   }
   __syncthreads();
-  if (t < NBUCKET) bcount[b * NBUCKET + t] = L.bk_run[t];
+  if (t < NBUCKET) bcount[(size_t)blockIdx.x * NBUCKET + t] = L.bk_run[t];
   if (t == 0) {
-    rcount[b] = r_run;
+    rcount[blockIdx.x] = r_run;
     if (r_run > rcap) atomicMax(rneed, r_run);
   }
 }
@@ -524,27 +531,42 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
 // and writes its items at boff + the same deterministic ranks.
 __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restrict__ raw, int rcap,
                                                              const int32_t *__restrict__ rcount,
+                                                             int npart,
                                                              const int32_t *__restrict__ boff,
                                                              WItem *__restrict__ items, size_t cap) {
   __shared__ EmitLDS L;
   const int b = blockIdx.x, t = threadIdx.x;
   if (t < NBUCKET) L.bk_run[t] = 0;
   __syncthreads();
-  const int n = min(rcount[b], rcap);
-  const WRaw *rblk = raw + (size_t)b * rcap;
-  for (int fb = 0; fb < n; fb += LB_THREADS) {
-    const int f = fb + t;
-    int js = -1, s = 0;
-    uint64_t m1 = 0, m2 = 0;
-    if (f < n) {
-      const WRaw r = rblk[f];
-      js = r.j;
-      s = r.s;
-      m1 = r.m1;
-      m2 = r.m2;
+  for (int part = 0; part < npart; part++) {
+    const int slice = b * npart + part;
+    const int n = min(rcount[slice], rcap);
+    const WRaw *rblk = raw + (size_t)slice * rcap;
+    for (int fb = 0; fb < n; fb += LB_THREADS) {
+      const int f = fb + t;
+      int js = -1, s = 0;
+      uint64_t m1 = 0, m2 = 0;
+      if (f < n) {
+        const WRaw r = rblk[f];
+        js = r.j;
+        s = r.s;
+        m1 = r.m1;
+        m2 = r.m2;
+      }
+      emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items, cap);
     }
-    emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items, cap);
   }
+}
+
+// item counts per block and bucket: the sum over the block's parts
+__global__ void k_bucket_sum(const int32_t *__restrict__ bpart, int nblk, int npart,
+                             int32_t *__restrict__ bsum) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nblk * NBUCKET) return;
+  const int b = q / NBUCKET, k = q - b * NBUCKET;
+  int v = 0;
+  for (int p = 0; p < npart; p++) v += bpart[((size_t)b * npart + p) * NBUCKET + k];
+  bsum[q] = v;
 }
 
 // items per partner slot j: the real-space kernel leaves each item's
@@ -715,6 +737,9 @@ int32_t lists_init(moil_ctx *c) {
     HIP_CHECK(hipMalloc(&c->d_scent, sizeof(double) * 3 * c->nwat));
     HIP_CHECK(hipMalloc(&c->d_smono, sizeof(int32_t) * c->nwat));
     c->nwblk = (c->nwat + WBLOCK - 1) / WBLOCK;
+    // parts per block for the predicate pass: at least ~2048 workgroups
+    c->wnpart = std::min(8, std::max(1, (2048 + c->nwblk - 1) / c->nwblk));
+    HIP_CHECK(hipMalloc(&c->d_bpart, sizeof(int32_t) * ((size_t)c->nwblk * c->wnpart * NBUCKET + 1)));
     HIP_CHECK(hipMalloc(&c->d_bcount, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
     HIP_CHECK(hipMalloc(&c->d_boff, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
   }
@@ -745,6 +770,7 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_wcpairs); c->d_wcpairs = nullptr;
   hipFree(c->d_worder); c->d_worder = nullptr;
   hipFree(c->d_status); c->d_status = nullptr;
+  hipFree(c->d_bpart); c->d_bpart = nullptr;
 }
 
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
@@ -787,7 +813,8 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     // and the chunk table are written on the device, and every capacity
     // overflow is reported in the status words that moil_gpu_build_lists_dev
     // reads once at the end (the build is then redone with enough room).
-    if (!c->d_rcount) HIP_CHECK(hipMalloc(&c->d_rcount, sizeof(int32_t) * (c->nwblk + 1)));
+    const int np = c->wnpart;
+    if (!c->d_rcount) HIP_CHECK(hipMalloc(&c->d_rcount, sizeof(int32_t) * (c->nwblk * np + 1)));
     if (!c->d_wccount) HIP_CHECK(hipMalloc(&c->d_wccount, sizeof(int32_t) * (c->nwblk + 1)));
     if (!c->d_cfirst) HIP_CHECK(hipMalloc(&c->d_cfirst, sizeof(int32_t) * (c->nwblk + 1)));
     if (!c->d_jcnt) {
@@ -796,7 +823,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     }
     if (!c->d_wraw) {
       c->wraw_cap = std::max(c->wraw_cap, 1024);
-      HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk));
+      HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk * np));
     }
     if (!c->d_witems) {
       c->witem_cap = std::max<size_t>(c->witem_cap, 4096);
@@ -810,13 +837,15 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
       HIP_CHECK(hipMalloc(&c->d_wcpairs, sizeof(int32_t) * (size_t)c->wchunk_cap));
       HIP_CHECK(hipMalloc(&c->d_worder, sizeof(int32_t) * (size_t)c->wchunk_cap));
     }
-    k_wblock_items<<<c->nwblk, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
-                                                    c->d_flags, c->d_bcount, c->d_wraw,
-                                                    c->wraw_cap, c->d_rcount,
-                                                    c->d_status + ST_WRAW);
+    k_wblock_items<<<c->nwblk * np, LB_THREADS, 0, st>>>(d_scent, d_smono, c->d_cell_start, g, wp,
+                                                         c->d_flags, c->d_bpart, c->d_wraw,
+                                                         c->wraw_cap, c->d_rcount,
+                                                         c->d_status + ST_WRAW, np);
+    k_bucket_sum<<<(c->nwblk * NBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_bpart, c->nwblk, np,
+                                                                    c->d_bcount);
     CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * NBUCKET, st));
-    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, c->d_boff,
-                                                    c->d_witems, c->witem_cap);
+    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, np,
+                                                    c->d_boff, c->d_witems, c->witem_cap);
     ChunkPar cp;
     cp.nblk = c->nwblk;
     cp.pairs = c->wchunk_pairs;
