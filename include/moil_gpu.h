@@ -221,14 +221,21 @@ int32_t moil_gpu_dump_water_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
 int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
                                  int64_t *n);
 
-/* Stage timing with HIP events on the launch stream.  moil_gpu_set_timing
- * (which also clears) selects level 0 (off), 1 (the real-space kernels,
- * timed by events their own dispatch records: no extra work between
- * kernels) or 2 (also PME, bonded and the whole eforce, by event records
- * between stages); moil_gpu_stage_times returns the summed milliseconds
- * since the last call: t[0] real-space water kernel, t[1] real-space atom
- * kernel, t[2] PME, t[3] bonded + exclusion + 1-4, t[4] whole eforce,
- * t[5] the number of eforce calls timed. */
+/* Stage timing with HIP events on the launch stream.  Evaluations run in
+ * order on the launch stream.  With MOIL_GPU_GRAPH=1 in the environment
+ * (experimental, off by default: a process running ~40 recaptures in a
+ * parameter finite-difference loop aborted with the cause not identified)
+ * they are replays of a captured hipGraph of the whole evaluation,
+ * recaptured when the lists, the parameters or the pointers change.
+ * moil_gpu_set_timing (which also clears) selects level 0 (off), 1 (the
+ * real-space kernels are timed by events their own dispatch records; under
+ * graph replay only the first evaluation after each such change runs in
+ * order and is timed) or 2 (every evaluation in order, with event records
+ * between its stages: PME, bonded, the whole eforce); moil_gpu_stage_times
+ * returns the summed
+ * milliseconds since the last call: t[0] real-space water kernel, t[1]
+ * real-space atom kernel, t[2] PME, t[3] bonded + exclusion + 1-4, t[4]
+ * whole eforce, t[5] the number of eforce calls timed. */
 int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t level);
 int32_t moil_gpu_stage_times(moil_ctx *ctx, double t[8]);
 

@@ -104,8 +104,11 @@ static int32_t upload(T **dst, const T *src, size_t n) {
   return MOIL_GPU_OK;
 }
 
+void egraph_drop(moil_ctx *c);
+
 static void ctx_free(moil_ctx *c) {
   if (!c) return;
+  egraph_drop(c);
   lists_free(c);
   atoms_free(c);
   pme_free(c);
@@ -118,6 +121,9 @@ static void ctx_free(moil_ctx *c) {
                   c->d_xall, c->d_adj, c->d_scan_ws};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  if (c->gev_in) hipEventDestroy(c->gev_in);
+  if (c->gev_out) hipEventDestroy(c->gev_out);
+  if (c->gstream) hipStreamDestroy(c->gstream);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -181,6 +187,9 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->gev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->gev_out, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     ctx_free(c);
@@ -189,6 +198,8 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   {
     const char *e = std::getenv("MOIL_GPU_PME_CONCURRENT");
     c->pme_concurrent = !(e && e[0] == '0');
+    const char *gr = std::getenv("MOIL_GPU_GRAPH");
+    c->use_graph = gr && gr[0] == '1';
   }
   // ---- Ewald set-up ----
   double cutoff = std::sqrt(p->cutele2);
@@ -703,25 +714,12 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
   p->en = {c->d_epart, C, c->d_etot, c->d_ticket, d_energies ? d_energies : c->d_energy};
 }
 
-extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
-                                       double *d_energies, void *stream) {
-  if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
-  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "eforce called before build_lists");
-  hipStream_t st = pick_stream(c, stream);
-  EvalPlan p;
-  eval_plan(c, d_coor, d_dpot, d_energies, &p);
-  if (c->facc_dirty) {
-    HIP_CHECK(hipMemsetAsync(c->d_facc, 0, sizeof(long long) * 3 * (size_t)c->npt, st));
-    c->facc_dirty = false;
-  }
-  c->facc_dirty = true;  // until the finalize kernel is enqueued
+// One evaluation in order on stream st (eager path; timing level 2 puts
+// events between the stages, level 1 has the real-space kernels record
+// their own dispatch events).
+static int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   if (c->timing >= 1) c->n_timed++;
   StageTimer whole(c, 4, st);
-  // reciprocal space on a second stream, overlapped with bonded + real
-  // space (the latency-bound PME kernels take workgroup slots as the
-  // real-space kernel's drain: 0.24 instead of 0.27 ms per evaluation of the
-  // 98k-atom water box); MOIL_GPU_PME_CONCURRENT=0 keeps it in order on
-  // the launch stream
   if (p.pme && c->pme_concurrent) {
     HIP_CHECK(hipEventRecord(c->ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
@@ -742,8 +740,6 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   }
   if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
   else launch_k_pack_water_f32(p.pack, p.n_pack, st);
-  // the real-space kernels are timed at any timing level by events that
-  // their own dispatch records (no extra packets between kernels)
   hipEvent_t ea = nullptr, eb = nullptr;
   if (c->timing >= 1 && p.n_water > 0) {
     ea = ev_get(c);
@@ -762,9 +758,111 @@ extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double
   launch_k_atoms_reduce(p.ared, p.n_ared, st);
   if (p.pme && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
   launch_k_finalize_forces(p.fin, p.n_fin, st);
-  c->facc_dirty = false;
   launch_k_energies(p.en, EN_COUNT, st);
   HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+// The same sequence into a graph being captured on the context's graph
+// stream (no events: event nodes of a graph carry no timestamps here).
+static int32_t eforce_capture_body(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
+  if (p.pme && c->pme_concurrent) {
+    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    CHECK_RC(pme_run(c, p, c->stream2));
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+  } else if (p.pme) {
+    CHECK_RC(pme_run(c, p, st));
+  }
+  launch_k_water_intra(p.intra, p.n_intra, st);
+  launch_k_bonded(p.bonded, p.n_bonded, st);
+  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
+  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
+  if (p.f64) launch_k_water_f64(p.water, p.n_water, st);
+  else launch_k_water_f32(p.water, p.n_water, st);
+  launch_k_water_reduce(p.wred, p.n_wred, st);
+  if (p.f64) launch_k_atoms_f64(p.atoms, p.n_atoms, st);
+  else launch_k_atoms_f32(p.atoms, p.n_atoms, st);
+  launch_k_atoms_reduce(p.ared, p.n_ared, st);
+  if (p.pme && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  launch_k_finalize_forces(p.fin, p.n_fin, st);
+  launch_k_energies(p.en, EN_COUNT, st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+// a replay of the old graph may still be in flight on the graph stream: it
+// ends before the exec is destroyed
+void egraph_drop(moil_ctx *c) {
+  if (c->eg.exec && c->gstream) hipStreamSynchronize(c->gstream);
+  if (c->eg.exec) hipGraphExecDestroy(c->eg.exec);
+  c->eg.exec = nullptr;
+}
+
+extern "C" int32_t moil_gpu_eforce_dev(moil_ctx *c, const double *d_coor, double *d_dpot,
+                                       double *d_energies, void *stream) {
+  if (!c || !d_coor || !d_dpot) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "eforce called before build_lists");
+  hipStream_t st = pick_stream(c, stream);
+  if (c->facc_dirty) {
+    HIP_CHECK(hipMemsetAsync(c->d_facc, 0, sizeof(long long) * 3 * (size_t)c->npt, st));
+    c->facc_dirty = false;
+  }
+  EvalPlan p;
+  eval_plan(c, d_coor, d_dpot, d_energies, &p);
+  // the accumulator is dirty until an evaluation's finalize is enqueued
+  c->facc_dirty = true;
+  if (c->timing >= 2 || !c->use_graph) {
+    CHECK_RC(eforce_eager(c, p, st));
+    c->facc_dirty = false;
+    return MOIL_GPU_OK;
+  }
+  // Replay of a captured evaluation: one submission instead of ~20 kernel
+  // launches and two transforms.  The graph is recaptured when the lists,
+  // the parameters or the pointers changed.  At timing level 1 the first
+  // evaluation after such a change runs in order with the real-space
+  // kernels' dispatch events (the timed sample); the replays are untimed.
+  EGraph &G = c->eg;
+  const void *key[3] = {d_coor, d_dpot, d_energies};
+  bool fresh = G.exec && G.gen == c->list_gen && G.pgen == c->param_gen;
+  for (int k = 0; fresh && k < 3; k++) fresh = G.key[k] == key[k];
+  if (!fresh && c->timing >= 1 && c->timed_gen != c->list_gen) {
+    c->timed_gen = c->list_gen;
+    CHECK_RC(eforce_eager(c, p, st));
+    c->facc_dirty = false;
+    return MOIL_GPU_OK;
+  }
+  if (!fresh) {
+    egraph_drop(c);
+    HIP_CHECK(hipEventRecord(c->gev_in, st));
+    HIP_CHECK(hipStreamWaitEvent(c->gstream, c->gev_in, 0));
+    HIP_CHECK(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
+    const int32_t rc = eforce_capture_body(c, p, c->gstream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->gstream, &g);
+    if (rc != MOIL_GPU_OK || e != hipSuccess || !g) {
+      if (g) hipGraphDestroy(g);
+      // capture not available: in order from now on
+      c->use_graph = false;
+      hipGetLastError();
+      CHECK_RC(rc);
+      CHECK_RC(eforce_eager(c, p, st));
+      c->facc_dirty = false;
+      return MOIL_GPU_OK;
+    }
+    const hipError_t ei = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIP_CHECK(ei);
+    G.gen = c->list_gen;
+    G.pgen = c->param_gen;
+    for (int k = 0; k < 3; k++) G.key[k] = key[k];
+  }
+  HIP_CHECK(hipEventRecord(c->gev_in, st));
+  HIP_CHECK(hipStreamWaitEvent(c->gstream, c->gev_in, 0));
+  HIP_CHECK(hipGraphLaunch(G.exec, c->gstream));
+  HIP_CHECK(hipEventRecord(c->gev_out, c->gstream));
+  HIP_CHECK(hipStreamWaitEvent(st, c->gev_out, 0));
+  c->facc_dirty = false;
   return MOIL_GPU_OK;
 }
 
@@ -895,6 +993,7 @@ extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t 
 extern "C" int32_t moil_gpu_set_timing(moil_ctx *c, int32_t level) {
   if (!c) return fail(MOIL_GPU_EINVAL, "null context");
   c->timing = level < 0 ? 0 : (level > 2 ? 2 : level);
+  c->timed_gen = -1;
   c->ev_rec.clear();
   c->ev_next = 0;
   c->n_timed = 0;
@@ -970,6 +1069,7 @@ extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const d
   c->v14 = v14;
   c->el14 = el14;
   c->have_scale14 = true;
+  c->param_gen++;  // captured evaluations hold the old derived parameters
   return MOIL_GPU_OK;
 }
 

@@ -106,6 +106,7 @@ struct moil_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<long> gen;           // list generation captured
+  std::vector<long> pgen;          // parameter generation captured
   std::vector<const void *> key;   // pointers captured
   std::vector<FftGroup> groups;
   std::vector<int> group_of, slot_of;  // per system: its group, index in it
@@ -114,7 +115,10 @@ struct moil_batch {
   size_t blob_cap = 0;
 };
 
+// a replay of the old graph may still be in flight on the capture stream:
+// it ends before its exec (and the tables it reads) are replaced
 static void batch_drop_graph(moil_batch *b) {
+  if (b->exec && b->cap) hipStreamSynchronize(b->cap);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -398,9 +402,10 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
   }
   hipStream_t st = (hipStream_t)stream;
   // (re)capture when lists or pointers changed
-  bool fresh = b->exec != nullptr && (int)b->gen.size() == n;
+  bool fresh = b->exec != nullptr && (int)b->gen.size() == n && (int)b->pgen.size() == n;
   for (int i = 0; fresh && i < n; i++)
-    fresh = b->gen[i] == b->ctx[i]->list_gen && b->key[3 * i] == d_coor[i] &&
+    fresh = b->gen[i] == b->ctx[i]->list_gen && b->pgen[i] == b->ctx[i]->param_gen &&
+            b->key[3 * i] == d_coor[i] &&
             b->key[3 * i + 1] == d_dpot[i] && b->key[3 * i + 2] == d_energies[i];
   HIP_CHECK(hipEventRecord(b->ev_in, st));
   HIP_CHECK(hipStreamWaitEvent(b->cap, b->ev_in, 0));
@@ -426,9 +431,11 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
     b->graph = g;
     HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->gen.resize(n);
+    b->pgen.resize(n);
     b->key.resize(3 * n);
     for (int i = 0; i < n; i++) {
       b->gen[i] = b->ctx[i]->list_gen;
+      b->pgen[i] = b->ctx[i]->param_gen;
       b->key[3 * i] = d_coor[i];
       b->key[3 * i + 1] = d_dpot[i];
       b->key[3 * i + 2] = d_energies[i];

@@ -8,6 +8,13 @@
 
 #include "common.h"
 
+// a captured evaluation (api.hip): graph exec and what it was captured for
+struct EGraph {
+  hipGraphExec_t exec = nullptr;
+  long gen = -1, pgen = -1;
+  const void *key[3] = {nullptr, nullptr, nullptr};
+};
+
 template <typename T>
 struct DBuf {
   T *p = nullptr;
@@ -180,6 +187,13 @@ struct moil_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;  // stage, (start, stop)
   size_t ev_next = 0;
+  // ---- captured evaluations (api.hip) ----
+  bool use_graph = false;          // on with MOIL_GPU_GRAPH=1 in the environment
+  EGraph eg;
+  hipStream_t gstream = nullptr;   // graph launch stream
+  hipEvent_t gev_in = nullptr, gev_out = nullptr;
+  long timed_gen = -1;             // list generation of the last timed in-order call
+  long param_gen = 0;              // incremented by set_params
 };
 
 // exclusive scan of n int32 into out[0..n] (out[n] = total), on stream st
