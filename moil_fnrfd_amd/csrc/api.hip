@@ -201,6 +201,17 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     const char *gr = std::getenv("MOIL_GPU_GRAPH");
     c->use_graph = gr && gr[0] == '1';
   }
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      ctx_free(c);
+      return fail(MOIL_GPU_EHIP, "device query failed");
+    }
+    const char *w = std::getenv("MOIL_GPU_WATER_WG_PER_CU");
+    if (w && w[0]) c->water_wg_per_cu = std::max(0, std::atoi(w));
+    c->water_wgs = c->water_wg_per_cu * ncu;
+  }
   // ---- Ewald set-up ----
   double cutoff = std::sqrt(p->cutele2);
   c->beta = ewald_coefficient(cutoff, p->dtol);
@@ -646,6 +657,16 @@ __device__ __forceinline__ void energies_body(const EnArgs &A, int bid, int nblk
 }
 DEFINE_KERNEL(k_energies, EnArgs, 256, energies_body)
 
+// force assembly and the energy record in one launch: workgroups
+// [0, n_fin) finalize the gradient, the next EN_COUNT reduce one energy
+// term each (the two halves touch disjoint memory)
+__device__ __forceinline__ void assemble_body(const AsmArgs &A, int bid, int nblk) {
+  if (bid < A.n_fin) finalize_impl(A.fin.facc, A.fin.fpme, A.fin.has_pme, A.fin.n, A.fin.inv,
+                                   A.fin.dpot, bid);
+  else energies_impl(A.en.ep, A.en.C, A.en.tot, A.en.ticket, A.en.out, bid - A.n_fin, EN_COUNT);
+}
+DEFINE_KERNEL(k_assemble, AsmArgs, 256, assemble_body)
+
 // stage timing: event pairs recorded on the launch stream, summed lazily
 static hipEvent_t ev_get(moil_ctx *c) {
   if (c->ev_next == c->ev_pool.size()) {
@@ -712,16 +733,40 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
   C.has_nb = nb;
   C.has_14 = c->totspe > 0 && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14));
   p->en = {c->d_epart, C, c->d_etot, c->d_ticket, d_energies ? d_energies : c->d_energy};
+  p->asmb = {p->fin, p->n_fin, p->en};
 }
 
 // One evaluation in order on stream st (eager path; timing level 2 puts
 // events between the stages, level 1 has the real-space kernels record
-// their own dispatch events).
+// their own dispatch events).  The packed water positions and the long
+// real-space pair kernels are submitted first; the reciprocal sum goes to
+// the second stream after them (forked at the evaluation's start, so it
+// waits only for the coordinates) and runs beside them; the intra-water
+// and bonded terms and the partial-row reductions follow on st, in the
+// shadow of the reciprocal sum.  The intra-water terms store (not add)
+// the water atoms' accumulator rows, so they precede both reductions.
 static int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   if (c->timing >= 1) c->n_timed++;
   StageTimer whole(c, 4, st);
-  if (p.pme && c->pme_concurrent) {
-    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+  const bool fork = p.pme && c->pme_concurrent;
+  if (fork) HIP_CHECK(hipEventRecord(c->ev_fork, st));
+  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
+  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (c->timing >= 1 && p.n_water > 0) {
+    ea = ev_get(c);
+    eb = ev_get(c);
+  }
+  launch_k_water_ev(p.water, p.f64, p.n_water, st, ea, eb, c->water_wgs);
+  if (ea && eb) c->ev_rec.push_back({0, {ea, eb}});
+  ea = eb = nullptr;
+  if (c->timing >= 1 && p.n_atoms > 0) {
+    ea = ev_get(c);
+    eb = ev_get(c);
+  }
+  launch_k_atoms_ev(p.atoms, p.f64, p.n_atoms, st, ea, eb);
+  if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
+  if (fork) {
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     {
       StageTimer tp(c, 2, c->stream2);
@@ -738,27 +783,10 @@ static int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
     launch_k_bonded(p.bonded, p.n_bonded, st);
     HIP_CHECK(hipGetLastError());
   }
-  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
-  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
-  hipEvent_t ea = nullptr, eb = nullptr;
-  if (c->timing >= 1 && p.n_water > 0) {
-    ea = ev_get(c);
-    eb = ev_get(c);
-  }
-  launch_k_water_ev(p.water, p.f64, p.n_water, st, ea, eb);
-  if (ea && eb) c->ev_rec.push_back({0, {ea, eb}});
   launch_k_water_reduce(p.wred, p.n_wred, st);
-  ea = eb = nullptr;
-  if (c->timing >= 1 && p.n_atoms > 0) {
-    ea = ev_get(c);
-    eb = ev_get(c);
-  }
-  launch_k_atoms_ev(p.atoms, p.f64, p.n_atoms, st, ea, eb);
-  if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
   launch_k_atoms_reduce(p.ared, p.n_ared, st);
-  if (p.pme && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
-  launch_k_finalize_forces(p.fin, p.n_fin, st);
-  launch_k_energies(p.en, EN_COUNT, st);
+  if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  launch_k_assemble(p.asmb, p.n_fin + EN_COUNT, st);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }
@@ -766,8 +794,14 @@ static int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
 // The same sequence into a graph being captured on the context's graph
 // stream (no events: event nodes of a graph carry no timestamps here).
 static int32_t eforce_capture_body(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
-  if (p.pme && c->pme_concurrent) {
-    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+  const bool fork = p.pme && c->pme_concurrent;
+  if (fork) HIP_CHECK(hipEventRecord(c->ev_fork, st));
+  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
+  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
+  launch_k_water_ev(p.water, p.f64, p.n_water, st, nullptr, nullptr, c->water_wgs);
+  if (p.f64) launch_k_atoms_f64(p.atoms, p.n_atoms, st);
+  else launch_k_atoms_f32(p.atoms, p.n_atoms, st);
+  if (fork) {
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     CHECK_RC(pme_run(c, p, c->stream2));
     HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
@@ -776,17 +810,10 @@ static int32_t eforce_capture_body(moil_ctx *c, const EvalPlan &p, hipStream_t s
   }
   launch_k_water_intra(p.intra, p.n_intra, st);
   launch_k_bonded(p.bonded, p.n_bonded, st);
-  if (p.f64) launch_k_pack_water_f64(p.pack, p.n_pack, st);
-  else launch_k_pack_water_f32(p.pack, p.n_pack, st);
-  if (p.f64) launch_k_water_f64(p.water, p.n_water, st);
-  else launch_k_water_f32(p.water, p.n_water, st);
   launch_k_water_reduce(p.wred, p.n_wred, st);
-  if (p.f64) launch_k_atoms_f64(p.atoms, p.n_atoms, st);
-  else launch_k_atoms_f32(p.atoms, p.n_atoms, st);
   launch_k_atoms_reduce(p.ared, p.n_ared, st);
-  if (p.pme && c->pme_concurrent) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
-  launch_k_finalize_forces(p.fin, p.n_fin, st);
-  launch_k_energies(p.en, EN_COUNT, st);
+  if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  launch_k_assemble(p.asmb, p.n_fin + EN_COUNT, st);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

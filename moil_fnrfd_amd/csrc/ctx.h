@@ -119,6 +119,10 @@ struct moil_ctx {
   int32_t *d_jpos = nullptr;       // [item] row of the item in jpart
   int32_t *d_cfirst = nullptr;     // [nwblk+1] first chunk of each block
   int wchunk_pairs = 4096;         // pair budget of one real-space workgroup
+  // single-system water kernel: workgroups per CU of the short-grid
+  // launch (0: one workgroup per chunk) and the resulting grid
+  int water_wg_per_cu = 4;
+  int water_wgs = 0;
   int wchunk_max_items = 2048;
   // packed per-slot water coordinates for the real-space kernel
   void *d_wpos = nullptr;          // FP32: 3 float4 per slot; FP64: 9 double per slot

@@ -35,13 +35,14 @@ struct WaterArgs {
   int nwat;
   WaterRS P;
   const double *erf;  // FP64 only
-  long long *jpart, *ipart;
+  long long *jpart, *ipart;  // jpart rows: float[9] (mixed) or int64[9] (FP64)
   double *epart;
   int max_parts;
 };
 struct WReduceArgs {
   const int32_t *jstart;
-  const long long *jpart;
+  const long long *jpart;  // float[9] rows (mixed) or fixed-point int64[9] (FP64)
+  int jf32;
   const int32_t *cfirst;
   const long long *ipart;
   const int32_t *slot_o;
@@ -204,10 +205,15 @@ struct EnArgs {
   unsigned int *ticket;
   double *out;
 };
+struct AsmArgs {
+  FinArgs fin;
+  int n_fin;
+  EnArgs en;
+};
 
-// Every launch of one evaluation of one system, in order: reciprocal
-// space, intra-water + generic bonded terms, water real space, atom real
-// space, force and energy assembly.  A count of 0 skips the launch.
+// Every launch of one evaluation of one system: reciprocal space,
+// intra-water + generic bonded terms, water real space, atom real space,
+// force and energy assembly.  A count of 0 skips the launch.
 struct EvalPlan {
   bool f64 = false;
   bool pme = false;
@@ -239,6 +245,7 @@ struct EvalPlan {
   FinArgs fin;
   int n_fin = 0;
   EnArgs en;
+  AsmArgs asmb;  // fin + en in one launch of n_fin + EN_COUNT workgroups
 };
 
 // plan builders (one per module; the caller fills d_coor / outputs)
@@ -278,12 +285,14 @@ DECLARE_LAUNCH(k_gather_f32, GatherArgs)
 DECLARE_LAUNCH(k_gather_f64, GatherArgs)
 DECLARE_LAUNCH(k_finalize_forces, FinArgs)
 DECLARE_LAUNCH(k_energies, EnArgs)
+DECLARE_LAUNCH(k_assemble, AsmArgs)
 #undef DECLARE_LAUNCH
 
 // the real-space kernels with dispatch-recorded start/stop events (either
-// may be null)
+// may be null); the water kernel optionally on a short grid of nwg
+// workgroups
 void launch_k_water_ev(const WaterArgs &a, bool f64, int nblk, hipStream_t st, hipEvent_t e0,
-                       hipEvent_t e1);
+                       hipEvent_t e1, int nwg = 0);
 void launch_k_atoms_ev(const AtomArgs &a, bool f64, int nblk, hipStream_t st, hipEvent_t e0,
                        hipEvent_t e1);
 

@@ -198,9 +198,11 @@ __device__ __forceinline__ void water_f32_impl(
 #pragma unroll
       for (int k = 0; k < 9; k++) atomicAdd(&iacc[i * 9 + k], (unsigned long long)fix32(fi[k]));
     }
-    long long *jrow = jpart + (size_t)jpos[it] * 9;
+    // the partner's row holds the FP32 sums; the reduction converts them
+    // to fixed point (the same fix32 of the same value, half the bytes)
+    float *jrow = reinterpret_cast<float *>(jpart) + (size_t)jpos[it] * 9;
 #pragma unroll
-    for (int k = 0; k < 9; k++) jrow[k] = fix32(fj[k]);
+    for (int k = 0; k < 9; k++) jrow[k] = fj[k];
     if (item.s == 0) {
       e_lj_r += (double)elj;
       e_el_r += (double)eel;
@@ -352,12 +354,13 @@ __device__ __forceinline__ void water_f64_body(const WaterArgs &A, int bid, int 
 DEFINE_KERNEL(k_water_f64, WaterArgs, RS_THREADS, water_f64_body)
 
 // one thread per (water slot, gradient component): the slot's partials --
-// its jpart rows (contiguous at the slot's CSR range) and its entries in
-// the ipart rows of its block's chunks, all 64-bit fixed point -- summed
+// its jpart rows (contiguous at the slot's CSR range; FP32 values turned
+// into 64-bit fixed point here in mixed precision, fixed point already in
+// FP64) and its entries in the ipart rows of its block's chunks -- summed
 // as integers (order independent) and added to the accumulator.  The
 // slot's atoms are touched by no other thread, so no atomics.
 __device__ __forceinline__ void water_reduce_impl(
-    const int32_t *__restrict__ jstart, const long long *__restrict__ jpart,
+    const int32_t *__restrict__ jstart, const long long *__restrict__ jpart, int jf32,
     const int32_t *__restrict__ cfirst, const long long *__restrict__ ipart,
     const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc, int bid) {
   const int g = bid * 256 + threadIdx.x;
@@ -366,17 +369,26 @@ __device__ __forceinline__ void water_reduce_impl(
   long long a0 = 0, a1 = 0;
   const int q0 = jstart[s], q1 = jstart[s + 1];
   int q = q0;
-  for (; q + 1 < q1; q += 2) {
-    a0 += jpart[(size_t)q * 9 + k];
-    a1 += jpart[(size_t)(q + 1) * 9 + k];
+  if (jf32) {
+    const float *jp = reinterpret_cast<const float *>(jpart);
+    for (; q + 1 < q1; q += 2) {
+      a0 += fix32(jp[(size_t)q * 9 + k]);
+      a1 += fix32(jp[(size_t)(q + 1) * 9 + k]);
+    }
+    if (q < q1) a0 += fix32(jp[(size_t)q * 9 + k]);
+  } else {
+    for (; q + 1 < q1; q += 2) {
+      a0 += jpart[(size_t)q * 9 + k];
+      a1 += jpart[(size_t)(q + 1) * 9 + k];
+    }
+    if (q < q1) a0 += jpart[(size_t)q * 9 + k];
   }
-  if (q < q1) a0 += jpart[(size_t)q * 9 + k];
   const int b = s / WBLOCK, i = s % WBLOCK;
   for (int c = cfirst[b]; c < cfirst[b + 1]; c++) a1 += ipart[(size_t)c * (WBLOCK * 9) + i * 9 + k];
   facc[3 * (size_t)slot_o[s] + k] += a0 + a1;
 }
 __device__ __forceinline__ void water_reduce_body(const WReduceArgs &A, int bid, int nblk) {
-  water_reduce_impl(A.jstart, A.jpart, A.cfirst, A.ipart, A.slot_o, A.nwat, A.facc, bid);
+  water_reduce_impl(A.jstart, A.jpart, A.jf32, A.cfirst, A.ipart, A.slot_o, A.nwat, A.facc, bid);
 }
 DEFINE_KERNEL(k_water_reduce, WReduceArgs, 256, water_reduce_body)
 
@@ -414,6 +426,7 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   WReduceArgs &R = p->wred;
   R.jstart = c->d_jstart;
   R.jpart = c->d_jpart;
+  R.jf32 = c->par.precision == MOIL_PREC_MIXED;
   R.cfirst = c->d_cfirst;
   R.ipart = c->d_ipart;
   R.slot_o = c->d_slot_o;
@@ -422,11 +435,38 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   p->n_wred = (9 * c->nwat + 255) / 256;
 }
 
+// Single-system form on a short grid (mixed precision): fewer workgroups
+// than chunks -- a fixed number per CU, below the CU's capacity for this
+// kernel (six at its register budget) -- walk the chunks in rounds of
+// gridDim.x, alternating direction every round (the chunks are in
+// dispatch order, largest first, so a workgroup that takes a large chunk
+// in one round takes a small one in the next).  The slots left free on
+// every CU let the reciprocal-space kernels of the same evaluation, on
+// the second stream, run beside it instead of after it.  The chunk index
+// depends only on blockIdx.x, gridDim.x and the round, so the loop and
+// its barriers are uniform over the workgroup.  Which workgroup runs a
+// chunk does not matter: each chunk writes its own partial rows and
+// energy partials.
+__global__ __launch_bounds__(RS_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
+void k_water_rounds(WaterArgs A, int nchunk) {
+  const int G = gridDim.x, b = blockIdx.x;
+  for (int r = 0; r * G < nchunk; r++) {
+    const int k = r * G + ((r & 1) ? G - 1 - b : b);
+    if (k < nchunk) water_f32_body(A, k, nchunk);
+  }
+}
+
 // launched through hipExtLaunchKernelGGL so that optional start/stop
-// events are recorded by the dispatch itself
+// events are recorded by the dispatch itself; mixed precision on nwg
+// workgroups (k_water_rounds) when 0 < nwg < nblk, else one workgroup per
+// chunk
 void launch_k_water_ev(const WaterArgs &a, bool f64, int nblk, hipStream_t st, hipEvent_t e0,
-                       hipEvent_t e1) {
+                       hipEvent_t e1, int nwg) {
   if (nblk <= 0) return;
+  if (!f64 && nwg > 0 && nwg < nblk) {
+    hipExtLaunchKernelGGL(k_water_rounds, dim3(nwg), dim3(RS_THREADS), 0, st, e0, e1, 0, a, nblk);
+    return;
+  }
   if (f64)
     hipExtLaunchKernelGGL(k_water_f64, dim3(nblk), dim3(RS_THREADS), 0, st, e0, e1, 0, a);
   else

@@ -42,6 +42,14 @@ def load_library():
     path = os.environ.get('MOIL_GPU_LIBRARY', LIB_PATH)
     if not os.path.exists(path):
         raise MoilGPUError('libmoil_gpu.so not found at %s: run __graft_entry__.build()' % path)
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7
+    # (the soname this library links), and whichever copy loads first
+    # serves both.  PyTorch's initialisation fails on the system copy, so
+    # it is loaded first whenever it is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     vp = C.c_void_p
     L.moil_gpu_abi_version.restype = C.c_int32

@@ -106,6 +106,28 @@ def test_mixed_precision_32k(gpu):
     assert np.array_equal(f32, f32b) and e32 == e32b
 
 
+def test_short_grid_water_kernel_matches_plain(gpu, monkeypatch):
+    """The mixed-precision water kernel on a short grid (a fixed number of
+    workgroups per CU walking the chunks in alternating rounds) against the
+    one-workgroup-per-chunk launch: every chunk writes the same partials
+    either way, so forces and energies are identical bit for bit."""
+    s, cut, ew = configs.water32k()
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_MIXED)
+    out = {}
+    for wg in ('0', '1', '5'):
+        _say('case water32k MIXED water workgroups per CU %s' % wg)
+        monkeypatch.setenv('MOIL_GPU_WATER_WG_PER_CU', wg)
+        m = gpu(s.topo, p)
+        m.nbondm(s.coor)
+        out[wg] = m.eforce(s.coor)
+        m.close()
+    e0, f0 = out['0']
+    for wg in ('1', '5'):
+        e, f = out[wg]
+        assert np.array_equal(f, f0), wg
+        assert e == e0, wg
+
+
 def test_lists_follow_perturbed_coordinates(gpu):
     """List rebuilds on perturbed / drifting coordinates (incl. squeeze)
     reproduce the oracle's pair sets exactly."""
