@@ -284,7 +284,7 @@ static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *
       p.conv.S = qc;
       p.gather.grid = qr;
       T.tbin.add(p.tbin, p.n_tbin);
-      T.tscan.add(p.tscan, 1);
+      T.tscan.add(p.tscan, p.tbin.fuse_scan ? 0 : 1);
       T.tscat.add(p.tscat, p.n_tbin);
       T.tspread.add(p.tspread, p.n_tile);
       T.tsum.add(p.tsum, p.n_tsum);

@@ -139,6 +139,8 @@ struct TbinArgs {
   PmePar P;
   TilePar T;
   int32_t *tcnt, *atile, *arank;
+  int32_t *tstart;   // scan output when the last workgroup scans (fuse_scan)
+  int fuse_scan;     // 1: the last workgroup scans the counts (tcnt[ntile] is its ticket)
 };
 struct TscanArgs {
   int32_t *cnt;

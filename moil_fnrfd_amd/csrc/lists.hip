@@ -604,23 +604,16 @@ __global__ void k_jfill(const WItem *__restrict__ items, const int32_t *__restri
 // its chunks, then (at the scanned offset) to write them.
 __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkPar P, int base,
                                  int32_t *__restrict__ chunks, int32_t *__restrict__ cpairs) {
-  const int32_t *o = boff + (size_t)b * NBUCKET;
+  // the block's bucket offsets, loaded once (independent loads); bucket k
+  // holds items of popcount WSPLIT - k
+  const int32_t *op = boff + (size_t)b * NBUCKET;
+  int o[NBUCKET + 1];
+#pragma unroll
+  for (int k = 0; k <= NBUCKET; k++) o[k] = op[k];
   const int end = o[NBUCKET];
-  int q = o[0], n = 0, k = 0;
-  while (q < end) {
-    int e = q;
-    long pairs = 0;
-    while (e < end && (pairs < P.pairs || e - q < P.min_items) && e - q < P.max_items) {
-      while (o[k + 1] <= e) k++;  // bucket of item e: popcount WSPLIT - k
-      const int pc = WSPLIT - k;
-      const int nxt = min(o[k + 1], end);
-      const long want = max(max(1L, (long)(P.min_items - (e - q))),
-                            (P.pairs - pairs + pc - 1) / pc);
-      int take = (int)min((long)(nxt - e), want);
-      take = min(take, P.max_items - (e - q));
-      pairs += (long)take * pc;
-      e += take;
-    }
+  int q = o[0], e = o[0], n = 0;
+  long pairs = 0;
+  auto emit = [&]() {
     if (chunks && base + n < P.cap) {
       chunks[3 * (size_t)(base + n)] = b;
       chunks[3 * (size_t)(base + n) + 1] = q;
@@ -629,7 +622,25 @@ __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkP
     }
     n++;
     q = e;
+    pairs = 0;
+  };
+  // a chunk grows while it holds fewer than P.pairs pairs or fewer than
+  // P.min_items items, and never past P.max_items items
+#pragma unroll
+  for (int k = 0; k < NBUCKET; k++) {
+    const int pc = WSPLIT - k;
+    const int hi = min(o[k + 1], end);
+    while (e < hi) {
+      const long want = max(max(1L, (long)(P.min_items - (e - q))),
+                            (P.pairs - pairs + pc - 1) / pc);
+      int take = (int)min((long)(hi - e), want);
+      take = min(take, P.max_items - (e - q));
+      pairs += (long)take * pc;
+      e += take;
+      if (!((pairs < P.pairs || e - q < P.min_items) && e - q < P.max_items)) emit();
+    }
   }
+  if (e > q) emit();
   return n;
 }
 

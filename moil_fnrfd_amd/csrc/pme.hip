@@ -132,8 +132,60 @@ __device__ __forceinline__ void tbin_impl(const double *__restrict__ coor, int n
     todo &= ~same;
   }
 }
+// With at most TSCAN_FUSED tiles the last workgroup of the binning pass
+// scans the counts itself (saving the dispatch of a one-workgroup kernel
+// behind the real-space work): every workgroup takes a ticket after its
+// counter atomics have returned; the last one reads the counts with
+// agent-scope atomic loads (the counts live at the coherence point, where
+// the atomics performed them), writes the exclusive scan and clears the
+// counts and the ticket with agent-scope atomic stores -- no fence that
+// would write back the L2.
+#define TSCAN_FUSED 4096
+__device__ __forceinline__ void tscan_last(int32_t *__restrict__ cnt, int n,
+                                           int32_t *__restrict__ start) {
+  __shared__ int part[PME_THREADS];
+  const int t = threadIdx.x;
+  const int per = (n + PME_THREADS - 1) / PME_THREADS;  // <= TSCAN_FUSED / PME_THREADS
+  const int b = t * per, e = min(n, b + per);
+  int v[TSCAN_FUSED / PME_THREADS];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < TSCAN_FUSED / PME_THREADS; k++) {
+    v[k] = (b + k < e) ? __hip_atomic_load(&cnt[b + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : 0;
+    sum += v[k];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < PME_THREADS; off <<= 1) {
+    const int x = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+#pragma unroll
+  for (int k = 0; k < TSCAN_FUSED / PME_THREADS; k++) {
+    if (b + k < e) {
+      start[b + k] = run;
+      run += v[k];
+      __hip_atomic_store(&cnt[b + k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (t == PME_THREADS - 1) {
+    start[n] = part[PME_THREADS - 1];
+    __hip_atomic_store(&cnt[n], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 __device__ __forceinline__ void tbin_body(const TbinArgs &A, int bid, int nblk) {
   tbin_impl(A.coor, A.npt, A.order, A.P, A.T, A.tcnt, A.atile, A.arank, bid);
+  if (!A.fuse_scan) return;  // a launch argument: uniform
+  __shared__ int last;
+  const int ntile = A.T.nt1 * A.T.nt2 * A.T.nt3;
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&A.tcnt[ntile], 1) == nblk - 1;
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(last)) tscan_last(A.tcnt, ntile, A.tstart);
 }
 DEFINE_KERNEL(k_pme_tbin, TbinArgs, PME_THREADS, tbin_body)
 
@@ -593,7 +645,9 @@ void pme_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   T.rsz = T.r1 * T.r2 * T.r3;
   p->pme = true;
   p->n_tbin = (c->npt + PME_THREADS - 1) / PME_THREADS;
-  p->tbin = {d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile, c->d_arank};
+  const int fuse = c->pt_ntile <= TSCAN_FUSED ? 1 : 0;
+  p->tbin = {d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile, c->d_arank,
+             c->d_tstart, fuse};
   p->tscan = {c->d_tcnt, c->pt_ntile, c->d_tstart};
   p->tscat = {d_coor, c->d_ptchg, c->npt, P, c->d_atile, c->d_arank, c->d_tstart,
               c->d_sw, c->d_sbase, c->d_sidx};
@@ -610,7 +664,7 @@ void pme_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
 // spreading (binning, scan, sorted records, tile regions, grid sum)
 void pme_launch_spread(const EvalPlan &p, hipStream_t st) {
   launch_k_pme_tbin(p.tbin, p.n_tbin, st);
-  launch_k_pme_tscan(p.tscan, 1, st);
+  if (!p.tbin.fuse_scan) launch_k_pme_tscan(p.tscan, 1, st);
   if (p.f64) {
     launch_k_pme_tscatter_f64(p.tscat, p.n_tbin, st);
     launch_k_pme_tspread_f64(p.tspread, p.n_tile, st);
