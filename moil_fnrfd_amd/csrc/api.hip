@@ -550,8 +550,7 @@ extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, v
   hipStream_t st = pick_stream(c, stream);
   int32_t status[ST_COUNT];
   for (int attempt = 0;; attempt++) {
-    HIP_CHECK(hipMemsetAsync(c->d_status, 0, sizeof(int32_t) * ST_COUNT, st));
-    CHECK_RC(lists_build(c, d_coor, st));
+    CHECK_RC(lists_build(c, d_coor, st));  // clears the status words first
     CHECK_RC(atoms_build(c, d_coor, st));
     HIP_CHECK(hipMemcpyAsync(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));

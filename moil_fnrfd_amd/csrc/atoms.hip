@@ -133,11 +133,6 @@ __global__ void k_row_pad(const int32_t *__restrict__ sstart, int nrow, int rowc
   rowcnt[r] = (n + WBLOCK - 1) / WBLOCK * WBLOCK;
 }
 
-__global__ void k_fill_i32(int32_t *__restrict__ a, int32_t v, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) a[q] = v;
-}
-
 __global__ void k_pad_slots(const int32_t *__restrict__ sslot_u, int nsol,
                             const int32_t *__restrict__ sstart, const int32_t *__restrict__ rowoff,
                             int nrow, int rowcells, int32_t *__restrict__ sslot2atom) {
@@ -499,22 +494,23 @@ __global__ void k_ajcount(const AItem *__restrict__ items, const int32_t *__rest
        q += (size_t)gridDim.x * blockDim.x)
     atomicAdd(&cnt[items[q].j], 1);
 }
+// (thread 0 also copies the build's item and chunk totals into the status
+// words the host reads at the end)
 __global__ void k_ajfill(const AItem *__restrict__ items, const int32_t *__restrict__ nitem,
                          size_t cap, const int32_t *__restrict__ dropped,
                          const int32_t *__restrict__ start, int32_t *__restrict__ fill,
-                         int32_t *__restrict__ jpos) {
+                         int32_t *__restrict__ jpos, const int32_t *__restrict__ nchunk,
+                         int32_t *__restrict__ st_items, int32_t *__restrict__ st_chunks) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *st_items = *nitem;
+    *st_chunks = *nchunk;
+  }
   const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (size_t)gridDim.x * blockDim.x) {
     const int j = items[q].j;
     jpos[q] = start[j] + atomicAdd(&fill[j], 1);
   }
-}
-
-__global__ void k_acopy_status(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
-                               int32_t *__restrict__ sa, int32_t *__restrict__ sb) {
-  *sa = *a;
-  *sb = *b;
 }
 
 // ---------------------------------------------------------------------------
@@ -766,7 +762,7 @@ int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
   }
   c->ancell = ((c->ag[0] + 1) / 2) * ((c->ag[1] + 1) / 2) * ((c->ag[2] + 1) / 2) * 8;
   HIP_CHECK(hipMalloc(&c->d_acell_of, sizeof(int32_t) * c->npt));
-  HIP_CHECK(hipMalloc(&c->d_acnt, sizeof(int32_t) * (c->ancell + 1)));
+  HIP_CHECK(hipMalloc(&c->d_acnt, sizeof(int32_t) * 4 * (size_t)(c->ancell + 1)));
   HIP_CHECK(hipMalloc(&c->d_astart, sizeof(int32_t) * (c->ancell + 1)));
   HIP_CHECK(hipMalloc(&c->d_aslot2atom, sizeof(int32_t) * c->npt));
   HIP_CHECK(hipMalloc(&c->d_sstart, sizeof(int32_t) * (c->ancell + 1)));
@@ -783,6 +779,7 @@ int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
   HIP_CHECK(hipMalloc(&c->d_abcount, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_aboff, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajcnt, sizeof(int32_t) * (c->npt + 1)));
+  HIP_CHECK(hipMalloc(&c->d_ajfill, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajstart, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_acfirst, sizeof(int32_t) * (c->nablk + 1)));
   HIP_CHECK(hipMalloc(&c->d_arcount, sizeof(int32_t) * (c->nablk + 1)));
@@ -793,7 +790,7 @@ void atoms_free(moil_ctx *c) {
   void *ptrs[] = {c->d_mono_of, c->d_aflag, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2,
                   c->d_sol, c->d_acell_of, c->d_acnt, c->d_astart, c->d_aslot2atom,
                   c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
-                  c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajstart,
+                  c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajfill, c->d_ajstart,
                   c->d_ajpos, c->d_acfirst, c->d_araw, c->d_arcount, c->d_sslot_u,
                   c->d_rowcnt, c->d_rowoff, c->d_account};
   for (void *p : ptrs)
@@ -811,32 +808,39 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   g.nbx = (g.gx + 1) / 2; g.nby = (g.gy + 1) / 2; g.nbz = (g.gz + 1) / 2;
   g.g0x = -0.5 * P.a; g.g0y = -0.5 * P.b; g.g0z = -0.5 * P.c;
   g.csx = c->acs[0]; g.csy = c->acs[1]; g.csz = c->acs[2];
+  // the build's counters (four cell-count passes, the partner CSR) and
+  // the padding markers, set in one launch
+  const size_t nc1 = (size_t)c->ancell + 1;
+  int32_t *cnt0 = c->d_acnt, *cnt1 = cnt0 + nc1, *cnt2 = cnt1 + nc1, *cnt3 = cnt2 + nc1;
+  {
+    ClearSet cs;
+    cs.add(c->d_acnt, (int)(4 * nc1), 0);
+    cs.add(c->d_ajcnt, c->npt + 1, 0);
+    cs.add(c->d_ajfill, c->npt + 1, 0);
+    cs.add(c->d_sslot_of, c->npt, -1);
+    cs.add(c->d_sslot2atom, c->nsslot, -1);
+    launch_clear(cs, st);
+  }
   // all atoms by cell (partner side)
-  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_cells<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(d_coor, nullptr, c->npt, g, c->d_acell_of,
-                                                      c->d_acnt);
-  CHECK_SCAN(scan_excl(c, c->d_acnt, c->d_astart, c->ancell, st));
-  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+                                                      cnt0);
+  CHECK_SCAN(scan_excl(c, cnt0, c->d_astart, c->ancell, st));
   k_atom_scatter<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, nullptr, c->npt,
-                                                        c->d_astart, c->d_acnt, c->d_aslot2atom);
+                                                        c->d_astart, cnt1, c->d_aslot2atom);
   k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_astart, c->ancell, c->d_aslot2atom);
   // solute atoms by cell (block side)
-  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
   k_atom_cells<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_sol, c->nsol, g, c->d_acell_of,
-                                                       c->d_acnt);
-  CHECK_SCAN(scan_excl(c, c->d_acnt, c->d_sstart, c->ancell, st));
-  HIP_CHECK(hipMemsetAsync(c->d_acnt, 0, sizeof(int32_t) * (c->ancell + 1), st));
+                                                       cnt2);
+  CHECK_SCAN(scan_excl(c, cnt2, c->d_sstart, c->ancell, st));
   k_atom_scatter<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, c->d_sol, c->nsol,
-                                                         c->d_sstart, c->d_acnt, c->d_sslot_u);
+                                                         c->d_sstart, cnt3, c->d_sslot_u);
   k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->ancell, c->d_sslot_u);
   // blocks end with their brick row (padding slots -1)
   k_row_pad<<<(c->nrow + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->nrow, c->rowcells, c->d_rowcnt);
   CHECK_SCAN(scan_excl(c, c->d_rowcnt, c->d_rowoff, c->nrow, st));
-  k_fill_i32<<<(c->nsslot + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, -1, c->nsslot);
   k_pad_slots<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_sslot_u, c->nsol, c->d_sstart,
                                                       c->d_rowoff, c->nrow, c->rowcells,
                                                       c->d_sslot2atom);
-  HIP_CHECK(hipMemsetAsync(c->d_sslot_of, 0xff, sizeof(int32_t) * c->npt, st));
   k_slot_of<<<(c->nsslot + TB - 1) / TB, TB, 0, st>>>(c->d_sslot2atom, c->nsslot, c->d_sslot_of);
 
   APar ap;
@@ -879,15 +883,12 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   CHECK_SCAN(chunk_table(c, c->d_aboff, cp, c->d_account, c->d_acfirst, c->d_achunk, nullptr,
                          nullptr, st));
   const int32_t *d_nitem = c->d_aboff + (size_t)c->nablk * NBUCKET;
-  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
   k_ajcount<<<1024, TB, 0, st>>>(c->d_aitems, d_nitem, c->aitem_cap, c->d_status + ST_ARAW,
                                  c->d_ajcnt);
   CHECK_SCAN(scan_excl(c, c->d_ajcnt, c->d_ajstart, c->npt, st));
-  HIP_CHECK(hipMemsetAsync(c->d_ajcnt, 0, sizeof(int32_t) * (c->npt + 1), st));
   k_ajfill<<<1024, TB, 0, st>>>(c->d_aitems, d_nitem, c->aitem_cap, c->d_status + ST_ARAW,
-                                c->d_ajstart, c->d_ajcnt, c->d_ajpos);
-  k_acopy_status<<<1, 1, 0, st>>>(d_nitem, c->d_acfirst + c->nablk, c->d_status + ST_AITEMS,
-                                  c->d_status + ST_ACHUNK);
+                                c->d_ajstart, c->d_ajfill, c->d_ajpos, c->d_acfirst + c->nablk,
+                                c->d_status + ST_AITEMS, c->d_status + ST_ACHUNK);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

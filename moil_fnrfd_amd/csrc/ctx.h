@@ -115,7 +115,7 @@ struct moil_ctx {
   // point; jstart: row range per slot j, jpos: row of each item
   long long *d_jpart = nullptr, *d_ipart = nullptr;
   int ipart_cap = 0;
-  int32_t *d_jcnt = nullptr, *d_jstart = nullptr;
+  int32_t *d_jcnt = nullptr, *d_jfill = nullptr, *d_jstart = nullptr;
   int32_t *d_jpos = nullptr;       // [item] row of the item in jpart
   int32_t *d_cfirst = nullptr;     // [nwblk+1] first chunk of each block
   int wchunk_pairs = 4096;         // pair budget of one real-space workgroup
@@ -143,7 +143,8 @@ struct moil_ctx {
   int32_t *d_exc1 = nullptr, *d_exc2 = nullptr;  // MOIL exclusion CSR
   int ag[3] = {0, 0, 0}, ancell = 0;     // all-atom cell grid
   double acs[3] = {0, 0, 0};
-  int32_t *d_acell_of = nullptr, *d_acnt = nullptr, *d_astart = nullptr, *d_aslot2atom = nullptr;
+  int32_t *d_acell_of = nullptr, *d_astart = nullptr, *d_aslot2atom = nullptr;
+  int32_t *d_acnt = nullptr;  // [4][ancell+1] cell counters of the four counting passes
   int32_t *d_sstart = nullptr, *d_sslot2atom = nullptr, *d_sslot_of = nullptr;  // solute slots
   int nablk = 0;                         // solute blocks of WBLOCK slots
   int32_t *d_abcount = nullptr, *d_aboff = nullptr;
@@ -158,7 +159,7 @@ struct moil_ctx {
   long long *d_ajpart = nullptr;         // [row][3] partner rows, fixed point
   long long *d_aipart = nullptr;         // [chunk][64][3] fixed point
   int aipart_cap = 0;
-  int32_t *d_ajcnt = nullptr, *d_ajstart = nullptr, *d_acfirst = nullptr;
+  int32_t *d_ajcnt = nullptr, *d_ajfill = nullptr, *d_ajstart = nullptr, *d_acfirst = nullptr;
   int32_t *d_ajpos = nullptr;
 
   // ---- accumulators ----
@@ -203,6 +204,21 @@ struct moil_ctx {
 // exclusive scan of n int32 into out[0..n] (out[n] = total), on stream st
 // (lists.hip); workspace grows in the context
 int32_t scan_excl(moil_ctx *c, const int32_t *in, int32_t *out, int n, hipStream_t st);
+
+// up to 8 int32 buffers set to a value each, in one launch (lists.hip)
+struct ClearSet {
+  int32_t *p[8];
+  int n[8];
+  int32_t v[8];
+  int count = 0;
+  void add(int32_t *ptr, int len, int32_t val) {
+    p[count] = ptr;
+    n[count] = len;
+    v[count] = val;
+    count++;
+  }
+};
+void launch_clear(const ClearSet &cs, hipStream_t st);
 #define CHECK_SCAN(x)                        \
   do {                                       \
     const int32_t rc_ = (x);                 \

@@ -584,10 +584,17 @@ __global__ void k_jcount(const WItem *__restrict__ items, const int32_t *__restr
 // jpos[item] = the item's row in the partial buffer, inside its partner's
 // CSR range (the order within a range does not matter: the rows are
 // summed as integers)
+// (thread 0 also copies the build's item and chunk totals into the status
+// words the host reads at the end)
 __global__ void k_jfill(const WItem *__restrict__ items, const int32_t *__restrict__ nitem,
                         size_t cap, const int32_t *__restrict__ dropped,
                         const int32_t *__restrict__ start, int32_t *__restrict__ fill,
-                        int32_t *__restrict__ jpos) {
+                        int32_t *__restrict__ jpos, const int32_t *__restrict__ nchunk,
+                        int32_t *__restrict__ st_items, int32_t *__restrict__ st_chunks) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *st_items = *nitem;
+    *st_chunks = *nchunk;
+  }
   const size_t n = *dropped ? 0 : min((size_t)*nitem, cap);
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (size_t)gridDim.x * blockDim.x) {
@@ -698,12 +705,6 @@ int32_t chunk_table(moil_ctx *c, const int32_t *d_boff, ChunkPar P, int32_t *d_c
   return MOIL_GPU_OK;
 }
 
-// sizes written by the build into the status words (read once by the host)
-__global__ void k_copy_status(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
-                              int32_t *__restrict__ sa, int32_t *__restrict__ sb) {
-  *sa = *a;
-  *sb = *b;
-}
 
 #define CHECK_RC_L(x)                    \
   do {                                   \
@@ -772,6 +773,7 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_wraw); c->d_wraw = nullptr;
   hipFree(c->d_rcount); c->d_rcount = nullptr;
   hipFree(c->d_jcnt); c->d_jcnt = nullptr;
+  hipFree(c->d_jfill); c->d_jfill = nullptr;
   hipFree(c->d_jstart); c->d_jstart = nullptr;
   hipFree(c->d_jpos); c->d_jpos = nullptr;
   hipFree(c->d_jpart); c->d_jpart = nullptr;
@@ -784,10 +786,38 @@ void lists_free(moil_ctx *c) {
   hipFree(c->d_bpart); c->d_bpart = nullptr;
 }
 
+__global__ void k_clear(ClearSet S) {
+  for (int k = 0; k < S.count; k++)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S.n[k]; i += gridDim.x * blockDim.x)
+      S.p[k][i] = S.v[k];
+}
+void launch_clear(const ClearSet &cs, hipStream_t st) {
+  int most = 0;
+  for (int k = 0; k < cs.count; k++) most = std::max(most, cs.n[k]);
+  if (most > 0) k_clear<<<std::min((most + 255) / 256, 512), 256, 0, st>>>(cs);
+}
+
 int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   const moil_params &P = c->par;
   const int TB = 256;
-  HIP_CHECK(hipMemsetAsync(c->d_flags, 0, sizeof(int32_t) * 4, st));
+  // the build's counters and status words, cleared in one launch
+  if (c->nwat > 0 && !c->d_jcnt) {
+    HIP_CHECK(hipMalloc(&c->d_jcnt, sizeof(int32_t) * (c->nwat + 1)));
+    HIP_CHECK(hipMalloc(&c->d_jfill, sizeof(int32_t) * (c->nwat + 1)));
+    HIP_CHECK(hipMalloc(&c->d_jstart, sizeof(int32_t) * (c->nwat + 1)));
+  }
+  {
+    ClearSet cs;
+    cs.add(c->d_status, ST_COUNT, 0);
+    cs.add(c->d_flags, 4, 0);
+    if (c->nwat > 0) {
+      cs.add(c->d_cell_cnt, c->ncell + 1, 0);
+      cs.add(c->d_cell_fill, c->ncell + 1, 0);
+      cs.add(c->d_jcnt, c->nwat + 1, 0);
+      cs.add(c->d_jfill, c->nwat + 1, 0);
+    }
+    launch_clear(cs, st);
+  }
   if (c->totmon > 0)
     k_mono_cent<<<(c->totmon + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_poipt, c->d_monoflag,
                                                            c->totmon, 0.5 * P.a, 0.5 * P.b,
@@ -798,8 +828,6 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     g.nbx = (c->gx + 1) / 2; g.nby = (c->gy + 1) / 2; g.nbz = (c->gz + 1) / 2;
     g.g0x = c->gmin[0]; g.g0y = c->gmin[1]; g.g0z = c->gmin[2];
     g.csx = c->cs[0]; g.csy = c->cs[1]; g.csz = c->cs[2];
-    HIP_CHECK(hipMemsetAsync(c->d_cell_cnt, 0, sizeof(int32_t) * (c->ncell + 1), st));
-    HIP_CHECK(hipMemsetAsync(c->d_cell_fill, 0, sizeof(int32_t) * (c->ncell + 1), st));
     int nblk = (c->nwat + TB - 1) / TB;
     k_wat_cells<<<nblk, TB, 0, st>>>(c->d_mono_cent, c->d_wat_mono, c->nwat, g, c->d_cell_of,
                                      c->d_cell_cnt);
@@ -828,10 +856,6 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     if (!c->d_rcount) HIP_CHECK(hipMalloc(&c->d_rcount, sizeof(int32_t) * (c->nwblk * np + 1)));
     if (!c->d_wccount) HIP_CHECK(hipMalloc(&c->d_wccount, sizeof(int32_t) * (c->nwblk + 1)));
     if (!c->d_cfirst) HIP_CHECK(hipMalloc(&c->d_cfirst, sizeof(int32_t) * (c->nwblk + 1)));
-    if (!c->d_jcnt) {
-      HIP_CHECK(hipMalloc(&c->d_jcnt, sizeof(int32_t) * (c->nwat + 1)));
-      HIP_CHECK(hipMalloc(&c->d_jstart, sizeof(int32_t) * (c->nwat + 1)));
-    }
     if (!c->d_wraw) {
       c->wraw_cap = std::max(c->wraw_cap, 1024);
       HIP_CHECK(hipMalloc(&c->d_wraw, sizeof(WRaw) * (size_t)c->wraw_cap * c->nwblk * np));
@@ -867,15 +891,12 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
                            c->d_worder, st));
     // row ranges of the partial buffer by partner slot j
     const int32_t *d_nitem = c->d_boff + (size_t)c->nwblk * NBUCKET;
-    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
     k_jcount<<<1024, TB, 0, st>>>(c->d_witems, d_nitem, c->witem_cap, c->d_status + ST_WRAW,
                                   c->d_jcnt);
     CHECK_SCAN(scan_excl(c, c->d_jcnt, c->d_jstart, c->nwat, st));
-    HIP_CHECK(hipMemsetAsync(c->d_jcnt, 0, sizeof(int32_t) * (c->nwat + 1), st));
     k_jfill<<<1024, TB, 0, st>>>(c->d_witems, d_nitem, c->witem_cap, c->d_status + ST_WRAW,
-                                 c->d_jstart, c->d_jcnt, c->d_jpos);
-    k_copy_status<<<1, 1, 0, st>>>(d_nitem, c->d_cfirst + c->nwblk, c->d_status + ST_WITEMS,
-                                   c->d_status + ST_WCHUNK);
+                                 c->d_jstart, c->d_jfill, c->d_jpos, c->d_cfirst + c->nwblk,
+                                 c->d_status + ST_WITEMS, c->d_status + ST_WCHUNK);
     HIP_CHECK(hipGetLastError());
   }
   return MOIL_GPU_OK;
