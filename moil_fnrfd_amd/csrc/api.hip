@@ -210,6 +210,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     }
     const char *w = std::getenv("MOIL_GPU_WATER_WG_PER_CU");
     if (w && w[0]) c->water_wg_per_cu = std::max(0, std::atoi(w));
+    const char *cp = std::getenv("MOIL_GPU_CHUNK_PAIRS");
+    if (cp && cp[0]) c->wchunk_pairs = std::max(256, std::atoi(cp));
+    const char *ap = std::getenv("MOIL_GPU_ATOM_CHUNK_PAIRS");
+    if (ap && ap[0]) c->achunk_pairs = std::max(256, std::atoi(ap));
     c->water_wgs = c->water_wg_per_cu * ncu;
   }
   // ---- Ewald set-up ----

@@ -876,7 +876,7 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
                                            c->d_aitems, c->aitem_cap);
   ChunkPar cp;
   cp.nblk = c->nablk;
-  cp.pairs = c->wchunk_pairs;
+  cp.pairs = c->achunk_pairs;
   cp.max_items = c->wchunk_max_items;
   cp.min_items = 1;
   cp.cap = c->achunk_cap;

@@ -118,7 +118,8 @@ struct moil_ctx {
   int32_t *d_jcnt = nullptr, *d_jfill = nullptr, *d_jstart = nullptr;
   int32_t *d_jpos = nullptr;       // [item] row of the item in jpart
   int32_t *d_cfirst = nullptr;     // [nwblk+1] first chunk of each block
-  int wchunk_pairs = 4096;         // pair budget of one real-space workgroup
+  int wchunk_pairs = 2048;         // pair budget of one water real-space workgroup
+  int achunk_pairs = 8192;         // pair budget of one atom-list workgroup
   // single-system water kernel: workgroups per CU of the short-grid
   // launch (0: one workgroup per chunk) and the resulting grid
   int water_wg_per_cu = 4;
