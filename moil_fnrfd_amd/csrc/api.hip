@@ -118,7 +118,7 @@ static void ctx_free(moil_ctx *c) {
                   c->d_erf, c->d_facc, c->d_epart, c->d_energy, c->d_coor, c->d_dpot, c->d_wpos,
                   c->d_wi_boff, c->d_wi_aoff, c->d_wi_eoff, c->d_wi_zero, c->d_wi_bidx,
                   c->d_wi_aidx, c->d_wi_eidx, c->d_wi_bpar, c->d_wi_apar, c->d_etot, c->d_ticket,
-                  c->d_xall, c->d_adj, c->d_scan_ws};
+                  c->d_xall, c->d_adj, c->d_scan_ws, c->d_brstart, c->d_brslot, c->d_bslot};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->gev_in) hipEventDestroy(c->gev_in);
@@ -284,8 +284,8 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   // per-water CSR of intra-water terms (original order kept within a water)
   std::vector<std::vector<int>> wb(c->nwat), wa(c->nwat), we(c->nwat);
   std::vector<int32_t> wi_eidx;
+  std::vector<int32_t> ei, ej;  // generic exclusion pairs (0-based)
   {
-    std::vector<int32_t> ei, ej;
     std::vector<std::pair<int, int>> all;
     for (int i = 1; i <= t->npt - 1; i++)  // pot/cdie_ewald.f:68,83 (atom npt ignored)
       for (int k = t->exc1[i - 1]; k < t->exc1[i]; k++) all.push_back({i - 1, t->exc2[k] - 1});
@@ -400,6 +400,42 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     TRY(upload(&c->d_sp, sp.data(), sp.size()));
     c->h_sp = sp;
     TRY(upload(&c->d_p14, t->p14, 3 * (size_t)c->totspe));
+    // gradient slots of the bonded terms and each atom's list of them:
+    // the terms the bonded kernel evaluates (the enabled types), in its
+    // flat order, one slot per atom position (arities 2, 3, 4, 4, 2, 2)
+    {
+      const uint32_t T = p->terms;
+      const bool nbon = (T & (MOIL_TERM_VDW | MOIL_TERM_EL)) != 0;
+      std::vector<int32_t> slot_atom;  // atom (0-based) of every slot
+      if (T & MOIL_TERM_BOND)
+        for (int32_t a : ib) slot_atom.push_back(a - 1);
+      if (T & MOIL_TERM_ANGLE)
+        for (int32_t a : ia) slot_atom.push_back(a - 1);
+      if (T & MOIL_TERM_TORS)
+        for (int32_t a : it) slot_atom.push_back(a - 1);
+      if (T & MOIL_TERM_IMP)
+        for (int32_t a : ii) slot_atom.push_back(a - 1);
+      if (nbon)
+        for (size_t m = 0; m < ei.size(); m++) {
+          slot_atom.push_back(ei[m]);
+          slot_atom.push_back(ej[m]);
+        }
+      if (nbon && (T & (MOIL_TERM_EL14 | MOIL_TERM_VDW14)))
+        for (int32_t a : sp) slot_atom.push_back(a - 1);
+      const size_t ns = slot_atom.size();
+      std::vector<int32_t> rstart(c->npt + 1, 0), rslot(std::max<size_t>(ns, 1));
+      for (size_t q = 0; q < ns; q++) rstart[slot_atom[q] + 1]++;
+      for (int a = 0; a < c->npt; a++) rstart[a + 1] += rstart[a];
+      std::vector<int32_t> fill(rstart.begin(), rstart.end() - 1);
+      for (size_t q = 0; q < ns; q++) rslot[fill[slot_atom[q]]++] = (int32_t)q;
+      c->nbslot = ns;
+      TRY(upload(&c->d_brstart, rstart.data(), rstart.size()));
+      TRY(upload(&c->d_brslot, rslot.data(), rslot.size()));
+      if (hipMalloc(&c->d_bslot, sizeof(long long) * 3 * std::max<size_t>(ns, 1)) != hipSuccess) {
+        ctx_free(c);
+        return fail(MOIL_GPU_EHIP, "hipMalloc of the bonded gradient slots failed");
+      }
+    }
   }
   if (c->nwat >= 2) {
     // parameters of the last water, pot/watwat_ewald.f:40-72
@@ -580,21 +616,22 @@ extern "C" int32_t moil_gpu_build_lists(moil_ctx *c, const double *coor) {
 
 // ---------------------------------------------------------------------------
 
-// dpot = fixed-point accumulator + reciprocal gradient; the accumulator
-// is cleared for the next evaluation on the way
-__device__ __forceinline__ void finalize_impl(long long *__restrict__ facc,
-                                              const double *__restrict__ fpme, int has_pme,
-                                              size_t n, double inv, double *__restrict__ dpot,
-                                              int bid) {
+// dpot = (fixed-point accumulator + the atom's bonded gradient slots) x
+// scale + reciprocal gradient; the accumulator is cleared for the next
+// evaluation on the way
+__device__ __forceinline__ void finalize_impl(const FinArgs &A, int bid) {
   size_t g = (size_t)bid * 256 + threadIdx.x;
-  if (g >= n) return;
-  double v = (double)facc[g] * inv;
-  if (has_pme) v += fpme[g];
-  dpot[g] = v;
-  facc[g] = 0;
+  if (g >= A.n) return;
+  long long acc = A.facc[g];
+  const int a = (int)(g / 3), k = (int)(g - 3 * (size_t)a);
+  for (int r = A.brstart[a]; r < A.brstart[a + 1]; r++) acc += A.bslot[3 * (size_t)A.brslot[r] + k];
+  double v = (double)acc * A.inv;
+  if (A.has_pme) v += A.fpme[g];
+  A.dpot[g] = v;
+  A.facc[g] = 0;
 }
 __device__ __forceinline__ void finalize_body(const FinArgs &A, int bid, int nblk) {
-  finalize_impl(A.facc, A.fpme, A.has_pme, A.n, A.inv, A.dpot, bid);
+  finalize_impl(A, bid);
 }
 DEFINE_KERNEL(k_finalize_forces, FinArgs, 256, finalize_body)
 
@@ -664,8 +701,7 @@ DEFINE_KERNEL(k_energies, EnArgs, 256, energies_body)
 // [0, n_fin) finalize the gradient, the next EN_COUNT reduce one energy
 // term each (the two halves touch disjoint memory)
 __device__ __forceinline__ void assemble_body(const AsmArgs &A, int bid, int nblk) {
-  if (bid < A.n_fin) finalize_impl(A.fin.facc, A.fin.fpme, A.fin.has_pme, A.fin.n, A.fin.inv,
-                                   A.fin.dpot, bid);
+  if (bid < A.n_fin) finalize_impl(A.fin, bid);
   else energies_impl(A.en.ep, A.en.C, A.en.tot, A.en.ticket, A.en.out, bid - A.n_fin, EN_COUNT);
 }
 DEFINE_KERNEL(k_assemble, AsmArgs, 256, assemble_body)
@@ -717,7 +753,8 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
     atoms_plan(c, d_coor, nwpart, p);
   }
   const size_t n3 = 3 * (size_t)c->npt;
-  p->fin = {c->d_facc, c->d_fpme, nb ? 1 : 0, n3, c->fix_inv, d_dpot};
+  p->fin = {c->d_facc, c->d_fpme, nb ? 1 : 0, n3, c->fix_inv, d_dpot,
+            c->d_brstart, c->d_brslot, c->d_bslot};
   p->n_fin = (int)((n3 + 255) / 256);
   EnCfg C;
   C.max_parts = c->max_parts;

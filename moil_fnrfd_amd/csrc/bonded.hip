@@ -25,23 +25,40 @@
 #define BT 256
 
 
-__device__ __forceinline__ void add3(long long *facc, int atom, double fx, double fy, double fz,
-                                     double s) {
-  long long v;
-  v = __double2ll_rn(fx * s);
-  if (v) atomicAdd((unsigned long long *)&facc[3 * atom], (unsigned long long)v);
-  v = __double2ll_rn(fy * s);
-  if (v) atomicAdd((unsigned long long *)&facc[3 * atom + 1], (unsigned long long)v);
-  v = __double2ll_rn(fz * s);
-  if (v) atomicAdd((unsigned long long *)&facc[3 * atom + 2], (unsigned long long)v);
-}
-
-// gradient sinks of the term functions: global atomics, or the registers
-// of a thread that owns atoms a0 .. a0+2
-struct AtomicSink {
-  long long *facc;
+// gradient sinks of the term functions: the registers of a thread that
+// owns atoms a0 .. a0+2, or the term's own gradient slots
+// (one 3-vector per atom position of the term, written whole -- zeros
+// included -- so they need no clearing; the assembly kernel sums each
+// atom's slots through a per-atom list built at set-up)
+struct SlotSink {
+  long long acc[4][3];
+  int id[4];
+  int n;
   double s;
-  __device__ void add(int atom, double fx, double fy, double fz) { add3(facc, atom, fx, fy, fz, s); }
+  __device__ SlotSink(int n_, int i0, int i1, int i2, int i3, double s_) : n(n_), s(s_) {
+    id[0] = i0; id[1] = i1; id[2] = i2; id[3] = i3;
+#pragma unroll
+    for (int p = 0; p < 4; p++) acc[p][0] = acc[p][1] = acc[p][2] = 0;
+  }
+  __device__ void add(int atom, double fx, double fy, double fz) {
+    const int p = atom == id[0] ? 0 : (atom == id[1] ? 1 : (atom == id[2] ? 2 : 3));
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q == p) {
+        acc[q][0] += __double2ll_rn(fx * s);
+        acc[q][1] += __double2ll_rn(fy * s);
+        acc[q][2] += __double2ll_rn(fz * s);
+      }
+  }
+  __device__ void flush(long long *__restrict__ out) const {
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+      if (p < n) {
+        out[3 * p] = acc[p][0];
+        out[3 * p + 1] = acc[p][1];
+        out[3 * p + 2] = acc[p][2];
+      }
+  }
 };
 struct WaterSink {
   long long acc[9];
@@ -122,9 +139,11 @@ __device__ inline void bond_calc(const double *coor, int i, int j, double kb, do
 }
 
 __device__ inline void term_bond(const double *coor, const BSeg &S, int m, const BPar &P,
-                                 long long *facc, double &e) {
-  AtomicSink sk{facc, P.fix_scale};
-  bond_calc(coor, S.ib[2 * m] - 1, S.ib[2 * m + 1] - 1, S.bp[2 * m], S.bp[2 * m + 1], sk, e);
+                                 long long *out, double &e) {
+  const int i = S.ib[2 * m] - 1, j = S.ib[2 * m + 1] - 1;
+  SlotSink sk(2, i, j, -1, -1, P.fix_scale);
+  bond_calc(coor, i, j, S.bp[2 * m], S.bp[2 * m + 1], sk, e);
+  sk.flush(out);
 }
 
 template <class Sink>
@@ -164,14 +183,15 @@ __device__ inline void angle_calc(const double *coor, int i, int j, int k, doubl
 }
 
 __device__ inline void term_angle(const double *coor, const BSeg &S, int m, const BPar &P,
-                                  long long *facc, double &e) {
-  AtomicSink sk{facc, P.fix_scale};
-  angle_calc(coor, S.ia[3 * m] - 1, S.ia[3 * m + 1] - 1, S.ia[3 * m + 2] - 1, S.ap[2 * m],
-             S.ap[2 * m + 1], sk, e);
+                                  long long *out, double &e) {
+  const int i = S.ia[3 * m] - 1, j = S.ia[3 * m + 1] - 1, k = S.ia[3 * m + 2] - 1;
+  SlotSink sk(3, i, j, k, -1, P.fix_scale);
+  angle_calc(coor, i, j, k, S.ap[2 * m], S.ap[2 * m + 1], sk, e);
+  sk.flush(out);
 }
 
 __device__ inline void term_tors(const double *coor, const BSeg &S, int m, const BPar &P,
-                                 long long *facc, double &e) {
+                                 long long *out, double &e) {
   int id[4] = {S.it[4 * m] - 1, S.it[4 * m + 1] - 1, S.it[4 * m + 2] - 1, S.it[4 * m + 3] - 1};
   double g[4][3];
   double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, nullptr);
@@ -179,11 +199,13 @@ __device__ inline void term_tors(const double *coor, const BSeg &S, int m, const
   e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
       p[5] * (p[2] * co * (4.0 * co * co - 3.0));
   double df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
-  for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
+  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale);
+  for (int q = 0; q < 4; q++) sk.add(id[q], df * g[q][0], df * g[q][1], df * g[q][2]);
+  sk.flush(out);
 }
 
 __device__ inline void term_imp(const double *coor, const BSeg &S, int m, const BPar &P,
-                                long long *facc, double &e) {
+                                long long *out, double &e) {
   const double pi = 3.14159265358979323846;
   int id[4] = {S.ii[4 * m] - 1, S.ii[4 * m + 1] - 1, S.ii[4 * m + 2] - 1, S.ii[4 * m + 3] - 1};
   double g[4][3], sgn;
@@ -209,7 +231,9 @@ __device__ inline void term_imp(const double *coor, const BSeg &S, int m, const 
     else if ((phi > 0.0 && phi < pi / 2.0) || (phi < 0.0 && phi > -pi / 2.0)) df = df * 1e6;
     else df = -df * 1e6;
   }
-  for (int q = 0; q < 4; q++) add3(facc, id[q], df * g[q][0], df * g[q][1], df * g[q][2], P.fix_scale);
+  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale);
+  for (int q = 0; q < 4; q++) sk.add(id[q], df * g[q][0], df * g[q][1], df * g[q][2]);
+  sk.flush(out);
 }
 
 template <class Sink>
@@ -233,13 +257,14 @@ __device__ inline void excl_calc(const double *coor, int i, int j, const double 
 }
 
 __device__ inline void term_excl(const double *coor, const BSeg &S, int m, const BPar &P,
-                                 long long *facc, double &e) {
-  AtomicSink sk{facc, P.fix_scale};
+                                 long long *out, double &e) {
+  SlotSink sk(2, S.ei[m], S.ej[m], -1, -1, P.fix_scale);
   excl_calc(coor, S.ei[m], S.ej[m], S.q, S.erf, P, sk, e);
+  sk.flush(out);
 }
 
 __device__ inline void term_14(const double *coor, const BSeg &S, int m, const BPar &P,
-                               long long *facc, double &e1, double &e2) {
+                               long long *out, double &e1, double &e2) {
   int j = S.sp[2 * m] - 1, k = S.sp[2 * m + 1] - 1;
   double rx = coor[3 * j] - coor[3 * k], ry = coor[3 * j + 1] - coor[3 * k + 1],
          rz = coor[3 * j + 2] - coor[3 * k + 2];
@@ -256,8 +281,10 @@ __device__ inline void term_14(const double *coor, const BSeg &S, int m, const B
     df2 = -e2 * s2;
   }
   double df = df1 + df2;
-  add3(facc, j, df * rx, df * ry, df * rz, P.fix_scale);
-  add3(facc, k, -df * rx, -df * ry, -df * rz, P.fix_scale);
+  SlotSink sk(2, j, k, -1, -1, P.fix_scale);
+  sk.add(j, df * rx, df * ry, df * rz);
+  sk.add(k, -df * rx, -df * ry, -df * rz);
+  sk.flush(out);
 }
 
 // deterministic workgroup sums of N per-thread values (wavefront shuffle
@@ -284,27 +311,32 @@ __device__ inline void block_sums(const double *v, double *red) {
 
 
 // all O(N) terms in one launch over a flat index space
-// [bonds | angles | torsions | impropers | exclusions | 1-4 pairs]
+// [bonds | angles | torsions | impropers | exclusions | 1-4 pairs]; term
+// m of a type writes the gradient slots bslot_base(type) + arity * m
+// (arities 2, 3, 4, 4, 2, 2), the layout of the per-atom slot lists
 __device__ __forceinline__ void bonded_impl(const double *__restrict__ coor, const BSeg &S,
-                                            const BPar &P, long long *__restrict__ facc,
+                                            const BPar &P, long long *__restrict__ bslot,
                                             double *__restrict__ ep, int mp, int bid) {
   __shared__ double red[BT];
   int m = bid * BT + threadIdx.x;
   double e[7] = {0, 0, 0, 0, 0, 0, 0};  // bond theta tors imp corr vdw14 el14
   int o1 = S.nb, o2 = o1 + S.nangl, o3 = o2 + S.ntors, o4 = o3 + S.nimp, o5 = o4 + S.nexcl,
       o6 = o5 + S.n14;
-  if (m < o1) term_bond(coor, S, m, P, facc, e[0]);
-  else if (m < o2) term_angle(coor, S, m - o1, P, facc, e[1]);
-  else if (m < o3) term_tors(coor, S, m - o2, P, facc, e[2]);
-  else if (m < o4) term_imp(coor, S, m - o3, P, facc, e[3]);
-  else if (m < o5) term_excl(coor, S, m - o4, P, facc, e[4]);
-  else if (m < o6) term_14(coor, S, m - o5, P, facc, e[5], e[6]);
+  const size_t b1 = 2 * (size_t)S.nb, b2 = b1 + 3 * (size_t)S.nangl, b3 = b2 + 4 * (size_t)S.ntors,
+               b4 = b3 + 4 * (size_t)S.nimp, b5 = b4 + 2 * (size_t)S.nexcl;
+  if (m < o1) term_bond(coor, S, m, P, bslot + 3 * (2 * (size_t)m), e[0]);
+  else if (m < o2) term_angle(coor, S, m - o1, P, bslot + 3 * (b1 + 3 * (size_t)(m - o1)), e[1]);
+  else if (m < o3) term_tors(coor, S, m - o2, P, bslot + 3 * (b2 + 4 * (size_t)(m - o2)), e[2]);
+  else if (m < o4) term_imp(coor, S, m - o3, P, bslot + 3 * (b3 + 4 * (size_t)(m - o3)), e[3]);
+  else if (m < o5) term_excl(coor, S, m - o4, P, bslot + 3 * (b4 + 2 * (size_t)(m - o4)), e[4]);
+  else if (m < o6)
+    term_14(coor, S, m - o5, P, bslot + 3 * (b5 + 2 * (size_t)(m - o5)), e[5], e[6]);
   const int slot[7] = {EN_BOND, EN_THETA, EN_TORS, EN_IMP, EN_CORR, EN_VDW14, EN_EL14};
   block_sums<7>(e, red);
   if (threadIdx.x < 7) ep[(size_t)slot[threadIdx.x] * mp + bid] = red[threadIdx.x];
 }
 __device__ __forceinline__ void bonded_body(const BondedArgs &A, int bid, int nblk) {
-  bonded_impl(A.coor, A.S, A.P, A.facc, A.ep, A.mp, bid);
+  bonded_impl(A.coor, A.S, A.P, A.bslot, A.ep, A.mp, bid);
 }
 DEFINE_KERNEL(k_bonded, BondedArgs, BT, bonded_body)
 
@@ -423,7 +455,7 @@ void bonded_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   B.coor = d_coor;
   B.S = make_seg(c);
   B.P = P;
-  B.facc = c->d_facc;
+  B.bslot = c->d_bslot;
   B.ep = c->d_epart;
   B.mp = c->max_parts;
   p->n_bonded = nblk;

@@ -28,6 +28,11 @@ struct moil_ctx {
   // generic (not intra-water) bonds, angles and exclusions; the
   // intra-water ones are listed per water in the d_wi_* arrays
   int gnb = 0, gnangl = 0, gnexcl = 0, wi_nterms = 0;
+  // bonded gradient slots ([nbslot][3] fixed point, written whole by the
+  // bonded kernel) and each atom's list of its slots (CSR, fixed topology)
+  size_t nbslot = 0;
+  long long *d_bslot = nullptr;
+  int32_t *d_brstart = nullptr, *d_brslot = nullptr;
   hipStream_t stream = nullptr;   // default stream of the context
   hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

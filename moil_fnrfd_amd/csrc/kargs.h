@@ -105,7 +105,7 @@ struct BondedArgs {
   const double *coor;
   BSeg S;
   BPar P;
-  long long *facc;
+  long long *bslot;  // per-term gradient slots, fixed point [nbslot][3]
   double *ep;
   int mp;
 };
@@ -193,6 +193,8 @@ struct FinArgs {
   size_t n;
   double inv;
   double *dpot;
+  const int32_t *brstart, *brslot;  // per-atom lists of bonded gradient slots
+  const long long *bslot;
 };
 struct EnCfg {
   int max_parts;
