@@ -99,6 +99,10 @@ def hetero_10k(seed=4000):
 
 
 def params_for(sysm, cut, ew, precision=abi.PREC_FP64, terms=abi.TERM_DEFAULT):
+    """Library parameters for a workload; cut['hvdw0'] = False switches
+    MOIL's hydrogen rule off (its 'hvdw' keyword)."""
+    if not cut.get('hvdw0', True):
+        terms &= ~abi.TERM_HVDW0
     return abi.make_params(sysm.box, rvmx=cut['rvmx'], relx=cut['relx'], rvbg=cut['rvbg'],
                            rebg=cut['rebg'], cutm=cut['cutm'], dtol=ew['dtol'], nfft=ew['nfft'],
                            order=ew.get('order', 4), eps=cut.get('eps', 1.0),

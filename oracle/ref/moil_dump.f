@@ -47,7 +47,7 @@ c
 
       character*512 pfile,confile,crdfile,outfile
       double precision rvmx,relx,rvbg,rebg,cutm,sigma
-      integer iord,dosqz,neval,nlist,iseed,dodump
+      integer iord,dosqz,neval,nlist,iseed,dodump,ihvdw
       integer i,k,ieval,ucon,ucrd,upar
       integer*8 c0,c1,crate,tlist,tforce
       double precision g1,g2
@@ -86,6 +86,11 @@ c
       read(upar,*) eps
       read(upar,*) dosqz,neval,nlist,sigma,iseed,dodump
       read(upar,'(a)') outfile
+c     optional last line: 0 switches the hydrogen rule off, as the 'hvdw'
+c     keyword of pot/energy.f:219 does (init_ef sets hvdw0 = .true.)
+      ihvdw = 1
+      read(upar,*,end=15) ihvdw
+ 15   if (ihvdw.eq.0) hvdw0 = .false.
       close(upar)
 
       esymyes = .true.

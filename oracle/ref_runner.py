@@ -70,6 +70,8 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
         f.write('%d %d %d %.17g %d %d\n' % (int(squeeze), neval, nlist, sigma, seed,
                                            int(dump_lists)))
         f.write(dmp + '\n')
+        # MOIL's hydrogen rule (pot/nbond.f:115), on unless cut['hvdw0'] is False
+        f.write('%d\n' % int(cut.get('hvdw0', True)))
     def _big_stack():
         # MOIL keeps large work arrays on the stack (e.g. symm/nbmsym.f:36-46)
         resource.setrlimit(resource.RLIMIT_STACK,

@@ -12,7 +12,15 @@ CASES = {
     'hetero_small': (configs.hetero_small, lambda s: perturb(s, 0.05, 2001), True),
     'hetero_small_out': (lambda: configs.hetero_small(seed=2100, n_chains=6, L=20.0),
                          lambda s: perturb(shift_out(s, n_out=20, seed=78), 0.05, 2101), False),
+    'hetero_small_nohvdw': (lambda: _no_hvdw(configs.hetero_small()),
+                            lambda s: perturb(s, 0.05, 2001), True),
 }
+
+
+def _no_hvdw(made):
+    """The same workload with MOIL's hydrogen rule off."""
+    s, cut, ew = made
+    return s, dict(cut, hvdw0=False), ew
 
 
 def make_case(name):

@@ -127,6 +127,9 @@ def main():
     write_case('hetero_small', perturb(s, 0.05, 2001), cut, ew, squeeze=True)
     s, cut, ew = configs.hetero_small(seed=2100, n_chains=6, L=20.0)
     write_case('hetero_small_out', perturb(shift_out(s, n_out=20, seed=78), 0.05, 2101), cut, ew)
+    s, cut, ew = configs.hetero_small()
+    write_case('hetero_small_nohvdw', perturb(s, 0.05, 2001), dict(cut, hvdw0=False), ew,
+               squeeze=True)
 
 
 if __name__ == '__main__':
