@@ -227,15 +227,16 @@ int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
  * parameter finite-difference loop aborted with the cause not identified)
  * they are replays of a captured hipGraph of the whole evaluation,
  * recaptured when the lists, the parameters or the pointers change.
- * moil_gpu_set_timing (which also clears) selects level 0 (off), 1 (the
- * real-space kernels are timed by events their own dispatch records; under
- * graph replay only the first evaluation after each such change runs in
- * order and is timed) or 2 (every evaluation in order, with event records
- * between its stages: PME, bonded, the whole eforce); moil_gpu_stage_times
- * returns the summed
- * milliseconds since the last call: t[0] real-space water kernel, t[1]
- * real-space atom kernel, t[2] PME, t[3] bonded + exclusion + 1-4, t[4]
- * whole eforce, t[5] the number of eforce calls timed. */
+ * moil_gpu_set_timing (which also clears) selects, in bits 0..7, level 0
+ * (off), 1 (the real-space kernels are timed by events their own dispatch
+ * records; under graph replay only the first evaluation after each such
+ * change runs in order and is timed) or 2 (every evaluation in order, with
+ * event records between its stages: PME, bonded, the whole eforce); bits
+ * 8..15 = n >= 2 time only every n-th evaluation at level 1 (the dispatch
+ * events cost ~2 % of the evaluation rate).  moil_gpu_stage_times returns
+ * the summed milliseconds since the last call: t[0] real-space water
+ * kernel, t[1] real-space atom kernel, t[2] PME, t[3] bonded + exclusion +
+ * 1-4, t[4] whole eforce, t[5] the number of eforce calls timed. */
 int32_t moil_gpu_set_timing(moil_ctx *ctx, int32_t level);
 int32_t moil_gpu_stage_times(moil_ctx *ctx, double t[8]);
 

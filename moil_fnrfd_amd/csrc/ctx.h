@@ -194,6 +194,8 @@ struct moil_ctx {
 
   // ---- timing (event pairs per stage, summed on request) ----
   int timing = 0;                  // 0 off, 1 real-space kernels, 2 all stages
+  int timing_stride = 1;           // level 1: every timing_stride-th evaluation
+  long timing_count = 0;
   long n_timed = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;  // stage, (start, stop)

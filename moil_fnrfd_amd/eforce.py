@@ -203,9 +203,10 @@ class MoilGPU:
         _check(load_library().moil_gpu_dump_atom_pairs(self._h, rows.ctypes.data, n.value, C.byref(n)))
         return rows[:n.value]
 
-    def set_timing(self, level=1):
-        """0 off, 1 real-space kernels only (no dispatch gaps), 2 all stages."""
-        _check(load_library().moil_gpu_set_timing(self._h, int(level)))
+    def set_timing(self, level=1, stride=1):
+        """0 off, 1 real-space kernels only (no dispatch gaps; every
+        stride-th evaluation), 2 all stages."""
+        _check(load_library().moil_gpu_set_timing(self._h, int(level) | (int(stride) << 8)))
 
     def stage_times(self):
         t = np.zeros(8)
