@@ -8,16 +8,19 @@
 // blocks and workgroup offsets (kargs.h, multi.h).  The 3-D transforms
 // run as one batched rocFFT plan per distinct grid size, on transform
 // buffers owned by the batch (the systems' spreading writes into them and
-// the gather reads from them).  The whole step -- about twenty kernel
-// launches plus two transforms per grid size, independent of the number
-// of systems -- is captured once into a hipGraph and replayed; it is
-// recaptured when a system's lists were rebuilt (chunk counts and list
-// buffers change) or the caller passes different device pointers.
-// Nothing captured is a memset: the per-evaluation clears are done by
-// the kernels themselves, and a context's one-time accumulator clear runs
-// eagerly before capture.  The caller's stream is ordered before and
-// after the batch through events.  Systems are laid out in the grids in
-// decreasing atom count, so the largest systems' workgroups start first.
+// the gather reads from them).  The step -- about twenty kernel launches
+// plus two transforms per grid size, independent of the number of systems
+// -- runs as two captured hipGraphs replayed side by side: the reciprocal
+// sums on one stream, the real-space part on another (each graph is one
+// stream's linear capture), joined by an event before the two assembly
+// launches.  The graphs are recaptured when a system's lists were rebuilt
+// (chunk counts and list buffers change), its parameters changed, or the
+// caller passes different device pointers.  Nothing captured is a memset:
+// the per-evaluation clears are done by the kernels themselves, and a
+// context's one-time accumulator clear runs eagerly before capture.  The
+// caller's stream is ordered before and after the batch through events.
+// Systems are laid out in the grids in decreasing atom count, so the
+// largest systems' workgroups start first.
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
@@ -101,10 +104,12 @@ struct moil_batch {
   std::vector<moil_ctx *> ctx;
   std::vector<int> order;  // systems by decreasing atom count
   bool f64 = false;
-  hipStream_t cap = nullptr;  // capture / launch stream
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // real space on cap, the reciprocal sums on cap2: each captured into a
+  // graph of its own (one stream each), forked and joined by events
+  hipStream_t cap = nullptr, cap2 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_pme = nullptr;
+  hipGraph_t graph = nullptr, graph2 = nullptr;
+  hipGraphExec_t exec = nullptr, exec2 = nullptr;
   std::vector<long> gen;           // list generation captured
   std::vector<long> pgen;          // parameter generation captured
   std::vector<const void *> key;   // pointers captured
@@ -119,10 +124,13 @@ struct moil_batch {
 // it ends before its exec (and the tables it reads) are replaced
 static void batch_drop_graph(moil_batch *b) {
   if (b->exec && b->cap) hipStreamSynchronize(b->cap);
+  if (b->exec2 && b->cap2) hipStreamSynchronize(b->cap2);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+  if (b->exec2) hipGraphExecDestroy(b->exec2);
+  if (b->graph2) hipGraphDestroy(b->graph2);
+  b->exec = b->exec2 = nullptr;
+  b->graph = b->graph2 = nullptr;
 }
 
 static void batch_free(moil_batch *b) {
@@ -140,7 +148,9 @@ static void batch_free(moil_batch *b) {
   hipFree(b->d_blob);
   if (b->ev_in) hipEventDestroy(b->ev_in);
   if (b->ev_out) hipEventDestroy(b->ev_out);
+  if (b->ev_pme) hipEventDestroy(b->ev_pme);
   if (b->cap) hipStreamDestroy(b->cap);
+  if (b->cap2) hipStreamDestroy(b->cap2);
   delete b;
 }
 
@@ -225,8 +235,10 @@ extern "C" int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nst
   std::stable_sort(b->order.begin(), b->order.end(),
                    [&](int x, int y) { return ctxs[x]->npt > ctxs[y]->npt; });
   bool ok = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->cap2, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_pme, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     batch_free(b);
     set_error("batch_create: stream/event creation failed");
@@ -244,6 +256,7 @@ extern "C" int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nst
 extern "C" int32_t moil_gpu_batch_destroy(moil_batch *b) {
   if (b) {
     hipStreamSynchronize(b->cap);
+    hipStreamSynchronize(b->cap2);
     batch_free(b);
   }
   return MOIL_GPU_OK;
@@ -330,8 +343,10 @@ static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *
   return MOIL_GPU_OK;
 }
 
-// the batch step: every kernel once over all systems, transforms per group
-static int32_t batch_enqueue(moil_batch *b, hipStream_t st) {
+// the batch step: every kernel once over all systems, transforms per
+// group; the reciprocal sums (spreading, transforms, convolution, gather)
+// and the real-space part are independent until the force assembly
+static int32_t batch_enqueue_pme(moil_batch *b, hipStream_t st) {
   const bool f = b->f64;
   launch_k_pme_tbin_multi(TAB_PTRS(b, tbin), st);
   launch_k_pme_tscan_multi(TAB_PTRS(b, tscan), st);
@@ -363,6 +378,12 @@ static int32_t batch_enqueue(moil_batch *b, hipStream_t st) {
   }
   if (f) launch_k_gather_f64_multi(TAB_PTRS(b, gather), st);
   else launch_k_gather_f32_multi(TAB_PTRS(b, gather), st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+static int32_t batch_enqueue_rs(moil_batch *b, hipStream_t st) {
+  const bool f = b->f64;
   launch_k_water_intra_multi(TAB_PTRS(b, intra), st);
   launch_k_bonded_multi(TAB_PTRS(b, bonded), st);
   if (f) {
@@ -376,9 +397,31 @@ static int32_t batch_enqueue(moil_batch *b, hipStream_t st) {
   if (f) launch_k_atoms_f64_multi(TAB_PTRS(b, atoms), st);
   else launch_k_atoms_f32_multi(TAB_PTRS(b, atoms), st);
   launch_k_atoms_reduce_multi(TAB_PTRS(b, ared), st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+static int32_t batch_enqueue_fin(moil_batch *b, hipStream_t st) {
   launch_k_finalize_forces_multi(TAB_PTRS(b, fin), st);
   launch_k_energies_multi(TAB_PTRS(b, en), st);
   HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
+}
+
+// one stream's work captured into a graph and instantiated
+static int32_t batch_capture(hipStream_t cs, int32_t (*enq)(moil_batch *, hipStream_t),
+                             moil_batch *b, hipGraph_t *graph, hipGraphExec_t *exec) {
+  HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  const int32_t rc = enq(b, cs);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(cs, &g);
+  if (rc != MOIL_GPU_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_CHECK(e);
+  *graph = g;
+  HIP_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
   return MOIL_GPU_OK;
 }
 
@@ -409,6 +452,7 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
             b->key[3 * i + 1] == d_dpot[i] && b->key[3 * i + 2] == d_energies[i];
   HIP_CHECK(hipEventRecord(b->ev_in, st));
   HIP_CHECK(hipStreamWaitEvent(b->cap, b->ev_in, 0));
+  HIP_CHECK(hipStreamWaitEvent(b->cap2, b->ev_in, 0));
   if (!fresh) {
     batch_drop_graph(b);
     // one-time accumulator clears (memsets) stay out of the graph
@@ -419,17 +463,8 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
         b->ctx[i]->facc_dirty = false;
       }
     CHECK_RC_B(batch_tables(b, d_coor, d_dpot, d_energies));
-    HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int32_t rc = batch_enqueue(b, b->cap);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->cap, &g);
-    if (rc != MOIL_GPU_OK) {
-      if (g) hipGraphDestroy(g);
-      return rc;
-    }
-    HIP_CHECK(e);
-    b->graph = g;
-    HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    CHECK_RC_B(batch_capture(b->cap, batch_enqueue_rs, b, &b->graph, &b->exec));
+    CHECK_RC_B(batch_capture(b->cap2, batch_enqueue_pme, b, &b->graph2, &b->exec2));
     b->gen.resize(n);
     b->pgen.resize(n);
     b->key.resize(3 * n);
@@ -441,7 +476,11 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
       b->key[3 * i + 2] = d_energies[i];
     }
   }
+  HIP_CHECK(hipGraphLaunch(b->exec2, b->cap2));
+  HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
   HIP_CHECK(hipGraphLaunch(b->exec, b->cap));
+  HIP_CHECK(hipStreamWaitEvent(b->cap, b->ev_pme, 0));
+  CHECK_RC_B(batch_enqueue_fin(b, b->cap));
   HIP_CHECK(hipEventRecord(b->ev_out, b->cap));
   HIP_CHECK(hipStreamWaitEvent(st, b->ev_out, 0));
   return MOIL_GPU_OK;
