@@ -191,6 +191,78 @@ def batched_leg(steps=10, nstreams=8):
     }
 
 
+def chain_leg(steps=5, igrid=32):
+    """MOIL's chain functional sds (chain/sds.f) on igrid replicas of the
+    batched workload's 12k-atom system (mixed precision): one
+    moil_gpu_chain_sds_dev call (list rebuild of every interior replica, one
+    batched evaluation, the chain terms) against MOIL's own pattern run
+    through the library one replica at a time (squeeze, lists, eforce per
+    interior replica on one context, as sds does with the Fortran shim)."""
+    import numpy as np
+    import torch
+
+    from moil_fnrfd_amd import abi, configs
+    from moil_fnrfd_amd.eforce import MoilChain, MoilGPU
+
+    dev = torch.device('cuda:0')
+    sh = torch.cuda.current_stream().cuda_stream
+    base, cut, ew = configs.batch_system(12000, 4000, 8)
+    npt = base.topo.npt
+    r = configs.chain_replicas(base.coor, igrid, seed=4001)
+    par = configs.params_for(base, cut, ew, precision=abi.PREC_MIXED)
+    ch = MoilChain([MoilGPU(base.topo, par) for _ in range(igrid - 2)], igrid)
+    d_r = torch.tensor(r, dtype=torch.float64, device=dev)
+    d_pick = torch.ones(npt, dtype=torch.int32, device=dev)
+    d_dv = torch.empty_like(d_r)
+    d_d0 = torch.empty(2 * igrid - 3, dtype=torch.float64, device=dev)
+    d_e1 = torch.empty_like(d_d0)
+    d_e0 = torch.zeros(igrid, dtype=torch.float64, device=dev)
+    d_s = torch.zeros(1, dtype=torch.float64, device=dev)
+    g, rho, lam = configs.CHAIN_GAMMA, configs.CHAIN_RHO, configs.CHAIN_LAMBDA
+
+    def chain():
+        ch.sds_dev(d_r.data_ptr(), d_pick.data_ptr(), g, rho, lam, d_dv.data_ptr(),
+                   d_d0.data_ptr(), d_e0.data_ptr(), d_e1.data_ptr(), d_s.data_ptr(), sh)
+
+    one = MoilGPU(base.topo, par)
+    scr = torch.empty_like(d_r[0])
+    dp = torch.empty_like(scr)
+    en = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
+
+    def per_replica():
+        for j in range(1, igrid - 1):
+            scr.copy_(d_r[j])
+            one.squeeze_dev(scr.data_ptr(), sh)
+            one.nbondm_dev(scr.data_ptr(), sh)
+            one.eforce_dev(scr.data_ptr(), dp.data_ptr(), en.data_ptr(), sh)
+
+    res = {}
+    for name, fn in (('chain', chain), ('per_replica', per_replica)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps
+    sener = float(d_s.cpu()[0])
+    ch.close()
+    one.close()
+    return {
+        'replicas': igrid,
+        'atoms_per_replica': npt,
+        'calls': steps,
+        'ms_per_call': 1e3 * res['chain'],
+        'per_replica_loop_ms': 1e3 * res['per_replica'],
+        'speedup': res['per_replica'] / res['chain'],
+        'sener_last': sener,
+        'workload': '%d replicas of the batched workload\'s 12k-atom system (straight line to '
+                    'a +N(0, 0.5 A) conformation, +N(0, 0.05 A) per replica), gamma=rho=100, '
+                    'lambda=2, all atoms selected; every call rebuilds the interior replicas\' '
+                    'lists as sds does' % igrid,
+    }
+
+
 def timed_loop(sysm, cut, ew, steps, warmup, seed):
     """The metric's loop on one system (mixed precision, coordinates and
     outputs resident in HBM): per step N(0, 0.01 A) noise added to the
@@ -433,6 +505,10 @@ def main():
     ap.add_argument('--batched-only', action='store_true',
                     help='run only the batched/plain leg and print its JSON')
     ap.add_argument('--batched-steps', type=int, default=10)
+    ap.add_argument('--no-chain', action='store_true',
+                    help='skip the chain-functional leg (32 replicas)')
+    ap.add_argument('--chain-only', action='store_true',
+                    help='run only the chain-functional leg and print its JSON')
     ap.add_argument('--no-adjoint', action='store_true',
                     help='skip the differentiation leg (128 conformations)')
     ap.add_argument('--large-only', action='store_true',
@@ -463,6 +539,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg()))
+        return
+    if args.chain_only:
+        print(json.dumps(chain_leg()))
         return
     if args.large_only:
         print(json.dumps(large_leg(steps=args.large_steps)))
@@ -516,6 +595,8 @@ def main():
     }
     if not args.no_batched:
         line['batched'] = batched_leg(steps=args.batched_steps)
+    if not args.no_chain:
+        line['chain'] = chain_leg()
     if not args.no_adjoint:
         line['adjoint'] = adjoint_leg()
     if not args.no_cpu_baseline:

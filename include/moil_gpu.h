@@ -185,6 +185,34 @@ int32_t moil_gpu_batch_eforce_dev(moil_batch *batch, const double *const *d_coor
                                   void *stream);
 int32_t moil_gpu_batch_destroy(moil_batch *batch);
 
+/* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
+ * gradient of a chain of igrid replicas of one system that the chain
+ * minimiser, annealer and chain dynamics evaluate (chain/chnneal.f:85,
+ * chain/chndyn.f:112, chain/pwl_chn.f:200):
+ *   S = sum_{j=2}^{igrid-1} e_total(r_j) + gamma sum_j (d_j - <d>)^2
+ *       + rho/lambda sum_j exp(-lambda D_j / <d>^2),
+ * d_j = |r_{j+1}-r_j|, D_j = |r_{j+2}-r_j|^2 over atoms with ipick > 0.
+ * moil_gpu_chain_create takes igrid-2 contexts, one per interior replica
+ * (same topology and parameters, same precision; igrid >= 3).  A call
+ * squeezes a copy of each interior replica and rebuilds its lists, as
+ * sds does on every call (chain/sds.f:84-86), evaluates the interior
+ * replicas as one batch, and adds the chain terms.  Arrays as sds's
+ * arguments: r and dv (3, igrid*npt) replica-major; ipick (npt);
+ * d0 and e1 (2*igrid-3); e0 (igrid), of which entries 2..igrid-1
+ * (1-based) are written.  dv must not alias r.  The _dev variant takes
+ * device pointers (sener a device scalar) and is ordered on `stream`;
+ * it synchronizes that stream once per replica for the list builds. */
+typedef struct moil_chain moil_chain;
+int32_t moil_gpu_chain_create(moil_ctx **interior, int32_t igrid, moil_chain **out);
+int32_t moil_gpu_chain_sds(moil_chain *chain, const double *r, const int32_t *ipick,
+                           double gamma, double rho, double lambda, double *sener,
+                           double *dv, double *d0, double *e0, double *e1);
+int32_t moil_gpu_chain_sds_dev(moil_chain *chain, const double *d_r, const int32_t *d_ipick,
+                               double gamma, double rho, double lambda, double *d_dv,
+                               double *d_d0, double *d_e0, double *d_e1, double *d_sener,
+                               void *stream);
+int32_t moil_gpu_chain_destroy(moil_chain *chain);
+
 /* Force-field parameters and the energy's gradient with respect to them
  * (no counterpart in MOIL; for force-field fitting and the differentiable
  * binding).  moil_gpu_set_params replaces the per-atom charges and LJ

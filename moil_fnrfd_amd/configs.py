@@ -157,3 +157,36 @@ def batch_workload(seed=3000, n_systems=64, n_slices=32, slice_atoms=12000):
         s.coor = (1.0 - lam) * base.coor + lam * far
         out.append((s, cut, ew))
     return out
+
+
+# MOIL's chain minimiser defaults (chain/chmin.f:134-145)
+CHAIN_GAMMA = 100.0
+CHAIN_RHO = 100.0
+CHAIN_LAMBDA = 2.0
+
+
+def chain_replicas(base_coor, igrid, seed, end_sigma=0.5, noise=0.05):
+    """igrid replicas of one system for the chain functional: replica j
+    lies at fraction j/(igrid-1) of the straight line from base_coor to
+    base_coor + N(0, end_sigma) per coordinate, interior and last replicas
+    displaced by a further N(0, noise).  Returns (igrid, npt, 3) float64."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    base = np.asarray(base_coor, np.float64)
+    far = base + rng.normal(0.0, end_sigma, base.shape)
+    r = np.empty((igrid,) + base.shape)
+    for j in range(igrid):
+        lam = j / max(igrid - 1, 1)
+        r[j] = (1.0 - lam) * base + lam * far
+        if j > 0:
+            r[j] += rng.normal(0.0, noise, base.shape)
+    return r
+
+
+def chain_pick(npt, every=5, skip=3):
+    """A partial selection (MOIL's ipick): atom i (0-based) is left out
+    when i % every == skip."""
+    import numpy as np
+    p = np.ones(npt, np.int32)
+    p[skip::every] = 0
+    return p

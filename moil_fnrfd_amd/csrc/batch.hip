@@ -118,7 +118,17 @@ struct moil_batch {
   Tables tab;
   char *d_blob = nullptr;
   size_t blob_cap = 0;
+  // eager: the step's kernels are launched directly on the two streams
+  // instead of replaying graphs (for callers that rebuild lists before
+  // every evaluation, where each call would otherwise recapture)
+  bool eager = false;
+  bool have_tables = false;  // eager: tables match gen/pgen/key
 };
+
+void batch_set_eager(moil_batch *b, bool on) {
+  b->eager = on;
+  b->have_tables = false;
+}
 
 // a replay of the old graph may still be in flight on the capture stream:
 // it ends before its exec (and the tables it reads) are replaced
@@ -445,7 +455,8 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
   }
   hipStream_t st = (hipStream_t)stream;
   // (re)capture when lists or pointers changed
-  bool fresh = b->exec != nullptr && (int)b->gen.size() == n && (int)b->pgen.size() == n;
+  bool fresh = (b->eager ? b->have_tables : b->exec != nullptr) && (int)b->gen.size() == n &&
+               (int)b->pgen.size() == n;
   for (int i = 0; fresh && i < n; i++)
     fresh = b->gen[i] == b->ctx[i]->list_gen && b->pgen[i] == b->ctx[i]->param_gen &&
             b->key[3 * i] == d_coor[i] &&
@@ -455,6 +466,13 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
   HIP_CHECK(hipStreamWaitEvent(b->cap2, b->ev_in, 0));
   if (!fresh) {
     batch_drop_graph(b);
+    if (b->eager) {
+      // the previous call's kernels read the argument tables about to be
+      // replaced
+      HIP_CHECK(hipStreamSynchronize(b->cap));
+      HIP_CHECK(hipStreamSynchronize(b->cap2));
+      b->have_tables = false;
+    }
     // one-time accumulator clears (memsets) stay out of the graph
     for (int i = 0; i < n; i++)
       if (b->ctx[i]->facc_dirty) {
@@ -463,8 +481,12 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
         b->ctx[i]->facc_dirty = false;
       }
     CHECK_RC_B(batch_tables(b, d_coor, d_dpot, d_energies));
-    CHECK_RC_B(batch_capture(b->cap, batch_enqueue_rs, b, &b->graph, &b->exec));
-    CHECK_RC_B(batch_capture(b->cap2, batch_enqueue_pme, b, &b->graph2, &b->exec2));
+    if (b->eager) {
+      b->have_tables = true;
+    } else {
+      CHECK_RC_B(batch_capture(b->cap, batch_enqueue_rs, b, &b->graph, &b->exec));
+      CHECK_RC_B(batch_capture(b->cap2, batch_enqueue_pme, b, &b->graph2, &b->exec2));
+    }
     b->gen.resize(n);
     b->pgen.resize(n);
     b->key.resize(3 * n);
@@ -476,9 +498,15 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
       b->key[3 * i + 2] = d_energies[i];
     }
   }
-  HIP_CHECK(hipGraphLaunch(b->exec2, b->cap2));
-  HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
-  HIP_CHECK(hipGraphLaunch(b->exec, b->cap));
+  if (b->eager) {
+    CHECK_RC_B(batch_enqueue_pme(b, b->cap2));
+    HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
+    CHECK_RC_B(batch_enqueue_rs(b, b->cap));
+  } else {
+    HIP_CHECK(hipGraphLaunch(b->exec2, b->cap2));
+    HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
+    HIP_CHECK(hipGraphLaunch(b->exec, b->cap));
+  }
   HIP_CHECK(hipStreamWaitEvent(b->cap, b->ev_pme, 0));
   CHECK_RC_B(batch_enqueue_fin(b, b->cap));
   HIP_CHECK(hipEventRecord(b->ev_out, b->cap));

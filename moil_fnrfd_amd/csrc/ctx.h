@@ -261,4 +261,7 @@ void pme_free(moil_ctx *c);
 // parameter gradients (adjoint.hip): dq, de12, de6 per atom
 int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_de12,
                     double *d_de6, hipStream_t st);
+// batched evaluation launched directly on its streams instead of through
+// captured graphs (batch.hip)
+void batch_set_eager(moil_batch *b, bool on);
 // The launches of an evaluation are described by plans (kargs.h).

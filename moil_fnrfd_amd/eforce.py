@@ -76,13 +76,19 @@ def load_library():
     L.moil_gpu_set_params.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.moil_gpu_param_grad.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_param_grad_dev.argtypes = [vp, vp, vp, vp, vp, vp]
+    dbl = C.c_double
+    L.moil_gpu_chain_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.moil_gpu_chain_sds.argtypes = [vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, vp, vp]
+    L.moil_gpu_chain_sds_dev.argtypes = [vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, vp, vp, vp]
+    L.moil_gpu_chain_destroy.argtypes = [vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
                  'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_batch_create',
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
-                 'moil_gpu_param_grad', 'moil_gpu_param_grad_dev'):
+                 'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
+                 'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -246,6 +252,65 @@ class MoilBatch:
     def close(self):
         if getattr(self, '_h', None):
             load_library().moil_gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MoilChain:
+    """MOIL's chain functional sds (chain/sds.f) over igrid replicas of one
+    system (moil_gpu_chain_*): `interior` are the igrid-2 MoilGPU contexts
+    of replicas 2..igrid-1, built from the same topology and parameters.
+    Each call squeezes copies of the interior replicas, rebuilds their
+    lists, evaluates them as one batch and adds the chain's spring and
+    repulsion terms."""
+
+    def __init__(self, interior, igrid):
+        if len(interior) != igrid - 2:
+            raise MoilGPUError('a chain of %d replicas takes %d interior contexts'
+                               % (igrid, igrid - 2))
+        arr = (C.c_void_p * len(interior))(*[m._h for m in interior])
+        h = C.c_void_p()
+        _check(load_library().moil_gpu_chain_create(arr, int(igrid), C.byref(h)))
+        self._h = h
+        self._keep = list(interior)
+        self.igrid = int(igrid)
+        self.npt = interior[0].npt
+
+    def sds(self, r, ipick, gamma, rho, lam):
+        """r (igrid, npt, 3) float64, ipick (npt,) -> dict(sener, dv
+        (igrid, npt, 3), d0, e0, e1) with sds's meanings (e0 of the end
+        replicas zero)."""
+        r = np.ascontiguousarray(r, np.float64)
+        if r.shape != (self.igrid, self.npt, 3):
+            raise MoilGPUError('r must be (%d, %d, 3)' % (self.igrid, self.npt))
+        pick = np.ascontiguousarray(ipick, np.int32)
+        if pick.shape != (self.npt,):
+            raise MoilGPUError('ipick must have %d entries' % self.npt)
+        nd = 2 * self.igrid - 3
+        dv = np.empty_like(r)
+        d0 = np.empty(nd)
+        e1 = np.empty(nd)
+        e0 = np.zeros(self.igrid)
+        s = C.c_double()
+        _check(load_library().moil_gpu_chain_sds(
+            self._h, r.ctypes.data, pick.ctypes.data, float(gamma), float(rho), float(lam),
+            C.byref(s), dv.ctypes.data, d0.ctypes.data, e0.ctypes.data, e1.ctypes.data))
+        return dict(sener=s.value, dv=dv, d0=d0, e0=e0, e1=e1)
+
+    def sds_dev(self, d_r, d_pick, gamma, rho, lam, d_dv, d_d0, d_e0, d_e1, d_sener, stream=0):
+        """Device pointers, as moil_gpu_chain_sds_dev."""
+        _check(load_library().moil_gpu_chain_sds_dev(
+            self._h, d_r, d_pick, float(gamma), float(rho), float(lam), d_dv, d_d0, d_e0, d_e1,
+            d_sener, C.c_void_p(stream or 0)))
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load_library().moil_gpu_chain_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -91,3 +91,57 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
             d['tlist_s'] = float(kv['tlist_s'])
             d['tforce_s'] = float(kv['tforce_s'])
     return d
+
+
+SDS_EXE = os.path.join(HERE, '_ref', 'moil_sds')
+# MOIL's unmodified chain functional over the product's Fortran shim
+SDS_SHIM_EXE = os.path.join(HERE, '_ref', 'moil_sds_gpu')
+
+
+def run_moil_sds(sysm: System, cut: dict, ewald: dict, r, ipick, gamma, rho, lam, *,
+                 workdir=None, timeout=3600, exe=SDS_EXE, env=None) -> dict:
+    """MOIL's chain functional (chain/sds.f) over replicas r (igrid, npt, 3)
+    of sysm's topology, via oracle/_ref/moil_sds (or SDS_SHIM_EXE).
+    Returns the parsed dump: SENER, DV, D0, E0, E1 as sds returns them,
+    plus DPOTS/ETOTS (each interior replica's own gradient and energy)
+    and BETA."""
+    if not available(exe):
+        raise RuntimeError('%s is not built (make -f oracle/ref/Makefile sds)' % exe)
+    r = np.ascontiguousarray(r, np.float64)
+    igrid = r.shape[0]
+    tmp = workdir or tempfile.mkdtemp(prefix='moilsds_')
+    os.makedirs(tmp, exist_ok=True)
+    con = os.path.join(tmp, 'sys.con')
+    rf = os.path.join(tmp, 'chain.bin')
+    pk = os.path.join(tmp, 'pick.bin')
+    par = os.path.join(tmp, 'sds.par')
+    dmp = os.path.join(tmp, 'sds.dump')
+    write_con(con, sysm.topo)
+    r.tofile(rf)
+    np.ascontiguousarray(ipick, np.int32).tofile(pk)
+    a, b, c = sysm.box
+    n1, n2, n3 = ewald['nfft']
+    with open(par, 'w') as f:
+        f.write(con + '\n' + rf + '\n' + pk + '\n')
+        f.write('%.17g %.17g %.17g\n' % (a, b, c))
+        f.write('%.17g %.17g %.17g %.17g %.17g\n' % (cut['rvmx'], cut['relx'], cut['rvbg'],
+                                                    cut['rebg'], cut['cutm']))
+        f.write('%.17g %d %d %d %d\n' % (ewald['dtol'], n1, n2, n3, ewald.get('order', 4)))
+        f.write('%.17g\n' % cut.get('eps', 1.0))
+        f.write('%d %.17g %.17g %.17g\n' % (igrid, gamma, rho, lam))
+        f.write(dmp + '\n')
+        f.write('%d\n' % int(cut.get('hvdw0', True)))
+
+    def _big_stack():
+        resource.setrlimit(resource.RLIMIT_STACK,
+                           (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    full_env = dict(os.environ)
+    full_env.update(env or {})
+    p = subprocess.run([exe, par], cwd=tmp, capture_output=True, text=True, timeout=timeout,
+                       preexec_fn=_big_stack, env=full_env)
+    if p.returncode != 0 or not os.path.exists(dmp):
+        raise RuntimeError('moil_sds failed (%d):\n%s\n%s' % (p.returncode, p.stdout[-3000:],
+                                                               p.stderr[-3000:]))
+    d = parse_dump(dmp)
+    d['stdout'] = p.stdout
+    return d

@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 
 from moil_fnrfd_amd import configs  # noqa: E402
 from oracle.oracle import ref_canonical_atom_pairs, ref_canonical_water_pairs, sort_rows  # noqa: E402
-from oracle.ref_runner import run_moil  # noqa: E402
+from oracle.ref_runner import run_moil, run_moil_sds  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 ENERGY_FIELDS = ['ewaldcof', 'e_self', 'e_corr', 'e_ew_receip', 'e_vdw', 'e_el', 'e_vsym',
@@ -110,6 +110,41 @@ def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None
              os.path.getsize(path)))
 
 
+def chain_case():
+    """The chain fixture's inputs: 5 replicas of the 648-atom water box and
+    a partial atom selection, MOIL's chain minimiser defaults."""
+    s, cut, ew = configs.water216()
+    r = configs.chain_replicas(s.coor, 5, seed=5100)
+    return s, cut, ew, r, configs.chain_pick(s.topo.npt)
+
+
+def write_chain_case(name='chain_water216'):
+    """MOIL's chain functional (chain/sds.f via oracle/_ref/moil_sds): its
+    outputs, and each interior replica's own energy and gradient."""
+    s, cut, ew, r, pick = chain_case()
+    g, rho, lam = configs.CHAIN_GAMMA, configs.CHAIN_RHO, configs.CHAIN_LAMBDA
+    d = run_moil_sds(s, cut, ew, r, pick, g, rho, lam)
+    igrid, npt = r.shape[0], r.shape[1]
+    out = dict(
+        r_sha=np.frombuffer(hashlib.sha256(np.ascontiguousarray(r).tobytes()).hexdigest()
+                            .encode(), np.uint8),
+        topo_sha=np.frombuffer(topo_checksum(s.topo).encode(), np.uint8),
+        params=np.array([g, rho, lam]),
+        sener=d['SENER'],
+        dv=d['DV'].reshape(igrid, npt, 3),
+        d0=d['D0'],
+        e0=d['E0'],
+        e1=d['E1'],
+        dpots=d['DPOTS'].reshape(igrid, npt, 3)[1:-1],
+        etots=d['ETOTS'],
+        beta=d['BETA'],
+    )
+    path = os.path.join(OUT, name + '.npz')
+    np.savez_compressed(path, **out)
+    print('%-22s igrid=%d npt=%d sener=%.10f -> %s (%d bytes)'
+          % (name, igrid, npt, d['SENER'][0], os.path.basename(path), os.path.getsize(path)))
+
+
 def main():
     s, cut, ew = configs.water216()
     write_case('water216', s, cut, ew)
@@ -130,6 +165,7 @@ def main():
     s, cut, ew = configs.hetero_small()
     write_case('hetero_small_nohvdw', perturb(s, 0.05, 2001), dict(cut, hvdw0=False), ew,
                squeeze=True)
+    write_chain_case()
 
 
 if __name__ == '__main__':
