@@ -201,7 +201,9 @@ int32_t moil_gpu_batch_destroy(moil_batch *batch);
  * d0 and e1 (2*igrid-3); e0 (igrid), of which entries 2..igrid-1
  * (1-based) are written.  dv must not alias r.  The _dev variant takes
  * device pointers (sener a device scalar) and is ordered on `stream`;
- * it synchronizes that stream once per replica for the list builds. */
+ * the replicas' list builds run side by side on their contexts' own
+ * streams, and the call waits on the host once for all of them (again
+ * for a replica whose lists outgrew their buffers). */
 typedef struct moil_chain moil_chain;
 int32_t moil_gpu_chain_create(moil_ctx **interior, int32_t igrid, moil_chain **out);
 int32_t moil_gpu_chain_sds(moil_chain *chain, const double *r, const int32_t *ipick,

@@ -606,6 +606,49 @@ extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, v
   return MOIL_GPU_OK;
 }
 
+// Lists of n systems at once: each system's squeeze (optional) and build
+// run on its own context stream after `ready` on st, with no host wait
+// between systems; the status words of all builds come back in one copy
+// (h_stat: n * ST_COUNT pinned words; d_stat: as many device words).
+// A system whose lists did not fit is rebuilt alone after its buffers
+// grew.  st waits for every build.
+int32_t build_lists_many(moil_ctx *const *cs, double *const *d_coor, int n, bool squeeze,
+                         hipStream_t st, int32_t *d_stat, int32_t *h_stat, hipEvent_t ready,
+                         const hipEvent_t *done) {
+  HIP_CHECK(hipEventRecord(ready, st));
+  for (int k = 0; k < n; k++) {
+    moil_ctx *c = cs[k];
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipStreamWaitEvent(s, ready, 0));
+    if (squeeze) CHECK_RC(moil_gpu_squeeze_dev(c, d_coor[k], s));
+    CHECK_RC(lists_build(c, d_coor[k], s));
+    CHECK_RC(atoms_build(c, d_coor[k], s));
+    HIP_CHECK(hipMemcpyAsync(d_stat + (size_t)ST_COUNT * k, c->d_status,
+                             sizeof(int32_t) * ST_COUNT, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipEventRecord(done[k], s));
+    HIP_CHECK(hipStreamWaitEvent(st, done[k], 0));
+  }
+  HIP_CHECK(hipMemcpyAsync(h_stat, d_stat, sizeof(int32_t) * ST_COUNT * n,
+                           hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  for (int k = 0; k < n; k++) {
+    moil_ctx *c = cs[k];
+    const int32_t *status = h_stat + (size_t)ST_COUNT * k;
+    const bool w = lists_grow(c, status), a = atoms_grow(c, status);
+    if (w || a) {
+      // buffers grew: the full build with its own retries (squeezed already)
+      CHECK_RC(moil_gpu_build_lists_dev(c, d_coor[k], st));
+      continue;
+    }
+    CHECK_RC(lists_finish(c, status));
+    CHECK_RC(atoms_finish(c, status));
+    CHECK_RC(ensure_parts(c, c->nwchunk + c->nachunk));
+    c->have_lists = true;
+    c->list_gen++;
+  }
+  return MOIL_GPU_OK;
+}
+
 extern "C" int32_t moil_gpu_build_lists(moil_ctx *c, const double *coor) {
   if (!c || !coor) return fail(MOIL_GPU_EINVAL, "null argument");
   HIP_CHECK(hipMemcpyAsync(c->d_coor, coor, sizeof(double) * 3 * c->npt, hipMemcpyHostToDevice, c->stream));

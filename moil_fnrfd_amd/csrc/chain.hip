@@ -12,7 +12,9 @@
 // One call:
 //   1. copies the interior replicas into scratch (sds.f:75-80 copies each
 //      into coor, so squeeze never moves the caller's replicas);
-//   2. squeezes each copy and rebuilds its lists (sds.f:84-86);
+//   2. squeezes each copy and rebuilds its lists (sds.f:84-86), the
+//      replicas' builds queued side by side on their own streams with one
+//      host wait for all;
 //   3. evaluates all interior replicas in one batched evaluation, whose
 //      gradients land straight in their dv rows (sds.f:87-106);
 //   4. runs three small kernels for the chain terms:
@@ -221,6 +223,10 @@ struct moil_chain {
   double *d_en = nullptr;    // (nin, MOIL_N_ENERGIES)
   double *d_part = nullptr;  // (2 igrid - 3, nblk)
   double *d_coef = nullptr;  // 3 igrid
+  // list builds: status words (device, pinned host), fork/join events
+  int32_t *d_stat = nullptr, *h_stat = nullptr;
+  hipEvent_t ready = nullptr;
+  std::vector<hipEvent_t> done;
   // staging of the host entry point
   double *d_r = nullptr, *d_dv = nullptr, *d_out = nullptr;
   int32_t *d_pick = nullptr;
@@ -238,6 +244,11 @@ static void chain_free(moil_chain *ch) {
   hipFree(ch->d_dv);
   hipFree(ch->d_out);
   hipFree(ch->d_pick);
+  hipFree(ch->d_stat);
+  if (ch->h_stat) hipHostFree(ch->h_stat);
+  if (ch->ready) hipEventDestroy(ch->ready);
+  for (hipEvent_t e : ch->done)
+    if (e) hipEventDestroy(e);
   delete ch;
 }
 
@@ -271,8 +282,16 @@ extern "C" int32_t moil_gpu_chain_create(moil_ctx **slices, int32_t igrid, moil_
       hipMalloc(&ch->d_scr, sizeof(double) * n3 * nin) == hipSuccess &&
       hipMalloc(&ch->d_en, sizeof(double) * MOIL_N_ENERGIES * nin) == hipSuccess &&
       hipMalloc(&ch->d_part, sizeof(double) * (2 * igrid - 3) * (size_t)ch->nblk) == hipSuccess &&
-      hipMalloc(&ch->d_coef, sizeof(double) * 3 * igrid) == hipSuccess;
-  if (!ok) {
+      hipMalloc(&ch->d_coef, sizeof(double) * 3 * igrid) == hipSuccess &&
+      hipMalloc(&ch->d_stat, sizeof(int32_t) * ST_COUNT * nin) == hipSuccess &&
+      hipHostMalloc(&ch->h_stat, sizeof(int32_t) * ST_COUNT * nin, hipHostMallocDefault) ==
+          hipSuccess &&
+      hipEventCreateWithFlags(&ch->ready, hipEventDisableTiming) == hipSuccess;
+  ch->done.assign(nin, nullptr);
+  bool ev_ok = ok;
+  for (int k = 0; ev_ok && k < nin; k++)
+    ev_ok = hipEventCreateWithFlags(&ch->done[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok || !ev_ok) {
     chain_free(ch);
     set_error("chain_create: device allocation failed");
     return MOIL_GPU_EHIP;
@@ -304,18 +323,21 @@ extern "C" int32_t moil_gpu_chain_sds_dev(moil_chain *ch, const double *d_r,
   const size_t n3 = 3 * (size_t)npt;
   HIP_CHECK(hipMemcpyAsync(ch->d_scr, d_r + n3, sizeof(double) * n3 * nin,
                            hipMemcpyDeviceToDevice, st));
+  std::vector<double *> xm(nin);
   std::vector<const double *> xs(nin);
   std::vector<double *> gs(nin), es(nin);
   for (int k = 0; k < nin; k++) {
-    double *x = ch->d_scr + n3 * k;
-    int32_t rc = moil_gpu_squeeze_dev(ch->ctx[k], x, st);
-    if (rc == MOIL_GPU_OK) rc = moil_gpu_build_lists_dev(ch->ctx[k], x, st);
-    if (rc != MOIL_GPU_OK) return rc;
-    xs[k] = x;
+    xm[k] = ch->d_scr + n3 * k;
+    xs[k] = xm[k];
     gs[k] = d_dv + n3 * (k + 1);
     es[k] = ch->d_en + (size_t)MOIL_N_ENERGIES * k;
   }
-  int32_t rc = moil_gpu_batch_eforce_dev(ch->batch, xs.data(), gs.data(), es.data(), st);
+  // every replica squeezed and its lists rebuilt, the builds side by side
+  // on the replicas' own streams, one host wait for all
+  int32_t rc = build_lists_many(ch->ctx.data(), xm.data(), nin, true, st, ch->d_stat,
+                                ch->h_stat, ch->ready, ch->done.data());
+  if (rc != MOIL_GPU_OK) return rc;
+  rc = moil_gpu_batch_eforce_dev(ch->batch, xs.data(), gs.data(), es.data(), st);
   if (rc != MOIL_GPU_OK) return rc;
   k_chain_dist<<<ch->nblk, CH_BLOCK, 0, st>>>(d_r, d_pick, npt, igrid, ch->d_part);
   k_chain_coef<<<1, CH_BLOCK, 0, st>>>(ch->d_part, ch->nblk, igrid, ch->d_en, gamma, rho, lambda,

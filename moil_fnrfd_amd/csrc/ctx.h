@@ -261,6 +261,11 @@ void pme_free(moil_ctx *c);
 // parameter gradients (adjoint.hip): dq, de12, de6 per atom
 int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_de12,
                     double *d_de6, hipStream_t st);
+// lists of n systems, their builds queued on the systems' own streams with
+// one host wait for all (api.hip)
+int32_t build_lists_many(moil_ctx *const *cs, double *const *d_coor, int n, bool squeeze,
+                         hipStream_t st, int32_t *d_stat, int32_t *h_stat, hipEvent_t ready,
+                         const hipEvent_t *done);
 // batched evaluation launched directly on its streams instead of through
 // captured graphs (batch.hip)
 void batch_set_eager(moil_batch *b, bool on);
