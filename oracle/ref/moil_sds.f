@@ -49,6 +49,7 @@ c
       double precision gamma,rho,lambda,sener
       integer iord,igrid,ihvdw,nselec,ntot
       integer i,j,k,m,ucon,ur,upar,u
+      integer*8 c0,c1,crate
       double precision, allocatable :: r(:,:),dv(:,:),d0(:),e0(:)
       double precision, allocatable :: e1(:),dps(:,:),ets(:)
       integer, allocatable :: ipick(:),pointr(:)
@@ -139,8 +140,13 @@ c     cutoffs exactly as pot/energy.f:382-400 (cutm always explicit)
       do 25 j=1,igrid
          e0(j) = 0.d0
  25   continue
+      call system_clock(count_rate=crate)
+      call system_clock(c0)
       call sds(sener,dv,r,d0,e0,e1,nselec,pointr,ipick,npt,3*ntot,
      1         gamma,rho,lambda,.true.,.false.)
+      call system_clock(c1)
+      write(stdo,'(a,es16.8)') 'MOILTIME sds_s=',
+     1     dble(c1-c0)/dble(crate)
 
 c     each interior replica's own energy and gradient, as sds obtains
 c     them (chain/sds.f:71-110); the end replicas get none

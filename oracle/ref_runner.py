@@ -96,6 +96,9 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
 SDS_EXE = os.path.join(HERE, '_ref', 'moil_sds')
 # MOIL's unmodified chain functional over the product's Fortran shim
 SDS_SHIM_EXE = os.path.join(HERE, '_ref', 'moil_sds_gpu')
+# the same driver with sds itself replaced by the chain shim
+# (moil_fnrfd_amd/fortran/moil_gpu_shim_chain.f -> moil_gpu_chain_sds)
+SDS_CHAIN_EXE = os.path.join(HERE, '_ref', 'moil_sds_gpuchain')
 
 
 def run_moil_sds(sysm: System, cut: dict, ewald: dict, r, ipick, gamma, rho, lam, *,
@@ -144,4 +147,7 @@ def run_moil_sds(sysm: System, cut: dict, ewald: dict, r, ipick, gamma, rho, lam
                                                                p.stderr[-3000:]))
     d = parse_dump(dmp)
     d['stdout'] = p.stdout
+    for line in p.stdout.splitlines():
+        if line.startswith('MOILTIME'):
+            d['sds_s'] = float(line.split('=')[1])
     return d

@@ -156,20 +156,24 @@ def test_chain_rejects_bad_arguments():
     ch.close()
 
 
-@pytest.mark.skipif(not ref_runner.available(ref_runner.SDS_SHIM_EXE),
-                    reason='oracle/_ref/moil_sds_gpu not built')
+@pytest.mark.parametrize('exe', ['sds_over_shim', 'chain_shim'])
 @pytest.mark.parametrize('fp64', [True, False])
-def test_moil_sds_through_shim(fp64):
-    """MOIL's unmodified chain functional calling the GPU library through
-    the Fortran shim, one replica at a time."""
+def test_moil_sds_through_shim(exe, fp64):
+    """MOIL's chain driver calling the GPU library through the Fortran
+    shims: MOIL's unmodified sds with each replica's lists and eforce on the
+    GPU one at a time (sds_over_shim), or sds itself replaced by
+    moil_gpu_chain_sds (chain_shim)."""
+    path = ref_runner.SDS_SHIM_EXE if exe == 'sds_over_shim' else ref_runner.SDS_CHAIN_EXE
+    if not ref_runner.available(path):
+        pytest.skip('%s not built' % path)
     s, cut, ew, r, pick = chain_case()
     g = load_golden('chain_water216')
-    print('case chain_water216 through the shim fp64=%s' % fp64, flush=True)
-    d = ref_runner.run_moil_sds(s, cut, ew, r, pick, *g['params'], timeout=300,
-                                exe=ref_runner.SDS_SHIM_EXE,
+    print('case chain_water216 through %s fp64=%s' % (exe, fp64), flush=True)
+    d = ref_runner.run_moil_sds(s, cut, ew, r, pick, *g['params'], timeout=300, exe=path,
                                 env={'MOIL_GPU_FP64': '1' if fp64 else '0'})
     scale = float(np.abs(g['etots']).sum())
     tol_e, tol_g = (1e-12, 1e-10) if fp64 else (1e-6, 1e-5)
     assert abs(d['SENER'][0] - g['sener'][0]) <= tol_e * 10 * scale
     assert rms_rel(d['DV'], g['dv'].reshape(-1)) < tol_g
-    assert np.allclose(d['D0'], g['d0'], rtol=1e-15, atol=0)
+    assert np.allclose(d['D0'], g['d0'], rtol=1e-13, atol=0)
+    assert np.allclose(d['E0'][1:-1], g['e0'][1:-1], rtol=0, atol=tol_e * 10 * scale)
