@@ -369,24 +369,28 @@ def large_leg(steps=200, warmup=20):
     }
 
 
-def adjoint_leg(n_conf=128, seed=4001):
+def adjoint_leg(n_conf=128, seed=4001, group=8):
     """Differentiation workload: the 10,048-atom heteropolymer-in-water
     system, n_conf conformations (base + N(0, 0.1 A) per coordinate).  Per
     conformation, timed: list build, forward energy + dE/dx, and the
     parameter adjoint (dE/dq per atom, dE/deps and dE/dRmin per LJ type)
-    in mixed precision (FP32 pairs, FP64 reductions).  Untimed: the same in
-    FP64 for the error report (max and RMS relative over all
-    conformations), and central finite differences (h = 1e-4, FP64) of a
-    few parameters on the first conformation."""
+    in mixed precision (FP32 pairs, FP64 reductions), `group` conformations
+    at a time on as many contexts: one batched list build, one batched
+    evaluation, the adjoints side by side on their own streams.  Untimed:
+    the same one conformation at a time in FP64 for the error report (max
+    and RMS relative over all conformations), and central finite
+    differences (h = 1e-4, FP64) of a few parameters on the first
+    conformation."""
     import numpy as np
     import torch
 
     from moil_fnrfd_amd import abi, configs
     from moil_fnrfd_amd.adjoint import lj_factors, type_gradients
-    from moil_fnrfd_amd.eforce import MoilGPU
+    from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
 
     dev = torch.device('cuda:0')
-    sh = torch.cuda.current_stream().cuda_stream
+    main = torch.cuda.current_stream()
+    sh = main.cuda_stream
     s, cut, ew = configs.hetero_10k()
     m = s.meta
     e12, e6 = (a.numpy() for a in lj_factors(torch.tensor(m['lj_eps']),
@@ -406,7 +410,32 @@ def adjoint_leg(n_conf=128, seed=4001):
                torch.zeros((n_conf, abi.N_ENERGIES), dtype=torch.float64, device=dev))
            for p in eng}
 
-    def run(prec):
+    # the mixed-precision group: `group` contexts, one batch over them
+    mix = [eng[abi.PREC_MIXED]]
+    for _ in range(group - 1):
+        g = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+        g.set_params(s.topo.ptchg, e6, e12, 0.5, 0.5)
+        mix.append(g)
+    batch = MoilBatch(mix)
+    batch.set_graph(False)  # lists change before every evaluation
+    side = [torch.cuda.Stream() for _ in range(group)]
+
+    def run_grouped():
+        dp, pg, en = out[abi.PREC_MIXED]
+        for b0 in range(0, n_conf, group):
+            idx = list(range(b0, b0 + group))
+            xs = [X[b].data_ptr() for b in idx]
+            batch.build_lists_dev(xs, squeeze=True, stream=sh)
+            batch.eforce_dev(xs, [dp[b].data_ptr() for b in idx],
+                             [en[b].data_ptr() for b in idx], sh)
+            for k, b in enumerate(idx):
+                side[k].wait_stream(main)
+                mix[k].param_grad_dev(X[b].data_ptr(), pg[b, 0].data_ptr(), pg[b, 1].data_ptr(),
+                                      pg[b, 2].data_ptr(), side[k].cuda_stream)
+            for st in side:
+                main.wait_stream(st)
+
+    def run_one(prec):
         g = eng[prec]
         dp, pg, en = out[prec]
         for b in range(n_conf):
@@ -417,14 +446,22 @@ def adjoint_leg(n_conf=128, seed=4001):
             g.param_grad_dev(x.data_ptr(), pg[b, 0].data_ptr(), pg[b, 1].data_ptr(),
                              pg[b, 2].data_ptr(), sh)
 
-    run(abi.PREC_MIXED)  # warm-up (and the squeezed coordinates)
+    assert n_conf % group == 0
+    run_grouped()  # warm-up (and the squeezed coordinates)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(abi.PREC_MIXED)
+    run_grouped()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    run(abi.PREC_FP64)
+    # one conformation at a time on one context, for comparison
+    t0 = time.perf_counter()
+    run_one(abi.PREC_MIXED)
     torch.cuda.synchronize()
+    dt_one = time.perf_counter() - t0
+    run_grouped()  # the grouped results are the ones checked below
+    run_one(abi.PREC_FP64)
+    torch.cuda.synchronize()
+    batch.close()
 
     def errs(a, b):
         a, b = a.reshape(len(a), -1), b.reshape(len(b), -1)
@@ -481,16 +518,22 @@ def adjoint_leg(n_conf=128, seed=4001):
     rep['fd_max_rel_fp64'] = worst
     for g in eng.values():
         g.close()
+    for g in mix[1:]:
+        g.close()
     return {
         'conformations': n_conf,
         'atoms': npt,
         'lj_types': len(m['lj_eps']),
         'conf_per_s': n_conf / dt,
         'ms_per_conf': 1e3 * dt / n_conf,
+        'group': group,
+        'one_at_a_time_conf_per_s': n_conf / dt_one,
         'errors_mixed_vs_fp64': rep,
         'workload': '10,048-atom heteropolymer in water (200 chains, 46.4 A box, 48^3 grid), '
                     'conformations = base + N(0, 0.1 A); per conformation list build + energy '
-                    '+ dE/dx + dE/dq per atom + dE/deps, dE/dRmin per LJ type (mixed)',
+                    '+ dE/dx + dE/dq per atom + dE/deps, dE/dRmin per LJ type (mixed), '
+                    '%d conformations at a time (batched lists and evaluation, adjoints on '
+                    'their own streams)' % group,
     }
 
 
@@ -514,6 +557,8 @@ def main():
     ap.add_argument('--large-only', action='store_true',
                     help='run only the ~1.0M-atom heteropolymer loop and print its JSON')
     ap.add_argument('--large-steps', type=int, default=200)
+    ap.add_argument('--adjoint-group', type=int, default=8,
+                    help='conformations per group in the differentiation leg')
     ap.add_argument('--adjoint-only', action='store_true',
                     help='run only the differentiation leg and print its JSON')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
@@ -538,7 +583,7 @@ def main():
         print(json.dumps(batched_leg(steps=args.batched_steps)))
         return
     if args.adjoint_only:
-        print(json.dumps(adjoint_leg()))
+        print(json.dumps(adjoint_leg(group=args.adjoint_group)))
         return
     if args.chain_only:
         print(json.dumps(chain_leg()))
@@ -598,7 +643,7 @@ def main():
     if not args.no_chain:
         line['chain'] = chain_leg()
     if not args.no_adjoint:
-        line['adjoint'] = adjoint_leg()
+        line['adjoint'] = adjoint_leg(group=args.adjoint_group)
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(sysm, cut, ew)
     print(json.dumps(line))

@@ -184,6 +184,19 @@ int32_t moil_gpu_batch_eforce_dev(moil_batch *batch, const double *const *d_coor
                                   double *const *d_dpot, double *const *d_energies,
                                   void *stream);
 int32_t moil_gpu_batch_destroy(moil_batch *batch);
+/* Lists of every system of the batch (as moil_gpu_build_lists_dev per
+ * system, preceded by moil_gpu_squeeze_dev when `squeeze` is nonzero;
+ * d_coor[i] is updated by the squeeze): the builds are queued side by side
+ * on the systems' own streams and the call waits on the host once for all
+ * of them (again for a system whose lists outgrew their buffers).
+ * Ordered after the caller's earlier work on `stream`. */
+int32_t moil_gpu_batch_build_lists_dev(moil_batch *batch, double *const *d_coor,
+                                       int32_t squeeze, void *stream);
+/* Graph replay of the batch step on (the default) or off.  Off, every
+ * evaluation launches the same kernels directly on the batch's two
+ * streams: for callers that rebuild lists before every evaluation, where
+ * each call would recapture.  Results are identical either way. */
+int32_t moil_gpu_batch_set_graph(moil_batch *batch, int32_t on);
 
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain

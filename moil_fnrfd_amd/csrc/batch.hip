@@ -123,6 +123,11 @@ struct moil_batch {
   // every evaluation, where each call would otherwise recapture)
   bool eager = false;
   bool have_tables = false;  // eager: tables match gen/pgen/key
+  // batched list builds: status words (device, pinned host), fork/join
+  // events, allocated on first use
+  int32_t *d_stat = nullptr, *h_stat = nullptr;
+  hipEvent_t ev_lists = nullptr;
+  std::vector<hipEvent_t> ev_built;
 };
 
 void batch_set_eager(moil_batch *b, bool on) {
@@ -156,6 +161,11 @@ static void batch_free(moil_batch *b) {
     hipFree(g.qcplx);
   }
   hipFree(b->d_blob);
+  hipFree(b->d_stat);
+  if (b->h_stat) hipHostFree(b->h_stat);
+  if (b->ev_lists) hipEventDestroy(b->ev_lists);
+  for (hipEvent_t e : b->ev_built)
+    if (e) hipEventDestroy(e);
   if (b->ev_in) hipEventDestroy(b->ev_in);
   if (b->ev_out) hipEventDestroy(b->ev_out);
   if (b->ev_pme) hipEventDestroy(b->ev_pme);
@@ -512,4 +522,41 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
   HIP_CHECK(hipEventRecord(b->ev_out, b->cap));
   HIP_CHECK(hipStreamWaitEvent(st, b->ev_out, 0));
   return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_batch_set_graph(moil_batch *b, int32_t on) {
+  if (!b) {
+    set_error("batch_set_graph: null batch");
+    return MOIL_GPU_EINVAL;
+  }
+  if (b->cap) hipStreamSynchronize(b->cap);
+  if (b->cap2) hipStreamSynchronize(b->cap2);
+  batch_drop_graph(b);
+  batch_set_eager(b, on == 0);
+  b->gen.clear();  // next evaluation rebuilds its tables
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_gpu_batch_build_lists_dev(moil_batch *b, double *const *d_coor,
+                                                  int32_t squeeze, void *stream) {
+  if (!b || !d_coor) {
+    set_error("batch_build_lists: null argument");
+    return MOIL_GPU_EINVAL;
+  }
+  const int n = (int)b->ctx.size();
+  for (int i = 0; i < n; i++)
+    if (!d_coor[i]) {
+      set_error("batch_build_lists: null device pointer");
+      return MOIL_GPU_EINVAL;
+    }
+  if (!b->d_stat) {
+    HIP_CHECK(hipMalloc(&b->d_stat, sizeof(int32_t) * ST_COUNT * n));
+    HIP_CHECK(hipHostMalloc(&b->h_stat, sizeof(int32_t) * ST_COUNT * n, hipHostMallocDefault));
+    HIP_CHECK(hipEventCreateWithFlags(&b->ev_lists, hipEventDisableTiming));
+    b->ev_built.assign(n, nullptr);
+    for (int i = 0; i < n; i++)
+      HIP_CHECK(hipEventCreateWithFlags(&b->ev_built[i], hipEventDisableTiming));
+  }
+  return build_lists_many(b->ctx.data(), d_coor, n, squeeze != 0, (hipStream_t)stream, b->d_stat,
+                          b->h_stat, b->ev_lists, b->ev_built.data());
 }

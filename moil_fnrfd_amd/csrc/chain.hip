@@ -223,10 +223,6 @@ struct moil_chain {
   double *d_en = nullptr;    // (nin, MOIL_N_ENERGIES)
   double *d_part = nullptr;  // (2 igrid - 3, nblk)
   double *d_coef = nullptr;  // 3 igrid
-  // list builds: status words (device, pinned host), fork/join events
-  int32_t *d_stat = nullptr, *h_stat = nullptr;
-  hipEvent_t ready = nullptr;
-  std::vector<hipEvent_t> done;
   // staging of the host entry point
   double *d_r = nullptr, *d_dv = nullptr, *d_out = nullptr;
   int32_t *d_pick = nullptr;
@@ -244,11 +240,6 @@ static void chain_free(moil_chain *ch) {
   hipFree(ch->d_dv);
   hipFree(ch->d_out);
   hipFree(ch->d_pick);
-  hipFree(ch->d_stat);
-  if (ch->h_stat) hipHostFree(ch->h_stat);
-  if (ch->ready) hipEventDestroy(ch->ready);
-  for (hipEvent_t e : ch->done)
-    if (e) hipEventDestroy(e);
   delete ch;
 }
 
@@ -276,22 +267,18 @@ extern "C" int32_t moil_gpu_chain_create(moil_ctx **slices, int32_t igrid, moil_
     return rc;
   }
   // every call rebuilds the lists, so graphs would be recaptured each time
-  batch_set_eager(ch->batch, true);
+  rc = moil_gpu_batch_set_graph(ch->batch, 0);
+  if (rc != MOIL_GPU_OK) {
+    chain_free(ch);
+    return rc;
+  }
   const size_t n3 = 3 * (size_t)ch->npt;
   const bool ok =
       hipMalloc(&ch->d_scr, sizeof(double) * n3 * nin) == hipSuccess &&
       hipMalloc(&ch->d_en, sizeof(double) * MOIL_N_ENERGIES * nin) == hipSuccess &&
       hipMalloc(&ch->d_part, sizeof(double) * (2 * igrid - 3) * (size_t)ch->nblk) == hipSuccess &&
-      hipMalloc(&ch->d_coef, sizeof(double) * 3 * igrid) == hipSuccess &&
-      hipMalloc(&ch->d_stat, sizeof(int32_t) * ST_COUNT * nin) == hipSuccess &&
-      hipHostMalloc(&ch->h_stat, sizeof(int32_t) * ST_COUNT * nin, hipHostMallocDefault) ==
-          hipSuccess &&
-      hipEventCreateWithFlags(&ch->ready, hipEventDisableTiming) == hipSuccess;
-  ch->done.assign(nin, nullptr);
-  bool ev_ok = ok;
-  for (int k = 0; ev_ok && k < nin; k++)
-    ev_ok = hipEventCreateWithFlags(&ch->done[k], hipEventDisableTiming) == hipSuccess;
-  if (!ok || !ev_ok) {
+      hipMalloc(&ch->d_coef, sizeof(double) * 3 * igrid) == hipSuccess;
+  if (!ok) {
     chain_free(ch);
     set_error("chain_create: device allocation failed");
     return MOIL_GPU_EHIP;
@@ -334,8 +321,7 @@ extern "C" int32_t moil_gpu_chain_sds_dev(moil_chain *ch, const double *d_r,
   }
   // every replica squeezed and its lists rebuilt, the builds side by side
   // on the replicas' own streams, one host wait for all
-  int32_t rc = build_lists_many(ch->ctx.data(), xm.data(), nin, true, st, ch->d_stat,
-                                ch->h_stat, ch->ready, ch->done.data());
+  int32_t rc = moil_gpu_batch_build_lists_dev(ch->batch, xm.data(), 1, st);
   if (rc != MOIL_GPU_OK) return rc;
   rc = moil_gpu_batch_eforce_dev(ch->batch, xs.data(), gs.data(), es.data(), st);
   if (rc != MOIL_GPU_OK) return rc;

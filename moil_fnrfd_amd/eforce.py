@@ -72,6 +72,8 @@ def load_library():
     L.moil_gpu_batch_create.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.moil_gpu_batch_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_batch_destroy.argtypes = [vp]
+    L.moil_gpu_batch_build_lists_dev.argtypes = [vp, vp, C.c_int32, vp]
+    L.moil_gpu_batch_set_graph.argtypes = [vp, C.c_int32]
     L.moil_gpu_stage_times.argtypes = [vp, vp]
     L.moil_gpu_set_params.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.moil_gpu_param_grad.argtypes = [vp, vp, vp, vp, vp]
@@ -88,6 +90,7 @@ def load_library():
                  'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_batch_create',
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
+                 'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
@@ -248,6 +251,17 @@ class MoilBatch:
         b = (C.c_void_p * self.n)(*d_dpot)
         e = (C.c_void_p * self.n)(*d_energies)
         _check(load_library().moil_gpu_batch_eforce_dev(self._h, a, b, e, C.c_void_p(stream or 0)))
+
+    def build_lists_dev(self, d_coor, squeeze=True, stream=0):
+        """Lists of every system (squeezed first when `squeeze`), the
+        builds side by side with one host wait."""
+        a = (C.c_void_p * self.n)(*d_coor)
+        _check(load_library().moil_gpu_batch_build_lists_dev(self._h, a, int(bool(squeeze)),
+                                                             C.c_void_p(stream or 0)))
+
+    def set_graph(self, on):
+        """Graph replay of the step on (default) or off (direct launches)."""
+        _check(load_library().moil_gpu_batch_set_graph(self._h, int(bool(on))))
 
     def close(self):
         if getattr(self, '_h', None):

@@ -100,3 +100,65 @@ def test_batch_refuses_mixed_precisions():
         MoilBatch(ms)
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize('prec', [abi.PREC_MIXED, abi.PREC_FP64], ids=['mixed', 'fp64'])
+def test_batch_list_builds_and_direct_launches(prec):
+    """moil_gpu_batch_build_lists_dev (all systems' squeeze + list builds
+    side by side) and graph replay switched off: after every move, the
+    batch's results equal those of separate contexts that squeezed, built
+    and evaluated one system at a time, bit for bit; switching the graph
+    back on changes nothing."""
+    import torch
+    from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
+    torch.cuda.set_device(0)
+    dev = torch.device('cuda:0')
+    sh = torch.cuda.current_stream().cuda_stream
+    mine, ref, x_b, x_p = [], [], [], []
+    for name, s, cut, ew in _systems():
+        p = configs.params_for(s, cut, ew, precision=prec)
+        mine.append(MoilGPU(s.topo, p))
+        ref.append(MoilGPU(s.topo, p))
+        x = torch.tensor(s.coor, dtype=torch.float64, device=dev)
+        x_b.append(x)
+        x_p.append(x.clone())
+    n = len(mine)
+    dp_b = [torch.empty_like(x) for x in x_b]
+    dp_p = [torch.empty_like(x) for x in x_b]
+    en_b = [torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev) for _ in range(n)]
+    en_p = [torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev) for _ in range(n)]
+    batch = MoilBatch(mine)
+    batch.set_graph(False)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    for rep in range(3):
+        print('case batch lists + direct launch %d prec=%d' % (rep, prec), flush=True)
+        if rep:
+            for xb, xp in zip(x_b, x_p):
+                d = torch.randn(xb.shape, generator=gen, device=dev, dtype=torch.float64)
+                xb.add_(d, alpha=0.05)
+                xp.add_(d, alpha=0.05)
+        batch.build_lists_dev([x.data_ptr() for x in x_b], squeeze=True, stream=sh)
+        batch.eforce_dev([x.data_ptr() for x in x_b], [d.data_ptr() for d in dp_b],
+                         [e.data_ptr() for e in en_b], sh)
+        for i, m in enumerate(ref):
+            m.squeeze_dev(x_p[i].data_ptr(), sh)
+            m.nbondm_dev(x_p[i].data_ptr(), sh)
+            m.eforce_dev(x_p[i].data_ptr(), dp_p[i].data_ptr(), en_p[i].data_ptr(), sh)
+        torch.cuda.synchronize()
+        for i in range(n):
+            assert torch.equal(x_b[i], x_p[i]), i  # same squeeze
+            assert torch.equal(dp_b[i], dp_p[i]), i
+            assert torch.equal(en_b[i], en_p[i]), i
+    batch.set_graph(True)
+    for d in dp_b:
+        d.fill_(0.0)
+    batch.eforce_dev([x.data_ptr() for x in x_b], [d.data_ptr() for d in dp_b],
+                     [e.data_ptr() for e in en_b], sh)
+    torch.cuda.synchronize()
+    for i in range(n):
+        assert torch.equal(dp_b[i], dp_p[i]), i
+        assert torch.equal(en_b[i], en_p[i]), i
+    batch.close()
+    for m in mine + ref:
+        m.close()
