@@ -369,7 +369,7 @@ def large_leg(steps=200, warmup=20):
     }
 
 
-def adjoint_leg(n_conf=128, seed=4001, group=8):
+def adjoint_leg(n_conf=128, seed=4001, group=32):
     """Differentiation workload: the 10,048-atom heteropolymer-in-water
     system, n_conf conformations (base + N(0, 0.1 A) per coordinate).  Per
     conformation, timed: list build, forward energy + dE/dx, and the
@@ -557,7 +557,7 @@ def main():
     ap.add_argument('--large-only', action='store_true',
                     help='run only the ~1.0M-atom heteropolymer loop and print its JSON')
     ap.add_argument('--large-steps', type=int, default=200)
-    ap.add_argument('--adjoint-group', type=int, default=8,
+    ap.add_argument("--adjoint-group", type=int, default=32,
                     help='conformations per group in the differentiation leg')
     ap.add_argument('--adjoint-only', action='store_true',
                     help='run only the differentiation leg and print its JSON')
