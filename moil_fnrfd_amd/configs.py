@@ -70,6 +70,18 @@ def hetero_small(seed=2000, n_chains=12, L=24.0):
     return s, cut, ew
 
 
+def chains_only(seed=2200, n_chains=8, L=22.0, n_water=0):
+    """Heteropolymer chains with no water at all (or n_water waters): the
+    water lists are empty and every pair goes through MOIL's atom lists;
+    15 % uncharged and 15 % zero-LJ chain atoms as in hetero_small."""
+    s = hetero_system(n_chains, L, seed=seed, n_water=n_water, frac_neutral=0.15,
+                      frac_hlike=0.15)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
+    n = int(round(L)) + int(round(L)) % 2
+    ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
 def hetero_box(n_chains, L, seed):
     """Heteropolymer in water by the large-system recipe (hetero_packed):
     n_chains chains of 10-20 atoms in an L-A box at 0.1006 atoms/A^3, 10 A

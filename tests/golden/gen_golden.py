@@ -165,6 +165,10 @@ def main():
     s, cut, ew = configs.hetero_small()
     write_case('hetero_small_nohvdw', perturb(s, 0.05, 2001), dict(cut, hvdw0=False), ew,
                squeeze=True)
+    s, cut, ew = configs.chains_only()
+    write_case('chains_only', perturb(s, 0.05, 2201), cut, ew, squeeze=True)
+    s, cut, ew = configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1)
+    write_case('chains_one_water', perturb(s, 0.05, 2301), cut, ew)
     write_chain_case()
 
 
