@@ -11,8 +11,9 @@
 // the gather reads from them).  The step -- about twenty kernel launches
 // plus two transforms per grid size, independent of the number of systems
 // -- runs as two captured hipGraphs replayed side by side: the reciprocal
-// sums on one stream, the real-space part on another (each graph is one
-// stream's linear capture), joined by an event before the two assembly
+// sums on one stream (the transform groups spread over up to three more
+// streams forked and joined inside the capture), the real-space part on
+// another (submitted first), joined by an event before the two assembly
 // launches.  The graphs are recaptured when a system's lists were rebuilt
 // (chunk counts and list buffers change), its parameters changed, or the
 // caller passes different device pointers.  Nothing captured is a memset:
@@ -128,6 +129,12 @@ struct moil_batch {
   int32_t *d_stat = nullptr, *h_stat = nullptr;
   hipEvent_t ev_lists = nullptr;
   std::vector<hipEvent_t> ev_built;
+  // the transform groups run side by side: group k on stream k % (1 +
+  // aux.size()) (0 = the reciprocal-sum stream), forked and joined by
+  // events around each direction
+  std::vector<hipStream_t> aux;
+  hipEvent_t ev_ffork = nullptr, ev_bfork = nullptr;
+  std::vector<hipEvent_t> ev_fdone, ev_bdone;
 };
 
 void batch_set_eager(moil_batch *b, bool on) {
@@ -161,8 +168,18 @@ static void batch_free(moil_batch *b) {
     hipFree(g.qcplx);
   }
   hipFree(b->d_blob);
+  for (hipStream_t a : b->aux)
+    if (a) hipStreamSynchronize(a);
   hipFree(b->d_stat);
   if (b->h_stat) hipHostFree(b->h_stat);
+  for (hipStream_t a : b->aux)
+    if (a) hipStreamDestroy(a);
+  if (b->ev_ffork) hipEventDestroy(b->ev_ffork);
+  if (b->ev_bfork) hipEventDestroy(b->ev_bfork);
+  for (hipEvent_t e : b->ev_fdone)
+    if (e) hipEventDestroy(e);
+  for (hipEvent_t e : b->ev_bdone)
+    if (e) hipEventDestroy(e);
   if (b->ev_lists) hipEventDestroy(b->ev_lists);
   for (hipEvent_t e : b->ev_built)
     if (e) hipEventDestroy(e);
@@ -269,6 +286,24 @@ extern "C" int32_t moil_gpu_batch_create(moil_ctx **ctxs, int32_t n, int32_t nst
     batch_free(b);
     return rc;
   }
+  // up to three more streams for the transform groups
+  const int ng = (int)b->groups.size();
+  const int naux = std::min(ng - 1, 3);
+  bool ok2 = hipEventCreateWithFlags(&b->ev_ffork, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b->ev_bfork, hipEventDisableTiming) == hipSuccess;
+  b->aux.assign(std::max(naux, 0), nullptr);
+  for (hipStream_t &a : b->aux)
+    ok2 = ok2 && hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess;
+  b->ev_fdone.assign(ng, nullptr);
+  b->ev_bdone.assign(ng, nullptr);
+  for (int k = 0; k < ng; k++)
+    ok2 = ok2 && hipEventCreateWithFlags(&b->ev_fdone[k], hipEventDisableTiming) == hipSuccess &&
+          hipEventCreateWithFlags(&b->ev_bdone[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok2) {
+    batch_free(b);
+    set_error("batch_create: stream/event creation failed");
+    return MOIL_GPU_EHIP;
+  }
   *out = b;
   return MOIL_GPU_OK;
 }
@@ -363,6 +398,32 @@ static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *
   return MOIL_GPU_OK;
 }
 
+// one direction of every group's transform, the groups spread over st and
+// the auxiliary streams (forked from st, joined back into st)
+static int32_t batch_transforms(moil_batch *b, hipStream_t st, bool forward) {
+  const int ns = 1 + (int)b->aux.size();
+  hipEvent_t fork = forward ? b->ev_ffork : b->ev_bfork;
+  std::vector<hipEvent_t> &done = forward ? b->ev_fdone : b->ev_bdone;
+  if (ns > 1) HIP_CHECK(hipEventRecord(fork, st));
+  for (size_t k = 0; k < b->groups.size(); k++) {
+    FftGroup &g = b->groups[k];
+    const int si = (int)(k % ns);
+    hipStream_t s = si == 0 ? st : b->aux[si - 1];
+    if (si != 0 && k < (size_t)ns) HIP_CHECK(hipStreamWaitEvent(s, fork, 0));
+    rocfft_execution_info info = forward ? g.info_f : g.info_b;
+    void *in[1] = {forward ? g.qreal : g.qcplx}, *out[1] = {forward ? g.qcplx : g.qreal};
+    RF_TRY(rocfft_execution_info_set_stream(info, s));
+    RF_TRY(rocfft_execute(forward ? g.fwd : g.bwd, in, out, info));
+    // the last group of each auxiliary stream signals its completion
+    if (si != 0 && k + ns >= b->groups.size()) HIP_CHECK(hipEventRecord(done[k], s));
+  }
+  for (size_t k = 0; k < b->groups.size(); k++) {
+    const int si = (int)(k % ns);
+    if (si != 0 && k + ns >= b->groups.size()) HIP_CHECK(hipStreamWaitEvent(st, done[k], 0));
+  }
+  return MOIL_GPU_OK;
+}
+
 // the batch step: every kernel once over all systems, transforms per
 // group; the reciprocal sums (spreading, transforms, convolution, gather)
 // and the real-space part are independent until the force assembly
@@ -380,22 +441,10 @@ static int32_t batch_enqueue_pme(moil_batch *b, hipStream_t st) {
     launch_k_pme_tsum_f32_multi(TAB_PTRS(b, tsum), st);
   }
   HIP_CHECK(hipGetLastError());
-  if (b->tab.tbin.nsys() > 0) {
-    for (FftGroup &g : b->groups) {
-      void *in[1] = {g.qreal}, *out[1] = {g.qcplx};
-      RF_TRY(rocfft_execution_info_set_stream(g.info_f, st));
-      RF_TRY(rocfft_execute(g.fwd, in, out, g.info_f));
-    }
-  }
+  if (b->tab.tbin.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, true));
   if (f) launch_k_conv_f64_multi(TAB_PTRS(b, conv), st);
   else launch_k_conv_f32_multi(TAB_PTRS(b, conv), st);
-  if (b->tab.tbin.nsys() > 0) {
-    for (FftGroup &g : b->groups) {
-      void *in[1] = {g.qcplx}, *out[1] = {g.qreal};
-      RF_TRY(rocfft_execution_info_set_stream(g.info_b, st));
-      RF_TRY(rocfft_execute(g.bwd, in, out, g.info_b));
-    }
-  }
+  if (b->tab.tbin.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, false));
   if (f) launch_k_gather_f64_multi(TAB_PTRS(b, gather), st);
   else launch_k_gather_f32_multi(TAB_PTRS(b, gather), st);
   HIP_CHECK(hipGetLastError());
@@ -508,15 +557,17 @@ extern "C" int32_t moil_gpu_batch_eforce_dev(moil_batch *b, const double *const 
       b->key[3 * i + 2] = d_energies[i];
     }
   }
+  // the real-space part (a few launches) is submitted first so that it
+  // starts while the reciprocal part's many transform launches are still
+  // being submitted
   if (b->eager) {
-    CHECK_RC_B(batch_enqueue_pme(b, b->cap2));
-    HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
     CHECK_RC_B(batch_enqueue_rs(b, b->cap));
+    CHECK_RC_B(batch_enqueue_pme(b, b->cap2));
   } else {
-    HIP_CHECK(hipGraphLaunch(b->exec2, b->cap2));
-    HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
     HIP_CHECK(hipGraphLaunch(b->exec, b->cap));
+    HIP_CHECK(hipGraphLaunch(b->exec2, b->cap2));
   }
+  HIP_CHECK(hipEventRecord(b->ev_pme, b->cap2));
   HIP_CHECK(hipStreamWaitEvent(b->cap, b->ev_pme, 0));
   CHECK_RC_B(batch_enqueue_fin(b, b->cap));
   HIP_CHECK(hipEventRecord(b->ev_out, b->cap));
