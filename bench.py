@@ -537,6 +537,11 @@ def adjoint_leg(n_conf=128, seed=4001, group=32):
     }
 
 
+def _progress(msg):
+    """One line per leg on stderr (stdout carries the JSON line only)."""
+    print('bench: %s' % msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -592,6 +597,7 @@ def main():
         print(json.dumps(large_leg(steps=args.large_steps)))
         return
     sysm, cut, ew = configs.water32k()
+    _progress('main loop: water32k, %d steps after %d warm-up' % (args.steps, args.warmup))
     r = timed_loop(sysm, cut, ew, args.steps, args.warmup, seed=1001)
     evals_per_s = args.steps / r['elapsed']
     n_pairs = r['n_pairs']
@@ -639,12 +645,16 @@ def main():
         },
     }
     if not args.no_batched:
+        _progress('batched leg: 96 systems')
         line['batched'] = batched_leg(steps=args.batched_steps)
     if not args.no_chain:
+        _progress('chain leg: 32 replicas')
         line['chain'] = chain_leg()
     if not args.no_adjoint:
+        _progress('differentiation leg: 128 conformations')
         line['adjoint'] = adjoint_leg(group=args.adjoint_group)
     if not args.no_cpu_baseline:
+        _progress('CPU baseline: MOIL on one core')
         line['cpu_baseline'] = cpu_baseline(sysm, cut, ew)
     print(json.dumps(line))
 
