@@ -19,9 +19,8 @@
 //      gradients land straight in their dv rows (sds.f:87-106);
 //   4. runs three small kernels for the chain terms:
 //      k_chain_dist   the 2 igrid - 3 squared distances, as partial sums
-//                     per workgroup (each workgroup streams its atoms
-//                     through all replicas once, keeping the previous two
-//                     in registers);
+//                     per workgroup (one workgroup per atom block and
+//                     replica: the pairs ending at that replica);
 //      k_chain_coef   one workgroup: the sums, <d>, the energy terms and
 //                     the per-distance gradient factors (sds.f:149-195);
 //      k_chain_grad   one thread per (replica, atom): the chain terms of
@@ -40,6 +39,7 @@
 namespace {
 
 constexpr int CH_BLOCK = 256;
+constexpr int CH_MAX_GRID = 1024;  // replicas per chain (LDS of k_chain_coef)
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -47,55 +47,44 @@ __device__ inline double wave_sum(double v) {
 }
 
 // part[j][blk] = selected-atom sum of |r_{j+1} - r_j|^2 (j < igrid-1) and
-// part[igrid-1+j][blk] = that of |r_{j+2} - r_j|^2 (j < igrid-2)
+// part[igrid-1+j][blk] = that of |r_{j+2} - r_j|^2 (j < igrid-2); workgroup
+// (blk, s-1) handles the pairs ending at replica s >= 1 of its atom block
 __global__ __launch_bounds__(CH_BLOCK) void k_chain_dist(const double *__restrict__ r,
                                                          const int32_t *__restrict__ pick,
                                                          int npt, int igrid,
                                                          double *__restrict__ part) {
   __shared__ double red[2][CH_BLOCK / 64];
   const int i = blockIdx.x * CH_BLOCK + threadIdx.x;
-  const bool on = i < npt && pick[i] > 0;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = blockIdx.y + 1;
   const int nblk = gridDim.x;
-  double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};  // replicas s-1, s-2
-  for (int s = 0; s < igrid; s++) {
-    double x[3] = {0.0, 0.0, 0.0};
-    if (on) {
-      const double *q = r + 3 * ((size_t)s * npt + i);
-      x[0] = q[0];
-      x[1] = q[1];
-      x[2] = q[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v1 = 0.0, v2 = 0.0;
+  if (i < npt && pick[i] > 0) {
+    const double *x = r + 3 * ((size_t)s * npt + i);
+    const double *p1 = r + 3 * ((size_t)(s - 1) * npt + i);
+    const double a = x[0] - p1[0], b = x[1] - p1[1], c = x[2] - p1[2];
+    v1 = a * a + b * b + c * c;
+    if (s >= 2) {
+      const double *p2 = r + 3 * ((size_t)(s - 2) * npt + i);
+      const double e = x[0] - p2[0], f = x[1] - p2[1], g = x[2] - p2[2];
+      v2 = e * e + f * f + g * g;
     }
-    if (s >= 1) {
-      const double a = x[0] - p1[0], b = x[1] - p1[1], c = x[2] - p1[2];
-      double v1 = a * a + b * b + c * c;
-      double v2 = 0.0;
-      if (s >= 2) {
-        const double e = x[0] - p2[0], f = x[1] - p2[1], g = x[2] - p2[2];
-        v2 = e * e + f * f + g * g;
-      }
-      v1 = wave_sum(v1);
-      v2 = wave_sum(v2);
-      if (lane == 0) {
-        red[0][w] = v1;
-        red[1][w] = v2;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double t1 = 0.0, t2 = 0.0;
-        for (int k = 0; k < CH_BLOCK / 64; k++) {
-          t1 += red[0][k];
-          t2 += red[1][k];
-        }
-        part[(size_t)(s - 1) * nblk + blockIdx.x] = t1;
-        if (s >= 2) part[(size_t)(igrid - 1 + s - 2) * nblk + blockIdx.x] = t2;
-      }
-      __syncthreads();
+  }
+  v1 = wave_sum(v1);
+  v2 = wave_sum(v2);
+  if (lane == 0) {
+    red[0][w] = v1;
+    red[1][w] = v2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t1 = 0.0, t2 = 0.0;
+    for (int k = 0; k < CH_BLOCK / 64; k++) {
+      t1 += red[0][k];
+      t2 += red[1][k];
     }
-    for (int k = 0; k < 3; k++) {
-      p2[k] = p1[k];
-      p1[k] = x[k];
-    }
+    part[(size_t)(s - 1) * nblk + blockIdx.x] = t1;
+    if (s >= 2) part[(size_t)(igrid - 1 + s - 2) * nblk + blockIdx.x] = t2;
   }
 }
 
@@ -103,62 +92,75 @@ __global__ __launch_bounds__(CH_BLOCK) void k_chain_dist(const double *__restric
 // squared next-nearest ones; spring factors, then the repulsion
 // exponentials over <d>^2); e0 and the energy of the interior replicas;
 // coef = [spring factors e1(j) | 2 rho e1(igrid-1+j) | d2w / d0(j)].
+// Everything lives in LDS (igrid <= CH_MAX_GRID); the elementwise work is
+// spread over the workgroup, the sums over replicas run in one thread in
+// sds.f's order.
 __global__ __launch_bounds__(CH_BLOCK) void k_chain_coef(
     const double *__restrict__ part, int nblk, int igrid, const double *__restrict__ en,
     double gamma, double rho, double lambda, double *__restrict__ coef, double *__restrict__ d0,
     double *__restrict__ e0, double *__restrict__ e1, double *__restrict__ sener) {
 #pragma clang fp contract(off)
-  const int nd = 2 * igrid - 3;
-  for (int j = threadIdx.x; j < nd; j += CH_BLOCK) {
-    double t = 0.0;
-    for (int b = 0; b < nblk; b++) t += part[(size_t)j * nblk + b];
-    d0[j] = t;
+  // [0, 2 igrid - 3): d0; [2 igrid - 1, 3 igrid - 3): the exponentials
+  __shared__ double dd[3 * CH_MAX_GRID];
+  __shared__ double ee[CH_MAX_GRID];      // interior energies
+  __shared__ double sc[4];                // dave, dave2, d2w, energy
+  const int nd = 2 * igrid - 3, t = threadIdx.x;
+  for (int j = t; j < nd; j += CH_BLOCK) {
+    double v = 0.0;
+    for (int b = 0; b < nblk; b++) v += part[(size_t)j * nblk + b];
+    dd[j] = j < igrid - 1 ? sqrt(v) : v;  // sds.f:149-152
+  }
+  for (int j = t + 1; j < igrid - 1; j += CH_BLOCK) {
+    ee[j] = en[(size_t)(j - 1) * MOIL_N_ENERGIES];  // e_total
+    e0[j] = ee[j];
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double gam2 = 2.0 * gamma, rho2 = 2.0 * rho;
-  // sds.f:71-104: interior energies in replica order
-  double s = 0.0;
-  for (int j = 1; j < igrid - 1; j++) {
-    const double e = en[(size_t)(j - 1) * MOIL_N_ENERGIES];  // e_total
-    e0[j] = e;
-    s = s + e;
+  if (t == 0) {
+    double s = 0.0;  // sds.f:71-104
+    for (int j = 1; j < igrid - 1; j++) s = s + ee[j];
+    double dave = 0.0;  // sds.f:149-154
+    for (int j = 0; j < igrid - 1; j++) dave = dave + dd[j];
+    dave = dave / (double)(igrid - 1);
+    double tmp = 0.0;  // sds.f:169-174
+    for (int j = 0; j < igrid - 1; j++) {
+      const double t1 = dd[j] - dave;
+      tmp = tmp + t1 * t1;
+    }
+    sc[0] = dave;
+    sc[1] = 1.0 / (dave * dave);
+    sc[3] = s + gamma * tmp;
   }
-  // sds.f:149-154
-  double dave = 0.0;
-  for (int j = 0; j < igrid - 1; j++) {
-    d0[j] = sqrt(d0[j]);
-    dave = dave + d0[j];
+  __syncthreads();
+  const double dave = sc[0], dave2 = sc[1];
+  // sds.f:180-188: the exponentials
+  for (int j = igrid - 1 + t; j < nd; j += CH_BLOCK)
+    dd[igrid + j] = exp(-(lambda * dd[j] / (dave * dave)));
+  __syncthreads();
+  if (t == 0) {
+    double tmp = 0.0, d2w = 0.0;
+    for (int j = igrid - 1; j < nd; j++) {
+      tmp = tmp + dd[igrid + j];
+      d2w = d2w + dd[igrid + j] * dd[j];
+    }
+    *sener = sc[3] + rho / lambda * tmp;
+    sc[2] = 2.0 * rho * dave2 / ((double)(igrid - 1) * dave) * d2w;
   }
-  dave = dave / (double)(igrid - 1);
-  const double dave2 = 1.0 / (dave * dave);
-  // sds.f:169-174
-  double tmp = 0.0;
-  for (int j = 0; j < igrid - 1; j++) {
-    const double t1 = d0[j] - dave;
-    tmp = tmp + t1 * t1;
-  }
-  s = s + gamma * tmp;
-  // sds.f:180-188
-  tmp = 0.0;
-  double d2w = 0.0;
-  for (int j = igrid - 1; j < nd; j++) {
-    const double x = exp(-(lambda * d0[j] / (dave * dave)));
-    tmp = tmp + x;
-    d2w = d2w + x * d0[j];
-    e1[j] = x * dave2;
-  }
-  s = s + rho / lambda * tmp;
-  d2w = rho2 * dave2 / ((double)(igrid - 1) * dave) * d2w;
-  // sds.f:193-195
-  for (int j = 0; j < igrid - 1; j++) e1[j] = gam2 * (1.0 - dave / d0[j]);
+  __syncthreads();
+  const double d2w = sc[2], gam2 = 2.0 * gamma, rho2 = 2.0 * rho;
   double *cA = coef, *cB = coef + (igrid - 1), *cC = coef + (2 * igrid - 3);
-  for (int j = 0; j < igrid - 1; j++) {
-    cA[j] = e1[j];
-    cC[j] = d2w / d0[j];
+  for (int j = t; j < nd; j += CH_BLOCK) {
+    d0[j] = dd[j];
+    if (j < igrid - 1) {
+      const double f = gam2 * (1.0 - dave / dd[j]);  // sds.f:193-195
+      e1[j] = f;
+      cA[j] = f;
+      cC[j] = d2w / dd[j];
+    } else {
+      const double f = dd[igrid + j] * dave2;
+      e1[j] = f;
+      cB[j - (igrid - 1)] = rho2 * f;
+    }
   }
-  for (int j = 0; j < igrid - 2; j++) cB[j] = rho2 * e1[j + igrid - 1];
-  *sener = s;
 }
 
 // dv(replica s, atom i): the replica's own gradient (interior) or zero
@@ -244,8 +246,9 @@ static void chain_free(moil_chain *ch) {
 }
 
 extern "C" int32_t moil_gpu_chain_create(moil_ctx **slices, int32_t igrid, moil_chain **out) {
-  if (!slices || !out || igrid < 3) {
-    set_error("chain_create: null argument or fewer than 3 replicas");
+  if (!slices || !out || igrid < 3 || igrid > CH_MAX_GRID) {
+    set_error("chain_create: null argument, or fewer than 3 or more than " +
+              std::to_string(CH_MAX_GRID) + " replicas");
     return MOIL_GPU_EINVAL;
   }
   const int nin = igrid - 2;
@@ -325,7 +328,8 @@ extern "C" int32_t moil_gpu_chain_sds_dev(moil_chain *ch, const double *d_r,
   if (rc != MOIL_GPU_OK) return rc;
   rc = moil_gpu_batch_eforce_dev(ch->batch, xs.data(), gs.data(), es.data(), st);
   if (rc != MOIL_GPU_OK) return rc;
-  k_chain_dist<<<ch->nblk, CH_BLOCK, 0, st>>>(d_r, d_pick, npt, igrid, ch->d_part);
+  k_chain_dist<<<dim3(ch->nblk, igrid - 1), CH_BLOCK, 0, st>>>(d_r, d_pick, npt, igrid,
+                                                              ch->d_part);
   k_chain_coef<<<1, CH_BLOCK, 0, st>>>(ch->d_part, ch->nblk, igrid, ch->d_en, gamma, rho, lambda,
                                        ch->d_coef, d_d0, d_e0, d_e1, d_sener);
   const size_t nt = (size_t)igrid * npt;
