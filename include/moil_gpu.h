@@ -198,6 +198,17 @@ int32_t moil_gpu_batch_build_lists_dev(moil_batch *batch, double *const *d_coor,
  * each call would recapture.  Results are identical either way. */
 int32_t moil_gpu_batch_set_graph(moil_batch *batch, int32_t on);
 
+/* MOIL connectivity (.con) files: moil_con_read parses one the way MOIL's
+ * rconn does (generic/rconn.f:46-497, version 12.1, fixed-column records)
+ * into a topology for moil_gpu_init; the arrays belong to the returned
+ * handle and stay valid until moil_con_free.  Monomer flags come from the
+ * monomer names as MOIL's list and squeeze code tests them.  Files with LES
+ * copies, Morse bonds, the beta list, monomer division or the arith rule
+ * are refused (MOIL_GPU_EINVAL).  Host only: no GPU is touched. */
+typedef struct moil_con moil_con;
+int32_t moil_con_read(const char *path, moil_con **out, moil_topology *topo);
+int32_t moil_con_free(moil_con *con);
+
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain
  * minimiser, annealer and chain dynamics evaluate (chain/chnneal.f:85,

@@ -83,6 +83,8 @@ def load_library():
     L.moil_gpu_chain_sds.argtypes = [vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, vp, vp]
     L.moil_gpu_chain_sds_dev.argtypes = [vp, vp, vp, dbl, dbl, dbl, vp, vp, vp, vp, vp, vp]
     L.moil_gpu_chain_destroy.argtypes = [vp]
+    L.moil_con_read.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(abi.MoilTopology)]
+    L.moil_con_free.argtypes = [vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -91,7 +93,8 @@ def load_library():
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
-                 'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy'):
+                 'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
+                 'moil_con_read', 'moil_con_free'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -116,10 +119,16 @@ class MoilGPU:
     """One MOIL system on the GPU (the state MOIL keeps in COMMON)."""
 
     def __init__(self, topo, params: abi.MoilParams):
+        """topo: a systems.Topology, or a MoilCon (a .con file read by the
+        library's reader)."""
         L = load_library()
-        self._ts, self._keep = abi.make_topology(topo)
+        if isinstance(topo, MoilCon):
+            self._ts, self._keep = topo.topo, [topo]
+            self.npt = topo.topo.npt
+        else:
+            self._ts, self._keep = abi.make_topology(topo)
+            self.npt = topo.npt
         self.params = params
-        self.npt = topo.npt
         h = C.c_void_p()
         _check(L.moil_gpu_init(C.byref(self._ts), C.byref(params), C.byref(h)))
         self._h = h
@@ -325,6 +334,38 @@ class MoilChain:
     def close(self):
         if getattr(self, '_h', None):
             load_library().moil_gpu_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MoilCon:
+    """A MOIL .con file read by the library's own reader (moil_con_read,
+    MOIL's rconn semantics): `topo` is the moil_topology struct to pass to
+    moil_gpu_init, valid while this object lives.  Host only."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        t = abi.MoilTopology()
+        _check(load_library().moil_con_read(os.fsencode(path), C.byref(h), C.byref(t)))
+        self._h = h
+        self.topo = t
+
+    def array(self, name, n, width=1):
+        """A topology field as a numpy copy of n (x width) values."""
+        p = getattr(self.topo, name)
+        if n == 0:
+            return np.zeros((0,) if width == 1 else (0, width))
+        a = np.ctypeslib.as_array(p, shape=(n * width,)).copy()
+        return a if width == 1 else a.reshape(n, width)
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load_library().moil_con_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,7 +1,7 @@
 """CPU restatement of MOIL's chain-of-replicas functional (chain/sds.f).
 
-Test infrastructure only: imported by tests/ and the benchmark's CPU leg
-as the checker of the GPU chain entry point, never by the product.
+Test infrastructure only: imported by tests/ as the checker of the GPU
+chain entry point, never by the product.
 
 MOIL evaluates a chain of igrid replicas r_1..r_igrid of one system
 (chain/sds.f:1-16):

@@ -1,0 +1,119 @@
+"""The library's .con reader (moil_con_read, MOIL's rconn semantics,
+generic/rconn.f:46-497) against the topology the file was written from:
+every array identical, for water boxes, heteropolymers in water, chains
+without water and a single water.  The written files are the ones MOIL's
+own rconn reads for the golden fixtures (the fixture topology checksums
+pin those), so the reader hands moil_gpu_init exactly what MOIL reads.
+Host only: no GPU call."""
+import os
+
+import numpy as np
+import pytest
+
+from moil_fnrfd_amd import abi, configs
+from moil_fnrfd_amd.systems import monomer_flags, write_con
+from tests._cases import make_case
+
+
+def _read(path):
+    from moil_fnrfd_amd.eforce import MoilCon
+    return MoilCon(path)
+
+
+CASES = ['water216', 'hetero_small', 'chains_only', 'chains_one_water', 'hetero_small_out']
+
+
+@pytest.mark.parametrize('name', CASES)
+def test_reader_returns_the_written_topology(name, tmp_path):
+    s, cut, ew, sqz = make_case(name)
+    t = s.topo
+    path = os.path.join(tmp_path, name + '.con')
+    write_con(path, t)
+    c = _read(path)
+    T = c.topo
+    assert (T.npt, T.totmon) == (t.npt, t.totmon)
+    assert (T.nb, T.nangl, T.ntors, T.nimp, T.totspe, T.nwaters) == (
+        len(t.ib1), len(t.iangl1), len(t.itor1), len(t.iimp1), len(t.spec1), t.nwaters)
+    eq = np.testing.assert_array_equal
+    eq(c.array('poipt', t.totmon + 1), t.poipt)
+    eq(c.array('monoflag', t.totmon), monomer_flags(t.moname))
+    for f in ('ptchg', 'epsgm6', 'epsgm12'):
+        eq(c.array(f, t.npt), getattr(t, f))
+    eq(c.array('flagchr', t.npt), np.asarray(t.flagchr).astype(np.int32))
+    eq(c.array('exc1', t.npt + 1), t.exc1)
+    eq(c.array('exc2', len(t.exc2)), t.exc2)
+    for f in ('ib1', 'ib2', 'kbond', 'req'):
+        eq(c.array(f, len(t.ib1)), getattr(t, f))
+    for f in ('iangl1', 'iangl2', 'iangl3', 'kangl', 'angleq'):
+        eq(c.array(f, len(t.iangl1)), getattr(t, f))
+    for f in ('itor1', 'itor2', 'itor3', 'itor4', 'period', 'ktors1', 'ktors2', 'ktors3',
+              'phase1', 'phase2', 'phase3'):
+        eq(c.array(f, len(t.itor1)), getattr(t, f))
+    for f in ('iimp1', 'iimp2', 'iimp3', 'iimp4', 'kimp', 'impeq'):
+        eq(c.array(f, len(t.iimp1)), getattr(t, f))
+    eq(c.array('spec1', len(t.spec1)), t.spec1)
+    eq(c.array('spec2', len(t.spec2)), t.spec2)
+    eq(c.array('p14', len(t.spec1), 3), t.p14)
+    eq(c.array('idxtip3', t.nwaters), t.idxtip3)
+    c.close()
+
+
+def test_exclusion_records_longer_than_one_line(tmp_path):
+    """An atom with more than ten exclusions continues its partner list on
+    the next record, which MOIL's format(1x,10(i7,1x)) reads from column 1
+    (format reversion to the group); the reader does the same."""
+    s, cut, ew = configs.water216()
+    path = os.path.join(tmp_path, 'w.con')
+    write_con(path, s.topo)
+    lines = open(path).read().splitlines()
+    # rewrite the first exclusion record of the file as one atom excluding
+    # 12 partners: a first line of ten (1x,i7,1x) items and a continuation
+    # of two (i7,1x) items starting in column 1
+    k = next(i for i, l in enumerate(lines) if l.startswith('~ atom number, number of excl'))
+    partners = list(range(2, 14))
+    head = ' %7d %7d' % (1, len(partners))
+    first = ' ' + ''.join('%7d ' % p for p in partners[:10])
+    cont = ''.join('%7d ' % p for p in partners[10:])
+    # drop the old records of atoms 1.. up to the charge flags, keep totex consistent
+    end = next(i for i, l in enumerate(lines) if l.startswith('~ Charge flags'))
+    new = lines[:k + 1] + [head, first, cont] + lines[end:]
+    # header count totex (7th integer of the counts record)
+    cnt = lines[3]
+    vals = [int(cnt[1 + 8 * i:9 + 8 * i]) for i in range(14)]
+    vals[6] = len(partners)
+    new[3] = ' ' + ''.join('%8d' % v for v in vals) + cnt[1 + 8 * 14:]
+    path2 = os.path.join(tmp_path, 'w2.con')
+    with open(path2, 'w') as f:
+        f.write('\n'.join(new) + '\n')
+    c = _read(path2)
+    exc1 = c.array('exc1', s.topo.npt + 1)
+    exc2 = c.array('exc2', len(partners))
+    assert exc1[0] == 0 and exc1[1] == 12 and (exc1[2:] == 12).all()
+    np.testing.assert_array_equal(exc2, partners)
+    c.close()
+
+
+def test_reader_refuses_what_the_library_does_not_support(tmp_path):
+    from moil_fnrfd_amd.eforce import MoilGPUError
+    s, cut, ew = configs.water216()
+    path = os.path.join(tmp_path, 'w.con')
+    write_con(path, s.topo)
+    lines = open(path).read().splitlines()
+    cnt = lines[3]
+    for idx, what in ((8, 'LES'), (10, 'Morse')):
+        vals = [int(cnt[1 + 8 * i:9 + 8 * i]) for i in range(14)]
+        vals[idx] = 1
+        bad = list(lines)
+        bad[3] = ' ' + ''.join('%8d' % v for v in vals) + cnt[1 + 8 * 14:]
+        p2 = os.path.join(tmp_path, 'bad%d.con' % idx)
+        with open(p2, 'w') as f:
+            f.write('\n'.join(bad) + '\n')
+        with pytest.raises(MoilGPUError, match=what):
+            _read(p2)
+    with pytest.raises(MoilGPUError, match='cannot open'):
+        _read(os.path.join(tmp_path, 'missing.con'))
+    p3 = os.path.join(tmp_path, 'short.con')
+    with open(p3, 'w') as f:
+        f.write('\n'.join(lines[:40]) + '\n')
+    with pytest.raises(MoilGPUError, match='error %d' % abi.EINVAL):
+        _read(p3)

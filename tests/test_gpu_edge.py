@@ -81,3 +81,30 @@ def test_batch_of_systems_without_water(prec):
     b.close()
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize('name', ['hetero_small', 'water216'])
+def test_context_from_a_con_file(name, tmp_path):
+    """A context initialised from a .con file read by the library's reader
+    evaluates exactly as one built from the topology the file was written
+    from, and to the FP64 bars against MOIL."""
+    import os
+    from moil_fnrfd_amd.eforce import MoilCon, MoilGPU
+    from moil_fnrfd_amd.systems import write_con
+    s, cut, ew, sqz = make_case(name)
+    path = os.path.join(tmp_path, name + '.con')
+    write_con(path, s.topo)
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_FP64)
+    print('case %s from .con' % name, flush=True)
+    out = []
+    for topo in (s.topo, MoilCon(path)):
+        m = MoilGPU(topo, p)
+        x = m.squeeze(s.coor) if sqz else s.coor
+        m.nbondm(x)
+        out.append(m.eforce(x))
+        m.close()
+    (e0, f0), (e1, f1) = out
+    assert e0 == e1
+    assert np.array_equal(f0, f1)
+    g = load_golden(name)
+    assert rms_rel(f1, g['dpot']) < 1e-10
