@@ -208,6 +208,11 @@ int32_t moil_gpu_batch_set_graph(moil_batch *batch, int32_t on);
 typedef struct moil_con moil_con;
 int32_t moil_con_read(const char *path, moil_con **out, moil_topology *topo);
 int32_t moil_con_free(moil_con *con);
+/* CHARMM-style coordinate file as MOIL's getcrd reads it
+ * (generic/getcrd.f:48-97, format(i7,i7,1x,a4,1x,a4,3(f10.5),...)) into
+ * coor(3,npt).  Only complete files: a particle record out of sequence is
+ * refused (getcrd would fill placeholders and build hydrogens). */
+int32_t moil_crd_read(const char *path, int32_t npt, double *coor);
 
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain

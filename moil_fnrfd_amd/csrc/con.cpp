@@ -1,6 +1,6 @@
-// Reader of MOIL's connectivity (.con) files into the library's topology
-// struct, for C and C++ callers that load a MOIL system without MOIL's
-// Fortran.  It follows MOIL's own reader rconn (generic/rconn.f:46-497):
+// Readers of MOIL's connectivity (.con) files into the library's topology
+// struct, and of its CHARMM-style coordinate files, for C and C++ callers
+// that load a MOIL system without MOIL's Fortran.  It follows MOIL's own reader rconn (generic/rconn.f:46-497):
 // the same records in the same order, each read with the fixed-column
 // layout of rconn's FORMAT statements, including Fortran's format
 // reversion (a list longer than one record continues on the next record
@@ -385,5 +385,39 @@ extern "C" int32_t moil_con_read(const char *path, moil_con **out, moil_topology
 
 extern "C" int32_t moil_con_free(moil_con *c) {
   delete c;
+  return MOIL_GPU_OK;
+}
+
+// CHARMM-style coordinate file as MOIL's getcrd reads it
+// (generic/getcrd.f:48-97): title records starting with '*', a record with
+// the particle count (0 or npt), then one record per particle,
+// format(i7,i7,1x,a4,1x,a4,3(f10.5),1x,a4,1x,a4,f10.5).  Only complete
+// files are read: getcrd's repair of missing particles (placeholders and
+// built hydrogens) is not done, a record out of sequence is an error.
+extern "C" int32_t moil_crd_read(const char *path, int32_t npt, double *coor) {
+  if (!path || !coor || npt <= 0) {
+    set_error("crd_read: null argument or no particles");
+    return MOIL_GPU_EINVAL;
+  }
+  std::ifstream in(path);
+  if (!in) return con_fail(path, "cannot open");
+  std::vector<std::string> lines;
+  for (std::string line; std::getline(in, line);) lines.push_back(line);
+  size_t k = 0;
+  while (k < lines.size() && !lines[k].empty() && lines[k][0] == '*') k++;
+  if (k == 0) return con_fail(path, "coordinate file with no title");
+  if (k >= lines.size()) return con_fail(path, "no particle count");
+  const long n = std::strtol(lines[k].c_str(), nullptr, 10);
+  if (n != 0 && n != npt) return con_fail(path, "number of particles does not match");
+  k++;
+  for (long i = 0; i < npt; i++, k++) {
+    if (k >= lines.size()) return con_fail(path, "file ends early");
+    Rec r;
+    r.s = &lines[k];
+    const long i1 = fint(r.field(7));
+    if (i1 != i + 1) return con_fail(path, "particle " + std::to_string(i + 1) + " missing");
+    r.skip(7 + 1 + 4 + 1 + 4);
+    for (int d = 0; d < 3; d++) coor[3 * i + d] = freal(r.field(10), 5);
+  }
   return MOIL_GPU_OK;
 }

@@ -85,6 +85,7 @@ def load_library():
     L.moil_gpu_chain_destroy.argtypes = [vp]
     L.moil_con_read.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(abi.MoilTopology)]
     L.moil_con_free.argtypes = [vp]
+    L.moil_crd_read.argtypes = [C.c_char_p, C.c_int32, vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -94,7 +95,7 @@ def load_library():
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
-                 'moil_con_read', 'moil_con_free'):
+                 'moil_con_read', 'moil_con_free', 'moil_crd_read'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -373,3 +374,11 @@ class MoilCon:
             self.close()
         except Exception:
             pass
+
+
+def read_crd(path, npt):
+    """Coordinates (npt, 3) from a CHARMM-style coordinate file, read by the
+    library's reader (moil_crd_read, MOIL's getcrd format).  Host only."""
+    x = np.zeros((npt, 3))
+    _check(load_library().moil_crd_read(os.fsencode(path), int(npt), x.ctypes.data))
+    return x

@@ -11,8 +11,9 @@ c     It then writes every hot-path output in full precision (pote's
 c     own report, pot/wener.f, prints f12.3 only).
 c
 c     Coordinates are read as raw float64 (3,npt) instead of a CHARMM
-c     CRD file so that no f10.5 quantisation is applied; the topology is
-c     read through MOIL's own rconn from a .con text file.
+c     CRD file so that no f10.5 quantisation is applied (a path ending in
+c     .crd is read by MOIL's getcrd instead); the topology is read
+c     through MOIL's own rconn from a .con text file.
 c
 c     Usage: moil_dump <param-file>
 c     param-file (list-directed, one item group per line):
@@ -112,9 +113,16 @@ c     keyword of pot/energy.f:219 does (init_ef sets hvdw0 = .true.)
  10   continue
 
       ucrd = 33
-      open(unit=ucrd,file=crdfile,status='old',access='stream',
-     1     form='unformatted')
-      read(ucrd) ((coor(k,i),k=1,3),i=1,npt)
+      k = len_trim(crdfile)
+      if (crdfile(max(k-3,1):k) .eq. '.crd') then
+c        a CHARMM-style coordinate file, read by MOIL's own getcrd
+         open(unit=ucrd,file=crdfile,status='old')
+         call getcrd(ucrd,'CHARM')
+      else
+         open(unit=ucrd,file=crdfile,status='old',access='stream',
+     1        form='unformatted')
+         read(ucrd) ((coor(k,i),k=1,3),i=1,npt)
+      end if
       close(ucrd)
 
 c     cutoffs exactly as pot/energy.f:382-400 (cutm always explicit)

@@ -44,10 +44,12 @@ def parse_dump(path: str) -> dict:
 
 def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
              nlist=1, sigma=0.0, seed=12345, dump_lists=True, workdir=None,
-             timeout=3600, exe=REF_EXE, env=None) -> dict:
+             timeout=3600, exe=REF_EXE, env=None, crd_file=None) -> dict:
     """cut: rvmx relx rvbg rebg cutm (Angstrom); ewald: dtol, nfft (3), order.
     exe: the MOIL driver to run (REF_EXE, or SHIM_EXE for MOIL calling the
-    GPU library); env: extra environment variables."""
+    GPU library); env: extra environment variables; crd_file: a coordinate
+    file (path ending in .crd) for MOIL's getcrd instead of the system's
+    raw coordinates."""
     if not available(exe):
         raise RuntimeError('%s is not built (make -f oracle/ref/Makefile)' % exe)
     tmp = workdir or tempfile.mkdtemp(prefix='moilref_')
@@ -57,7 +59,10 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
     par = os.path.join(tmp, 'sys.par')
     dmp = os.path.join(tmp, 'sys.dump')
     write_con(con, sysm.topo)
-    np.ascontiguousarray(sysm.coor, np.float64).tofile(crd)
+    if crd_file:
+        crd = crd_file
+    else:
+        np.ascontiguousarray(sysm.coor, np.float64).tofile(crd)
     a, b, c = sysm.box
     n1, n2, n3 = ewald['nfft']
     with open(par, 'w') as f:

@@ -117,3 +117,53 @@ def test_reader_refuses_what_the_library_does_not_support(tmp_path):
         f.write('\n'.join(lines[:40]) + '\n')
     with pytest.raises(MoilGPUError, match='error %d' % abi.EINVAL):
         _read(p3)
+
+
+def _write_crd(path, s, title=('* synthetic system', '* written for the reader test')):
+    """A coordinate file in MOIL's getcrd layout (generic/getcrd.f:100)."""
+    t = s.topo
+    mono_of = np.searchsorted(t.poipt[1:], np.arange(1, t.npt + 1))
+    lines = list(title) + ['%7d' % t.npt]
+    for i in range(t.npt):
+        m = int(mono_of[i])
+        x, y, z = s.coor[i]
+        lines.append('%7d%7d %-4s %-4s%10.5f%10.5f%10.5f %-4s %-4s%10.5f' % (
+            i + 1, m + 1, t.moname[m][:4], t.ptnm[i][:4], x, y, z, 'SYNT', str(m + 1)[:4], 0.0))
+    with open(path, 'w') as f:
+        f.write('\n'.join(lines) + '\n')
+
+
+def test_coordinate_reader(tmp_path):
+    from moil_fnrfd_amd.eforce import MoilGPUError, read_crd
+    s, cut, ew, sqz = make_case('hetero_small')
+    path = os.path.join(tmp_path, 's.crd')
+    _write_crd(path, s)
+    x = read_crd(path, s.topo.npt)
+    # f10.5 fields: the values as written, to the last bit
+    ref = np.array([[float('%10.5f' % v) for v in row] for row in s.coor])
+    np.testing.assert_array_equal(x, ref)
+    assert np.abs(x - s.coor).max() <= 5e-6
+    lines = open(path).read().splitlines()
+    bad = os.path.join(tmp_path, 'bad.crd')
+    with open(bad, 'w') as f:
+        f.write('\n'.join(lines[:3] + lines[4:]) + '\n')  # particle 1 missing
+    with pytest.raises(MoilGPUError, match='missing'):
+        read_crd(bad, s.topo.npt)
+    with pytest.raises(MoilGPUError, match='does not match'):
+        read_crd(path, s.topo.npt + 1)
+
+
+def test_coordinate_reader_matches_moil_getcrd(tmp_path):
+    """MOIL's own getcrd (through oracle/_ref/moil_dump) reads the same file
+    to the same coordinates, bit for bit (the file's particle names are
+    distinct within each monomer, where getcrd's name matching and the
+    library's record order agree)."""
+    from oracle import ref_runner
+    if not ref_runner.available():
+        pytest.skip('oracle/_ref/moil_dump not built')
+    from moil_fnrfd_amd.eforce import read_crd
+    s, cut, ew, sqz = make_case('hetero_small')
+    path = os.path.join(tmp_path, 's.crd')
+    _write_crd(path, s)
+    d = ref_runner.run_moil(s, cut, ew, dump_lists=False, crd_file=path, timeout=300)
+    np.testing.assert_array_equal(read_crd(path, s.topo.npt), d['COOR'].reshape(-1, 3))
