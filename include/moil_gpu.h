@@ -213,6 +213,14 @@ int32_t moil_con_free(moil_con *con);
  * coor(3,npt).  Only complete files: a particle record out of sequence is
  * refused (getcrd would fill placeholders and build hydrogens). */
 int32_t moil_crd_read(const char *path, int32_t npt, double *coor);
+/* Binary trajectory as MOIL's wdyncrd writes it (dynamics/wdyncrd.f:24-88,
+ * binary, no frozen particles): moil_traj_open writes the header for nfrm
+ * frames of npt particles, each moil_traj_write one frame from coor(3,npt)
+ * (host, stored in single precision).  Byte-identical to MOIL's file. */
+typedef struct moil_traj moil_traj;
+int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, moil_traj **out);
+int32_t moil_traj_write(moil_traj *traj, const double *coor);
+int32_t moil_traj_close(moil_traj *traj);
 
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain

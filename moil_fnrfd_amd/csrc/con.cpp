@@ -1,6 +1,7 @@
 // Readers of MOIL's connectivity (.con) files into the library's topology
-// struct, and of its CHARMM-style coordinate files, for C and C++ callers
-// that load a MOIL system without MOIL's Fortran.  It follows MOIL's own reader rconn (generic/rconn.f:46-497):
+// struct and of its CHARMM-style coordinate files, and a writer of its
+// binary trajectories, for C and C++ callers that use MOIL's files
+// without MOIL's Fortran.  It follows MOIL's own reader rconn (generic/rconn.f:46-497):
 // the same records in the same order, each read with the fixed-column
 // layout of rconn's FORMAT statements, including Fortran's format
 // reversion (a list longer than one record continues on the next record
@@ -14,6 +15,7 @@
 // combination rule -- the library supports none of them.
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -418,6 +420,84 @@ extern "C" int32_t moil_crd_read(const char *path, int32_t npt, double *coor) {
     if (i1 != i + 1) return con_fail(path, "particle " + std::to_string(i + 1) + " missing");
     r.skip(7 + 1 + 4 + 1 + 4);
     for (int d = 0; d < 3; d++) coor[3 * i + d] = freal(r.field(10), 5);
+  }
+  return MOIL_GPU_OK;
+}
+
+// Binary trajectory as MOIL's wdyncrd writes it (dynamics/wdyncrd.f, bin
+// = 1, no frozen particles): Fortran unformatted sequential records, each
+// framed by its byte length (4-byte integer) before and after.  Header:
+// 'CORD' and 20 integers (the frame count first, the frozen-particle
+// count ninth), then 1 with an 80-character title, then npt; each frame:
+// x, y and z of all particles in single precision, one record each.
+struct moil_traj {
+  FILE *f = nullptr;
+  int32_t npt = 0;
+  std::vector<float> buf;
+};
+
+namespace {
+
+bool frec(FILE *f, const void *data, uint32_t nbytes) {
+  return std::fwrite(&nbytes, 4, 1, f) == 1 &&
+         (nbytes == 0 || std::fwrite(data, 1, nbytes, f) == nbytes) &&
+         std::fwrite(&nbytes, 4, 1, f) == 1;
+}
+
+}  // namespace
+
+extern "C" int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, moil_traj **out) {
+  if (!path || !out || npt <= 0 || nfrm < 0) {
+    set_error("traj_open: null argument or no particles");
+    return MOIL_GPU_EINVAL;
+  }
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return con_fail(path, "cannot create");
+  char hdr[4 + 20 * 4];
+  int32_t vari[20] = {0};
+  vari[0] = nfrm;  // vari(1): frames; vari(9): frozen particles (none)
+  std::memcpy(hdr, "CORD", 4);
+  std::memcpy(hdr + 4, vari, sizeof(vari));
+  char title[4 + 80];
+  const int32_t one = 1;
+  std::memcpy(title, &one, 4);
+  std::memset(title + 4, ' ', 80);
+  const char *dummy = "A dummy title to dynamics coordinates";
+  std::memcpy(title + 4, dummy, std::strlen(dummy));
+  if (!frec(f, hdr, sizeof(hdr)) || !frec(f, title, sizeof(title)) || !frec(f, &npt, 4)) {
+    std::fclose(f);
+    return con_fail(path, "write failed");
+  }
+  moil_traj *t = new moil_traj();
+  t->f = f;
+  t->npt = npt;
+  t->buf.resize(npt);
+  *out = t;
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_traj_write(moil_traj *t, const double *coor) {
+  if (!t || !coor) {
+    set_error("traj_write: null argument");
+    return MOIL_GPU_EINVAL;
+  }
+  for (int k = 0; k < 3; k++) {
+    for (int32_t i = 0; i < t->npt; i++) t->buf[i] = (float)coor[3 * (size_t)i + k];
+    if (!frec(t->f, t->buf.data(), (uint32_t)(4 * (size_t)t->npt))) {
+      set_error("traj_write: write failed");
+      return MOIL_GPU_EINVAL;
+    }
+  }
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_traj_close(moil_traj *t) {
+  if (!t) return MOIL_GPU_OK;
+  const bool ok = std::fclose(t->f) == 0;
+  delete t;
+  if (!ok) {
+    set_error("traj_close: close failed");
+    return MOIL_GPU_EINVAL;
   }
   return MOIL_GPU_OK;
 }

@@ -86,6 +86,9 @@ def load_library():
     L.moil_con_read.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(abi.MoilTopology)]
     L.moil_con_free.argtypes = [vp]
     L.moil_crd_read.argtypes = [C.c_char_p, C.c_int32, vp]
+    L.moil_traj_open.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.moil_traj_write.argtypes = [vp, vp]
+    L.moil_traj_close.argtypes = [vp]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -95,7 +98,8 @@ def load_library():
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
-                 'moil_con_read', 'moil_con_free', 'moil_crd_read'):
+                 'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
+                 'moil_traj_write', 'moil_traj_close'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -382,3 +386,29 @@ def read_crd(path, npt):
     x = np.zeros((npt, 3))
     _check(load_library().moil_crd_read(os.fsencode(path), int(npt), x.ctypes.data))
     return x
+
+
+class MoilTraj:
+    """A binary trajectory in MOIL's wdyncrd format (moil_traj_*): nfrm
+    frames of npt particles, single precision.  Host only."""
+
+    def __init__(self, path, npt, nfrm):
+        h = C.c_void_p()
+        _check(load_library().moil_traj_open(os.fsencode(path), int(npt), int(nfrm), C.byref(h)))
+        self._h = h
+        self.npt = int(npt)
+
+    def write(self, coor):
+        x = _f64(coor, self.npt)
+        _check(load_library().moil_traj_write(self._h, x.ctypes.data))
+
+    def close(self):
+        if getattr(self, '_h', None):
+            h, self._h = self._h, None
+            _check(load_library().moil_traj_close(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

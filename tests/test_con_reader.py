@@ -167,3 +167,39 @@ def test_coordinate_reader_matches_moil_getcrd(tmp_path):
     _write_crd(path, s)
     d = ref_runner.run_moil(s, cut, ew, dump_lists=False, crd_file=path, timeout=300)
     np.testing.assert_array_equal(read_crd(path, s.topo.npt), d['COOR'].reshape(-1, 3))
+
+
+def test_trajectory_writer_matches_moil_wdyncrd(tmp_path):
+    """Three frames written by the library and by MOIL's own wdyncrd
+    (oracle/_ref/moil_traj): the files are identical byte for byte."""
+    import subprocess
+
+    from moil_fnrfd_amd.eforce import MoilTraj
+    from oracle import ref_runner
+    exe = os.path.join(os.path.dirname(ref_runner.REF_EXE), 'moil_traj')
+    s, cut, ew, sqz = make_case('hetero_small')
+    npt = s.topo.npt
+    rng = np.random.default_rng(9)
+    frames = [s.coor + rng.normal(0.0, 0.1, s.coor.shape) for _ in range(3)]
+    mine = os.path.join(tmp_path, 'lib.dcd')
+    t = MoilTraj(mine, npt, len(frames))
+    for x in frames:
+        t.write(x)
+    t.close()
+    data = open(mine, 'rb').read()
+    # layout: header records then 3 single-precision records per frame
+    head = 4 + 84 + 4 + 4 + 84 + 4 + 4 + 4 + 4
+    assert len(data) == head + len(frames) * 3 * (8 + 4 * npt)
+    off = head + 4
+    np.testing.assert_array_equal(np.frombuffer(data, np.float32, npt, off),
+                                  frames[0][:, 0].astype(np.float32))
+    if not os.path.exists(exe):
+        pytest.skip('oracle/_ref/moil_traj not built')
+    raw = os.path.join(tmp_path, 'frames.bin')
+    np.ascontiguousarray(np.stack(frames), np.float64).tofile(raw)
+    ref = os.path.join(tmp_path, 'moil.dcd')
+    par = os.path.join(tmp_path, 'traj.par')
+    with open(par, 'w') as f:
+        f.write('%d %d\n%s\n%s\n' % (npt, len(frames), raw, ref))
+    subprocess.run([exe, par], check=True, capture_output=True, timeout=60)
+    assert open(ref, 'rb').read() == data
