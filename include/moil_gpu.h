@@ -172,9 +172,11 @@ int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
  * them, d_coor[i], d_dpot[i], d_energies[i] being system i's device arrays
  * as for moil_gpu_eforce_dev.  Every kernel runs once over all systems
  * (per-system argument tables and workgroup offsets), the transforms as
- * one batched plan per grid size; the step is captured once into a
- * hipGraph and replayed, and recaptured when a system's lists were
- * rebuilt or the pointers change.  `nstreams` is accepted for
+ * one batched plan per grid size; the step is captured once into two
+ * hipGraphs (real space; reciprocal sums with the transform groups on up
+ * to four streams) replayed side by side, and recaptured when a system's
+ * lists were rebuilt or the pointers change (moil_gpu_batch_set_graph
+ * switches to direct launches).  `nstreams` is accepted for
  * compatibility and unused.  Ordered after the caller's earlier work on
  * `stream` (NULL = null stream) and before its later work.  Results are
  * identical to one moil_gpu_eforce_dev per system. */
