@@ -223,6 +223,11 @@ typedef struct moil_traj moil_traj;
 int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, moil_traj **out);
 int32_t moil_traj_write(moil_traj *traj, const double *coor);
 int32_t moil_traj_close(moil_traj *traj);
+/* Frame `frame` (0-based) of such a trajectory into coor(3,npt), as MOIL's
+ * rdyncrd reads it (dynamics/rdyncrd.f:40-70); *nframes (if not NULL)
+ * gets the header's frame count. */
+int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, double *coor,
+                       int32_t *nframes);
 
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain

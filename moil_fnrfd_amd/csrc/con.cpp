@@ -501,3 +501,49 @@ extern "C" int32_t moil_traj_close(moil_traj *t) {
   }
   return MOIL_GPU_OK;
 }
+
+// Frame `frame` (0-based) of a binary trajectory written as above, read
+// the way MOIL's rdyncrd reads it (dynamics/rdyncrd.f:40-70, binary, no
+// frozen particles): the header's particle count must equal npt; each
+// frame's x, y and z records are widened to double precision into
+// coor(3,npt).  The record lengths are checked.
+extern "C" int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, double *coor,
+                                  int32_t *nframes) {
+  if (!path || !coor || npt <= 0 || frame < 0) {
+    set_error("traj_read: null argument, no particles or a negative frame");
+    return MOIL_GPU_EINVAL;
+  }
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return con_fail(path, "cannot open");
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(f, std::fclose);
+  auto rd = [&](std::vector<char> &buf) -> bool {
+    uint32_t n = 0, m = 0;
+    if (std::fread(&n, 4, 1, f) != 1) return false;
+    buf.resize(n);
+    if (n && std::fread(buf.data(), 1, n, f) != n) return false;
+    return std::fread(&m, 4, 1, f) == 1 && m == n;
+  };
+  std::vector<char> b;
+  if (!rd(b) || b.size() != 84 || std::memcmp(b.data(), "CORD", 4) != 0)
+    return con_fail(path, "not a trajectory ('CORD' header)");
+  int32_t vari[20];
+  std::memcpy(vari, b.data() + 4, sizeof(vari));
+  if (vari[8] != 0) return con_fail(path, "frozen particles are not supported");
+  if (!rd(b)) return con_fail(path, "truncated header");
+  if (!rd(b) || b.size() != 4) return con_fail(path, "truncated header");
+  int32_t n;
+  std::memcpy(&n, b.data(), 4);
+  if (n != npt) return con_fail(path, "number of particles does not match");
+  if (nframes) *nframes = vari[0];
+  for (int32_t fr = 0; fr <= frame; fr++) {
+    for (int k = 0; k < 3; k++) {
+      if (!rd(b) || b.size() != 4 * (size_t)npt)
+        return con_fail(path, "frame " + std::to_string(frame) + " not in the file");
+      if (fr == frame) {
+        const float *v = reinterpret_cast<const float *>(b.data());
+        for (int32_t i = 0; i < npt; i++) coor[3 * (size_t)i + k] = (double)v[i];
+      }
+    }
+  }
+  return MOIL_GPU_OK;
+}

@@ -89,6 +89,7 @@ def load_library():
     L.moil_traj_open.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.moil_traj_write.argtypes = [vp, vp]
     L.moil_traj_close.argtypes = [vp]
+    L.moil_traj_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp, C.POINTER(C.c_int32)]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -99,7 +100,7 @@ def load_library():
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
                  'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
-                 'moil_traj_write', 'moil_traj_close'):
+                 'moil_traj_write', 'moil_traj_close', 'moil_traj_read'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -412,3 +413,13 @@ class MoilTraj:
             self.close()
         except Exception:
             pass
+
+
+def read_traj(path, npt, frame):
+    """(coordinates (npt, 3), frame count) of frame `frame` (0-based) of a
+    binary trajectory in MOIL's format, read by moil_traj_read."""
+    x = np.zeros((npt, 3))
+    n = C.c_int32()
+    _check(load_library().moil_traj_read(os.fsencode(path), int(npt), int(frame), x.ctypes.data,
+                                         C.byref(n)))
+    return x, n.value

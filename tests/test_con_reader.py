@@ -203,3 +203,13 @@ def test_trajectory_writer_matches_moil_wdyncrd(tmp_path):
         f.write('%d %d\n%s\n%s\n' % (npt, len(frames), raw, ref))
     subprocess.run([exe, par], check=True, capture_output=True, timeout=60)
     assert open(ref, 'rb').read() == data
+    # and MOIL's file reads back frame by frame, widened from single precision
+    from moil_fnrfd_amd.eforce import MoilGPUError, read_traj
+    for k, x in enumerate(frames):
+        y, nf = read_traj(ref, npt, k)
+        assert nf == len(frames)
+        np.testing.assert_array_equal(y, x.astype(np.float32).astype(np.float64))
+    with pytest.raises(MoilGPUError, match='not in the file'):
+        read_traj(ref, npt, len(frames))
+    with pytest.raises(MoilGPUError, match='does not match'):
+        read_traj(ref, npt + 1, 0)
