@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <exception>
 #include <memory>
 #include <string>
 #include <vector>
@@ -151,7 +152,7 @@ int32_t mono_flag(std::string nm) {
 
 }  // namespace
 
-extern "C" int32_t moil_con_read(const char *path, moil_con **out, moil_topology *topo) {
+static int32_t con_read_impl(const char *path, moil_con **out, moil_topology *topo) {
   if (!path || !out || !topo) {
     set_error("con_read: null argument");
     return MOIL_GPU_EINVAL;
@@ -184,6 +185,9 @@ extern "C" int32_t moil_con_read(const char *path, moil_con **out, moil_topology
              totex = h[6], totspe = h[7], lestyp = h[8], nbulk = h[9], nmb = h[10],
              nbeta = h[12], nwaters = h[13];
   if (nbulk <= 0 || totmon <= 0 || npt <= 0) return con_fail(path, "no molecules, monomers or particles");
+  const long cap = 1L << 28;  // far above any MOIL LENGTH block
+  for (long x : h)
+    if (x < 0 || x > cap) return con_fail(path, "implausible counts in the header");
   if (lestyp > 0) return con_fail(path, "LES copies are not supported");
   if (nmb > 0) return con_fail(path, "Morse bonds are not supported");
   if (nbeta > 0) return con_fail(path, "the beta list is not supported");
@@ -396,7 +400,7 @@ extern "C" int32_t moil_con_free(moil_con *c) {
 // format(i7,i7,1x,a4,1x,a4,3(f10.5),1x,a4,1x,a4,f10.5).  Only complete
 // files are read: getcrd's repair of missing particles (placeholders and
 // built hydrogens) is not done, a record out of sequence is an error.
-extern "C" int32_t moil_crd_read(const char *path, int32_t npt, double *coor) {
+static int32_t crd_read_impl(const char *path, int32_t npt, double *coor) {
   if (!path || !coor || npt <= 0) {
     set_error("crd_read: null argument or no particles");
     return MOIL_GPU_EINVAL;
@@ -446,11 +450,13 @@ bool frec(FILE *f, const void *data, uint32_t nbytes) {
 
 }  // namespace
 
-extern "C" int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, moil_traj **out) {
+static int32_t traj_open_impl(const char *path, int32_t npt, int32_t nfrm, moil_traj **out) {
   if (!path || !out || npt <= 0 || nfrm < 0) {
     set_error("traj_open: null argument or no particles");
     return MOIL_GPU_EINVAL;
   }
+  auto t = std::make_unique<moil_traj>();
+  t->buf.resize(npt);  // before the file exists: nothing to clean up if it throws
   FILE *f = std::fopen(path, "wb");
   if (!f) return con_fail(path, "cannot create");
   char hdr[4 + 20 * 4];
@@ -468,12 +474,21 @@ extern "C" int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, m
     std::fclose(f);
     return con_fail(path, "write failed");
   }
-  moil_traj *t = new moil_traj();
   t->f = f;
   t->npt = npt;
-  t->buf.resize(npt);
-  *out = t;
+  *out = t.release();
   return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_traj_open(const char *path, int32_t npt, int32_t nfrm, moil_traj **out) {
+  try {
+    return traj_open_impl(path, npt, nfrm, out);
+  } catch (const std::exception &e) {
+    set_error(std::string("traj_open: ") + e.what());
+  } catch (...) {
+    set_error("traj_open: failed");
+  }
+  return MOIL_GPU_EINVAL;
 }
 
 extern "C" int32_t moil_traj_write(moil_traj *t, const double *coor) {
@@ -507,8 +522,8 @@ extern "C" int32_t moil_traj_close(moil_traj *t) {
 // frozen particles): the header's particle count must equal npt; each
 // frame's x, y and z records are widened to double precision into
 // coor(3,npt).  The record lengths are checked.
-extern "C" int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, double *coor,
-                                  int32_t *nframes) {
+static int32_t traj_read_impl(const char *path, int32_t npt, int32_t frame, double *coor,
+                               int32_t *nframes) {
   if (!path || !coor || npt <= 0 || frame < 0) {
     set_error("traj_read: null argument, no particles or a negative frame");
     return MOIL_GPU_EINVAL;
@@ -546,4 +561,40 @@ extern "C" int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, 
     }
   }
   return MOIL_GPU_OK;
+}
+
+// the readers never let an exception (a bad allocation on a corrupt file)
+// cross the C boundary
+extern "C" int32_t moil_con_read(const char *path, moil_con **out, moil_topology *topo) {
+  try {
+    return con_read_impl(path, out, topo);
+  } catch (const std::exception &e) {
+    set_error(std::string("con_read: ") + e.what());
+  } catch (...) {
+    set_error("con_read: failed");
+  }
+  return MOIL_GPU_EINVAL;
+}
+
+extern "C" int32_t moil_crd_read(const char *path, int32_t npt, double *coor) {
+  try {
+    return crd_read_impl(path, npt, coor);
+  } catch (const std::exception &e) {
+    set_error(std::string("crd_read: ") + e.what());
+  } catch (...) {
+    set_error("crd_read: failed");
+  }
+  return MOIL_GPU_EINVAL;
+}
+
+extern "C" int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, double *coor,
+                                  int32_t *nframes) {
+  try {
+    return traj_read_impl(path, npt, frame, coor, nframes);
+  } catch (const std::exception &e) {
+    set_error(std::string("traj_read: ") + e.what());
+  } catch (...) {
+    set_error("traj_read: failed");
+  }
+  return MOIL_GPU_EINVAL;
 }
