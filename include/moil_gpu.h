@@ -229,6 +229,11 @@ int32_t moil_traj_close(moil_traj *traj);
 int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, double *coor,
                        int32_t *nframes);
 
+/* force_norm (pot/force_norm.f:1-44): the RMS gradient component
+ * sqrt(sum |dpot_i|^2 / (3 npt)) of dpot(3,npt), summed in MOIL's order.
+ * Host only. */
+int32_t moil_force_norm(const double *dpot, int32_t npt, double *gradf);
+
 /* Chain of replicas: MOIL's sds (chain/sds.f:1-257), the energy and
  * gradient of a chain of igrid replicas of one system that the chain
  * minimiser, annealer and chain dynamics evaluate (chain/chnneal.f:85,

@@ -598,3 +598,22 @@ extern "C" int32_t moil_traj_read(const char *path, int32_t npt, int32_t frame, 
   }
   return MOIL_GPU_EINVAL;
 }
+
+// MOIL's force_norm (pot/force_norm.f:1-44): the RMS gradient component,
+// sqrt(sum_i |dpot_i|^2 / (3 npt)), summed in force_norm's order (atom by
+// atom, ((gradf + x^2) + y^2) + z^2).  Host; dpot(3,npt).
+extern "C" int32_t moil_force_norm(const double *dpot, int32_t npt, double *gradf) {
+  if (!dpot || !gradf || npt <= 0) {
+    set_error("force_norm: null argument or no particles");
+    return MOIL_GPU_EINVAL;
+  }
+  double g = 0.0;
+  for (int32_t i = 0; i < npt; i++) {
+    const double *d = dpot + 3 * (size_t)i;
+    g = g + d[0] * d[0];
+    g = g + d[1] * d[1];
+    g = g + d[2] * d[2];
+  }
+  *gradf = std::sqrt(g / (double)(3 * npt));
+  return MOIL_GPU_OK;
+}

@@ -90,6 +90,7 @@ def load_library():
     L.moil_traj_write.argtypes = [vp, vp]
     L.moil_traj_close.argtypes = [vp]
     L.moil_traj_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp, C.POINTER(C.c_int32)]
+    L.moil_force_norm.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -100,7 +101,7 @@ def load_library():
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
                  'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
-                 'moil_traj_write', 'moil_traj_close', 'moil_traj_read'):
+                 'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -423,3 +424,11 @@ def read_traj(path, npt, frame):
     _check(load_library().moil_traj_read(os.fsencode(path), int(npt), int(frame), x.ctypes.data,
                                          C.byref(n)))
     return x, n.value
+
+
+def force_norm(dpot):
+    """MOIL's force_norm of a gradient dpot (npt, 3) (moil_force_norm)."""
+    d = np.ascontiguousarray(dpot, np.float64).reshape(-1, 3)
+    g = C.c_double()
+    _check(load_library().moil_force_norm(d.ctypes.data, len(d), C.byref(g)))
+    return g.value

@@ -213,3 +213,22 @@ def test_trajectory_writer_matches_moil_wdyncrd(tmp_path):
         read_traj(ref, npt, len(frames))
     with pytest.raises(MoilGPUError, match='does not match'):
         read_traj(ref, npt + 1, 0)
+
+
+def test_force_norm_follows_moil():
+    """moil_force_norm against force_norm's own sequential sum
+    (pot/force_norm.f:20-25), bit for bit, on MOIL's gradient of a golden
+    fixture; and against the closed form to rounding."""
+    import math
+
+    from moil_fnrfd_amd.eforce import force_norm
+    from tests._util import load_golden
+    dp = load_golden('hetero_small')['dpot']
+    g = 0.0
+    for x, y, z in dp:
+        g = g + x * x
+        g = g + y * y
+        g = g + z * z
+    ref = math.sqrt(g / (3 * len(dp)))
+    assert force_norm(dp) == ref
+    assert abs(ref - np.sqrt((dp ** 2).mean())) <= 1e-13 * ref
