@@ -272,7 +272,11 @@ int32_t moil_gpu_chain_destroy(moil_chain *chain);
  * last-water parameters (pot/watwat_ewald.f:40-72) and the 1-4 pair
  * parameters p14 from the scalings v14 and el14 (generic/wconn.f:296-325).
  * The list classification (flagchr, the hydrogen rule) keeps the values of
- * moil_gpu_init; no neutrality check.
+ * moil_gpu_init; no neutrality check.  Refused (MOIL_GPU_EINVAL): water
+ * charges or water O LJ factors that are not the same in every water
+ * (MOIL's water-water pairs use the last water's, so per-water values
+ * would make the energy's parameter gradient differ from what is
+ * returned).
  * moil_gpu_param_grad[_dev] returns dE/dq, dE/d(epsgm12) and dE/d(epsgm6)
  * per atom, (npt) float64 each, of the energy moil_gpu_eforce evaluates on
  * the current lists (dE/dx is eforce's dpot).  Every pair distance is
@@ -299,6 +303,12 @@ int32_t moil_gpu_dump_water_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
  * symmetry operation, list (1, 2, 3 as pot/nbond.f:136-149). */
 int32_t moil_gpu_dump_atom_pairs(moil_ctx *ctx, int32_t *rows, int64_t cap,
                                  int64_t *n);
+
+/* The number of list builds of the context so far (moil_gpu_build_lists,
+ * its _dev form and batched builds of the context each add one; no MOIL
+ * counterpart): a caller that keeps results of an evaluation checks with
+ * it that the lists they were computed on are still the context's. */
+int32_t moil_gpu_list_generation(moil_ctx *ctx, int64_t *gen);
 
 /* Stage timing with HIP events on the launch stream.  Evaluations run in
  * order on the launch stream.  With MOIL_GPU_GRAPH=1 in the environment

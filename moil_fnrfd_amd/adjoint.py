@@ -21,7 +21,9 @@ gradients are the chain rule over the atoms of each type:
 (Rmin = 2 rmin_half, so dE/dRmin = dE/drmh / 2.)
 
 The lists are the engine's (built by ``engine.nbondm`` for the
-conformation, exactly as MOIL's buffered lists); every call sets the
+conformation, exactly as MOIL's buffered lists); backward raises if the
+engine's lists were rebuilt after the forward evaluation (the parameter
+gradients would be those of another pair set).  Every call sets the
 parameters on the engine (``moil_gpu_set_params``), so e_self, the
 last-water parameters and the 1-4 pair parameters follow them.
 Coordinates may be host (CPU) or device (GPU) tensors; parameters are
@@ -68,6 +70,7 @@ class _MoilEnergy(torch.autograd.Function):
             e_total = torch.tensor(energies['e_total'], dtype=torch.float64)
             xc = xd
         ctx.engine = engine
+        ctx.list_gen = engine.list_generation()
         ctx.lj_type = torch.as_tensor(lj_type, dtype=torch.long)
         ctx.save_for_backward(xc, dpot, e12, e6, eps.detach().cpu(), rmin_half.detach().cpu(),
                               torch.from_numpy(qh))
@@ -79,6 +82,13 @@ class _MoilEnergy(torch.autograd.Function):
     def backward(ctx, g):
         xc, dpot, e12, e6, eps, rmh, qh = ctx.saved_tensors
         engine = ctx.engine
+        if any(ctx.needs_input_grad[1:4]) and engine.list_generation() != ctx.list_gen:
+            # the parameter gradients are computed on the engine's lists; the
+            # forward's pair set (MOIL's buffered lists, built at earlier
+            # coordinates) is gone
+            raise RuntimeError('moil_energy: the engine rebuilt its pair lists after this '
+                               'forward evaluation; back-propagate before the next nbondm, '
+                               'or give each conformation its own engine')
         # the parameters of this forward evaluation (another call may have
         # replaced them on the engine since)
         engine.set_params(qh.numpy(), e6.numpy(), e12.numpy(), *ctx.scale14)

@@ -140,10 +140,97 @@ static int32_t ensure_parts(moil_ctx *c, int need) {
   return MOIL_GPU_OK;
 }
 
+// Every index array of the topology against the sizes it indexes, before
+// any of it is used: atom numbers in 1..npt, monomer numbers in 1..totmon,
+// the monomer and exclusion CSR pointers non-decreasing and in range.  A
+// .con record with a blank index field (read as 0) ends here as EINVAL
+// instead of as an access outside an array, on the host or the device.
+// The exclusion pointers are returned as a complete CSR in exc1n: MOIL's
+// rconn sets exc1 only up to the last atom that has an exclusion record
+// (generic/rconn.f:313-338), so in MOIL's own COMMON the entries after it
+// are still 0; they are read as empty ranges, as MOIL's loops over
+// exc1(i-1)+1..exc1(i) read them.
+static int32_t check_topology(const moil_topology *t, std::vector<int32_t> &exc1n) {
+  const int32_t npt = t->npt, nmo = t->totmon;
+  if (npt <= 0 || nmo <= 0) return fail(MOIL_GPU_EINVAL, "empty system");
+  if (t->nb < 0 || t->nangl < 0 || t->ntors < 0 || t->nimp < 0 || t->totspe < 0 ||
+      t->nwaters < 0)
+    return fail(MOIL_GPU_EINVAL, "negative term count");
+  if (!t->poipt || !t->monoflag || !t->ptchg || !t->epsgm6 || !t->epsgm12 || !t->flagchr ||
+      !t->exc1)
+    return fail(MOIL_GPU_EINVAL, "null topology array");
+  if (t->poipt[0] != 0 || t->poipt[nmo] != npt)
+    return fail(MOIL_GPU_EINVAL, "poipt must start at 0 and end at npt");
+  for (int32_t m = 0; m < nmo; m++)
+    if (t->poipt[m + 1] < t->poipt[m])
+      return fail(MOIL_GPU_EINVAL, "poipt decreases at monomer " + std::to_string(m + 1));
+  if (t->exc1[0] != 0) return fail(MOIL_GPU_EINVAL, "exc1 must start at 0");
+  exc1n.assign((size_t)npt + 1, 0);
+  {
+    int32_t top = 0;
+    bool tail = false;
+    for (int32_t a = 1; a <= npt; a++) {
+      const int32_t v = t->exc1[a];
+      if (!tail && v >= top) {
+        top = v;
+      } else if (v == 0) {
+        tail = true;
+      } else {
+        return fail(MOIL_GPU_EINVAL, "exc1 decreases at atom " + std::to_string(a));
+      }
+      exc1n[a] = top;
+    }
+  }
+  const int64_t nexc = exc1n[npt];
+  if (nexc > 0 && !t->exc2) return fail(MOIL_GPU_EINVAL, "null exc2");
+  for (int64_t k = 0; k < nexc; k++)
+    if (t->exc2[k] < 1 || t->exc2[k] > npt)
+      return fail(MOIL_GPU_EINVAL, "exclusion partner " + std::to_string(t->exc2[k]) +
+                                       " outside 1.." + std::to_string(npt));
+  struct Idx {
+    const char *name;
+    const int32_t *v;
+    int32_t n;
+  };
+  const Idx ids[] = {{"ib1", t->ib1, t->nb},         {"ib2", t->ib2, t->nb},
+                     {"iangl1", t->iangl1, t->nangl}, {"iangl2", t->iangl2, t->nangl},
+                     {"iangl3", t->iangl3, t->nangl}, {"itor1", t->itor1, t->ntors},
+                     {"itor2", t->itor2, t->ntors},   {"itor3", t->itor3, t->ntors},
+                     {"itor4", t->itor4, t->ntors},   {"iimp1", t->iimp1, t->nimp},
+                     {"iimp2", t->iimp2, t->nimp},    {"iimp3", t->iimp3, t->nimp},
+                     {"iimp4", t->iimp4, t->nimp},    {"spec1", t->spec1, t->totspe},
+                     {"spec2", t->spec2, t->totspe}};
+  for (const Idx &x : ids) {
+    if (x.n > 0 && !x.v) return fail(MOIL_GPU_EINVAL, std::string("null ") + x.name);
+    for (int32_t m = 0; m < x.n; m++)
+      if (x.v[m] < 1 || x.v[m] > npt)
+        return fail(MOIL_GPU_EINVAL, std::string(x.name) + "(" + std::to_string(m + 1) +
+                                         ") = " + std::to_string(x.v[m]) + " outside 1.." +
+                                         std::to_string(npt));
+  }
+  const void *par[] = {t->kbond, t->req, t->kangl, t->angleq, t->ktors1, t->ktors2,
+                       t->ktors3, t->phase1, t->phase2, t->phase3, t->kimp, t->impeq, t->p14};
+  const int32_t npar[] = {t->nb, t->nb, t->nangl, t->nangl, t->ntors, t->ntors, t->ntors,
+                          t->ntors, t->ntors, t->ntors, t->nimp, t->nimp, t->totspe};
+  for (int q = 0; q < 13; q++)
+    if (npar[q] > 0 && !par[q]) return fail(MOIL_GPU_EINVAL, "null term parameter array");
+  if (t->nwaters > 0 && !t->idxtip3) return fail(MOIL_GPU_EINVAL, "null idxtip3");
+  for (int32_t w = 0; w < t->nwaters; w++)
+    if (t->idxtip3[w] < 1 || t->idxtip3[w] > nmo)
+      return fail(MOIL_GPU_EINVAL, "idxtip3(" + std::to_string(w + 1) + ") = " +
+                                       std::to_string(t->idxtip3[w]) + " outside 1.." +
+                                       std::to_string(nmo));
+  return MOIL_GPU_OK;
+}
+
 extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, moil_ctx **out) {
   if (!t || !p || !out) return fail(MOIL_GPU_EINVAL, "null argument");
   *out = nullptr;
-  if (t->npt <= 0 || t->totmon <= 0) return fail(MOIL_GPU_EINVAL, "empty system");
+  std::vector<int32_t> exc1n;
+  CHECK_RC(check_topology(t, exc1n));
+  moil_topology tn = *t;  // the same arrays, the exclusion pointers complete
+  tn.exc1 = exc1n.data();
+  t = &tn;
   if (p->order != 4) return fail(MOIL_GPU_EINVAL, "PME interpolation order must be 4");
   if (!(p->a > 0 && p->b > 0 && p->c > 0)) return fail(MOIL_GPU_EINVAL, "box edges must be > 0");
   if (p->nfft1 < 4 || p->nfft2 < 4 || p->nfft3 < 4) return fail(MOIL_GPU_EINVAL, "PME grid too small");
@@ -159,8 +246,6 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   if (std::fabs(qtot) > 1e-7)  // symm/ewald_init.f:62-72
     return fail(MOIL_GPU_ECHARGE, "The charge of the system is " + std::to_string(qtot) +
                                       ": add counterions and try again");
-  if (t->poipt[0] != 0 || t->poipt[t->totmon] != t->npt)
-    return fail(MOIL_GPU_EINVAL, "poipt must start at 0 and end at npt");
 
   moil_ctx *c = new moil_ctx();
   c->par = *p;
@@ -255,6 +340,12 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     ctx_free(c);
     return fail(MOIL_GPU_EINVAL, "nwaters does not match the number of TIP3/SPC monomers");
   }
+  for (int w = 0; w < t->nwaters; w++)
+    if (!(t->monoflag[t->idxtip3[w] - 1] & MOIL_MONO_WATER)) {
+      ctx_free(c);
+      return fail(MOIL_GPU_EINVAL, "idxtip3(" + std::to_string(w + 1) +
+                                       ") is not a TIP3/SPC monomer");
+    }
   // MOIL's water routines assume the three intra-water pairs are excluded
   // (the atom lists then never hold an intra-water pair)
   for (int w = 0; w < c->nwat; w++) {
@@ -1104,6 +1195,12 @@ extern "C" int32_t moil_gpu_dump_atom_pairs(moil_ctx *c, int32_t *rows, int64_t 
   return MOIL_GPU_OK;
 }
 
+extern "C" int32_t moil_gpu_list_generation(moil_ctx *c, int64_t *gen) {
+  if (!c || !gen) return fail(MOIL_GPU_EINVAL, "null argument");
+  *gen = (int64_t)c->list_gen;
+  return MOIL_GPU_OK;
+}
+
 extern "C" int32_t moil_gpu_set_timing(moil_ctx *c, int32_t level) {
   if (!c) return fail(MOIL_GPU_EINVAL, "null context");
   const int lv = level & 0xff, stride = (level >> 8) & 0xff;
@@ -1143,6 +1240,24 @@ extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const d
                                        const double *epsgm12, double v14, double el14) {
   if (!c || !ptchg || !epsgm6 || !epsgm12) return fail(MOIL_GPU_EINVAL, "null argument");
   const size_t n = (size_t)c->npt;
+  // Water-water pairs take the last water's charges and O LJ factors
+  // (pot/watwat_ewald.f:40-72, the H charge being the last H's); the
+  // parameter gradient spreads each such pair over the two waters' own
+  // atoms, which is the gradient of the energy eforce evaluates only when
+  // every water carries those same values.
+  if (c->nwat >= 2) {
+    const int lo = c->h_wat_o[c->nwat - 1];
+    const double qo = ptchg[lo], qh = ptchg[lo + 2], e6o = epsgm6[lo], e12o = epsgm12[lo];
+    for (int w = 0; w < c->nwat; w++) {
+      const int o = c->h_wat_o[w];
+      if (ptchg[o] != qo || ptchg[o + 1] != qh || ptchg[o + 2] != qh || epsgm6[o] != e6o ||
+          epsgm12[o] != e12o)
+        return fail(MOIL_GPU_EINVAL,
+                    "set_params: water " + std::to_string(w + 1) +
+                        " differs from the last water in its charges or O LJ factors "
+                        "(water-water pairs use the last water's)");
+    }
+  }
   c->h_q.assign(ptchg, ptchg + n);
   c->h_e6.assign(epsgm6, epsgm6 + n);
   c->h_e12.assign(epsgm12, epsgm12 + n);

@@ -354,6 +354,12 @@ extern "C" int32_t moil_gpu_chain_sds(moil_chain *ch, const double *r, const int
                     hipMalloc(&ch->d_out, sizeof(double) * (2 * nd + igrid + 1)) == hipSuccess &&
                     hipMalloc(&ch->d_pick, sizeof(int32_t) * ch->npt) == hipSuccess;
     if (!ok) {
+      // all four or none: the next call allocates again
+      void *bufs[4] = {ch->d_r, ch->d_dv, ch->d_out, ch->d_pick};
+      for (void *b : bufs)
+        if (b) hipFree(b);
+      ch->d_r = ch->d_dv = ch->d_out = nullptr;
+      ch->d_pick = nullptr;
       set_error("chain_sds: device allocation failed");
       return MOIL_GPU_EHIP;
     }

@@ -69,6 +69,7 @@ def load_library():
     L.moil_gpu_dump_water_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_dump_atom_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_set_timing.argtypes = [vp, C.c_int32]
+    L.moil_gpu_list_generation.argtypes = [vp, C.POINTER(C.c_int64)]
     L.moil_gpu_batch_create.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.moil_gpu_batch_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_batch_destroy.argtypes = [vp]
@@ -95,7 +96,7 @@ def load_library():
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
-                 'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_batch_create',
+                 'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_list_generation', 'moil_gpu_batch_create',
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
@@ -227,6 +228,12 @@ class MoilGPU:
         rows = np.zeros((max(n.value, 1), 4), np.int32)
         _check(load_library().moil_gpu_dump_atom_pairs(self._h, rows.ctypes.data, n.value, C.byref(n)))
         return rows[:n.value]
+
+    def list_generation(self):
+        """Number of list builds of this context so far."""
+        g = C.c_int64()
+        _check(load_library().moil_gpu_list_generation(self._h, C.byref(g)))
+        return g.value
 
     def set_timing(self, level=1, stride=1):
         """0 off, 1 real-space kernels only (no dispatch gaps; every

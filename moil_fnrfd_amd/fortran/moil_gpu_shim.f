@@ -191,6 +191,66 @@ c     copies them to the GPU during the call
       end module moil_gpu_c
 
 
+c     The energy terms and modes of pot/eforce.f:86-311 that the library
+c     does not evaluate: with any of them switched on MOIL would compute a
+c     different energy, so the shim stops through alert(...,1) instead of
+c     returning the library's.  The library evaluates the constant
+c     dielectric (ctrue) Ewald path with periodic images (ewaldyes,
+c     esymyes), MOIL's geometric combination rule, no LES copies, no
+c     frozen particles, one process, and the terms bond, angle,
+c     torsion, improper, LJ, Coulomb and the scaled 1-4 pairs.
+      subroutine mg_modes(where)
+      use moil_gpu_c
+      character(len=*), intent(in) :: where
+      include 'COMMON/LENGTH.BLOCK'
+      include 'COMMON/CONNECT.BLOCK'
+      include 'COMMON/ENERGY.BLOCK'
+      include 'COMMON/CONSTRAN.BLOCK'
+      include 'COMMON/EWALD.BLOCK'
+      include 'COMMON/PARALLEL.BLOCK'
+      include 'COMMON/SGB.BLOCK'
+      include 'COMMON/SSBP.BLOCK'
+      include 'COMMON/REPWALL.BLOCK'
+      include 'COMMON/EFIELD.BLOCK'
+      include 'COMMON/PRESS.BLOCK'
+      include 'COMMON/FREEZ.BLOCK'
+      character(len=60) :: why
+      why = ' '
+      if (.not. ctrue) why = 'distance-dependent dielectric (rdie)'
+      if (shift) why = 'shifted nonbonded potential (shift)'
+      if (efyes) why = 'finite nonbonded interactions (efyes)'
+      if (.not. ewaldyes) why = 'nonbonded terms without Ewald'
+      if (.not. esymyes) why = 'no periodic images (esymyes off)'
+      if (metalyes) why = 'metal walls (metalyes)'
+      if (gbsabool) why = 'generalized Born solvation (gbsa)'
+      if (vp_flag) why = 'virtual particles (vp_flag)'
+      if (ethyes .and. nurey .gt. 0) why = 'Urey-Bradley terms'
+      if (ecnyes .and. ncnst .gt. 0) why = 'internal constraints'
+      if (lcent .and. icenter .gt. 0) why = 'centre-of-mass constraint'
+      if (eballyes) why = 'water sphere constraint (eball)'
+      if (nmb .gt. 0) why = 'Morse or repulsion bonds'
+      if (eteth_yes) why = 'tether energy'
+      if (ehyes) why = 'hydrophobic energy'
+      if (eenmyes) why = 'elastic network energy'
+      if (eCGyes) why = 'coarse-grained energy terms'
+      if (e_Kumbyes) why = 'umbrella constraint energy'
+      if (qssbp) why = 'solvent boundary potential (ssbp)'
+      if (torscstr) why = 'torsion constraints'
+      if (alabar) why = 'alanine barrier'
+      if (rgcst) why = 'radius of gyration constraint'
+      if (rwall) why = 'repulsive wall'
+      if (efield_yes) why = 'external electric field'
+      if (esteeryes) why = 'steering energy'
+      if (arith) why = 'arithmetic combination rule (arith)'
+      if (lesflag) why = 'LES copies'
+      if (pressON) why = 'pressure / virial (pressON)'
+      if (inofrz .ne. npt) why = 'frozen particles'
+      if (prll_on_off .and. num_pes .gt. 1) why = 'several processes'
+      if (why .ne. ' ') call alert(where, len(where),
+     &  'not supported by the GPU library: '//trim(why),
+     &  len_trim('not supported by the GPU library: '//trim(why)), 1)
+      end subroutine
+
 c     ewald_init (symm/ewald_init.f:8-242): called once after the
 c     cutoffs are squared (pot/energy.f:392-410)
       subroutine ewald_init
@@ -201,10 +261,12 @@ c     cutoffs are squared (pot/energy.f:392-410)
       include 'COMMON/SYMM.BLOCK'
       include 'COMMON/ENERGY.BLOCK'
       include 'COMMON/EWALD.BLOCK'
+      include 'COMMON/UNITS.BLOCK'
       type(mg_params) :: p
       integer :: i, ntab
       double precision :: dens
       character*4 nm
+      call mg_modes('ewald_init')
 c     monomer flags from the names MOIL's list and squeeze code test
 c     (pot/nbondm.f:81-82, symm/squeeze.f:64-79)
       if (allocated(mg_flags)) deallocate(mg_flags)
@@ -243,6 +305,9 @@ c     asks for the FP64 parity mode
       p%precision = 1
       call get_environment_variable('MOIL_GPU_FP64', nm)
       if (nm(1:1) .eq. '1') p%precision = 0
+      if (p%precision .eq. 1) write(stdo,'(1x,a)')
+     &  'GPU library: FP32 pair arithmetic with fixed-point forces'//
+     &  ' (MOIL_GPU_FP64=1 for the FP64 mode)'
       p%terms = 0
       if (ebyes) p%terms = ior(p%terms, 1)
       if (ethyes) p%terms = ior(p%terms, 2)
@@ -294,6 +359,7 @@ c     gradient into dpot
       include 'COMMON/ENERGY.BLOCK'
       include 'COMMON/EWALD.BLOCK'
       type(mg_energies) :: e
+      call mg_modes('eforce')
       call mg_check(moil_gpu_eforce(mg_ctx, coor, dpot, e), 'eforce')
       e_total = e%e_total
       e_bond = e%e_bond

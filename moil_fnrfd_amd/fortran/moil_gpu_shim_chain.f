@@ -73,6 +73,10 @@ c     pointr, fixend and debug are not used by sds's arithmetic either
       integer igrid,j,rc
       type(c_ptr) :: keep
       igrid = ndegf/(3*npt)
+c     sds squeezes each replica only when esymyes (chain/sds.f:84-86);
+c     the library always does, and the mode check stops a run without
+c     periodic images (as it stops every other mode it does not evaluate)
+      call mg_modes('sds')
       if (mg_chain_igrid .ne. igrid) then
         if (c_associated(mg_chain)) then
           rc = moil_gpu_chain_destroy(mg_chain)

@@ -45,10 +45,11 @@ c
       include 'COMMON/PARALLEL.BLOCK'
       include 'COMMON/CONVERT.BLOCK'
       include 'COMMON/EWALD.BLOCK'
+      include 'COMMON/PRESS.BLOCK'
 
       character*512 pfile,confile,crdfile,outfile
       double precision rvmx,relx,rvbg,rebg,cutm,sigma
-      integer iord,dosqz,neval,nlist,iseed,dodump,ihvdw
+      integer iord,dosqz,neval,nlist,iseed,dodump,ihvdw,imode
       integer i,k,ieval,ucon,ucrd,upar
       integer*8 c0,c1,crate,tlist,tforce
       double precision g1,g2
@@ -90,7 +91,12 @@ c
 c     optional last line: 0 switches the hydrogen rule off, as the 'hvdw'
 c     keyword of pot/energy.f:219 does (init_ef sets hvdw0 = .true.)
       ihvdw = 1
+      imode = 0
       read(upar,*,end=15) ihvdw
+c     optional line after it: a MOIL mode to switch on (for the test that
+c     the GPU shim refuses modes it does not evaluate): 1 distance-
+c     dependent dielectric, 2 pressure, 3 one frozen particle, 4 arith
+      read(upar,*,end=15) imode
  15   if (ihvdw.eq.0) hvdw0 = .false.
       close(upar)
 
@@ -111,6 +117,13 @@ c     keyword of pot/energy.f:219 does (init_ef sets hvdw0 = .true.)
          zerofrz(i) = 1
          prtc_pointer(i) = i
  10   continue
+      if (imode.eq.1) ctrue = .false.
+      if (imode.eq.2) pressON = .true.
+      if (imode.eq.3) then
+         inofrz = npt - 1
+         zerofrz(npt) = 0
+      end if
+      if (imode.eq.4) arith = .true.
 
       ucrd = 33
       k = len_trim(crdfile)

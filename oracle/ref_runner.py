@@ -44,12 +44,15 @@ def parse_dump(path: str) -> dict:
 
 def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
              nlist=1, sigma=0.0, seed=12345, dump_lists=True, workdir=None,
-             timeout=3600, exe=REF_EXE, env=None, crd_file=None) -> dict:
+             timeout=3600, exe=REF_EXE, env=None, crd_file=None, mode=0) -> dict:
     """cut: rvmx relx rvbg rebg cutm (Angstrom); ewald: dtol, nfft (3), order.
     exe: the MOIL driver to run (REF_EXE, or SHIM_EXE for MOIL calling the
     GPU library); env: extra environment variables; crd_file: a coordinate
     file (path ending in .crd) for MOIL's getcrd instead of the system's
-    raw coordinates."""
+    raw coordinates; mode: a MOIL mode the driver switches on after
+    reading the input (1 distance-dependent dielectric, 2 pressure, 3 a
+    frozen particle, 4 the arith combination rule), for the test that the
+    GPU shim refuses what the library does not evaluate."""
     if not available(exe):
         raise RuntimeError('%s is not built (make -f oracle/ref/Makefile)' % exe)
     tmp = workdir or tempfile.mkdtemp(prefix='moilref_')
@@ -77,6 +80,7 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
         f.write(dmp + '\n')
         # MOIL's hydrogen rule (pot/nbond.f:115), on unless cut['hvdw0'] is False
         f.write('%d\n' % int(cut.get('hvdw0', True)))
+        f.write('%d\n' % int(mode))
     def _big_stack():
         # MOIL keeps large work arrays on the stack (e.g. symm/nbmsym.f:36-46)
         resource.setrlimit(resource.RLIMIT_STACK,
