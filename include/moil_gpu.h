@@ -139,8 +139,18 @@ int32_t moil_gpu_finalize(moil_ctx *ctx);
 /* beta (ewaldcof), e_self, table rows (mxerftab) and density (erftbdns). */
 int32_t moil_gpu_ewald_info(moil_ctx *ctx, double *ewaldcof, double *e_self,
                             int32_t *ntab, double *erftbdns);
-/* erfc spline table as MOIL's erf_arr(4,ntab) (symm/ewald_init.f:418-449) */
+/* erfc spline table as MOIL's erf_arr(4,ntab) (symm/ewald_init.f:418-449),
+ * read back from the device copy the FP64 kernels use */
 int32_t moil_gpu_erf_table(moil_ctx *ctx, double *erf_arr, int32_t cap_rows);
+
+/* ewald_init's host arithmetic without a context (symm/ewald_init.f:53-126,
+ * 298-335, 418-574): beta from relx^2 and dtol by MOIL's bisection over
+ * Cody's erfc (symm/erfcfun.f), the table size mxerftab and density
+ * erftbdns, and (if erf_arr is not NULL) the first cap_rows rows of the
+ * erfc spline table erf_arr(4,mxerftab), all bit-identical to MOIL's.
+ * moil_gpu_init derives the same numbers.  Host only. */
+int32_t moil_ewald_setup(double cutele2, double dtol, double *ewaldcof, int32_t *ntab,
+                         double *erftbdns, double *erf_arr, int32_t cap_rows);
 
 /* squeeze (symm/squeeze.f:1-147): wraps molecules back into the box;
  * coor(3,npt) is updated in place. */

@@ -70,6 +70,17 @@ def hetero_small(seed=2000, n_chains=12, L=24.0):
     return s, cut, ew
 
 
+def hetero_small_phi0(seed=2400, n_chains=10, L=22.0):
+    """hetero_small's recipe with improper phases away from 0 and 180 deg
+    (+-U(10, 170)): MOIL's general k (phi - phi0)^2 improper branch."""
+    s = hetero_system(n_chains, L, seed=seed, frac_neutral=0.15, frac_hlike=0.15,
+                      imp_general=True)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
+    n = int(round(L)) + int(round(L)) % 2
+    ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
 def chains_only(seed=2200, n_chains=8, L=22.0, n_water=0):
     """Heteropolymer chains with no water at all (or n_water waters): the
     water lists are empty and every pair goes through MOIL's atom lists;

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "ewald_setup.h"
 #include "multi.h"
 
 static thread_local std::string g_err;
@@ -34,67 +35,6 @@ static int32_t fail(int32_t code, const std::string &msg) {
 
 // ---------------------------------------------------------------------------
 // ewald_init (symm/ewald_init.f:8-242) on the host, FP64.
-
-// find_ewaldcof, symm/ewald_init.f:298-335: double x until
-// erfc(x rc)/rc < dtol, then bisect for 2^-60 tolerance
-static double ewald_coefficient(double cutoff, double dtol) {
-  double x = 0.5, term;
-  int i = 0;
-  do {
-    x = 2.0 * x;
-    i = i + 1;
-    term = std::erfc(x * cutoff) / cutoff;
-  } while (term >= dtol);
-  int n = i + 60;
-  double xlo = 0.0, xhi = x;
-  for (i = 1; i <= n; i++) {
-    x = (xlo + xhi) / 2;
-    term = std::erfc(x * cutoff) / cutoff;
-    if (term >= dtol) xlo = x;
-    else xhi = x;
-  }
-  return x;
-}
-
-// erfc table: node values erfc(i*del) with clamped end slopes, cubic
-// spline coefficients in MOIL's erf_arr(4,i) meaning (value, first,
-// second, third derivative on interval i); same table as
-// symm/ewald_init.f:418-449.  Slopes from the tridiagonal system
-// s_{i-1} + 4 s_i + s_{i+1} = 3 (f_{i+1} - f_{i-1}) / del.
-static void erfc_spline_table(std::vector<double> &tab, int n, double del) {
-  tab.assign((size_t)4 * n, 0.0);
-  const double fac = 2.0 / std::sqrt(3.14159265358979323846);
-  for (int i = 0; i < n; i++) tab[4 * i] = std::erfc(del * i);
-  std::vector<double> s(n), cp(n), dp(n);
-  s[0] = -fac;
-  double xe = (n - 1) * del;
-  s[n - 1] = -fac * std::exp(-xe * xe);
-  for (int i = 1; i <= n - 2; i++) {
-    double rhs = 3.0 * (tab[4 * (i + 1)] - tab[4 * (i - 1)]) / del;
-    if (i == 1) rhs -= s[0];
-    if (i == n - 2) rhs -= s[n - 1];
-    if (i == 1) {
-      cp[i] = 0.25;
-      dp[i] = rhs / 4.0;
-    } else {
-      double m = 4.0 - cp[i - 1];
-      cp[i] = 1.0 / m;
-      dp[i] = (rhs - dp[i - 1]) / m;
-    }
-  }
-  if (n > 2) {
-    s[n - 2] = dp[n - 2];
-    for (int i = n - 3; i >= 1; i--) s[i] = dp[i] - cp[i] * s[i + 1];
-  }
-  for (int i = 0; i < n - 1; i++) {
-    double dd = (tab[4 * (i + 1)] - tab[4 * i]) / del;
-    double d3 = s[i] + s[i + 1] - 2.0 * dd;
-    tab[4 * i + 1] = s[i];
-    tab[4 * i + 2] = 2.0 * (dd - s[i] - d3) / del;
-    tab[4 * i + 3] = (d3 / del) * (6.0 / del);
-  }
-  tab[4 * (n - 1) + 1] = s[n - 1];
-}
 
 template <typename T>
 static int32_t upload(T **dst, const T *src, size_t n) {
@@ -303,10 +243,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   }
   // ---- Ewald set-up ----
   double cutoff = std::sqrt(p->cutele2);
-  c->beta = ewald_coefficient(cutoff, p->dtol);
+  c->beta = moil_ewald_beta(cutoff, moil_ewald_dtol(p->dtol));
   c->erftbdns = 5000.0;  // 0.5*perft, symm/ewald_init.f:103-104
   c->ntab = (int)(c->beta * c->erftbdns * cutoff * 1.5);
-  erfc_spline_table(c->h_erf, c->ntab, 1.0 / c->erftbdns);
+  moil_erfc_table(c->erftbdns, c->ntab, c->h_erf);
   {
     double pi = 4.0 * std::atan(1.0), spi = std::sqrt(1.0 / pi), s = 0;
     for (int i = 0; i < t->npt; i++) s += t->ptchg[i] * t->ptchg[i];
@@ -599,10 +539,29 @@ extern "C" int32_t moil_gpu_ewald_info(moil_ctx *c, double *ewaldcof, double *e_
   return MOIL_GPU_OK;
 }
 
+extern "C" int32_t moil_ewald_setup(double cutele2, double dtol, double *ewaldcof,
+                                    int32_t *ntab, double *erftbdns, double *erf_arr,
+                                    int32_t cap_rows) {
+  if (!(cutele2 > 0) || !(dtol > 0)) return fail(MOIL_GPU_EINVAL, "bad cutoff or dtol");
+  const double cutoff = std::sqrt(cutele2), dens = 5000.0;
+  const double beta = moil_ewald_beta(cutoff, moil_ewald_dtol(dtol));
+  const int n = (int)(beta * dens * cutoff * 1.5);
+  if (ewaldcof) *ewaldcof = beta;
+  if (ntab) *ntab = n;
+  if (erftbdns) *erftbdns = dens;
+  if (erf_arr && cap_rows > 0) {
+    std::vector<double> tab;
+    moil_erfc_table(dens, n, tab);
+    std::memcpy(erf_arr, tab.data(), sizeof(double) * 4 * (size_t)std::min(cap_rows, n));
+  }
+  return MOIL_GPU_OK;
+}
+
 extern "C" int32_t moil_gpu_erf_table(moil_ctx *c, double *erf_arr, int32_t cap_rows) {
   if (!c || !erf_arr) return fail(MOIL_GPU_EINVAL, "null argument");
   int n = std::min(cap_rows, c->ntab);
-  std::memcpy(erf_arr, c->h_erf.data(), sizeof(double) * 4 * n);
+  // the device copy the FP64 kernels read
+  if (n > 0) HIP_CHECK(hipMemcpy(erf_arr, c->d_erf, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
   return MOIL_GPU_OK;
 }
 

@@ -92,6 +92,8 @@ def load_library():
     L.moil_traj_close.argtypes = [vp]
     L.moil_traj_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp, C.POINTER(C.c_int32)]
     L.moil_force_norm.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.moil_ewald_setup.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, C.c_int32]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
@@ -102,7 +104,8 @@ def load_library():
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
                  'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
-                 'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm'):
+                 'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm',
+                 'moil_ewald_setup'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -431,6 +434,18 @@ def read_traj(path, npt, frame):
     _check(load_library().moil_traj_read(os.fsencode(path), int(npt), int(frame), x.ctypes.data,
                                          C.byref(n)))
     return x, n.value
+
+
+def ewald_setup(relx, dtol):
+    """MOIL's ewald_init host arithmetic for cutoff relx and tolerance
+    dtol: (beta, erftbdns, erf_arr as an (mxerftab, 4) array).  Host only."""
+    L = load_library()
+    b, dens, n = C.c_double(), C.c_double(), C.c_int32()
+    _check(L.moil_ewald_setup(relx * relx, dtol, C.byref(b), C.byref(n), C.byref(dens), None, 0))
+    tab = np.zeros((n.value, 4))
+    _check(L.moil_ewald_setup(relx * relx, dtol, None, None, None,
+                              tab.ctypes.data_as(C.c_void_p), n.value))
+    return b.value, dens.value, tab
 
 
 def force_norm(dpot):

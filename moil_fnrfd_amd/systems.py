@@ -417,12 +417,14 @@ def _chain_contacts_ok(xyz, hashed, L, dmin):
 
 
 def _chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types, mono_size,
-                    frac_neutral, frac_hlike):
+                    frac_neutral, frac_hlike, imp_general=False):
     """Atoms, monomers and bonded terms of one chain at coordinates xyz
     (chain number c): charges N(0,0.4) shifted to net zero (quantised
     exactly to zero per chain), LJ types, bonds/angles at their current
     geometry, torsions, impropers and the 1-2/1-3/1-4 exclusions.
-    Returns the chain's 1-based atom numbers."""
+    Improper phases are 0 or 180 deg, or with imp_general drawn from
+    +-U(10, 170) deg (MOIL's general k (phi - phi0)^2 branch,
+    pot/eimphi.f:188-249).  Returns the chain's 1-based atom numbers."""
     n = len(xyz)
     q = rng_par.normal(0.0, 0.4, n)
     neutral = rng_par.random(n) < frac_neutral
@@ -465,8 +467,12 @@ def _chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types, mono_siz
         tb.torsion(idx[a], idx[a + 1], idx[a + 2], idx[a + 3], per, kk[0], kk[1], kk[2],
                    ph[0], ph[1], ph[2])
         tb.exclude(idx[a], idx[a + 3])
-        tb.improper(idx[a], idx[a + 1], idx[a + 2], idx[a + 3],
-                    rng_par.uniform(5.0, 20.0), float(rng_par.choice([0.0, 180.0])))
+        kimp = rng_par.uniform(5.0, 20.0)
+        if imp_general:
+            ph0 = float(rng_par.uniform(10.0, 170.0) * rng_par.choice([-1.0, 1.0]))
+        else:
+            ph0 = float(rng_par.choice([0.0, 180.0]))
+        tb.improper(idx[a], idx[a + 1], idx[a + 2], idx[a + 3], kimp, ph0)
     return idx
 
 
@@ -474,7 +480,8 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
                   chain_len=(10, 20), n_types: int = 24, mono_size: int = 4,
                   v14: float = 0.5, el14: float = 0.5,
                   water_min_dist: float = 2.5, chain_min_dist: float = 3.0,
-                  frac_neutral: float = 0.0, frac_hlike: float = 0.0) -> System:
+                  frac_neutral: float = 0.0, frac_hlike: float = 0.0,
+                  imp_general: bool = False) -> System:
     """Random-walk heteropolymer chains plus 3-site water in a cubic box
     (centred at the origin).  Bonds/angles/proper torsions from chain
     topology (k_b U(300,500), k_theta U(50,100), torsion period n in
@@ -489,7 +496,8 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
     (minimum image) to another chain atom four or more bonds away.
     frac_neutral of the chain atoms get charge 0 (uncharged: MOIL's list2)
     and frac_hlike get zero LJ (MOIL's hydrogen rule, pot/nbond.f:115-126);
-    both default to 0, the plain heteropolymer recipe."""
+    both default to 0, the plain heteropolymer recipe.  imp_general draws
+    general improper phases (see _chain_topology)."""
     ss = np.random.SeedSequence(seed)
     rng_walk, rng_par, rng_wat, rng_rot = [np.random.default_rng(s) for s in ss.spawn(4)]
     types_eps = rng_par.uniform(0.02, 0.2, n_types)
@@ -515,7 +523,7 @@ def hetero_system(n_chains: int, L: float, seed: int, n_water: Optional[int] = N
         for x in xyz:
             hashed.add(x)
         idx = _chain_topology(tb, rng_par, xyz, c, types_eps, types_rmh, n_types, mono_size,
-                              frac_neutral, frac_hlike)
+                              frac_neutral, frac_hlike, imp_general)
         chain_xyz.append(xyz)
         chain_atoms.append(idx)
     # 1-4 specials (scaled) for torsion end atoms

@@ -86,71 +86,130 @@ static double *dup_d(const double *s, int64_t n) {
 /* ------------------------------------------------------------------ */
 /* ewald_init restatement                                              */
 
+/* Cody's erfc, the NETLIB SPECFUN CALERF with JINT = 1 that MOIL ships as
+ * symm/erfcfun.f:14-288 (W. J. Cody, Math. Comp. 23 (1969) 631-638):
+ * three rational approximations, on |x| <= 0.46875 (erf), 0.46875..4 and
+ * above 4 (in 1/x^2), the Gaussian factor split at a multiple of 1/16.
+ * Coefficients as published; MOIL's operation order. */
+static double cody_erfc(double x) {
+  static const double a[5] = {3.16112374387056560e00, 1.13864154151050156e02,
+                              3.77485237685302021e02, 3.20937758913846947e03,
+                              1.85777706184603153e-1};
+  static const double b[4] = {2.36012909523441209e01, 2.44024637934444173e02,
+                              1.28261652607737228e03, 2.84423683343917062e03};
+  static const double cc[9] = {5.64188496988670089e-1, 8.88314979438837594e0,
+                               6.61191906371416295e01, 2.98635138197400131e02,
+                               8.81952221241769090e02, 1.71204761263407058e03,
+                               2.05107837782607147e03, 1.23033935479799725e03,
+                               2.15311535474403846e-8};
+  static const double d[8] = {1.57449261107098347e01, 1.17693950891312499e02,
+                              5.37181101862009858e02, 1.62138957456669019e03,
+                              3.29079923573345963e03, 4.36261909014324716e03,
+                              3.43936767414372164e03, 1.23033935480374942e03};
+  static const double p[6] = {3.05326634961232344e-1, 3.60344899949804439e-1,
+                              1.25781726111229246e-1, 1.60837851487422766e-2,
+                              6.58749161529837803e-4, 1.63153871373020978e-2};
+  static const double q[5] = {2.56852019228982242e00, 1.87295284992346047e00,
+                              5.27905102951428412e-1, 6.05183413124413191e-2,
+                              2.33520497626869185e-3};
+  double y = fabs(x), res, num, den, ysq, dl;
+  int i;
+  if (y <= 0.46875) {
+    ysq = y > 1.11e-16 ? y * y : 0.0;
+    num = a[4] * ysq;
+    den = ysq;
+    for (i = 0; i < 3; i++) {
+      num = (num + a[i]) * ysq;
+      den = (den + b[i]) * ysq;
+    }
+    res = x * (num + a[3]) / (den + b[3]);
+    return 1.0 - res;
+  }
+  if (y <= 4.0) {
+    num = cc[8] * y;
+    den = y;
+    for (i = 0; i < 7; i++) {
+      num = (num + cc[i]) * y;
+      den = (den + d[i]) * y;
+    }
+    res = (num + cc[7]) / (den + d[7]);
+  } else {
+    if (y >= 26.543) return x < 0.0 ? 2.0 : 0.0;
+    ysq = 1.0 / (y * y);
+    num = p[5] * ysq;
+    den = ysq;
+    for (i = 0; i < 4; i++) {
+      num = (num + p[i]) * ysq;
+      den = (den + q[i]) * ysq;
+    }
+    res = ysq * (num + p[4]) / (den + q[4]);
+    res = (5.6418958354775628695e-1 - res) / y;
+  }
+  ysq = trunc(y * 16.0) / 16.0;
+  dl = (y - ysq) * (y + ysq);
+  res = exp(-ysq * ysq) * exp(-dl) * res;
+  return x < 0.0 ? 2.0 - res : res;
+}
+
 /* find_ewaldcof, symm/ewald_init.f:298-335: doubling then 2^-60 bisection
- * of erfc(x*rc)/rc >= dtol.  erfc is the C library's (MOIL uses Cody's
- * CALERF, symm/erfcfun.f; both are accurate to ~1 ulp). */
+ * of erfc(x*rc)/rc >= dtol with Cody's erfc; dtol first clamped as
+ * symm/ewald_init.f:78-82 does. */
 static double find_ewaldcof(double cutoff, double dtol) {
   double x = 0.5, xlo, xhi, term;
   int i = 0, n;
+  if (dtol < 1e-10) dtol = 1e-10;
+  else if (dtol > 0.1) dtol = 5.2e-05;
   do {
     x = 2.0 * x;
     i = i + 1;
-    term = erfc(x * cutoff) / cutoff;
+    term = cody_erfc(x * cutoff) / cutoff;
   } while (term >= dtol);
   n = i + 60;
   xlo = 0.0;
   xhi = x;
   for (i = 1; i <= n; i++) {
     x = (xlo + xhi) / 2;
-    term = erfc(x * cutoff) / cutoff;
+    term = cody_erfc(x * cutoff) / cutoff;
     if (term >= dtol) xlo = x; else xhi = x;
   }
   return x;
 }
 
-/* Clamped cubic spline through (i*del, f_i) with end slopes s0, sn;
- * writes MOIL's erf_arr(4,i) layout: value, slope, 2nd, 3rd derivative
- * of the interval's cubic (symm/ewald_init.f:418-449 builds the same
- * table with de Boor's cubspl, symm/ewald_init.f:453-574). */
+/* de Boor's cubspl (symm/ewald_init.f:453-574) for MOIL's table: abscissae
+ * tau_i = i*del, values in tab[4i], end slopes in tab[1] and tab[4(n-1)+1].
+ * Gaussian elimination of the slope system written with a step h_m =
+ * tau_m - tau_{m-1} and divided difference dd_m per node (MOIL keeps
+ * them in rows 3 and 4 of erf_arr, which is why its last row ends up
+ * holding h and dd), then each piece's derivatives at its left end. */
 static void clamped_spline(double *tab, int n, double del) {
-  double *s = (double *)malloc(sizeof(double) * n);
-  double *cp = (double *)malloc(sizeof(double) * n);
-  double *dp = (double *)malloc(sizeof(double) * n);
-  int i;
-  /* slopes: s_{i-1} + 4 s_i + s_{i+1} = 3 (f_{i+1}-f_{i-1})/del */
-  s[0] = tab[4 * 0 + 1];
-  s[n - 1] = tab[4 * (n - 1) + 1];
-  if (n > 2) {
-    /* Thomas algorithm on unknowns s_1..s_{n-2} */
-    for (i = 1; i <= n - 2; i++) {
-      double rhs = 3.0 * (tab[4 * (i + 1)] - tab[4 * (i - 1)]) / del;
-      if (i == 1) rhs -= s[0];
-      if (i == n - 2) rhs -= s[n - 1];
-      double b = 4.0;
-      if (i == 1) {
-        cp[i] = 1.0 / b;
-        dp[i] = rhs / b;
-      } else {
-        double m = b - cp[i - 1];
-        cp[i] = 1.0 / m;
-        dp[i] = (rhs - dp[i - 1]) / m;
-      }
-    }
-    s[n - 2] = dp[n - 2];
-    for (i = n - 3; i >= 1; i--) s[i] = dp[i] - cp[i] * s[i + 1];
+  double *h = (double *)malloc(sizeof(double) * n);   /* h[m], m >= 1 */
+  double *dd = (double *)malloc(sizeof(double) * n);  /* dd[m], then the pivots */
+  double *sl = (double *)malloc(sizeof(double) * n);  /* slopes (right sides first) */
+  int m;
+  for (m = 1; m < n; m++) {
+    h[m] = del * m - del * (m - 1);
+    dd[m] = (tab[4 * m] - tab[4 * (m - 1)]) / h[m];
   }
-  for (i = 0; i < n - 1; i++) {
-    double f0 = tab[4 * i], f1 = tab[4 * (i + 1)];
-    double dd = (f1 - f0) / del;
-    double d3 = s[i] + s[i + 1] - 2.0 * dd;
-    tab[4 * i + 1] = s[i];
-    tab[4 * i + 2] = 2.0 * (dd - s[i] - d3) / del;
-    tab[4 * i + 3] = (d3 / del) * (6.0 / del);
+  h[0] = 0.0;   /* the first equation is s_0 = given slope */
+  dd[0] = 1.0;
+  sl[0] = tab[1];
+  sl[n - 1] = tab[4 * (n - 1) + 1];
+  for (m = 1; m <= n - 2; m++) {
+    double g = -h[m + 1] / dd[m - 1];
+    sl[m] = g * sl[m - 1] + 3.0 * (h[m] * dd[m + 1] + h[m + 1] * dd[m]);
+    dd[m] = g * h[m - 1] + 2.0 * (h[m] + h[m + 1]);
   }
-  tab[4 * (n - 1) + 1] = s[n - 1];
-  tab[4 * (n - 1) + 2] = 0.0;
-  tab[4 * (n - 1) + 3] = 0.0;
-  free(s); free(cp); free(dp);
+  for (m = n - 2; m >= 0; m--) sl[m] = (sl[m] - h[m] * sl[m + 1]) / dd[m];
+  for (m = 0; m < n; m++) tab[4 * m + 1] = sl[m];
+  for (m = 1; m < n; m++) {
+    double d1 = (tab[4 * m] - tab[4 * (m - 1)]) / h[m];
+    double d3 = sl[m - 1] + sl[m] - 2.0 * d1;
+    tab[4 * (m - 1) + 2] = 2.0 * (d1 - sl[m - 1] - d3) / h[m];
+    tab[4 * (m - 1) + 3] = (d3 / h[m]) * (6.0 / h[m]);
+  }
+  tab[4 * (n - 1) + 2] = h[n - 1];
+  tab[4 * (n - 1) + 3] = (tab[4 * (n - 1)] - tab[4 * (n - 2)]) / h[n - 1];
+  free(h); free(dd); free(sl);
 }
 
 /* fill_bspline/init/one_pass/diff, pot/ewald_receip.f:1013-1080 */
@@ -254,7 +313,7 @@ orc_ctx *orc_create(const moil_topology *t, const moil_params *p, char *err, int
   fac = 2.0 / sqrt(3.14159265358979323846);
   for (i = 1; i <= c->ntab; i++) {
     double x = c->del * (i - 1);
-    c->erf_arr[4 * (i - 1)] = erfc(x);
+    c->erf_arr[4 * (i - 1)] = cody_erfc(x);
   }
   c->erf_arr[1] = -fac;
   {

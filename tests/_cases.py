@@ -15,6 +15,7 @@ CASES = {
     'hetero_small_nohvdw': (lambda: _no_hvdw(configs.hetero_small()),
                             lambda s: perturb(s, 0.05, 2001), True),
     'chains_only': (configs.chains_only, lambda s: perturb(s, 0.05, 2201), True),
+    'hetero_small_phi0': (configs.hetero_small_phi0, lambda s: perturb(s, 0.05, 2401), True),
     'chains_one_water': (lambda: configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1),
                          lambda s: perturb(s, 0.05, 2301), False),
 }

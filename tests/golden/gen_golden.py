@@ -7,7 +7,8 @@ from its seed and checked against a stored checksum) and MOIL's outputs:
 beta, all energy terms, the gradient dpot (full, or a seeded atom subset
 for the 98k-atom box), the canonical water pair set (full rows for the
 small box, counts + SHA-256 of the sorted rows otherwise), the erfc
-spline table and B-spline moduli.
+spline table (its first rows, every 97th row and the SHA-256 of the whole
+table) and B-spline moduli.
 
 Run from the repository root:  python tests/golden/gen_golden.py
 """
@@ -84,6 +85,9 @@ def write_case(name, s, cut, ew, squeeze=False, full_rows=True, dpot_subset=None
                           [int((arows[:, 2] > 0).sum())], np.int64),
         topo_sha=np.frombuffer(topo_checksum(s.topo).encode(), np.uint8),
         erftab_head=d['ERFTAB'].reshape(-1, 4)[:64],
+        erftab_sha=np.frombuffer(hashlib.sha256(np.ascontiguousarray(d['ERFTAB'], np.float64)
+                                                .tobytes()).hexdigest().encode(), np.uint8),
+        erftab_sample=d['ERFTAB'].reshape(-1, 4)[::97],
         erftab_rows=np.array([len(d['ERFTAB']) // 4], np.int64),
         bspmod1=d['BSPMOD1'],
         box=np.array(s.box, np.float64),
@@ -167,6 +171,8 @@ def main():
                squeeze=True)
     s, cut, ew = configs.chains_only()
     write_case('chains_only', perturb(s, 0.05, 2201), cut, ew, squeeze=True)
+    s, cut, ew = configs.hetero_small_phi0()
+    write_case('hetero_small_phi0', perturb(s, 0.05, 2401), cut, ew, squeeze=True)
     s, cut, ew = configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1)
     write_case('chains_one_water', perturb(s, 0.05, 2301), cut, ew)
     write_chain_case()

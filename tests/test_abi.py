@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, 'include', 'moil_gpu.h')).read()
     txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
-    return sorted(set(re.findall(r'\b(moil_gpu_\w+)\s*\(', txt)))
+    return sorted(set(re.findall(r'\b(moil_\w+)\s*\(', txt)))
 
 
 def test_library_exports_all_declared_symbols():

@@ -1,0 +1,81 @@
+"""The batched workload at its full size: 64 independent periodic systems of
+log-uniform size in [3,000, 50,000] atoms (water + 1-8 heteropolymer chains,
+~1 A PME grids) plus 32 slices of a 12,000-atom system interpolated between
+two conformations (configs.batch_workload, ~1.45M atoms in 96 systems),
+evaluated by one batched call (moil_gpu_batch_eforce_dev, graph replay).
+
+* every system's batched result equals its own moil_gpu_eforce_dev bit for
+  bit (energies and gradient), over two replays;
+* every system's mixed-precision (FP32 pairs, fixed-point forces) gradient
+  is within 1e-5 RMS-relative of the same system in FP64 mode, and its
+  e_total within 1e-6 of the sum of the magnitudes of its terms."""
+import time
+
+import numpy as np
+import pytest
+
+from moil_fnrfd_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+TERMS = ('e_bond', 'e_theta', 'e_tors', 'e_imp', 'e_vdw', 'e_el', 'e_vsym', 'e_lsym',
+         'e_el14', 'e_vdw14', 'e_corr', 'e_self', 'e_ew_receip')
+
+
+def test_full_batched_workload():
+    import torch
+    from moil_fnrfd_amd.eforce import MoilBatch, MoilGPU
+    torch.cuda.set_device(0)
+    dev = torch.device('cuda:0')
+    sh = torch.cuda.current_stream().cuda_stream
+    t0 = time.time()
+    work = configs.batch_workload()
+    print('case batch workload: %d systems, %d atoms generated in %.1f s'
+          % (len(work), sum(w[0].topo.npt for w in work), time.time() - t0), flush=True)
+    systems, coor, dpot_b, en_b, dpot_p, en_p = [], [], [], [], [], []
+    for k, (s, cut, ew) in enumerate(work):
+        m = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+        x = torch.tensor(s.coor, dtype=torch.float64, device=dev)
+        m.squeeze_dev(x.data_ptr(), sh)
+        m.nbondm_dev(x.data_ptr(), sh)
+        systems.append(m)
+        coor.append(x)
+        for lst in (dpot_b, dpot_p):
+            lst.append(torch.empty_like(x))
+        for lst in (en_b, en_p):
+            lst.append(torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev))
+    print('case batch workload: contexts and lists ready', flush=True)
+    batch = MoilBatch(systems)
+    for rep in range(2):
+        for i, m in enumerate(systems):
+            m.eforce_dev(coor[i].data_ptr(), dpot_p[i].data_ptr(), en_p[i].data_ptr(), sh)
+        for d in dpot_b:
+            d.fill_(0.0)
+        batch.eforce_dev([c.data_ptr() for c in coor], [d.data_ptr() for d in dpot_b],
+                         [e.data_ptr() for e in en_b], sh)
+        torch.cuda.synchronize()
+        for i in range(len(systems)):
+            assert torch.equal(dpot_b[i], dpot_p[i]), (rep, i)
+            assert torch.equal(en_b[i], en_p[i]), (rep, i)
+        print('case batch workload: replay %d bit-identical to %d plain calls'
+              % (rep, len(systems)), flush=True)
+    batch.close()
+    mixed_f = [d.cpu().numpy() for d in dpot_b]
+    mixed_e = [dict(zip(abi.ENERGY_NAMES, e.cpu().numpy())) for e in en_b]
+    for m in systems:
+        m.close()
+    worst_f = worst_e = 0.0
+    for k, (s, cut, ew) in enumerate(work):
+        m64 = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+        x = m64.squeeze(s.coor)
+        m64.nbondm(x)
+        e64, f64 = m64.eforce(x)
+        m64.close()
+        ef = float(np.sqrt(((mixed_f[k] - f64) ** 2).sum() / (f64 ** 2).sum()))
+        scale = sum(abs(e64[t]) for t in TERMS)
+        ee = abs(mixed_e[k]['e_total'] - e64['e_total']) / scale
+        worst_f, worst_e = max(worst_f, ef), max(worst_e, ee)
+        assert ef < 1e-5, (k, s.topo.npt, ef)
+        assert ee < 1e-6, (k, s.topo.npt, ee)
+    print('case batch workload: mixed vs FP64 on all %d systems: worst force rms-rel %.2e, '
+          'worst e_total %.2e of sum|terms|' % (len(work), worst_f, worst_e), flush=True)

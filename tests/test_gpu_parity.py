@@ -8,6 +8,7 @@ Bars (per the energy-path contract):
   * mixed mode (FP32 pairs, fixed-point forces): e_total <= 1e-6
     relative, forces <= 1e-5 RMS-relative against the FP64 run.
 """
+import hashlib
 import sys
 
 import numpy as np
@@ -61,8 +62,18 @@ def test_fp64_matches_moil(gpu, name):
     assert int(cnt[7]) == int(g['n_apairs'][4])
     e, dp = m.eforce(x)
     ref = golden_energies(g)
-    assert m.ewald_info()['beta'] == ref['ewaldcof']
+    info = m.ewald_info()
+    assert info['beta'] == ref['ewaldcof']
+    # the device's erfc table is MOIL's, byte for byte
+    assert len(info['erf_arr']) == int(g['erftab_rows'][0])
+    assert hashlib.sha256(np.ascontiguousarray(info['erf_arr']).tobytes()).hexdigest() == \
+        bytes(g['erftab_sha']).decode()
     err = energy_errors(e, ref)
+    own = abs(e['e_total'] - ref['e_total']) / abs(ref['e_total'])
+    _say('  %s e_total %.12f (MOIL %.12f): %.2e of sum|terms|, %.2e of e_total; max term %.2e; '
+         'force rms-rel %.2e' % (name, e['e_total'], ref['e_total'], err['e_total'], own,
+                                 max(err[k] for k in err if k not in ('e_total', 'e_dir')),
+                                 rms_rel(dp, g['dpot'])))
     assert max(err.values()) < 1e-12, err
     assert rms_rel(dp, g['dpot']) < 1e-10
     m.close()

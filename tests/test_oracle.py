@@ -74,7 +74,7 @@ def test_erfc_table_closed_form():
     tab, n = info['erf_arr'], info['ntab']
     dl = 1.0 / 5000.0
     x = np.arange(n) * dl
-    assert np.max(np.abs(tab[:, 0] - erfc(x)) / erfc(x)) < 1e-14  # a few ulp of two libms
+    assert np.max(np.abs(tab[:, 0] - erfc(x)) / erfc(x)) < 1e-14  # Cody's erfc vs scipy's
     # mid-interval values against scipy
     xm = (np.arange(n - 1) + 0.37) * dl
     i = (5000.0 * xm).astype(int)
@@ -84,12 +84,53 @@ def test_erfc_table_closed_form():
     der = -t[:, 1] - dx * (t[:, 2] + 0.5 * dx * t[:, 3])
     assert np.max(np.abs(val - erfc(xm))) < 1e-15
     assert np.max(np.abs(der - 2 / math.sqrt(math.pi) * np.exp(-xm ** 2))) < 1e-11
-    # the table reproduces MOIL's own (de Boor cubspl) table
-    g = load_golden('water216')
-    head = g['erftab_head']
-    assert np.max(np.abs(tab[:64, 0] - head[:, 0])) <= 2.3e-16  # glibc erfc vs CALERF: <= 1 ulp
-    assert np.max(np.abs(tab[:64, 1] - head[:, 1])) < 1e-11
-    assert int(g['erftab_rows'][0]) == n
+    assert int(load_golden('water216')['erftab_rows'][0]) == n
+
+
+def _tab_sha(tab):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(tab, np.float64).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_erfc_table_identical_to_moil(name):
+    """The whole erfc spline table -- Cody's erfc at the nodes (the
+    oracle's restatement of symm/erfcfun.f) and de Boor's clamped spline
+    (symm/ewald_init.f:418-574) -- is MOIL's table byte for byte, for the
+    oracle and for the library's host set-up (moil_ewald_setup, the
+    numbers moil_gpu_init uploads), and beta is MOIL's to the bit."""
+    from moil_fnrfd_amd import eforce
+    g = load_golden(name)
+    s, cut, ew, sqz = _case(name)
+    info = Oracle(s.topo, configs.params_for(s, cut, ew)).ewald_info()
+    beta, dens, tab = eforce.ewald_setup(cut['relx'], ew['dtol'])
+    ref = golden_energies(g)
+    for t, b in ((info['erf_arr'], info['beta']), (tab, beta)):
+        assert b == ref['ewaldcof']
+        assert len(t) == int(g['erftab_rows'][0])
+        assert np.array_equal(t[:64], g['erftab_head'])
+        assert np.array_equal(t[::97], g['erftab_sample'])
+        assert _tab_sha(t) == bytes(g['erftab_sha']).decode()
+    assert dens == ref['erftbdns']
+
+
+def test_cody_erfc_against_scipy():
+    """The library's erfc (Cody's three rational approximations, the table
+    nodes) against scipy on each of its intervals: |x| <= 0.46875 (erf),
+    0.46875..4, and above 4 (rational in 1/x^2, where the relative error of
+    exp(-x^2) grows with x^2)."""
+    from moil_fnrfd_amd import eforce
+    # dtol below MOIL's floor is raised to 1e-10 (symm/ewald_init.f:78-79):
+    # beta 0.48 at rc 9, a table reaching x = 6.5 across every interval
+    b, dens, tab = eforce.ewald_setup(9.0, 1e-300)
+    x = np.arange(len(tab)) / dens
+    assert x[-1] > 6.0
+    rel = np.abs(tab[:, 0] - erfc(x)) / erfc(x)
+    for lo, hi, bar in ((0.0, 0.46875, 1e-15), (0.46875, 4.0, 1e-14), (4.0, 7.0, 3e-14)):
+        m = (x >= lo) & (x < hi)
+        assert m.sum() > 100 and rel[m].max() < bar, (lo, hi, rel[m].max())
+    b2, _, _ = eforce.ewald_setup(9.0, 1e-10)
+    assert b2 == b
 
 
 def test_beta_closed_form():
@@ -263,6 +304,60 @@ def _four_atom(th_deg=110.0, phi_deg=60.0):
     return t, np.array([p1, p2, p3, p4])
 
 
+def _wrap(d):
+    """An angle difference folded into [-pi, pi] (pot/eimphi.f:212-214)."""
+    if d > math.pi:
+        d -= 2 * math.pi
+    if d < -math.pi:
+        d += 2 * math.pi
+    return d
+
+
+def _iupac(x):
+    b1, b2, b3 = x[1] - x[0], x[2] - x[1], x[3] - x[2]
+    nA, nB = np.cross(b1, b2), np.cross(b2, b3)
+    return math.atan2(np.dot(np.cross(nA, nB), b2 / np.linalg.norm(b2)), np.dot(nA, nB))
+
+
+@pytest.mark.parametrize('phi_deg,phi0_deg', [(60.0, 35.0), (-60.0, 35.0), (-150.0, 170.0),
+                                              (150.0, -170.0), (-20.0, -35.0)])
+def test_improper_sign_convention(phi_deg, phi0_deg):
+    """MOIL's general improper branch on geometries of either handedness:
+    E = k (phi - phi0)^2 with phi the signed IUPAC dihedral and the
+    difference wrapped into [-pi, pi] -- a mirrored geometry (phi -> -phi)
+    changes E, so a dropped sign or an |phi| fails."""
+    from moil_fnrfd_amd.systems import _TopoBuilder
+    t, x = _four_atom(phi_deg=phi_deg)
+    tb = _TopoBuilder()
+    tb.add_monomer('MOL', [('C%d' % k, 12.0, 0.0, 0.0, 0.0) for k in range(1, 5)])
+    tb.improper(1, 2, 3, 4, 10.0, phi0_deg)
+    for i in range(1, 5):
+        for j in range(i + 1, 5):
+            tb.exclude(i, j)
+    t = tb.build()
+    p = abi.make_params((30.0, 30.0, 30.0), rvmx=5, relx=5, rvbg=5, rebg=5, cutm=6, dtol=1e-5,
+                        nfft=(16, 16, 16))
+    o = Oracle(t, p)
+    o.build_lists(x)
+    e, dp = o.eforce(x)
+    phi = _iupac(x)
+    assert abs(abs(phi) - math.radians(180.0 - abs(phi_deg))) < 1e-9  # the builder's angle
+    expect = 10.0 * _wrap(phi - float(t.impeq[0])) ** 2
+    assert abs(e['e_imp'] - expect) < 1e-9 * max(1.0, expect), (e['e_imp'], expect)
+    # the mirrored geometry's energy differs unless phi0 is 0 or 180
+    xm = x.copy()
+    xm[:, 2] = -xm[:, 2]
+    em, _ = o.eforce(xm)
+    assert abs(em['e_imp'] - 10.0 * _wrap(-phi - float(t.impeq[0])) ** 2) < 1e-9 * max(1.0, expect)
+    h = 1e-6
+    for a in range(4):
+        for k in range(3):
+            xp = x.copy(); xp[a, k] += h
+            xn = x.copy(); xn[a, k] -= h
+            fd = (o.eforce(xp)[0]['e_imp'] - o.eforce(xn)[0]['e_imp']) / (2 * h)
+            assert abs(fd - dp[a, k]) < 1e-5 * max(1.0, abs(dp[a, k]))
+
+
 def test_bonded_closed_forms():
     t, x = _four_atom()
     p = abi.make_params((30.0, 30.0, 30.0), rvmx=5, relx=5, rvbg=5, rebg=5, cutm=6, dtol=1e-5,
@@ -280,9 +375,10 @@ def test_bonded_closed_forms():
     nA, nB = np.cross(b1, b2), np.cross(b2, b3)
     phi = math.atan2(np.dot(np.cross(nA, nB), b2 / np.linalg.norm(b2)), np.dot(nA, nB))
     assert abs(e['e_tors'] - (1.5 - 1.5 * math.cos(3 * phi))) < 1e-10
-    # improper: k (phi - phi0)^2 with MOIL's sign convention
-    assert abs(e['e_imp'] - 10.0 * (abs(phi) - math.radians(35.0)) ** 2) < 1e-8 or \
-        abs(e['e_imp'] - 10.0 * (phi - math.radians(35.0)) ** 2) < 1e-8
+    # improper: k (phi - phi0)^2 with phi signed as the IUPAC dihedral
+    # (MOIL's sign is that of ((a x b) x (b x c)) . b, pot/eimphi.f:188-206,
+    # the same sign); checked against the IUPAC angle itself
+    assert abs(e['e_imp'] - 10.0 * _wrap(phi - math.radians(35.0)) ** 2) < 1e-8
     # gradient by central differences
     h = 1e-6
     for a in range(4):
