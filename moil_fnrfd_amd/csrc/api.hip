@@ -505,6 +505,7 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   TRY(mal((void **)&c->d_energy, sizeof(double) * 32));
   TRY(mal((void **)&c->d_etot, sizeof(double) * EN_COUNT));
   TRY(mal((void **)&c->d_ticket, sizeof(unsigned int) * 4));
+  c->d_ovf = (int *)(c->d_ticket + 1);
   if (hipMemset(c->d_ticket, 0, sizeof(unsigned int) * 4) != hipSuccess) {
     ctx_free(c);
     return fail(MOIL_GPU_EHIP, "hipMemset failed");
@@ -642,6 +643,7 @@ extern "C" int32_t moil_gpu_build_lists_dev(moil_ctx *c, const double *d_coor, v
   for (int attempt = 0;; attempt++) {
     CHECK_RC(lists_build(c, d_coor, st));  // clears the status words first
     CHECK_RC(atoms_build(c, d_coor, st));
+    CHECK_RC(pme_bin(c, d_coor, st));      // spreading tiles, kept until the next build
     HIP_CHECK(hipMemcpyAsync(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     const bool w = lists_grow(c, status), a = atoms_grow(c, status);
@@ -673,6 +675,7 @@ int32_t build_lists_many(moil_ctx *const *cs, double *const *d_coor, int n, bool
     if (squeeze) CHECK_RC(moil_gpu_squeeze_dev(c, d_coor[k], s));
     CHECK_RC(lists_build(c, d_coor[k], s));
     CHECK_RC(atoms_build(c, d_coor[k], s));
+    CHECK_RC(pme_bin(c, d_coor[k], s));
     HIP_CHECK(hipMemcpyAsync(d_stat + (size_t)ST_COUNT * k, c->d_status,
                              sizeof(int32_t) * ST_COUNT, hipMemcpyDeviceToDevice, s));
     HIP_CHECK(hipEventRecord(done[k], s));
@@ -782,7 +785,12 @@ __device__ __forceinline__ void energies_impl(const double *__restrict__ ep, con
   e.e_total = et;
   e.e_dir = e.e_el + e.e_lsym;
   const double *src = &e.e_total;
-  for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = src[k];
+  // a force term beyond the fixed-point range made this gradient wrong:
+  // the record says so (every energy NaN), and the word is cleared
+  volatile int *ovf = (volatile int *)(ticket + 1);
+  const bool bad = *ovf != 0;
+  for (int k = 0; k < MOIL_N_ENERGIES; k++) out[k] = bad ? __builtin_nan("") : src[k];
+  if (bad) *ovf = 0;
   *ticket = 0u;
 }
 __device__ __forceinline__ void energies_body(const EnArgs &A, int bid, int nblk) {
@@ -1040,6 +1048,11 @@ extern "C" int32_t moil_gpu_eforce(moil_ctx *c, const double *coor, double *dpot
   HIP_CHECK(hipMemcpyAsync(en, c->d_energy, sizeof(en), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   if (e) std::memcpy(e, en, sizeof(en));
+  if (std::isnan(en[0]))
+    return fail(MOIL_GPU_ECAPACITY,
+                "eforce: a force term beyond the fixed-point range (" +
+                    std::to_string(FIX_TERM_UNITS / c->fix_scale) +
+                    " kcal/mol/A); energies are NaN and the gradient is not valid");
   return MOIL_GPU_OK;
 }
 

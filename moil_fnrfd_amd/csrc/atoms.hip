@@ -653,14 +653,14 @@ __device__ __forceinline__ void atoms_impl(
       for (int k = 0; k < 3; k++) {
         const double g = df * d[k];
         fj[k] -= g;
-        const long long v = F64 ? __double2ll_rn(g * P.fix_scale) : fix32((float)g);
+        const long long v = F64 ? fixd(g, P.fix_scale, P.ovf) : fixf((float)g, P.ovf);
         if (v) atomicAdd(&iacc[i * 3 + k], (unsigned long long)v);
       }
     }
     long long *jrow = jpart + (size_t)jpos[it] * 3;
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      jrow[k] = F64 ? __double2ll_rn(fj[k] * P.fix_scale) : fix32((float)fj[k]);
+      jrow[k] = F64 ? fixd(fj[k], P.fix_scale, P.ovf) : fixf((float)fj[k], P.ovf);
     if (s == 0) {
       ev_r += ev;
       ee_r += ee;
@@ -952,6 +952,7 @@ void atoms_plan(const moil_ctx *c, const double *d_coor, int part0, EvalPlan *p)
   P.erftbdns = c->erftbdns;
   P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale;
+  P.ovf = c->d_ovf;
   AtomArgs &A = p->atoms;
   A.chunks = c->d_achunk;
   A.items = c->d_aitems;

@@ -81,9 +81,6 @@ struct Tab {
 };
 
 struct Tables {
-  Tab<TbinArgs> tbin;
-  Tab<TscanArgs> tscan;
-  Tab<TscatterArgs> tscat;
   Tab<TspreadArgs> tspread;
   Tab<TsumArgs> tsum;
   Tab<ConvArgs> conv;
@@ -351,9 +348,6 @@ static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *
       p.tsum.grid = qr;
       p.conv.S = qc;
       p.gather.grid = qr;
-      T.tbin.add(p.tbin, p.n_tbin);
-      T.tscan.add(p.tscan, p.tbin.fuse_scan ? 0 : 1);
-      T.tscat.add(p.tscat, p.n_tbin);
       T.tspread.add(p.tspread, p.n_tile);
       T.tsum.add(p.tsum, p.n_tsum);
       T.conv.add(p.conv, p.n_conv);
@@ -370,14 +364,12 @@ static int32_t batch_tables(moil_batch *b, const double *const *d_coor, double *
     T.en.add(p.en, EN_COUNT);
   }
   size_t off = 0;
-  blob_layout(T.tbin, off); blob_layout(T.tscan, off); blob_layout(T.tscat, off);
   blob_layout(T.tspread, off); blob_layout(T.tsum, off); blob_layout(T.conv, off);
   blob_layout(T.gather, off); blob_layout(T.intra, off); blob_layout(T.bonded, off);
   blob_layout(T.pack, off); blob_layout(T.water, off); blob_layout(T.wred, off);
   blob_layout(T.atoms, off); blob_layout(T.ared, off); blob_layout(T.fin, off);
   blob_layout(T.en, off);
   std::vector<char> h(off, 0);
-  blob_fill(T.tbin, h); blob_fill(T.tscan, h); blob_fill(T.tscat, h);
   blob_fill(T.tspread, h); blob_fill(T.tsum, h); blob_fill(T.conv, h);
   blob_fill(T.gather, h); blob_fill(T.intra, h); blob_fill(T.bonded, h);
   blob_fill(T.pack, h); blob_fill(T.water, h); blob_fill(T.wred, h);
@@ -429,22 +421,18 @@ static int32_t batch_transforms(moil_batch *b, hipStream_t st, bool forward) {
 // and the real-space part are independent until the force assembly
 static int32_t batch_enqueue_pme(moil_batch *b, hipStream_t st) {
   const bool f = b->f64;
-  launch_k_pme_tbin_multi(TAB_PTRS(b, tbin), st);
-  launch_k_pme_tscan_multi(TAB_PTRS(b, tscan), st);
   if (f) {
-    launch_k_pme_tscatter_f64_multi(TAB_PTRS(b, tscat), st);
     launch_k_pme_tspread_f64_multi(TAB_PTRS(b, tspread), st);
     launch_k_pme_tsum_f64_multi(TAB_PTRS(b, tsum), st);
   } else {
-    launch_k_pme_tscatter_f32_multi(TAB_PTRS(b, tscat), st);
     launch_k_pme_tspread_f32_multi(TAB_PTRS(b, tspread), st);
     launch_k_pme_tsum_f32_multi(TAB_PTRS(b, tsum), st);
   }
   HIP_CHECK(hipGetLastError());
-  if (b->tab.tbin.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, true));
+  if (b->tab.tspread.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, true));
   if (f) launch_k_conv_f64_multi(TAB_PTRS(b, conv), st);
   else launch_k_conv_f32_multi(TAB_PTRS(b, conv), st);
-  if (b->tab.tbin.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, false));
+  if (b->tab.tspread.nsys() > 0) CHECK_RC_B(batch_transforms(b, st, false));
   if (f) launch_k_gather_f64_multi(TAB_PTRS(b, gather), st);
   else launch_k_gather_f32_multi(TAB_PTRS(b, gather), st);
   HIP_CHECK(hipGetLastError());

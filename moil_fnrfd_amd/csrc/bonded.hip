@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "multi.h"
+#include "pairmath.h"
 
 #define BT 256
 
@@ -35,7 +36,9 @@ struct SlotSink {
   int id[4];
   int n;
   double s;
-  __device__ SlotSink(int n_, int i0, int i1, int i2, int i3, double s_) : n(n_), s(s_) {
+  int *ovf;
+  __device__ SlotSink(int n_, int i0, int i1, int i2, int i3, double s_, int *ovf_)
+      : n(n_), s(s_), ovf(ovf_) {
     id[0] = i0; id[1] = i1; id[2] = i2; id[3] = i3;
 #pragma unroll
     for (int p = 0; p < 4; p++) acc[p][0] = acc[p][1] = acc[p][2] = 0;
@@ -45,9 +48,9 @@ struct SlotSink {
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (q == p) {
-        acc[q][0] += __double2ll_rn(fx * s);
-        acc[q][1] += __double2ll_rn(fy * s);
-        acc[q][2] += __double2ll_rn(fz * s);
+        acc[q][0] += fixd(fx, s, ovf);
+        acc[q][1] += fixd(fy, s, ovf);
+        acc[q][2] += fixd(fz, s, ovf);
       }
   }
   __device__ void flush(long long *__restrict__ out) const {
@@ -64,14 +67,15 @@ struct WaterSink {
   long long acc[9];
   int a0;
   double s;
+  int *ovf;
   __device__ void add(int atom, double fx, double fy, double fz) {
     const int l = atom - a0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
       if (k == l) {
-        acc[3 * k] += __double2ll_rn(fx * s);
-        acc[3 * k + 1] += __double2ll_rn(fy * s);
-        acc[3 * k + 2] += __double2ll_rn(fz * s);
+        acc[3 * k] += fixd(fx, s, ovf);
+        acc[3 * k + 1] += fixd(fy, s, ovf);
+        acc[3 * k + 2] += fixd(fz, s, ovf);
       }
   }
 };
@@ -141,7 +145,7 @@ __device__ inline void bond_calc(const double *coor, int i, int j, double kb, do
 __device__ inline void term_bond(const double *coor, const BSeg &S, int m, const BPar &P,
                                  long long *out, double &e) {
   const int i = S.ib[2 * m] - 1, j = S.ib[2 * m + 1] - 1;
-  SlotSink sk(2, i, j, -1, -1, P.fix_scale);
+  SlotSink sk(2, i, j, -1, -1, P.fix_scale, P.ovf);
   bond_calc(coor, i, j, S.bp[2 * m], S.bp[2 * m + 1], sk, e);
   sk.flush(out);
 }
@@ -185,7 +189,7 @@ __device__ inline void angle_calc(const double *coor, int i, int j, int k, doubl
 __device__ inline void term_angle(const double *coor, const BSeg &S, int m, const BPar &P,
                                   long long *out, double &e) {
   const int i = S.ia[3 * m] - 1, j = S.ia[3 * m + 1] - 1, k = S.ia[3 * m + 2] - 1;
-  SlotSink sk(3, i, j, k, -1, P.fix_scale);
+  SlotSink sk(3, i, j, k, -1, P.fix_scale, P.ovf);
   angle_calc(coor, i, j, k, S.ap[2 * m], S.ap[2 * m + 1], sk, e);
   sk.flush(out);
 }
@@ -199,7 +203,7 @@ __device__ inline void term_tors(const double *coor, const BSeg &S, int m, const
   e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
       p[5] * (p[2] * co * (4.0 * co * co - 3.0));
   double df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
-  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale);
+  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale, P.ovf);
   for (int q = 0; q < 4; q++) sk.add(id[q], df * g[q][0], df * g[q][1], df * g[q][2]);
   sk.flush(out);
 }
@@ -231,7 +235,7 @@ __device__ inline void term_imp(const double *coor, const BSeg &S, int m, const 
     else if ((phi > 0.0 && phi < pi / 2.0) || (phi < 0.0 && phi > -pi / 2.0)) df = df * 1e6;
     else df = -df * 1e6;
   }
-  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale);
+  SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale, P.ovf);
   for (int q = 0; q < 4; q++) sk.add(id[q], df * g[q][0], df * g[q][1], df * g[q][2]);
   sk.flush(out);
 }
@@ -258,7 +262,7 @@ __device__ inline void excl_calc(const double *coor, int i, int j, const double 
 
 __device__ inline void term_excl(const double *coor, const BSeg &S, int m, const BPar &P,
                                  long long *out, double &e) {
-  SlotSink sk(2, S.ei[m], S.ej[m], -1, -1, P.fix_scale);
+  SlotSink sk(2, S.ei[m], S.ej[m], -1, -1, P.fix_scale, P.ovf);
   excl_calc(coor, S.ei[m], S.ej[m], S.q, S.erf, P, sk, e);
   sk.flush(out);
 }
@@ -281,7 +285,7 @@ __device__ inline void term_14(const double *coor, const BSeg &S, int m, const B
     df2 = -e2 * s2;
   }
   double df = df1 + df2;
-  SlotSink sk(2, j, k, -1, -1, P.fix_scale);
+  SlotSink sk(2, j, k, -1, -1, P.fix_scale, P.ovf);
   sk.add(j, df * rx, df * ry, df * rz);
   sk.add(k, -df * rx, -df * ry, -df * rz);
   sk.flush(out);
@@ -355,6 +359,7 @@ __device__ __forceinline__ void water_intra_impl(const double *__restrict__ coor
     WaterSink sk;
     sk.a0 = a0;
     sk.s = P.fix_scale;
+    sk.ovf = P.ovf;
 #pragma unroll
     for (int k = 0; k < 9; k++) sk.acc[k] = 0;
     double et;
@@ -421,6 +426,7 @@ void bonded_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   BPar P;
   const uint32_t T = c->par.terms;
   P.fix_scale = c->fix_scale;
+  P.ovf = c->d_ovf;
   P.epstmp14 = (T & MOIL_TERM_EL) ? 1.0 : 0.0;
   P.pick = (T & MOIL_TERM_VDW) ? 1.0 : 0.0;
   P.qfac = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;

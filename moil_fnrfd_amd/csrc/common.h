@@ -24,8 +24,15 @@
 #define NBUCKET WSPLIT
 
 // Fixed-point force accumulators: int64 with FIX_SHIFT fractional bits.
-#define FIX_SHIFT_FP64 40
-#define FIX_SHIFT_MIXED 32
+// Every term converted to fixed point must stay below 2^59 units (a
+// sixteenth of the int64 range, so sums of a few such terms still fit):
+// |term| < 2^23 kcal/mol/A in FP64 mode (resolution 1.5e-11), < 2^39 in
+// mixed mode (resolution 9.5e-7, as fine as the FP32 pair arithmetic for
+// forces above ~10).  A term beyond it sets the context's overflow word,
+// and the evaluation's energies come back as NaN.
+#define FIX_SHIFT_FP64 36
+#define FIX_SHIFT_MIXED 20
+#define FIX_TERM_UNITS 576460752303423488.0  // 2^59
 
 // Energy accumulator slots (per-workgroup partial sums).
 enum {

@@ -76,9 +76,8 @@ struct moil_ctx {
   int32_t *d_tcnt = nullptr, *d_tstart = nullptr;  // [ntile+1] atoms per tile, scan
   long long *d_tbuf = nullptr;                      // [ntile][r1 r2 r3] tile regions
   int32_t *d_atile = nullptr, *d_arank = nullptr;   // [npt] tile, rank in tile
-  void *d_sw = nullptr;          // [npt][12] theta1, theta2, q*fc*theta3, tile order
-  int32_t *d_sbase = nullptr;   // [npt] packed base point, tile order
-  int32_t *d_sidx = nullptr;    // [npt] atom index, tile order
+  int32_t *d_sidx = nullptr;    // [npt] atom index, tile order (from the last list build)
+  unsigned long long *d_qover = nullptr;  // [G] fixed-point charges of atoms outside their region
   void *d_qreal = nullptr;      // real grid (float or double)
   void *d_qcplx = nullptr;      // half spectrum
   rocfft_plan plan_fwd = nullptr, plan_bwd = nullptr;
@@ -174,7 +173,8 @@ struct moil_ctx {
   int max_parts = 0;
   double *d_energy = nullptr;      // [MOIL_N_ENERGIES]
   double *d_etot = nullptr;          // [EN_COUNT] per-term totals
-  unsigned int *d_ticket = nullptr;  // last-workgroup ticket of k_energies
+  unsigned int *d_ticket = nullptr;  // last-workgroup ticket of k_energies, then the overflow word
+  int *d_ovf = nullptr;              // = d_ticket + 1: set by a fixed-point term out of range
   double *d_coor = nullptr;        // [3*npt] staging for host-pointer API
   double *d_dpot = nullptr;        // [3*npt]
   bool have_lists = false;

@@ -19,6 +19,7 @@ struct WaterRS {
   double a, b, c;
   double fix_scale;
   int ntab;
+  int *ovf;  // fixed-point overflow word of the context
 };
 struct PackArgs {
   const double *coor;
@@ -48,6 +49,7 @@ struct WReduceArgs {
   const int32_t *slot_o;
   int nwat;
   long long *facc;
+  int *ovf;
 };
 
 // ---- real space, atom lists (atoms.hip) ----
@@ -56,6 +58,7 @@ struct ARS {
   double beta, cutvdw2, cutele2, erftbdns;
   double a, b, c;
   double fix_scale;
+  int *ovf;
 };
 struct AtomArgs {
   const int32_t *chunks;
@@ -82,6 +85,7 @@ struct AReduceArgs {
 // ---- bonded terms (bonded.hip) ----
 struct BPar {
   double fix_scale;
+  int *ovf;
   double epstmp14, pick;  // ener14 switches
   double qfac;            // kofdie/eps (exclusion correction)
   double beta, erftbdns;
@@ -126,6 +130,7 @@ struct PmePar {
   double qscale, qinv;
 };
 // grid tiles: dimension k cut into nt_k tiles; LDS regions r1 r2 r3
+// (the largest tile + 3 stencil points + the margin on both sides)
 struct TilePar {
   int n1, n2, n3;
   int nt1, nt2, nt3;
@@ -147,24 +152,22 @@ struct TscanArgs {
   int n;
   int32_t *start;
 };
-struct TscatterArgs {
-  const double *coor, *q;
+struct TplaceArgs {
   int npt;
-  PmePar P;
   const int32_t *atile, *arank, *tstart;
-  void *sw;
-  int32_t *sbase, *sidx;
+  int32_t *sidx;
 };
 struct TspreadArgs {
-  const int32_t *tstart;
-  const void *sw;
-  const int32_t *sbase;
+  const double *coor, *q;
+  const int32_t *tstart, *sidx;
+  PmePar P;
   TilePar T;
-  double qscale;
   long long *tbuf;
+  unsigned long long *qover;  // grid points of atoms outside their tile's region
 };
 struct TsumArgs {
   const long long *tbuf;
+  unsigned long long *qover;  // read and cleared
   TilePar T;
   double qinv;
   void *grid;
@@ -206,7 +209,7 @@ struct EnArgs {
   const double *ep;
   EnCfg C;
   double *tot;
-  unsigned int *ticket;
+  unsigned int *ticket;  // [0] last-workgroup ticket, [1] fixed-point overflow word
   double *out;
 };
 struct AsmArgs {
@@ -221,10 +224,6 @@ struct AsmArgs {
 struct EvalPlan {
   bool f64 = false;
   bool pme = false;
-  TbinArgs tbin;
-  int n_tbin = 0;
-  TscanArgs tscan;
-  TscatterArgs tscat;
   TspreadArgs tspread;
   int n_tile = 0;
   TsumArgs tsum;
@@ -277,8 +276,7 @@ DECLARE_LAUNCH(k_water_intra, IntraArgs)
 DECLARE_LAUNCH(k_bonded, BondedArgs)
 DECLARE_LAUNCH(k_pme_tbin, TbinArgs)
 DECLARE_LAUNCH(k_pme_tscan, TscanArgs)
-DECLARE_LAUNCH(k_pme_tscatter_f32, TscatterArgs)
-DECLARE_LAUNCH(k_pme_tscatter_f64, TscatterArgs)
+DECLARE_LAUNCH(k_pme_tplace, TplaceArgs)
 DECLARE_LAUNCH(k_pme_tspread_f32, TspreadArgs)
 DECLARE_LAUNCH(k_pme_tspread_f64, TspreadArgs)
 DECLARE_LAUNCH(k_pme_tsum_f32, TsumArgs)
@@ -301,7 +299,9 @@ void launch_k_atoms_ev(const AtomArgs &a, bool f64, int nblk, hipStream_t st, hi
                        hipEvent_t e1);
 
 // PME of one system: spreading, transforms on the context's buffers,
-// convolution, gather
+// convolution, gather; pme_bin assigns the atoms to spreading tiles (at
+// every list build)
+int32_t pme_bin(moil_ctx *c, const double *d_coor, hipStream_t st);
 void pme_launch_spread(const EvalPlan &p, hipStream_t st);
 int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st);
 int pme_conv_blocks(const moil_ctx *c);

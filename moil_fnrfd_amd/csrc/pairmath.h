@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
+
 #define ERFC_P 0.418f
 __device__ __forceinline__ float erfc_poly(float t) {
   float p = 7.663338518e-02f;
@@ -21,15 +23,28 @@ __device__ __forceinline__ float erfc_poly(float t) {
   return p * t;
 }
 
-// floor(f * 2^32) as a 64-bit integer from FP32 operations only: the high
-// word is floor(f), the low word fract(f) 2^32 (f - floor(f) is exact and
-// below 1, its product with 2^32 truncates to the floor).  The two 32-bit
-// halves are the int64 directly, no shifts.  Valid for |f| < 2^31.
-__device__ __forceinline__ long long fix32(float f) {
-  const float fl = floorf(f);
+// floor(f * 2^FIX_SHIFT_MIXED) as a 64-bit integer from FP32 operations
+// only: with x = f 2^(FIX_SHIFT_MIXED - 32) (exact), the high word is
+// floor(x) and the low word fract(x) 2^32 (x - floor(x) is exact and below
+// 1; its product with 2^32 truncates to the floor), so the two 32-bit
+// halves are the int64 directly.  A term beyond the accumulators'
+// per-term range (FIX_TERM_UNITS, common.h) or a NaN sets *ovf.
+__device__ __forceinline__ long long fixf(float f, int *ovf) {
+  const float lim = (float)(FIX_TERM_UNITS / (double)(1ull << FIX_SHIFT_MIXED));
+  if (!(fabsf(f) < lim)) atomicOr(ovf, 1);
+  const float x = f * (float)(1.0 / (double)(1ull << (32 - FIX_SHIFT_MIXED)));
+  const float fl = floorf(x);
   const unsigned hi = (unsigned)(int)fl;
-  const unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
+  const unsigned lo = (unsigned)((x - fl) * 4294967296.0f);
   return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// round(v s), s = 2^FIX_SHIFT, for the FP64 conversions, with the same
+// range check
+__device__ __forceinline__ long long fixd(double v, double s, int *ovf) {
+  const double x = v * s;
+  if (!(fabs(x) < FIX_TERM_UNITS)) atomicOr(ovf, 1);
+  return __double2ll_rn(x);
 }
 
 // Each lane walks its block mask starting at its own lane number (bit b of

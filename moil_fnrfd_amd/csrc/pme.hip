@@ -12,20 +12,28 @@
 // eterm = exp(-pi^2 m^2/beta^2) / (pi V |b1 b2 b3|^2 m^2)
 // (pot/ewald_receip.f:723-812).
 //
-// GPU design: atoms are counting-sorted by grid tile (8x8x8 points; the
-// tile holding the atom's stencil base point int(fr)).  One workgroup per
-// tile scatters its atoms' 64 B-spline weighted charges (one lane per
-// stencil point) into a tile + 3-point halo region in LDS with 64-bit
-// fixed-point atomics, which are order independent (bit-reproducible);
-// a per-point pass sums the <= 8 tile regions covering each point and
-// writes the float (mixed) or double (FP64) grid for rocFFT's
-// real-to-complex transform -- the only library call.  The convolution
-// runs on the half spectrum with the influence function tabulated once in
-// FP64 at init; the reciprocal energy is summed in FP64 with
-// self-conjugate planes weighted once and all others twice.  The gather
-// runs one workgroup per tile with the tile + halo grid region in LDS,
-// recomputes the B-splines and derivatives from the coordinates, and
-// writes each atom's reciprocal gradient directly (no atomics).
+// GPU design: at every list build the atoms are counting-sorted by grid
+// tile (8x8x8 points; the tile holding the atom's stencil base point
+// int(fr)), and the assignment is kept until the next build.  Each
+// evaluation runs one workgroup per tile: its atoms' B-splines go to LDS,
+// then every atom adds its 64 weighted charges (one lane per stencil
+// point) into the tile's region in LDS -- the tile, the 3-point stencil
+// reach and a margin of PME_MARGIN base points on each side, so an atom
+// that has moved up to that far since the build still lands inside --
+// with 64-bit fixed-point atomics, which are order independent
+// (bit-reproducible).  An atom that moved further adds its stencil to a
+// fixed-point overflow grid with device atomics instead (the same
+// integers, so the sum is unchanged).  A per-point pass sums the tile
+// regions covering each point and the overflow grid and writes the float
+// (mixed) or double (FP64) grid for rocFFT's real-to-complex transform --
+// the only library call.  The convolution runs on the half spectrum with
+// the influence function tabulated once in FP64 at init; the reciprocal
+// energy is summed in FP64 with self-conjugate planes weighted once and
+// all others twice.  The gather runs one workgroup per tile with the
+// tile's region of the convolved grid in LDS, recomputes the B-splines
+// and derivatives from the coordinates (an atom outside the region reads
+// the grid itself), and writes each atom's reciprocal gradient directly
+// (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,6 +43,10 @@
 
 #define PME_THREADS 256
 #define QFIX_SHIFT 40  // grid charge fixed point: 2^-40
+// base points a tile's region accepts beyond the tile on each side, and
+// the largest region edge (tile <= 8 points + the stencil's 3 + margins)
+#define PME_MARGIN 1
+#define PME_RMAX (8 + 3 + 2 * PME_MARGIN)
 
 
 // order-4 cardinal B-spline weights and derivatives at fractional offset
@@ -78,10 +90,25 @@ __device__ __forceinline__ void frac_coord(double x, double r, int n, int *i0, d
 // tile's 3-point low-side halo reaches one neighbour only).  An atom
 // belongs to the tile holding its stencil base point int(fr); its 64
 // stencil points base-3 .. base lie in that tile or its halo.
-// (TilePar: r1 r2 r3 = LDS region dims, max tile size + 3; rsz = r1 r2 r3)
+// (TilePar: r1 r2 r3 = LDS region dims, max tile size + 3 + 2 margins;
+// rsz = r1 r2 r3)
 
 __device__ __forceinline__ int tile_of(int g, int n, int nt) { return ((g + 1) * nt - 1) / n; }
 __device__ __forceinline__ int tile_lo(int t, int n, int nt) { return (t * n) / nt; }
+
+// Region of tile [lo, hi) in one dimension: grid points lo - M - 3 ..
+// hi + M - 1 (periodic), the stencils of base points lo - M .. hi + M - 1.
+// local_base gives an atom's base point c relative to the region (the
+// region index of its first stencil point c - 3), using whichever
+// periodic image of c lies in the accepted range, or -1 if none does.
+__device__ __forceinline__ int local_base(int c, int lo, int hi, int n) {
+#pragma unroll
+  for (int k = -1; k <= 1; k++) {
+    const int b = c + k * n;
+    if (b >= lo - PME_MARGIN && b < hi + PME_MARGIN) return b - lo + PME_MARGIN;
+  }
+  return -1;
+}
 
 __device__ __forceinline__ void base_point(const double *x, const PmePar &P, int &c1, int &c2,
                                            int &c3, double &w1, double &w2, double &w3) {
@@ -221,130 +248,163 @@ __device__ __forceinline__ void tscan_body(const TscanArgs &A, int bid, int nblk
 }
 DEFINE_KERNEL(k_pme_tscan, TscanArgs, 1024, tscan_body)
 
-// counting-sort scatter; the sorted record of an atom holds its B-spline
-// weights theta1(4), theta2(4) and q sqrt(kofdie/eps) theta3(4), its base
-// point (packed 10 bits per dimension) and its atom index
-template <typename R>
-__device__ __forceinline__ void tscatter_impl(const double *__restrict__ coor,
-                                              const double *__restrict__ q, int npt,
-                                              const PmePar &P, const int32_t *__restrict__ atile,
-                                              const int32_t *__restrict__ arank,
-                                              const int32_t *__restrict__ tstart,
-                                              R *__restrict__ sw, int32_t *__restrict__ sbase,
-                                              int32_t *__restrict__ sidx, int bid) {
-  int n = bid * PME_THREADS + threadIdx.x;
-  if (n >= npt) return;
-  int c1, c2, c3;
-  double w1, w2, w3;
-  base_point(coor + 3 * n, P, c1, c2, c3, w1, w2, w3);
-  R t1[4], t2[4], t3[4], dd[4];
-  bspline4<R>((R)w1, t1, dd);
-  bspline4<R>((R)w2, t2, dd);
-  bspline4<R>((R)w3, t3, dd);
-  const R qf = (R)(q[n] * P.fc);
-  const int p = tstart[atile[n]] + arank[n];
-  R *o = sw + 12 * (size_t)p;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    o[k] = t1[k];
-    o[4 + k] = t2[k];
-    o[8 + k] = qf * t3[k];
-  }
-  sbase[p] = c1 | (c2 << 10) | (c3 << 20);
-  sidx[p] = n;
+// counting-sort placement: the atoms of every tile, contiguous in tile
+// order (sidx)
+__device__ __forceinline__ void tplace_body(const TplaceArgs &A, int bid, int nblk) {
+  const int n = bid * PME_THREADS + threadIdx.x;
+  if (n >= A.npt) return;
+  A.sidx[A.tstart[A.atile[n]] + A.arank[n]] = n;
 }
-template <typename R>
-__device__ __forceinline__ void tscatter_body(const TscatterArgs &A, int bid, int nblk) {
-  tscatter_impl<R>(A.coor, A.q, A.npt, A.P, A.atile, A.arank, A.tstart, (R *)A.sw, A.sbase,
-                   A.sidx, bid);
-}
-DEFINE_KERNEL(k_pme_tscatter_f32, TscatterArgs, PME_THREADS, tscatter_body<float>)
-DEFINE_KERNEL(k_pme_tscatter_f64, TscatterArgs, PME_THREADS, tscatter_body<double>)
+DEFINE_KERNEL(k_pme_tplace, TplaceArgs, PME_THREADS, tplace_body)
 
-// one workgroup per tile: the tile's atom records are staged in LDS
-// (coalesced loads, 256 atoms per pass), then every atom adds its 64
-// stencil weights (one lane per stencil point) into the tile + halo
-// region in LDS, in 64-bit fixed point (order independent); the region
-// is then written to the tile's slot of tbuf
+// one workgroup per tile: up to TS_STAGE of the tile's atoms at a time get
+// their B-spline weights theta1(4), theta2(4), q sqrt(kofdie/eps)
+// theta3(4) and region base in LDS (one thread per atom), then every atom
+// adds its 64 stencil weights (one lane per stencil point) into the
+// tile's region in LDS, in 64-bit fixed point (order independent); the
+// region is then written to the tile's slot of tbuf.  An atom whose base
+// point left the region's accepted range adds its 64 weights to the
+// overflow grid with device atomics (the same fixed-point integers).
 #define TS_STAGE 256
 template <typename R>
-__device__ __forceinline__ void tspread_impl(const int32_t *__restrict__ tstart,
-                                             const R *__restrict__ sw,
-                                             const int32_t *__restrict__ sbase, const TilePar &T,
-                                             double qscale, long long *__restrict__ tbuf, int bid) {
-  __shared__ unsigned long long acc[11 * 11 * 11];
+__device__ __forceinline__ void tspread_impl(const TspreadArgs &A, int bid) {
+  __shared__ unsigned long long acc[PME_RMAX * PME_RMAX * PME_RMAX];
   __shared__ R rec[TS_STAGE * 12];
   __shared__ int32_t rb[TS_STAGE];
+  const TilePar &T = A.T;
+  const PmePar &P = A.P;
   const int tile = bid, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
+  const int hi1 = tile_lo(tt1 + 1, T.n1, T.nt1), hi2 = tile_lo(tt2 + 1, T.n2, T.nt2),
+            hi3 = tile_lo(tt3 + 1, T.n3, T.nt3);
   for (int u = t; u < T.rsz; u += PME_THREADS) acc[u] = 0;
   const int lane = t & 63, wv = t >> 6;
   const int la = lane & 3, lb = (lane >> 2) & 3, lc = lane >> 4;
-  const int beg = tstart[tile], end = tstart[tile + 1];
+  const int beg = A.tstart[tile], end = A.tstart[tile + 1];
   for (int b0 = beg; b0 < end; b0 += TS_STAGE) {
     const int n = min(TS_STAGE, end - b0);
     __syncthreads();
-    for (int u = t; u < 12 * n; u += PME_THREADS) rec[u] = sw[12 * (size_t)b0 + u];
-    for (int u = t; u < n; u += PME_THREADS) rb[u] = sbase[b0 + u];
+    if (t < n) {
+      const int at = A.sidx[b0 + t];
+      int c1, c2, c3;
+      double w1, w2, w3;
+      base_point(A.coor + 3 * at, P, c1, c2, c3, w1, w2, w3);
+      R t1[4], t2[4], t3[4], dd[4];
+      bspline4<R>((R)w1, t1, dd);
+      bspline4<R>((R)w2, t2, dd);
+      bspline4<R>((R)w3, t3, dd);
+      const R qf = (R)(A.q[at] * P.fc);
+      R *o = rec + 12 * t;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        o[k] = t1[k];
+        o[4 + k] = t2[k];
+        o[8 + k] = qf * t3[k];
+      }
+      const int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
+                b3 = local_base(c3, lo3, hi3, T.n3);
+      if (b1 >= 0 && b2 >= 0 && b3 >= 0) {
+        rb[t] = b1 | (b2 << 8) | (b3 << 16);
+      } else {
+        rb[t] = -1;
+        // outside the region: the stencil points c-3..c straight into the
+        // overflow grid (rare: the atom moved beyond the margin since the
+        // list build)
+        for (int c = 0; c < 4; c++) {
+          int g3 = c3 - 3 + c;
+          g3 += g3 < 0 ? T.n3 : 0;
+          for (int bb = 0; bb < 4; bb++) {
+            int g2 = c2 - 3 + bb;
+            g2 += g2 < 0 ? T.n2 : 0;
+            for (int aa = 0; aa < 4; aa++) {
+              int g1 = c1 - 3 + aa;
+              g1 += g1 < 0 ? T.n1 : 0;
+              const long long f = __double2ll_rn((double)(o[8 + c] * o[4 + bb] * o[aa]) * P.qscale);
+              if (f)
+                atomicAdd(&A.qover[((size_t)g3 * T.n2 + g2) * T.n1 + g1], (unsigned long long)f);
+            }
+          }
+        }
+      }
+    }
     __syncthreads();
     for (int a = wv; a < n; a += PME_THREADS / 64) {
       const int bp = rb[a];
-      const int b1 = bp & 1023, b2 = (bp >> 10) & 1023, b3 = bp >> 20;
-      // stencil point base-3+l has local coordinate base - lo + l
-      const int l1 = b1 - lo1 + la, l2 = b2 - lo2 + lb, l3 = b3 - lo3 + lc;
+      if (bp < 0) continue;  // uniform over the wavefront
+      const int l1 = (bp & 255) + la, l2 = ((bp >> 8) & 255) + lb, l3 = (bp >> 16) + lc;
       const R *o = rec + 12 * a;
       const R v = o[8 + lc] * o[4 + lb] * o[la];
-      const long long f = __double2ll_rn((double)v * qscale);
+      const long long f = __double2ll_rn((double)v * P.qscale);
       if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
     }
   }
   __syncthreads();
-  long long *dst = tbuf + (size_t)tile * T.rsz;
+  long long *dst = A.tbuf + (size_t)tile * T.rsz;
   for (int u = t; u < T.rsz; u += PME_THREADS) dst[u] = (long long)acc[u];
 }
 template <typename R>
 __device__ __forceinline__ void tspread_body(const TspreadArgs &A, int bid, int nblk) {
-  tspread_impl<R>(A.tstart, (const R *)A.sw, A.sbase, A.T, A.qscale, A.tbuf, bid);
+  tspread_impl<R>(A, bid);
 }
 DEFINE_KERNEL(k_pme_tspread_f32, TspreadArgs, PME_THREADS, tspread_body<float>)
 DEFINE_KERNEL(k_pme_tspread_f64, TspreadArgs, PME_THREADS, tspread_body<double>)
 
-// per grid point: its own tile's region entry plus the halo entries of
-// the next tile in each dimension where it lies in that tile's halo
-// (at most 8 terms), converted to the transform's real type
+// per grid point: the entries of every tile region covering it (the
+// point's own tile and its neighbours in each dimension, every periodic
+// image the region holds) plus the overflow grid's entry, which is
+// cleared for the next evaluation; converted to the transform's real type
 template <typename R>
-__device__ __forceinline__ void tsum_impl(const long long *__restrict__ tbuf, const TilePar &T,
-                                          double qinv, R *__restrict__ grid, int bid) {
+__device__ __forceinline__ void tsum_impl(const TsumArgs &A, int bid) {
+  const TilePar &T = A.T;
   const size_t G = (size_t)T.n1 * T.n2 * T.n3;
   size_t g = (size_t)bid * PME_THREADS + threadIdx.x;
   if (g >= G) return;
   const int gg[3] = {(int)(g % T.n1), (int)((g / T.n1) % T.n2), (int)(g / ((size_t)T.n1 * T.n2))};
   const int nn[3] = {T.n1, T.n2, T.n3}, nt[3] = {T.nt1, T.nt2, T.nt3};
-  int tl[3][2], lc[3][2], nc[3];
+  // per dimension: (tile, region index) of every region cell holding the
+  // point (at most 4: see the region sizes)
+  int ct[3][4], cl[3][4], nc[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const int tk = tile_of(gg[d], nn[d], nt[d]);
-    const int lo = tile_lo(tk, nn[d], nt[d]), hi = tile_lo(tk + 1, nn[d], nt[d]);
-    tl[d][0] = tk;
-    lc[d][0] = gg[d] - lo + 3;
-    tl[d][1] = (tk + 1 == nt[d]) ? 0 : tk + 1;
-    lc[d][1] = gg[d] - hi + 3;
-    nc[d] = (gg[d] >= hi - 3) ? 2 : 1;
+    const int own = tile_of(gg[d], nn[d], nt[d]);
+    const int prev = own == 0 ? nt[d] - 1 : own - 1, next = own + 1 == nt[d] ? 0 : own + 1;
+    const int cand[3] = {own, prev, next};
+    const int ncand = nt[d] == 1 ? 1 : (nt[d] == 2 ? 2 : 3);
+    int m = 0;
+    for (int q = 0; q < ncand; q++) {
+      const int tk = cand[q];
+      const int lo = tile_lo(tk, nn[d], nt[d]), hi = tile_lo(tk + 1, nn[d], nt[d]);
+      const int s0 = lo - PME_MARGIN - 3, rlen = hi - lo + 3 + 2 * PME_MARGIN;
+      for (int k = -1; k <= 1; k++) {
+        const int l = gg[d] + k * nn[d] - s0;
+        if (l >= 0 && l < rlen && m < 4) {
+          ct[d][m] = tk;
+          cl[d][m] = l;
+          m++;
+        }
+      }
+    }
+    nc[d] = m;
   }
   long long sum = 0;
   for (int c = 0; c < nc[2]; c++)
     for (int b = 0; b < nc[1]; b++)
       for (int a = 0; a < nc[0]; a++) {
-        const int tile = (tl[2][c] * T.nt2 + tl[1][b]) * T.nt1 + tl[0][a];
-        sum += tbuf[(size_t)tile * T.rsz + (lc[2][c] * T.r2 + lc[1][b]) * T.r1 + lc[0][a]];
+        const int tile = (ct[2][c] * T.nt2 + ct[1][b]) * T.nt1 + ct[0][a];
+        sum += A.tbuf[(size_t)tile * T.rsz + (cl[2][c] * T.r2 + cl[1][b]) * T.r1 + cl[0][a]];
       }
-  grid[g] = (R)((double)sum * qinv);
+  const unsigned long long ov = A.qover[g];
+  if (ov) {
+    sum += (long long)ov;
+    A.qover[g] = 0;
+  }
+  ((R *)A.grid)[g] = (R)((double)sum * A.qinv);
 }
 template <typename R>
 __device__ __forceinline__ void tsum_body(const TsumArgs &A, int bid, int nblk) {
-  tsum_impl<R>(A.tbuf, A.T, A.qinv, (R *)A.grid, bid);
+  tsum_impl<R>(A, bid);
 }
 DEFINE_KERNEL(k_pme_tsum_f32, TsumArgs, PME_THREADS, tsum_body<float>)
 DEFINE_KERNEL(k_pme_tsum_f64, TsumArgs, PME_THREADS, tsum_body<double>)
@@ -383,11 +443,41 @@ __device__ __forceinline__ void conv_body(const ConvArgs &A, int bid, int nblk) 
 DEFINE_KERNEL(k_conv_f32, ConvArgs, PME_THREADS, conv_body<float>)
 DEFINE_KERNEL(k_conv_f64, ConvArgs, PME_THREADS, conv_body<double>)
 
-// one workgroup per tile: the tile + halo region of the convolved grid
-// goes to LDS, then one thread per atom of the tile (tile order)
-// recomputes its B-splines and derivatives from the coordinates and sums
-// its 64 stencil points from LDS; each atom's reciprocal gradient is
-// written directly (no atomics)
+// one workgroup per tile: the tile's region of the convolved grid goes to
+// LDS, then one thread per atom of the tile (tile order) recomputes its
+// B-splines and derivatives from the coordinates and sums its 64 stencil
+// points from LDS (or, for an atom outside the region, from the grid);
+// each atom's reciprocal gradient is written directly (no atomics)
+template <typename R>
+__device__ __forceinline__ void load_region(const TilePar &T, int lo1, int lo2, int lo3,
+                                            const R *__restrict__ grid, R *reg) {
+  for (int u = threadIdx.x; u < T.rsz; u += PME_THREADS) {
+    const int l1 = u % T.r1, l2 = (u / T.r1) % T.r2, l3 = u / (T.r1 * T.r2);
+    // region index l <-> grid point lo - M - 3 + l (periodic); entries
+    // past a smaller tile's own region are never read
+    int g1 = lo1 - PME_MARGIN - 3 + l1, g2 = lo2 - PME_MARGIN - 3 + l2,
+        g3 = lo3 - PME_MARGIN - 3 + l3;
+    g1 = g1 < 0 ? g1 + T.n1 : (g1 >= T.n1 ? g1 - T.n1 : g1);
+    g2 = g2 < 0 ? g2 + T.n2 : (g2 >= T.n2 ? g2 - T.n2 : g2);
+    g3 = g3 < 0 ? g3 + T.n3 : (g3 >= T.n3 ? g3 - T.n3 : g3);
+    reg[u] = grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
+  }
+}
+
+// the 4x4x4 stencil values of an atom: from the LDS region at local base
+// (b1, b2, b3), or (b1 < 0) from the grid around base point (c1, c2, c3)
+template <typename R>
+__device__ __forceinline__ R stencil_value(const TilePar &T, const R *reg,
+                                           const R *__restrict__ grid, int b1, int b2, int b3,
+                                           int c1, int c2, int c3, int a, int b, int c) {
+  if (b1 >= 0) return reg[((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1 + a];
+  int g1 = c1 - 3 + a, g2 = c2 - 3 + b, g3 = c3 - 3 + c;
+  g1 += g1 < 0 ? T.n1 : 0;
+  g2 += g2 < 0 ? T.n2 : 0;
+  g3 += g3 < 0 ? T.n3 : 0;
+  return grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
+}
+
 template <typename R>
 __device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
                                             const double *__restrict__ q,
@@ -395,21 +485,14 @@ __device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
                                             const int32_t *__restrict__ sidx, const PmePar &P,
                                             const TilePar &T, const R *__restrict__ grid,
                                             double *__restrict__ fpme, int bid) {
-  __shared__ R reg[11 * 11 * 11];
+  __shared__ R reg[PME_RMAX * PME_RMAX * PME_RMAX];
   const int tile = bid, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
-  for (int u = t; u < T.rsz; u += PME_THREADS) {
-    const int l1 = u % T.r1, l2 = (u / T.r1) % T.r2, l3 = u / (T.r1 * T.r2);
-    // local l <-> grid point lo - 3 + l (periodic); points past the tile's
-    // own extent are never read
-    int g1 = lo1 - 3 + l1, g2 = lo2 - 3 + l2, g3 = lo3 - 3 + l3;
-    g1 = g1 < 0 ? g1 + T.n1 : (g1 >= T.n1 ? g1 - T.n1 : g1);
-    g2 = g2 < 0 ? g2 + T.n2 : (g2 >= T.n2 ? g2 - T.n2 : g2);
-    g3 = g3 < 0 ? g3 + T.n3 : (g3 >= T.n3 ? g3 - T.n3 : g3);
-    reg[u] = grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
-  }
+  const int hi1 = tile_lo(tt1 + 1, T.n1, T.nt1), hi2 = tile_lo(tt2 + 1, T.n2, T.nt2),
+            hi3 = tile_lo(tt3 + 1, T.n3, T.nt3);
+  load_region<R>(T, lo1, lo2, lo3, grid, reg);
   __syncthreads();
   const int beg = tstart[tile], end = tstart[tile + 1];
   for (int p = beg + t; p < end; p += PME_THREADS) {
@@ -426,18 +509,18 @@ __device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
     bspline4<R>((R)w1, t1, d1);
     bspline4<R>((R)w2, t2, d2);
     bspline4<R>((R)w3, t3, d3);
-    // stencil point base-3+a has local coordinate base - lo + a
-    const int b1 = c1 - lo1, b2 = c2 - lo2, b3 = c3 - lo3;
+    int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
+        b3 = local_base(c3, lo3, hi3, T.n3);
+    if (b2 < 0 || b3 < 0) b1 = -1;
     double f1 = 0, f2 = 0, f3 = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const R *row = reg + ((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1;
         R g1 = 0, g2 = 0;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-          const R Q = row[a];
+          const R Q = stencil_value<R>(T, reg, grid, b1, b2, b3, c1, c2, c3, a, b, c);
           g1 += Q * d1[a];
           g2 += Q * t1[a];
         }
@@ -555,7 +638,7 @@ int32_t pme_init(moil_ctx *c) {
     const int n[3] = {c->nf1, c->nf2, c->nf3};
     for (int d = 0; d < 3; d++) {
       c->pt_nt[d] = (n[d] + 7) / 8;
-      c->pt_r[d] = (n[d] + c->pt_nt[d] - 1) / c->pt_nt[d] + 3;
+      c->pt_r[d] = (n[d] + c->pt_nt[d] - 1) / c->pt_nt[d] + 3 + 2 * PME_MARGIN;
     }
     c->pt_ntile = c->pt_nt[0] * c->pt_nt[1] * c->pt_nt[2];
   }
@@ -566,11 +649,11 @@ int32_t pme_init(moil_ctx *c) {
   HIP_CHECK(hipMalloc(&c->d_tbuf, sizeof(long long) * rsz * c->pt_ntile));
   HIP_CHECK(hipMalloc(&c->d_atile, sizeof(int32_t) * na));
   HIP_CHECK(hipMalloc(&c->d_arank, sizeof(int32_t) * na));
-  HIP_CHECK(hipMalloc(&c->d_sw, 12 * rs * na));
-  HIP_CHECK(hipMalloc(&c->d_sbase, sizeof(int32_t) * na));
+  HIP_CHECK(hipMalloc(&c->d_qover, sizeof(unsigned long long) * G));
+  HIP_CHECK(hipMemset(c->d_qover, 0, sizeof(unsigned long long) * G));
   HIP_CHECK(hipMalloc(&c->d_sidx, sizeof(int32_t) * na));
   {
-    // first evaluation visits the atoms in index order
+    // the first binning visits the atoms in index order
     std::vector<int32_t> iota(na);
     for (int k = 0; k < na; k++) iota[k] = k;
     HIP_CHECK(hipMemcpy(c->d_sidx, iota.data(), sizeof(int32_t) * na, hipMemcpyHostToDevice));
@@ -614,8 +697,7 @@ void pme_free(moil_ctx *c) {
   hipFree(c->d_tbuf); c->d_tbuf = nullptr;
   hipFree(c->d_atile); c->d_atile = nullptr;
   hipFree(c->d_arank); c->d_arank = nullptr;
-  hipFree(c->d_sw); c->d_sw = nullptr;
-  hipFree(c->d_sbase); c->d_sbase = nullptr;
+  hipFree(c->d_qover); c->d_qover = nullptr;
   hipFree(c->d_sidx); c->d_sidx = nullptr;
   hipFree(c->d_qreal); c->d_qreal = nullptr;
   hipFree(c->d_qcplx); c->d_qcplx = nullptr;
@@ -644,36 +726,53 @@ void pme_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   T.r1 = c->pt_r[0]; T.r2 = c->pt_r[1]; T.r3 = c->pt_r[2];
   T.rsz = T.r1 * T.r2 * T.r3;
   p->pme = true;
-  p->n_tbin = (c->npt + PME_THREADS - 1) / PME_THREADS;
-  const int fuse = c->pt_ntile <= TSCAN_FUSED ? 1 : 0;
-  p->tbin = {d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile, c->d_arank,
-             c->d_tstart, fuse};
-  p->tscan = {c->d_tcnt, c->pt_ntile, c->d_tstart};
-  p->tscat = {d_coor, c->d_ptchg, c->npt, P, c->d_atile, c->d_arank, c->d_tstart,
-              c->d_sw, c->d_sbase, c->d_sidx};
   p->n_tile = c->pt_ntile;
-  p->tspread = {c->d_tstart, c->d_sw, c->d_sbase, T, P.qscale, c->d_tbuf};
+  p->tspread = {d_coor, c->d_ptchg, c->d_tstart, c->d_sidx, P, T, c->d_tbuf, c->d_qover};
   p->n_tsum = (int)((G + PME_THREADS - 1) / PME_THREADS);
-  p->tsum = {c->d_tbuf, T, P.qinv, c->d_qreal};
+  p->tsum = {c->d_tbuf, c->d_qover, T, P.qinv, c->d_qreal};
   p->n_conv = (int)((H + PME_THREADS - 1) / PME_THREADS);
   p->conv = {c->d_qcplx, c->d_eterm, c->nc1, c->nf1, H,
              c->d_epart + (size_t)EN_RECIP * c->max_parts};
   p->gather = {d_coor, c->d_ptchg, c->d_tstart, c->d_sidx, P, T, c->d_qreal, c->d_fpme};
 }
 
-// spreading (binning, scan, sorted records, tile regions, grid sum)
+// spreading (tile regions, grid sum)
 void pme_launch_spread(const EvalPlan &p, hipStream_t st) {
-  launch_k_pme_tbin(p.tbin, p.n_tbin, st);
-  if (!p.tbin.fuse_scan) launch_k_pme_tscan(p.tscan, 1, st);
   if (p.f64) {
-    launch_k_pme_tscatter_f64(p.tscat, p.n_tbin, st);
     launch_k_pme_tspread_f64(p.tspread, p.n_tile, st);
     launch_k_pme_tsum_f64(p.tsum, p.n_tsum, st);
   } else {
-    launch_k_pme_tscatter_f32(p.tscat, p.n_tbin, st);
     launch_k_pme_tspread_f32(p.tspread, p.n_tile, st);
     launch_k_pme_tsum_f32(p.tsum, p.n_tsum, st);
   }
+}
+
+// The atoms' spreading tiles from coordinates d_coor (counting sort by the
+// tile of the stencil base point), kept until the next list build: the
+// atoms are binned in the previous sort's order, so the lanes of a
+// wavefront mostly share a tile; with at most TSCAN_FUSED tiles the last
+// binning workgroup also scans the counts.
+int32_t pme_bin(moil_ctx *c, const double *d_coor, hipStream_t st) {
+  PmePar P;
+  P.n1 = c->nf1; P.n2 = c->nf2; P.n3 = c->nf3; P.nc1 = c->nc1;
+  P.ra = 1.0 / c->par.a; P.rb = 1.0 / c->par.b; P.rc = 1.0 / c->par.c;
+  P.fc = 0.0;
+  P.qscale = P.qinv = 0.0;
+  TilePar T;
+  T.n1 = c->nf1; T.n2 = c->nf2; T.n3 = c->nf3;
+  T.nt1 = c->pt_nt[0]; T.nt2 = c->pt_nt[1]; T.nt3 = c->pt_nt[2];
+  T.r1 = c->pt_r[0]; T.r2 = c->pt_r[1]; T.r3 = c->pt_r[2];
+  T.rsz = T.r1 * T.r2 * T.r3;
+  const int nblk = (c->npt + PME_THREADS - 1) / PME_THREADS;
+  const int fuse = c->pt_ntile <= TSCAN_FUSED ? 1 : 0;
+  TbinArgs B = {d_coor, c->npt, c->d_sidx, P, T, c->d_tcnt, c->d_atile, c->d_arank,
+                c->d_tstart, fuse};
+  launch_k_pme_tbin(B, nblk, st);
+  if (!fuse) launch_k_pme_tscan(TscanArgs{c->d_tcnt, c->pt_ntile, c->d_tstart}, 1, st);
+  launch_k_pme_tplace(TplaceArgs{c->npt, c->d_atile, c->d_arank, c->d_tstart, c->d_sidx}, nblk,
+                      st);
+  HIP_CHECK(hipGetLastError());
+  return MOIL_GPU_OK;
 }
 
 int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
@@ -702,19 +801,14 @@ __global__ __launch_bounds__(PME_THREADS) void k_recip_pot(
     const double *__restrict__ coor, const int32_t *__restrict__ tstart,
     const int32_t *__restrict__ sidx, PmePar P, TilePar T, const R *__restrict__ grid,
     double *__restrict__ dq) {
-  __shared__ R reg[11 * 11 * 11];
+  __shared__ R reg[PME_RMAX * PME_RMAX * PME_RMAX];
   const int tile = blockIdx.x, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
   const int lo1 = tile_lo(tt1, T.n1, T.nt1), lo2 = tile_lo(tt2, T.n2, T.nt2),
             lo3 = tile_lo(tt3, T.n3, T.nt3);
-  for (int u = t; u < T.rsz; u += PME_THREADS) {
-    const int l1 = u % T.r1, l2 = (u / T.r1) % T.r2, l3 = u / (T.r1 * T.r2);
-    int g1 = lo1 - 3 + l1, g2 = lo2 - 3 + l2, g3 = lo3 - 3 + l3;
-    g1 = g1 < 0 ? g1 + T.n1 : (g1 >= T.n1 ? g1 - T.n1 : g1);
-    g2 = g2 < 0 ? g2 + T.n2 : (g2 >= T.n2 ? g2 - T.n2 : g2);
-    g3 = g3 < 0 ? g3 + T.n3 : (g3 >= T.n3 ? g3 - T.n3 : g3);
-    reg[u] = grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
-  }
+  const int hi1 = tile_lo(tt1 + 1, T.n1, T.nt1), hi2 = tile_lo(tt2 + 1, T.n2, T.nt2),
+            hi3 = tile_lo(tt3 + 1, T.n3, T.nt3);
+  load_region<R>(T, lo1, lo2, lo3, grid, reg);
   __syncthreads();
   const int beg = tstart[tile], end = tstart[tile + 1];
   for (int p = beg + t; p < end; p += PME_THREADS) {
@@ -726,16 +820,18 @@ __global__ __launch_bounds__(PME_THREADS) void k_recip_pot(
     bspline4<R>((R)w1, t1, dd);
     bspline4<R>((R)w2, t2, dd);
     bspline4<R>((R)w3, t3, dd);
-    const int b1 = c1 - lo1, b2 = c2 - lo2, b3 = c3 - lo3;
+    int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
+        b3 = local_base(c3, lo3, hi3, T.n3);
+    if (b2 < 0 || b3 < 0) b1 = -1;
     double s = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const R *row = reg + ((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1;
         R g = 0;
 #pragma unroll
-        for (int a = 0; a < 4; a++) g += row[a] * t1[a];
+        for (int a = 0; a < 4; a++)
+          g += stencil_value<R>(T, reg, grid, b1, b2, b3, c1, c2, c3, a, b, c) * t1[a];
         s += (double)(g * t2[b] * t3[c]);
       }
     }

@@ -27,7 +27,8 @@
 // erfc(x) = exp(-x^2) t P(t), t = 1/(1+p x): the functional form of
 // Abramowitz & Stegun 7.1.26 with a degree-8 polynomial fitted here for
 // relative accuracy 3e-8 on [0, 6].  Forces are rounded per term to
-// 64-bit fixed point with 32 fractional bits (floor) before any sum.
+// 64-bit fixed point before any sum: 32 fractional bits (floor) for the
+// partner's rows, 22 (nearest) for the block members' LDS sums.
 // MOIL_PREC_FP64: FP64 arithmetic in MOIL's operation order with MOIL's
 // cubic-spline erfc table (pot/cdie_ewald.f:152-160).
 #include <hip/hip_ext.h>
@@ -38,9 +39,10 @@
 
 #define RS_THREADS 256
 
-__device__ __forceinline__ long long fix64(double v, double scale) {
-  return __double2ll_rn(v * scale);
-}
+// the bit pattern of 1.5 2^32 (the water kernel's i-side sums, see there);
+// FIX_SHIFT_MIXED must be 20 for them
+static_assert(FIX_SHIFT_MIXED == 20, "the water kernel's LDS sums are in units of 2^-20");
+#define kBias32 0x41F8000000000000ull
 
 // ---------------------------------------------------------------------------
 // pack water positions per slot
@@ -95,9 +97,17 @@ __device__ __forceinline__ void water_f32_impl(
     const int32_t *__restrict__ jpos, const float4 *__restrict__ wpos, int nwat, const WaterRS &P,
     long long *__restrict__ jpart, long long *__restrict__ ipart, double *__restrict__ epart,
     int max_parts, int bid, int nblk) {
-  // i-force partials: 64-bit fixed point (2^-32) in LDS
+  // i-force partials in LDS, component-major (member i's component k at
+  // k * WBLOCK + i): sums of the bit patterns of (double)f + 1.5 2^32,
+  // which for |f| < 2^31 is the 64-bit integer round(f 2^20) plus the
+  // pattern of 1.5 2^32 (the fixed point of FIX_SHIFT_MIXED = 20), and
+  // each member's number of added terms, which takes the patterns'
+  // constant part out again at the end (two instructions per value in
+  // place of a float-to-fixed-point conversion).  An entry with a larger
+  // component converts its values exactly and adds the same constant.
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
+  __shared__ unsigned int icnt[WBLOCK];
   __shared__ double red[RS_THREADS];
   const int ch = order[bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
@@ -110,13 +120,12 @@ __device__ __forceinline__ void water_f32_impl(
     }
   }
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
+  if (t < WBLOCK) icnt[t] = 0;
   __syncthreads();
 
-  // x' = beta sqrt(log2 e) r: exp(-beta^2 r^2) = exp2(-x'^2), and
-  // 1 + p beta r = 1 + p' x' with p' = p / sqrt(log2 e)
-  const double sl2e = sqrt(1.4426950408889634);
-  const float betap = (float)(P.beta * sl2e);
-  const float pe = (float)(ERFC_P / sl2e);
+  // exp(-beta^2 r^2) = exp2(-beta^2 log2(e) r^2), t = 1 / (1 + p beta r)
+  const float nb2 = (float)(-P.beta * P.beta * 1.4426950408889634);
+  const float pb = (float)(ERFC_P * P.beta);
   const float qoo = (float)P.qq_oo, qoh = (float)P.qq_oh, qhh = (float)P.qq_hh;
   const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
   const float qcoo = qoo * c2b, qcoh = qoh * c2b, qchh = qhh * c2b;
@@ -174,9 +183,8 @@ __device__ __forceinline__ void water_f32_impl(
           const float qc = (sa == 0 && sb == 0) ? qcoo : ((sa == 0 || sb == 0) ? qcoh : qchh);
           const float s = __builtin_amdgcn_rsqf(r2);
           const float s2 = s * s;
-          const float xp = betap * (r2 * s);
-          const float tt = __builtin_amdgcn_rcpf(fmaf(pe, xp, 1.0f));
-          const float ex = __builtin_amdgcn_exp2f(-(xp * xp));
+          const float tt = __builtin_amdgcn_rcpf(fmaf(pb, r2 * s, 1.0f));
+          const float ex = __builtin_amdgcn_exp2f(r2 * nb2);
           const float e = qq * (ex * erfc_poly(tt) * s);
           eel += e;
           float df = -s2 * fmaf(qc, ex, e);
@@ -195,11 +203,25 @@ __device__ __forceinline__ void water_f32_impl(
           fj[3 * sb + 2] = fmaf(-df, rz, fj[3 * sb + 2]);
         }
       }
+      const float fmax = fmaxf(fmaxf(fmaxf(fabsf(fi[0]), fabsf(fi[1])), fmaxf(fabsf(fi[2]), fabsf(fi[3]))),
+                               fmaxf(fmaxf(fabsf(fi[4]), fabsf(fi[5])),
+                                     fmaxf(fmaxf(fabsf(fi[6]), fabsf(fi[7])), fabsf(fi[8]))));
+      if (fmax < 2147483648.0f) {
 #pragma unroll
-      for (int k = 0; k < 9; k++) atomicAdd(&iacc[i * 9 + k], (unsigned long long)fix32(fi[k]));
+        for (int k = 0; k < 9; k++)
+          atomicAdd(&iacc[k * WBLOCK + i],
+                    (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
+      } else {
+        // huge forces (or NaN): exact conversion, range-checked
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+          atomicAdd(&iacc[k * WBLOCK + i],
+                    (unsigned long long)fixd((double)fi[k], 1048576.0, P.ovf) + kBias32);
+      }
+      atomicAdd(&icnt[i], 1u);
     }
     // the partner's row holds the FP32 sums; the reduction converts them
-    // to fixed point (the same fix32 of the same value, half the bytes)
+    // to fixed point (the same conversion of the same value, half the bytes)
     float *jrow = reinterpret_cast<float *>(jpart) + (size_t)jpos[it] * 9;
 #pragma unroll
     for (int k = 0; k < 9; k++) jrow[k] = fj[k];
@@ -212,8 +234,12 @@ __device__ __forceinline__ void water_f32_impl(
     }
   }
   __syncthreads();
-  for (int u = t; u < WBLOCK * 9; u += RS_THREADS)
-    ipart[(size_t)ch * (WBLOCK * 9) + u] = (long long)iacc[u];
+  // the chunk's row in fixed point (member-major as the reduction reads it)
+  for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
+    const int i = u / 9, k = u - 9 * i;
+    ipart[(size_t)ch * (WBLOCK * 9) + u] =
+        (long long)(iacc[k * WBLOCK + i] - (unsigned long long)icnt[i] * kBias32);
+  }
   double v;
   v = block_sum(e_lj_r, red);
   if (t == 0) epart[EN_VDW * max_parts + ch] = v;
@@ -319,13 +345,13 @@ __device__ __forceinline__ void water_f64_impl(
       }
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        long long v = fix64(fi[k], fs);
+        long long v = fixd(fi[k], fs, P.ovf);
         if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
       }
     }
     long long *jrow = jpart + (size_t)jpos[it] * 9;
 #pragma unroll
-    for (int k = 0; k < 9; k++) jrow[k] = fix64(fj[k], fs);
+    for (int k = 0; k < 9; k++) jrow[k] = fixd(fj[k], fs, P.ovf);
     if (item.s == 0) {
       e_lj_r += elj;
       e_el_r += eel;
@@ -362,7 +388,8 @@ DEFINE_KERNEL(k_water_f64, WaterArgs, RS_THREADS, water_f64_body)
 __device__ __forceinline__ void water_reduce_impl(
     const int32_t *__restrict__ jstart, const long long *__restrict__ jpart, int jf32,
     const int32_t *__restrict__ cfirst, const long long *__restrict__ ipart,
-    const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc, int bid) {
+    const int32_t *__restrict__ slot_o, int nwat, long long *__restrict__ facc, int *ovf,
+    int bid) {
   const int g = bid * 256 + threadIdx.x;
   if (g >= 9 * nwat) return;
   const int s = g / 9, k = g - 9 * s;
@@ -372,10 +399,10 @@ __device__ __forceinline__ void water_reduce_impl(
   if (jf32) {
     const float *jp = reinterpret_cast<const float *>(jpart);
     for (; q + 1 < q1; q += 2) {
-      a0 += fix32(jp[(size_t)q * 9 + k]);
-      a1 += fix32(jp[(size_t)(q + 1) * 9 + k]);
+      a0 += fixf(jp[(size_t)q * 9 + k], ovf);
+      a1 += fixf(jp[(size_t)(q + 1) * 9 + k], ovf);
     }
-    if (q < q1) a0 += fix32(jp[(size_t)q * 9 + k]);
+    if (q < q1) a0 += fixf(jp[(size_t)q * 9 + k], ovf);
   } else {
     for (; q + 1 < q1; q += 2) {
       a0 += jpart[(size_t)q * 9 + k];
@@ -388,7 +415,8 @@ __device__ __forceinline__ void water_reduce_impl(
   facc[3 * (size_t)slot_o[s] + k] += a0 + a1;
 }
 __device__ __forceinline__ void water_reduce_body(const WReduceArgs &A, int bid, int nblk) {
-  water_reduce_impl(A.jstart, A.jpart, A.jf32, A.cfirst, A.ipart, A.slot_o, A.nwat, A.facc, bid);
+  water_reduce_impl(A.jstart, A.jpart, A.jf32, A.cfirst, A.ipart, A.slot_o, A.nwat, A.facc,
+                    A.ovf, bid);
 }
 DEFINE_KERNEL(k_water_reduce, WReduceArgs, 256, water_reduce_body)
 
@@ -409,6 +437,7 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   P.beta = c->beta; P.cutvdw2 = c->par.cutvdw2; P.cutele2 = c->par.cutele2;
   P.erftbdns = c->erftbdns; P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale; P.ntab = c->ntab;
+  P.ovf = c->d_ovf;
   WaterArgs &W = p->water;
   W.chunks = c->d_wchunk;
   W.order = c->d_worder;
@@ -432,6 +461,7 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   R.slot_o = c->d_slot_o;
   R.nwat = c->nwat;
   R.facc = c->d_facc;
+  R.ovf = c->d_ovf;
   p->n_wred = (9 * c->nwat + 255) / 256;
 }
 

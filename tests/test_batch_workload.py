@@ -1,14 +1,22 @@
 """The batched workload at its full size: 64 independent periodic systems of
 log-uniform size in [3,000, 50,000] atoms (water + 1-8 heteropolymer chains,
 ~1 A PME grids) plus 32 slices of a 12,000-atom system interpolated between
-two conformations (configs.batch_workload, ~1.45M atoms in 96 systems),
+two conformations (configs.batch_workload, ~1.39M atoms in 96 systems),
 evaluated by one batched call (moil_gpu_batch_eforce_dev, graph replay).
 
 * every system's batched result equals its own moil_gpu_eforce_dev bit for
   bit (energies and gradient), over two replays;
 * every system's mixed-precision (FP32 pairs, fixed-point forces) gradient
-  is within 1e-5 RMS-relative of the same system in FP64 mode, and its
-  e_total within 1e-6 of the sum of the magnitudes of its terms."""
+  is within 1e-5 RMS-relative of the CPU oracle's FP64 gradient on the same
+  lists, and its e_total within 1e-6 of the sum of the magnitudes of its
+  terms.  The interpolated slices overlap atoms of neighbouring molecules
+  more and more along the path: their largest gradient components grow
+  from ~5e2 to ~1e11 kcal/mol/A, inside the mixed mode's fixed-point range
+  (terms below 2^39);
+* the FP64 mode agrees with the oracle to its parity bars on a sample of
+  systems, and on the slices whose terms pass its fixed-point range
+  (2^23) it reports the overflow (MOIL_GPU_ECAPACITY, NaN energies)
+  instead of returning a wrapped gradient."""
 import time
 
 import numpy as np
@@ -40,6 +48,8 @@ def test_full_batched_workload():
         m.nbondm_dev(x.data_ptr(), sh)
         systems.append(m)
         coor.append(x)
+        if k % 8 == 0:
+            print('case batch workload: context %d npt=%d' % (k, s.topo.npt), flush=True)
         for lst in (dpot_b, dpot_p):
             lst.append(torch.empty_like(x))
         for lst in (en_b, en_p):
@@ -64,18 +74,51 @@ def test_full_batched_workload():
     mixed_e = [dict(zip(abi.ENERGY_NAMES, e.cpu().numpy())) for e in en_b]
     for m in systems:
         m.close()
+    from oracle.oracle import Oracle
     worst_f = worst_e = 0.0
+    refs = {}
     for k, (s, cut, ew) in enumerate(work):
-        m64 = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
-        x = m64.squeeze(s.coor)
-        m64.nbondm(x)
-        e64, f64 = m64.eforce(x)
-        m64.close()
+        o = Oracle(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+        x = o.squeeze(s.coor)
+        o.build_lists(x)
+        e64, f64 = o.eforce(x)
+        refs[k] = (x, e64, f64)
         ef = float(np.sqrt(((mixed_f[k] - f64) ** 2).sum() / (f64 ** 2).sum()))
         scale = sum(abs(e64[t]) for t in TERMS)
         ee = abs(mixed_e[k]['e_total'] - e64['e_total']) / scale
         worst_f, worst_e = max(worst_f, ef), max(worst_e, ee)
+        print('case batch workload system %d npt=%d max|dpot| %.2e: mixed vs FP64 oracle force '
+              'rms-rel %.2e, e_total %.2e' % (k, s.topo.npt, np.abs(f64).max(), ef, ee),
+              flush=True)
         assert ef < 1e-5, (k, s.topo.npt, ef)
         assert ee < 1e-6, (k, s.topo.npt, ee)
     print('case batch workload: mixed vs FP64 on all %d systems: worst force rms-rel %.2e, '
           'worst e_total %.2e of sum|terms|' % (len(work), worst_f, worst_e), flush=True)
+    # the FP64 mode: the oracle's result, or the overflow reported
+    from moil_fnrfd_amd.eforce import MoilGPUError
+    from tests._util import energy_errors
+    flagged = []
+    for k in list(range(0, 64, 8)) + [64, 80, 84, 86, 90, 95]:
+        s, cut, ew = work[k]
+        x, e64, f64 = refs[k]
+        m64 = MoilGPU(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+        m64.nbondm(x)
+        try:
+            e, f = m64.eforce(x)
+        except MoilGPUError as err:
+            assert 'fixed-point range' in str(err)
+            flagged.append(k)
+            print('case batch workload system %d FP64 mode: overflow reported' % k, flush=True)
+            m64.close()
+            continue
+        m64.close()
+        err = energy_errors(e, e64)
+        fr = float(np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum()))
+        print('case batch workload system %d FP64 mode vs oracle: max term %.2e, force %.2e'
+              % (k, max(err.values()), fr), flush=True)
+        assert max(err.values()) < 1e-12, (k, err)
+        assert fr < 1e-10, (k, fr)
+    # never on a physical system; always where the gradient itself passes
+    # the range
+    assert all(k >= 64 for k in flagged), flagged
+    assert 90 in flagged and 95 in flagged, flagged

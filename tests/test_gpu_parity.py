@@ -227,3 +227,44 @@ def test_hetero_lists_follow_perturbed_coordinates(gpu):
         assert np.array_equal(sort_rows(m.atom_pairs()), sort_rows(o.atom_pairs()))
         x = xg
     m.close()
+
+
+@pytest.mark.parametrize('name', ['water216', 'hetero_small'])
+def test_atoms_moved_beyond_spreading_margin(gpu, name):
+    """The reciprocal sum keeps the spreading tiles of the last list build;
+    atoms that moved further than the tiles' margin since then take the
+    exact fallback (fixed-point overflow grid for the charges, direct grid
+    reads in the gather).  With lists built at x and the energy evaluated
+    at x' (40 atoms displaced by 1.5-3 A, some across the box faces), the
+    GPU's FP64 result equals the oracle's on the same stale lists (its
+    reciprocal sum has no tiles)."""
+    from oracle.oracle import Oracle
+    _say('case %s stale spreading tiles' % name)
+    s, cut, ew, sqz = make_case(name)
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_FP64)
+    m = gpu(s.topo, p)
+    x = m.squeeze(s.coor) if sqz else s.coor.copy()
+    m.nbondm(x)
+    rng = np.random.default_rng(77)
+    xp = x.copy()
+    idx = rng.choice(len(x), 40, replace=False)
+    d = rng.normal(size=(40, 3))
+    d *= (rng.uniform(1.5, 3.0, 40) / np.linalg.norm(d, axis=1))[:, None]
+    xp[idx] += d
+    xp[idx[:5], 0] += s.box[0]  # five of them one box length away
+    e, dp = m.eforce(xp)
+    o = Oracle(s.topo, p)
+    o.build_lists(x)
+    e_ref, dp_ref = o.eforce(xp)
+    err = energy_errors(e, e_ref)
+    _say('  max term %.2e, e_ew_receip %.2e, force rms-rel %.2e'
+         % (max(err.values()), err['e_ew_receip'], rms_rel(dp, dp_ref)))
+    assert max(err.values()) < 1e-12, err
+    assert rms_rel(dp, dp_ref) < 1e-10
+    # mixed precision on the same stale tiles against the FP64 result
+    mm = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+    mm.nbondm(x)
+    em, dpm = mm.eforce(xp)
+    assert rms_rel(dpm, dp_ref) < 1e-5
+    m.close()
+    mm.close()
