@@ -274,6 +274,47 @@ int32_t moil_gpu_chain_sds_dev(moil_chain *chain, const double *d_r, const int32
                                void *stream);
 int32_t moil_gpu_chain_destroy(moil_chain *chain);
 
+/* Resident molecular dynamics: MOIL's dynamics step loop
+ * (dynamics/dyna.f:105-301) on coordinates and velocities that stay on the
+ * GPU, with the matrix SHAKE of rigid waters (dynamics/mshakpt.f,
+ * dynamics/mshakvl.f).  Every nlist steps: squeeze, list build and one
+ * force evaluation on the new lists; then per step velo <- velo dt -
+ * factor1 dpot, matrix SHAKE of that step, coor <- coor + velo,
+ * velo <- velo / dt, eforce, velo <- velo - factor2 dpot, matrix SHAKE of
+ * the velocities, with factor1 = dt^2/2 / m and factor2 = dt/2 / m
+ * (dynamics/init_dyna.f:279-288).  Velocities in A per MOIL time unit,
+ * dt in MOIL's time unit (dt[ps] / tconv, tconv = 0.0488882 ps,
+ * COMMON/CONVERT.BLOCK).  The matrix SHAKE constants are MOIL's
+ * MSHAKE.BLOCK after mshakinit (dynamics/mshakinit.f: the inverted 3x3
+ * constraint matrix, column-major as in Fortran, tolcons, the squared O-H
+ * and H-H distances and the inverse H and O masses).  No temperature
+ * control, frozen particles or bond SHAKE (the Fortran shim refuses those
+ * modes for the energy path).  A water whose matrix SHAKE does not
+ * converge where MOIL would rebuild it or stop makes the run return
+ * MOIL_GPU_ECAPACITY (checked at each list update and at the end). */
+typedef struct moil_dyna_params {
+  double dt;                 /* time step, MOIL time units                  */
+  int32_t nlist;             /* steps between list updates                  */
+  int32_t mshake;            /* nonzero: matrix SHAKE of every water (shakm) */
+  double cmat[9];            /* MSHAKE.BLOCK cmat(3,3) (inverted), column-major */
+  double tolcons, reqoh2, reqhh2, invmh, invmo;
+} moil_dyna_params;
+typedef struct moil_dyna moil_dyna;
+/* ptms: [npt] masses (MOIL's ptms).  The dynamics object uses the
+ * context's lists and evaluation. */
+int32_t moil_gpu_dyna_create(moil_ctx *ctx, const double *ptms, const moil_dyna_params *par,
+                             moil_dyna **out);
+/* nstep steps from d_coor, d_velo (device (3,npt) float64, updated in
+ * place); d_dpot gets the last gradient; d_energies (device, may be NULL)
+ * the MOIL_N_ENERGIES record of every step's evaluation, step-major.
+ * Ordered on `stream` (the list updates synchronize it). */
+int32_t moil_gpu_dyna_run_dev(moil_dyna *dyna, double *d_coor, double *d_velo, double *d_dpot,
+                              int32_t nstep, double *d_energies, void *stream);
+/* The same from host arrays (energies: [nstep][MOIL_N_ENERGIES] or NULL). */
+int32_t moil_gpu_dyna_run(moil_dyna *dyna, double *coor, double *velo, double *dpot,
+                          int32_t nstep, double *energies);
+int32_t moil_gpu_dyna_destroy(moil_dyna *dyna);
+
 /* Force-field parameters and the energy's gradient with respect to them
  * (no counterpart in MOIL; for force-field fitting and the differentiable
  * binding).  moil_gpu_set_params replaces the per-atom charges and LJ

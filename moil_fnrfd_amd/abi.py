@@ -134,3 +134,15 @@ def make_params(box, *, rvmx, relx, rvbg, rebg, cutm, dtol, nfft, order=4, eps=1
     p.precision = precision
     p.terms = terms
     return p
+
+
+class MoilDynaParams(C.Structure):
+    _fields_ = [('dt', C.c_double), ('nlist', C.c_int32), ('mshake', C.c_int32),
+                ('cmat', C.c_double * 9), ('tolcons', C.c_double), ('reqoh2', C.c_double),
+                ('reqhh2', C.c_double), ('invmh', C.c_double), ('invmo', C.c_double)]
+
+
+# MOIL's time unit in picoseconds (COMMON/CONVERT.BLOCK tconv) and kB in
+# kcal/mol/K (kboltzmann)
+TCONV = 0.04888821290839616117
+KBOLTZMANN = 1.987e-3

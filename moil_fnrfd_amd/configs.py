@@ -213,3 +213,14 @@ def chain_pick(npt, every=5, skip=3):
     p = np.ones(npt, np.int32)
     p[skip::every] = 0
     return p
+
+
+def maxwell_velocities(masses, temperature, seed):
+    """Velocities (npt, 3) in A per MOIL time unit drawn from the
+    Maxwell-Boltzmann distribution at `temperature` K: sqrt(kB T / m) N(0, 1)
+    per component (MOIL's velinit draws the same distribution,
+    dynamics/velinit.f:37)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    m = np.asarray(masses, np.float64)
+    return rng.normal(size=(len(m), 3)) * np.sqrt(abi.KBOLTZMANN * temperature / m)[:, None]

@@ -609,7 +609,7 @@ __global__ void k_squeeze(double *__restrict__ coor, const int32_t *__restrict__
 
 // the caller's stream; NULL is the default (null) stream, so device-pointer
 // calls stay ordered with the caller's own work on that stream
-static hipStream_t pick_stream(moil_ctx *c, void *s) {
+hipStream_t pick_stream(moil_ctx *c, void *s) {
   (void)c;
   return (hipStream_t)s;
 }

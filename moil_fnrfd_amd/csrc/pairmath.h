@@ -23,20 +23,20 @@ __device__ __forceinline__ float erfc_poly(float t) {
   return p * t;
 }
 
-// floor(f * 2^FIX_SHIFT_MIXED) as a 64-bit integer from FP32 operations
-// only: with x = f 2^(FIX_SHIFT_MIXED - 32) (exact), the high word is
-// floor(x) and the low word fract(x) 2^32 (x - floor(x) is exact and below
-// 1; its product with 2^32 truncates to the floor), so the two 32-bit
-// halves are the int64 directly.  A term beyond the accumulators'
-// per-term range (FIX_TERM_UNITS, common.h) or a NaN sets *ovf.
+// round(f * 2^FIX_SHIFT_MIXED) as a 64-bit integer.  For |f| < 2^31 the
+// double sum f + 1.5 2^32 has its last mantissa bit at 2^-20, so its bit
+// pattern minus that of 1.5 2^32 is the rounded fixed-point value (one
+// conversion, one add, one 64-bit subtract); larger terms take the full
+// double-to-integer conversion.  A term beyond the accumulators' per-term
+// range (FIX_TERM_UNITS, common.h) or a NaN sets *ovf.
+static_assert(FIX_SHIFT_MIXED == 20, "fixf's bias constant is that of 2^-20 units");
+#define FIX_BIAS20 0x41F8000000000000ull  // the bit pattern of 1.5 2^32
+__device__ __forceinline__ long long fixd(double v, double s, int *ovf);
 __device__ __forceinline__ long long fixf(float f, int *ovf) {
-  const float lim = (float)(FIX_TERM_UNITS / (double)(1ull << FIX_SHIFT_MIXED));
-  if (!(fabsf(f) < lim)) atomicOr(ovf, 1);
-  const float x = f * (float)(1.0 / (double)(1ull << (32 - FIX_SHIFT_MIXED)));
-  const float fl = floorf(x);
-  const unsigned hi = (unsigned)(int)fl;
-  const unsigned lo = (unsigned)((x - fl) * 4294967296.0f);
-  return (long long)(((unsigned long long)hi << 32) | lo);
+  if (fabsf(f) < 2147483648.0f)
+    return (long long)((unsigned long long)__double_as_longlong((double)f + 6442450944.0) -
+                       FIX_BIAS20);
+  return fixd((double)f, 1048576.0, ovf);
 }
 
 // round(v s), s = 2^FIX_SHIFT, for the FP64 conversions, with the same

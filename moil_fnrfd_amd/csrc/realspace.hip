@@ -39,10 +39,8 @@
 
 #define RS_THREADS 256
 
-// the bit pattern of 1.5 2^32 (the water kernel's i-side sums, see there);
-// FIX_SHIFT_MIXED must be 20 for them
-static_assert(FIX_SHIFT_MIXED == 20, "the water kernel's LDS sums are in units of 2^-20");
-#define kBias32 0x41F8000000000000ull
+// the bit pattern of 1.5 2^32 (the water kernel's i-side sums, see there)
+#define kBias32 FIX_BIAS20
 
 // ---------------------------------------------------------------------------
 // pack water positions per slot

@@ -92,6 +92,10 @@ def load_library():
     L.moil_traj_close.argtypes = [vp]
     L.moil_traj_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, vp, C.POINTER(C.c_int32)]
     L.moil_force_norm.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.moil_gpu_dyna_create.argtypes = [vp, vp, C.POINTER(abi.MoilDynaParams), C.POINTER(vp)]
+    L.moil_gpu_dyna_run_dev.argtypes = [vp, vp, vp, vp, C.c_int32, vp, vp]
+    L.moil_gpu_dyna_run.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
+    L.moil_gpu_dyna_destroy.argtypes = [vp]
     L.moil_ewald_setup.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, C.c_int32]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
@@ -105,7 +109,8 @@ def load_library():
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
                  'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
                  'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm',
-                 'moil_ewald_setup'):
+                 'moil_ewald_setup', 'moil_gpu_dyna_create', 'moil_gpu_dyna_run_dev',
+                 'moil_gpu_dyna_run', 'moil_gpu_dyna_destroy'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -351,6 +356,59 @@ class MoilChain:
     def close(self):
         if getattr(self, '_h', None):
             load_library().moil_gpu_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MoilDyna:
+    """MOIL's dynamics loop on the GPU for an engine's system
+    (moil_gpu_dyna_*): nlist steps per list update, dt in picoseconds,
+    matrix SHAKE of the waters with MOIL's mshakinit constants
+    (mshake_consts: cmat (3,3) as MOIL holds it, tolcons, reqoh2, reqhh2,
+    invmh, invmo; None = no SHAKE)."""
+
+    def __init__(self, engine, ptms, dt_ps, nlist, mshake_consts=None):
+        p = abi.MoilDynaParams()
+        p.dt = dt_ps / abi.TCONV
+        p.nlist = nlist
+        p.mshake = 0 if mshake_consts is None else 1
+        if mshake_consts is not None:
+            cm, tol, roh2, rhh2, imh, imo = mshake_consts
+            p.cmat[:] = [float(v) for v in np.asarray(cm, np.float64).reshape(3, 3).T.ravel()]
+            p.tolcons, p.reqoh2, p.reqhh2, p.invmh, p.invmo = tol, roh2, rhh2, imh, imo
+        self.npt = engine.npt
+        self._engine = engine
+        self._m = np.ascontiguousarray(ptms, np.float64)
+        h = C.c_void_p()
+        _check(load_library().moil_gpu_dyna_create(engine._h, self._m.ctypes.data, C.byref(p),
+                                                   C.byref(h)))
+        self._h = h
+
+    def run(self, coor, velo, nstep, energies=False):
+        """nstep steps from host arrays; returns (coor, velo, dpot[,
+        energies (nstep, 15)])."""
+        x = _f64(coor, self.npt).copy()
+        v = _f64(velo, self.npt).copy()
+        dp = np.zeros(3 * self.npt)
+        en = np.zeros((max(nstep, 1), abi.N_ENERGIES)) if energies else None
+        _check(load_library().moil_gpu_dyna_run(self._h, x.ctypes.data, v.ctypes.data,
+                                                dp.ctypes.data, nstep,
+                                                en.ctypes.data if energies else None))
+        out = (x.reshape(-1, 3), v.reshape(-1, 3), dp.reshape(-1, 3))
+        return out + (en[:nstep],) if energies else out
+
+    def run_dev(self, d_coor, d_velo, d_dpot, nstep, d_energies=0, stream=None):
+        _check(load_library().moil_gpu_dyna_run_dev(self._h, d_coor, d_velo, d_dpot, nstep,
+                                                    d_energies or None, stream))
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load_library().moil_gpu_dyna_destroy(self._h)
             self._h = None
 
     def __del__(self):

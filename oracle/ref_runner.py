@@ -160,3 +160,49 @@ def run_moil_sds(sysm: System, cut: dict, ewald: dict, r, ipick, gamma, rho, lam
         if line.startswith('MOILTIME'):
             d['sds_s'] = float(line.split('=')[1])
     return d
+
+
+DYNA_EXE = os.path.join(HERE, '_ref', 'moil_dyna')
+
+
+def run_moil_dyna(sysm: System, cut: dict, ewald: dict, velo, nstep, nlist, dt_ps, *,
+                  mshake=True, tolcons=1e-7, workdir=None, timeout=3600) -> dict:
+    """MOIL's dynamics step loop (dynamics/dyna.f:105-301 on MOIL's eforce and
+    matrix SHAKE) via oracle/_ref/moil_dyna from sysm's coordinates and the
+    velocities velo (npt, 3) in A per MOIL time unit, for nstep steps of
+    dt_ps picoseconds with the lists updated every nlist steps.  Returns
+    the parsed dump: COOR, VELO, DPOT (final), ETOT (e_total after each
+    step) and MSHAKE (cmat(3,3) column-major, tolcons, reqoh2, reqhh2,
+    invmh, invmo as mshakinit set them)."""
+    if not available(DYNA_EXE):
+        raise RuntimeError('%s is not built (make -f oracle/ref/Makefile dyna)' % DYNA_EXE)
+    tmp = workdir or tempfile.mkdtemp(prefix='moildyn_')
+    os.makedirs(tmp, exist_ok=True)
+    con, crd, vel = (os.path.join(tmp, n) for n in ('sys.con', 'sys.bin', 'vel.bin'))
+    par, dmp = os.path.join(tmp, 'dyna.par'), os.path.join(tmp, 'dyna.dump')
+    write_con(con, sysm.topo)
+    np.ascontiguousarray(sysm.coor, np.float64).tofile(crd)
+    np.ascontiguousarray(velo, np.float64).tofile(vel)
+    a, b, c = sysm.box
+    n1, n2, n3 = ewald['nfft']
+    with open(par, 'w') as f:
+        f.write(con + '\n' + crd + '\n')
+        f.write('%.17g %.17g %.17g\n' % (a, b, c))
+        f.write('%.17g %.17g %.17g %.17g %.17g\n' % (cut['rvmx'], cut['relx'], cut['rvbg'],
+                                                    cut['rebg'], cut['cutm']))
+        f.write('%.17g %d %d %d %d\n' % (ewald['dtol'], n1, n2, n3, ewald.get('order', 4)))
+        f.write('%.17g\n' % cut.get('eps', 1.0))
+        f.write('%d %d %.17g %d %.17g\n' % (nstep, nlist, dt_ps, int(mshake), tolcons))
+        f.write(vel + '\n' + dmp + '\n')
+
+    def _big_stack():
+        resource.setrlimit(resource.RLIMIT_STACK,
+                           (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    r = subprocess.run([DYNA_EXE, par], cwd=tmp, capture_output=True, text=True,
+                       timeout=timeout, preexec_fn=_big_stack)
+    if r.returncode != 0 or not os.path.exists(dmp):
+        raise RuntimeError('moil_dyna failed (%d):\n%s\n%s' % (r.returncode, r.stdout[-3000:],
+                                                                r.stderr[-3000:]))
+    d = parse_dump(dmp)
+    d['stdout'] = r.stdout
+    return d
