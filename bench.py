@@ -3,10 +3,15 @@ PME water box (32,768 rigid 3-site waters, 68.9 A box, 10 A cutoffs with a
 1.5 A list skin, 72^3 grid, beta 0.31), FP32 pair arithmetic with 64-bit
 fixed-point forces and FP64 energies, on one MI355X.
 
-One step = one eforce call on device-resident coordinates; between steps
-the coordinates get N(0, 0.01 A) noise, and every 20 steps the list
-pipeline (squeeze + nbondm + syminit) is rebuilt on the GPU -- both inside
-the timed region.  Then the batched leg: 96 systems evaluated by one
+One step = one eforce call on device-resident coordinates: the
+configuration of the last list build displaced by fresh N(0, 0.01 A) noise
+(drawn on the device), and every 20 steps the list pipeline (squeeze +
+nbondm + syminit) is rebuilt on the GPU on such a configuration, which the
+next 20 steps then displace -- both inside the timed region.  The
+displacements accumulate only once per list build, so atoms do not drift
+into contact over a long run.  An
+evaluation whose forces left the fixed-point range (NaN energies) stops
+the benchmark.  Then the batched leg: 96 systems evaluated by one
 batched call against one call per system (speedup and force error).
 Prints one JSON line.
 
@@ -265,9 +270,10 @@ def chain_leg(steps=5, igrid=32):
 
 def timed_loop(sysm, cut, ew, steps, warmup, seed):
     """The metric's loop on one system (mixed precision, coordinates and
-    outputs resident in HBM): per step N(0, 0.01 A) noise added to the
-    coordinates (drawn on the device NLIST steps at a time), every NLIST-th
-    step a list rebuild (squeeze + build_lists), then eforce.  Times `steps`
+    outputs resident in HBM): per step fresh N(0, 0.01 A) noise added to the
+    configuration of the last list build (drawn on the device NLIST steps
+    at a time), every NLIST-th step a list rebuild (squeeze + build_lists)
+    on such a configuration, then eforce.  Times `steps`
     steps after `warmup` untimed ones with the real-space kernel's own
     dispatch events (no packets between kernels) on every 10th evaluation,
     then a short untimed pass with events between the stages for the
@@ -283,7 +289,8 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed):
     dev = torch.device('cuda:0')
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-    coor = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
+    base = torch.tensor(sysm.coor, dtype=torch.float64, device=dev).contiguous()
+    coor = base.clone()
     dpot = torch.empty_like(coor)
     energ = torch.zeros(abi.N_ENERGIES, dtype=torch.float64, device=dev)
     gen = torch.Generator(device=dev)
@@ -295,7 +302,7 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed):
         if k % NLIST == 0:
             torch.randn(noise.shape, generator=gen, device=dev, dtype=torch.float64, out=noise)
         if k > 0:
-            coor.add_(noise[k % NLIST], alpha=SIGMA)
+            torch.add(base, noise[k % NLIST], alpha=SIGMA, out=coor)
         if k % NLIST == 0:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
@@ -303,6 +310,7 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed):
             eng.squeeze_dev(coor.data_ptr(), sh)
             eng.nbondm_dev(coor.data_ptr(), sh)
             e1.record(stream)
+            base.copy_(coor)
             ev_list.append((e0, e1))
         eng.eforce_dev(coor.data_ptr(), dpot.data_ptr(), energ.data_ptr(), sh)
 
@@ -333,6 +341,8 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed):
     counts = eng.list_counts()
     e_last = float(energ.cpu().numpy()[0])
     eng.close()
+    if not np.isfinite(e_last):
+        raise RuntimeError('the last evaluation reported a fixed-point overflow (NaN energies)')
     return {
         'elapsed': t1 - t0,
         'n_pairs': n_pairs,

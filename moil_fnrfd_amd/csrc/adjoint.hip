@@ -40,6 +40,7 @@ struct AdjWater {
   double qfac;              // kofdie/eps (0 without electrostatics)
   double e12O, e6O, pick;   // LJ factors of the last water's O, 1 or 0
   double beta, cutvdw2, cutele2, erftbdns;
+  int ntab;
   double a, b, c;
 };
 
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_water(
             const double r2 = rx * rx + ry * ry + rz * rz;
             const double rij = sqrt(r2);
             double erfcc, derfc;
-            erfc_table(erf_arr, P.erftbdns, P.beta * rij, erfcc, derfc);
+            erfc_table(erf_arr, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
             const double phi = P.qfac * erfcc / rij;
             pi[sa] += qs[sb] * phi;
             pj[sb] += qs[sa] * phi;
@@ -213,6 +214,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_water(
 struct AdjAtom {
   double qfac, pick;
   double beta, cutvdw2, cutele2, erftbdns;
+  int ntab;
   double a, b, c;
 };
 
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_atoms(
         if (el) {
           const double rij = sqrt(r2);
           double erfcc, derfc;
-          erfc_table(erf_arr, P.erftbdns, P.beta * rij, erfcc, derfc);
+          erfc_table(erf_arr, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
           phi = erfcc / rij;
         }
         if (lj) {
@@ -353,6 +355,7 @@ struct AdjTerms {
   const double *q, *e12, *e6, *erf;
   double qfac_excl;       // kofdie/eps with electrostatics on, else 0
   double beta, erftbdns;
+  int ntab;
   double v14, k14;        // 1-4 scalings: v14, el14 eps kofdie
   double pick, epst14;    // LJ on (1/0), electrostatics on (1/0)
   int do_v14, do_el14;
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_terms(const double *__restr
                  rz = coor[3 * i + 2] - coor[3 * j + 2];
     const double rij = sqrt(rx * rx + ry * ry + rz * rz);
     double erfcc, derfc;
-    erfc_table(P.erf, P.erftbdns, P.beta * rij, erfcc, derfc);
+    erfc_table(P.erf, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
     const double phi = -P.qfac_excl * (1.0 - erfcc) / rij;
     atomicAdd(&dq[i], phi * P.q[j]);
     atomicAdd(&dq[j], phi * P.q[i]);
@@ -447,6 +450,7 @@ int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_d
       W.cutvdw2 = c->par.cutvdw2;
       W.cutele2 = c->par.cutele2;
       W.erftbdns = c->erftbdns;
+      W.ntab = c->ntab;
       W.a = c->par.a; W.b = c->par.b; W.c = c->par.c;
       if (f64)
         k_adj_water<true><<<p.n_water, ADJ_THREADS, 0, st>>>(
@@ -465,6 +469,7 @@ int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_d
       A.cutvdw2 = c->par.cutvdw2;
       A.cutele2 = c->par.cutele2;
       A.erftbdns = c->erftbdns;
+      A.ntab = c->ntab;
       A.a = c->par.a; A.b = c->par.b; A.c = c->par.c;
       if (f64)
         k_adj_atoms<true><<<c->nachunk, ADJ_THREADS, 0, st>>>(
@@ -489,6 +494,7 @@ int32_t adjoint_run(moil_ctx *c, const double *d_coor, double *d_dq, double *d_d
   P.qfac_excl = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;
   P.beta = c->beta;
   P.erftbdns = c->erftbdns;
+  P.ntab = c->ntab;
   P.v14 = c->v14;
   P.k14 = c->el14 * c->par.eps * KOFDIE;
   P.pick = (T & MOIL_TERM_VDW) ? 1.0 : 0.0;

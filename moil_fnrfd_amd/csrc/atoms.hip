@@ -557,10 +557,10 @@ __device__ __forceinline__ void atoms_impl(
   for (int u = t; u < WBLOCK * 3; u += ABT) iacc[u] = 0;
   __syncthreads();
   const int rot = t & 63;
+  bool bad = false;  // a term beyond the fixed-point range (or a NaN)
   const float betaf = (float)P.beta, c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
   const float cutvdw2f = (float)P.cutvdw2, cutele2f = (float)P.cutele2;
   const float log2e = 1.4426950408889634f;
-  const double del = 1.0 / P.erftbdns, ovt = 1.0 / 3.0;
   double ev_r = 0, ee_r = 0, ev_i = 0, ee_i = 0;
   for (int it = beg + t; it < end; it += ABT) {
     const AItem item = items[it];
@@ -606,12 +606,8 @@ __device__ __forceinline__ void atoms_impl(
           df = -6.0 * s2 * (A12 + e_v);
         } else if (el) {
           const double rij = sqrt(r2), sv = 1.0 / rij;
-          const double xerfc = P.beta * rij;
-          const int idx = (int)(P.erftbdns * xerfc);
-          const double dx = xerfc - idx * del;
-          const double *tb = erf_arr + 4 * idx;
-          const double derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
-          const double erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * ovt)) + tb[0];
+          double erfcc, derfc;
+          erfc_table(erf_arr, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
           e_e = qq * sv * erfcc;
           df = -qq * sv * (s2 * erfcc + sv * P.beta * derfc);
           if (kind == 1 && lj) {
@@ -653,14 +649,14 @@ __device__ __forceinline__ void atoms_impl(
       for (int k = 0; k < 3; k++) {
         const double g = df * d[k];
         fj[k] -= g;
-        const long long v = F64 ? fixd(g, P.fix_scale, P.ovf) : fixf((float)g, P.ovf);
+        const long long v = F64 ? fixd(g, P.fix_scale, bad) : fixf((float)g, bad);
         if (v) atomicAdd(&iacc[i * 3 + k], (unsigned long long)v);
       }
     }
     long long *jrow = jpart + (size_t)jpos[it] * 3;
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      jrow[k] = F64 ? fixd(fj[k], P.fix_scale, P.ovf) : fixf((float)fj[k], P.ovf);
+      jrow[k] = F64 ? fixd(fj[k], P.fix_scale, bad) : fixf((float)fj[k], bad);
     if (s == 0) {
       ev_r += ev;
       ee_r += ee;
@@ -669,6 +665,7 @@ __device__ __forceinline__ void atoms_impl(
       ee_i += ee;
     }
   }
+  fix_report(bad, P.ovf);
   __syncthreads();
   for (int u = t; u < WBLOCK * 3; u += ABT) ipart[(size_t)ch * (WBLOCK * 3) + u] = (long long)iacc[u];
   double v;
@@ -950,6 +947,7 @@ void atoms_plan(const moil_ctx *c, const double *d_coor, int part0, EvalPlan *p)
   P.cutvdw2 = c->par.cutvdw2;
   P.cutele2 = c->par.cutele2;
   P.erftbdns = c->erftbdns;
+  P.ntab = c->ntab;
   P.a = c->par.a; P.b = c->par.b; P.c = c->par.c;
   P.fix_scale = c->fix_scale;
   P.ovf = c->d_ovf;

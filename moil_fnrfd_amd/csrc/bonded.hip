@@ -37,6 +37,7 @@ struct SlotSink {
   int n;
   double s;
   int *ovf;
+  bool bad = false;
   __device__ SlotSink(int n_, int i0, int i1, int i2, int i3, double s_, int *ovf_)
       : n(n_), s(s_), ovf(ovf_) {
     id[0] = i0; id[1] = i1; id[2] = i2; id[3] = i3;
@@ -48,12 +49,13 @@ struct SlotSink {
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (q == p) {
-        acc[q][0] += fixd(fx, s, ovf);
-        acc[q][1] += fixd(fy, s, ovf);
-        acc[q][2] += fixd(fz, s, ovf);
+        acc[q][0] += fixd(fx, s, bad);
+        acc[q][1] += fixd(fy, s, bad);
+        acc[q][2] += fixd(fz, s, bad);
       }
   }
   __device__ void flush(long long *__restrict__ out) const {
+    fix_report(bad, ovf);
 #pragma unroll
     for (int p = 0; p < 4; p++)
       if (p < n) {
@@ -67,15 +69,15 @@ struct WaterSink {
   long long acc[9];
   int a0;
   double s;
-  int *ovf;
+  bool bad;
   __device__ void add(int atom, double fx, double fy, double fz) {
     const int l = atom - a0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
       if (k == l) {
-        acc[3 * k] += fixd(fx, s, ovf);
-        acc[3 * k + 1] += fixd(fy, s, ovf);
-        acc[3 * k + 2] += fixd(fz, s, ovf);
+        acc[3 * k] += fixd(fx, s, bad);
+        acc[3 * k + 1] += fixd(fy, s, bad);
+        acc[3 * k + 2] += fixd(fz, s, bad);
       }
   }
 };
@@ -247,12 +249,8 @@ __device__ inline void excl_calc(const double *coor, int i, int j, const double 
          rz = coor[3 * i + 2] - coor[3 * j + 2];
   double r2 = rx * rx + ry * ry + rz * rz, s2 = 1.0 / r2, rij = sqrt(r2), s = 1.0 / rij;
   double qq = q[i] * P.qfac * q[j];
-  double xerfc = P.beta * rij;
-  int idx = (int)(P.erftbdns * xerfc);
-  double dx = xerfc - idx * (1.0 / P.erftbdns);
-  const double *t = erf_arr + 4 * idx;
-  double derfc = -t[1] - dx * (t[2] + 0.5 * dx * t[3]);
-  double erfcc = dx * (t[1] + 0.5 * dx * (t[2] + dx * t[3] * (1.0 / 3.0))) + t[0];
+  double erfcc, derfc;
+  erfc_table(erf_arr, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
   double erf = 1.0 - erfcc;
   e = -qq * s * erf;
   double df = qq * s * (s2 * erf - s * P.beta * derfc);
@@ -359,7 +357,7 @@ __device__ __forceinline__ void water_intra_impl(const double *__restrict__ coor
     WaterSink sk;
     sk.a0 = a0;
     sk.s = P.fix_scale;
-    sk.ovf = P.ovf;
+    sk.bad = false;
 #pragma unroll
     for (int k = 0; k < 9; k++) sk.acc[k] = 0;
     double et;
@@ -379,6 +377,7 @@ __device__ __forceinline__ void water_intra_impl(const double *__restrict__ coor
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) facc[3 * a0 + k] = sk.acc[k];
+    fix_report(sk.bad, P.ovf);
   }
   block_sums<3, WIT>(e, red);
   const int slot[3] = {EN_BOND, EN_THETA, EN_CORR};
@@ -432,6 +431,7 @@ void bonded_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   P.qfac = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;
   P.beta = c->beta;
   P.erftbdns = c->erftbdns;
+  P.ntab = c->ntab;
   P.do_el14 = (T & MOIL_TERM_EL14) != 0;
   P.do_v14 = (T & MOIL_TERM_VDW14) != 0;
   P.max_parts = c->max_parts;

@@ -58,6 +58,7 @@ struct ARS {
   double beta, cutvdw2, cutele2, erftbdns;
   double a, b, c;
   double fix_scale;
+  int ntab;  // rows of the erfc table
   int *ovf;
 };
 struct AtomArgs {
@@ -89,6 +90,7 @@ struct BPar {
   double epstmp14, pick;  // ener14 switches
   double qfac;            // kofdie/eps (exclusion correction)
   double beta, erftbdns;
+  int ntab;  // rows of the erfc table
   int do_el14, do_v14;
   int max_parts;
 };

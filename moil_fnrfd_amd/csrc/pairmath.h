@@ -28,23 +28,31 @@ __device__ __forceinline__ float erfc_poly(float t) {
 // pattern minus that of 1.5 2^32 is the rounded fixed-point value (one
 // conversion, one add, one 64-bit subtract); larger terms take the full
 // double-to-integer conversion.  A term beyond the accumulators' per-term
-// range (FIX_TERM_UNITS, common.h) or a NaN sets *ovf.
+// range (FIX_TERM_UNITS, common.h) or a NaN sets `bad`, which the thread
+// reports once with fix_report after its conversions (an atomic on the
+// overflow word in the middle of a pair loop costs the loop registers).
 static_assert(FIX_SHIFT_MIXED == 20, "fixf's bias constant is that of 2^-20 units");
 #define FIX_BIAS20 0x41F8000000000000ull  // the bit pattern of 1.5 2^32
-__device__ __forceinline__ long long fixd(double v, double s, int *ovf);
-__device__ __forceinline__ long long fixf(float f, int *ovf) {
+__device__ __forceinline__ long long fixd(double v, double s, bool &bad);
+__device__ __forceinline__ long long fixf(float f, bool &bad) {
   if (fabsf(f) < 2147483648.0f)
     return (long long)((unsigned long long)__double_as_longlong((double)f + 6442450944.0) -
                        FIX_BIAS20);
-  return fixd((double)f, 1048576.0, ovf);
+  return fixd((double)f, 1048576.0, bad);
 }
 
 // round(v s), s = 2^FIX_SHIFT, for the FP64 conversions, with the same
 // range check
-__device__ __forceinline__ long long fixd(double v, double s, int *ovf) {
+__device__ __forceinline__ long long fixd(double v, double s, bool &bad) {
   const double x = v * s;
-  if (!(fabs(x) < FIX_TERM_UNITS)) atomicOr(ovf, 1);
+  bad |= !(fabs(x) < FIX_TERM_UNITS);
   return __double2ll_rn(x);
+}
+
+// sets the context's overflow word if this thread converted a term out of
+// range
+__device__ __forceinline__ void fix_report(bool bad, int *ovf) {
+  if (bad) atomicOr(ovf, 1);
 }
 
 // Each lane walks its block mask starting at its own lane number (bit b of
@@ -55,11 +63,22 @@ __device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
 }
 
 // MOIL's cubic-spline erfc table (pot/cdie_ewald.f:152-160): erfc and its
-// derivative at x = beta r
+// derivative at x = beta r from row int(erftbdns x).  MOIL reads that row
+// without a bound; its lists keep x inside the table (1.5 relx long) while
+// molecules are whole and atoms moved less than the list skin.  Beyond the
+// last of the ntab rows (stale lists, a molecule split across the box, a NaN)
+// the last row is read at its right end, so no read leaves the table.
 __device__ __forceinline__ void erfc_table(const double *__restrict__ erf_arr, double erftbdns,
-                                           double x, double &erfcc, double &derfc) {
-  const int idx = (int)(erftbdns * x);
-  const double dx = x - idx * (1.0 / erftbdns);
+                                           int ntab, double x, double &erfcc, double &derfc) {
+  int idx;
+  double dx;
+  if (erftbdns * x < (double)ntab) {
+    idx = (int)(erftbdns * x);
+    dx = x - idx * (1.0 / erftbdns);
+  } else {
+    idx = ntab - 1;
+    dx = 1.0 / erftbdns;
+  }
   const double *tb = erf_arr + 4 * idx;
   derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
   erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * (1.0 / 3.0))) + tb[0];

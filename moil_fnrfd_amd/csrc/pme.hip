@@ -266,6 +266,7 @@ DEFINE_KERNEL(k_pme_tplace, TplaceArgs, PME_THREADS, tplace_body)
 // point left the region's accepted range adds its 64 weights to the
 // overflow grid with device atomics (the same fixed-point integers).
 #define TS_STAGE 256
+#define RB_GRID 0x40000000
 template <typename R>
 __device__ __forceinline__ void tspread_impl(const TspreadArgs &A, int bid) {
   __shared__ unsigned long long acc[PME_RMAX * PME_RMAX * PME_RMAX];
@@ -305,39 +306,30 @@ __device__ __forceinline__ void tspread_impl(const TspreadArgs &A, int bid) {
       }
       const int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
                 b3 = local_base(c3, lo3, hi3, T.n3);
-      if (b1 >= 0 && b2 >= 0 && b3 >= 0) {
-        rb[t] = b1 | (b2 << 8) | (b3 << 16);
-      } else {
-        rb[t] = -1;
-        // outside the region: the stencil points c-3..c straight into the
-        // overflow grid (rare: the atom moved beyond the margin since the
-        // list build)
-        for (int c = 0; c < 4; c++) {
-          int g3 = c3 - 3 + c;
-          g3 += g3 < 0 ? T.n3 : 0;
-          for (int bb = 0; bb < 4; bb++) {
-            int g2 = c2 - 3 + bb;
-            g2 += g2 < 0 ? T.n2 : 0;
-            for (int aa = 0; aa < 4; aa++) {
-              int g1 = c1 - 3 + aa;
-              g1 += g1 < 0 ? T.n1 : 0;
-              const long long f = __double2ll_rn((double)(o[8 + c] * o[4 + bb] * o[aa]) * P.qscale);
-              if (f)
-                atomicAdd(&A.qover[((size_t)g3 * T.n2 + g2) * T.n1 + g1], (unsigned long long)f);
-            }
-          }
-        }
-      }
+      // the local base in the region, or (bit 30) the grid base point of
+      // an atom outside the region (it moved beyond the margin since the
+      // list build; grid dimensions are below 1024)
+      rb[t] = (b1 >= 0 && b2 >= 0 && b3 >= 0) ? (b1 | (b2 << 8) | (b3 << 16))
+                                               : (RB_GRID | c1 | (c2 << 10) | (c3 << 20));
     }
     __syncthreads();
     for (int a = wv; a < n; a += PME_THREADS / 64) {
       const int bp = rb[a];
-      if (bp < 0) continue;  // uniform over the wavefront
-      const int l1 = (bp & 255) + la, l2 = ((bp >> 8) & 255) + lb, l3 = (bp >> 16) + lc;
       const R *o = rec + 12 * a;
       const R v = o[8 + lc] * o[4 + lb] * o[la];
       const long long f = __double2ll_rn((double)v * P.qscale);
-      if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
+      if (!(bp & RB_GRID)) {  // uniform over the wavefront
+        const int l1 = (bp & 255) + la, l2 = ((bp >> 8) & 255) + lb, l3 = (bp >> 16) + lc;
+        if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
+      } else {
+        // the stencil point c-3+l straight into the overflow grid
+        int g1 = (bp & 1023) - 3 + la, g2 = ((bp >> 10) & 1023) - 3 + lb,
+            g3 = ((bp >> 20) & 1023) - 3 + lc;
+        g1 += g1 < 0 ? T.n1 : 0;
+        g2 += g2 < 0 ? T.n2 : 0;
+        g3 += g3 < 0 ? T.n3 : 0;
+        if (f) atomicAdd(&A.qover[((size_t)g3 * T.n2 + g2) * T.n1 + g1], (unsigned long long)f);
+      }
     }
   }
   __syncthreads();
@@ -464,18 +456,63 @@ __device__ __forceinline__ void load_region(const TilePar &T, int lo1, int lo2, 
   }
 }
 
-// the 4x4x4 stencil values of an atom: from the LDS region at local base
-// (b1, b2, b3), or (b1 < 0) from the grid around base point (c1, c2, c3)
-template <typename R>
-__device__ __forceinline__ R stencil_value(const TilePar &T, const R *reg,
-                                           const R *__restrict__ grid, int b1, int b2, int b3,
-                                           int c1, int c2, int c3, int a, int b, int c) {
-  if (b1 >= 0) return reg[((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1 + a];
-  int g1 = c1 - 3 + a, g2 = c2 - 3 + b, g3 = c3 - 3 + c;
+// the 4x4x4 stencil values of an atom: LOCAL, from the LDS region at
+// local base (b1, b2, b3); else from the grid around base point
+// (b1, b2, b3) = (c1, c2, c3).  The two forms are separate code paths of
+// the callers, so the rare grid path costs the common one no registers.
+template <typename R, bool LOCAL>
+__device__ __forceinline__ R stencil_value(const TilePar &T, const R *__restrict__ src, int b1,
+                                           int b2, int b3, int a, int b, int c) {
+  if (LOCAL) return src[((b3 + c) * T.r2 + (b2 + b)) * T.r1 + b1 + a];
+  int g1 = b1 - 3 + a, g2 = b2 - 3 + b, g3 = b3 - 3 + c;
   g1 += g1 < 0 ? T.n1 : 0;
   g2 += g2 < 0 ? T.n2 : 0;
   g3 += g3 < 0 ? T.n3 : 0;
-  return grid[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
+  return src[((size_t)g3 * T.n2 + g2) * T.n1 + g1];
+}
+
+// the gather's three sums over an atom's stencil (derivative weights in
+// one dimension each)
+template <typename R, bool LOCAL>
+__device__ __forceinline__ void gather_sums(const TilePar &T, const R *__restrict__ src, int b1,
+                                            int b2, int b3, const R *t1, const R *t2,
+                                            const R *t3, const R *d1, const R *d2, const R *d3,
+                                            double &f1, double &f2, double &f3) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      R g1 = 0, g2 = 0;
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const R Q = stencil_value<R, LOCAL>(T, src, b1, b2, b3, a, b, c);
+        g1 += Q * d1[a];
+        g2 += Q * t1[a];
+      }
+      f1 += (double)(g1 * t2[b] * t3[c]);
+      f2 += (double)(g2 * d2[b] * t3[c]);
+      f3 += (double)(g2 * t2[b] * d3[c]);
+    }
+  }
+}
+
+// the potential's sum over an atom's stencil
+template <typename R, bool LOCAL>
+__device__ __forceinline__ double potential_sum(const TilePar &T, const R *__restrict__ src,
+                                                int b1, int b2, int b3, const R *t1,
+                                                const R *t2, const R *t3) {
+  double s = 0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      R g = 0;
+#pragma unroll
+      for (int a = 0; a < 4; a++) g += stencil_value<R, LOCAL>(T, src, b1, b2, b3, a, b, c) * t1[a];
+      s += (double)(g * t2[b] * t3[c]);
+    }
+  }
+  return s;
 }
 
 template <typename R>
@@ -509,26 +546,13 @@ __device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
     bspline4<R>((R)w1, t1, d1);
     bspline4<R>((R)w2, t2, d2);
     bspline4<R>((R)w3, t3, d3);
-    int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
-        b3 = local_base(c3, lo3, hi3, T.n3);
-    if (b2 < 0 || b3 < 0) b1 = -1;
+    const int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
+              b3 = local_base(c3, lo3, hi3, T.n3);
     double f1 = 0, f2 = 0, f3 = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        R g1 = 0, g2 = 0;
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-          const R Q = stencil_value<R>(T, reg, grid, b1, b2, b3, c1, c2, c3, a, b, c);
-          g1 += Q * d1[a];
-          g2 += Q * t1[a];
-        }
-        f1 += (double)(g1 * t2[b] * t3[c]);
-        f2 += (double)(g2 * d2[b] * t3[c]);
-        f3 += (double)(g2 * t2[b] * d3[c]);
-      }
-    }
+    if (b1 >= 0 && b2 >= 0 && b3 >= 0)
+      gather_sums<R, true>(T, reg, b1, b2, b3, t1, t2, t3, d1, d2, d3, f1, f2, f3);
+    else  // outside the region (moved beyond the margin): the grid itself
+      gather_sums<R, false>(T, grid, c1, c2, c3, t1, t2, t3, d1, d2, d3, f1, f2, f3);
     // dE/dfr_k = qn * sum(...) * n_k ; dpot -= recip * f with f = -dE/dfr
     fpme[3 * n] = qn * f1 * (double)P.n1 * P.ra;
     fpme[3 * n + 1] = qn * f2 * (double)P.n2 * P.rb;
@@ -820,21 +844,11 @@ __global__ __launch_bounds__(PME_THREADS) void k_recip_pot(
     bspline4<R>((R)w1, t1, dd);
     bspline4<R>((R)w2, t2, dd);
     bspline4<R>((R)w3, t3, dd);
-    int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
-        b3 = local_base(c3, lo3, hi3, T.n3);
-    if (b2 < 0 || b3 < 0) b1 = -1;
-    double s = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        R g = 0;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-          g += stencil_value<R>(T, reg, grid, b1, b2, b3, c1, c2, c3, a, b, c) * t1[a];
-        s += (double)(g * t2[b] * t3[c]);
-      }
-    }
+    const int b1 = local_base(c1, lo1, hi1, T.n1), b2 = local_base(c2, lo2, hi2, T.n2),
+              b3 = local_base(c3, lo3, hi3, T.n3);
+    const double s = (b1 >= 0 && b2 >= 0 && b3 >= 0)
+                         ? potential_sum<R, true>(T, reg, b1, b2, b3, t1, t2, t3)
+                         : potential_sum<R, false>(T, grid, c1, c2, c3, t1, t2, t3);
     dq[n] += P.fc * s;
   }
 }

@@ -27,8 +27,9 @@
 // erfc(x) = exp(-x^2) t P(t), t = 1/(1+p x): the functional form of
 // Abramowitz & Stegun 7.1.26 with a degree-8 polynomial fitted here for
 // relative accuracy 3e-8 on [0, 6].  Forces are rounded per term to
-// 64-bit fixed point before any sum: 32 fractional bits (floor) for the
-// partner's rows, 22 (nearest) for the block members' LDS sums.
+// 64-bit fixed point with FIX_SHIFT_MIXED = 20 fractional bits (nearest)
+// before any sum: the partner's rows as FP32 sums converted by the
+// reduction, the block members' terms in the LDS sums.
 // MOIL_PREC_FP64: FP64 arithmetic in MOIL's operation order with MOIL's
 // cubic-spline erfc table (pot/cdie_ewald.f:152-160).
 #include <hip/hip_ext.h>
@@ -88,6 +89,69 @@ __device__ inline double block_sum(double v, double *red) {
   return r;
 }
 
+// One listed water pair in FP32: the partner's O (hi + lo, translated)
+// and H offsets, and the constants of the pair terms
+struct WPartner {
+  float jhx, jhy, jhz, jlx, jly, jlz;
+  float j1x, j1y, j1z, j2x, j2y, j2z;
+};
+struct WConst {
+  float nb2, pb, qoo, qoh, qhh, qcoo, qcoh, qchh, a12, b6, cutvdw2, cutele2;
+};
+// the nine site pairs of block member (A, B, C) and partner J: the
+// member's gradient components in fi, the partner's added to fj, the pair
+// energies added to elj / eel; false (and nothing computed) for a
+// listwt2 pair beyond relx
+__device__ __forceinline__ bool water_entry(const float4 &A, const float4 &B, const float4 &C,
+                                            const WPartner &J, bool k2, const WConst &K,
+                                            float fi[9], float fj[9], float &elj, float &eel) {
+  const float dx = (A.x - J.jhx) + (B.x - J.jlx);
+  const float dy = (A.y - J.jhy) + (B.y - J.jly);
+  const float dz = (A.z - J.jhz) + (B.z - J.jlz);
+  const float roo2 = dx * dx + dy * dy + dz * dz;
+  if (k2 && roo2 > K.cutele2) return false;
+  // site vectors r_ab = x_ia - x_jb
+  float ux[3], uy[3], uz[3];
+  ux[0] = dx; uy[0] = dy; uz[0] = dz;
+  ux[1] = dx + A.w; uy[1] = dy + B.w; uz[1] = dz + C.x;
+  ux[2] = dx + C.y; uy[2] = dy + C.z; uz[2] = dz + C.w;
+  const float vx[3] = {0.f, J.j1x, J.j2x}, vy[3] = {0.f, J.j1y, J.j2y},
+              vz[3] = {0.f, J.j1z, J.j2z};
+#pragma unroll
+  for (int k = 0; k < 9; k++) fi[k] = 0.f;
+#pragma unroll
+  for (int sa = 0; sa < 3; sa++) {
+#pragma unroll
+    for (int sb = 0; sb < 3; sb++) {
+      const float rx = ux[sa] - vx[sb], ry = uy[sa] - vy[sb], rz = uz[sa] - vz[sb];
+      const float r2 = (sa == 0 && sb == 0) ? roo2 : (rx * rx + ry * ry + rz * rz);
+      const float qq = (sa == 0 && sb == 0) ? K.qoo : ((sa == 0 || sb == 0) ? K.qoh : K.qhh);
+      const float qc = (sa == 0 && sb == 0) ? K.qcoo : ((sa == 0 || sb == 0) ? K.qcoh : K.qchh);
+      const float s = __builtin_amdgcn_rsqf(r2);
+      const float s2 = s * s;
+      const float tt = __builtin_amdgcn_rcpf(fmaf(K.pb, r2 * s, 1.0f));
+      const float ex = __builtin_amdgcn_exp2f(r2 * K.nb2);
+      const float e = qq * (ex * erfc_poly(tt) * s);
+      eel += e;
+      float df = -s2 * fmaf(qc, ex, e);
+      if (sa == 0 && sb == 0 && !k2 && roo2 <= K.cutvdw2) {
+        const float s6 = s2 * s2 * s2;
+        const float A12 = K.a12 * s6 * s6, B6 = K.b6 * s6;
+        const float e1 = A12 - B6;
+        elj += e1;
+        df += -6.0f * s2 * (A12 + e1);
+      }
+      fi[3 * sa] = fmaf(df, rx, fi[3 * sa]);
+      fi[3 * sa + 1] = fmaf(df, ry, fi[3 * sa + 1]);
+      fi[3 * sa + 2] = fmaf(df, rz, fi[3 * sa + 2]);
+      fj[3 * sb] = fmaf(-df, rx, fj[3 * sb]);
+      fj[3 * sb + 1] = fmaf(-df, ry, fj[3 * sb + 1]);
+      fj[3 * sb + 2] = fmaf(-df, rz, fj[3 * sb + 2]);
+    }
+  }
+  return true;
+}
+
 // FP32 (mixed precision) water kernel
 __device__ __forceinline__ void water_f32_impl(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ order,
@@ -101,8 +165,7 @@ __device__ __forceinline__ void water_f32_impl(
   // pattern of 1.5 2^32 (the fixed point of FIX_SHIFT_MIXED = 20), and
   // each member's number of added terms, which takes the patterns'
   // constant part out again at the end (two instructions per value in
-  // place of a float-to-fixed-point conversion).  An entry with a larger
-  // component converts its values exactly and adds the same constant.
+  // place of a float-to-fixed-point conversion).
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ unsigned int icnt[WBLOCK];
@@ -122,14 +185,16 @@ __device__ __forceinline__ void water_f32_impl(
   __syncthreads();
 
   // exp(-beta^2 r^2) = exp2(-beta^2 log2(e) r^2), t = 1 / (1 + p beta r)
-  const float nb2 = (float)(-P.beta * P.beta * 1.4426950408889634);
-  const float pb = (float)(ERFC_P * P.beta);
-  const float qoo = (float)P.qq_oo, qoh = (float)P.qq_oh, qhh = (float)P.qq_hh;
+  WConst K;
+  K.nb2 = (float)(-P.beta * P.beta * 1.4426950408889634);
+  K.pb = (float)(ERFC_P * P.beta);
+  K.qoo = (float)P.qq_oo; K.qoh = (float)P.qq_oh; K.qhh = (float)P.qq_hh;
   const float c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
-  const float qcoo = qoo * c2b, qcoh = qoh * c2b, qchh = qhh * c2b;
-  const float a12 = (float)P.oo12, b6 = (float)P.oo6;
-  const float cutvdw2 = (float)P.cutvdw2, cutele2 = (float)P.cutele2;
+  K.qcoo = K.qoo * c2b; K.qcoh = K.qoh * c2b; K.qchh = K.qhh * c2b;
+  K.a12 = (float)P.oo12; K.b6 = (float)P.oo6;
+  K.cutvdw2 = (float)P.cutvdw2; K.cutele2 = (float)P.cutele2;
   const int rot = t & 63;
+  bool bad = false;  // a term beyond the LDS sums' range (or a NaN)
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
   for (int it = beg + t; it < end; it += RS_THREADS) {
@@ -142,81 +207,55 @@ __device__ __forceinline__ void water_f32_impl(
     const double ojx = (double)jA.x + (double)jB.x - ia * P.a;
     const double ojy = (double)jA.y + (double)jB.y - jb * P.b;
     const double ojz = (double)jA.z + (double)jB.z - kc * P.c;
-    const float jhx = (float)ojx, jhy = (float)ojy, jhz = (float)ojz;
-    const float jlx = (float)(ojx - (double)jhx), jly = (float)(ojy - (double)jhy),
-                jlz = (float)(ojz - (double)jhz);
-    const float j1x = jA.w, j1y = jB.w, j1z = jC.x;
-    const float j2x = jC.y, j2y = jC.z, j2z = jC.w;
+    WPartner J;
+    J.jhx = (float)ojx; J.jhy = (float)ojy; J.jhz = (float)ojz;
+    J.jlx = (float)(ojx - (double)J.jhx); J.jly = (float)(ojy - (double)J.jhy);
+    J.jlz = (float)(ojz - (double)J.jhz);
+    J.j1x = jA.w; J.j1y = jB.w; J.j1z = jC.x;
+    J.j2x = jC.y; J.j2y = jC.z; J.j2z = jC.w;
     float fj[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) fj[k] = 0.f;
     float elj = 0.f, eel = 0.f;
+    uint64_t big = 0;  // members whose entry is beyond the bit-pattern conversion
     uint64_t mask = rotr64(item.m1 | item.m2, rot);
     while (mask) {
       const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
       const bool k2 = (item.m2 >> i) & 1ull;
       mask &= mask - 1;
-      const float4 A = iA[i], B = iB[i], C = iC[i];
-      const float dx = (A.x - jhx) + (B.x - jlx);
-      const float dy = (A.y - jhy) + (B.y - jly);
-      const float dz = (A.z - jhz) + (B.z - jlz);
-      const float roo2 = dx * dx + dy * dy + dz * dz;
-      if (k2 && roo2 > cutele2) continue;
-      // site vectors r_ab = x_ia - x_jb
-      float ux[3], uy[3], uz[3];
-      ux[0] = dx; uy[0] = dy; uz[0] = dz;
-      ux[1] = dx + A.w; uy[1] = dy + B.w; uz[1] = dz + C.x;
-      ux[2] = dx + C.y; uy[2] = dy + C.z; uz[2] = dz + C.w;
-      const float vx[3] = {0.f, j1x, j2x}, vy[3] = {0.f, j1y, j2y}, vz[3] = {0.f, j1z, j2z};
       float fi[9];
+      if (!water_entry(iA[i], iB[i], iC[i], J, k2, K, fi, fj, elj, eel)) continue;
+      // the nine components' magnitudes summed (a NaN stays a NaN): the
+      // bit-pattern conversion holds for every |f| < 2^31; a larger entry
+      // is converted again, exactly, below
+      const bool b = !((((fabsf(fi[0]) + fabsf(fi[1])) + (fabsf(fi[2]) + fabsf(fi[3]))) +
+                        ((fabsf(fi[4]) + fabsf(fi[5])) + (fabsf(fi[6]) + fabsf(fi[7]))) +
+                        fabsf(fi[8])) < 2147483648.0f);
+      big |= (uint64_t)b << i;
 #pragma unroll
-      for (int k = 0; k < 9; k++) fi[k] = 0.f;
-#pragma unroll
-      for (int sa = 0; sa < 3; sa++) {
-#pragma unroll
-        for (int sb = 0; sb < 3; sb++) {
-          const float rx = ux[sa] - vx[sb], ry = uy[sa] - vy[sb], rz = uz[sa] - vz[sb];
-          const float r2 = (sa == 0 && sb == 0) ? roo2 : (rx * rx + ry * ry + rz * rz);
-          const float qq = (sa == 0 && sb == 0) ? qoo : ((sa == 0 || sb == 0) ? qoh : qhh);
-          const float qc = (sa == 0 && sb == 0) ? qcoo : ((sa == 0 || sb == 0) ? qcoh : qchh);
-          const float s = __builtin_amdgcn_rsqf(r2);
-          const float s2 = s * s;
-          const float tt = __builtin_amdgcn_rcpf(fmaf(pb, r2 * s, 1.0f));
-          const float ex = __builtin_amdgcn_exp2f(r2 * nb2);
-          const float e = qq * (ex * erfc_poly(tt) * s);
-          eel += e;
-          float df = -s2 * fmaf(qc, ex, e);
-          if (sa == 0 && sb == 0 && !k2 && roo2 <= cutvdw2) {
-            const float s6 = s2 * s2 * s2;
-            const float A12 = a12 * s6 * s6, B6 = b6 * s6;
-            const float e1 = A12 - B6;
-            elj += e1;
-            df += -6.0f * s2 * (A12 + e1);
-          }
-          fi[3 * sa] = fmaf(df, rx, fi[3 * sa]);
-          fi[3 * sa + 1] = fmaf(df, ry, fi[3 * sa + 1]);
-          fi[3 * sa + 2] = fmaf(df, rz, fi[3 * sa + 2]);
-          fj[3 * sb] = fmaf(-df, rx, fj[3 * sb]);
-          fj[3 * sb + 1] = fmaf(-df, ry, fj[3 * sb + 1]);
-          fj[3 * sb + 2] = fmaf(-df, rz, fj[3 * sb + 2]);
-        }
-      }
-      const float fmax = fmaxf(fmaxf(fmaxf(fabsf(fi[0]), fabsf(fi[1])), fmaxf(fabsf(fi[2]), fabsf(fi[3]))),
-                               fmaxf(fmaxf(fabsf(fi[4]), fabsf(fi[5])),
-                                     fmaxf(fmaxf(fabsf(fi[6]), fabsf(fi[7])), fabsf(fi[8]))));
-      if (fmax < 2147483648.0f) {
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-          atomicAdd(&iacc[k * WBLOCK + i],
-                    (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
-      } else {
-        // huge forces (or NaN): exact conversion, range-checked
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-          atomicAdd(&iacc[k * WBLOCK + i],
-                    (unsigned long long)fixd((double)fi[k], 1048576.0, P.ovf) + kBias32);
-      }
+      for (int k = 0; k < 9; k++)
+        atomicAdd(&iacc[k * WBLOCK + i],
+                  (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
       atomicAdd(&icnt[i], 1u);
+    }
+    if (big) {
+      // rare (forces beyond 2^31 / 9 kcal/mol/A): the entries recomputed,
+      // the bit patterns added above replaced by the exact fixed-point
+      // values (range-checked up to FIX_TERM_UNITS)
+      float fjd[9], ed = 0.f;  // the partner's side was added in the loop above
+#pragma unroll
+      for (int k = 0; k < 9; k++) fjd[k] = 0.f;
+      while (big) {
+        const int i = __ffsll((long long)big) - 1;
+        big &= big - 1;
+        float fi[9];
+        water_entry(iA[i], iB[i], iC[i], J, (item.m2 >> i) & 1ull, K, fi, fjd, ed, ed);
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+          atomicAdd(&iacc[k * WBLOCK + i],
+                    (unsigned long long)fixd((double)fi[k], 1048576.0, bad) + kBias32 -
+                        (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
+      }
     }
     // the partner's row holds the FP32 sums; the reduction converts them
     // to fixed point (the same conversion of the same value, half the bytes)
@@ -231,6 +270,7 @@ __device__ __forceinline__ void water_f32_impl(
       e_el_i += (double)eel;
     }
   }
+  fix_report(bad, P.ovf);
   __syncthreads();
   // the chunk's row in fixed point (member-major as the reduction reads it)
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) {
@@ -273,9 +313,9 @@ __device__ __forceinline__ void water_f64_impl(
     iacc[u] = 0;
   }
   __syncthreads();
-  const double del = 1.0 / P.erftbdns, ovt = 1.0 / 3.0;
   const double fs = P.fix_scale;
   const int rot = t & 63;
+  bool bad = false;  // a term beyond the fixed-point range (or a NaN)
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
   for (int it = beg + t; it < end; it += RS_THREADS) {
     const WItem item = items[it];
@@ -325,12 +365,8 @@ __device__ __forceinline__ void water_f64_impl(
             elj += e1;
             df = -6.0 * s2 * (A12 + e1);
           }
-          const double xerfc = P.beta * rij;
-          const int idx = (int)(P.erftbdns * xerfc);
-          const double dx = xerfc - idx * del;
-          const double *tb = erf_arr + 4 * idx;
-          const double derfc = -tb[1] - dx * (tb[2] + 0.5 * dx * tb[3]);
-          const double erfcc = dx * (tb[1] + 0.5 * dx * (tb[2] + dx * tb[3] * ovt)) + tb[0];
+          double erfcc, derfc;
+          erfc_table(erf_arr, P.erftbdns, P.ntab, P.beta * rij, erfcc, derfc);
           eel += qq * s * erfcc;
           df += -qq * s * (s2 * erfcc + s * P.beta * derfc);
           fi[3 * sa] += df * rx;
@@ -343,13 +379,13 @@ __device__ __forceinline__ void water_f64_impl(
       }
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        long long v = fixd(fi[k], fs, P.ovf);
+        long long v = fixd(fi[k], fs, bad);
         if (v != 0) atomicAdd((unsigned long long *)&iacc[i * 9 + k], (unsigned long long)v);
       }
     }
     long long *jrow = jpart + (size_t)jpos[it] * 9;
 #pragma unroll
-    for (int k = 0; k < 9; k++) jrow[k] = fixd(fj[k], fs, P.ovf);
+    for (int k = 0; k < 9; k++) jrow[k] = fixd(fj[k], fs, bad);
     if (item.s == 0) {
       e_lj_r += elj;
       e_el_r += eel;
@@ -358,6 +394,7 @@ __device__ __forceinline__ void water_f64_impl(
       e_el_i += eel;
     }
   }
+  fix_report(bad, P.ovf);
   __syncthreads();
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) ipart[(size_t)ch * (WBLOCK * 9) + u] = iacc[u];
   double v;
@@ -396,11 +433,13 @@ __device__ __forceinline__ void water_reduce_impl(
   int q = q0;
   if (jf32) {
     const float *jp = reinterpret_cast<const float *>(jpart);
+    bool bad = false;
     for (; q + 1 < q1; q += 2) {
-      a0 += fixf(jp[(size_t)q * 9 + k], ovf);
-      a1 += fixf(jp[(size_t)(q + 1) * 9 + k], ovf);
+      a0 += fixf(jp[(size_t)q * 9 + k], bad);
+      a1 += fixf(jp[(size_t)(q + 1) * 9 + k], bad);
     }
-    if (q < q1) a0 += fixf(jp[(size_t)q * 9 + k], ovf);
+    if (q < q1) a0 += fixf(jp[(size_t)q * 9 + k], bad);
+    fix_report(bad, ovf);
   } else {
     for (; q + 1 < q1; q += 2) {
       a0 += jpart[(size_t)q * 9 + k];

@@ -775,10 +775,21 @@ int64_t orc_dump_atom_pairs(orc_ctx *c, int32_t *rows, int64_t cap) {
 typedef struct { double e_bond, e_theta, e_tors, e_imp, e_vdw, e_el, e_vsym, e_lsym,
                  e_el14, e_vdw14, e_corr, e_ew_receip; } eterms;
 
-/* cubic-spline erfc: pot/cdie_ewald.f:152-160 */
+/* cubic-spline erfc: pot/cdie_ewald.f:152-160.  MOIL reads row
+ * int(erftbdns x) without a bound; past the table's last row (a distance
+ * MOIL's lists reach only with stale lists or split molecules, where MOIL
+ * reads beyond its array) the last row is read at its right end, the
+ * library's definition of that case. */
 static inline void spline_erfc(const orc_ctx *c, double xerfc, double *erfcc, double *derfc) {
-  int idx = (int)(c->erftbdns * xerfc);
-  double dx = xerfc - idx * c->del;
+  int idx;
+  double dx;
+  if (c->erftbdns * xerfc < (double)c->ntab) {
+    idx = (int)(c->erftbdns * xerfc);
+    dx = xerfc - idx * c->del;
+  } else {
+    idx = c->ntab - 1;
+    dx = c->del;
+  }
   const double *t = &c->erf_arr[4 * idx];
   *derfc = -t[1] - dx * (t[2] + 0.5 * dx * t[3]);
   *erfcc = dx * (t[1] + 0.5 * dx * (t[2] + dx * t[3] * (1.0 / 3.0))) + t[0];

@@ -235,7 +235,7 @@ def test_atoms_moved_beyond_spreading_margin(gpu, name):
     atoms that moved further than the tiles' margin since then take the
     exact fallback (fixed-point overflow grid for the charges, direct grid
     reads in the gather).  With lists built at x and the energy evaluated
-    at x' (40 atoms displaced by 1.5-3 A, some across the box faces), the
+    at x' (40 atoms displaced by 1.5-3 A, five waters by a box length), the
     GPU's FP64 result equals the oracle's on the same stale lists (its
     reciprocal sum has no tiles)."""
     from oracle.oracle import Oracle
@@ -248,10 +248,25 @@ def test_atoms_moved_beyond_spreading_margin(gpu, name):
     rng = np.random.default_rng(77)
     xp = x.copy()
     idx = rng.choice(len(x), 40, replace=False)
-    d = rng.normal(size=(40, 3))
-    d *= (rng.uniform(1.5, 3.0, 40) / np.linalg.norm(d, axis=1))[:, None]
-    xp[idx] += d
-    xp[idx[:5], 0] += s.box[0]  # five of them one box length away
+    L = np.asarray(s.box, np.float64)
+    for i in idx:
+        # a displacement that keeps the atom 1.5 A from every other atom
+        # (closer contacts give force terms beyond the FP64 fixed-point range)
+        while True:
+            d = rng.normal(size=3)
+            d *= rng.uniform(1.5, 3.0) / np.linalg.norm(d)
+            r = xp - (xp[i] + d)
+            r -= L * np.round(r / L)
+            r2 = (r * r).sum(1)
+            r2[i] = np.inf
+            if r2.min() >= 1.5 ** 2:
+                break
+        xp[i] += d
+    # five whole waters one box length away (molecules stay whole, as
+    # MOIL's exclusion correction assumes)
+    for w in rng.choice(len(s.topo.idxtip3), 5, replace=False):
+        a0 = int(s.topo.poipt[s.topo.idxtip3[w] - 1])
+        xp[a0:a0 + 3, 0] += s.box[0]
     e, dp = m.eforce(xp)
     o = Oracle(s.topo, p)
     o.build_lists(x)
@@ -262,6 +277,41 @@ def test_atoms_moved_beyond_spreading_margin(gpu, name):
     assert max(err.values()) < 1e-12, err
     assert rms_rel(dp, dp_ref) < 1e-10
     # mixed precision on the same stale tiles against the FP64 result
+    mm = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
+    mm.nbondm(x)
+    em, dpm = mm.eforce(xp)
+    assert rms_rel(dpm, dp_ref) < 1e-5
+    m.close()
+    mm.close()
+
+
+def test_split_water_beyond_erfc_table(gpu):
+    """A water split across the box (one H one box length from its O, as
+    per-atom wrapping leaves it): its O-H exclusion correction and its site
+    pairs lie beyond MOIL's erfc table (1.5 relx), where MOIL reads past its
+    array.  The library reads the table's last row at its right end there,
+    as the oracle does: FP64 energies and gradient equal the oracle's, the
+    mixed gradient is within its bar of them, and every value is finite."""
+    from oracle.oracle import Oracle
+    _say('case water216 split water beyond the erfc table')
+    s, cut, ew, _ = make_case('water216')
+    p = configs.params_for(s, cut, ew, precision=abi.PREC_FP64)
+    x = s.coor.copy()
+    m = gpu(s.topo, p)
+    m.nbondm(x)
+    a0 = int(s.topo.poipt[s.topo.idxtip3[7] - 1])
+    xp = x.copy()
+    xp[a0 + 1, 1] += s.box[1]
+    e, dp = m.eforce(xp)
+    o = Oracle(s.topo, p)
+    o.build_lists(x)
+    e_ref, dp_ref = o.eforce(xp)
+    err = energy_errors(e, e_ref)
+    _say('  max term %.2e, e_corr %.6f, force rms-rel %.2e'
+         % (max(err.values()), e['e_corr'], rms_rel(dp, dp_ref)))
+    assert np.isfinite(dp).all() and all(np.isfinite(v) for v in e.values())
+    assert max(err.values()) < 1e-12, err
+    assert rms_rel(dp, dp_ref) < 1e-10
     mm = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_MIXED))
     mm.nbondm(x)
     em, dpm = mm.eforce(xp)
