@@ -175,6 +175,22 @@ int32_t moil_gpu_build_lists_dev(moil_ctx *ctx, const double *d_coor, void *stre
 int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
                             double *d_energies, void *stream);
 
+/* No counterpart in MOIL: moil_gpu_eforce_dev for callers that cannot
+ * allocate during the call (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The same evaluation with the same
+ * results, in order on `stream`, with its reduction scratch (energy
+ * partial sums, per-term totals, the reduction's ticket and the
+ * fixed-point overflow word) in a static 8 MiB device array of the
+ * library, one per GPU, held until the process ends: no allocation, free,
+ * graph or event during the call.  The lists must have been built
+ * (moil_gpu_build_lists*).  MOIL_GPU_ECAPACITY when the evaluation needs
+ * more than the array's 87,376 partial sums per energy term (real-space
+ * chunks, reciprocal convolution blocks or bonded blocks).  Every call on
+ * one GPU shares that GPU's array: calls that overlap in time on one GPU
+ * race. */
+int32_t moil_gpu_eforce_noalloc_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
+                                    double *d_energies, void *stream);
+
 /* Batched evaluation (no counterpart in MOIL, which evaluates one system
  * per process; the chain/path drivers loop over slices, chain/sds.f:71-140).
  * A batch groups existing contexts of one precision (independent systems

@@ -233,6 +233,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       ctx_free(c);
       return fail(MOIL_GPU_EHIP, "device query failed");
     }
+    c->device = dev;
+    // the device address of the no-allocation evaluation's scratch on this
+    // device (looked up once per device)
+    TRY(scratch_attach(dev));
     const char *w = std::getenv("MOIL_GPU_WATER_WG_PER_CU");
     if (w && w[0]) c->water_wg_per_cu = std::max(0, std::atoi(w));
     const char *cp = std::getenv("MOIL_GPU_CHUNK_PAIRS");
@@ -886,7 +890,7 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
 // and bonded terms and the partial-row reductions follow on st, in the
 // shadow of the reciprocal sum.  The intra-water terms store (not add)
 // the water atoms' accumulator rows, so they precede both reductions.
-static int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
+int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   // level 1 times every timing_stride-th evaluation (dispatch events cost
   // ~2 % of the evaluation rate when recorded on every one)
   const bool sample =

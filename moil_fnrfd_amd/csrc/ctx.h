@@ -33,6 +33,7 @@ struct moil_ctx {
   size_t nbslot = 0;
   long long *d_bslot = nullptr;
   int32_t *d_brstart = nullptr, *d_brslot = nullptr;
+  int device = 0;                 // the HIP device current at moil_gpu_init
   hipStream_t stream = nullptr;   // default stream of the context
   hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

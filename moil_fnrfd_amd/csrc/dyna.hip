@@ -23,7 +23,6 @@
 #include "multi.h"
 
 void set_error(const std::string &msg);
-hipStream_t pick_stream(moil_ctx *c, void *stream);
 
 struct moil_dyna {
   moil_ctx *ctx = nullptr;

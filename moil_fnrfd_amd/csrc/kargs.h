@@ -314,3 +314,10 @@ int bonded_max_blocks(const moil_ctx *c);
 // (the context's own record)
 void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_energies,
                EvalPlan *p);
+// the plan's launches in order on st (api.hip)
+int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st);
+// the caller's stream argument as a HIP stream
+hipStream_t pick_stream(moil_ctx *c, void *stream);
+// looks up the no-allocation evaluation's scratch address on device dev
+// (noalloc.hip; once per device, with dev current)
+int32_t scratch_attach(int dev);

@@ -65,6 +65,7 @@ def load_library():
     L.moil_gpu_squeeze_dev.argtypes = [vp, vp, vp]
     L.moil_gpu_build_lists_dev.argtypes = [vp, vp, vp]
     L.moil_gpu_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
+    L.moil_gpu_eforce_noalloc_dev.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_list_counts.argtypes = [vp, vp]
     L.moil_gpu_dump_water_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_dump_atom_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
@@ -101,6 +102,7 @@ def load_library():
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
+                 'moil_gpu_eforce_noalloc_dev',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
                  'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_list_generation', 'moil_gpu_batch_create',
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
@@ -188,6 +190,15 @@ class MoilGPU:
     def eforce_dev(self, d_coor: int, d_dpot: int, d_energies: int, stream: Optional[int] = None):
         _check(load_library().moil_gpu_eforce_dev(self._h, C.c_void_p(d_coor), C.c_void_p(d_dpot),
                                                   C.c_void_p(d_energies), C.c_void_p(stream or 0)))
+
+    def eforce_noalloc_dev(self, d_coor: int, d_dpot: int, d_energies: int,
+                           stream: Optional[int] = None):
+        """eforce_dev without any allocation during the call (the reduction
+        scratch is the library's static device array; calls that overlap on
+        one GPU race)."""
+        _check(load_library().moil_gpu_eforce_noalloc_dev(
+            self._h, C.c_void_p(d_coor), C.c_void_p(d_dpot), C.c_void_p(d_energies),
+            C.c_void_p(stream or 0)))
 
     # -- force-field parameters and their gradients ----------------------
     def set_params(self, ptchg, epsgm6, epsgm12, v14=0.5, el14=0.5):
