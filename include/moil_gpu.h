@@ -175,14 +175,43 @@ int32_t moil_gpu_build_lists_dev(moil_ctx *ctx, const double *d_coor, void *stre
 int32_t moil_gpu_eforce_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
                             double *d_energies, void *stream);
 
+/* Launch shapes of a context (no MOIL counterpart; the gradient is
+ * bit-identical for every shape, the energies agree to FP64 summation
+ * order, since a chunk budget regroups their sums): the pair budget of one
+ * water real-space workgroup and of one atom-list workgroup, the
+ * mixed-precision water kernel's workgroups per CU (0: one per chunk) and
+ * the largest edge of a PME spreading tile (grid points, 4..8).
+ * moil_gpu_autotune times evaluations of d_coor (device pointer, the
+ * context's stream; reps timed evaluations per candidate, <= 0 for 20)
+ * over a fixed candidate set, one shape at a time, keeps the fastest
+ * shapes and rebuilds the lists with them; us_default / us_tuned are the
+ * mean evaluation times before and after.  Results are cached per device
+ * and system signature for the process (and in the file named by
+ * MOIL_GPU_TUNE_CACHE, if set); a cached system is not timed again.
+ * moil_gpu_set_tuning applies shapes directly (chunk budgets and tiles
+ * take effect at the next moil_gpu_build_lists*, which must precede the
+ * next evaluation when the tile shape changed). */
+typedef struct moil_tuning {
+  int32_t wchunk_pairs;     /* 256..65536, default 2048 */
+  int32_t achunk_pairs;     /* 256..65536, default 8192 */
+  int32_t water_wg_per_cu;  /* 0..16, default 4 */
+  int32_t pme_tile;         /* 4..8, default 8 */
+  double us_default, us_tuned;
+} moil_tuning;
+int32_t moil_gpu_get_tuning(moil_ctx *ctx, moil_tuning *t);
+int32_t moil_gpu_set_tuning(moil_ctx *ctx, const moil_tuning *t);
+int32_t moil_gpu_autotune(moil_ctx *ctx, const double *d_coor, int32_t reps, moil_tuning *out);
+
 /* No counterpart in MOIL: moil_gpu_eforce_dev for callers that cannot
  * allocate during the call (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  The same evaluation with the same
- * results, in order on `stream`, with its reduction scratch (energy
- * partial sums, per-term totals, the reduction's ticket and the
- * fixed-point overflow word) in a static 8 MiB device array of the
- * library, one per GPU, held until the process ends: no allocation, free,
- * graph or event during the call.  The lists must have been built
+ * results, ordered on `stream` as moil_gpu_eforce_dev's (the reciprocal
+ * sum forks to the context's second stream and joins back through the
+ * context's own events), never replayed as a graph, with its reduction
+ * scratch (energy partial sums, per-term totals, the reduction's ticket
+ * and the fixed-point overflow word) in a static 8 MiB device array of
+ * the library, one per GPU, held until the process ends: no allocation
+ * or free during the call.  The lists must have been built
  * (moil_gpu_build_lists*).  MOIL_GPU_ECAPACITY when the evaluation needs
  * more than the array's 87,376 partial sums per energy term (real-space
  * chunks, reciprocal convolution blocks or bonded blocks).  Every call on

@@ -142,6 +142,16 @@ class MoilDynaParams(C.Structure):
                 ('reqhh2', C.c_double), ('invmh', C.c_double), ('invmo', C.c_double)]
 
 
+class MoilTuning(C.Structure):
+    """Launch shapes of a context (include/moil_gpu.h moil_tuning)."""
+    _fields_ = [('wchunk_pairs', C.c_int32), ('achunk_pairs', C.c_int32),
+                ('water_wg_per_cu', C.c_int32), ('pme_tile', C.c_int32),
+                ('us_default', C.c_double), ('us_tuned', C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # MOIL's time unit in picoseconds (COMMON/CONVERT.BLOCK tconv) and kB in
 # kcal/mol/K (kboltzmann)
 TCONV = 0.04888821290839616117

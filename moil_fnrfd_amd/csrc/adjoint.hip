@@ -75,8 +75,10 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_water(
   const double sl2e = sqrt(1.4426950408889634);
   const float betap = (float)(P.beta * sl2e), pe = (float)(ERFC_P / sl2e);
   const float cutvdw2f = (float)P.cutvdw2, cutele2f = (float)P.cutele2;
-  const int rot = t & 63;
   for (int it = beg + t; it < end; it += ADJ_THREADS) {
+    // the mask walk starts at a member given by the item, so the order of
+    // an item's sums does not depend on the lane or chunk that runs it
+    const int rot = it & 63;
     const WItem item = items[it];
     const int js = item.j;
     int ia, jb, kc;
@@ -252,11 +254,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_atoms(
   }
   for (int u = t; u < WBLOCK * 3; u += ADJ_THREADS) acc[u] = 0.0;
   __syncthreads();
-  const int rot = t & 63;
   const float betaf = (float)P.beta;
   const float cutvdw2f = (float)P.cutvdw2, cutele2f = (float)P.cutele2;
   const float log2e = 1.4426950408889634f;
   for (int it = beg + t; it < end; it += ADJ_THREADS) {
+    // the mask walk starts at a member given by the item, so the order of
+    // an item's sums does not depend on the lane or chunk that runs it
+    const int rot = it & 63;
     const AItem item = items[it];
     const int ja = item.j;
     int s;

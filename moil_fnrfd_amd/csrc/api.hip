@@ -67,6 +67,7 @@ static void ctx_free(moil_ctx *c) {
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_intra) hipEventDestroy(c->ev_intra);
   if (c->stream2) hipStreamDestroy(c->stream2);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -186,6 +187,10 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   if (std::fabs(qtot) > 1e-7)  // symm/ewald_init.f:62-72
     return fail(MOIL_GPU_ECHARGE, "The charge of the system is " + std::to_string(qtot) +
                                       ": add counterions and try again");
+  {
+    std::string why;
+    if (!pme_charges_fit(t->ptchg, (size_t)t->npt, p->eps, &why)) return fail(MOIL_GPU_EINVAL, why);
+  }
 
   moil_ctx *c = new moil_ctx();
   c->par = *p;
@@ -216,7 +221,8 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       hipEventCreateWithFlags(&c->gev_in, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->gev_out, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_intra, hipEventDisableTiming) != hipSuccess) {
     ctx_free(c);
     return fail(MOIL_GPU_EHIP, "hipStreamCreate/hipEventCreate failed");
   }
@@ -234,6 +240,7 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
       return fail(MOIL_GPU_EHIP, "device query failed");
     }
     c->device = dev;
+    c->ncu = ncu;
     // the device address of the no-allocation evaluation's scratch on this
     // device (looked up once per device)
     TRY(scratch_attach(dev));
@@ -884,12 +891,12 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
 // One evaluation in order on stream st (eager path; timing level 2 puts
 // events between the stages, level 1 has the real-space kernels record
 // their own dispatch events).  The packed water positions and the long
-// real-space pair kernels are submitted first; the reciprocal sum goes to
-// the second stream after them (forked at the evaluation's start, so it
-// waits only for the coordinates) and runs beside them; the intra-water
-// and bonded terms and the partial-row reductions follow on st, in the
-// shadow of the reciprocal sum.  The intra-water terms store (not add)
-// the water atoms' accumulator rows, so they precede both reductions.
+// real-space pair kernels are submitted first; the intra-water terms and
+// then the reciprocal sum go to the second stream after them (forked at
+// the evaluation's start, so they wait only for the coordinates) and run
+// beside them; the bonded terms and the partial-row reductions follow on
+// st.  The intra-water terms store (not add) the water atoms' accumulator
+// rows, so both reductions wait for them.
 int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   // level 1 times every timing_stride-th evaluation (dispatch events cost
   // ~2 % of the evaluation rate when recorded on every one)
@@ -916,7 +923,11 @@ int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   launch_k_atoms_ev(p.atoms, p.f64, p.n_atoms, st, ea, eb);
   if (ea && eb) c->ev_rec.push_back({1, {ea, eb}});
   if (fork) {
+    // the intra-water terms first on the second stream (they touch no
+    // buffer of the pair kernels), then the reciprocal sum
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    launch_k_water_intra(p.intra, p.n_intra, c->stream2);
+    HIP_CHECK(hipEventRecord(c->ev_intra, c->stream2));
     {
       StageTimer tp(c, 2, c->stream2);
       CHECK_RC(pme_run(c, p, c->stream2));
@@ -928,10 +939,13 @@ int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   }
   {
     StageTimer tb(c, 3, st);
-    launch_k_water_intra(p.intra, p.n_intra, st);
+    if (!fork) launch_k_water_intra(p.intra, p.n_intra, st);
     launch_k_bonded(p.bonded, p.n_bonded, st);
     HIP_CHECK(hipGetLastError());
   }
+  // the intra-water terms store the water atoms' accumulator rows, which
+  // the reductions then add to
+  if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_intra, 0));
   launch_k_water_reduce(p.wred, p.n_wred, st);
   launch_k_atoms_reduce(p.ared, p.n_ared, st);
   if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
@@ -952,13 +966,16 @@ static int32_t eforce_capture_body(moil_ctx *c, const EvalPlan &p, hipStream_t s
   else launch_k_atoms_f32(p.atoms, p.n_atoms, st);
   if (fork) {
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    launch_k_water_intra(p.intra, p.n_intra, c->stream2);
+    HIP_CHECK(hipEventRecord(c->ev_intra, c->stream2));
     CHECK_RC(pme_run(c, p, c->stream2));
     HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
   } else if (p.pme) {
     CHECK_RC(pme_run(c, p, st));
   }
-  launch_k_water_intra(p.intra, p.n_intra, st);
+  if (!fork) launch_k_water_intra(p.intra, p.n_intra, st);
   launch_k_bonded(p.bonded, p.n_bonded, st);
+  if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_intra, 0));
   launch_k_water_reduce(p.wred, p.n_wred, st);
   launch_k_atoms_reduce(p.ared, p.n_ared, st);
   if (fork) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
@@ -1095,13 +1112,13 @@ extern "C" int32_t moil_gpu_dump_water_pairs(moil_ctx *c, int32_t *rows, int64_t
   *n = 0;
   if (c->nwat == 0 || c->nwitems == 0) return MOIL_GPU_OK;
   std::vector<WItem> it(c->nwitems);
-  std::vector<int32_t> s2w(c->nwat), boff(c->nwblk * NBUCKET + 1);
+  std::vector<int32_t> s2w(c->nwat), boff(c->nwblk * WNBUCKET + 1);
   HIP_CHECK(hipMemcpy(it.data(), c->d_witems, sizeof(WItem) * it.size(), hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(s2w.data(), c->d_slot2wat, sizeof(int32_t) * s2w.size(), hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(boff.data(), c->d_boff, sizeof(int32_t) * boff.size(), hipMemcpyDeviceToHost));
   int64_t k = 0;
   for (int b = 0; b < c->nwblk; b++)
-    for (int q = boff[b * NBUCKET]; q < boff[(b + 1) * NBUCKET]; q++) {
+    for (int q = boff[b * WNBUCKET]; q < boff[(b + 1) * WNBUCKET]; q++) {
       const WItem &w = it[q];
       int jm = c->h_wat_mono[s2w[w.j]];
       for (int kind = 1; kind <= 2; kind++) {
@@ -1233,6 +1250,10 @@ extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const d
                         " differs from the last water in its charges or O LJ factors "
                         "(water-water pairs use the last water's)");
     }
+  }
+  {
+    std::string why;
+    if (!pme_charges_fit(ptchg, n, c->par.eps, &why)) return fail(MOIL_GPU_EINVAL, "set_params: " + why);
   }
   c->h_q.assign(ptchg, ptchg + n);
   c->h_e6.assign(epsgm6, epsgm6 + n);

@@ -556,13 +556,15 @@ __device__ __forceinline__ void atoms_impl(
   }
   for (int u = t; u < WBLOCK * 3; u += ABT) iacc[u] = 0;
   __syncthreads();
-  const int rot = t & 63;
   bool bad = false;  // a term beyond the fixed-point range (or a NaN)
   const float betaf = (float)P.beta, c2b = (float)(2.0 / sqrt(M_PI) * P.beta);
   const float cutvdw2f = (float)P.cutvdw2, cutele2f = (float)P.cutele2;
   const float log2e = 1.4426950408889634f;
   double ev_r = 0, ee_r = 0, ev_i = 0, ee_i = 0;
   for (int it = beg + t; it < end; it += ABT) {
+    // the mask walk starts at a member given by the item, so the order of
+    // an item's sums does not depend on the lane or chunk that runs it
+    const int rot = it & 63;
     const AItem item = items[it];
     const int ja = item.j;
     int s;
@@ -877,6 +879,8 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   cp.max_items = c->wchunk_max_items;
   cp.min_items = 1;
   cp.cap = c->achunk_cap;
+  cp.nbucket = NBUCKET;
+  cp.m2_div = 1;
   CHECK_SCAN(chunk_table(c, c->d_aboff, cp, c->d_account, c->d_acfirst, c->d_achunk, nullptr,
                          nullptr, st));
   const int32_t *d_nitem = c->d_aboff + (size_t)c->nablk * NBUCKET;

@@ -22,6 +22,9 @@
 #define WSPLIT 32
 #endif
 #define NBUCKET WSPLIT
+// water lists: the listwt1 and the listwt2 items in buckets of their own
+// (lists.hip emit_kind)
+#define WNBUCKET (2 * WSPLIT)
 
 // Fixed-point force accumulators: int64 with FIX_SHIFT fractional bits.
 // Every term converted to fixed point must stay below 2^59 units (a

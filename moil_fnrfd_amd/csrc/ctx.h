@@ -34,9 +34,11 @@ struct moil_ctx {
   long long *d_bslot = nullptr;
   int32_t *d_brstart = nullptr, *d_brslot = nullptr;
   int device = 0;                 // the HIP device current at moil_gpu_init
+  int ncu = 0;                    // its compute units
   hipStream_t stream = nullptr;   // default stream of the context
   hipStream_t stream2 = nullptr;  // reciprocal space, overlapped with real space
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_intra = nullptr;  // the intra-water terms done on the second stream
   bool pme_concurrent = true;     // off with MOIL_GPU_PME_CONCURRENT=0 in the environment
   bool facc_dirty = false;
   int fix_shift = FIX_SHIFT_FP64;
@@ -74,6 +76,7 @@ struct moil_ctx {
   std::vector<double> bsp[3];
   int nf1 = 0, nf2 = 0, nf3 = 0, nc1 = 0;  // grid dims; nc1 = nf1/2+1
   int pt_nt[3] = {0, 0, 0}, pt_r[3] = {0, 0, 0}, pt_ntile = 0;  // spreading tiles
+  int pt_edge = 0;                                   // their largest edge (points)
   int32_t *d_tcnt = nullptr, *d_tstart = nullptr;  // [ntile+1] atoms per tile, scan
   long long *d_tbuf = nullptr;                      // [ntile][r1 r2 r3] tile regions
   int32_t *d_atile = nullptr, *d_arank = nullptr;   // [npt] tile, rank in tile
@@ -240,6 +243,8 @@ enum { ST_WRAW = 0, ST_WITEMS, ST_WCHUNK, ST_ARAW, ST_AITEMS, ST_ACHUNK, ST_COUN
 // `pairs` pairs, min_items..max_items items; cap = table capacity
 struct ChunkPar {
   int nblk, pairs, max_items, min_items, cap;
+  int nbucket;  // NBUCKET (atom lists) or WNBUCKET (water lists)
+  int m2_div;   // pairs counted per listwt2 item: popcount / m2_div
 };
 int32_t chunk_table(moil_ctx *c, const int32_t *d_boff, ChunkPar P, int32_t *d_ccount,
                     int32_t *d_cfirst, int32_t *d_chunks, int32_t *d_cpairs, int32_t *d_order,

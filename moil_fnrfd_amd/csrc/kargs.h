@@ -10,6 +10,8 @@
 // precision (float or double) are carried untyped.
 #pragma once
 
+#include <string>
+
 #include "ctx.h"
 
 // ---- real space, water groups (realspace.hip) ----
@@ -304,9 +306,14 @@ void launch_k_atoms_ev(const AtomArgs &a, bool f64, int nblk, hipStream_t st, hi
 // convolution, gather; pme_bin assigns the atoms to spreading tiles (at
 // every list build)
 int32_t pme_bin(moil_ctx *c, const double *d_coor, hipStream_t st);
+// spreading tiles of at most `edge` points a side (4..8), buffers resized
+int32_t pme_set_tile(moil_ctx *c, int edge);
 void pme_launch_spread(const EvalPlan &p, hipStream_t st);
 int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st);
 int pme_conv_blocks(const moil_ctx *c);
+// whether every charge's stencil weight q sqrt(kofdie/eps) fits the grid's
+// fixed-point conversion (|q| sqrt(kofdie/eps) < 2^11); else the message
+bool pme_charges_fit(const double *q, size_t n, double eps, std::string *why);
 int bonded_blocks(const moil_ctx *c);
 int water_intra_blocks(const moil_ctx *c);
 int bonded_max_blocks(const moil_ctx *c);

@@ -252,22 +252,28 @@ __device__ inline uint64_t take_low_bits(uint64_t &m, int n) {
 }
 
 // Deterministic emission of one batch of candidates (one per thread of
-// the workgroup, mask 0 = none): each mask is split into nfull items of
-// WSPLIT pairs (bucket 0) and at most one remainder item (bucket WSPLIT -
-// popcount remainder); per-bucket positions come from wavefront scans and
-// ballots in thread order, so the item order is fixed.  Every thread of
-// the workgroup calls it.
+// the workgroup, masks 0 = none).  The two lists stay apart: the listwt1
+// members (m1) and the listwt2 members (m2) of a candidate become items of
+// their own kind, so the real-space kernel's lanes in one wavefront walk
+// masks of one kind and equal length (listwt2 members take part only
+// within relx, a cheap test, while every listwt1 member costs a full
+// entry).  Each kind's mask is split into nfull items of WSPLIT pairs
+// (bucket base + 0) and at most one remainder item (bucket base + WSPLIT -
+// popcount remainder), base = 0 for listwt1 and WSPLIT for listwt2;
+// per-bucket positions come from wavefront scans and ballots in thread
+// order, so the item order is fixed.  Every thread of the workgroup calls
+// it.
 struct EmitLDS {
-  int32_t bk_run[NBUCKET];
-  int32_t wcnt[LB_THREADS / 64][NBUCKET];
+  int32_t bk_run[WNBUCKET];
+  int32_t wcnt[LB_THREADS / 64][WSPLIT];
 };
 
-__device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
-                                  const int32_t *__restrict__ boff_b, WItem *__restrict__ items,
-                                  size_t cap) {
+__device__ inline void emit_kind(EmitLDS &L, int kind, int js, int s, uint64_t mk,
+                                 const int32_t *__restrict__ boff_b, WItem *__restrict__ items,
+                                 size_t cap) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const uint64_t mm = m1 | m2;
-  const int pc = __popcll(mm);
+  const int base = kind * WSPLIT;
+  const int pc = __popcll(mk);
   const int nfull = pc / WSPLIT, rem = pc % WSPLIT;
   const int bk = rem ? WSPLIT - rem : -1;
   int incl = nfull;
@@ -277,52 +283,66 @@ __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64
   }
   const int wtot0 = __shfl(incl, 63, 64);
   if (lane == 0) L.wcnt[wv][0] = wtot0;
-  for (int k = 1; k < NBUCKET; k++) {
+  for (int k = 1; k < WSPLIT; k++) {
     uint64_t bal = __ballot(bk == k);
     if (lane == 0) L.wcnt[wv][k] = __popcll(bal);
   }
   __syncthreads();
   {
-    uint64_t rest = mm;  // bits left for the remainder item
+    uint64_t rest = mk;  // bits left for the remainder item
     if (nfull > 0) {
       int before0 = 0;
       for (int w2 = 0; w2 < wv; w2++) before0 += L.wcnt[w2][0];
-      const int pos0 = boff_b[0] + L.bk_run[0] + before0 + (incl - nfull);
+      const int pos0 = boff_b[base] + L.bk_run[base] + before0 + (incl - nfull);
       for (int q = 0; q < nfull; q++) {
         const uint64_t piece = take_low_bits(rest, WSPLIT);
         WItem it;
         it.j = js;
         it.s = s;
-        it.m1 = m1 & piece;
-        it.m2 = m2 & piece;
+        it.m1 = kind ? 0ull : piece;
+        it.m2 = kind ? piece : 0ull;
         it.pad = 0;
         if ((size_t)(pos0 + q) < cap) items[pos0 + q] = it;
       }
     }
-    for (int k = 1; k < NBUCKET; k++) {
+    for (int k = 1; k < WSPLIT; k++) {
       uint64_t bal = __ballot(bk == k);
       if (bk == k) {
         int before = 0;
         for (int w2 = 0; w2 < wv; w2++) before += L.wcnt[w2][k];
         uint64_t lower = (lane == 0) ? 0ull : (bal & ((1ull << lane) - 1ull));
-        const int pos = boff_b[k] + L.bk_run[k] + before + __popcll(lower);
+        const int pos = boff_b[base + k] + L.bk_run[base + k] + before + __popcll(lower);
         WItem it;
         it.j = js;
         it.s = s;
-        it.m1 = m1 & rest;
-        it.m2 = m2 & rest;
+        it.m1 = kind ? 0ull : rest;
+        it.m2 = kind ? rest : 0ull;
         it.pad = 0;
         if ((size_t)pos < cap) items[pos] = it;
       }
     }
   }
   __syncthreads();
-  if (t < NBUCKET) {
+  if (t < WSPLIT) {
     int tot = 0;
     for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += L.wcnt[w2][t];
-    L.bk_run[t] += tot;
+    L.bk_run[base + t] += tot;
   }
   __syncthreads();
+}
+
+__device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
+                                  const int32_t *__restrict__ boff_b, WItem *__restrict__ items,
+                                  size_t cap) {
+  emit_kind(L, 0, js, s, m1, boff_b, items, cap);
+  emit_kind(L, 1, js, s, m2, boff_b, items, cap);
+}
+
+// items of one kind's mask per bucket (counts only: order independent)
+__device__ inline void count_kind(EmitLDS &L, int kind, uint64_t mk) {
+  const int pc = __popcll(mk);
+  if (pc >= WSPLIT) atomicAdd(&L.bk_run[kind * WSPLIT], pc / WSPLIT);
+  if (pc % WSPLIT) atomicAdd(&L.bk_run[kind * WSPLIT + WSPLIT - pc % WSPLIT], 1);
 }
 
 // One workgroup per water block: evaluates MOIL's predicates for every
@@ -362,7 +382,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
       im[t] = 0x7fffffff;
     }
   }
-  if (t < NBUCKET) L.bk_run[t] = 0;
+  if (t < WNBUCKET) L.bk_run[t] = 0;
   if (t == 0) r_run = 0;
   __syncthreads();
   if (t == 0) {
@@ -510,17 +530,16 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
           for (int w2 = 0; w2 < LB_THREADS / 64; w2++) tot += rw[w2];
           r_run += tot;
         }
-        // items per popcount bucket (counts only: order independent)
-        const int pc = __popcll(m1 | m2);
-        if (pc >= WSPLIT) atomicAdd(&L.bk_run[0], pc / WSPLIT);
-        if (pc % WSPLIT) atomicAdd(&L.bk_run[WSPLIT - pc % WSPLIT], 1);
+        // items per kind and popcount bucket
+        count_kind(L, 0, m1);
+        count_kind(L, 1, m2);
       }
       __syncthreads();
     }
     __syncthreads();
   }
   __syncthreads();
-  if (t < NBUCKET) bcount[(size_t)blockIdx.x * NBUCKET + t] = L.bk_run[t];
+  if (t < WNBUCKET) bcount[(size_t)blockIdx.x * WNBUCKET + t] = L.bk_run[t];
   if (t == 0) {
     rcount[blockIdx.x] = r_run;
     if (r_run > rcap) atomicMax(rneed, r_run);
@@ -536,7 +555,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restr
                                                              WItem *__restrict__ items, size_t cap) {
   __shared__ EmitLDS L;
   const int b = blockIdx.x, t = threadIdx.x;
-  if (t < NBUCKET) L.bk_run[t] = 0;
+  if (t < WNBUCKET) L.bk_run[t] = 0;
   __syncthreads();
   for (int part = 0; part < npart; part++) {
     const int slice = b * npart + part;
@@ -553,7 +572,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restr
         m1 = r.m1;
         m2 = r.m2;
       }
-      emit_batch(L, js, s, m1, m2, boff + b * NBUCKET, items, cap);
+      emit_batch(L, js, s, m1, m2, boff + b * WNBUCKET, items, cap);
     }
   }
 }
@@ -562,10 +581,10 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restr
 __global__ void k_bucket_sum(const int32_t *__restrict__ bpart, int nblk, int npart,
                              int32_t *__restrict__ bsum) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nblk * NBUCKET) return;
-  const int b = q / NBUCKET, k = q - b * NBUCKET;
+  if (q >= nblk * WNBUCKET) return;
+  const int b = q / WNBUCKET, k = q - b * WNBUCKET;
   int v = 0;
-  for (int p = 0; p < npart; p++) v += bpart[((size_t)b * npart + p) * NBUCKET + k];
+  for (int p = 0; p < npart; p++) v += bpart[((size_t)b * npart + p) * WNBUCKET + k];
   bsum[q] = v;
 }
 
@@ -611,13 +630,16 @@ __global__ void k_jfill(const WItem *__restrict__ items, const int32_t *__restri
 // its chunks, then (at the scanned offset) to write them.
 __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkPar P, int base,
                                  int32_t *__restrict__ chunks, int32_t *__restrict__ cpairs) {
-  // the block's bucket offsets, loaded once (independent loads); bucket k
-  // holds items of popcount WSPLIT - k
-  const int32_t *op = boff + (size_t)b * NBUCKET;
-  int o[NBUCKET + 1];
+  // the block's bucket offsets, loaded once (independent loads); bucket
+  // k < WSPLIT holds items of popcount WSPLIT - k, and (water lists)
+  // bucket WSPLIT + k the listwt2 items of popcount WSPLIT - k, counted as
+  // popcount / m2_div pairs (a listwt2 member costs a distance test unless
+  // it is within relx)
+  const int32_t *op = boff + (size_t)b * P.nbucket;
+  int o[WNBUCKET + 1];
 #pragma unroll
-  for (int k = 0; k <= NBUCKET; k++) o[k] = op[k];
-  const int end = o[NBUCKET];
+  for (int k = 0; k <= WNBUCKET; k++) o[k] = k <= P.nbucket ? op[k] : 0;
+  const int end = o[P.nbucket];
   int q = o[0], e = o[0], n = 0;
   long pairs = 0;
   auto emit = [&]() {
@@ -634,8 +656,9 @@ __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkP
   // a chunk grows while it holds fewer than P.pairs pairs or fewer than
   // P.min_items items, and never past P.max_items items
 #pragma unroll
-  for (int k = 0; k < NBUCKET; k++) {
-    const int pc = WSPLIT - k;
+  for (int k = 0; k < WNBUCKET; k++) {
+    // (buckets past P.nbucket have o = 0: no items)
+    const int pc = k < WSPLIT ? WSPLIT - k : max(1, (2 * WSPLIT - k) / P.m2_div);
     const int hi = min(o[k + 1], end);
     while (e < hi) {
       const long want = max(max(1L, (long)(P.min_items - (e - q))),
@@ -751,9 +774,9 @@ int32_t lists_init(moil_ctx *c) {
     c->nwblk = (c->nwat + WBLOCK - 1) / WBLOCK;
     // parts per block for the predicate pass: at least ~2048 workgroups
     c->wnpart = std::min(8, std::max(1, (2048 + c->nwblk - 1) / c->nwblk));
-    HIP_CHECK(hipMalloc(&c->d_bpart, sizeof(int32_t) * ((size_t)c->nwblk * c->wnpart * NBUCKET + 1)));
-    HIP_CHECK(hipMalloc(&c->d_bcount, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
-    HIP_CHECK(hipMalloc(&c->d_boff, sizeof(int32_t) * (c->nwblk * NBUCKET + 1)));
+    HIP_CHECK(hipMalloc(&c->d_bpart, sizeof(int32_t) * ((size_t)c->nwblk * c->wnpart * WNBUCKET + 1)));
+    HIP_CHECK(hipMalloc(&c->d_bcount, sizeof(int32_t) * (c->nwblk * WNBUCKET + 1)));
+    HIP_CHECK(hipMalloc(&c->d_boff, sizeof(int32_t) * (c->nwblk * WNBUCKET + 1)));
   }
   return MOIL_GPU_OK;
 }
@@ -876,9 +899,9 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
                                                          c->d_flags, c->d_bpart, c->d_wraw,
                                                          c->wraw_cap, c->d_rcount,
                                                          c->d_status + ST_WRAW, np);
-    k_bucket_sum<<<(c->nwblk * NBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_bpart, c->nwblk, np,
+    k_bucket_sum<<<(c->nwblk * WNBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_bpart, c->nwblk, np,
                                                                     c->d_bcount);
-    CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * NBUCKET, st));
+    CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * WNBUCKET, st));
     k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, np,
                                                     c->d_boff, c->d_witems, c->witem_cap);
     ChunkPar cp;
@@ -887,10 +910,12 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     cp.max_items = c->wchunk_max_items;
     cp.min_items = LB_THREADS;
     cp.cap = c->wchunk_cap;
+    cp.nbucket = WNBUCKET;
+    cp.m2_div = 8;
     CHECK_RC_L(chunk_table(c, c->d_boff, cp, c->d_wccount, c->d_cfirst, c->d_wchunk, c->d_wcpairs,
                            c->d_worder, st));
     // row ranges of the partial buffer by partner slot j
-    const int32_t *d_nitem = c->d_boff + (size_t)c->nwblk * NBUCKET;
+    const int32_t *d_nitem = c->d_boff + (size_t)c->nwblk * WNBUCKET;
     k_jcount<<<1024, TB, 0, st>>>(c->d_witems, d_nitem, c->witem_cap, c->d_status + ST_WRAW,
                                   c->d_jcnt);
     CHECK_SCAN(scan_excl(c, c->d_jcnt, c->d_jstart, c->nwat, st));

@@ -1,13 +1,15 @@
 // Force evaluation that allocates nothing (moil_gpu_eforce_noalloc_dev).
 //
-// The evaluation of moil_gpu_eforce_dev, in order on the caller's stream,
-// with its reduction scratch in a file-scope device array instead of the
-// context's buffers: the per-workgroup energy partial sums, the per-term
-// totals, the last-workgroup ticket of the energy reduction and the
-// fixed-point overflow word.  No graph is captured or replayed, no event is
-// created or recorded and nothing is allocated or freed during the call;
-// the context's own buffers (lists, accumulators, grids, transform plans)
-// were allocated by moil_gpu_init and moil_gpu_build_lists*.
+// The evaluation of moil_gpu_eforce_dev launched directly (never captured
+// or replayed as a graph; the reciprocal sum forks to the context's second
+// stream and joins back through the context's own events, created at
+// init), with its reduction scratch in a file-scope device array instead
+// of the context's buffers: the per-workgroup energy partial sums, the
+// per-term totals, the last-workgroup ticket of the energy reduction and
+// the fixed-point overflow word.  Nothing is allocated or freed during the
+// call; the context's own buffers (lists, accumulators, grids, transform
+// plans) were allocated by moil_gpu_init and moil_gpu_build_lists*, and
+// no timing events are taken.
 //
 // Layout of the array (SCRATCH_BYTES):
 //   [0, 8)        ticket, overflow word (unsigned int each), cleared by

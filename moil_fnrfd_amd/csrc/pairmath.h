@@ -55,9 +55,12 @@ __device__ __forceinline__ void fix_report(bool bad, int *ovf) {
   if (bad) atomicOr(ovf, 1);
 }
 
-// Each lane walks its block mask starting at its own lane number (bit b of
-// the rotated mask is block member (b + rot) mod 64), so lanes with similar
-// masks do not all add into the same LDS accumulators at once.
+// Each lane walks its item's block mask starting at a member given by the
+// item's index (bit b of the rotated mask is block member (b + rot) mod
+// 64): the consecutive items of a wavefront start at different members,
+// so lanes with similar masks do not all add into the same LDS
+// accumulators at once, and an item's sums run in the same order whichever
+// lane or chunk takes it.
 __device__ __forceinline__ uint64_t rotr64(uint64_t x, int r) {
   return r ? ((x >> r) | (x << (64 - r))) : x;
 }

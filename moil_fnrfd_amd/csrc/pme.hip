@@ -43,6 +43,22 @@
 
 #define PME_THREADS 256
 #define QFIX_SHIFT 40  // grid charge fixed point: 2^-40
+// round(v 2^40) for |v| < 2^11 as the bit pattern of v + 1.5 2^12 (whose
+// last mantissa bit is 2^-40) minus that of 1.5 2^12: one add and one
+// 64-bit subtract in place of a double-to-integer conversion.  Every
+// stencil weight q sqrt(kofdie/eps) theta1 theta2 theta3 is below
+// max |q| sqrt(kofdie/eps), which pme_init holds below 2^11.
+static_assert(QFIX_SHIFT == 40, "qfix's bias is that of 2^-40 units");
+#define QFIX_BIAS 6144.0  // 1.5 2^12
+#define QFIX_MAXW 2048.0
+__device__ __forceinline__ long long qfix(double v) {
+  return (long long)((unsigned long long)__double_as_longlong(v + QFIX_BIAS) -
+                     (unsigned long long)__double_as_longlong(QFIX_BIAS));
+}
+// The reciprocal-space kernels run beside the real-space pair kernel (on
+// the context's second stream) and form the evaluation's longer chain:
+// their waves take the SIMDs' issue slots first.
+#define PME_WAVE_PRIORITY() __builtin_amdgcn_s_setprio(2)
 // base points a tile's region accepts beyond the tile on each side, and
 // the largest region edge (tile <= 8 points + the stencil's 3 + margins)
 #define PME_MARGIN 1
@@ -269,6 +285,7 @@ DEFINE_KERNEL(k_pme_tplace, TplaceArgs, PME_THREADS, tplace_body)
 #define RB_GRID 0x40000000
 template <typename R>
 __device__ __forceinline__ void tspread_impl(const TspreadArgs &A, int bid) {
+  PME_WAVE_PRIORITY();
   __shared__ unsigned long long acc[PME_RMAX * PME_RMAX * PME_RMAX];
   __shared__ R rec[TS_STAGE * 12];
   __shared__ int32_t rb[TS_STAGE];
@@ -317,7 +334,7 @@ __device__ __forceinline__ void tspread_impl(const TspreadArgs &A, int bid) {
       const int bp = rb[a];
       const R *o = rec + 12 * a;
       const R v = o[8 + lc] * o[4 + lb] * o[la];
-      const long long f = __double2ll_rn((double)v * P.qscale);
+      const long long f = qfix((double)v);
       if (!(bp & RB_GRID)) {  // uniform over the wavefront
         const int l1 = (bp & 255) + la, l2 = ((bp >> 8) & 255) + lb, l3 = (bp >> 16) + lc;
         if (f) atomicAdd(&acc[(l3 * T.r2 + l2) * T.r1 + l1], (unsigned long long)f);
@@ -349,6 +366,7 @@ DEFINE_KERNEL(k_pme_tspread_f64, TspreadArgs, PME_THREADS, tspread_body<double>)
 // cleared for the next evaluation; converted to the transform's real type
 template <typename R>
 __device__ __forceinline__ void tsum_impl(const TsumArgs &A, int bid) {
+  PME_WAVE_PRIORITY();
   const TilePar &T = A.T;
   const size_t G = (size_t)T.n1 * T.n2 * T.n3;
   size_t g = (size_t)bid * PME_THREADS + threadIdx.x;
@@ -407,6 +425,7 @@ template <typename R>
 __device__ __forceinline__ void conv_impl(R *__restrict__ S, const double *__restrict__ eterm,
                                           int nc1, int n1, size_t H, double *__restrict__ epart,
                                           int bid) {
+  PME_WAVE_PRIORITY();
   __shared__ double red[PME_THREADS];
   size_t g = (size_t)bid * PME_THREADS + threadIdx.x;
   double e = 0.0;
@@ -522,6 +541,7 @@ __device__ __forceinline__ void gather_impl(const double *__restrict__ coor,
                                             const int32_t *__restrict__ sidx, const PmePar &P,
                                             const TilePar &T, const R *__restrict__ grid,
                                             double *__restrict__ fpme, int bid) {
+  PME_WAVE_PRIORITY();
   __shared__ R reg[PME_RMAX * PME_RMAX * PME_RMAX];
   const int tile = bid, t = threadIdx.x;
   const int tt1 = tile % T.nt1, tt2 = (tile / T.nt1) % T.nt2, tt3 = tile / (T.nt1 * T.nt2);
@@ -612,6 +632,50 @@ static const char *rf_err(rocfft_status s) {
     }                                                                 \
   } while (0)
 
+// Tiles of at most `edge` points a side (4..8): per dimension
+// nt = ceil(n / edge) tiles, fewer where a tile would hold fewer than 4
+// points (the halo sum reads the regions of a point's own tile and its two
+// neighbours only, which covers every region holding the point while
+// tiles are at least 4 wide); the tile buffers are sized to match.  The
+// atoms' tiles are assigned at the next list build.
+int32_t pme_set_tile(moil_ctx *c, int edge) {
+  if (edge < 4 || edge > 8) {
+    set_error("PME spreading tile edge must lie in 4..8");
+    return MOIL_GPU_EINVAL;
+  }
+  const int n[3] = {c->nf1, c->nf2, c->nf3};
+  int nt[3], r[3];
+  for (int d = 0; d < 3; d++) {
+    nt[d] = (n[d] + edge - 1) / edge;
+    while (nt[d] > 1 && n[d] / nt[d] < 4) nt[d]--;
+    r[d] = (n[d] + nt[d] - 1) / nt[d] + 3 + 2 * PME_MARGIN;
+    if (r[d] > PME_RMAX) {
+      set_error("PME grid dimension " + std::to_string(n[d]) + " has no tiling with edge " +
+                std::to_string(edge));
+      return MOIL_GPU_EINVAL;
+    }
+  }
+  if (c->d_tbuf && nt[0] == c->pt_nt[0] && nt[1] == c->pt_nt[1] && nt[2] == c->pt_nt[2]) {
+    c->pt_edge = edge;
+    return MOIL_GPU_OK;
+  }
+  hipFree(c->d_tcnt); c->d_tcnt = nullptr;
+  hipFree(c->d_tstart); c->d_tstart = nullptr;
+  hipFree(c->d_tbuf); c->d_tbuf = nullptr;
+  for (int d = 0; d < 3; d++) {
+    c->pt_nt[d] = nt[d];
+    c->pt_r[d] = r[d];
+  }
+  c->pt_ntile = nt[0] * nt[1] * nt[2];
+  c->pt_edge = edge;
+  const size_t rsz = (size_t)r[0] * r[1] * r[2];
+  HIP_CHECK(hipMalloc(&c->d_tcnt, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMemset(c->d_tcnt, 0, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMalloc(&c->d_tstart, sizeof(int32_t) * (c->pt_ntile + 1)));
+  HIP_CHECK(hipMalloc(&c->d_tbuf, sizeof(long long) * rsz * c->pt_ntile));
+  return MOIL_GPU_OK;
+}
+
 int32_t pme_init(moil_ctx *c) {
   const moil_params &P = c->par;
   c->nf1 = P.nfft1; c->nf2 = P.nfft2; c->nf3 = P.nfft3;
@@ -657,20 +721,12 @@ int32_t pme_init(moil_ctx *c) {
   const bool f32 = c->par.precision == MOIL_PREC_MIXED;
   const size_t rs = f32 ? sizeof(float) : sizeof(double);
   const int na = std::max(c->npt, 1);
-  // grid tiles for the spreading
+  // grid tiles for the spreading (8 points a side; the autotuner may
+  // choose smaller ones)
   {
-    const int n[3] = {c->nf1, c->nf2, c->nf3};
-    for (int d = 0; d < 3; d++) {
-      c->pt_nt[d] = (n[d] + 7) / 8;
-      c->pt_r[d] = (n[d] + c->pt_nt[d] - 1) / c->pt_nt[d] + 3 + 2 * PME_MARGIN;
-    }
-    c->pt_ntile = c->pt_nt[0] * c->pt_nt[1] * c->pt_nt[2];
+    const int32_t rc = pme_set_tile(c, 8);
+    if (rc != MOIL_GPU_OK) return rc;
   }
-  const size_t rsz = (size_t)c->pt_r[0] * c->pt_r[1] * c->pt_r[2];
-  HIP_CHECK(hipMalloc(&c->d_tcnt, sizeof(int32_t) * (c->pt_ntile + 1)));
-  HIP_CHECK(hipMemset(c->d_tcnt, 0, sizeof(int32_t) * (c->pt_ntile + 1)));
-  HIP_CHECK(hipMalloc(&c->d_tstart, sizeof(int32_t) * (c->pt_ntile + 1)));
-  HIP_CHECK(hipMalloc(&c->d_tbuf, sizeof(long long) * rsz * c->pt_ntile));
   HIP_CHECK(hipMalloc(&c->d_atile, sizeof(int32_t) * na));
   HIP_CHECK(hipMalloc(&c->d_arank, sizeof(int32_t) * na));
   HIP_CHECK(hipMalloc(&c->d_qover, sizeof(unsigned long long) * G));
@@ -726,6 +782,18 @@ void pme_free(moil_ctx *c) {
   hipFree(c->d_qreal); c->d_qreal = nullptr;
   hipFree(c->d_qcplx); c->d_qcplx = nullptr;
   hipFree(c->d_fpme); c->d_fpme = nullptr;
+}
+
+bool pme_charges_fit(const double *q, size_t n, double eps, std::string *why) {
+  const double fc = sqrt(KOFDIE / eps);
+  for (size_t i = 0; i < n; i++)
+    if (!(fabs(q[i]) * fc < QFIX_MAXW)) {
+      if (why)
+        *why = "charge " + std::to_string(q[i]) + " of atom " + std::to_string(i + 1) +
+               " beyond the reciprocal grid's fixed-point range (|q| sqrt(332.0716/eps) < 2048)";
+      return false;
+    }
+  return true;
 }
 
 int pme_conv_blocks(const moil_ctx *c) {

@@ -193,11 +193,13 @@ __device__ __forceinline__ void water_f32_impl(
   K.qcoo = K.qoo * c2b; K.qcoh = K.qoh * c2b; K.qchh = K.qhh * c2b;
   K.a12 = (float)P.oo12; K.b6 = (float)P.oo6;
   K.cutvdw2 = (float)P.cutvdw2; K.cutele2 = (float)P.cutele2;
-  const int rot = t & 63;
   bool bad = false;  // a term beyond the LDS sums' range (or a NaN)
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
   for (int it = beg + t; it < end; it += RS_THREADS) {
+    // the mask walk starts at a member given by the item, so the order of
+    // an item's sums does not depend on the lane or chunk that runs it
+    const int rot = it & 63;
     const WItem item = items[it];
     const int js = item.j;
     const float4 jA = wpos[3 * js], jB = wpos[3 * js + 1], jC = wpos[3 * js + 2];
@@ -218,7 +220,20 @@ __device__ __forceinline__ void water_f32_impl(
     for (int k = 0; k < 9; k++) fj[k] = 0.f;
     float elj = 0.f, eel = 0.f;
     uint64_t big = 0;  // members whose entry is beyond the bit-pattern conversion
-    uint64_t mask = rotr64(item.m1 | item.m2, rot);
+    // listwt2 members take part only within relx (pot/watwat_ewald.f:624+):
+    // their O-O distances first, in a cheap loop of their own, so that the
+    // full entries below run only for members that contribute (a member
+    // skipped inside the entry loop would idle its lane for a whole entry)
+    uint64_t act = item.m1;
+    for (uint64_t m2 = item.m2; m2; m2 &= m2 - 1) {
+      const int i = __ffsll((long long)m2) - 1;
+      const float4 A = iA[i], B = iB[i];
+      const float dx = (A.x - J.jhx) + (B.x - J.jlx);
+      const float dy = (A.y - J.jhy) + (B.y - J.jly);
+      const float dz = (A.z - J.jhz) + (B.z - J.jlz);
+      if (dx * dx + dy * dy + dz * dz <= K.cutele2) act |= 1ull << i;
+    }
+    uint64_t mask = rotr64(act, rot);
     while (mask) {
       const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
       const bool k2 = (item.m2 >> i) & 1ull;
@@ -314,10 +329,12 @@ __device__ __forceinline__ void water_f64_impl(
   }
   __syncthreads();
   const double fs = P.fix_scale;
-  const int rot = t & 63;
   bool bad = false;  // a term beyond the fixed-point range (or a NaN)
   double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
   for (int it = beg + t; it < end; it += RS_THREADS) {
+    // the mask walk starts at a member given by the item, so the order of
+    // an item's sums does not depend on the lane or chunk that runs it
+    const int rot = it & 63;
     const WItem item = items[it];
     const int js = item.j;
     double xj[9];

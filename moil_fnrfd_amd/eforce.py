@@ -71,6 +71,9 @@ def load_library():
     L.moil_gpu_dump_atom_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.moil_gpu_set_timing.argtypes = [vp, C.c_int32]
     L.moil_gpu_list_generation.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.moil_gpu_get_tuning.argtypes = [vp, C.POINTER(abi.MoilTuning)]
+    L.moil_gpu_set_tuning.argtypes = [vp, C.POINTER(abi.MoilTuning)]
+    L.moil_gpu_autotune.argtypes = [vp, vp, C.c_int32, C.POINTER(abi.MoilTuning)]
     L.moil_gpu_batch_create.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.moil_gpu_batch_eforce_dev.argtypes = [vp, vp, vp, vp, vp]
     L.moil_gpu_batch_destroy.argtypes = [vp]
@@ -102,7 +105,8 @@ def load_library():
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
                  'moil_gpu_squeeze', 'moil_gpu_build_lists', 'moil_gpu_eforce',
                  'moil_gpu_squeeze_dev', 'moil_gpu_build_lists_dev', 'moil_gpu_eforce_dev',
-                 'moil_gpu_eforce_noalloc_dev',
+                 'moil_gpu_eforce_noalloc_dev', 'moil_gpu_get_tuning', 'moil_gpu_set_tuning',
+                 'moil_gpu_autotune',
                  'moil_gpu_list_counts', 'moil_gpu_dump_water_pairs', 'moil_gpu_dump_atom_pairs',
                  'moil_gpu_set_timing', 'moil_gpu_stage_times', 'moil_gpu_list_generation', 'moil_gpu_batch_create',
                  'moil_gpu_batch_eforce_dev', 'moil_gpu_batch_destroy', 'moil_gpu_set_params',
@@ -253,6 +257,31 @@ class MoilGPU:
         g = C.c_int64()
         _check(load_library().moil_gpu_list_generation(self._h, C.byref(g)))
         return g.value
+
+    def get_tuning(self):
+        """The context's launch shapes (dict of moil_tuning's fields)."""
+        t = abi.MoilTuning()
+        _check(load_library().moil_gpu_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **shapes):
+        """Sets launch shapes (wchunk_pairs, achunk_pairs, water_wg_per_cu,
+        pme_tile); the others keep their values.  Chunk budgets and tiles
+        take effect at the next list build."""
+        t = abi.MoilTuning()
+        _check(load_library().moil_gpu_get_tuning(self._h, C.byref(t)))
+        for k, v in shapes.items():
+            setattr(t, k, int(v))
+        _check(load_library().moil_gpu_set_tuning(self._h, C.byref(t)))
+
+    def autotune_dev(self, d_coor: int, reps: int = 20):
+        """Times the candidate launch shapes on device coordinates, keeps
+        the fastest and rebuilds the lists; returns the chosen shapes with
+        the mean evaluation times before (us_default) and after (us_tuned)."""
+        t = abi.MoilTuning()
+        _check(load_library().moil_gpu_autotune(self._h, C.c_void_p(d_coor), int(reps),
+                                                C.byref(t)))
+        return t.as_dict()
 
     def set_timing(self, level=1, stride=1):
         """0 off, 1 real-space kernels only (no dispatch gaps; every

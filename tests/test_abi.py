@@ -63,6 +63,19 @@ def _expect_einval(topo, p, needle):
         assert needle in str(e), str(e)
 
 
+def test_init_rejects_charges_beyond_the_grid_range_without_gpu():
+    """The reciprocal grid's fixed-point conversion holds stencil weights
+    |q| sqrt(kofdie/eps) < 2^11: a neutral system holding charges of
+    +-120 e (weight 2187) is refused before any device call."""
+    from moil_fnrfd_amd import configs
+    s, cut, ew = configs.water216()
+    p = configs.params_for(s, cut, ew)
+    o1, o2 = 0, 3  # two water O atoms
+    q0 = (s.topo.ptchg[o1], s.topo.ptchg[o2])
+    s.topo.ptchg[o1], s.topo.ptchg[o2] = 120.0, -120.0 + q0[0] + q0[1]
+    _expect_einval(s.topo, p, 'fixed-point range')
+
+
 def test_init_rejects_out_of_range_indices_without_gpu():
     """Every index array is checked against 1..npt (1..totmon for the water
     list) and the CSR pointers for monotonicity before anything reads them
