@@ -147,11 +147,14 @@ __global__ void k_pad_slots(const int32_t *__restrict__ sslot_u, int nsol,
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup per solute block: evaluates MOIL's predicates once for
-// every candidate partner and direction, keeps the non-empty records
-// (partner, direction, list1/2/3 masks over the block) in candidate order
-// in the block's slice of `raw` (capacity rcap; the count is exact even
-// when the slice overflows), and counts the items per popcount bucket.
+// One workgroup per (solute block, part): evaluates MOIL's predicates once
+// for every candidate partner and direction, the part taking every
+// npart-th batch of its block's candidates (batches numbered over
+// directions and cells; several parts keep the GPU busy when the system
+// has few solute blocks), keeps the non-empty records (partner,
+// direction, list1/2/3 masks over the block) in candidate order in its
+// slice of `raw` (capacity rcap; the count is exact even when the slice
+// overflows), and counts the items per popcount bucket.
 // For each (block atom, partner) the distance test comes first: a pair
 // beyond max(rvbg, rebg) is in no list whatever the monomer prefilter and
 // the exclusions say, so those (with their memory lookups) are only
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
     const uint64_t *__restrict__ xmask, const int32_t *__restrict__ xfar,
     const int32_t *__restrict__ exc1, const int32_t *__restrict__ exc2, APar P,
     int32_t *__restrict__ bcount, AItem *__restrict__ raw, int rcap,
-    int32_t *__restrict__ rcount, int32_t *__restrict__ rneed) {
+    int32_t *__restrict__ rcount, int32_t *__restrict__ rneed, int npart) {
 #pragma clang fp contract(off)
   __shared__ double ix[WBLOCK][3], ic[WBLOCK][3];
   __shared__ int32_t iatom[WBLOCK], imono[WBLOCK], iflag[WBLOCK];
@@ -178,8 +181,10 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
   __shared__ int32_t rw[ABT / 64];
   __shared__ int32_t r_run;
 
-  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int b = blockIdx.x / npart, part = blockIdx.x - b * npart;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int s0 = b * WBLOCK;
+  int batch = 0;
   if (t < WBLOCK) {
     const int a = (s0 + t < P.nsol) ? sslot2atom[s0 + t] : -1;
     iatom[t] = a;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
   __syncthreads();
   const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
   const double reach2 = fmax(P.cutvbig2, P.cutebig2);
-  AItem *rblk = raw + (size_t)b * rcap;
+  AItem *rblk = raw + (size_t)blockIdx.x * rcap;
 
   for (int d = 0; d < ADIRS; d++) {
     int s;
@@ -291,6 +296,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
       __syncthreads();
       const int ncand = cellpre[AB_MAXCELLS];
       for (int fb = 0; fb < ncand; fb += ABT) {
+        if (batch++ % npart != part) continue;  // uniform over the workgroup
         const int f = fb + t;
         uint64_t m1 = 0, m2 = 0, m3 = 0;
         int ja = -1;
@@ -387,19 +393,32 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
     __syncthreads();
   }
   __syncthreads();
-  if (t < NBUCKET) bcount[b * NBUCKET + t] = bk_cnt[t];
+  if (t < NBUCKET) bcount[(size_t)blockIdx.x * NBUCKET + t] = bk_cnt[t];
   if (t == 0) {
-    rcount[b] = r_run;
+    rcount[blockIdx.x] = r_run;
     if (r_run > rcap) atomicMax(rneed, r_run);
   }
 }
 
+// item counts per block and bucket: the sum over the block's parts
+__global__ void k_abucket_sum(const int32_t *__restrict__ bpart, int nblk, int npart,
+                              int32_t *__restrict__ bsum) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nblk * NBUCKET) return;
+  const int b = q / NBUCKET, k = q - b * NBUCKET;
+  int v = 0;
+  for (int p = 0; p < npart; p++) v += bpart[((size_t)b * npart + p) * NBUCKET + k];
+  bsum[q] = v;
+}
+
 // One workgroup per solute block: reads the block's records back in order
-// and writes its items -- each record split into items of <= WSPLIT pairs
-// (consecutive mask bits), bucketed by popcount -- at boff + ranks taken in
-// thread order (wavefront scans and ballots), so the item order is fixed.
+// (part by part) and writes its items -- each record split into items of
+// <= WSPLIT pairs (consecutive mask bits), bucketed by popcount -- at boff
+// + ranks taken in thread order (wavefront scans and ballots), so the item
+// order is fixed.
 __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ raw, int rcap,
                                                       const int32_t *__restrict__ rcount,
+                                                      int npart,
                                                       const int32_t *__restrict__ boff,
                                                       AItem *__restrict__ items, size_t cap) {
   __shared__ int32_t bk_run[NBUCKET];
@@ -407,8 +426,10 @@ __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ 
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (t < NBUCKET) bk_run[t] = 0;
   __syncthreads();
-  const int n = min(rcount[b], rcap);
-  const AItem *rblk = raw + (size_t)b * rcap;
+  for (int part = 0; part < npart; part++) {
+  const int slice = b * npart + part;
+  const int n = min(rcount[slice], rcap);
+  const AItem *rblk = raw + (size_t)slice * rcap;
   for (int fb = 0; fb < n; fb += ABT) {
     const int f = fb + t;
     int ja = -1, d = 0;
@@ -481,6 +502,7 @@ __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ 
       bk_run[t] += tot;
     }
     __syncthreads();
+  }
   }
 }
 
@@ -775,13 +797,16 @@ int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
   HIP_CHECK(hipMalloc(&c->d_rowcnt, sizeof(int32_t) * (c->nrow + 1)));
   HIP_CHECK(hipMalloc(&c->d_rowoff, sizeof(int32_t) * (c->nrow + 1)));
   HIP_CHECK(hipMalloc(&c->d_sslot_of, sizeof(int32_t) * c->npt));
+  // parts per block for the predicate pass: at least ~2048 workgroups
+  c->anpart = std::min(8, std::max(1, (2048 + c->nablk - 1) / std::max(c->nablk, 1)));
+  HIP_CHECK(hipMalloc(&c->d_abpart, sizeof(int32_t) * ((size_t)c->nablk * c->anpart * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_abcount, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_aboff, sizeof(int32_t) * (c->nablk * NBUCKET + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajcnt, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajfill, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_ajstart, sizeof(int32_t) * (c->npt + 1)));
   HIP_CHECK(hipMalloc(&c->d_acfirst, sizeof(int32_t) * (c->nablk + 1)));
-  HIP_CHECK(hipMalloc(&c->d_arcount, sizeof(int32_t) * (c->nablk + 1)));
+  HIP_CHECK(hipMalloc(&c->d_arcount, sizeof(int32_t) * ((size_t)c->nablk * c->anpart + 1)));
   return MOIL_GPU_OK;
 }
 
@@ -791,7 +816,7 @@ void atoms_free(moil_ctx *c) {
                   c->d_sstart, c->d_sslot2atom, c->d_sslot_of, c->d_abcount, c->d_aboff,
                   c->d_aitems, c->d_achunk, c->d_ajpart, c->d_aipart, c->d_ajcnt, c->d_ajfill, c->d_ajstart,
                   c->d_ajpos, c->d_acfirst, c->d_araw, c->d_arcount, c->d_sslot_u,
-                  c->d_rowcnt, c->d_rowoff, c->d_account};
+                  c->d_rowcnt, c->d_rowoff, c->d_account, c->d_abpart};
   for (void *p : ptrs)
     if (p) hipFree(p);
 }
@@ -853,7 +878,7 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   // builds, overflows reported in the status words (ST_A*)
   if (!c->d_araw) {
     c->araw_cap = std::max(c->araw_cap, 1024);
-    HIP_CHECK(hipMalloc(&c->d_araw, sizeof(AItem) * (size_t)c->araw_cap * c->nablk));
+    HIP_CHECK(hipMalloc(&c->d_araw, sizeof(AItem) * (size_t)c->araw_cap * c->nablk * c->anpart));
   }
   if (!c->d_aitems) {
     c->aitem_cap = std::max<size_t>(c->aitem_cap, 4096);
@@ -866,12 +891,15 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     HIP_CHECK(hipMalloc(&c->d_achunk, sizeof(int32_t) * 3 * (size_t)c->achunk_cap));
   }
   if (!c->d_account) HIP_CHECK(hipMalloc(&c->d_account, sizeof(int32_t) * (c->nablk + 1)));
-  k_ablock_raw<<<c->nablk, ABT, 0, st>>>(
+  const int np = c->anpart;
+  k_ablock_raw<<<c->nablk * np, ABT, 0, st>>>(
       d_coor, c->d_sslot2atom, c->d_aslot2atom, c->d_astart, g, c->d_mono_of, c->d_aflag,
-      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abcount,
-      c->d_araw, c->araw_cap, c->d_arcount, c->d_status + ST_ARAW);
+      c->d_mono_cent, c->d_flags, c->d_xmask, c->d_xfar, c->d_exc1, c->d_exc2, ap, c->d_abpart,
+      c->d_araw, c->araw_cap, c->d_arcount, c->d_status + ST_ARAW, np);
+  k_abucket_sum<<<(c->nablk * NBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_abpart, c->nablk, np,
+                                                                  c->d_abcount);
   CHECK_SCAN(scan_excl(c, c->d_abcount, c->d_aboff, c->nablk * NBUCKET, st));
-  k_ablock_write<<<c->nablk, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, c->d_aboff,
+  k_ablock_write<<<c->nablk, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, np, c->d_aboff,
                                            c->d_aitems, c->aitem_cap);
   ChunkPar cp;
   cp.nblk = c->nablk;

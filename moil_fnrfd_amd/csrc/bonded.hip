@@ -349,6 +349,9 @@ __device__ __forceinline__ void water_intra_impl(const double *__restrict__ coor
                                                  const BPar &P, long long *__restrict__ facc,
                                                  double *__restrict__ ep, int mp, int part0,
                                                  int bid) {
+  // it runs on the second stream beside the water pair kernel, ahead of
+  // the reciprocal sum: its waves take the SIMDs' issue slots first
+  __builtin_amdgcn_s_setprio(2);
   __shared__ double red[WIT];
   const int w = bid * WIT + threadIdx.x;
   double e[3] = {0, 0, 0};  // bond theta corr

@@ -156,10 +156,12 @@ struct moil_ctx {
   int32_t *d_acnt = nullptr;  // [4][ancell+1] cell counters of the four counting passes
   int32_t *d_sstart = nullptr, *d_sslot2atom = nullptr, *d_sslot_of = nullptr;  // solute slots
   int nablk = 0;                         // solute blocks of WBLOCK slots
+  int anpart = 1;                        // parts of a block's atom-list predicate pass
+  int32_t *d_abpart = nullptr;           // [nablk][anpart][NBUCKET] item counts per part
   int32_t *d_abcount = nullptr, *d_aboff = nullptr;
-  AItem *d_araw = nullptr;               // [nablk][araw_cap] raw records of the predicate pass
+  AItem *d_araw = nullptr;               // [nablk][anpart][araw_cap] raw records of the predicate pass
   int araw_cap = 0;
-  int32_t *d_arcount = nullptr;          // [nablk] records per block
+  int32_t *d_arcount = nullptr;          // [nablk][anpart] records per block and part
   int32_t *d_account = nullptr;          // [nablk+1] chunks per block
   AItem *d_aitems = nullptr;
   size_t aitem_cap = 0, naitems = 0;

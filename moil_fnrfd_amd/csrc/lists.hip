@@ -334,8 +334,10 @@ __device__ inline void emit_kind(EmitLDS &L, int kind, int js, int s, uint64_t m
 __device__ inline void emit_batch(EmitLDS &L, int js, int s, uint64_t m1, uint64_t m2,
                                   const int32_t *__restrict__ boff_b, WItem *__restrict__ items,
                                   size_t cap) {
-  emit_kind(L, 0, js, s, m1, boff_b, items, cap);
-  emit_kind(L, 1, js, s, m2, boff_b, items, cap);
+  // a kind no thread of the batch holds is skipped (uniform over the
+  // workgroup)
+  if (__syncthreads_or(m1 != 0)) emit_kind(L, 0, js, s, m1, boff_b, items, cap);
+  if (__syncthreads_or(m2 != 0)) emit_kind(L, 1, js, s, m2, boff_b, items, cap);
 }
 
 // items of one kind's mask per bucket (counts only: order independent)

@@ -170,6 +170,9 @@ __device__ __forceinline__ void water_f32_impl(
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ unsigned int icnt[WBLOCK];
   __shared__ double red[RS_THREADS];
+  // each thread's FP64 energy sums (LJ / Coulomb, real / image items), in
+  // LDS: in registers they cost the entry loop its registers
+  __shared__ double eacc[4][RS_THREADS];
   const int ch = order[bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
@@ -182,6 +185,8 @@ __device__ __forceinline__ void water_f32_impl(
   }
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) iacc[u] = 0;
   if (t < WBLOCK) icnt[t] = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) eacc[k][t] = 0.0;
   __syncthreads();
 
   // exp(-beta^2 r^2) = exp2(-beta^2 log2(e) r^2), t = 1 / (1 + p beta r)
@@ -194,7 +199,6 @@ __device__ __forceinline__ void water_f32_impl(
   K.a12 = (float)P.oo12; K.b6 = (float)P.oo6;
   K.cutvdw2 = (float)P.cutvdw2; K.cutele2 = (float)P.cutele2;
   bool bad = false;  // a term beyond the LDS sums' range (or a NaN)
-  double e_lj_r = 0, e_el_r = 0, e_lj_i = 0, e_el_i = 0;
 
   for (int it = beg + t; it < end; it += RS_THREADS) {
     // the mask walk starts at a member given by the item, so the order of
@@ -277,13 +281,9 @@ __device__ __forceinline__ void water_f32_impl(
     float *jrow = reinterpret_cast<float *>(jpart) + (size_t)jpos[it] * 9;
 #pragma unroll
     for (int k = 0; k < 9; k++) jrow[k] = fj[k];
-    if (item.s == 0) {
-      e_lj_r += (double)elj;
-      e_el_r += (double)eel;
-    } else {
-      e_lj_i += (double)elj;
-      e_el_i += (double)eel;
-    }
+    const int es = item.s == 0 ? 0 : 2;
+    eacc[es][t] += (double)elj;
+    eacc[es + 1][t] += (double)eel;
   }
   fix_report(bad, P.ovf);
   __syncthreads();
@@ -294,13 +294,13 @@ __device__ __forceinline__ void water_f32_impl(
         (long long)(iacc[k * WBLOCK + i] - (unsigned long long)icnt[i] * kBias32);
   }
   double v;
-  v = block_sum(e_lj_r, red);
+  v = block_sum(eacc[0][t], red);
   if (t == 0) epart[EN_VDW * max_parts + ch] = v;
-  v = block_sum(e_el_r, red);
+  v = block_sum(eacc[1][t], red);
   if (t == 0) epart[EN_EL * max_parts + ch] = v;
-  v = block_sum(e_lj_i, red);
+  v = block_sum(eacc[2][t], red);
   if (t == 0) epart[EN_VSYM * max_parts + ch] = v;
-  v = block_sum(e_el_i, red);
+  v = block_sum(eacc[3][t], red);
   if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
 }
 
