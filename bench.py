@@ -270,7 +270,7 @@ def chain_leg(steps=5, igrid=32):
     }
 
 
-def timed_loop(sysm, cut, ew, steps, warmup, seed, tune=False):
+def timed_loop(sysm, cut, ew, steps, warmup, seed, tune=False, shapes=None):
     """The metric's loop on one system (mixed precision, coordinates and
     outputs resident in HBM): per step N(0, 0.01 A) noise added to the
     coordinates (cumulative; drawn on the device NLIST steps at a time),
@@ -301,7 +301,15 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed, tune=False):
     # launch shapes from the library's autotuner (before the timed region;
     # the shapes change no force, only how the evaluation maps onto the GPU)
     torch.cuda.synchronize()  # the tuner runs on the context's own stream
-    tuning = eng.autotune_dev(coor.data_ptr(), reps=10) if tune else eng.get_tuning()
+    if shapes:
+        # fixed launch shapes (profiled runs): set, then the lists rebuilt
+        eng.set_tuning(**shapes)
+        eng.squeeze_dev(coor.data_ptr(), sh)
+        eng.nbondm_dev(coor.data_ptr(), sh)
+        torch.cuda.synchronize()
+        tuning = eng.get_tuning()
+    else:
+        tuning = eng.autotune_dev(coor.data_ptr(), reps=10) if tune else eng.get_tuning()
 
     def step(k):
         if k % NLIST == 0:
@@ -565,6 +573,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-tune', action='store_true',
                     help='keep the default launch shapes instead of autotuning them')
+    ap.add_argument('--shapes', default='',
+                    help='fixed launch shapes instead of the autotuner, e.g. '
+                         'wchunk_pairs=1024,water_wg_per_cu=4 (profiled runs)')
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the batched/plain leg (96 systems)')
     ap.add_argument('--batched-only', action='store_true',
@@ -615,7 +626,9 @@ def main():
         return
     sysm, cut, ew = configs.water32k()
     _progress('main loop: water32k, %d steps after %d warm-up' % (args.steps, args.warmup))
-    r = timed_loop(sysm, cut, ew, args.steps, args.warmup, seed=1001, tune=not args.no_tune)
+    shapes = dict((kv.split('=')[0], int(kv.split('=')[1])) for kv in args.shapes.split(',') if kv)
+    r = timed_loop(sysm, cut, ew, args.steps, args.warmup, seed=1001, tune=not args.no_tune,
+                   shapes=shapes)
     evals_per_s = args.steps / r['elapsed']
     n_pairs = r['n_pairs']
     t_rs_ms = r['t_rs_ms']

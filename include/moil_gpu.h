@@ -386,6 +386,61 @@ int32_t moil_gpu_param_grad(moil_ctx *ctx, const double *coor, double *dq, doubl
 int32_t moil_gpu_param_grad_dev(moil_ctx *ctx, const double *d_coor, double *d_dq,
                                 double *d_de12, double *d_de6, void *stream);
 
+/* The molecular virial of MOIL's constant-pressure path (pressON,
+ * COMMON/PRESS.BLOCK): the terms eforce adds up when pressON is set
+ * (pot/eforce.f:70-85 zeroes them), in MOIL's names and order.  Real
+ * space, for every listed pair with r = x_i - x_j and molecular
+ * separation R = M_i - M_j of the two atoms' molecules' centres of mass
+ * (each translated with its atom for an image pair):
+ *   virial, virXX..ZZ  from the LJ part df of the pair's force factor:
+ *                      virXX -= df rx Rx, virial -= df (r . R)
+ *                      (pot/cdie_ewald.f:440-460, pot/watwat_ewald.f:162-174,
+ *                      symm/symcdie_ewald.f:150-166, symm/symwat_ewald.f:160-173);
+ *   V_PIdirXX..ZZ      V_PIdir rx Rx, V_PIdir = qq (erfc(beta r)/r^3 +
+ *                      2 beta/sqrt(pi) exp(-beta^2 r^2)/r^2) for every
+ *                      Coulomb pair and water site pair (pot/cdie_ewald.f:351-368);
+ *   V_PIrecXX..ZZ      the reciprocal sum's sum over k of E(k) (1 - 2 (1 +
+ *                      pi^2 m^2/beta^2) m_x^2/m^2) (pot/ewald_receip.f:799-807);
+ *   V_PIrelXX..ZZ      the reciprocal force on every atom times its
+ *                      position relative to its molecule's centre
+ *                      (pot/ewald_receip.f:929-933).
+ * The exclusion correction, 1-4 pairs and bonded terms add nothing
+ * (MOIL's correction term is disabled, pot/cdie_ewald.f:162-171).
+ * The real-space terms are evaluated in FP64 with MOIL's erfc table in
+ * either precision mode; the reciprocal ones in the context's precision. */
+#define MOIL_N_VIRIAL 13
+typedef struct moil_virial {
+  double virial, virXX, virYY, virZZ;
+  double V_PIdirXX, V_PIdirYY, V_PIdirZZ;
+  double V_PIrecXX, V_PIrecYY, V_PIrecZZ;
+  double V_PIrelXX, V_PIrelYY, V_PIrelZZ;
+} moil_virial;
+
+/* Molecules for the virial (PRESS.BLOCK as MOIL's init_pressure fills it,
+ * analysis/init_pressure.f:1-77): nmol molecules; patom[m] (1-based) the
+ * position in ordlist of molecule m+1's last atom; ordlist the atoms
+ * molecule by molecule; pmol[i] the molecule of atom i+1; ptms the atom
+ * masses (CONNECT.BLOCK).  The molecular masses are summed in ordlist order
+ * (compute_massmol, analysis/init_pressure.f:411-428).  nmol = 0 removes
+ * the molecules.  Refused (MOIL_GPU_EINVAL): an atom in no molecule or in
+ * one out of range, ordlist and pmol disagreeing, a molecule of mass <= 0. */
+int32_t moil_gpu_set_molecules(moil_ctx *ctx, int32_t nmol, const int32_t *patom,
+                               const int32_t *ordlist, const int32_t *pmol, const double *ptms);
+/* MOIL's molecule division (create_mol_BFS, analysis/init_pressure.f:79-187)
+ * for callers without MOIL: patom and pmol hold npt entries, ordlist npt.
+ * Host only. */
+int32_t moil_molecules(const moil_topology *topo, int32_t *nmol, int32_t *patom,
+                       int32_t *ordlist, int32_t *pmol);
+/* eforce with pressON (pot/eforce.f:1-325 with the virial terms above):
+ * energies, the gradient and the virial.  The molecules' centres of mass
+ * are computed from coor first (the coordinate part of atom2molecule,
+ * analysis/pressure.f:419-505).  Requires moil_gpu_set_molecules.  The
+ * _dev form writes MOIL_N_VIRIAL float64 (moil_virial order) to d_virial. */
+int32_t moil_gpu_eforce_virial(moil_ctx *ctx, const double *coor, double *dpot,
+                               moil_energies *e, moil_virial *v);
+int32_t moil_gpu_eforce_virial_dev(moil_ctx *ctx, const double *d_coor, double *d_dpot,
+                                   double *d_energies, double *d_virial, void *stream);
+
 /* List inspection for parity tests.  counts[0..7]: water pairs real
  * listwt1, real listwt2, image listwt1, image listwt2; atom pairs list1,
  * list2, list3 (real+image), total image atom pairs. */

@@ -25,6 +25,10 @@ ENERGY_NAMES = ('e_total', 'e_bond', 'e_theta', 'e_tors', 'e_imp', 'e_vdw', 'e_e
                 'e_vsym', 'e_lsym', 'e_el14', 'e_vdw14', 'e_corr', 'e_self',
                 'e_ew_receip', 'e_dir')
 
+N_VIRIAL = 13
+VIRIAL_NAMES = ('virial', 'virXX', 'virYY', 'virZZ', 'V_PIdirXX', 'V_PIdirYY', 'V_PIdirZZ',
+                'V_PIrecXX', 'V_PIrecYY', 'V_PIrecZZ', 'V_PIrelXX', 'V_PIrelYY', 'V_PIrelZZ')
+
 _pi = C.POINTER(C.c_int32)
 _pd = C.POINTER(C.c_double)
 
@@ -57,6 +61,13 @@ class MoilParams(C.Structure):
         ('nfft1', C.c_int32), ('nfft2', C.c_int32), ('nfft3', C.c_int32),
         ('order', C.c_int32), ('precision', C.c_int32), ('terms', C.c_uint32),
     ]
+
+
+class MoilVirial(C.Structure):
+    _fields_ = [(n, C.c_double) for n in VIRIAL_NAMES]
+
+    def as_array(self):
+        return np.array([getattr(self, n) for n in VIRIAL_NAMES])
 
 
 class MoilEnergies(C.Structure):

@@ -45,10 +45,12 @@ static int32_t upload(T **dst, const T *src, size_t n) {
 }
 
 void egraph_drop(moil_ctx *c);
+void virial_free(moil_ctx *c);
 
 static void ctx_free(moil_ctx *c) {
   if (!c) return;
   egraph_drop(c);
+  virial_free(c);
   lists_free(c);
   atoms_free(c);
   pme_free(c);
@@ -896,7 +898,11 @@ void eval_plan(moil_ctx *c, const double *d_coor, double *d_dpot, double *d_ener
 // the evaluation's start, so they wait only for the coordinates) and run
 // beside them; the bonded terms and the partial-row reductions follow on
 // st.  The intra-water terms store (not add) the water atoms' accumulator
-// rows, so both reductions wait for them.
+// rows, so both reductions wait for them.  (A third stream per context
+// for the intra-water terms was measured: no faster here, and with four
+// hardware queues per process it put the main streams of all contexts on
+// one queue, which serialised the side-by-side list builds of many
+// contexts: chain call 6 -> 17 ms.)
 int32_t eforce_eager(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   // level 1 times every timing_stride-th evaluation (dispatch events cost
   // ~2 % of the evaluation rate when recorded on every one)

@@ -58,10 +58,6 @@ struct APar : BoxPar {
   int nsol;
 };
 
-__device__ inline void dir_of(int d, int &s, bool &rev) {
-  rev = d > 13;
-  s = rev ? d - 13 : d;
-}
 
 // exclusion test for atoms lo < hi (0-based): partners up to lo+64 from
 // the bit mask, farther ones from MOIL's exclusion list (exc1/exc2)
@@ -538,15 +534,6 @@ __global__ void k_ajfill(const AItem *__restrict__ items, const int32_t *__restr
 // ---------------------------------------------------------------------------
 // real-space energy over the atom items
 
-__device__ inline double a_block_sum(double v, double *red) {
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  __syncthreads();
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  return (red[0] + red[1]) + (red[2] + red[3]);
-}
-
 template <bool F64>
 __device__ __forceinline__ void atoms_impl(
     const int32_t *__restrict__ chunks, const AItem *__restrict__ items,
@@ -558,7 +545,7 @@ __device__ __forceinline__ void atoms_impl(
   __shared__ double ix[WBLOCK][3];
   __shared__ double iq[WBLOCK], ia12[WBLOCK], ib6[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 3];
-  __shared__ double red[4];
+  __shared__ double red[16];
   const int ch = bid, t = threadIdx.x;
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
@@ -692,15 +679,9 @@ __device__ __forceinline__ void atoms_impl(
   fix_report(bad, P.ovf);
   __syncthreads();
   for (int u = t; u < WBLOCK * 3; u += ABT) ipart[(size_t)ch * (WBLOCK * 3) + u] = (long long)iacc[u];
-  double v;
-  v = a_block_sum(ev_r, red);
-  if (t == 0) epart[EN_VDW * max_parts + part0 + ch] = v;
-  v = a_block_sum(ee_r, red);
-  if (t == 0) epart[EN_EL * max_parts + part0 + ch] = v;
-  v = a_block_sum(ev_i, red);
-  if (t == 0) epart[EN_VSYM * max_parts + part0 + ch] = v;
-  v = a_block_sum(ee_i, red);
-  if (t == 0) epart[EN_LSYM * max_parts + part0 + ch] = v;
+  const double v = wg_sum4(ev_r, ee_r, ev_i, ee_i, red);
+  if (t < 4) epart[(t == 0 ? EN_VDW : t == 1 ? EN_EL : t == 2 ? EN_VSYM : EN_LSYM) * max_parts +
+                   part0 + ch] = v;
 }
 
 template <bool F64>

@@ -88,6 +88,14 @@ struct __attribute__((aligned(32))) AItem {
 
 void set_error(const std::string &msg);
 
+// an atom-list item's direction d: 0 real, 1..13 the block atom is the
+// image under symmetry operation s = d, 14..26 the partner is the image
+// under s = d - 13 (rev)
+__host__ __device__ inline void dir_of(int d, int &s, bool &rev) {
+  rev = d > 13;
+  s = rev ? d - 13 : d;
+}
+
 // MOIL's 13 half-shell symmetry operations in its order
 // (symm/syminit.f:80-164); s = 0 is the untranslated (real) cell.
 __host__ __device__ inline void symop_of(int s, int *ia, int *jb, int *kc) {

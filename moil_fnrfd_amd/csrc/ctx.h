@@ -213,6 +213,15 @@ struct moil_ctx {
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
   long timed_gen = -1;             // list generation of the last timed in-order call
   long param_gen = 0;              // incremented by set_params
+  // ---- molecular virial (virial.hip) ----
+  int nmol = 0;                    // molecules (PRESS.BLOCK); 0: none set
+  int32_t *d_patom = nullptr, *d_ordlist = nullptr;  // [nmol], [npt] (1-based values)
+  double *d_ptms = nullptr, *d_massmol = nullptr;    // [npt], [nmol]
+  double *d_acom = nullptr;        // [3*npt] centre of mass of each atom's molecule
+  double *d_vpart = nullptr;       // [7][vpart_cap] real-space virial partials per chunk
+  int vpart_cap = 0;
+  double *d_vrec = nullptr, *d_vrel = nullptr;  // [3][blocks] reciprocal partials
+  double *d_virial = nullptr;      // [MOIL_N_VIRIAL] host-pointer call's result
 };
 
 // exclusive scan of n int32 into out[0..n] (out[n] = total), on stream st

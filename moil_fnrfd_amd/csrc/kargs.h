@@ -253,7 +253,15 @@ struct EvalPlan {
   int n_fin = 0;
   EnArgs en;
   AsmArgs asmb;  // fin + en in one launch of n_fin + EN_COUNT workgroups
+  // the reciprocal virial terms (moil_gpu_eforce_virial_dev): V_PIrec per
+  // k after the forward transform, V_PIrel per atom after the gather
+  bool vir = false;
+  const double *acom = nullptr;
+  double *vrec = nullptr, *vrel = nullptr;
 };
+// the reciprocal virial kernels (virial.hip), launched by pme_run
+void launch_pme_virial(const moil_ctx *c, const EvalPlan &p, hipStream_t st);
+void launch_rel_virial(const moil_ctx *c, const EvalPlan &p, hipStream_t st);
 
 // plan builders (one per module; the caller fills d_coor / outputs)
 void pme_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p);

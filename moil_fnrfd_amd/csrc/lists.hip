@@ -359,6 +359,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   __shared__ double ic[WBLOCK][3];
   __shared__ int32_t im[WBLOCK];
   __shared__ double bb[6];
+  __shared__ double sbb[WBLOCK / 8][6];  // bounding boxes of the 8-water sub-blocks
   __shared__ uint64_t imask;
   __shared__ int32_t cellk[LB_MAXCELLS];
   __shared__ int32_t cellpre[LB_MAXCELLS + 1];
@@ -398,9 +399,24 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
       bb[k] = lo[k];
       bb[3 + k] = hi[k];
     }
+  } else if (t >= 64 && t < 64 + WBLOCK / 8) {
+    // sub-block u: block members 8u..8u+7 (consecutive slots in cell order,
+    // about one cell); an empty one gets a box no candidate is near
+    const int u = t - 64;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 8 * u; i < min(8 * u + 8, nb); i++)
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fmin(lo[k], ic[i][k]);
+        hi[k] = fmax(hi[k], ic[i][k]);
+      }
+    for (int k = 0; k < 3; k++) {
+      sbb[u][k] = lo[k];
+      sbb[u][3 + k] = hi[k];
+    }
   }
   __syncthreads();
   const double large_cutmono = flags[0] ? P.cutmono2 * 4.0 : P.cutmono2;
+  const double R2 = P.R * P.R;
   WRaw *rblk = raw + (size_t)blockIdx.x * rcap;
 
   for (int s = 0; s < 14; s++) {
@@ -493,17 +509,38 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
             const double ex = fmax(fmax(q0[0] + P.R - cjx, cjx - (q1[0] - P.R)), 0.0);
             const double ey = fmax(fmax(q0[1] + P.R - cjy, cjy - (q1[1] - P.R)), 0.0);
             const double ez = fmax(fmax(q0[2] + P.R - cjz, cjz - (q1[2] - P.R)), 0.0);
-            if (ex * ex + ey * ey + ez * ez > P.R * P.R) jok = false;
+            if (ex * ex + ey * ey + ez * ez > R2) jok = false;
           }
           if (jok) {
-            uint64_t iv = ivalid;
-            while (iv) {
-              int i = __ffsll((long long)iv) - 1;
-              iv &= iv - 1;
-              if (!image && !(im[i] < jm)) continue;
-              int kd = wpair_kind(ic[i][0], ic[i][1], ic[i][2], da, db, dc, cjx, cjy, cjz, image, P);
-              if (kd == 1) m1 |= (1ull << i);
-              else if (kd == 2) m2 |= (1ull << i);
+            // the same test per sub-block: only the members of sub-blocks
+            // within R of j are tested below (a member beyond R is no
+            // pair: every listed pair has r2 <= R^2, R carrying a margin
+            // over the rounding of these distances)
+            uint64_t near = 0;
+#pragma unroll
+            for (int u = 0; u < WBLOCK / 8; u++) {
+              const double ex = fmax(fmax((sbb[u][0] + da) - cjx, cjx - (sbb[u][3] + da)), 0.0);
+              const double ey = fmax(fmax((sbb[u][1] + db) - cjy, cjy - (sbb[u][4] + db)), 0.0);
+              const double ez = fmax(fmax((sbb[u][2] + dc) - cjz, cjz - (sbb[u][5] + dc)), 0.0);
+              if (ex * ex + ey * ey + ez * ez <= R2) near |= 0xffull << (8 * u);
+            }
+            const uint64_t iv = ivalid & near;
+            // eight members at a time, their centres and monomers loaded
+            // together (the tests of one sub-block are independent), so a
+            // lane waits on one round of LDS reads per sub-block instead of
+            // one per member
+            for (int u = 0; u < WBLOCK / 8; u++) {
+              if (((iv >> (8 * u)) & 0xffull) == 0) continue;
+#pragma unroll
+              for (int q = 0; q < 8; q++) {
+                const int i = 8 * u + q;
+                const double cx = ic[i][0], cy = ic[i][1], cz = ic[i][2];
+                const int mi = im[i];
+                if (!((iv >> i) & 1ull) || (!image && !(mi < jm))) continue;
+                const int kd = wpair_kind(cx, cy, cz, da, db, dc, cjx, cjy, cjz, image, P);
+                if (kd == 1) m1 |= (1ull << i);
+                else if (kd == 2) m2 |= (1ull << i);
+              }
             }
           }
         }
@@ -628,24 +665,29 @@ __global__ void k_jfill(const WItem *__restrict__ items, const int32_t *__restri
 // Chunk tables on the device.  A chunk (one workgroup of the pair kernel)
 // is a run of one block's items (popcount-descending buckets, boff) holding
 // about `pairs` pairs, at least `min_items` items (one per lane) and at
-// most `max_items`.  One thread per block walks its items twice: to count
-// its chunks, then (at the scanned offset) to write them.
+// most `max_items`.  One wavefront per block walks its items twice: to
+// count its chunks, then (at the scanned offset) to write them.  Lane k
+// holds the block's bucket offset k and the walk reads them with
+// readlane, so every lane runs the same (uniform) walk and lane 0 writes.
+static_assert(WNBUCKET <= 64, "a wavefront holds one bucket offset per lane");
 __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkPar P, int base,
                                  int32_t *__restrict__ chunks, int32_t *__restrict__ cpairs) {
-  // the block's bucket offsets, loaded once (independent loads); bucket
-  // k < WSPLIT holds items of popcount WSPLIT - k, and (water lists)
-  // bucket WSPLIT + k the listwt2 items of popcount WSPLIT - k, counted as
-  // popcount / m2_div pairs (a listwt2 member costs a distance test unless
-  // it is within relx)
+  // bucket k < WSPLIT holds items of popcount WSPLIT - k, and (water
+  // lists) bucket WSPLIT + k the listwt2 items of popcount WSPLIT - k,
+  // counted as popcount / m2_div pairs (a listwt2 member costs a distance
+  // test unless it is within relx)
+  const int lane = threadIdx.x & 63;
   const int32_t *op = boff + (size_t)b * P.nbucket;
-  int o[WNBUCKET + 1];
-#pragma unroll
-  for (int k = 0; k <= WNBUCKET; k++) o[k] = k <= P.nbucket ? op[k] : 0;
-  const int end = o[P.nbucket];
-  int q = o[0], e = o[0], n = 0;
+  const int ol = lane <= P.nbucket ? op[lane] : 0;
+  const int end = op[P.nbucket];
+  // offset k (0 past P.nbucket)
+  auto O = [&](int k) -> int {
+    return k < 64 ? __builtin_amdgcn_readlane(ol, k) : (k <= P.nbucket ? end : 0);
+  };
+  int q = O(0), e = q, n = 0;
   long pairs = 0;
   auto emit = [&]() {
-    if (chunks && base + n < P.cap) {
+    if (chunks && lane == 0 && base + n < P.cap) {
       chunks[3 * (size_t)(base + n)] = b;
       chunks[3 * (size_t)(base + n) + 1] = q;
       chunks[3 * (size_t)(base + n) + 2] = e;
@@ -657,11 +699,10 @@ __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkP
   };
   // a chunk grows while it holds fewer than P.pairs pairs or fewer than
   // P.min_items items, and never past P.max_items items
-#pragma unroll
   for (int k = 0; k < WNBUCKET; k++) {
-    // (buckets past P.nbucket have o = 0: no items)
+    // (buckets past P.nbucket have offset 0: no items)
     const int pc = k < WSPLIT ? WSPLIT - k : max(1, (2 * WSPLIT - k) / P.m2_div);
-    const int hi = min(o[k + 1], end);
+    const int hi = min(O(k + 1), end);
     while (e < hi) {
       const long want = max(max(1L, (long)(P.min_items - (e - q))),
                             (P.pairs - pairs + pc - 1) / pc);
@@ -676,17 +717,21 @@ __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkP
   return n;
 }
 
-__global__ void k_chunk_count(const int32_t *__restrict__ boff, ChunkPar P,
-                              int32_t *__restrict__ ccount) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < P.nblk) ccount[b] = chunk_walk(boff, b, P, 0, nullptr, nullptr);
+__global__ __launch_bounds__(256) void k_chunk_count(const int32_t *__restrict__ boff, ChunkPar P,
+                                                     int32_t *__restrict__ ccount) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= P.nblk) return;  // uniform over the wavefront
+  const int n = chunk_walk(boff, b, P, 0, nullptr, nullptr);
+  if ((threadIdx.x & 63) == 0) ccount[b] = n;
 }
 
-__global__ void k_chunk_write(const int32_t *__restrict__ boff, const int32_t *__restrict__ cfirst,
-                              ChunkPar P, int32_t *__restrict__ chunks,
-                              int32_t *__restrict__ cpairs) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < P.nblk) chunk_walk(boff, b, P, cfirst[b], chunks, cpairs);
+__global__ __launch_bounds__(256) void k_chunk_write(const int32_t *__restrict__ boff,
+                                                     const int32_t *__restrict__ cfirst,
+                                                     ChunkPar P, int32_t *__restrict__ chunks,
+                                                     int32_t *__restrict__ cpairs) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= P.nblk) return;
+  chunk_walk(boff, b, P, cfirst[b], chunks, cpairs);
 }
 
 // dispatch order, largest chunks first: counting sort on the pair count in
@@ -722,9 +767,10 @@ int32_t chunk_table(moil_ctx *c, const int32_t *d_boff, ChunkPar P, int32_t *d_c
                     hipStream_t st) {
   const int TB = 256;
   if (P.nblk <= 0) return MOIL_GPU_OK;
-  k_chunk_count<<<(P.nblk + TB - 1) / TB, TB, 0, st>>>(d_boff, P, d_ccount);
+  // one wavefront per block, four per workgroup
+  k_chunk_count<<<(P.nblk + 3) / 4, TB, 0, st>>>(d_boff, P, d_ccount);
   CHECK_SCAN(scan_excl(c, d_ccount, d_cfirst, P.nblk, st));
-  k_chunk_write<<<(P.nblk + TB - 1) / TB, TB, 0, st>>>(d_boff, d_cfirst, P, d_chunks, d_cpairs);
+  k_chunk_write<<<(P.nblk + 3) / 4, TB, 0, st>>>(d_boff, d_cfirst, P, d_chunks, d_cpairs);
   if (d_order) k_chunk_order<<<1, 1024, 0, st>>>(d_cfirst + P.nblk, d_cpairs, P.cap, d_order);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;

@@ -49,6 +49,32 @@ __device__ __forceinline__ long long fixd(double v, double s, bool &bad) {
   return __double2ll_rn(x);
 }
 
+// Sums of four per-thread FP64 values over a workgroup of four wavefronts
+// in a fixed order: each wavefront folds its 64 lanes by cross-lane swaps
+// (xor 32, 16, ..., 1), lane 0 of each wavefront stores its four sums in
+// ws[16], and after one barrier thread k < 4 adds the wavefronts' sums of
+// value k as (w0 + w1) + (w2 + w3) and returns it (other threads get 0).
+// ws may be reused after the caller's next barrier.
+__device__ __forceinline__ double wg_sum4(double v0, double v1, double v2, double v3,
+                                          double *ws) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+    v2 += __shfl_xor(v2, off, 64);
+    v3 += __shfl_xor(v3, off, 64);
+  }
+  if (lane == 0) {
+    ws[wv] = v0;
+    ws[4 + wv] = v1;
+    ws[8 + wv] = v2;
+    ws[12 + wv] = v3;
+  }
+  __syncthreads();
+  return t < 4 ? (ws[4 * t] + ws[4 * t + 1]) + (ws[4 * t + 2] + ws[4 * t + 3]) : 0.0;
+}
+
 // sets the context's overflow word if this thread converted a term out of
 // range
 __device__ __forceinline__ void fix_report(bool bad, int *ovf) {

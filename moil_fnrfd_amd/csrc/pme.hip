@@ -873,6 +873,8 @@ int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   void *in[1] = {c->d_qreal}, *out[1] = {c->d_qcplx};
   RF_CHECK(rocfft_execution_info_set_stream(c->exec_fwd, st));
   RF_CHECK(rocfft_execute(c->plan_fwd, in, out, c->exec_fwd));
+  // the structure factor before the convolution scales it in place
+  if (p.vir) launch_pme_virial(c, p, st);
   if (p.f64) launch_k_conv_f64(p.conv, p.n_conv, st);
   else launch_k_conv_f32(p.conv, p.n_conv, st);
   void *in2[1] = {c->d_qcplx}, *out2[1] = {c->d_qreal};
@@ -880,6 +882,7 @@ int32_t pme_run(moil_ctx *c, const EvalPlan &p, hipStream_t st) {
   RF_CHECK(rocfft_execute(c->plan_bwd, in2, out2, c->exec_bwd));
   if (p.f64) launch_k_gather_f64(p.gather, p.n_tile, st);
   else launch_k_gather_f32(p.gather, p.n_tile, st);
+  if (p.vir) launch_rel_virial(c, p, st);
   HIP_CHECK(hipGetLastError());
   return MOIL_GPU_OK;
 }

@@ -74,20 +74,9 @@ __device__ __forceinline__ void pack_f64_body(const PackArgs &A, int bid, int nb
 }
 DEFINE_KERNEL(k_pack_water_f64, PackArgs, RS_THREADS, pack_f64_body)
 
-// ---------------------------------------------------------------------------
-// block-level deterministic reduction of per-thread doubles
-__device__ inline double block_sum(double v, double *red) {
-  int t = threadIdx.x;
-  red[t] = v;
-  __syncthreads();
-  for (int off = RS_THREADS / 2; off > 0; off >>= 1) {
-    if (t < off) red[t] += red[t + off];
-    __syncthreads();
-  }
-  double r = red[0];
-  __syncthreads();
-  return r;
-}
+// the energy terms of a water chunk's four sums (real LJ, real Coulomb,
+// image LJ, image Coulomb), in the order wg_sum4 returns them
+__constant__ const int kWaterEn[4] = {EN_VDW, EN_EL, EN_VSYM, EN_LSYM};
 
 // One listed water pair in FP32: the partner's O (hi + lo, translated)
 // and H offsets, and the constants of the pair terms
@@ -98,18 +87,20 @@ struct WPartner {
 struct WConst {
   float nb2, pb, qoo, qoh, qhh, qcoo, qcoh, qchh, a12, b6, cutvdw2, cutele2;
 };
-// the nine site pairs of block member (A, B, C) and partner J: the
-// member's gradient components in fi, the partner's added to fj, the pair
-// energies added to elj / eel; false (and nothing computed) for a
-// listwt2 pair beyond relx
-__device__ __forceinline__ bool water_entry(const float4 &A, const float4 &B, const float4 &C,
-                                            const WPartner &J, bool k2, const WConst &K,
+// The nine site pairs of block member (A, B, C) and partner J, whose O-O
+// distance is within the pair's list cutoff (relx for a listwt2 member,
+// which the caller has tested): the member's gradient components in fi,
+// the partner's added to fj and the pair energies to elj / eel, O-O LJ for
+// a listwt1 member (lj) within rvmx.  Every sum of a product is an
+// explicit fmaf (no add of a multiply is left for the compiler to fuse),
+// so every call site computes the same values.
+__device__ __forceinline__ void water_entry(const float4 &A, const float4 &B, const float4 &C,
+                                            const WPartner &J, bool lj, const WConst &K,
                                             float fi[9], float fj[9], float &elj, float &eel) {
   const float dx = (A.x - J.jhx) + (B.x - J.jlx);
   const float dy = (A.y - J.jhy) + (B.y - J.jly);
   const float dz = (A.z - J.jhz) + (B.z - J.jlz);
-  const float roo2 = dx * dx + dy * dy + dz * dz;
-  if (k2 && roo2 > K.cutele2) return false;
+  const float roo2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   // site vectors r_ab = x_ia - x_jb
   float ux[3], uy[3], uz[3];
   ux[0] = dx; uy[0] = dy; uz[0] = dz;
@@ -124,22 +115,23 @@ __device__ __forceinline__ bool water_entry(const float4 &A, const float4 &B, co
 #pragma unroll
     for (int sb = 0; sb < 3; sb++) {
       const float rx = ux[sa] - vx[sb], ry = uy[sa] - vy[sb], rz = uz[sa] - vz[sb];
-      const float r2 = (sa == 0 && sb == 0) ? roo2 : (rx * rx + ry * ry + rz * rz);
+      const float r2 = (sa == 0 && sb == 0) ? roo2 : fmaf(rz, rz, fmaf(ry, ry, rx * rx));
       const float qq = (sa == 0 && sb == 0) ? K.qoo : ((sa == 0 || sb == 0) ? K.qoh : K.qhh);
       const float qc = (sa == 0 && sb == 0) ? K.qcoo : ((sa == 0 || sb == 0) ? K.qcoh : K.qchh);
       const float s = __builtin_amdgcn_rsqf(r2);
       const float s2 = s * s;
       const float tt = __builtin_amdgcn_rcpf(fmaf(K.pb, r2 * s, 1.0f));
       const float ex = __builtin_amdgcn_exp2f(r2 * K.nb2);
-      const float e = qq * (ex * erfc_poly(tt) * s);
-      eel += e;
+      const float x = ex * erfc_poly(tt) * s;
+      const float e = qq * x;
+      eel = fmaf(qq, x, eel);
       float df = -s2 * fmaf(qc, ex, e);
-      if (sa == 0 && sb == 0 && !k2 && roo2 <= K.cutvdw2) {
+      if (sa == 0 && sb == 0 && lj && roo2 <= K.cutvdw2) {
         const float s6 = s2 * s2 * s2;
-        const float A12 = K.a12 * s6 * s6, B6 = K.b6 * s6;
-        const float e1 = A12 - B6;
+        const float a6 = K.a12 * s6;  // A12 = a6 s6
+        const float e1 = fmaf(a6, s6, -(K.b6 * s6));
         elj += e1;
-        df += -6.0f * s2 * (A12 + e1);
+        df = fmaf(-6.0f * s2, fmaf(a6, s6, e1), df);
       }
       fi[3 * sa] = fmaf(df, rx, fi[3 * sa]);
       fi[3 * sa + 1] = fmaf(df, ry, fi[3 * sa + 1]);
@@ -149,7 +141,27 @@ __device__ __forceinline__ bool water_entry(const float4 &A, const float4 &B, co
       fj[3 * sb + 2] = fmaf(-df, rz, fj[3 * sb + 2]);
     }
   }
-  return true;
+}
+
+// An entry's i-side components into the block's LDS sums as the bit
+// patterns of (double)f + 1.5 2^32 (exact for |f| < 2^31, see
+// water_f32_impl); the result tells whether the entry has a larger
+// component, whose pattern the caller replaces afterwards by the exact
+// value (the entry recomputed bit for bit).  A NaN component, which the
+// maximum passes over, comes from a NaN or infinite site-pair term that
+// reaches the partner's row as well, where the reduction's conversion
+// reports it.
+__device__ __forceinline__ bool add_i_side(const float fi[9], int i, unsigned long long *iacc,
+                                           unsigned int *icnt) {
+  const float fmx = fmaxf(fmaxf(fmaxf(fabsf(fi[0]), fabsf(fi[1])), fabsf(fi[2])),
+                          fmaxf(fmaxf(fmaxf(fabsf(fi[3]), fabsf(fi[4])), fabsf(fi[5])),
+                                fmaxf(fmaxf(fabsf(fi[6]), fabsf(fi[7])), fabsf(fi[8]))));
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    atomicAdd(&iacc[k * WBLOCK + i],
+              (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
+  atomicAdd(&icnt[i], 1u);
+  return !(fmx < 2147483648.0f);
 }
 
 // FP32 (mixed precision) water kernel
@@ -169,7 +181,7 @@ __device__ __forceinline__ void water_f32_impl(
   __shared__ float4 iA[WBLOCK], iB[WBLOCK], iC[WBLOCK];
   __shared__ unsigned long long iacc[WBLOCK * 9];
   __shared__ unsigned int icnt[WBLOCK];
-  __shared__ double red[RS_THREADS];
+  __shared__ double wsum[16];
   // each thread's FP64 energy sums (LJ / Coulomb, real / image items), in
   // LDS: in registers they cost the entry loop its registers
   __shared__ double eacc[4][RS_THREADS];
@@ -235,40 +247,30 @@ __device__ __forceinline__ void water_f32_impl(
       const float dx = (A.x - J.jhx) + (B.x - J.jlx);
       const float dy = (A.y - J.jhy) + (B.y - J.jly);
       const float dz = (A.z - J.jhz) + (B.z - J.jlz);
-      if (dx * dx + dy * dy + dz * dz <= K.cutele2) act |= 1ull << i;
+      if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= K.cutele2) act |= 1ull << i;
     }
     uint64_t mask = rotr64(act, rot);
     while (mask) {
       const int i = (__ffsll((long long)mask) - 1 + rot) & 63;
-      const bool k2 = (item.m2 >> i) & 1ull;
       mask &= mask - 1;
       float fi[9];
-      if (!water_entry(iA[i], iB[i], iC[i], J, k2, K, fi, fj, elj, eel)) continue;
-      // the nine components' magnitudes summed (a NaN stays a NaN): the
-      // bit-pattern conversion holds for every |f| < 2^31; a larger entry
-      // is converted again, exactly, below
-      const bool b = !((((fabsf(fi[0]) + fabsf(fi[1])) + (fabsf(fi[2]) + fabsf(fi[3]))) +
-                        ((fabsf(fi[4]) + fabsf(fi[5])) + (fabsf(fi[6]) + fabsf(fi[7]))) +
-                        fabsf(fi[8])) < 2147483648.0f);
-      big |= (uint64_t)b << i;
-#pragma unroll
-      for (int k = 0; k < 9; k++)
-        atomicAdd(&iacc[k * WBLOCK + i],
-                  (unsigned long long)__double_as_longlong((double)fi[k] + 6442450944.0));
-      atomicAdd(&icnt[i], 1u);
+      water_entry(iA[i], iB[i], iC[i], J, !((item.m2 >> i) & 1ull), K, fi, fj, elj, eel);
+      big |= (uint64_t)add_i_side(fi, i, iacc, icnt) << i;
     }
     if (big) {
-      // rare (forces beyond 2^31 / 9 kcal/mol/A): the entries recomputed,
-      // the bit patterns added above replaced by the exact fixed-point
-      // values (range-checked up to FIX_TERM_UNITS)
-      float fjd[9], ed = 0.f;  // the partner's side was added in the loop above
+      // rare (forces beyond 2^31 kcal/mol/A): these entries recomputed
+      // (the same values bit for bit), the bit patterns added above
+      // replaced by the exact fixed-point values (range-checked up to
+      // FIX_TERM_UNITS); the partner's side and the energies were added in
+      // the walk above
+      float fjd[9], ed = 0.f;
 #pragma unroll
       for (int k = 0; k < 9; k++) fjd[k] = 0.f;
       while (big) {
         const int i = __ffsll((long long)big) - 1;
         big &= big - 1;
         float fi[9];
-        water_entry(iA[i], iB[i], iC[i], J, (item.m2 >> i) & 1ull, K, fi, fjd, ed, ed);
+        water_entry(iA[i], iB[i], iC[i], J, !((item.m2 >> i) & 1ull), K, fi, fjd, ed, ed);
 #pragma unroll
         for (int k = 0; k < 9; k++)
           atomicAdd(&iacc[k * WBLOCK + i],
@@ -293,15 +295,8 @@ __device__ __forceinline__ void water_f32_impl(
     ipart[(size_t)ch * (WBLOCK * 9) + u] =
         (long long)(iacc[k * WBLOCK + i] - (unsigned long long)icnt[i] * kBias32);
   }
-  double v;
-  v = block_sum(eacc[0][t], red);
-  if (t == 0) epart[EN_VDW * max_parts + ch] = v;
-  v = block_sum(eacc[1][t], red);
-  if (t == 0) epart[EN_EL * max_parts + ch] = v;
-  v = block_sum(eacc[2][t], red);
-  if (t == 0) epart[EN_VSYM * max_parts + ch] = v;
-  v = block_sum(eacc[3][t], red);
-  if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
+  const double v = wg_sum4(eacc[0][t], eacc[1][t], eacc[2][t], eacc[3][t], wsum);
+  if (t < 4) epart[kWaterEn[t] * max_parts + ch] = v;
 }
 
 __device__ __forceinline__ void water_f32_body(const WaterArgs &A, int bid, int nblk) {
@@ -319,7 +314,7 @@ __device__ __forceinline__ void water_f64_impl(
     double *__restrict__ epart, int max_parts, int bid, int nblk) {
   __shared__ double ipos[WBLOCK * 9];
   __shared__ long long iacc[WBLOCK * 9];
-  __shared__ double red[RS_THREADS];
+  __shared__ double wsum[16];
   const int ch = order[bid], t = threadIdx.x;  // largest first
   const int blk = chunks[3 * ch], beg = chunks[3 * ch + 1], end = chunks[3 * ch + 2];
   const int s0 = blk * WBLOCK;
@@ -414,15 +409,8 @@ __device__ __forceinline__ void water_f64_impl(
   fix_report(bad, P.ovf);
   __syncthreads();
   for (int u = t; u < WBLOCK * 9; u += RS_THREADS) ipart[(size_t)ch * (WBLOCK * 9) + u] = iacc[u];
-  double v;
-  v = block_sum(e_lj_r, red);
-  if (t == 0) epart[EN_VDW * max_parts + ch] = v;
-  v = block_sum(e_el_r, red);
-  if (t == 0) epart[EN_EL * max_parts + ch] = v;
-  v = block_sum(e_lj_i, red);
-  if (t == 0) epart[EN_VSYM * max_parts + ch] = v;
-  v = block_sum(e_el_i, red);
-  if (t == 0) epart[EN_LSYM * max_parts + ch] = v;
+  const double v = wg_sum4(e_lj_r, e_el_r, e_lj_i, e_el_i, wsum);
+  if (t < 4) epart[kWaterEn[t] * max_parts + ch] = v;
 }
 
 __device__ __forceinline__ void water_f64_body(const WaterArgs &A, int bid, int nblk) {
@@ -521,7 +509,10 @@ void water_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
 
 // Single-system form on a short grid (mixed precision): fewer workgroups
 // than chunks -- a fixed number per CU, below the CU's capacity for this
-// kernel (six at its register budget) -- walk the chunks in rounds of
+// kernel (six at its budget of 80 VGPRs, which leaves room on every SIMD
+// for the reciprocal-space kernels' waves; the few values it spills are
+// reloaded once per chunk, outside the entry loop) -- walk the chunks in
+// rounds of
 // gridDim.x, alternating direction every round (the chunks are in
 // dispatch order, largest first, so a workgroup that takes a large chunk
 // in one round takes a small one in the next).  The slots left free on

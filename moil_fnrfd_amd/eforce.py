@@ -100,6 +100,11 @@ def load_library():
     L.moil_gpu_dyna_run_dev.argtypes = [vp, vp, vp, vp, C.c_int32, vp, vp]
     L.moil_gpu_dyna_run.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
     L.moil_gpu_dyna_destroy.argtypes = [vp]
+    L.moil_gpu_set_molecules.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+    L.moil_molecules.argtypes = [C.POINTER(abi.MoilTopology), C.POINTER(C.c_int32), vp, vp, vp]
+    L.moil_gpu_eforce_virial.argtypes = [vp, vp, vp, C.POINTER(abi.MoilEnergies),
+                                         C.POINTER(abi.MoilVirial)]
+    L.moil_gpu_eforce_virial_dev.argtypes = [vp, vp, vp, vp, vp, vp]
     L.moil_ewald_setup.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, C.c_int32]
     for name in ('moil_gpu_init', 'moil_gpu_finalize', 'moil_gpu_ewald_info', 'moil_gpu_erf_table',
@@ -116,7 +121,8 @@ def load_library():
                  'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
                  'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm',
                  'moil_ewald_setup', 'moil_gpu_dyna_create', 'moil_gpu_dyna_run_dev',
-                 'moil_gpu_dyna_run', 'moil_gpu_dyna_destroy'):
+                 'moil_gpu_dyna_run', 'moil_gpu_dyna_destroy', 'moil_gpu_set_molecules',
+                 'moil_molecules', 'moil_gpu_eforce_virial', 'moil_gpu_eforce_virial_dev'):
         getattr(L, name).restype = C.c_int32
     if L.moil_gpu_abi_version() != abi.ABI_VERSION:
         raise MoilGPUError('ABI version mismatch')
@@ -182,6 +188,31 @@ class MoilGPU:
         en = abi.MoilEnergies()
         _check(load_library().moil_gpu_eforce(self._h, x.ctypes.data, dpot.ctypes.data, C.byref(en)))
         return en.as_dict(), dpot
+
+    def set_molecules(self, patom, ordlist, pmol, ptms):
+        """Molecules for the virial (MOIL's PRESS.BLOCK arrays, 1-based, as
+        molecules() returns them) and the atom masses; empty patom removes
+        them."""
+        self._mol = [np.ascontiguousarray(patom, np.int32), np.ascontiguousarray(ordlist, np.int32),
+                     np.ascontiguousarray(pmol, np.int32), np.ascontiguousarray(ptms, np.float64)]
+        _check(load_library().moil_gpu_set_molecules(self._h, len(self._mol[0]),
+                                                     *[a.ctypes.data for a in self._mol]))
+
+    def eforce_virial(self, coor):
+        """eforce with MOIL's pressON: (energies dict, dpot, virial terms as a
+        float64 array in abi.VIRIAL_NAMES order)."""
+        x = _f64(coor, self.npt)
+        dpot = np.zeros((self.npt, 3))
+        en, vr = abi.MoilEnergies(), abi.MoilVirial()
+        _check(load_library().moil_gpu_eforce_virial(self._h, x.ctypes.data, dpot.ctypes.data,
+                                                     C.byref(en), C.byref(vr)))
+        return en.as_dict(), dpot, vr.as_array()
+
+    def eforce_virial_dev(self, d_coor: int, d_dpot: int, d_energies: int, d_virial: int,
+                          stream: Optional[int] = None):
+        _check(load_library().moil_gpu_eforce_virial_dev(
+            self._h, C.c_void_p(d_coor), C.c_void_p(d_dpot), C.c_void_p(d_energies),
+            C.c_void_p(d_virial), C.c_void_p(stream or 0)))
 
     # -- device-resident variants (pointers are device addresses) ----------
     def squeeze_dev(self, d_coor: int, stream: Optional[int] = None):
@@ -532,6 +563,19 @@ def read_traj(path, npt, frame):
     _check(load_library().moil_traj_read(os.fsencode(path), int(npt), int(frame), x.ctypes.data,
                                          C.byref(n)))
     return x, n.value
+
+
+def molecules(topo):
+    """MOIL's molecule division for the virial (init_pressure's tables,
+    computed by the library's host code): (patom, ordlist, pmol), 1-based."""
+    ts, keep = abi.make_topology(topo)
+    n = C.c_int32()
+    patom = np.zeros(topo.npt, np.int32)
+    ordlist = np.zeros(topo.npt, np.int32)
+    pmol = np.zeros(topo.npt, np.int32)
+    _check(load_library().moil_molecules(C.byref(ts), C.byref(n), patom.ctypes.data,
+                                         ordlist.ctypes.data, pmol.ctypes.data))
+    return patom[:n.value].copy(), ordlist, pmol
 
 
 def ewald_setup(relx, dtol):

@@ -7,7 +7,8 @@ c       symm/ewald_init.f  (ewald_init)   -> moil_gpu_init
 c       symm/squeeze.f     (squeeze)      -> moil_gpu_squeeze
 c       pot/nbondm.f       (nbondm)       -> moil_gpu_build_lists
 c       symm/syminit.f     (syminit)      -> no-op (lists built above)
-c       pot/eforce.f       (eforce)       -> moil_gpu_eforce
+c       pot/eforce.f       (eforce)       -> moil_gpu_eforce (with
+c                                            pressON: eforce_virial)
 c     so every MOIL driver (dyna, chain, milestoning, ...) keeps
 c     calling the same argument-less subroutines on the same COMMON
 c     data.  A non-zero status from the library is reported through
@@ -54,6 +55,13 @@ c     struct moil_topology / moil_params / moil_energies (moil_gpu.h)
         real(c_double) :: e_vdw14, e_corr, e_self, e_ew_receip
         real(c_double) :: e_dir
       end type
+c     struct moil_virial
+      type, bind(C) :: mg_virial
+        real(c_double) :: virial, virxx, viryy, virzz
+        real(c_double) :: dirxx, diryy, dirzz
+        real(c_double) :: recxx, recyy, reczz
+        real(c_double) :: relxx, relyy, relzz
+      end type
       interface
         integer(c_int32_t) function moil_gpu_init(topo, par, ctx)
      &      bind(C, name='moil_gpu_init')
@@ -89,12 +97,33 @@ c     struct moil_topology / moil_params / moil_energies (moil_gpu.h)
           real(c_double), intent(out) :: dpot(*)
           type(mg_energies), intent(out) :: e
         end function
+        integer(c_int32_t) function moil_gpu_eforce_virial(ctx, coor,
+     &      dpot, e, v) bind(C, name='moil_gpu_eforce_virial')
+          import :: c_ptr, c_int32_t, c_double, mg_energies, mg_virial
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: coor(*)
+          real(c_double), intent(out) :: dpot(*)
+          type(mg_energies), intent(out) :: e
+          type(mg_virial), intent(out) :: v
+        end function
+        integer(c_int32_t) function moil_gpu_set_molecules(ctx, nmol,
+     &      patom, ordlist, pmol, ptms)
+     &      bind(C, name='moil_gpu_set_molecules')
+          import :: c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: nmol
+          integer(c_int32_t), intent(in) :: patom(*), ordlist(*)
+          integer(c_int32_t), intent(in) :: pmol(*)
+          real(c_double), intent(in) :: ptms(*)
+        end function
         type(c_ptr) function moil_gpu_last_error()
      &      bind(C, name='moil_gpu_last_error')
           import :: c_ptr
         end function
       end interface
       type(c_ptr), save :: mg_ctx = c_null_ptr
+c     molecules handed to the library (init_pressure's, PRESS.BLOCK)
+      integer, save :: mg_nmol = 0
       integer(c_int32_t), allocatable, target, save :: mg_flags(:)
       contains
 c     status check: MOIL's alert(...,1) prints and stops
@@ -243,7 +272,8 @@ c     torsion, improper, LJ, Coulomb and the scaled 1-4 pairs.
       if (esteeryes) why = 'steering energy'
       if (arith) why = 'arithmetic combination rule (arith)'
       if (lesflag) why = 'LES copies'
-      if (pressON) why = 'pressure / virial (pressON)'
+      if (pressON .and. nmol .le. 0)
+     &  why = 'pressure / virial without molecules (init_pressure)'
       if (inofrz .ne. npt) why = 'frozen particles'
       if (prll_on_off .and. num_pes .gt. 1) why = 'several processes'
       if (why .ne. ' ') call alert(where, len(where),
@@ -355,12 +385,44 @@ c     gradient into dpot
       subroutine eforce
       use moil_gpu_c
       include 'COMMON/LENGTH.BLOCK'
+      include 'COMMON/CONNECT.BLOCK'
       include 'COMMON/COORD.BLOCK'
       include 'COMMON/ENERGY.BLOCK'
       include 'COMMON/EWALD.BLOCK'
+      include 'COMMON/PRESS.BLOCK'
       type(mg_energies) :: e
+      type(mg_virial) :: v
       call mg_modes('eforce')
-      call mg_check(moil_gpu_eforce(mg_ctx, coor, dpot, e), 'eforce')
+      if (.not. pressON) then
+        call mg_check(moil_gpu_eforce(mg_ctx, coor, dpot, e), 'eforce')
+      else
+c       the molecules of init_pressure, handed over once; the virial
+c       terms into PRESS.BLOCK (pot/eforce.f:70-85 zeroes them, the
+c       correction terms stay zero as in MOIL)
+        if (mg_nmol .ne. nmol) then
+          call mg_check(moil_gpu_set_molecules(mg_ctx, nmol, patom,
+     &      ordLISTatm, pmol, ptms), 'eforce')
+          mg_nmol = nmol
+        end if
+        call mg_check(moil_gpu_eforce_virial(mg_ctx, coor, dpot, e, v),
+     &    'eforce')
+        virial = v%virial
+        virXX = v%virxx
+        virYY = v%viryy
+        virZZ = v%virzz
+        V_PIdirXX = v%dirxx
+        V_PIdirYY = v%diryy
+        V_PIdirZZ = v%dirzz
+        V_PIrecXX = v%recxx
+        V_PIrecYY = v%recyy
+        V_PIrecZZ = v%reczz
+        V_PIrelXX = v%relxx
+        V_PIrelYY = v%relyy
+        V_PIrelZZ = v%relzz
+        V_PIcorXX = 0.d0
+        V_PIcorYY = 0.d0
+        V_PIcorZZ = 0.d0
+      end if
       e_total = e%e_total
       e_bond = e%e_bond
       e_theta = e%e_theta

@@ -70,6 +70,12 @@ typedef struct orc_ctx {
   int iblock1[14], iblock2[14], iblock3[14];
   int symop[14][3];
   int have_lists;
+  /* molecular virial (MOIL's pressON, COMMON/PRESS.BLOCK): molecules as
+   * init_pressure defines them, centres of mass before every eforce */
+  int nmol;
+  int32_t *patom, *ordlist, *pmol; /* [nmol], [npt], [npt] (1-based) */
+  double *ptms, *massmol, *xmol, *xrel; /* [npt], [nmol], [3*nmol], [3*npt] */
+  double vir[13]; /* virial, virXX..ZZ, V_PIdir, V_PIrec, V_PIrel (XX, YY, ZZ) */
 } orc_ctx;
 
 static int32_t *dup_i(const int32_t *s, int64_t n) {
@@ -351,6 +357,8 @@ orc_ctx *orc_create(const moil_topology *t, const moil_params *p, char *err, int
 
 void orc_destroy(orc_ctx *c) {
   if (!c) return;
+  free(c->patom); free(c->ordlist); free(c->pmol); free(c->ptms); free(c->massmol);
+  free(c->xmol); free(c->xrel);
   free(c->poipt); free(c->monoflag); free(c->flagchr); free(c->exc1); free(c->exc2);
   free(c->ptchg); free(c->epsgm6); free(c->epsgm12);
   free(c->ib1); free(c->ib2); free(c->kbond); free(c->req);
@@ -978,11 +986,15 @@ static void ener14(const orc_ctx *c, const double *coor, double *dpot, eterms *E
 
 /* One real-space pair (atom lists).  kind: 1 = list1 (LJ if r2<=cutvdw2,
  * Coulomb always), 2 = list2 (LJ if r2<=cutvdw2), 3 = list3 (Coulomb if
- * r2<=cutele2).  Returns the energies via ev, ee and the scalar df. */
+ * r2<=cutele2).  Returns the energies via ev, ee and the scalar df; with
+ * vir, also the LJ part of df (vir[0]) and MOIL's V_PIdir of the pair
+ * (vir[1], pot/cdie_ewald.f:351-355: its own exp, not the table's
+ * derivative; 0 without a Coulomb term). */
 static inline int pair_atoms(const orc_ctx *c, int kind, double r2, double qq, double A, double B,
-                             double *ev, double *ee, double *df) {
+                             double *ev, double *ee, double *df, double *vir) {
   double rij, s, s2 = 1.0 / r2, erfcc, derfc;
   *ev = *ee = *df = 0;
+  if (vir) vir[0] = vir[1] = 0;
   if (kind == 2) {
     double s6;
     if (r2 > c->p.cutvdw2) return 0;
@@ -991,6 +1003,7 @@ static inline int pair_atoms(const orc_ctx *c, int kind, double r2, double qq, d
     B = B * s6;
     *ev = A - B;
     *df = -6.0 * s2 * (A + *ev);
+    if (vir) vir[0] = *df;
     return 1;
   }
   if (kind == 3 && r2 > c->p.cutele2) return 0;
@@ -999,19 +1012,53 @@ static inline int pair_atoms(const orc_ctx *c, int kind, double r2, double qq, d
   spline_erfc(c, c->beta * rij, &erfcc, &derfc);
   *ee = qq * s * erfcc;
   *df = -qq * s * (s2 * erfcc + s * c->beta * derfc);
+  if (vir) {
+    const double xerfc = c->beta * rij;
+    vir[1] = qq * (erfcc * s * s * s +
+                   2.0 * c->beta / sqrt(M_PI) * exp(-(xerfc * xerfc)) * s * s);
+  }
   if (kind == 1 && r2 <= c->p.cutvdw2) {
-    double s6 = s2 * s2 * s2;
+    double s6 = s2 * s2 * s2, dfv;
     A = A * s6 * s6;
     B = B * s6;
     *ev = A - B;
-    *df += -6.0 * s2 * (A + *ev);
+    dfv = -6.0 * s2 * (A + *ev);
+    if (vir) vir[0] = dfv;
+    *df += dfv;
   }
   return 1;
 }
 
+/* The virial terms of one pair (pot/cdie_ewald.f:351-368,440-460 for a
+ * real pair, symm/symcdie_ewald.f:150-200 for an image): r = x_i + T -
+ * x_j, molecular separation rm = M_i + T - M_j written as MOIL writes it
+ * (M_i - M_j for a real pair; M_j - M_i - T, with the signs folded in,
+ * for an image), dfv the LJ part of df, vpd MOIL's V_PIdir. */
+static inline void pair_virial(const orc_ctx *c, double *V, int img, int i, int j, const double *T,
+                               const double *r, double dfv, double vpd) {
+  const double *mi = c->xmol + 3 * (c->pmol[i - 1] - 1), *mj = c->xmol + 3 * (c->pmol[j - 1] - 1);
+  double rm[3];
+  int k;
+  if (!img) {
+    for (k = 0; k < 3; k++) rm[k] = mi[k] - mj[k];
+    if (dfv != 0.0) {
+      V[0] = V[0] - rm[0] * dfv * r[0] - rm[1] * dfv * r[1] - rm[2] * dfv * r[2];
+      for (k = 0; k < 3; k++) V[1 + k] = V[1 + k] - dfv * r[k] * rm[k];
+    }
+    for (k = 0; k < 3; k++) V[4 + k] = V[4 + k] + vpd * r[k] * rm[k];
+  } else {
+    for (k = 0; k < 3; k++) rm[k] = mj[k] - mi[k] - T[k];
+    if (dfv != 0.0) {
+      V[0] = V[0] + rm[0] * dfv * r[0] + rm[1] * dfv * r[1] + rm[2] * dfv * r[2];
+      for (k = 0; k < 3; k++) V[1 + k] = V[1 + k] + dfv * r[k] * rm[k];
+    }
+    for (k = 0; k < 3; k++) V[4 + k] = V[4 + k] + vpd * (-r[k]) * rm[k];
+  }
+}
+
 /* cdie_ewald exclusion correction + list loops, pot/cdie_ewald.f:67-673,
  * and symcdie_ewald, symm/symcdie_ewald.f:58-549. */
-static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
+static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterms *E, double *V) {
   double epstmp = (c->p.terms & MOIL_TERM_EL) ? KOFDIE / c->p.eps : 0.0;
   double pick = (c->p.terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
   int i, kind, s;
@@ -1045,10 +1092,14 @@ static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterm
       for (int64_t k = pt[i - 1]; k < pt[i]; k++) {
         int j = ls->v[k];
         double rx = X(i, 0) - X(j, 0), ry = X(i, 1) - X(j, 1), rz = X(i, 2) - X(j, 2);
-        double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df;
+        double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2];
         if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], ai * c->epsgm12[j - 1],
-                        bi * c->epsgm6[j - 1], &ev, &ee, &df))
+                        bi * c->epsgm6[j - 1], &ev, &ee, &df, V ? vr : NULL))
           continue;
+        if (V) {
+          const double r[3] = {rx, ry, rz};
+          pair_virial(c, V, 0, i, j, NULL, r, vr[0], vr[1]);
+        }
         E->e_vdw += ev;
         E->e_el += ee;
         d[0] += df * rx; d[1] += df * ry; d[2] += df * rz;
@@ -1069,10 +1120,14 @@ static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterm
           for (int64_t k = jb; k < ps->v[w]; k++) {
             int j = ls->v[k];
             double rx = xi - X(j, 0), ry = yi - X(j, 1), rz = zi - X(j, 2);
-            double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df;
+            double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2];
             if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], ai * c->epsgm12[j - 1],
-                            bi * c->epsgm6[j - 1], &ev, &ee, &df))
+                            bi * c->epsgm6[j - 1], &ev, &ee, &df, V ? vr : NULL))
               continue;
+            if (V) {
+              const double r[3] = {rx, ry, rz};
+              pair_virial(c, V, 1, ir, j, T, r, vr[0], vr[1]);
+            }
             E->e_vsym += ev;
             E->e_lsym += ee;
             dpot[3 * (ir - 1)] += df * rx; dpot[3 * (ir - 1) + 1] += df * ry; dpot[3 * (ir - 1) + 2] += df * rz;
@@ -1090,7 +1145,7 @@ static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterm
  * the O-O distance exceeds relx (pot/watwat_ewald.f:131-612, 624-1085). */
 static void water_pair(const orc_ctx *c, int io, const double *T, int jo, int kind,
                        double qq[3][3], double oo12, double oo6, double *dpot,
-                       const double *coor, double *ev, double *ee) {
+                       const double *coor, double *ev, double *ee, double *V, int img) {
   double xi[3][3], r[3], r2;
   int a, b, k;
   for (a = 0; a < 3; a++)
@@ -1100,7 +1155,7 @@ static void water_pair(const orc_ctx *c, int io, const double *T, int jo, int ki
   if (kind == 2 && r2 > c->p.cutele2) return;
   for (a = 0; a < 3; a++)
     for (b = 0; b < 3; b++) {
-      double rr[3], q2, s2, rij, s, erfcc, derfc, df = 0;
+      double rr[3], q2, s2, rij, s, erfcc, derfc, df = 0, dfv = 0;
       for (k = 0; k < 3; k++) rr[k] = xi[a][k] - X(jo + b, k);
       q2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
       s2 = 1.0 / q2;
@@ -1108,11 +1163,20 @@ static void water_pair(const orc_ctx *c, int io, const double *T, int jo, int ki
         double s6 = s2 * s2 * s2, A = oo12 * s6 * s6, B = oo6 * s6, e1 = A - B;
         *ev += e1;
         df = -6.0 * s2 * (A + e1);
+        dfv = df;
       }
       rij = sqrt(q2);
       s = 1.0 / rij;
       spline_erfc(c, c->beta * rij, &erfcc, &derfc);
       *ee += qq[a][b] * s * erfcc;
+      if (V) {
+        /* pot/watwat_ewald.f:162-206, symm/symwat_ewald.f:160-203: every
+         * site pair with the two waters' molecular separation */
+        const double xerfc = c->beta * rij;
+        const double vpd = qq[a][b] * (erfcc * s * s * s +
+                                       2.0 * c->beta / sqrt(M_PI) * exp(-(xerfc * xerfc)) * s * s);
+        pair_virial(c, V, img, io + a, jo + b, T, rr, dfv, vpd);
+      }
       df += -qq[a][b] * s * (s2 * erfcc + s * c->beta * derfc);
       for (k = 0; k < 3; k++) {
         dpot[3 * (io + a - 1) + k] += df * rr[k];
@@ -1121,7 +1185,7 @@ static void water_pair(const orc_ctx *c, int io, const double *T, int jo, int ki
     }
 }
 
-static void real_waters(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
+static void real_waters(const orc_ctx *c, const double *coor, double *dpot, eterms *E, double *V) {
   double epstmp = (c->p.terms & MOIL_TERM_EL) ? 1.0 : 0.0;
   double pick = (c->p.terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
   double qq[3][3], oo12, oo6, zero[3] = {0, 0, 0};
@@ -1152,7 +1216,7 @@ static void real_waters(const orc_ctx *c, const double *coor, double *dpot, eter
       int i0 = c->poipt[c->indxwat[iw]] - 2;
       for (int64_t k = poin[iw]; k < poin[iw + 1]; k++) {
         int j0 = c->poipt[lw->v[k]] - 2;
-        water_pair(c, i0, zero, j0, kind, qq, oo12, oo6, dpot, coor, &E->e_vdw, &E->e_el);
+        water_pair(c, i0, zero, j0, kind, qq, oo12, oo6, dpot, coor, &E->e_vdw, &E->e_el, V, 0);
       }
     }
     {
@@ -1163,7 +1227,7 @@ static void real_waters(const orc_ctx *c, const double *coor, double *dpot, eter
           int i0 = c->poipt[sw->v[w]] - 2;
           for (int64_t k = jb; k < pw->v[w]; k++) {
             int j0 = c->poipt[lw->v[k]] - 2;
-            water_pair(c, i0, T, j0, kind, qq, oo12, oo6, dpot, coor, &E->e_vsym, &E->e_lsym);
+            water_pair(c, i0, T, j0, kind, qq, oo12, oo6, dpot, coor, &E->e_vsym, &E->e_lsym, V, 1);
           }
           jb = pw->v[w];
         }
@@ -1236,7 +1300,7 @@ static void fft3(cplx *Q, int n1, int n2, int n3, int sign) {
   free(line);
 }
 
-static void pme(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
+static void pme(const orc_ctx *c, const double *coor, double *dpot, eterms *E, double *V) {
   int n1 = c->p.nfft1, n2 = c->p.nfft2, n3 = c->p.nfft3, order = c->p.order;
   int npt = c->npt, n, a, b, g;
   size_t G = (size_t)n1 * n2 * n3;
@@ -1297,6 +1361,13 @@ static void pme(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
       double eterm = exp(-fac * msq) / denom;
       cplx *q = &Q[((size_t)k3 * n2 + k2) * n1 + k1];
       energy += eterm * (q->re * q->re + q->im * q->im);
+      if (V) {
+        /* ewald_receip.f:799-807 */
+        const double vr = 0.5 * eterm * (q->re * q->re + q->im * q->im);
+        const double mh[3] = {mh1, mh2, mh3};
+        for (a = 0; a < 3; a++)
+          V[7 + a] = V[7 + a] + vr * (1.0 - 2.0 * (1.0 + fac * msq) / msq * mh[a] * mh[a]);
+      }
       q->re *= eterm;
       q->im *= eterm;
     }
@@ -1330,9 +1401,76 @@ static void pme(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
     dpot[3 * n] -= recip[0] * f1;
     dpot[3 * n + 1] -= recip[1] * f2;
     dpot[3 * n + 2] -= recip[2] * f3;
+    if (V) {
+      /* ewald_receip.f:929-933: the reciprocal force on n times its
+       * position relative to its molecule's centre */
+      V[10] = V[10] + recip[0] * f1 * c->xrel[3 * n];
+      V[11] = V[11] + recip[1] * f2 * c->xrel[3 * n + 1];
+      V[12] = V[12] + recip[2] * f3 * c->xrel[3 * n + 2];
+    }
   }
   (void)b; (void)g;
   free(fr); free(th); free(dth); free(Q);
+}
+
+/* Molecules for the virial (COMMON/PRESS.BLOCK as analysis/init_pressure.f
+ * fills it: patom[m] the last ordlist position of molecule m, ordlist the
+ * atoms molecule by molecule, pmol the molecule of each atom) and the
+ * masses; the molecular masses summed as compute_massmol does
+ * (analysis/init_pressure.f:411-428).  nmol = 0 switches the virial off. */
+int orc_set_molecules(orc_ctx *c, int nmol, const int32_t *patom, const int32_t *ordlist,
+                      const int32_t *pmol, const double *ptms) {
+  int m, k = 1;
+  free(c->patom); free(c->ordlist); free(c->pmol); free(c->ptms); free(c->massmol);
+  free(c->xmol); free(c->xrel);
+  c->patom = c->ordlist = c->pmol = NULL;
+  c->ptms = c->massmol = c->xmol = c->xrel = NULL;
+  c->nmol = 0;
+  if (nmol <= 0) return 0;
+  c->patom = dup_i(patom, nmol);
+  c->ordlist = dup_i(ordlist, c->npt);
+  c->pmol = dup_i(pmol, c->npt);
+  c->ptms = dup_d(ptms, c->npt);
+  c->massmol = (double *)calloc(nmol, sizeof(double));
+  c->xmol = (double *)calloc(3 * (size_t)nmol, sizeof(double));
+  c->xrel = (double *)calloc(3 * (size_t)c->npt, sizeof(double));
+  for (m = 1; m <= nmol; m++) {
+    double sm = 0.0;
+    for (int a = k; a <= patom[m - 1]; a++) sm = sm + ptms[ordlist[a - 1] - 1];
+    c->massmol[m - 1] = sm;
+    k = patom[m - 1] + 1;
+  }
+  c->nmol = nmol;
+  return 0;
+}
+
+/* centres of mass and relative positions, as the coordinate part of
+ * atom2molecule (analysis/pressure.f:419-505): per molecule and
+ * component the mass-weighted sum in ordlist order over the molecular
+ * mass */
+static void mol_coords(orc_ctx *c, const double *coor) {
+  int m, k0 = 0, d, i;
+  for (m = 1; m <= c->nmol; m++) {
+    for (d = 0; d < 3; d++) {
+      double sm = 0.0;
+      for (int k = k0 + 1; k <= c->patom[m - 1]; k++) {
+        const int a = c->ordlist[k - 1];
+        sm = sm + coor[3 * (a - 1) + d] * c->ptms[a - 1];
+      }
+      c->xmol[3 * (m - 1) + d] = sm / c->massmol[m - 1];
+    }
+    k0 = c->patom[m - 1];
+  }
+  for (i = 1; i <= c->npt; i++)
+    for (d = 0; d < 3; d++)
+      c->xrel[3 * (i - 1) + d] = coor[3 * (i - 1) + d] - c->xmol[3 * (c->pmol[i - 1] - 1) + d];
+}
+
+/* the virial terms of the last orc_eforce with molecules set: virial,
+ * virXX/YY/ZZ, V_PIdirXX/YY/ZZ, V_PIrecXX/YY/ZZ, V_PIrelXX/YY/ZZ */
+int orc_virial(const orc_ctx *c, double *out) {
+  memcpy(out, c->vir, sizeof(c->vir));
+  return c->nmol > 0 ? 0 : MOIL_GPU_ESTATE;
 }
 
 /* eforce, pot/eforce.f:37-311 (energy path subset) */
@@ -1340,17 +1478,20 @@ int orc_eforce(orc_ctx *c, const double *coor, double *dpot, double *en /* moil_
   eterms E;
   moil_energies *out = (moil_energies *)en;
   uint32_t t = c->p.terms;
+  double *V = c->nmol > 0 ? c->vir : NULL;
   if (!c->have_lists) return MOIL_GPU_ESTATE;
   memset(&E, 0, sizeof(E));
   memset(dpot, 0, sizeof(double) * 3 * c->npt);
+  memset(c->vir, 0, sizeof(c->vir));
+  if (V) mol_coords(c, coor);
   if ((t & MOIL_TERM_BOND) && c->nb > 0) ebond(c, coor, dpot, &E);
   if ((t & MOIL_TERM_ANGLE) && c->nangl > 0) etheta(c, coor, dpot, &E);
   if ((t & MOIL_TERM_TORS) && c->ntors > 0) etors(c, coor, dpot, &E);
   if ((t & MOIL_TERM_IMP) && c->nimp > 0) eimphi(c, coor, dpot, &E);
   if (t & (MOIL_TERM_VDW | MOIL_TERM_EL)) {
-    real_atoms(c, coor, dpot, &E);
-    pme(c, coor, dpot, &E);
-    real_waters(c, coor, dpot, &E);
+    real_atoms(c, coor, dpot, &E, V);
+    pme(c, coor, dpot, &E, V);
+    real_waters(c, coor, dpot, &E, V);
     if (c->totspe > 0) ener14(c, coor, dpot, &E);
   }
   memset(out, 0, sizeof(*out));

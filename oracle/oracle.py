@@ -41,6 +41,9 @@ def lib():
         L.orc_dump_atom_pairs.restype = C.c_int64
         L.orc_dump_atom_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.orc_eforce.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_molecules.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
+        L.orc_virial.argtypes = [C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -101,6 +104,20 @@ class Oracle:
         rows = np.zeros((max(n, 1), 4), np.int32)
         lib().orc_dump_atom_pairs(self._h, rows.ctypes.data, n)
         return rows[:n]
+
+    def set_molecules(self, patom, ordlist, pmol, ptms):
+        """Molecules for the virial (MOIL's PRESS.BLOCK arrays, 1-based);
+        an empty patom switches it off."""
+        self._mol = [np.ascontiguousarray(patom, np.int32), np.ascontiguousarray(ordlist, np.int32),
+                     np.ascontiguousarray(pmol, np.int32), np.ascontiguousarray(ptms, np.float64)]
+        lib().orc_set_molecules(self._h, len(self._mol[0]), *[a.ctypes.data for a in self._mol])
+
+    def virial(self):
+        """The virial terms of the last eforce (abi.VIRIAL_NAMES order)."""
+        v = np.zeros(13)
+        if lib().orc_virial(self._h, v.ctypes.data):
+            raise RuntimeError('orc_virial: no molecules set')
+        return v
 
     def eforce(self, coor):
         x = np.ascontiguousarray(coor, np.float64)

@@ -95,7 +95,10 @@ c     keyword of pot/energy.f:219 does (init_ef sets hvdw0 = .true.)
       read(upar,*,end=15) ihvdw
 c     optional line after it: a MOIL mode to switch on (for the test that
 c     the GPU shim refuses modes it does not evaluate): 1 distance-
-c     dependent dielectric, 2 pressure, 3 one frozen particle, 4 arith
+c     dependent dielectric, 2 pressure, 3 one frozen particle, 4 arith;
+c     5 is the pressure calculation as MOIL's pressure program runs it
+c     (analysis/pressure.f:77-196): molecules from init_pressure, their
+c     centres of mass before every eforce, the virial terms dumped
       read(upar,*,end=15) imode
  15   if (ihvdw.eq.0) hvdw0 = .false.
       close(upar)
@@ -124,6 +127,10 @@ c     dependent dielectric, 2 pressure, 3 one frozen particle, 4 arith
          zerofrz(npt) = 0
       end if
       if (imode.eq.4) arith = .true.
+      if (imode.eq.5) then
+         pressON = .true.
+         call init_pressure()
+      end if
 
       ucrd = 33
       k = len_trim(crdfile)
@@ -166,6 +173,7 @@ c     cutoffs exactly as pot/energy.f:382-400 (cutm always explicit)
          end if
          call system_clock(c1)
          tlist = tlist + (c1-c0)
+         if (imode.eq.5) call mol_coords()
          call eforce()
          call system_clock(c0)
          tforce = tforce + (c0-c1)
@@ -177,6 +185,40 @@ c     cutoffs exactly as pot/energy.f:382-400 (cutm always explicit)
 
       call dump_all(outfile,dodump)
       stop
+      end
+
+c     Molecular centres of mass and atom positions relative to them, as
+c     the coordinate part of atom2molecule (analysis/pressure.f:419-505):
+c     per molecule and component, the mass-weighted sum over its atoms in
+c     ordLISTatm order, divided by the molecule's mass.
+      subroutine mol_coords()
+      implicit none
+      include 'COMMON/LENGTH.BLOCK'
+      include 'COMMON/CONNECT.BLOCK'
+      include 'COMMON/COORD.BLOCK'
+      include 'COMMON/PRESS.BLOCK'
+      integer imol,k,k0,i,kd
+      double precision sm
+      k0 = 0
+      do 20 imol=1,nmol
+         do 10 kd=1,3
+            sm = 0.d0
+            do 5 k=k0+1,patom(imol)
+               sm = sm + coor(kd,ordLISTatm(k))*ptms(ordLISTatm(k))
+ 5          continue
+            sm = sm/massmol(imol)
+            if (kd.eq.1) xmol(imol) = sm
+            if (kd.eq.2) ymol(imol) = sm
+            if (kd.eq.3) zmol(imol) = sm
+ 10      continue
+         k0 = patom(imol)
+ 20   continue
+      do 30 i=1,npt
+         xrel(i) = coor(1,i) - xmol(pmol(i))
+         yrel(i) = coor(2,i) - ymol(pmol(i))
+         zrel(i) = coor(3,i) - zmol(pmol(i))
+ 30   continue
+      return
       end
 
 c     Marsaglia polar normal deviate from a 64-bit LCG; only used to
@@ -226,10 +268,11 @@ c     perturb coordinates in the timing loop.
       include 'COMMON/COORD.BLOCK'
       include 'COMMON/ENERGY.BLOCK'
       include 'COMMON/EWALD.BLOCK'
+      include 'COMMON/PRESS.BLOCK'
       character*(*) outfile
       integer dodump
       integer u,n,ntab
-      double precision en(20)
+      double precision en(20),vr(16)
       integer iv(40)
 
       u = 41
@@ -283,6 +326,31 @@ c     perturb coordinates in the timing loop.
       iv(20) = ntab
       call wri(u,'COUNTS  ',20,iv)
       call wrd(u,'MONOCENT',3*totdmon,mono_cent)
+      if (pressON .and. nmol.gt.0) then
+c        the virial terms of eforce (pot/eforce.f:70-82 zeroes them) and
+c        the molecule tables of init_pressure
+         vr(1) = virial
+         vr(2) = virXX
+         vr(3) = virYY
+         vr(4) = virZZ
+         vr(5) = V_PIdirXX
+         vr(6) = V_PIdirYY
+         vr(7) = V_PIdirZZ
+         vr(8) = V_PIrecXX
+         vr(9) = V_PIrecYY
+         vr(10) = V_PIrecZZ
+         vr(11) = V_PIrelXX
+         vr(12) = V_PIrelYY
+         vr(13) = V_PIrelZZ
+         call wrd(u,'VIRIAL  ',13,vr)
+         iv(1) = nmol
+         call wri(u,'NMOL    ',1,iv)
+         call wri(u,'PATOM   ',nmol,patom)
+         call wri(u,'ORDLIST ',npt,ordLISTatm)
+         call wri(u,'PMOL    ',npt,pmol)
+         call wrd(u,'MASSMOL ',nmol,massmol)
+         call wrd(u,'PTMS    ',npt,ptms)
+      end if
       if (dodump.ne.0) then
          call wri(u,'POINT1  ',npt+1,point1)
          call wri(u,'POINT2  ',npt+1,point2)

@@ -149,7 +149,43 @@ def write_chain_case(name='chain_water216'):
           % (name, igrid, npt, d['SENER'][0], os.path.basename(path), os.path.getsize(path)))
 
 
+PRESS_CASES = ('water216', 'water216_out_sqz', 'hetero_small', 'hetero_small_out',
+               'chains_one_water')
+
+
+def write_press_case(name):
+    """MOIL's eforce with pressON on a parity system (the driver's mode 5:
+    init_pressure's molecules, their centres of mass before eforce): the
+    virial terms, the molecule tables and the energies and gradient."""
+    from tests._cases import make_case
+    s, cut, ew, sqz = make_case(name)
+    d = run_moil(s, cut, ew, squeeze=sqz, mode=5)
+    out = dict(
+        coor=s.coor.astype(np.float64),
+        topo_sha=np.frombuffer(topo_checksum(s.topo).encode(), np.uint8),
+        squeeze=np.array([int(sqz)]),
+        virial=d['VIRIAL'],
+        nmol=d['NMOL'],
+        patom=d['PATOM'],
+        ordlist=d['ORDLIST'],
+        pmol=d['PMOL'],
+        massmol=d['MASSMOL'],
+        ptms=d['PTMS'],
+        energies=d['ENERGIES'],
+        dpot=d['DPOT'].reshape(-1, 3),
+    )
+    path = os.path.join(OUT, 'press_' + name + '.npz')
+    np.savez_compressed(path, **out)
+    print('%-22s npt=%6d nmol=%5d virial=%.6f V_PIrecXX=%.6f -> %s (%d bytes)'
+          % ('press_' + name, s.topo.npt, int(d['NMOL'][0]), d['VIRIAL'][0], d['VIRIAL'][7],
+             os.path.basename(path), os.path.getsize(path)))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == 'press':
+        for name in PRESS_CASES:
+            write_press_case(name)
+        return
     s, cut, ew = configs.water216()
     write_case('water216', s, cut, ew)
     s, cut, ew = configs.water216()
@@ -176,6 +212,8 @@ def main():
     s, cut, ew = configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1)
     write_case('chains_one_water', perturb(s, 0.05, 2301), cut, ew)
     write_chain_case()
+    for name in PRESS_CASES:
+        write_press_case(name)
 
 
 if __name__ == '__main__':

@@ -1,0 +1,114 @@
+"""The molecular virial (MOIL's pressON) in the CPU oracle, pinned against
+MOIL's own eforce with pressON (golden fixtures tests/golden/press_*.npz,
+written by oracle/_ref/moil_dump in its pressure mode: init_pressure's
+molecules, their centres of mass before eforce) and against the
+derivative of the energy with respect to the box: scaling the box edge and
+every molecule's centre of mass along one axis by (1 + eps), molecules
+rigid, changes the energy by -eps (virXX + V_PIdirXX + V_PIrecXX -
+V_PIrelXX), the combination MOIL's pressure program adds up
+(analysis/pressure.f:218-221), so the sum of the four terms is fixed by
+finite differences and a wrong sign or a missing term shows.  The library's
+molecule division (moil_molecules) is checked against MOIL's tables."""
+import numpy as np
+import pytest
+
+from moil_fnrfd_amd import abi, configs
+from moil_fnrfd_amd.eforce import molecules
+from oracle.oracle import Oracle
+from tests._cases import make_case
+from tests._util import load_golden
+from tests.golden.gen_golden import PRESS_CASES, topo_checksum
+
+
+def virial_errors(v, ref):
+    """Relative error of every virial term; a term that cancels to far
+    below its group's size is compared on that size (the group: the LJ
+    tensor, the direct, reciprocal and relative-coordinate Coulomb tensors)."""
+    v, ref = np.asarray(v), np.asarray(ref)
+    out = {}
+    for k, n in enumerate(abi.VIRIAL_NAMES):
+        g = 0 if k == 0 else (k - 1) // 3 * 3 + 1
+        grp = ref[g:g + (1 if k == 0 else 3)]
+        out[n] = abs(v[k] - ref[k]) / max(abs(ref[k]), 1e-3 * np.abs(grp).max())
+    return out
+
+
+def _oracle(name):
+    g = load_golden('press_' + name)
+    s, cut, ew, sqz = make_case(name)
+    assert topo_checksum(s.topo) == bytes(g['topo_sha']).decode()
+    o = Oracle(s.topo, configs.params_for(s, cut, ew))
+    x = o.squeeze(s.coor) if sqz else s.coor
+    o.build_lists(x)
+    o.set_molecules(g['patom'], g['ordlist'], g['pmol'], g['ptms'])
+    return g, s, o, x
+
+
+@pytest.mark.parametrize('name', PRESS_CASES)
+def test_oracle_virial_matches_moil(name):
+    g, s, o, x = _oracle(name)
+    e, dp = o.eforce(x)
+    err = virial_errors(o.virial(), g['virial'])
+    assert max(err.values()) < 1e-12, err
+    # pressON changes no energy or force
+    assert abs(e['e_total'] - g['energies'][14]) < 1e-9 * abs(g['energies'][:15]).sum()
+    assert np.abs(dp - g['dpot']).max() < 1e-10 * np.abs(g['dpot']).max()
+
+
+@pytest.mark.parametrize('name', PRESS_CASES)
+def test_molecules_match_moil(name):
+    g = load_golden('press_' + name)
+    s, cut, ew, sqz = make_case(name)
+    patom, ordlist, pmol = molecules(s.topo)
+    assert len(patom) == int(g['nmol'][0])
+    assert np.array_equal(patom, g['patom'])
+    assert np.array_equal(ordlist, g['ordlist'])
+    assert np.array_equal(pmol, g['pmol'])
+    assert np.array_equal(s.topo.ptms, g['ptms'])
+
+
+def _energy_scaled(s, cut, ew, x, mol, ax, f):
+    """e_total with box edge ax and the centres of mass of every molecule
+    scaled by f along ax (each molecule moved rigidly)."""
+    box = list(s.box)
+    box[ax] *= f
+    old = s.box
+    s.box = tuple(box)
+    try:
+        p = configs.params_for(s, cut, ew)
+    finally:
+        s.box = old
+    patom, ordlist, pmol, m = mol
+    xs = x.copy()
+    k0 = 0
+    for k1 in patom:
+        idx = ordlist[k0:k1] - 1
+        com = (x[idx, ax] * m[idx]).sum() / m[idx].sum()
+        xs[idx, ax] += (f - 1.0) * com
+        k0 = k1
+    o = Oracle(s.topo, p)
+    o.build_lists(xs)
+    e, _ = o.eforce(xs)
+    return e['e_total']
+
+
+@pytest.mark.parametrize('name', ['water216', 'hetero_small'])
+def test_virial_is_the_box_derivative(name):
+    s, cut, ew, sqz = make_case(name)
+    o = Oracle(s.topo, configs.params_for(s, cut, ew))
+    x = o.squeeze(s.coor) if sqz else s.coor
+    patom, ordlist, pmol = molecules(s.topo)
+    mol = (patom, ordlist, pmol, s.topo.ptms)
+    o.build_lists(x)
+    o.set_molecules(patom, ordlist, pmol, s.topo.ptms)
+    o.eforce(x)
+    v = o.virial()
+    # the scalar is the LJ tensor's trace
+    assert abs(v[0] - v[1:4].sum()) < 1e-12 * np.abs(v[1:4]).sum()
+    # a step small enough that no pair crosses a cutoff in these systems
+    eps = 1e-7
+    for ax in range(3):
+        fd = (_energy_scaled(s, cut, ew, x, mol, ax, 1 + eps) -
+              _energy_scaled(s, cut, ew, x, mol, ax, 1 - eps)) / (2 * eps)
+        w = v[1 + ax] + v[4 + ax] + v[7 + ax] - v[10 + ax]
+        assert abs(fd + w) < 1e-6 * abs(w), (ax, fd, w)
