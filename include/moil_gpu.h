@@ -65,6 +65,25 @@ extern "C" {
 #define MOIL_TERM_EL14   (1u << 6)   /* e14el_yes */
 #define MOIL_TERM_VDW14  (1u << 7)   /* e14v_yes  */
 #define MOIL_TERM_HVDW0  (1u << 8)   /* hvdw0 hydrogen rule (pot/nbond.f:115) */
+/* arith: MOIL's 'arit' connectivity keyword (connect/connect.f:124), the
+ * combination and term forms of its AMBER-style parameter files.  The
+ * topology's per-atom arrays then hold epsgm12 = sqrt(4 eps) and epsgm6 =
+ * sigma (connect/fillpt.f:136-179), and
+ *   pair LJ   A = epsgm12_i epsgm12_j, sigma_ij = (epsgm6_i + epsgm6_j)/2,
+ *             E = A sigma_ij^12 / r^12 - A sigma_ij^6 / r^6
+ *             (pot/cdie_ewald.f:377-383,432-473,515-548,
+ *             symm/symcdie_ewald.f:127-140,275-314); the water O-O
+ *             constants follow (pot/watwat_ewald.f:62-68)
+ *   torsion   E = k2 + phase2 k2 cos(n phi), n = period 1..4 (ktors1,
+ *             ktors3, phase1, phase3 unused; pot/ephi.f:113-138,238-259)
+ *   improper  atoms taken as (iimp4, iimp2, iimp1, iimp3),
+ *             E = k - k cos(2 phi) (pot/eimphi.f:98-115,251-262)
+ *   lists     no hydrogen rule (pot/nbond.f:115-117, symm/nbsym.f:104-106)
+ * The 1-4 pair coefficients p14 are inputs either way.  Torsion periods
+ * outside 1..4 are refused at init (MOIL leaves such a term's energy
+ * unset).  The parameter-gradient entry points and moil_gpu_set_params
+ * assume the geometric rule and refuse an arith context. */
+#define MOIL_TERM_ARITH  (1u << 9)
 #define MOIL_TERM_DEFAULT 0x1FFu
 
 /* Topology: the energy-path part of MOIL's /CONNINT/ /CONNDBL/ /CONNLOG/
@@ -260,10 +279,13 @@ int32_t moil_gpu_batch_set_graph(moil_batch *batch, int32_t on);
  * into a topology for moil_gpu_init; the arrays belong to the returned
  * handle and stay valid until moil_con_free.  Monomer flags come from the
  * monomer names as MOIL's list and squeeze code tests them.  Files with LES
- * copies, Morse bonds, the beta list, monomer division or the arith rule
- * are refused (MOIL_GPU_EINVAL).  Host only: no GPU is touched. */
+ * copies, Morse bonds, the beta list or monomer division are refused
+ * (MOIL_GPU_EINVAL).  Host only: no GPU is touched. */
 typedef struct moil_con moil_con;
 int32_t moil_con_read(const char *path, moil_con **out, moil_topology *topo);
+/* the file's arith flag (rconn.f:60-61): *arith = 1 when the context needs
+ * MOIL_TERM_ARITH in its parameters' term bits */
+int32_t moil_con_arith(const moil_con *con, int32_t *arith);
 int32_t moil_con_free(moil_con *con);
 /* CHARMM-style coordinate file as MOIL's getcrd reads it
  * (generic/getcrd.f:48-97, format(i7,i7,1x,a4,1x,a4,3(f10.5),...)) into

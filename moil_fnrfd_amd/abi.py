@@ -18,6 +18,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 TERM_BOND, TERM_ANGLE, TERM_TORS, TERM_IMP = 1, 2, 4, 8
 TERM_VDW, TERM_EL, TERM_EL14, TERM_VDW14, TERM_HVDW0 = 16, 32, 64, 128, 256
+TERM_ARITH = 512  # MOIL's arith rule: Lorentz-Berthelot LJ, AMBER torsion/improper forms
 TERM_DEFAULT = 0x1FF
 
 N_ENERGIES = 15

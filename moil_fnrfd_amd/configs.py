@@ -10,7 +10,7 @@ bisection (symm/ewald_init.f:298-335).
 from __future__ import annotations
 
 from . import abi
-from .systems import hetero_packed, hetero_system, water_box
+from .systems import hetero_packed, hetero_system, to_arith, water_box
 
 # dtol giving beta = 0.34 at rc = 9 A and beta = 0.31 at rc = 10 A
 DTOL_B034_RC9 = 1.67573104443267911e-06
@@ -64,6 +64,19 @@ def hetero_small(seed=2000, n_chains=12, L=24.0):
     cases: 15 % uncharged and 15 % zero-LJ chain atoms so that every MOIL
     atom list (list1/list2/list3) and the hydrogen rule are exercised."""
     s = hetero_system(n_chains, L, seed=seed, frac_neutral=0.15, frac_hlike=0.15)
+    cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
+    n = int(round(L)) + int(round(L)) % 2
+    ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
+    return s, cut, ew
+
+
+def hetero_small_arith(seed=2500, n_chains=10, L=22.0):
+    """hetero_small's recipe converted to MOIL's arith rule (systems.to_arith):
+    Lorentz-Berthelot LJ in every pair kernel including the water O-O
+    constants, one-cosine torsions of periods 1..4 (some dihedrals with
+    two terms), k - k cos 2 phi impropers, no hydrogen rule."""
+    s = hetero_system(n_chains, L, seed=seed, frac_neutral=0.15, frac_hlike=0.15)
+    s.topo = to_arith(s.topo, seed + 1)
     cut = dict(rvmx=9.0, relx=9.0, rvbg=10.0, rebg=10.0, cutm=15.0)
     n = int(round(L)) + int(round(L)) % 2
     ew = dict(dtol=1e-5, nfft=(n, n, n), order=4)
@@ -126,6 +139,8 @@ def params_for(sysm, cut, ew, precision=abi.PREC_FP64, terms=abi.TERM_DEFAULT):
     MOIL's hydrogen rule off (its 'hvdw' keyword)."""
     if not cut.get('hvdw0', True):
         terms &= ~abi.TERM_HVDW0
+    if getattr(sysm.topo, 'arith', False):
+        terms |= abi.TERM_ARITH
     return abi.make_params(sysm.box, rvmx=cut['rvmx'], relx=cut['relx'], rvbg=cut['rvbg'],
                            rebg=cut['rebg'], cutm=cut['cutm'], dtol=ew['dtol'], nfft=ew['nfft'],
                            order=ew.get('order', 4), eps=cut.get('eps', 1.0),

@@ -166,6 +166,21 @@ static int32_t check_topology(const moil_topology *t, std::vector<int32_t> &exc1
   return MOIL_GPU_OK;
 }
 
+// O-O LJ coefficients of the water pair kernels, pot/watwat_ewald.f:62-68:
+// geometric A = pick e12^2, B = pick e6^2; arith A = pick e12^2 sigma^12,
+// B = pick e12^2 sigma^6 (the integer powers by repeated squaring)
+static void water_oo_lj(double pick, double e12, double e6, bool arith, double &oo12,
+                        double &oo6) {
+  if (!arith) {
+    oo6 = pick * e6 * e6;
+    oo12 = pick * e12 * e12;
+    return;
+  }
+  const double s2 = e6 * e6, s4 = s2 * s2, s8 = s4 * s4;
+  oo6 = pick * (e12 * e12) * (s2 * s4);
+  oo12 = pick * (e12 * e12) * (s4 * s8);
+}
+
 extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, moil_ctx **out) {
   if (!t || !p || !out) return fail(MOIL_GPU_EINVAL, "null argument");
   *out = nullptr;
@@ -180,6 +195,15 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
   if (p->precision != MOIL_PREC_FP64 && p->precision != MOIL_PREC_MIXED)
     return fail(MOIL_GPU_EINVAL, "unknown precision mode");
   if (p->cutele2 <= 0 || p->eps <= 0 || p->dtol <= 0) return fail(MOIL_GPU_EINVAL, "bad cutoffs/eps/dtol");
+  if (p->terms & ~(MOIL_TERM_DEFAULT | MOIL_TERM_ARITH))
+    return fail(MOIL_GPU_EINVAL, "unknown term bits");
+  // the arith torsion form exists for periods 1..4 only (pot/ephi.f:113-138)
+  if ((p->terms & MOIL_TERM_ARITH) && (p->terms & MOIL_TERM_TORS))
+    for (int32_t m = 0; m < t->ntors; m++)
+      if (t->period[m] < 1 || t->period[m] > 4)
+        return fail(MOIL_GPU_EINVAL, "torsion " + std::to_string(m + 1) + " has period " +
+                                         std::to_string(t->period[m]) +
+                                         ": the arith torsion form needs 1..4");
   // A list reach above half a box edge makes MOIL's half-shell images
   // double count pairs (symm/syminit.f:20-21); MOIL does not refuse it, and
   // neither does this library: the pair set is whatever MOIL's predicates
@@ -420,11 +444,14 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     TRY(upload(&c->d_wi_apar, wi_apar.data(), wi_apar.size()));
     std::vector<int32_t> it(4 * (size_t)c->ntors);
     std::vector<double> tp(6 * (size_t)c->ntors);
+    const bool arith = (p->terms & MOIL_TERM_ARITH) != 0;
     for (int m = 0; m < c->ntors; m++) {
       it[4 * m] = t->itor1[m]; it[4 * m + 1] = t->itor2[m];
       it[4 * m + 2] = t->itor3[m]; it[4 * m + 3] = t->itor4[m];
       tp[6 * m] = t->ktors1[m]; tp[6 * m + 1] = t->ktors2[m]; tp[6 * m + 2] = t->ktors3[m];
       tp[6 * m + 3] = t->phase1[m]; tp[6 * m + 4] = t->phase2[m]; tp[6 * m + 5] = t->phase3[m];
+      // the arith form reads k2, phase2 and the period, kept in k1's place
+      if (arith) tp[6 * m] = (double)t->period[m];
     }
     TRY(upload(&c->d_it, it.data(), it.size()));
     TRY(upload(&c->d_tpar, tp.data(), tp.size()));
@@ -433,6 +460,11 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     for (int m = 0; m < c->nimp; m++) {
       ii[4 * m] = t->iimp1[m]; ii[4 * m + 1] = t->iimp2[m];
       ii[4 * m + 2] = t->iimp3[m]; ii[4 * m + 3] = t->iimp4[m];
+      if (arith) {  // the dihedral of (iimp4, iimp2, iimp1, iimp3), pot/eimphi.f:110-115
+        ii[4 * m] = t->iimp4[m];
+        ii[4 * m + 2] = t->iimp1[m];
+        ii[4 * m + 3] = t->iimp3[m];
+      }
       ip[2 * m] = t->kimp[m]; ip[2 * m + 1] = t->impeq[m];
     }
     TRY(upload(&c->d_ii, ii.data(), ii.size()));
@@ -487,8 +519,8 @@ extern "C" int32_t moil_gpu_init(const moil_topology *t, const moil_params *p, m
     int io = t->poipt[m] - 2 - 1, ih = t->poipt[m] - 1;
     double epst = (p->terms & MOIL_TERM_EL) ? 1.0 : 0.0;
     double pick = (p->terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
-    c->oo6 = pick * t->epsgm6[io] * t->epsgm6[io];
-    c->oo12 = pick * t->epsgm12[io] * t->epsgm12[io];
+    water_oo_lj(pick, t->epsgm12[io], t->epsgm6[io], (p->terms & MOIL_TERM_ARITH) != 0, c->oo12,
+                c->oo6);
     c->qq_oo = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[io];
     c->qq_oh = epst * KOFDIE / p->eps * t->ptchg[io] * t->ptchg[ih];
     c->qq_hh = epst * KOFDIE / p->eps * t->ptchg[ih] * t->ptchg[ih];
@@ -1238,6 +1270,9 @@ extern "C" int32_t moil_gpu_stage_times(moil_ctx *c, double t[8]) {
 extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const double *epsgm6,
                                        const double *epsgm12, double v14, double el14) {
   if (!c || !ptchg || !epsgm6 || !epsgm12) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (c->par.terms & MOIL_TERM_ARITH)
+    return fail(MOIL_GPU_EINVAL, "set_params derives the 1-4 pairs by the geometric rule: "
+                                 "not for an arith context");
   const size_t n = (size_t)c->npt;
   // Water-water pairs take the last water's charges and O LJ factors
   // (pot/watwat_ewald.f:40-72, the H charge being the last H's); the
@@ -1311,6 +1346,9 @@ extern "C" int32_t moil_gpu_set_params(moil_ctx *c, const double *ptchg, const d
 extern "C" int32_t moil_gpu_param_grad_dev(moil_ctx *c, const double *d_coor, double *d_dq,
                                            double *d_de12, double *d_de6, void *stream) {
   if (!c || !d_coor || !d_dq || !d_de12 || !d_de6) return fail(MOIL_GPU_EINVAL, "null argument");
+  if (c->par.terms & MOIL_TERM_ARITH)
+    return fail(MOIL_GPU_EINVAL, "param_grad differentiates the geometric rule's factors: "
+                                 "not for an arith context");
   if (!c->have_lists) return fail(MOIL_GPU_ESTATE, "param_grad called before build_lists");
   if (c->totspe > 0 && !c->have_scale14)
     return fail(MOIL_GPU_ESTATE,

@@ -557,7 +557,7 @@ __device__ __forceinline__ void atoms_impl(
       ix[t][2] = coor[3 * a + 2];
       iq[t] = q[a] * P.qfac;
       ia12[t] = e12[a] * P.pick;
-      ib6[t] = e6[a] * P.pick;
+      ib6[t] = P.arith ? e6[a] : e6[a] * P.pick;  // sigma under arith
     } else {
       ix[t][0] = ix[t][1] = ix[t][2] = 0.0;
       iq[t] = ia12[t] = ib6[t] = 0.0;
@@ -603,7 +603,9 @@ __device__ __forceinline__ void atoms_impl(
       } else {
         for (int k = 0; k < 3; k++) d[k] = ix[i][k] - xj[k];
       }
-      const double qq = iq[i] * qj, A = ia12[i] * aj, B = ib6[i] * bj;
+      const double qq = iq[i] * qj;
+      double A, B;
+      lj_coeffs(P.arith, ia12[i], ib6[i], aj, bj, A, B);
       double df = 0.0, e_v = 0.0, e_e = 0.0;
       if (F64) {
 #pragma clang fp contract(off)
@@ -731,7 +733,9 @@ int32_t atoms_init(moil_ctx *c, const moil_topology *t) {
     }
   for (int a = 0; a < c->npt; a++) {
     if (t->flagchr[a]) aflag[a] |= AF_CHG;
-    if (t->epsgm12[a] < 1e-3) aflag[a] |= AF_SMALL;
+    // the hydrogen rule is off under the arith rule (pot/nbond.f:115-117,
+    // symm/nbsym.f:104-106)
+    if (t->epsgm12[a] < 1e-3 && !(P.terms & MOIL_TERM_ARITH)) aflag[a] |= AF_SMALL;
     for (int k = t->exc1[a]; k < t->exc1[a + 1]; k++) {
       const int m = t->exc2[k] - 1;
       if (m > a && m - a <= 64) xmask[a] |= 1ull << (m - a - 1);
@@ -956,6 +960,7 @@ void atoms_plan(const moil_ctx *c, const double *d_coor, int part0, EvalPlan *p)
   const uint32_t T = c->par.terms;
   P.qfac = (T & MOIL_TERM_EL) ? KOFDIE / c->par.eps : 0.0;
   P.pick = (T & MOIL_TERM_VDW) ? 1.0 : 0.0;
+  P.arith = (T & MOIL_TERM_ARITH) ? 1 : 0;
   P.beta = c->beta;
   P.cutvdw2 = c->par.cutvdw2;
   P.cutele2 = c->par.cutele2;

@@ -7,6 +7,9 @@
 //   etors   pot/ephi.f:66-277     E = sum k_n + sum phase_n k_n T_n(cos phi)
 //   eimphi  pot/eimphi.f:86-249   phi0 = 0 / pi special forms, else
 //                                  k (phi - phi0)^2 with signed phi
+//   arith forms (MOIL_TERM_ARITH): torsion k2 + phase2 k2 cos(n phi),
+//   n = 1..4 (pot/ephi.f:113-138); improper k - k cos(2 phi) with MOIL's
+//   atom order (pot/eimphi.f:98-115,251-262)
 //   ener14  pot/ener14.f:38-95    p14 LJ + plain Coulomb
 //   e_corr  pot/cdie_ewald.f:83-199  -q_i q_j erf(beta r)/r over the
 //                                  exclusion list (atoms 1..npt-1)
@@ -201,10 +204,30 @@ __device__ inline void term_tors(const double *coor, const BSeg &S, int m, const
   int id[4] = {S.it[4 * m] - 1, S.it[4 * m + 1] - 1, S.it[4 * m + 2] - 1, S.it[4 * m + 3] - 1};
   double g[4][3];
   double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, nullptr);
-  const double *p = S.tp + 6 * m;  // k1 k2 k3 ph1 ph2 ph3
-  e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
-      p[5] * (p[2] * co * (4.0 * co * co - 3.0));
-  double df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
+  const double *p = S.tp + 6 * m;  // k1 k2 k3 ph1 ph2 ph3 (arith: period in k1's place)
+  double df;
+  if (P.arith) {
+    // k2 + phase2 k2 T_n(cos phi), n = 1..4 (pot/ephi.f:113-138,242-259)
+    const int n = (int)p[0];
+    const double k2 = p[1], h2 = p[4], c2 = 2.0 * co * co - 1.0;
+    if (n == 1) {
+      e = k2 + h2 * k2 * co;
+      df = k2 * h2;
+    } else if (n == 2) {
+      e = k2 + h2 * k2 * c2;
+      df = 4.0 * k2 * h2 * co;
+    } else if (n == 3) {
+      e = k2 + h2 * k2 * co * (4.0 * co * co - 3.0);
+      df = k2 * h2 * (12.0 * co * co - 3.0);
+    } else {
+      e = k2 + h2 * k2 * (2.0 * c2 * c2 - 1.0);
+      df = 16.0 * k2 * h2 * co * c2;
+    }
+  } else {
+    e = (p[0] + p[1] + p[2]) + p[3] * (p[0] * co) + p[4] * (p[1] * (2.0 * co * co - 1.0)) +
+        p[5] * (p[2] * co * (4.0 * co * co - 3.0));
+    df = p[3] * p[0] + p[4] * p[1] * 4.0 * co + p[5] * p[2] * (12.0 * co * co - 3.0);
+  }
   SlotSink sk(4, id[0], id[1], id[2], id[3], P.fix_scale, P.ovf);
   for (int q = 0; q < 4; q++) sk.add(id[q], df * g[q][0], df * g[q][1], df * g[q][2]);
   sk.flush(out);
@@ -218,7 +241,12 @@ __device__ inline void term_imp(const double *coor, const BSeg &S, int m, const 
   double co = dihedral(&coor[3 * id[0]], &coor[3 * id[1]], &coor[3 * id[2]], &coor[3 * id[3]], g, &sgn);
   co = fmin(1.0, fmax(-1.0, co));
   double k = S.ip[2 * m], eq = S.ip[2 * m + 1], df;
-  if (fabs(eq) < 1e-6) {
+  if (P.arith) {
+    // k - k cos(2 phi) over the atoms (iimp4, iimp2, iimp1, iimp3), put in
+    // that order at set-up (pot/eimphi.f:110-115,251-262)
+    e = k - k * (2.0 * co * co - 1.0);
+    df = -4.0 * k * co;
+  } else if (fabs(eq) < 1e-6) {
     df = -2.0 * k;
     e = -df * (1.0 - co);
   } else if (fabs(fabs(eq) - pi) < 1e-6) {
@@ -437,6 +465,7 @@ void bonded_plan(const moil_ctx *c, const double *d_coor, EvalPlan *p) {
   P.ntab = c->ntab;
   P.do_el14 = (T & MOIL_TERM_EL14) != 0;
   P.do_v14 = (T & MOIL_TERM_VDW14) != 0;
+  P.arith = (T & MOIL_TERM_ARITH) ? 1 : 0;
   P.max_parts = c->max_parts;
   const int nblk = bonded_blocks(c);
   p->n_intra = 0;

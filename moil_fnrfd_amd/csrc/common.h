@@ -104,3 +104,24 @@ __host__ __device__ inline void symop_of(int s, int *ia, int *jb, int *kc) {
   int k0 = (s >= 11) ? 11 : (s >= 8) ? 8 : (s >= 5) ? 5 : 1;
   *kc = (s == 0) ? 0 : (s == 4) ? 1 : 1 - (s - k0);
 }
+
+// LJ coefficients A (of r^-12) and B (of r^-6) of a pair from the i atom's
+// factors (ai = epsgm12_i pick, bi) and the partner's (aj, bj).  Geometric
+// rule: A = ai aj, B = bi bj.  Arith rule (epsgm6 holds sigma):
+// a = ai aj, sigma = (bi + bj)/2, B = a sigma^6, A = a sigma^12 with
+// MOIL's products (pot/cdie_ewald.f:377-383,461-470); A and B then enter
+// A s6 s6 and B s6 as in the geometric case.
+__host__ __device__ inline void lj_coeffs(int arith, double ai, double bi, double aj, double bj,
+                                          double &A, double &B) {
+  if (!arith) {
+    A = ai * aj;
+    B = bi * bj;
+    return;
+  }
+  const double a = ai * aj, b = 0.5 * (bi + bj);
+  double b1 = b * b * b;
+  b1 = b1 * b1;
+  B = a * b1;
+  b1 = b1 * b1;
+  A = a * b1;
+}

@@ -11,8 +11,8 @@
 //
 // Supported: the sections the energy path uses.  Refused (status
 // MOIL_GPU_EINVAL): LES copies (lestyp > 0), Morse bonds (nmb > 0), the
-// beta list (nbeta > 0), monomer division (mdivyes) and the arith
-// combination rule -- the library supports none of them.
+// beta list (nbeta > 0) and monomer division (mdivyes) -- the library
+// supports none of them.  The arith flag is kept for moil_con_arith.
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -126,6 +126,7 @@ struct moil_con {
   std::vector<int32_t> ib1, ib2, ia1, ia2, ia3, it1, it2, it3, it4, period, ii1, ii2, ii3, ii4;
   std::vector<int32_t> sp1, sp2, idxtip3;
   std::vector<double> kbond, req, kangl, angleq, kt1, kt2, kt3, ph1, ph2, ph3, kimp, impeq, p14;
+  bool arith = false;
 };
 
 namespace {
@@ -192,7 +193,7 @@ static int32_t con_read_impl(const char *path, moil_con **out, moil_topology *to
   if (nmb > 0) return con_fail(path, "Morse bonds are not supported");
   if (nbeta > 0) return con_fail(path, "the beta list is not supported");
   if (mdivyes) return con_fail(path, "monomer division is not supported");
-  if (arith) return con_fail(path, "the arith combination rule is not supported");
+  c->arith = arith;
   // molecule names, bulk pointers, monomer names and pointers
   // (rconn.f:84-94: format(a); (/,10(1x,i7)); (/,10(1x,a4)); (/,10(1x,i7)))
   R.skip_lines((int)nbulk);
@@ -386,6 +387,15 @@ static int32_t con_read_impl(const char *path, moil_con **out, moil_topology *to
   t.idxtip3 = c->idxtip3.data();
   *topo = t;
   *out = c.release();
+  return MOIL_GPU_OK;
+}
+
+extern "C" int32_t moil_con_arith(const moil_con *c, int32_t *arith) {
+  if (!c || !arith) {
+    set_error("moil_con_arith: null argument");
+    return MOIL_GPU_EINVAL;
+  }
+  *arith = c->arith ? 1 : 0;
   return MOIL_GPU_OK;
 }
 

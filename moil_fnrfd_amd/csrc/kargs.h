@@ -61,6 +61,7 @@ struct ARS {
   double a, b, c;
   double fix_scale;
   int ntab;  // rows of the erfc table
+  int arith;  // LJ by the arith rule (epsgm6 holds sigma)
   int *ovf;
 };
 struct AtomArgs {
@@ -94,6 +95,7 @@ struct BPar {
   double beta, erftbdns;
   int ntab;  // rows of the erfc table
   int do_el14, do_v14;
+  int arith;  // arith torsion and improper forms
   int max_parts;
 };
 struct BSeg {

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(VT) void k_vir_atoms(AtomArgs A, const double *__re
     }
     iq[t] = a >= 0 ? A.q[a] * P.qfac : 0.0;
     ia12[t] = a >= 0 ? A.e12[a] * P.pick : 0.0;
-    ib6[t] = a >= 0 ? A.e6[a] * P.pick : 0.0;
+    ib6[t] = a >= 0 ? (P.arith ? A.e6[a] : A.e6[a] * P.pick) : 0.0;
   }
   __syncthreads();
   const double c2b = 2.0 * P.beta / sqrt(M_PI);
@@ -233,9 +233,13 @@ __global__ __launch_bounds__(VT) void k_vir_atoms(AtomArgs A, const double *__re
       const bool lj = (kind != 3) && r2 <= P.cutvdw2;
       const bool el = (kind == 1) || (kind == 3 && r2 <= P.cutele2);
       double dfv = 0.0, vpd = 0.0;
-      if (lj) {
+      // under the arith rule MOIL adds no LJ virial for a real list1 pair
+      // (pot/cdie_ewald.f:432-473: the virial sits in the geometric branch)
+      if (lj && !(P.arith && kind == 1 && s == 0)) {
         const double s6 = s2 * s2 * s2;
-        const double A12 = ia12[i] * aj * s6 * s6, B6 = ib6[i] * bj * s6;
+        double Ac, Bc;
+        lj_coeffs(P.arith, ia12[i], ib6[i], aj, bj, Ac, Bc);
+        const double A12 = Ac * s6 * s6, B6 = Bc * s6;
         dfv = -6.0 * s2 * (A12 + (A12 - B6));
       }
       if (el) {

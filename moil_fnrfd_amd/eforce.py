@@ -90,6 +90,7 @@ def load_library():
     L.moil_gpu_chain_destroy.argtypes = [vp]
     L.moil_con_read.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(abi.MoilTopology)]
     L.moil_con_free.argtypes = [vp]
+    L.moil_con_arith.argtypes = [vp, C.POINTER(C.c_int32)]
     L.moil_crd_read.argtypes = [C.c_char_p, C.c_int32, vp]
     L.moil_traj_open.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.moil_traj_write.argtypes = [vp, vp]
@@ -118,7 +119,8 @@ def load_library():
                  'moil_gpu_param_grad', 'moil_gpu_param_grad_dev', 'moil_gpu_chain_create',
                  'moil_gpu_batch_build_lists_dev', 'moil_gpu_batch_set_graph',
                  'moil_gpu_chain_sds', 'moil_gpu_chain_sds_dev', 'moil_gpu_chain_destroy',
-                 'moil_con_read', 'moil_con_free', 'moil_crd_read', 'moil_traj_open',
+                 'moil_con_read', 'moil_con_free', 'moil_con_arith', 'moil_crd_read',
+                 'moil_traj_open',
                  'moil_traj_write', 'moil_traj_close', 'moil_traj_read', 'moil_force_norm',
                  'moil_ewald_setup', 'moil_gpu_dyna_create', 'moil_gpu_dyna_run_dev',
                  'moil_gpu_dyna_run', 'moil_gpu_dyna_destroy', 'moil_gpu_set_molecules',
@@ -156,6 +158,10 @@ class MoilGPU:
         else:
             self._ts, self._keep = abi.make_topology(topo)
             self.npt = topo.npt
+        if getattr(topo, 'arith', False) and not params.terms & abi.TERM_ARITH:
+            # the file (MOIL's arith flag) decides the combination rule, as in MOIL
+            params = abi.MoilParams.from_buffer_copy(params)
+            params.terms |= abi.TERM_ARITH
         self.params = params
         h = C.c_void_p()
         _check(L.moil_gpu_init(C.byref(self._ts), C.byref(params), C.byref(h)))
@@ -500,6 +506,9 @@ class MoilCon:
         _check(load_library().moil_con_read(os.fsencode(path), C.byref(h), C.byref(t)))
         self._h = h
         self.topo = t
+        a = C.c_int32()
+        _check(load_library().moil_con_arith(h, C.byref(a)))
+        self.arith = bool(a.value)  # the context needs abi.TERM_ARITH
 
     def array(self, name, n, width=1):
         """A topology field as a numpy copy of n (x width) values."""

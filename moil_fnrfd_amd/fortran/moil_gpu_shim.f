@@ -225,8 +225,8 @@ c     does not evaluate: with any of them switched on MOIL would compute a
 c     different energy, so the shim stops through alert(...,1) instead of
 c     returning the library's.  The library evaluates the constant
 c     dielectric (ctrue) Ewald path with periodic images (ewaldyes,
-c     esymyes), MOIL's geometric combination rule, no LES copies, no
-c     frozen particles, one process, and the terms bond, angle,
+c     esymyes), MOIL's geometric or arith combination rule, no LES
+c     copies, no frozen particles, one process, and the terms bond, angle,
 c     torsion, improper, LJ, Coulomb and the scaled 1-4 pairs.
       subroutine mg_modes(where)
       use moil_gpu_c
@@ -270,7 +270,6 @@ c     torsion, improper, LJ, Coulomb and the scaled 1-4 pairs.
       if (rwall) why = 'repulsive wall'
       if (efield_yes) why = 'external electric field'
       if (esteeryes) why = 'steering energy'
-      if (arith) why = 'arithmetic combination rule (arith)'
       if (lesflag) why = 'LES copies'
       if (pressON .and. nmol .le. 0)
      &  why = 'pressure / virial without molecules (init_pressure)'
@@ -348,6 +347,7 @@ c     asks for the FP64 parity mode
       if (e14el_yes) p%terms = ior(p%terms, 64)
       if (e14v_yes) p%terms = ior(p%terms, 128)
       if (hvdw0) p%terms = ior(p%terms, 256)
+      if (arith) p%terms = ior(p%terms, 512)
       call mg_init_topo(p, npt, totmon, poipt, ptchg, epsgm6, epsgm12,
      &  flagchr, exc1, exc2, nb, ib1, ib2, kbond, req, nangl, iangl1,
      &  iangl2, iangl3, kangl, angleq, ntors, itor1, itor2, itor3,

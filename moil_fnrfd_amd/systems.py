@@ -63,6 +63,52 @@ def lj_epsgm(eps: float, rmin_half: float):
     return e12, e6
 
 
+def to_arith(t: 'Topology', seed: int, v14: float = 0.5, dup_frac: float = 0.25) -> 'Topology':
+    """The same system under MOIL's arith rule (its AMBER-style parameter
+    sets): per-atom epsgm12 = sqrt(4 eps) and epsgm6 = sigma recovered from
+    the geometric factors (connect/fillpt.f:136-179; an atom without LJ
+    keeps epsgm12 = 0 and takes sigma 1 A), 1-4 coefficients A = v14 a
+    sigma^12, B = v14 a sigma^6 from the arith pair rule with the 1-4
+    charge products kept, torsions of one cosine each with period 1..4,
+    amplitude k2 and phase2 = +-1 (ktors1, ktors3 and the other phases
+    zero; pot/ephi.f:113-138) plus a second term of another period on a
+    fraction dup_frac of them, as MOIL's AMBER reader lists several terms
+    of one dihedral (connect/angtor.f:251-259).  Impropers keep their
+    force constants (the arith form k - k cos 2 phi has no phase)."""
+    rng = np.random.default_rng(seed)
+    e12, e6 = t.epsgm12, t.epsgm6
+    has = e12 > 0
+    sig = np.ones_like(e6)
+    sig[has] = (e12[has] / e6[has]) ** (1.0 / 3.0)
+    a12 = np.zeros_like(e12)
+    a12[has] = e6[has] ** 2 / e12[has]
+    a12 = np.array([_q('12.5E', v) for v in a12])
+    sig = np.array([_q('12.5E', v) for v in sig])
+    p14 = t.p14.copy()
+    for m in range(len(t.spec1)):
+        i, j = int(t.spec1[m]) - 1, int(t.spec2[m]) - 1
+        a = a12[i] * a12[j]
+        sg = 0.5 * (sig[i] + sig[j])
+        p14[m, 0] = _q('30.5f', v14 * a * sg ** 12)
+        p14[m, 1] = _q('15.5f', v14 * a * sg ** 6)
+    nt = len(t.itor1)
+    per = rng.integers(1, 5, nt)
+    dup = np.flatnonzero(rng.random(nt) < dup_frac)
+    per2 = (per[dup] % 4) + 1
+    rows = np.concatenate([np.arange(nt), dup])
+    order = np.argsort(rows, kind='stable')
+    rows = rows[order]
+    period = np.concatenate([per, per2])[order].astype(np.int32)
+    k2 = np.array([_q('10.4f', v) for v in rng.uniform(0.1, 2.0, len(rows))])
+    ph2 = np.array([_q('6.2f', v) for v in rng.choice([-1.0, 1.0], len(rows))])
+    z = np.zeros(len(rows))
+    return dataclasses.replace(
+        t, epsgm12=a12, epsgm6=sig, p14=p14,
+        itor1=t.itor1[rows].copy(), itor2=t.itor2[rows].copy(), itor3=t.itor3[rows].copy(),
+        itor4=t.itor4[rows].copy(), period=period, ktors1=z.copy(), ktors2=k2,
+        ktors3=z.copy(), phase1=z.copy(), phase2=ph2, phase3=z.copy(), arith=True)
+
+
 def monomer_flags(monames) -> np.ndarray:
     out = np.zeros(len(monames), np.int32)
     for k, nm in enumerate(monames):
@@ -123,6 +169,7 @@ class Topology:
     spec2: np.ndarray
     p14: np.ndarray              # float64 [totspe, 3]
     idxtip3: np.ndarray          # monomer numbers of waters (1-based)
+    arith: bool = False          # MOIL's arith rule (the .con header flag)
 
     @property
     def totmon(self) -> int:
@@ -775,7 +822,7 @@ def _fmt_rows(vals, per, fmt):
     return out
 
 
-def write_con(path: str, t: Topology, arith: bool = False):
+def write_con(path: str, t: Topology, arith: Optional[bool] = None):
     L = []
     L.append('12.1')
     L.append('~ CONNECTIVITY FILE FOR MOLECULES: ')
@@ -783,6 +830,8 @@ def write_con(path: str, t: Topology, arith: bool = False):
     nchg = int(np.sum(t.flagchr))
     hdr = [t.totmon, t.npt, len(t.ib1), len(t.iangl1), len(t.itor1), len(t.iimp1),
            len(t.exc2), len(t.spec1), 0, 1, 0, nchg, 0, t.nwaters]
+    if arith is None:
+        arith = t.arith
     L.append(' ' + ''.join('%8d' % v for v in hdr) + ' %s F F' % ('T' if arith else 'F'))
     L.append('SYNT')
     L.append('~ Pointers to last particle of BULK')

@@ -14,8 +14,10 @@
  *
  * Restrictions (same as the GPU library): orthorhombic box, monomer
  * division off (mdivyes=.false.), no LES copies, no metal walls, no
- * virtual particles, geometric LJ combination (arith=.false.), PME
- * order 4, one process (the serial comm_dummy path).
+ * virtual particles, PME order 4, one process (the serial comm_dummy
+ * path).  Both LJ combination rules: geometric (arith=.false.) and
+ * MOIL's arith rule with its AMBER torsion and improper forms
+ * (MOIL_TERM_ARITH).
  */
 #include <math.h>
 #include <stdint.h>
@@ -309,6 +311,13 @@ orc_ctx *orc_create(const moil_topology *t, const moil_params *p, char *err, int
     free(c);
     return NULL;
   }
+  if (p->terms & MOIL_TERM_ARITH)
+    for (i = 0; i < t->ntors; i++)
+      if (t->period[i] < 1 || t->period[i] > 4) {
+        if (err) snprintf(err, errlen, "arith torsion %d with period %d", i + 1, t->period[i]);
+        free(c);
+        return NULL;
+      }
   cutoff = sqrt(p->cutele2);
   c->beta = find_ewaldcof(cutoff, p->dtol);
   /* erftbdns = 0.5*perft with perft=10000 (symm/ewald_init.f:103-104) */
@@ -437,7 +446,8 @@ static int is_water(const orc_ctx *c, int mono) { return (c->monoflag[mono - 1] 
 static void nbond(orc_ctx *c, const double *coor, const int32_t *pointm, const int32_t *nblistm) {
   int totdmon = c->totmon, i, l, j, m;
   double cutvbig2 = c->p.cutvbig2, cutebig2 = c->p.cutebig2;
-  int hvdw0 = (c->p.terms & MOIL_TERM_HVDW0) != 0;
+  /* flag2 = (.not.arith) .and. ... (pot/nbond.f:115-117) */
+  int hvdw0 = (c->p.terms & MOIL_TERM_HVDW0) != 0 && !(c->p.terms & MOIL_TERM_ARITH);
   const int32_t *dpoipt = c->poipt;
   c->point1[0] = c->point2[0] = c->point3[0] = 0;
   for (i = 1; i <= totdmon; i++) {
@@ -574,7 +584,8 @@ static void nbsym(orc_ctx *c, const double *coor, const int32_t *pointm, const i
         for (m = kj; m <= c->poipt[k]; m++) {
           double rx = xl - X(m, 0), ry = yl - X(m, 1), rz = zl - X(m, 2);
           double r2 = rx * rx + ry * ry + rz * rz;
-          int flag2 = (c->epsgm12[l - 1] < 1e-3 || c->epsgm12[m - 1] < 1e-3) && (r2 < cutebig2);
+          int flag2 = !(c->p.terms & MOIL_TERM_ARITH) &&
+                      (c->epsgm12[l - 1] < 1e-3 || c->epsgm12[m - 1] < 1e-3) && (r2 < cutebig2);
           int chg = c->flagchr[l - 1] && c->flagchr[m - 1];
           if (flag2 && chg) { o3 = 1; iv_push(&c->list3, m); continue; }
           if (flag2) continue;
@@ -901,35 +912,61 @@ static double dihedral_cos_grad(const double *xi, const double *xj, const double
   return co;
 }
 
-/* etors (non-arith), pot/ephi.f:66-277 */
+/* etors, pot/ephi.f:66-277 */
 static void etors(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
   int n, k, q;
   for (n = 0; n < c->ntors; n++) {
     int id[4] = {c->it1[n], c->it2[n], c->it3[n], c->it4[n]};
     double g[4][3], co, e, df;
     co = dihedral_cos_grad(&X(id[0], 0), &X(id[1], 0), &X(id[2], 0), &X(id[3], 0), g, NULL);
-    e = (c->kt1[n] + c->kt2[n] + c->kt3[n]) + c->ph1[n] * (c->kt1[n] * co) +
-        c->ph2[n] * (c->kt2[n] * (2.0 * co * co - 1.0)) +
-        c->ph3[n] * (c->kt3[n] * co * (4.0 * co * co - 3.0));
+    if (c->p.terms & MOIL_TERM_ARITH) {
+      /* arith: one cosine of period 1..4 (pot/ephi.f:113-138, 238-259) */
+      double k2 = c->kt2[n], h2 = c->ph2[n];
+      switch (c->period[n]) {
+        case 1: e = k2 + h2 * k2 * co; df = k2 * h2; break;
+        case 2: e = k2 + h2 * k2 * (2.0 * co * co - 1.0); df = 4.0 * k2 * h2 * co; break;
+        case 3:
+          e = k2 + h2 * k2 * co * (4.0 * co * co - 3.0);
+          df = k2 * h2 * (12.0 * co * co - 3.0);
+          break;
+        case 4:
+          e = k2 + h2 * k2 * (2.0 * (2.0 * co * co - 1.0) * (2.0 * co * co - 1.0) - 1.0);
+          df = 16.0 * k2 * h2 * co * (2.0 * co * co - 1.0);
+          break;
+        default: /* MOIL leaves e and df unset; the library refuses these */
+          fprintf(stderr, "oracle: arith torsion %d with period %d\n", n + 1, c->period[n]);
+          abort();
+      }
+    } else {
+      e = (c->kt1[n] + c->kt2[n] + c->kt3[n]) + c->ph1[n] * (c->kt1[n] * co) +
+          c->ph2[n] * (c->kt2[n] * (2.0 * co * co - 1.0)) +
+          c->ph3[n] * (c->kt3[n] * co * (4.0 * co * co - 3.0));
+      df = c->ph1[n] * c->kt1[n] + c->ph2[n] * c->kt2[n] * 4.0 * co +
+           c->ph3[n] * c->kt3[n] * (12.0 * co * co - 3.0);
+    }
     E->e_tors += e;
-    df = c->ph1[n] * c->kt1[n] + c->ph2[n] * c->kt2[n] * 4.0 * co +
-         c->ph3[n] * c->kt3[n] * (12.0 * co * co - 3.0);
     for (q = 0; q < 4; q++)
       for (k = 0; k < 3; k++) dpot[3 * (id[q] - 1) + k] += df * g[q][k];
   }
 }
 
-/* eimphi (non-arith), pot/eimphi.f:86-249 */
+/* eimphi, pot/eimphi.f:86-293 */
 static void eimphi(const orc_ctx *c, const double *coor, double *dpot, eterms *E) {
   int n, k, q;
   double pi = 4.0 * atan(1.0);
   for (n = 0; n < c->nimp; n++) {
-    int id[4] = {c->ii1[n], c->ii2[n], c->ii3[n], c->ii4[n]};
+    const int arith = (c->p.terms & MOIL_TERM_ARITH) != 0;
+    /* arith: i = iimp4, j = iimp2, k = iimp1, l = iimp3 (pot/eimphi.f:98-115) */
+    int id[4] = {arith ? c->ii4[n] : c->ii1[n], c->ii2[n], arith ? c->ii1[n] : c->ii3[n],
+                 arith ? c->ii3[n] : c->ii4[n]};
     double g[4][3], co, e, df, sgn;
     co = dihedral_cos_grad(&X(id[0], 0), &X(id[1], 0), &X(id[2], 0), &X(id[3], 0), g, &sgn);
     if (co > 1.0) co = 1.0;
     if (co < -1.0) co = -1.0;
-    if (fabs(c->impeq[n]) < 1e-6) {
+    if (arith) { /* k - k cos(2 phi), pot/eimphi.f:251-262 */
+      e = c->kimp[n] - c->kimp[n] * (2.0 * co * co - 1.0);
+      df = -4.0 * c->kimp[n] * co;
+    } else if (fabs(c->impeq[n]) < 1e-6) {
       df = -2.0 * c->kimp[n];
       e = -df * (1.0 - co);
     } else if (fabs(fabs(c->impeq[n]) - pi) < 1e-6) {
@@ -1056,11 +1093,34 @@ static inline void pair_virial(const orc_ctx *c, double *V, int img, int i, int 
   }
 }
 
+/* The LJ coefficients of a pair as pair_atoms takes them (A of s6*s6, B
+ * of s6).  Geometric: A = ai*aj, B = bi*bj.  Arith (epsgm6 = sigma):
+ * a = ai*aj, b = 0.5*(bi+bj); b1 = b*b*b; b1 = b1*b1; B = a*b1;
+ * b1 = b1*b1; A = a*b1, MOIL's products (pot/cdie_ewald.f:377-383,
+ * 461-470, 536-548; symm/symcdie_ewald.f:127-140, 301-314). */
+static inline void lj_pair(int arith, double ai, double bi, double aj, double bj, double *A,
+                           double *B) {
+  double a, b, b1;
+  if (!arith) {
+    *A = ai * aj;
+    *B = bi * bj;
+    return;
+  }
+  a = ai * aj;
+  b = 0.5 * (bi + bj);
+  b1 = b * b * b;
+  b1 = b1 * b1;
+  *B = a * b1;
+  b1 = b1 * b1;
+  *A = a * b1;
+}
+
 /* cdie_ewald exclusion correction + list loops, pot/cdie_ewald.f:67-673,
  * and symcdie_ewald, symm/symcdie_ewald.f:58-549. */
 static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterms *E, double *V) {
   double epstmp = (c->p.terms & MOIL_TERM_EL) ? KOFDIE / c->p.eps : 0.0;
   double pick = (c->p.terms & MOIL_TERM_VDW) ? 1.0 : 0.0;
+  const int arith = (c->p.terms & MOIL_TERM_ARITH) != 0;
   int i, kind, s;
   /* exclusion correction (e_corr), atoms 1..npt-1 (pot/cdie_ewald.f:83-199) */
   for (i = 1; i <= c->npt - 1; i++) {
@@ -1087,17 +1147,22 @@ static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterm
     int nreal = kind == 1 ? c->nreal1 : kind == 2 ? c->nreal2 : c->nreal3;
     /* real pairs */
     for (i = 1; i <= c->npt; i++) {
-      double qi = c->ptchg[i - 1] * epstmp, ai = c->epsgm12[i - 1] * pick, bi = c->epsgm6[i - 1] * pick;
+      /* arith: bi = epsgm6(i) without pick (pot/cdie_ewald.f:98-104) */
+      double qi = c->ptchg[i - 1] * epstmp, ai = c->epsgm12[i - 1] * pick,
+             bi = arith ? c->epsgm6[i - 1] : c->epsgm6[i - 1] * pick;
       double d[3] = {0, 0, 0};
       for (int64_t k = pt[i - 1]; k < pt[i]; k++) {
         int j = ls->v[k];
         double rx = X(i, 0) - X(j, 0), ry = X(i, 1) - X(j, 1), rz = X(i, 2) - X(j, 2);
-        double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2];
-        if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], ai * c->epsgm12[j - 1],
-                        bi * c->epsgm6[j - 1], &ev, &ee, &df, V ? vr : NULL))
+        double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2], A, B;
+        lj_pair(arith, ai, bi, c->epsgm12[j - 1], c->epsgm6[j - 1], &A, &B);
+        if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], A, B, &ev, &ee, &df, V ? vr : NULL))
           continue;
         if (V) {
           const double r[3] = {rx, ry, rz};
+          /* arith list1: the LJ virial is only in the geometric branch
+           * (pot/cdie_ewald.f:432-473) */
+          if (arith && kind == 1) vr[0] = 0.0;
           pair_virial(c, V, 0, i, j, NULL, r, vr[0], vr[1]);
         }
         E->e_vdw += ev;
@@ -1115,14 +1180,16 @@ static void real_atoms(const orc_ctx *c, const double *coor, double *dpot, eterm
         for (int w = ib[s - 1]; w < ib[s]; w++) {
           int ir = sr->v[w];
           double xi = X(ir, 0) + T[0], yi = X(ir, 1) + T[1], zi = X(ir, 2) + T[2];
+          /* arith: list1 keeps pick on bi, list2 drops it
+           * (symm/symcdie_ewald.f:101-102,275-284) */
           double qi = c->ptchg[ir - 1] * epstmp, ai = c->epsgm12[ir - 1] * pick,
-                 bi = c->epsgm6[ir - 1] * pick;
+                 bi = (arith && kind == 2) ? c->epsgm6[ir - 1] : c->epsgm6[ir - 1] * pick;
           for (int64_t k = jb; k < ps->v[w]; k++) {
             int j = ls->v[k];
             double rx = xi - X(j, 0), ry = yi - X(j, 1), rz = zi - X(j, 2);
-            double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2];
-            if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], ai * c->epsgm12[j - 1],
-                            bi * c->epsgm6[j - 1], &ev, &ee, &df, V ? vr : NULL))
+            double r2 = rx * rx + ry * ry + rz * rz, ev, ee, df, vr[2], A, B;
+            lj_pair(arith, ai, bi, c->epsgm12[j - 1], c->epsgm6[j - 1], &A, &B);
+            if (!pair_atoms(c, kind, r2, qi * c->ptchg[j - 1], A, B, &ev, &ee, &df, V ? vr : NULL))
               continue;
             if (V) {
               const double r[3] = {rx, ry, rz};
@@ -1196,8 +1263,14 @@ static void real_waters(const orc_ctx *c, const double *coor, double *dpot, eter
     int m = c->idxtip3[c->nwaters - 1];
     io = c->poipt[m] - 2;
     ih = c->poipt[m];
-    oo6 = pick * c->epsgm6[io - 1] * c->epsgm6[io - 1];
-    oo12 = pick * c->epsgm12[io - 1] * c->epsgm12[io - 1];
+    if (!(c->p.terms & MOIL_TERM_ARITH)) {
+      oo6 = pick * c->epsgm6[io - 1] * c->epsgm6[io - 1];
+      oo12 = pick * c->epsgm12[io - 1] * c->epsgm12[io - 1];
+    } else { /* pick*epsgm12(io)**2*epsgm6(io)**6, **12 (pot/watwat_ewald.f:65-66) */
+      double e12 = c->epsgm12[io - 1], sg = c->epsgm6[io - 1];
+      oo6 = pick * pow(e12, 2) * pow(sg, 6);
+      oo12 = pick * pow(e12, 2) * pow(sg, 12);
+    }
     qq[0][0] = epstmp * KOFDIE / c->p.eps * c->ptchg[io - 1] * c->ptchg[io - 1];
     qq[0][1] = qq[0][2] = qq[1][0] = qq[2][0] =
         epstmp * KOFDIE / c->p.eps * c->ptchg[io - 1] * c->ptchg[ih - 1];

@@ -16,6 +16,7 @@ CASES = {
                             lambda s: perturb(s, 0.05, 2001), True),
     'chains_only': (configs.chains_only, lambda s: perturb(s, 0.05, 2201), True),
     'hetero_small_phi0': (configs.hetero_small_phi0, lambda s: perturb(s, 0.05, 2401), True),
+    'hetero_small_arith': (configs.hetero_small_arith, lambda s: perturb(s, 0.05, 2501), True),
     'chains_one_water': (lambda: configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1),
                          lambda s: perturb(s, 0.05, 2301), False),
 }

@@ -38,6 +38,10 @@ def topo_checksum(t) -> str:
               t.idxtip3):
         h.update(np.ascontiguousarray(a).tobytes())
     h.update(','.join(t.moname).encode())
+    if getattr(t, 'arith', False):
+        for a in (t.period, t.ktors2, t.phase2):
+            h.update(np.ascontiguousarray(a).tobytes())
+        h.update(b'arith')
     return h.hexdigest()
 
 
@@ -150,7 +154,7 @@ def write_chain_case(name='chain_water216'):
 
 
 PRESS_CASES = ('water216', 'water216_out_sqz', 'hetero_small', 'hetero_small_out',
-               'chains_one_water')
+               'chains_one_water', 'hetero_small_arith')
 
 
 def write_press_case(name):
@@ -186,6 +190,11 @@ def main():
         for name in PRESS_CASES:
             write_press_case(name)
         return
+    if len(sys.argv) > 1 and sys.argv[1] == 'arith':
+        s, cut, ew = configs.hetero_small_arith()
+        write_case('hetero_small_arith', perturb(s, 0.05, 2501), cut, ew, squeeze=True)
+        write_press_case('hetero_small_arith')
+        return
     s, cut, ew = configs.water216()
     write_case('water216', s, cut, ew)
     s, cut, ew = configs.water216()
@@ -211,6 +220,8 @@ def main():
     write_case('hetero_small_phi0', perturb(s, 0.05, 2401), cut, ew, squeeze=True)
     s, cut, ew = configs.chains_only(seed=2300, n_chains=4, L=20.0, n_water=1)
     write_case('chains_one_water', perturb(s, 0.05, 2301), cut, ew)
+    s, cut, ew = configs.hetero_small_arith()
+    write_case('hetero_small_arith', perturb(s, 0.05, 2501), cut, ew, squeeze=True)
     write_chain_case()
     for name in PRESS_CASES:
         write_press_case(name)

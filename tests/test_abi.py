@@ -76,6 +76,24 @@ def test_init_rejects_charges_beyond_the_grid_range_without_gpu():
     _expect_einval(s.topo, p, 'fixed-point range')
 
 
+def test_init_rejects_arith_inputs_it_cannot_evaluate_without_gpu():
+    """Under the arith rule a torsion of period outside 1..4 has no MOIL
+    energy (pot/ephi.f:113-138 sets none), and unknown term bits are not
+    guessed at: both refused before any device call."""
+    import dataclasses
+    from moil_fnrfd_amd import configs
+    s, cut, ew = configs.hetero_small_arith()
+    p = configs.params_for(s, cut, ew)
+    assert p.terms & abi.TERM_ARITH
+    per = s.topo.period.copy()
+    per[3] = 0
+    _expect_einval(dataclasses.replace(s.topo, period=per), p, 'period 0')
+    per[3] = 5
+    _expect_einval(dataclasses.replace(s.topo, period=per), p, 'period 5')
+    p.terms |= 1 << 12
+    _expect_einval(s.topo, p, 'unknown term bits')
+
+
 def test_init_rejects_out_of_range_indices_without_gpu():
     """Every index array is checked against 1..npt (1..totmon for the water
     list) and the CSR pointers for monotonicity before anything reads them

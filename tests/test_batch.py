@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 def _systems():
     out = []
-    for name in ('water216', 'water1000_buffered', 'hetero_small', 'hetero_small_out'):
+    for name in ('water216', 'water1000_buffered', 'hetero_small', 'hetero_small_out',
+                 'hetero_small_arith'):
         s, cut, ew, sqz = make_case(name)
         out.append((name, s, cut, ew))
     s, cut, ew = configs.water_scaled(8, seed=1300)

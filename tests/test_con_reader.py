@@ -20,7 +20,8 @@ def _read(path):
     return MoilCon(path)
 
 
-CASES = ['water216', 'hetero_small', 'chains_only', 'chains_one_water', 'hetero_small_out']
+CASES = ['water216', 'hetero_small', 'chains_only', 'chains_one_water', 'hetero_small_out',
+         'hetero_small_arith']
 
 
 @pytest.mark.parametrize('name', CASES)
@@ -55,6 +56,7 @@ def test_reader_returns_the_written_topology(name, tmp_path):
     eq(c.array('spec2', len(t.spec2)), t.spec2)
     eq(c.array('p14', len(t.spec1), 3), t.p14)
     eq(c.array('idxtip3', t.nwaters), t.idxtip3)
+    assert c.arith == t.arith
     c.close()
 
 

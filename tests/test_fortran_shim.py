@@ -17,7 +17,7 @@ pytestmark = [pytest.mark.gpu,
                                  reason='oracle/_ref/moil_dump_gpu not built')]
 
 
-@pytest.mark.parametrize('name', ['water216_out_sqz', 'hetero_small'])
+@pytest.mark.parametrize('name', ['water216_out_sqz', 'hetero_small', 'hetero_small_arith'])
 @pytest.mark.parametrize('fp64', [True, False])
 def test_moil_driver_through_shim(name, fp64):
     print('case %s shim fp64=%s' % (name, fp64), flush=True)

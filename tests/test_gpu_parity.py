@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from moil_fnrfd_amd import abi, configs
+from moil_fnrfd_amd.eforce import MoilGPUError
 from tests._util import (energy_errors, golden_energies, load_golden, rms_rel, rows_sha,
                          sort_rows)
 from tests._cases import CASES, make_case
@@ -178,7 +179,7 @@ def test_energy_conservation_properties_32k(gpu):
     m.close()
 
 
-@pytest.mark.parametrize('name', ['hetero_small', 'hetero_small_out'])
+@pytest.mark.parametrize('name', ['hetero_small', 'hetero_small_out', 'hetero_small_arith'])
 def test_mixed_precision_hetero(gpu, name):
     """Mixed mode on the atom lists (list1/list2/list3, images both ways)
     against the FP64 path, and run-to-run bit reproducibility."""
@@ -318,3 +319,23 @@ def test_split_water_beyond_erfc_table(gpu):
     assert rms_rel(dpm, dp_ref) < 1e-5
     m.close()
     mm.close()
+
+
+def test_arith_context_refuses_geometric_parameter_paths(gpu):
+    """set_params derives 1-4 coefficients and param_grad differentiates
+    per-atom factors by the geometric rule; an arith context refuses both
+    and keeps evaluating (its eforce still matches MOIL's arith run)."""
+    _say('case hetero_small_arith refusals')
+    s, cut, ew, sqz = make_case('hetero_small_arith')
+    g = load_golden('hetero_small_arith')
+    m = gpu(s.topo, configs.params_for(s, cut, ew, precision=abi.PREC_FP64))
+    x = m.squeeze(s.coor) if sqz else s.coor
+    m.nbondm(x)
+    t = s.topo
+    with pytest.raises(MoilGPUError, match='arith'):
+        m.set_params(t.ptchg, t.epsgm6, t.epsgm12, 0.5, 0.5)
+    with pytest.raises(MoilGPUError, match='arith'):
+        m.param_grad(x)
+    e, dp = m.eforce(x)
+    assert rms_rel(dp, g['dpot']) < 1e-10
+    m.close()

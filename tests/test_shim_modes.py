@@ -16,8 +16,7 @@ pytestmark = pytest.mark.skipif(not ref_runner.available(ref_runner.SHIM_EXE),
 
 @pytest.mark.parametrize('mode,why', [(1, 'distance-dependent dielectric'),
                                       (2, 'pressure / virial'),
-                                      (3, 'frozen particles'),
-                                      (4, 'arithmetic combination rule')])
+                                      (3, 'frozen particles')])
 def test_shim_stops_on_unsupported_mode(mode, why):
     s, cut, ew, sqz = make_case('hetero_small')
     with pytest.raises(RuntimeError) as ei:
