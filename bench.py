@@ -74,10 +74,12 @@ def cpu_baseline(sysm, cut, ew):
             t_list = d['tlist_s']
             t_eval = d['tforce_s'] / nev
             rate = 1.0 / (t_eval + t_list / NLIST)
-            return dict(value=rate, unit='evals/s', cores=1, kind='reference',
-                        sample='MOIL Fortran (flang -O2), 98,304-atom water box: 1 list build '
-                               '(%.2f s) + %d eforce calls (%.2f s each), list cost amortised '
-                               'over %d evals' % (t_list, nev, t_eval, NLIST))
+            out = dict(value=rate, unit='evals/s', cores=1, kind='reference',
+                       sample='MOIL Fortran (flang -O2), 98,304-atom water box: 1 list build '
+                              '(%.2f s) + %d eforce calls (%.2f s each), list cost amortised '
+                              'over %d evals' % (t_list, nev, t_eval, NLIST))
+            out['aggregate'] = cpu_aggregate(sysm, cut, ew)
+            return out
     except Exception as ex:  # fall back to the C restatement
         print('reference CPU baseline unavailable: %s' % ex, file=sys.stderr)
     from moil_fnrfd_amd import configs
@@ -95,6 +97,33 @@ def cpu_baseline(sysm, cut, ew):
     return dict(value=rate, unit='evals/s', cores=1, kind='port',
                 sample='C restatement (gcc -O2), 98,304-atom water box: 1 list build + %d evals'
                        % nev)
+
+
+def cpu_aggregate(sysm, cut, ew, nproc=8, nev=3):
+    """MOIL on nproc host cores at once: nproc independent MOIL processes
+    on the same workload side by side (MOIL's serial build has no threads),
+    each one list build plus nev eforce calls; the aggregate rate is the sum
+    of the processes' own rates (list cost amortised as above)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import ref_runner
+
+    def one(k):
+        d = ref_runner.run_moil(sysm, cut, ew, squeeze=True, neval=nev, nlist=1000,
+                                sigma=SIGMA, seed=12345 + k, dump_lists=False, timeout=900)
+        return 1.0 / (d['tforce_s'] / nev + d['tlist_s'] / NLIST)
+
+    try:
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(nproc) as ex:
+            rates = list(ex.map(one, range(nproc)))
+        wall = time.perf_counter() - t0
+    except Exception as e:
+        print('CPU aggregate unavailable: %s' % e, file=sys.stderr)
+        return None
+    return dict(value=float(sum(rates)), unit='evals/s', cores=nproc, processes=nproc,
+                per_process=[float(r) for r in rates], wall_s=wall,
+                sample='%d MOIL processes side by side, each 1 list build + %d eforce calls '
+                       'on the 98,304-atom water box' % (nproc, nev))
 
 
 def copy_probe(mib):
@@ -332,8 +361,9 @@ def timed_loop(sysm, cut, ew, steps, warmup, seed, tune=False, shapes=None):
     n_pairs = int(eng.list_counts()[:4].sum())
     ev_list.clear()
     # the real-space kernels' dispatch events on every 10th evaluation of
-    # the timed loop (on every one they cost ~2 % of the rate)
-    eng.set_timing(1, stride=10)
+    # the timed loop (on every one they cost ~2 % of the rate), on every
+    # one in a short run
+    eng.set_timing(1, stride=1 if steps <= 50 else 10)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):

@@ -51,8 +51,8 @@ def run_moil(sysm: System, cut: dict, ewald: dict, *, squeeze=False, neval=1,
     file (path ending in .crd) for MOIL's getcrd instead of the system's
     raw coordinates; mode: a MOIL mode the driver switches on after
     reading the input (1 distance-dependent dielectric, 2 pressure, 3 a
-    frozen particle, 4 the arith combination rule), for the test that the
-    GPU shim refuses what the library does not evaluate."""
+    frozen particle: the modes the GPU shim refuses; 4 the arith flag on
+    whatever parameters the file holds; 5 MOIL's pressure calculation)."""
     if not available(exe):
         raise RuntimeError('%s is not built (make -f oracle/ref/Makefile)' % exe)
     tmp = workdir or tempfile.mkdtemp(prefix='moilref_')
