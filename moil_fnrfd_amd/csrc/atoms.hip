@@ -836,14 +836,14 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   CHECK_SCAN(scan_excl(c, cnt0, c->d_astart, c->ancell, st));
   k_atom_scatter<<<(c->npt + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, nullptr, c->npt,
                                                         c->d_astart, cnt1, c->d_aslot2atom);
-  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_astart, c->ancell, c->d_aslot2atom);
+  launch_cell_sort(c->d_astart, c->ancell, c->d_aslot2atom, st);
   // solute atoms by cell (block side)
   k_atom_cells<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(d_coor, c->d_sol, c->nsol, g, c->d_acell_of,
                                                        cnt2);
   CHECK_SCAN(scan_excl(c, cnt2, c->d_sstart, c->ancell, st));
   k_atom_scatter<<<(c->nsol + TB - 1) / TB, TB, 0, st>>>(c->d_acell_of, c->d_sol, c->nsol,
                                                          c->d_sstart, cnt3, c->d_sslot_u);
-  k_cell_sort<<<(c->ancell + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->ancell, c->d_sslot_u);
+  launch_cell_sort(c->d_sstart, c->ancell, c->d_sslot_u, st);
   // blocks end with their brick row (padding slots -1)
   k_row_pad<<<(c->nrow + TB - 1) / TB, TB, 0, st>>>(c->d_sstart, c->nrow, c->rowcells, c->d_rowcnt);
   CHECK_SCAN(scan_excl(c, c->d_rowcnt, c->d_rowoff, c->nrow, st));

@@ -63,5 +63,5 @@ __device__ inline bool need_only_ok(double cjx, double cjy, double cjz, double d
 }
 
 
-// insertion sort of each range [start[k], start[k+1]) of arr
-__global__ void k_cell_sort(const int32_t *__restrict__ start, int ncell, int32_t *__restrict__ arr);
+// sort of each range [start[k], start[k+1]) of arr (distinct values), ascending
+void launch_cell_sort(const int32_t *start, int ncell, int32_t *arr, hipStream_t st);

@@ -179,19 +179,43 @@ __global__ void k_wat_scatter(const int32_t *__restrict__ cell_of, int nwat,
   slot2wat[p] = w;
 }
 
-// sort each cell's waters by index so the slot order is deterministic
-__global__ void k_cell_sort(const int32_t *__restrict__ start, int ncell, int32_t *__restrict__ s2w) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+// sort each cell's entries (distinct indices) ascending so the slot order
+// is deterministic: one wavefront per cell, the entries staged in LDS and
+// each written at its rank (the number of smaller entries); a cell of more
+// than CS_MAX entries is sorted by insertion in one lane
+#define CS_MAX 512
+__global__ __launch_bounds__(64) void k_cell_sort(const int32_t *__restrict__ start, int ncell,
+                                                  int32_t *__restrict__ s2w) {
+  __shared__ int32_t v[CS_MAX];
+  const int k = blockIdx.x, t = threadIdx.x;
   if (k >= ncell) return;
-  int b = start[k], e = start[k + 1];
-  for (int i = b + 1; i < e; i++) {
-    int v = s2w[i], j = i - 1;
-    while (j >= b && s2w[j] > v) {
-      s2w[j + 1] = s2w[j];
-      j--;
-    }
-    s2w[j + 1] = v;
+  const int b = start[k], n = start[k + 1] - b;
+  if (n <= 1) return;
+  if (n > CS_MAX) {
+    if (t == 0)
+      for (int i = b + 1; i < b + n; i++) {
+        const int x = s2w[i];
+        int j = i - 1;
+        while (j >= b && s2w[j] > x) {
+          s2w[j + 1] = s2w[j];
+          j--;
+        }
+        s2w[j + 1] = x;
+      }
+    return;
   }
+  for (int i = t; i < n; i += 64) v[i] = s2w[b + i];
+  __syncthreads();
+  for (int i = t; i < n; i += 64) {
+    const int x = v[i];
+    int r = 0;
+    for (int j = 0; j < n; j++) r += v[j] < x ? 1 : 0;
+    s2w[b + r] = x;
+  }
+}
+
+void launch_cell_sort(const int32_t *start, int ncell, int32_t *arr, hipStream_t st) {
+  if (ncell > 0) k_cell_sort<<<ncell, 64, 0, st>>>(start, ncell, arr);
 }
 
 __global__ void k_slot_fill(const int32_t *__restrict__ s2w, const int32_t *__restrict__ wat_mono,
@@ -905,7 +929,7 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     CHECK_SCAN(scan_excl(c, c->d_cell_cnt, c->d_cell_start, c->ncell, st));
     k_wat_scatter<<<nblk, TB, 0, st>>>(c->d_cell_of, c->nwat, c->d_cell_start, c->d_cell_fill,
                                        c->d_slot2wat);
-    k_cell_sort<<<(c->ncell + TB - 1) / TB, TB, 0, st>>>(c->d_cell_start, c->ncell, c->d_slot2wat);
+    launch_cell_sort(c->d_cell_start, c->ncell, c->d_slot2wat, st);
     double *d_scent = c->d_scent;
     int32_t *d_smono = c->d_smono;
     k_slot_fill<<<nblk, TB, 0, st>>>(c->d_slot2wat, c->d_wat_mono, c->d_wat_o, c->d_mono_cent,
