@@ -407,23 +407,30 @@ __global__ void k_abucket_sum(const int32_t *__restrict__ bpart, int nblk, int n
   bsum[q] = v;
 }
 
-// One workgroup per solute block: reads the block's records back in order
-// (part by part) and writes its items -- each record split into items of
-// <= WSPLIT pairs (consecutive mask bits), bucketed by popcount -- at boff
-// + ranks taken in thread order (wavefront scans and ballots), so the item
-// order is fixed.
+// One workgroup per part of a solute block: reads the part's records back
+// in order and writes its items -- each record split into items of <=
+// WSPLIT pairs (consecutive mask bits), bucketed by popcount -- at boff +
+// the earlier parts' bucket counts (bcount, from the predicate pass) +
+// ranks taken in thread order (wavefront scans and ballots), so the item
+// order is that of the block's records part after part.
 __global__ __launch_bounds__(ABT) void k_ablock_write(const AItem *__restrict__ raw, int rcap,
                                                       const int32_t *__restrict__ rcount,
                                                       int npart,
+                                                      const int32_t *__restrict__ bcount,
                                                       const int32_t *__restrict__ boff,
                                                       AItem *__restrict__ items, size_t cap) {
   __shared__ int32_t bk_run[NBUCKET];
   __shared__ int32_t wcnt[ABT / 64][NBUCKET];
-  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (t < NBUCKET) bk_run[t] = 0;
+  const int b = blockIdx.x / npart, part = blockIdx.x - b * npart;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < NBUCKET) {
+    int v = 0;
+    for (int p = 0; p < part; p++) v += bcount[((size_t)b * npart + p) * NBUCKET + t];
+    bk_run[t] = v;
+  }
   __syncthreads();
-  for (int part = 0; part < npart; part++) {
-  const int slice = b * npart + part;
+  {
+  const int slice = blockIdx.x;
   const int n = min(rcount[slice], rcap);
   const AItem *rblk = raw + (size_t)slice * rcap;
   for (int fb = 0; fb < n; fb += ABT) {
@@ -884,8 +891,9 @@ int32_t atoms_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
   k_abucket_sum<<<(c->nablk * NBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_abpart, c->nablk, np,
                                                                   c->d_abcount);
   CHECK_SCAN(scan_excl(c, c->d_abcount, c->d_aboff, c->nablk * NBUCKET, st));
-  k_ablock_write<<<c->nablk, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, np, c->d_aboff,
-                                           c->d_aitems, c->aitem_cap);
+  k_ablock_write<<<c->nablk * np, ABT, 0, st>>>(c->d_araw, c->araw_cap, c->d_arcount, np,
+                                                c->d_abpart, c->d_aboff, c->d_aitems,
+                                                c->aitem_cap);
   ChunkPar cp;
   cp.nblk = c->nablk;
   cp.pairs = c->achunk_pairs;

@@ -609,19 +609,26 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
   }
 }
 
-// One workgroup per water block: reads the block's records back in order
-// and writes its items at boff + the same deterministic ranks.
+// One workgroup per part of a water block: reads the part's records back
+// in order and writes its items at boff + the earlier parts' bucket counts
+// (bpart, from the predicate pass) + the same deterministic ranks, so the
+// item order is that of the block's records part after part.
 __global__ __launch_bounds__(LB_THREADS) void k_wblock_write(const WRaw *__restrict__ raw, int rcap,
                                                              const int32_t *__restrict__ rcount,
                                                              int npart,
+                                                             const int32_t *__restrict__ bpart,
                                                              const int32_t *__restrict__ boff,
                                                              WItem *__restrict__ items, size_t cap) {
   __shared__ EmitLDS L;
-  const int b = blockIdx.x, t = threadIdx.x;
-  if (t < WNBUCKET) L.bk_run[t] = 0;
+  const int b = blockIdx.x / npart, part = blockIdx.x - b * npart, t = threadIdx.x;
+  if (t < WNBUCKET) {
+    int v = 0;
+    for (int p = 0; p < part; p++) v += bpart[((size_t)b * npart + p) * WNBUCKET + t];
+    L.bk_run[t] = v;
+  }
   __syncthreads();
-  for (int part = 0; part < npart; part++) {
-    const int slice = b * npart + part;
+  {
+    const int slice = blockIdx.x;
     const int n = min(rcount[slice], rcap);
     const WRaw *rblk = raw + (size_t)slice * rcap;
     for (int fb = 0; fb < n; fb += LB_THREADS) {
@@ -974,8 +981,9 @@ int32_t lists_build(moil_ctx *c, const double *d_coor, hipStream_t st) {
     k_bucket_sum<<<(c->nwblk * WNBUCKET + TB - 1) / TB, TB, 0, st>>>(c->d_bpart, c->nwblk, np,
                                                                     c->d_bcount);
     CHECK_SCAN(scan_excl(c, c->d_bcount, c->d_boff, c->nwblk * WNBUCKET, st));
-    k_wblock_write<<<c->nwblk, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount, np,
-                                                    c->d_boff, c->d_witems, c->witem_cap);
+    k_wblock_write<<<c->nwblk * np, LB_THREADS, 0, st>>>(c->d_wraw, c->wraw_cap, c->d_rcount,
+                                                         np, c->d_bpart, c->d_boff, c->d_witems,
+                                                         c->witem_cap);
     ChunkPar cp;
     cp.nblk = c->nwblk;
     cp.pairs = c->wchunk_pairs;
