@@ -253,6 +253,9 @@ class MoilGPU:
             raise ValueError('expected %d values per parameter' % self.npt)
         _check(load_library().moil_gpu_set_params(self._h, q.ctypes.data, a6.ctypes.data,
                                                   a12.ctypes.data, float(v14), float(el14)))
+        # a token that changes with every parameter set (the autograd binding
+        # skips an upload when the engine still holds its forward's values)
+        self.param_token = getattr(self, 'param_token', 0) + 1
 
     def param_grad(self, coor):
         """(dE/dq, dE/d epsgm12, dE/d epsgm6), each (npt,) float64, on the

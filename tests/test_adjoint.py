@@ -185,3 +185,39 @@ def test_autograd_device(system):
     assert np.allclose(ge.numpy(), geps, rtol=1e-12, atol=1e-12)
     assert np.allclose(gr.numpy(), grmh, rtol=1e-12, atol=1e-12)
     g.close()
+
+
+def test_autograd_parameters_on_device_and_interleaved_calls(system):
+    """Parameters as device tensors (per-type reduction on the device), a
+    second forward with other charges on the same engine before the first
+    backward: the first backward re-uploads its own parameters and gives
+    the gradients of a lone call; a repeated forward with unchanged
+    parameters uploads nothing and gives the same energy."""
+    import torch
+    s, cut, ew, e12, e6 = system
+    m = s.meta
+    g, x = _engine(system, abi.PREC_FP64)
+    dev = torch.device('cuda:0')
+
+    def call(q):
+        xt = torch.tensor(x, device=dev, requires_grad=True)
+        qt = torch.tensor(q, device=dev, requires_grad=True)
+        et = torch.tensor(m['lj_eps'], device=dev, requires_grad=True)
+        rt = torch.tensor(m['lj_rmin_half'], device=dev, requires_grad=True)
+        return moil_energy(g, xt, qt, et, rt, m['lj_type']), (xt, qt, et, rt)
+
+    q0 = np.array(s.topo.ptchg, np.float64)
+    E0, a0 = call(q0)
+    ref = [t.cpu().numpy() for t in torch.autograd.grad(E0, a0)]
+    tok = g.param_token
+    E0b, _ = call(q0)
+    assert g.param_token == tok  # unchanged parameters: no upload
+    assert float(E0b.detach()) == float(E0.detach())
+    E1st, a1st = call(q0)
+    q1 = q0 * 0.97  # still neutral
+    E2, a2 = call(q1)
+    assert float(E2.detach()) != float(E1st.detach())
+    got = [t.cpu().numpy() for t in torch.autograd.grad(E1st, a1st)]
+    for a, b in zip(got, ref):
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
+    g.close()
