@@ -437,13 +437,22 @@ __device__ __forceinline__ void water_reduce_impl(
   const int q0 = jstart[s], q1 = jstart[s + 1];
   int q = q0;
   if (jf32) {
+    // four rows in flight per iteration (integer sums: any grouping gives
+    // the same total)
     const float *jp = reinterpret_cast<const float *>(jpart);
     bool bad = false;
-    for (; q + 1 < q1; q += 2) {
-      a0 += fixf(jp[(size_t)q * 9 + k], bad);
-      a1 += fixf(jp[(size_t)(q + 1) * 9 + k], bad);
+    long long a2 = 0, a3 = 0;
+    for (; q + 3 < q1; q += 4) {
+      const float v0 = jp[(size_t)q * 9 + k], v1 = jp[(size_t)(q + 1) * 9 + k],
+                  v2 = jp[(size_t)(q + 2) * 9 + k], v3 = jp[(size_t)(q + 3) * 9 + k];
+      a0 += fixf(v0, bad);
+      a1 += fixf(v1, bad);
+      a2 += fixf(v2, bad);
+      a3 += fixf(v3, bad);
     }
-    if (q < q1) a0 += fixf(jp[(size_t)q * 9 + k], bad);
+    for (; q < q1; q++) a0 += fixf(jp[(size_t)q * 9 + k], bad);
+    a0 += a2;
+    a1 += a3;
     fix_report(bad, ovf);
   } else {
     for (; q + 1 < q1; q += 2) {
