@@ -270,24 +270,30 @@ __global__ __launch_bounds__(ABT) void k_ablock_raw(
       }
       __syncthreads();
       {
-        int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
-            v3 = cellpre[4 * t + 3];
-        int tot = v0 + v1 + v2 + v3;
-        __syncthreads();
-        scanbuf[t] = tot;
-        __syncthreads();
-        for (int off = 1; off < ABT; off <<= 1) {
-          int add = (t >= off) ? scanbuf[t - off] : 0;
-          __syncthreads();
-          scanbuf[t] += add;
-          __syncthreads();
+        // exclusive scan of the cell sizes: four cells per thread, a
+        // wavefront scan by shuffles, the wavefront totals through LDS
+        // (one barrier instead of a barrier pair per step)
+        static_assert(AB_MAXCELLS == 4 * ABT, "four cells per thread");
+        const int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
+                  v3 = cellpre[4 * t + 3];
+        const int tot = v0 + v1 + v2 + v3;
+        int incl = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
         }
-        int base = scanbuf[t] - tot;
+        if (lane == 63) scanbuf[wv] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w2 = 0; w2 < wv; w2++) before += scanbuf[w2];
+        incl += before;
+        const int base = incl - tot;
         cellpre[4 * t] = base;
         cellpre[4 * t + 1] = base + v0;
         cellpre[4 * t + 2] = base + v0 + v1;
         cellpre[4 * t + 3] = base + v0 + v1 + v2;
-        if (t == ABT - 1) cellpre[AB_MAXCELLS] = scanbuf[t];
+        if (t == ABT - 1) cellpre[AB_MAXCELLS] = incl;
       }
       __syncthreads();
       const int ncand = cellpre[AB_MAXCELLS];

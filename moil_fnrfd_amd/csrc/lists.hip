@@ -488,26 +488,31 @@ __global__ __launch_bounds__(LB_THREADS) void k_wblock_items(
         cellpre[u] = sz;
       }
       __syncthreads();
-      // exclusive scan of the 1024 sizes: 4 per thread + 256-wide scan
+      // exclusive scan of the 1024 sizes: 4 per thread, a wavefront scan by
+      // shuffles, the wavefront totals through LDS (one barrier)
       {
-        int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
-            v3 = cellpre[4 * t + 3];
-        int tot = v0 + v1 + v2 + v3;
-        __syncthreads();
-        scanbuf[t] = tot;
-        __syncthreads();
-        for (int off = 1; off < LB_THREADS; off <<= 1) {
-          int add = (t >= off) ? scanbuf[t - off] : 0;
-          __syncthreads();
-          scanbuf[t] += add;
-          __syncthreads();
+        static_assert(LB_MAXCELLS == 4 * LB_THREADS, "four cells per thread");
+        const int lane = t & 63, wv = t >> 6;
+        const int v0 = cellpre[4 * t], v1 = cellpre[4 * t + 1], v2 = cellpre[4 * t + 2],
+                  v3 = cellpre[4 * t + 3];
+        const int tot = v0 + v1 + v2 + v3;
+        int incl = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
         }
-        int base = scanbuf[t] - tot;
+        if (lane == 63) scanbuf[wv] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w2 = 0; w2 < wv; w2++) before += scanbuf[w2];
+        incl += before;
+        const int base = incl - tot;
         cellpre[4 * t] = base;
         cellpre[4 * t + 1] = base + v0;
         cellpre[4 * t + 2] = base + v0 + v1;
         cellpre[4 * t + 3] = base + v0 + v1 + v2;
-        if (t == LB_THREADS - 1) cellpre[LB_MAXCELLS] = scanbuf[t];
+        if (t == LB_THREADS - 1) cellpre[LB_MAXCELLS] = incl;
       }
       __syncthreads();
       const int ncand = cellpre[LB_MAXCELLS];
