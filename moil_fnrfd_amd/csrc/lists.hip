@@ -740,9 +740,12 @@ __device__ inline int chunk_walk(const int32_t *__restrict__ boff, int b, ChunkP
     const int pc = k < WSPLIT ? WSPLIT - k : max(1, (2 * WSPLIT - k) / P.m2_div);
     const int hi = min(O(k + 1), end);
     while (e < hi) {
-      const long want = max(max(1L, (long)(P.min_items - (e - q))),
-                            (P.pairs - pairs + pc - 1) / pc);
-      int take = (int)min((long)(hi - e), want);
+      // pairs still wanted (none once the budget is met), in 32-bit
+      // unsigned arithmetic: the division is the walk's costliest step
+      const unsigned need = pairs < P.pairs ? (unsigned)(P.pairs - pairs) : 0u;
+      const int want = max(max(1, P.min_items - (e - q)),
+                           (int)((need + (unsigned)pc - 1u) / (unsigned)pc));
+      int take = min(hi - e, want);
       take = min(take, P.max_items - (e - q));
       pairs += (long)take * pc;
       e += take;
