@@ -112,22 +112,22 @@ __global__ void k_mshake_pos(const double *__restrict__ coor, double *__restrict
     // MOIL's check, in its order: the first error past 0.4 (or the 19th
     // iteration) ends the water, the first error above tolerance asks
     // for one more correction, and no error above tolerance converges
-    int verdict = 0;  // 0 converged, 1 correct again, 2 failed
+    int outcome = 0;  // 0 converged, 1 correct again, 2 failed
     for (int i = 0; i < 3; i++) {
       if (fabs(err[i]) > 0.4 || iter == 19) {
-        verdict = 2;
+        outcome = 2;
         break;
       }
       if (fabs(err[i]) > M.tol) {
-        verdict = 1;
+        outcome = 1;
         break;
       }
     }
-    if (verdict == 0) {
+    if (outcome == 0) {
       ok = true;
       break;
     }
-    if (verdict == 2) break;
+    if (outcome == 2) break;
     double lam[3];
     for (int i = 0; i < 3; i++) {
       lam[i] = 0.0;
